@@ -1,0 +1,195 @@
+/* ucuda.h — C ABI of the MI355X ensemble ODE analysis engine.
+ *
+ * Replaces the reference's NVRTC/driver layer (U-CUDA/nvrtc_engine.h:16-73,
+ * U-CUDA/parametric_engine.cpp:1486-1652) and exposes the analysis entry
+ * points that U-CUDA's ParametricEngine facade forwards to
+ * (U-CUDA/parametric_engine.h:844-889). Plain C types only: no exceptions
+ * cross this boundary, every call returns a status code and the engine keeps
+ * the last error message (the reference's `Result.error` / NvrtcEngine::error()).
+ *
+ * Memory: request inputs are host pointers; outputs are caller-owned DEVICE
+ * pointers laid out structure-of-arrays (plane-major, cell = iy * nx + ix).
+ * Launches are enqueued on the caller's HIP stream (NULL = default stream).
+ */
+#ifndef UCUDA_H
+#define UCUDA_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define UC_OK 0
+#define UC_ERR_ARGUMENT 1   /* request validation failed (message in uc_last_error) */
+#define UC_ERR_PARSE 2      /* system text could not be parsed                        */
+#define UC_ERR_COMPILE 3    /* hipRTC compilation failed (log in uc_last_error)       */
+#define UC_ERR_HIP 4        /* HIP runtime error                                      */
+#define UC_ERR_NO_DEVICE 5  /* engine created without a device                        */
+#define UC_ERR_CANCELLED 6  /* serving: job cancelled                                 */
+#define UC_ERR_BUSY 7       /* serving: job not finished yet                          */
+
+/* ---- precision policies ---- */
+#define UC_PREC_F64 0   /* everything in double                                   */
+#define UC_PREC_F32C 1  /* float stages, Kahan-compensated state, double GS/logs   */
+
+/* ---- sweep axis kinds (match UcJob) ---- */
+#define UC_AXIS_PARAM 1
+#define UC_AXIS_VAR 2
+#define UC_AXIS_STEP 3
+
+/* ---- Lyapunov methods ---- */
+#define UC_METHOD_VARIATIONAL 0
+#define UC_METHOD_FINITE_DIFF 1
+
+/* ---- integrator schemes for uc_codegen_scheme (order of U-CUDA/codegen.hpp:21) ---- */
+#define UC_SCHEME_EULER 0
+#define UC_SCHEME_EULER_CROMER 1
+#define UC_SCHEME_MIDPOINT 2
+#define UC_SCHEME_RK4 3
+
+typedef struct uc_engine uc_engine;
+typedef struct uc_system uc_system;
+typedef struct uc_plan uc_plan;
+
+/* System definition (U-CUDA/codegen.hpp:10-15). vars -> X[0..N-1],
+ * params -> a[1..M], a[0] reserved. time_symbol: name of the independent
+ * variable usable in rhs (NULL or "" for autonomous systems). */
+typedef struct uc_system_desc {
+    const char* const* vars;
+    int n_vars;
+    const char* const* params;
+    int n_params;
+    const char* const* rhs;
+    int latex;
+    const char* time_symbol;
+} uc_system_desc;
+
+/* One sweep axis. Values follow the reference kernel's grid
+ * (getValueByIdx / getValueByIdx_log, U-CUDA/cudaLibrary.cu:1344-1396):
+ * linear (1-t)*lo + t*hi with t = i/(n-1), n == 1 -> hi; log scale
+ * 10^(log10 lo + i*(log10 hi - log10 lo)/(n-1)). */
+typedef struct uc_axis {
+    int kind;    /* UC_AXIS_*                                   */
+    int index;   /* a[] index (1-based) or state index (0-based) */
+    int n_pts;
+    double lo, hi;
+    int log_scale;
+} uc_axis;
+
+/* Fields shared by every analysis request (Bifurcation1DRequest,
+ * LLE2DRequest, LS2DRequest: U-CUDA/parametric_engine.h:27-95,411-455,581-619). */
+typedef struct uc_common {
+    const double* initial_conditions; /* host, n_vars values                        */
+    const double* base_values;        /* host, a[0..n_values-1]                     */
+    int n_values;
+    int n_axes;                       /* 1 or 2                                     */
+    uc_axis axis[2];
+    double h;
+    double transient_time;
+    double t_max;
+    double max_value;                 /* 0 disables the magnitude check             */
+    int precision;                    /* UC_PREC_*                                  */
+    int strict;                       /* 1: no FMA contraction (bit-reproducible)   */
+    /* optional per-cell random initial state X0 = lo + (hi-lo)*u, seed 0 = off */
+    unsigned long long ic_seed;
+    double ic_lo, ic_hi;
+} uc_common;
+
+/* Bifurcation sweep (run_bifurcation_1d / _2d, parametric_engine.cpp:1674-2091). */
+typedef struct uc_bif_request {
+    uc_common c;
+    int writable_var;   /* sampled state index, -1 = x0 + pi*x1 + e*x2 */
+    int pre_scaller;
+    int max_peaks;      /* stored peaks per cell (reference cap 2500) */
+} uc_bif_request;
+
+typedef struct uc_bif_output {
+    int* flags;          /* [cells] peak count, -1 fixed point / bad step, 0 diverged */
+    double* peaks;       /* [max_peaks][cells]                                         */
+    double* intervals;   /* [max_peaks][cells] inter-peak intervals                   */
+    double* traj;        /* optional [traj_len][cells] first recorded samples          */
+    int traj_len;
+    double* final_state; /* optional [n_vars][cells]                                   */
+} uc_bif_output;
+
+/* Lyapunov exponent maps (run_lle_1d/_2d, run_ls_1d/_2d). */
+typedef struct uc_lyap_request {
+    uc_common c;
+    double NT;                       /* renormalisation interval (time units)  */
+    double eps;                      /* finite-difference probe distance       */
+    int method;                      /* UC_METHOD_*                            */
+    unsigned long long probe_seed;   /* reference: 1234567891                  */
+} uc_lyap_request;
+
+typedef struct uc_lyap_output {
+    double* values;      /* [n_exponents][cells]; 999 diverged, NaN bad step */
+    int* flags;          /* [cells] 1 ok, -1 diverged / bad step             */
+    double* final_state; /* optional [n_vars][cells]                          */
+} uc_lyap_output;
+
+/* Basins of attraction with adaptive Dormand-Prince 5(4) and Poincare-section
+ * attractor signatures (new method; the reference clusters features with
+ * DBSCAN, U-CUDA/parametric_engine.cpp:6035-6514). */
+typedef struct uc_basins_request {
+    uc_common c;                 /* axes sweep initial conditions; h = first step */
+    double rtol, atol;
+    double h_min, h_max;
+    double section_period;       /* e.g. 2*pi/omega                               */
+    int n_sections;              /* periods integrated in total                  */
+    int n_record;                /* trailing sections used for the signature     */
+    int max_period;              /* longest cycle searched                       */
+    int label_vars;              /* leading state variables in the signature     */
+    double period_tol;           /* closure tolerance                            */
+    double quantum;              /* signature quantisation                        */
+    long long max_attempts;      /* per-cell step cap (0 = default)              */
+} uc_basins_request;
+
+typedef struct uc_basins_output {
+    int* labels;          /* [cells] canonical attractor label, 0 = unbound        */
+    int* periods;         /* [cells] detected period (0 = none found, -1 unbound)  */
+    double* cycle_point;  /* optional [label_vars][cells] canonical cycle point    */
+    long long* steps;     /* optional [2][cells] accepted / rejected steps         */
+} uc_basins_output;
+
+/* ---- engine ---- */
+/* device < 0 creates a compile-only engine (no HIP device is touched). */
+int uc_engine_create(int device, uc_engine** out);
+void uc_engine_destroy(uc_engine* e);
+const char* uc_last_error(const uc_engine* e);
+const char* uc_version(void);
+
+/* ---- systems / codegen (U-CUDA/codegen.hpp:25-48) ---- */
+int uc_system_create(uc_engine* e, const uc_system_desc* desc, uc_system** out);
+void uc_system_destroy(uc_system* s);
+int uc_system_n_vars(const uc_system* s);
+int uc_system_n_values(const uc_system* s);
+/* Copies generated source (NUL-terminated) into buf; *len gets the full size.
+ * dialect 0: the complete hipRTC program; 1: the system functions as C. */
+int uc_system_source(const uc_system* s, int dialect, char* buf, size_t cap, size_t* len);
+/* Compiles the system's module with hipRTC for gfx950 (no device needed).
+ * *code_size receives the code-object size in bytes. */
+int uc_system_compile(uc_engine* e, uc_system* s, int strict, size_t* code_size);
+/* Algorithmic flop counts of one right-hand side / Jacobian evaluation. */
+int uc_system_op_counts(const uc_system* s, int* rhs_flops, int* rhs_calls, int* jac_nonzeros, int* jac_flops);
+/* codegen_scheme: KRS step body text for a scheme (UC_SCHEME_*). */
+int uc_codegen_scheme(uc_engine* e, const uc_system_desc* desc, int scheme, char* buf, size_t cap, size_t* len);
+/* Host evaluation of the right-hand side in double (SystemEvaluator::eval). */
+int uc_system_eval_rhs(const uc_system* s, const double* X, const double* a, double t, double* dX);
+
+/* ---- analyses: enqueue on `stream`, return when enqueued ---- */
+int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, void* stream);
+int uc_run_lle(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
+int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
+int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream);
+
+/* Number of cells of a request's grid (product of axis point counts). */
+long long uc_grid_cells(const uc_common* c);
+/* Axis values exactly as the kernels use them (host, n_pts doubles). */
+int uc_axis_values(const uc_axis* ax, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,292 @@
+"""ctypes glue for the FP64 CPU oracle.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker. The engine never imports it.
+
+`KrsStep` restates krs_cpu (U-CUDA/krs_cpu.cpp:21-99, 293-332, 405-479) for
+Linux: the step body is wrapped in the reference prelude with numb = double,
+compiled by g++ with -ffp-contract=off into a cached shared object keyed by
+FNV-1a over (body, nx, nv, prelude version, sizeof numb), and `krs_step` is
+loaded with dlopen. `SystemFuncs` does the same for the C right-hand side and
+tangent functions generated by the engine's codegen.
+"""
+import ctypes as C
+import hashlib
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+PRELUDE_VERSION = 3          # kPreludeVersion, U-CUDA/krs_cpu.cpp:274
+NUMB_BYTES = 8               # numb = double
+
+AXIS_NONE, AXIS_PARAM, AXIS_VAR, AXIS_STEP = 0, 1, 2, 3
+
+
+class OcJob(C.Structure):
+    _fields_ = [("n_vars", C.c_int), ("n_values", C.c_int),
+                ("ic", C.POINTER(C.c_double)), ("params", C.POINTER(C.c_double)),
+                ("n_axes", C.c_int),
+                ("axis_kind", C.c_int * 2), ("axis_index", C.c_int * 2), ("axis_n", C.c_int * 2),
+                ("axis_log", C.c_int * 2), ("axis_lo", C.c_double * 2), ("axis_hi", C.c_double * 2),
+                ("h", C.c_double), ("transient_time", C.c_double), ("t_max", C.c_double),
+                ("max_value", C.c_double),
+                ("ic_seed", C.c_ulonglong), ("ic_lo", C.c_double), ("ic_hi", C.c_double),
+                ("writable_var", C.c_int), ("pre_scaller", C.c_int), ("max_peaks", C.c_int),
+                ("NT", C.c_double), ("eps", C.c_double), ("probe_seed", C.c_ulonglong)]
+
+
+_lib = None
+_P = C.c_void_p
+_LL = C.POINTER(C.c_longlong)
+_D = C.POINTER(C.c_double)
+_I = C.POINTER(C.c_int)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = C.CDLL(LIB_PATH)
+        L.oc_rng_init.argtypes = [C.c_ulonglong, C.c_ulonglong, C.c_ulonglong, C.POINTER(C.c_ulonglong)]
+        L.oc_rng_uniform.argtypes = [C.POINTER(C.c_ulonglong)]
+        L.oc_rng_uniform.restype = C.c_float
+        L.oc_axis_values.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _D]
+        L.oc_cell_setup.argtypes = [C.POINTER(OcJob), C.c_longlong, _D, _D, _D]
+        L.oc_loop_model.argtypes = [_P, _D, _D, C.c_double, C.c_longlong, C.c_int, C.c_int, C.c_int,
+                                    C.c_double, _D]
+        L.oc_peak_finder.argtypes = [_D, C.c_longlong, _D, _D, C.c_double]
+        L.oc_cgs.argtypes = [_D, _D, C.c_int, _D]
+        L.oc_mgs.argtypes = [_D, C.c_int, C.c_int, _D]
+        L.oc_rk4.argtypes = [_P, _D, _D, C.c_double, C.c_int]
+        L.oc_rk4_tan.argtypes = [_P, _P, _D, _D, C.c_int, _D, C.c_double, C.c_int]
+        L.oc_bif_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _I, _D, _D, _D, C.c_int, C.c_int]
+        L.oc_lle_fd_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
+        L.oc_ls_fd_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
+        L.oc_lyap_var_cells.argtypes = [_P, _P, C.c_int, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _cache_root():
+    root = os.path.join(tempfile.gettempdir(), "ucuda-krs")
+    os.makedirs(root, exist_ok=True)
+    return root
+
+
+def fnv1a_key(body, nx, nv):
+    """hash_key, U-CUDA/krs_cpu.cpp:318-332 (FNV-1a over body, nx, nv, version, sizeof numb)."""
+    h = 1469598103934665603
+    data = body.encode() + int(nx).to_bytes(4, "little", signed=True) + int(nv).to_bytes(4, "little", signed=True) \
+        + PRELUDE_VERSION.to_bytes(4, "little") + NUMB_BYTES.to_bytes(4, "little")
+    for b in data:
+        h ^= b
+        h = (h * 1099511628211) & 0xFFFFFFFFFFFFFFFF
+    return h
+
+
+def check_indices(body, nx, nv):
+    """krs_cpu_check_indices, U-CUDA/krs_cpu.cpp:21-99: constant X[k] / a[k]
+    subscripts must be in range. Returns a list of (line, message)."""
+    diags = []
+    i, line, n = 0, 1, len(body)
+    while i < n:
+        c = body[i]
+        if c == "\n":
+            line += 1
+            i += 1
+            continue
+        if body.startswith("//", i):
+            while i < n and body[i] != "\n":
+                i += 1
+            continue
+        if body.startswith("/*", i):
+            i += 2
+            while i + 1 < n and not body.startswith("*/", i):
+                if body[i] == "\n":
+                    line += 1
+                i += 1
+            i = i + 2 if i + 1 < n else n
+            continue
+        if c in "\"'":
+            q = c
+            i += 1
+            while i < n and body[i] != q:
+                if body[i] == "\\":
+                    i += 1
+                elif body[i] == "\n":
+                    line += 1
+                i += 1
+            i += 1
+            continue
+        if not (c.isalpha() or c == "_"):
+            i += 1
+            continue
+        s = i
+        while i < n and (body[i].isalnum() or body[i] == "_"):
+            i += 1
+        ident = body[s:i]
+        if ident not in ("X", "a"):
+            continue
+        m = re.match(r"[ \t]*\[[ \t]*(\d+)[ \t]*\]", body[i:])
+        if not m:
+            continue
+        idx = int(m.group(1))
+        if ident == "X" and not (0 <= idx < nx):
+            diags.append((line, "X[%d]: the system has %d variables" % (idx, nx)))
+        if ident == "a" and not (0 <= idx < nv):
+            diags.append((line, "a[%d]: a[] has %d entries" % (idx, nv)))
+    return diags
+
+
+def make_source(body, nx):
+    """make_source, U-CUDA/krs_cpu.cpp:293-316 (Linux export, numb = double)."""
+    return ("#include <cmath>\n#include <cstdlib>\nusing std::abs;\ntypedef double numb;\n"
+            "static inline numb min(numb x, numb y) { return x < y ? x : y; }\n"
+            "static inline numb max(numb x, numb y) { return x > y ? x : y; }\n"
+            "#define AMOUNTOFX %d\n"
+            "static const numb pi    = (numb)3.1415926535897932384626433832795;\n"
+            "static const numb euler = (numb)2.7182818284590452353602874713527;\n"
+            "extern \"C\" __attribute__((visibility(\"default\")))\n"
+            "void krs_step(numb* X, const numb* a, numb h) {\n#line 1 \"krs\"\n%s\n}\n") % (nx, body)
+
+
+def _compile(source, suffix, compiler, key):
+    d = os.path.join(_cache_root(), "%016x" % key)
+    so = os.path.join(d, "unit.so")
+    if not os.path.exists(so):
+        os.makedirs(d, exist_ok=True)
+        src = os.path.join(d, "unit" + suffix)
+        with open(src, "w") as f:
+            f.write(source)
+        tmp = so + ".%d.tmp" % os.getpid()
+        subprocess.run([compiler, "-O2", "-ffp-contract=off", "-fno-fast-math", "-shared", "-fPIC",
+                        "-o", tmp, src, "-lm"], check=True, capture_output=True)
+        os.replace(tmp, so)
+    return C.CDLL(so)
+
+
+class KrsStep:
+    """A KRS step body compiled to native `krs_step(numb*, const numb*, numb)`."""
+
+    def __init__(self, body, n_vars, n_values):
+        diags = check_indices(body, n_vars, n_values)
+        if diags:
+            raise ValueError("CPU KRS: " + "; ".join("line %d: %s" % d for d in diags))
+        self.so = _compile(make_source(body, n_vars), ".cpp", "g++", fnv1a_key(body, n_vars, n_values))
+        self.fn = self.so.krs_step
+        self.ptr = C.cast(self.fn, C.c_void_p)
+        self.n_vars = n_vars
+
+    def step(self, x, a, h, n_steps=1):
+        xa = (C.c_double * len(x))(*x)
+        aa = (C.c_double * len(a))(*a)
+        self.fn.argtypes = [_D, _D, C.c_double]
+        for _ in range(n_steps):
+            self.fn(xa, aa, float(h))
+        return list(xa)
+
+
+class SystemFuncs:
+    """uc_rhs / uc_tangent from the engine codegen's C dialect, compiled by gcc."""
+
+    def __init__(self, c_source):
+        src = "#include <math.h>\n" + c_source + (
+            "\n__attribute__((visibility(\"default\"))) void oc_sys_rhs(const double* X, const double* a, double t,"
+            " double* dX) { uc_rhs(X, a, t, dX); }\n"
+            "__attribute__((visibility(\"default\"))) void oc_sys_tan(const double* X, const double* a, double t,"
+            " const double* V, double* JV, int K) { uc_tangent(X, a, t, V, JV, K); }\n")
+        key = int(hashlib.sha1(src.encode()).hexdigest()[:16], 16)
+        self.so = _compile(src, ".c", "gcc", key)
+        self.rhs = C.cast(self.so.oc_sys_rhs, C.c_void_p)
+        self.tan = C.cast(self.so.oc_sys_tan, C.c_void_p)
+
+
+def make_job(n_vars, axes, ic, values, h, transient_time, t_max, max_value=1e6, writable_var=0, pre_scaller=1,
+             max_peaks=256, NT=1.0, eps=1e-4, probe_seed=1234567891, ic_seed=0, ic_lo=-1.0, ic_hi=1.0):
+    j = OcJob()
+    ic = list(ic) if ic is not None else [0.0] * n_vars
+    j._ic = (C.c_double * max(1, len(ic)))(*ic)
+    j._params = (C.c_double * len(values))(*values)
+    j.n_vars, j.n_values = n_vars, len(values)
+    j.ic, j.params = j._ic, j._params
+    j.n_axes = len(axes)
+    for i, ax in enumerate(axes):
+        j.axis_kind[i], j.axis_index[i], j.axis_n[i] = ax["kind"], ax["index"], ax["n_pts"]
+        j.axis_log[i], j.axis_lo[i], j.axis_hi[i] = int(ax["log_scale"]), ax["lo"], ax["hi"]
+    j.h, j.transient_time, j.t_max, j.max_value = h, transient_time, t_max, max_value
+    j.ic_seed, j.ic_lo, j.ic_hi = ic_seed, ic_lo, ic_hi
+    j.writable_var, j.pre_scaller, j.max_peaks = writable_var, pre_scaller, max_peaks
+    j.NT, j.eps, j.probe_seed = NT, eps, probe_seed
+    return j
+
+
+def _cells(cells):
+    cells = np.ascontiguousarray(np.asarray(cells, dtype=np.int64))
+    return cells, cells.ctypes.data_as(_LL)
+
+
+def bif_cells(step, job, cells, traj_len=0, threads=1):
+    cells, cp = _cells(cells)
+    n = len(cells)
+    mp = max(1, job.max_peaks)
+    flags = np.zeros(n, dtype=np.int32)
+    peaks = np.full((mp, n), np.nan)
+    intervals = np.full((mp, n), np.nan)
+    traj = np.full((traj_len, n), np.nan) if traj_len else None
+    lib().oc_bif_cells(step.ptr, C.byref(job), cp, n, flags.ctypes.data_as(_I), peaks.ctypes.data_as(_D),
+                       intervals.ctypes.data_as(_D), traj.ctypes.data_as(_D) if traj is not None else None,
+                       traj_len, threads)
+    return dict(flags=flags, peaks=peaks, intervals=intervals, traj=traj)
+
+
+def lle_fd_cells(step, job, cells, threads=1):
+    cells, cp = _cells(cells)
+    out = np.zeros((1, len(cells)))
+    flags = np.zeros(len(cells), dtype=np.int32)
+    lib().oc_lle_fd_cells(step.ptr, C.byref(job), cp, len(cells), out.ctypes.data_as(_D), flags.ctypes.data_as(_I),
+                          threads)
+    return dict(values=out, flags=flags)
+
+
+def ls_fd_cells(step, job, cells, threads=1):
+    cells, cp = _cells(cells)
+    out = np.zeros((job.n_vars, len(cells)))
+    flags = np.zeros(len(cells), dtype=np.int32)
+    lib().oc_ls_fd_cells(step.ptr, C.byref(job), cp, len(cells), out.ctypes.data_as(_D), flags.ctypes.data_as(_I),
+                         threads)
+    return dict(values=out, flags=flags)
+
+
+def lyap_var_cells(funcs, K, job, cells, threads=1):
+    cells, cp = _cells(cells)
+    out = np.zeros((K, len(cells)))
+    flags = np.zeros(len(cells), dtype=np.int32)
+    lib().oc_lyap_var_cells(funcs.rhs, funcs.tan, K, C.byref(job), cp, len(cells), out.ctypes.data_as(_D),
+                            flags.ctypes.data_as(_I), threads)
+    return dict(values=out, flags=flags)
+
+
+def peak_finder(data, h):
+    d = np.ascontiguousarray(np.asarray(data, dtype=np.float64))
+    pk = np.zeros(max(1, len(d)))
+    tm = np.zeros(max(1, len(d)))
+    n = lib().oc_peak_finder(d.ctypes.data_as(_D), len(d), pk.ctypes.data_as(_D), tm.ctypes.data_as(_D), float(h))
+    return pk[:n].copy(), tm[:n].copy()
+
+
+def rng_uniforms(seed, seq, count):
+    st = C.c_ulonglong()
+    lib().oc_rng_init(seed, seq, 0, C.byref(st))
+    return [lib().oc_rng_uniform(C.byref(st)) for _ in range(count)]
+
+
+def axis_values(log_scale, n, lo, hi):
+    out = (C.c_double * n)()
+    lib().oc_axis_values(int(log_scale), n, float(lo), float(hi), out)
+    return list(out)

@@ -1,0 +1,491 @@
+/* ucuda_oracle.c — FP64 CPU oracle (TEST INFRASTRUCTURE ONLY; see the header).
+ *
+ * Every function cites the reference code it restates. Where the
+ * reference's CPU twin and GPU kernel disagree, the GPU kernel's semantics
+ * are used (the 2-D maps have no CPU twin in the reference), and the choice
+ * is stated at the function.
+ */
+#include "ucuda_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OC_CHECK_INTERVAL 100        /* configCUDA.h:12 */
+#define OC_EPS_FIXED_POINT 1e-8      /* configCUDA.h:84 */
+#define OC_EPS_PEAK_DELTA 1e-14      /* configCUDA.h:85 */
+#define OC_PEAK_THRESHOLD (-1e25)    /* configCUDA.h:86 */
+#define OC_MAX_PEAKS 2500            /* configCUDA.h:88 */
+#define OC_PI 3.1415926535897932384626433832795
+#define OC_E 2.7182818284590452353602874713527
+#define OC_MAXN 32
+#define OC_MAXV 64
+
+typedef unsigned long long u64;
+
+/* ---------------------------------------------------------------- RNG ----
+ * cpu_curand_init / cpu_curand_uniform, U-CUDA/parametric_engine.cpp:196-209
+ * (same constants as the kernel stub, U-CUDA/kernels/lle1d.template.cu:39-57). */
+void oc_rng_init(u64 seed, u64 seq, u64 off, u64* state) {
+    u64 z = seed + seq * 0x9E3779B97F4A7C15ULL + off;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    *state = z ^ (z >> 31);
+}
+
+float oc_rng_uniform(u64* state) {
+    *state = *state * 6364136223846793005ULL + 1442695040888963407ULL;
+    return (float)(((*state >> 40) & 0xFFFFFFULL) + 1ULL) / 16777216.0f;
+}
+
+/* ---------------------------------------------------------------- grid ---
+ * getValueByIdx / getValueByIdx_log, U-CUDA/cudaLibrary.cu:1344-1396 (the
+ * kernel formula; the CPU twin's lo + (hi-lo)*t differs in rounding and in
+ * the n == 1 case, parametric_engine.cpp:177-186). */
+void oc_axis_values(int log_scale, int n, double lo, double hi, double* out) {
+    for (int i = 0; i < n; ++i) {
+        if (log_scale) {
+            if (n == 1) { out[i] = pow(10.0, log10(lo)); continue; }
+            const double l0 = log10(lo), l1 = log10(hi);
+            out[i] = pow(10.0, l0 + (double)i * ((l1 - l0) / (double)(n - 1)));
+        } else {
+            if (n == 1) { out[i] = hi; continue; }
+            const double t = (double)i / (double)(n - 1);
+            out[i] = (1.0 - t) * lo + t * hi;
+        }
+    }
+}
+
+static double axis_at(const oc_job* j, int ax, long long idx) {
+    double v;
+    /* one value only: cheap to recompute, identical formula */
+    const int n = j->axis_n[ax];
+    if (j->axis_log[ax]) {
+        if (n == 1) return pow(10.0, log10(j->axis_lo[ax]));
+        const double l0 = log10(j->axis_lo[ax]), l1 = log10(j->axis_hi[ax]);
+        v = pow(10.0, l0 + (double)idx * ((l1 - l0) / (double)(n - 1)));
+    } else {
+        if (n == 1) return j->axis_hi[ax];
+        const double t = (double)idx / (double)(n - 1);
+        v = (1.0 - t) * j->axis_lo[ax] + t * j->axis_hi[ax];
+    }
+    return v;
+}
+
+/* Cell -> parameters / initial state / step, as calculateDiscreteModelCUDA
+ * does (U-CUDA/cudaLibrary.cu:990-1058): axis 0 index = cell % n0, axis 1
+ * index = (cell / n0) % n1. Optional per-cell random initial state keyed by
+ * (ic_seed, cell). Returns 0 when the step is not positive. */
+int oc_cell_setup(const oc_job* j, long long cell, double* a, double* x0, double* h) {
+    for (int i = 0; i < j->n_values; ++i) a[i] = j->params[i];
+    if (j->ic_seed) {
+        u64 st;
+        oc_rng_init(j->ic_seed, (u64)cell, 0ULL, &st);
+        for (int i = 0; i < j->n_vars; ++i) x0[i] = j->ic_lo + (j->ic_hi - j->ic_lo) * (double)oc_rng_uniform(&st);
+    } else {
+        for (int i = 0; i < j->n_vars; ++i) x0[i] = j->ic[i];
+    }
+    *h = j->h;
+    for (int ax = 0; ax < j->n_axes; ++ax) {
+        const long long idx = ax == 0 ? cell % j->axis_n[0] : (cell / j->axis_n[0]) % j->axis_n[1];
+        const double v = axis_at(j, ax, idx);
+        if (j->axis_kind[ax] == OC_AXIS_PARAM) a[j->axis_index[ax]] = v;
+        else if (j->axis_kind[ax] == OC_AXIS_VAR) x0[j->axis_index[ax]] = v;
+        else if (j->axis_kind[ax] == OC_AXIS_STEP) *h = v;
+    }
+    return *h > 0.0;
+}
+
+static int sweeps_step(const oc_job* j) {
+    for (int ax = 0; ax < j->n_axes; ++ax)
+        if (j->axis_kind[ax] == OC_AXIS_STEP) return 1;
+    return 0;
+}
+
+static double worst_step(const oc_job* j) {
+    for (int ax = 0; ax < j->n_axes; ++ax)
+        if (j->axis_kind[ax] == OC_AXIS_STEP) return j->axis_lo[ax] < j->axis_hi[ax] ? j->axis_lo[ax] : j->axis_hi[ax];
+    return j->h;
+}
+
+/* ----------------------------------------------------------- step loop ---
+ * cpu_loop_model, U-CUDA/parametric_engine.cpp:59-101 (≡ the kernel's
+ * loopCalculateDiscreteModel_int, cudaLibrary.cu:782-895). A generic
+ * "advance" callback lets the same loop drive a compiled KRS step, the
+ * engine-order RK4 on a right-hand side, or state + tangent vectors. */
+typedef void (*oc_adv_fn)(void* ctx, double* x, const double* a, double h);
+
+static int loop_generic(oc_adv_fn adv, void* ctx, double* x, const double* a, double h, long long iters, int n,
+                        int pre, int wvar, double max_value, double* data) {
+    for (long long i = 0; i < iters; ++i) {
+        if (data) {
+            if (wvar < 0) {
+                if (n >= 3) data[i] = x[0] + OC_PI * x[1] + OC_E * x[2];
+                else if (n == 2) data[i] = x[0] + OC_PI * x[1];
+                else data[i] = x[0];
+            } else {
+                data[i] = x[wvar];
+            }
+            for (int p = 0; p < pre; ++p) adv(ctx, x, a, h);
+        } else {
+            adv(ctx, x, a, h);
+        }
+        if (i % OC_CHECK_INTERVAL == 0) {
+            double chk = 0.0;
+            for (int q = 0; q < n; ++q) chk += fabs(x[q]);
+            if (isnan(chk) || isinf(chk)) return 0;
+            if (max_value != 0.0 && fabs(chk) > max_value) return 0;
+        }
+    }
+    double prev[OC_MAXN];
+    memcpy(prev, x, sizeof(double) * (size_t)n);
+    for (int p = 0; p < pre; ++p) adv(ctx, x, a, h);
+    double d = 0.0;
+    for (int q = 0; q < n; ++q) d += fabs(x[q] - prev[q]);
+    if (fabs(d) < OC_EPS_FIXED_POINT) return -1;
+    return 1;
+}
+
+static void adv_krs(void* ctx, double* x, const double* a, double h) { ((oc_step_fn)ctx)(x, a, h); }
+
+int oc_loop_model(oc_step_fn step, double* x, const double* a, double h, long long iters, int n, int pre,
+                  int wvar, double max_value, double* data) {
+    return loop_generic(adv_krs, (void*)step, x, a, h, iters, n, pre, wvar, max_value, data);
+}
+
+/* --------------------------------------------------------- peak finder ---
+ * cpu_peak_finder, U-CUDA/parametric_engine.cpp:111-166 (≡ peakFinder,
+ * cudaLibrary.cu:1503-1633): maxima with a plateau walk, parabolic
+ * refinement, first peak dropped, times turned into intervals * h. */
+int oc_peak_finder(const double* d, long long n, double* peaks, double* times, double h) {
+    if (n < 5) return 0;
+    int count = 0;
+    for (long long i = 2; i + 2 < n; ++i) {
+        if (d[i] - d[i - 1] > OC_EPS_PEAK_DELTA && d[i] > OC_PEAK_THRESHOLD && d[i] >= d[i + 1]) {
+            for (long long j = i; j + 2 < n; ++j) {
+                if (d[j] < d[j + 1]) { i = j + 1; break; }
+                if (d[j] - d[j + 1] > OC_EPS_PEAK_DELTA) {
+                    const double denom = d[j - 1] - 2.0 * d[j] + d[j + 1];
+                    double delta = 0.0;
+                    if (fabs(denom) > 1e-12) delta = 0.5 * (d[j - 1] - d[j + 1]) / denom;
+                    peaks[count] = d[j] - 0.25 * (d[j - 1] - d[j + 1]) * delta;
+                    times[count] = (double)(j - 1) + delta;
+                    ++count;
+                    i = j + 1;
+                    break;
+                }
+            }
+        }
+    }
+    if (count > 1) {
+        for (int i = 0; i < count - 1; ++i) {
+            peaks[i] = peaks[i + 1];
+            times[i] = (times[i + 1] - times[i]) * h;
+        }
+        count -= 1;
+    } else {
+        count = 0;
+    }
+    if (count >= OC_MAX_PEAKS) count = OC_MAX_PEAKS;
+    return count;
+}
+
+/* ------------------------------------------------------- Gram-Schmidt ---
+ * cpu_gram_schmidt / cpu_projection_operator, U-CUDA/parametric_engine.cpp:858-885
+ * (classical: projections of the ORIGINAL a_i on the unnormalised b_j). */
+void oc_cgs(const double* a, double* b, int n, double* den) {
+    for (int i = 0; i < n; ++i) {
+        for (int q = 0; q < n; ++q) b[q + i * n] = a[q + i * n];
+        for (int jj = 0; jj < i; ++jj) {
+            double num = 0.0, dd = 0.0;
+            for (int q = 0; q < n; ++q) { num += a[i * n + q] * b[jj * n + q]; dd += b[jj * n + q] * b[jj * n + q]; }
+            const double f = (dd == 0.0) ? 0.0 : num / dd;
+            for (int q = 0; q < n; ++q) b[i * n + q] -= f * b[jj * n + q];
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        double nrm = 0.0;
+        for (int q = 0; q < n; ++q) nrm += b[i * n + q] * b[i * n + q];
+        nrm = sqrt(nrm);
+        for (int q = 0; q < n; ++q) b[i * n + q] = (nrm == 0.0) ? 0.0 : b[i * n + q] / nrm;
+        if (den) den[i] = nrm;
+    }
+}
+
+/* Modified Gram-Schmidt used by the variational method (K row vectors). */
+void oc_mgs(double* v, int K, int n, double* d) {
+    for (int k = 0; k < K; ++k) {
+        for (int q = 0; q < k; ++q) {
+            double r = 0.0;
+            for (int i = 0; i < n; ++i) r = r + v[k * n + i] * v[q * n + i];
+            for (int i = 0; i < n; ++i) v[k * n + i] = v[k * n + i] - r * v[q * n + i];
+        }
+        double nrm = 0.0;
+        for (int i = 0; i < n; ++i) nrm = nrm + v[k * n + i] * v[k * n + i];
+        nrm = sqrt(nrm);
+        d[k] = nrm;
+        for (int i = 0; i < n; ++i) v[k * n + i] = (nrm == 0.0) ? 0.0 : v[k * n + i] / nrm;
+    }
+}
+
+/* ----------------------------------------------------------------- RK4 ---
+ * Classic RK4 in the stage order of the reference KRS body
+ * (U-CUDA/codegen.cpp:597-606): X1 = X + 0.5*h*k (stages 0, 1), X1 = X + h*k
+ * (stage 2), X += h*(k0 + 2*k1 + 2*k2 + k3)/6, summed left to right. */
+void oc_rk4(oc_rhs_fn rhs, double* x, const double* a, double h, int n) {
+    double x1[OC_MAXN], k[OC_MAXN], acc[OC_MAXN];
+    const double hh = 0.5 * h;
+    rhs(x, a, 0.0, k);
+    for (int i = 0; i < n; ++i) { acc[i] = k[i]; x1[i] = x[i] + hh * k[i]; }
+    rhs(x1, a, hh, k);
+    for (int i = 0; i < n; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = x[i] + hh * k[i]; }
+    rhs(x1, a, hh, k);
+    for (int i = 0; i < n; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = x[i] + h * k[i]; }
+    rhs(x1, a, h, k);
+    for (int i = 0; i < n; ++i) { acc[i] = acc[i] + k[i]; x[i] = x[i] + h * acc[i] / 6.0; }
+}
+
+/* RK4 applied to the variational system V' = J(X) V with the same stages
+ * (the exact derivative of the discrete RK4 map). */
+void oc_rk4_tan(oc_rhs_fn rhs, oc_tan_fn tan, double* x, double* V, int K, const double* a, double h, int n) {
+    double x1[OC_MAXN], k[OC_MAXN], acc[OC_MAXN];
+    double v1[OC_MAXN * OC_MAXN], l[OC_MAXN * OC_MAXN], lacc[OC_MAXN * OC_MAXN];
+    const double hh = 0.5 * h;
+    const int m = K * n;
+    rhs(x, a, 0.0, k);
+    tan(x, a, 0.0, V, l, K);
+    for (int i = 0; i < n; ++i) { acc[i] = k[i]; x1[i] = x[i] + hh * k[i]; }
+    for (int i = 0; i < m; ++i) { lacc[i] = l[i]; v1[i] = V[i] + hh * l[i]; }
+    rhs(x1, a, hh, k);
+    tan(x1, a, hh, v1, l, K);
+    for (int i = 0; i < n; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = x[i] + hh * k[i]; }
+    for (int i = 0; i < m; ++i) { lacc[i] = lacc[i] + 2.0 * l[i]; v1[i] = V[i] + hh * l[i]; }
+    rhs(x1, a, hh, k);
+    tan(x1, a, hh, v1, l, K);
+    for (int i = 0; i < n; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = x[i] + h * k[i]; }
+    for (int i = 0; i < m; ++i) { lacc[i] = lacc[i] + 2.0 * l[i]; v1[i] = V[i] + h * l[i]; }
+    rhs(x1, a, h, k);
+    tan(x1, a, h, v1, l, K);
+    for (int i = 0; i < n; ++i) { acc[i] = acc[i] + k[i]; x[i] = x[i] + h * acc[i] / 6.0; }
+    for (int i = 0; i < m; ++i) { lacc[i] = lacc[i] + l[i]; V[i] = V[i] + h * lacc[i] / 6.0; }
+}
+
+typedef struct { oc_rhs_fn rhs; oc_tan_fn tan; double* V; int K; int n; } tan_ctx;
+static void adv_rhs(void* ctx, double* x, const double* a, double h) {
+    tan_ctx* c = (tan_ctx*)ctx;
+    oc_rk4(c->rhs, x, a, h, c->n);
+}
+static void adv_tan(void* ctx, double* x, const double* a, double h) {
+    tan_ctx* c = (tan_ctx*)ctx;
+    oc_rk4_tan(c->rhs, c->tan, x, c->V, c->K, a, h, c->n);
+}
+
+/* Random unit probes (LLEKernelCUDA :2603-2626, LSKernelCUDA :2980-2997):
+ * K vectors of n draws u - 0.5, each normalised; subsequence = global cell. */
+static void probe_basis(u64 seed, long long cell, int K, int n, double* z) {
+    u64 st;
+    oc_rng_init(seed, (u64)cell, 0ULL, &st);
+    for (int k = 0; k < K; ++k) {
+        double p = 0.0;
+        for (int i = 0; i < n; ++i) { z[k * n + i] = (double)(oc_rng_uniform(&st) - 0.5); p += z[k * n + i] * z[k * n + i]; }
+        p = sqrt(p);
+        for (int i = 0; i < n; ++i) z[k * n + i] /= p;
+    }
+}
+
+/* -------------------------------------------------------- bifurcation ---
+ * One cell of calculateDiscreteModelCUDA + peakFinderCUDA
+ * (cudaLibrary.cu:944-1077, 1641-1674; host counts parametric_engine.cpp:1796-1797).
+ * Transient as the kernel runs it: one step per iteration, pre_scaller
+ * trailing steps (the CPU twin passes pre_scaller = 1 there, :472-475).
+ * Recorded run only for transient flag 1 / -1. Under an h axis the counts
+ * are recomputed per cell with ceil (cudaLibrary.cu:1028-1030). */
+static void bif_cell(oc_step_fn step, const oc_job* j, long long cell, long long iters_max, double* data,
+                     double* pk, double* tm, int* flag_out, double* peaks, double* intervals, long long stride,
+                     double* traj, int traj_len) {
+    double a[OC_MAXV], x[OC_MAXN], h;
+    if (!oc_cell_setup(j, cell, a, x, &h)) { *flag_out = -1; return; }
+    long long skip = j->h > 0.0 ? (long long)(j->transient_time / j->h) : 0, iters = iters_max;
+    if (sweeps_step(j)) {
+        skip = (long long)ceil(j->transient_time / h);
+        long long it = (long long)ceil(j->t_max / h / (double)j->pre_scaller);
+        if (it < iters) iters = it;
+    }
+    int flag = oc_loop_model(step, x, a, h, skip, j->n_vars, j->pre_scaller, 0, j->max_value, NULL);
+    if (flag == 1 || flag == -1)
+        flag = oc_loop_model(step, x, a, h, iters, j->n_vars, j->pre_scaller, j->writable_var, j->max_value, data);
+    else
+        return (void)(*flag_out = flag);
+    for (int i = 0; i < traj_len && i < iters; ++i) traj[(long long)i * stride] = data[i];
+    if (flag == -1 || flag == 0) { *flag_out = flag; return; }
+    int cnt = oc_peak_finder(data, iters, pk, tm, h * (double)j->pre_scaller);
+    if (cnt > j->max_peaks) cnt = j->max_peaks;
+    *flag_out = cnt;
+    for (int k = 0; k < cnt; ++k) {
+        peaks[(long long)k * stride] = pk[k];
+        intervals[(long long)k * stride] = tm[k];
+    }
+}
+
+void oc_bif_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int* flags,
+                  double* peaks, double* intervals, double* traj, int traj_len, int threads) {
+    const long long iters_max = (long long)ceil(j->t_max / worst_step(j) / j->pre_scaller);
+#pragma omp parallel num_threads(threads > 1 ? threads : 1)
+    {
+        double* data = (double*)malloc(sizeof(double) * (size_t)(iters_max + 8));
+        double* pk = (double*)malloc(sizeof(double) * (size_t)(iters_max + 8));
+        double* tm = (double*)malloc(sizeof(double) * (size_t)(iters_max + 8));
+#pragma omp for schedule(dynamic, 1)
+        for (long long s = 0; s < n_sel; ++s)
+            bif_cell(step, j, cells[s], iters_max, data, pk, tm, &flags[s], peaks + s, intervals + s, n_sel,
+                     traj ? traj + s : NULL, traj ? traj_len : 0);
+        free(data);
+        free(pk);
+        free(tm);
+    }
+}
+
+static void lyap_counts(const oc_job* j, double h, long long* skip, long long* nts) {
+    if (sweeps_step(j)) {
+        *nts = (long long)ceil(j->NT / h);
+        *skip = (long long)ceil(j->transient_time / h);
+    } else {
+        *nts = (long long)(j->NT / j->h);
+        *skip = (long long)(j->transient_time / j->h);
+    }
+}
+
+/* -------------------------------------------------------------- LLE FD ---
+ * One cell of LLEKernelCUDA (cudaLibrary.cu:2486-2693; CPU twin
+ * run_lle1d_cpu, parametric_engine.cpp:618-787). The distance is formed as
+ * (1/eps)*(x - y) as in the kernel (the CPU twin divides, :697-700). */
+static void lle_fd_cell(oc_step_fn step, const oc_job* j, long long cell, double* out, int* flag) {
+    const int n = j->n_vars;
+    double a[OC_MAXV], x[OC_MAXN], y[OC_MAXN], z[OC_MAXN], h;
+    if (!oc_cell_setup(j, cell, a, x, &h)) { *out = NAN; *flag = -1; return; }
+    long long skip, nts;
+    lyap_counts(j, h, &skip, &nts);
+    const long long blocks = (long long)(j->t_max / j->NT);
+    if (oc_loop_model(step, x, a, h, skip, n, 1, 0, j->max_value, NULL) == 0) { *out = 999.0; *flag = -1; return; }
+    probe_basis(j->probe_seed, cell, 1, n, z);
+    for (int i = 0; i < n; ++i) y[i] = z[i] * j->eps + x[i];
+    double sum = 0.0;
+    for (long long b = 0; b < blocks; ++b) {
+        if (oc_loop_model(step, x, a, h, nts, n, 1, 0, j->max_value, NULL) == 0) { *out = 999.0; *flag = -1; return; }
+        if (oc_loop_model(step, y, a, h, nts, n, 1, 0, j->max_value, NULL) == 0) { *out = 999.0; *flag = -1; return; }
+        double d = 0.0;
+        for (int l = 0; l < n; ++l) {
+            const double t2 = (1.0 / j->eps) * (x[l] - y[l]);
+            d += t2 * t2;
+        }
+        d = sqrt(d);
+        if (d <= 1e-14) d = 1e-14;
+        sum += log(d);
+        const double inv = 1.0 / d;
+        for (int l = 0; l < n; ++l) y[l] = x[l] - ((x[l] - y[l] + 1e-14) * inv);
+    }
+    *out = sum / j->t_max;
+    *flag = 1;
+}
+
+void oc_lle_fd_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, double* out,
+                     int* flags, int threads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 1 ? threads : 1)
+    for (long long s = 0; s < n_sel; ++s) lle_fd_cell(step, j, cells[s], &out[s], &flags[s]);
+}
+
+/* --------------------------------------------------------------- LS FD ---
+ * One cell of LSKernelCUDA (cudaLibrary.cu:2878-3073; CPU twin
+ * run_ls1d_cpu, parametric_engine.cpp:901-1074). out[k * stride]. */
+static void ls_fd_cell(oc_step_fn step, const oc_job* j, long long cell, double* out, long long stride, int* flag) {
+    const int n = j->n_vars;
+    double a[OC_MAXV], x[OC_MAXN], h;
+    double y[OC_MAXN * OC_MAXN], z[OC_MAXN * OC_MAXN], den[OC_MAXN], sum[OC_MAXN];
+    if (!oc_cell_setup(j, cell, a, x, &h)) {
+        for (int k = 0; k < n; ++k) out[k * stride] = NAN;
+        *flag = -1;
+        return;
+    }
+    long long skip, nts;
+    lyap_counts(j, h, &skip, &nts);
+    const long long blocks = (long long)(j->t_max / j->NT);
+    probe_basis(j->probe_seed, cell, n, n, z);
+    int alive = oc_loop_model(step, x, a, h, skip, n, 1, 0, j->max_value, NULL) != 0;
+    if (alive) {
+        oc_cgs(z, y, n, NULL);
+        for (int k = 0; k < n; ++k)
+            for (int i = 0; i < n; ++i) y[k * n + i] = y[k * n + i] * j->eps + x[i];
+        for (int k = 0; k < n; ++k) sum[k] = 0.0;
+        for (long long b = 0; b < blocks && alive; ++b) {
+            if (oc_loop_model(step, x, a, h, nts, n, 1, 0, j->max_value, NULL) == 0) { alive = 0; break; }
+            for (int k = 0; k < n; ++k)
+                if (oc_loop_model(step, y + k * n, a, h, nts, n, 1, 0, j->max_value, NULL) == 0) { alive = 0; break; }
+            if (!alive) break;
+            for (int k = 0; k < n; ++k)
+                for (int l = 0; l < n; ++l) y[k * n + l] = y[k * n + l] - x[l];
+            oc_cgs(y, z, n, den);
+            for (int k = 0; k < n; ++k) {
+                sum[k] += log(den[k] / j->eps);
+                for (int q = 0; q < n; ++q) y[k * n + q] = x[q] + z[k * n + q] * j->eps;
+            }
+        }
+    }
+    for (int k = 0; k < n; ++k) out[k * stride] = alive ? sum[k] / j->t_max : 999.0;
+    *flag = alive ? 1 : -1;
+}
+
+void oc_ls_fd_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, double* out,
+                    int* flags, int threads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 1 ? threads : 1)
+    for (long long s = 0; s < n_sel; ++s) ls_fd_cell(step, j, cells[s], out + s, n_sel, &flags[s]);
+}
+
+/* ------------------------------------------------------ variational ---
+ * The engine's variational method on the reference block schedule: the
+ * transient as in the FD kernels, then blocks of NT/h steps plus the step
+ * loop's trailing step; K = 1 renormalises (floor 1e-14 like the FD
+ * distance), K > 1 re-orthonormalises by modified Gram-Schmidt; exponents
+ * are the log-norm sums over t_max. */
+static void lyap_var_cell(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, long long cell, double* out,
+                          long long stride, int* flag) {
+    const int n = j->n_vars;
+    double a[OC_MAXV], x[OC_MAXN], h, V[OC_MAXN * OC_MAXN], d[OC_MAXN], sum[OC_MAXN];
+    if (!oc_cell_setup(j, cell, a, x, &h)) {
+        for (int k = 0; k < K; ++k) out[k * stride] = NAN;
+        *flag = -1;
+        return;
+    }
+    long long skip, nts;
+    lyap_counts(j, h, &skip, &nts);
+    const long long blocks = (long long)(j->t_max / j->NT);
+    tan_ctx ctx = {rhs, tan, V, K, n};
+    int alive = loop_generic(adv_rhs, &ctx, x, a, h, skip, n, 1, 0, j->max_value, NULL) != 0;
+    for (int k = 0; k < K; ++k) sum[k] = 0.0;
+    if (alive) {
+        probe_basis(j->probe_seed, cell, K, n, V);
+        if (K > 1) oc_mgs(V, K, n, d);
+        for (long long b = 0; b < blocks; ++b) {
+            if (loop_generic(adv_tan, &ctx, x, a, h, nts, n, 1, 0, j->max_value, NULL) == 0) { alive = 0; break; }
+            if (K == 1) {
+                double nrm = 0.0;
+                for (int i = 0; i < n; ++i) nrm = nrm + V[i] * V[i];
+                nrm = sqrt(nrm);
+                if (nrm <= 1e-14) nrm = 1e-14;
+                sum[0] = sum[0] + log(nrm);
+                const double inv = 1.0 / nrm;
+                for (int i = 0; i < n; ++i) V[i] = V[i] * inv;
+            } else {
+                oc_mgs(V, K, n, d);
+                for (int k = 0; k < K; ++k) sum[k] = sum[k] + log(d[k]);
+            }
+        }
+    }
+    for (int k = 0; k < K; ++k) out[k * stride] = alive ? sum[k] / j->t_max : 999.0;
+    *flag = alive ? 1 : -1;
+}
+
+void oc_lyap_var_cells(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, const long long* cells, long long n_sel,
+                       double* out, int* flags, int threads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 1 ? threads : 1)
+    for (long long s = 0; s < n_sel; ++s) lyap_var_cell(rhs, tan, K, j, cells[s], out + s, n_sel, &flags[s]);
+}

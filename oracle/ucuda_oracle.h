@@ -1,0 +1,84 @@
+/* ucuda_oracle.h — FP64 CPU oracle for the ensemble ODE analysis path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is the checker the GPU engine is compared
+ * against; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg use it. The product library never links or calls it.
+ *
+ * It restates, in C with numb = double, the reference's CPU path:
+ *   krs_cpu            step body -> native krs_step (U-CUDA/krs_cpu.cpp:293-332,405-479)
+ *   cpu_loop_model     (U-CUDA/parametric_engine.cpp:59-101)
+ *   cpu_peak_finder    (U-CUDA/parametric_engine.cpp:111-166)
+ *   cpu_curand_*       (U-CUDA/parametric_engine.cpp:196-209)
+ *   Gram-Schmidt       (U-CUDA/parametric_engine.cpp:858-885)
+ *   bif / LLE / LS per-point drivers (U-CUDA/parametric_engine.cpp:374-521,
+ *   618-787, 901-1074) with the GPU kernels' cell semantics where the two
+ *   differ (U-CUDA/cudaLibrary.cu:944-1077, 1344-1396, 2486-2693, 2878-3073),
+ * plus the variational Lyapunov method (new in this engine, same block
+ * schedule as the reference).
+ */
+#ifndef UCUDA_ORACLE_H
+#define UCUDA_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void (*oc_step_fn)(double* X, const double* a, double h);
+typedef void (*oc_rhs_fn)(const double* X, const double* a, double t, double* dX);
+typedef void (*oc_tan_fn)(const double* X, const double* a, double t, const double* V, double* JV, int K);
+
+#define OC_AXIS_NONE 0
+#define OC_AXIS_PARAM 1
+#define OC_AXIS_VAR 2
+#define OC_AXIS_STEP 3
+
+typedef struct oc_job {
+    int n_vars, n_values;
+    const double* ic;
+    const double* params;
+    int n_axes;
+    int axis_kind[2], axis_index[2], axis_n[2], axis_log[2];
+    double axis_lo[2], axis_hi[2];
+    double h, transient_time, t_max, max_value;
+    unsigned long long ic_seed;
+    double ic_lo, ic_hi;
+    int writable_var, pre_scaller, max_peaks;
+    double NT, eps;
+    unsigned long long probe_seed;
+} oc_job;
+
+/* RNG */
+void oc_rng_init(unsigned long long seed, unsigned long long seq, unsigned long long off, unsigned long long* state);
+float oc_rng_uniform(unsigned long long* state);
+
+/* grid */
+void oc_axis_values(int log_scale, int n, double lo, double hi, double* out);
+int oc_cell_setup(const oc_job* j, long long cell, double* a, double* x0, double* h);
+
+/* loop + peaks */
+int oc_loop_model(oc_step_fn step, double* x, const double* a, double h, long long iters, int n, int pre,
+                  int wvar, double max_value, double* data);
+int oc_peak_finder(const double* data, long long n, double* peaks, double* times, double h);
+
+/* Gram-Schmidt */
+void oc_cgs(const double* a, double* b, int n, double* den);
+void oc_mgs(double* v, int K, int n, double* d);
+
+/* one RK4 step of state (+ K tangents) with the engine's stage order */
+void oc_rk4(oc_rhs_fn rhs, double* x, const double* a, double h, int n);
+void oc_rk4_tan(oc_rhs_fn rhs, oc_tan_fn tan, double* x, double* V, int K, const double* a, double h, int n);
+
+/* per-cell drivers over a selection of global cell indices (threads <= 1: serial) */
+void oc_bif_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int* flags,
+                  double* peaks, double* intervals, double* traj, int traj_len, int threads);
+void oc_lle_fd_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, double* out,
+                     int* flags, int threads);
+void oc_ls_fd_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, double* out,
+                    int* flags, int threads);
+void oc_lyap_var_cells(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, const long long* cells, long long n_sel,
+                       double* out, int* flags, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

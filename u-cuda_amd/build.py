@@ -1,0 +1,69 @@
+"""Builds the engine library in-tree.
+
+libucuda.so is the C-ABI engine (codegen + hipRTC + launches). The ensemble
+kernels (csrc/kernels/ensemble.hip) and the launch descriptor (csrc/uc_job.h)
+are embedded into it as text; hipRTC compiles them for gfx950 together with
+each system's generated functions. `check_kernels()` runs that compilation
+for the bundled systems without a GPU, so a broken kernel fails the build.
+"""
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libucuda.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+def _raw_literal(name, text):
+    delim = "UCSRC"
+    assert ")" + delim + '"' not in text
+    # split into chunks: some compilers limit a single literal's length
+    parts = [text[i:i + 30000] for i in range(0, len(text), 30000)] or [""]
+    body = "\n".join('R"%s(%s)%s"' % (delim, p, delim) for p in parts)
+    return "static const char* %s =\n%s;\n" % (name, body)
+
+
+def write_embed():
+    with open(os.path.join(CSRC, "kernels", "ensemble.hip")) as f:
+        ens = f.read()
+    with open(os.path.join(CSRC, "uc_job.h")) as f:
+        job = f.read()
+    out = "// generated by u-cuda_amd/build.py — do not edit\n"
+    out += _raw_literal("kEnsembleSource", ens) + _raw_literal("kJobHeader", job)
+    path = os.path.join(CSRC, "kernels_embed.inc")
+    old = open(path).read() if os.path.exists(path) else None
+    if old != out:
+        with open(path, "w") as f:
+            f.write(out)
+
+
+def build_library(verbose=False):
+    write_embed()
+    cmd = [os.path.join(ROCM, "bin", "hipcc"), "-O2", "-std=c++17", "-fPIC", "-shared",
+           "-Wall", "-Wno-unused-function",
+           "-o", LIB + ".tmp",
+           os.path.join(CSRC, "engine.cpp"), os.path.join(CSRC, "codegen.cpp"),
+           "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-lamdhip64",
+           "-Wl,-rpath," + os.path.join(ROCM, "lib")]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(LIB + ".tmp", LIB)
+    return LIB
+
+
+def check_kernels():
+    """hipRTC-compiles the kernel library for bundled systems (no GPU needed)."""
+    import ucuda  # noqa: E402  (sys.path holds this directory)
+    eng = ucuda.Engine(device=-1)
+    sizes = {}
+    for name in ("lorenz", "rossler", "hyper_rossler", "duffing"):
+        sysm = ucuda.preset(name, eng)
+        for strict in (False, True):
+            sizes[(name, strict)] = sysm.compile(strict=strict)
+    return sizes
+
+
+if __name__ == "__main__":
+    build_library(verbose=True)

@@ -1,0 +1,769 @@
+// codegen.cpp — parser, expression trees, KRS step text and HIP/C emitters.
+//
+// Parsing follows the reference grammar (U-CUDA/codegen.cpp:59-234) so the same
+// input text means the same mathematics:
+//   expr  := term (('+'|'-') term)*
+//   term  := power (('*'|'/'|'\cdot'|implicit product in LaTeX) power)*
+//   power := unary ['^' power]        (LaTeX: '^' takes a braced argument)
+//   unary := '-' unary | '+' unary | primary
+// so a leading minus binds tighter than '^' ("-x^2" is (-x)^2), as in the
+// reference. Printing follows U-CUDA/codegen.cpp:236-291 (precedence-driven
+// parentheses, integer powers 2..4 expanded to products, integers printed as
+// "N.0", other numbers as the shortest round-trip "%.{p}g").
+#include "codegen.hpp"
+
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <memory>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+
+namespace ucuda {
+namespace {
+
+// ------------------------------------------------------------------ trees --
+enum class K { Num, Var, Par, Time, Const, Call, Add, Sub, Mul, Div, Pow, Neg };
+
+struct Node;
+using P = std::shared_ptr<const Node>;
+struct Node {
+    K k;
+    double num = 0.0;      // Num
+    int idx = 0;           // Var: state index, Par: a[] index (1-based)
+    std::string name;      // Call / Const / unresolved symbol
+    P a, b;                // operands
+    std::vector<P> args;   // Call
+};
+
+P num(double v) { auto n = std::make_shared<Node>(); n->k = K::Num; n->num = v; return n; }
+P bin(K k, P a, P b) { auto n = std::make_shared<Node>(); n->k = k; n->a = std::move(a); n->b = std::move(b); return n; }
+P neg(P a) { auto n = std::make_shared<Node>(); n->k = K::Neg; n->a = std::move(a); return n; }
+P call(const std::string& f, std::vector<P> args) {
+    auto n = std::make_shared<Node>(); n->k = K::Call; n->name = f; n->args = std::move(args); return n;
+}
+P sym(const std::string& s) { auto n = std::make_shared<Node>(); n->k = K::Const; n->name = s; return n; }
+
+const std::set<std::string>& plain_functions() {
+    static const std::set<std::string> f = {
+        "sin", "cos", "tan", "asin", "acos", "atan", "sinh", "cosh", "tanh",
+        "exp", "log", "log2", "log10", "sqrt", "cbrt", "fabs", "abs",
+        "pow", "atan2", "fmod", "floor", "ceil"};
+    return f;
+}
+const std::map<std::string, std::string>& latex_functions() {
+    static const std::map<std::string, std::string> m = {
+        {"sin", "sin"}, {"cos", "cos"}, {"tan", "tan"}, {"arcsin", "asin"},
+        {"arccos", "acos"}, {"arctan", "atan"}, {"sinh", "sinh"}, {"cosh", "cosh"},
+        {"tanh", "tanh"}, {"exp", "exp"}, {"ln", "log"}, {"log", "log"}, {"sqrt", "sqrt"}};
+    return m;
+}
+const std::map<std::string, std::string>& latex_letters() {
+    static const std::map<std::string, std::string> m = {
+        {"alpha", "alpha"}, {"beta", "beta"}, {"gamma", "gamma"}, {"delta", "delta"},
+        {"epsilon", "epsilon"}, {"varepsilon", "epsilon"}, {"zeta", "zeta"}, {"eta", "eta"},
+        {"theta", "theta"}, {"vartheta", "theta"}, {"iota", "iota"}, {"kappa", "kappa"},
+        {"lambda", "lambda"}, {"mu", "mu"}, {"nu", "nu"}, {"xi", "xi"}, {"pi", "pi"},
+        {"rho", "rho"}, {"sigma", "sigma"}, {"tau", "tau"}, {"phi", "phi"}, {"varphi", "phi"},
+        {"chi", "chi"}, {"psi", "psi"}, {"omega", "omega"}, {"Gamma", "Gamma"},
+        {"Delta", "Delta"}, {"Theta", "Theta"}, {"Lambda", "Lambda"}, {"Sigma", "Sigma"},
+        {"Phi", "Phi"}, {"Psi", "Psi"}, {"Omega", "Omega"}};
+    return m;
+}
+
+// ----------------------------------------------------------------- parser --
+class Parser {
+public:
+    Parser(const std::string& text, bool latex) : s_(text), latex_(latex) {}
+
+    P parse() {
+        P e = expr();
+        skip();
+        if (pos_ != s_.size()) fail("trailing input at pos " + std::to_string(pos_));
+        return e;
+    }
+
+private:
+    const std::string s_;
+    size_t pos_ = 0;
+    bool latex_;
+
+    [[noreturn]] void fail(const std::string& m) const { throw std::runtime_error(m); }
+
+    void skip() {
+        for (;;) {
+            while (pos_ < s_.size() && std::isspace((unsigned char)s_[pos_])) ++pos_;
+            // LaTeX spacing commands carry no meaning in an expression.
+            if (latex_ && pos_ + 1 < s_.size() && s_[pos_] == '\\' &&
+                (s_[pos_ + 1] == ',' || s_[pos_ + 1] == ';' || s_[pos_ + 1] == '!' ||
+                 s_[pos_ + 1] == ' ' || s_[pos_ + 1] == ':')) {
+                pos_ += 2;
+                continue;
+            }
+            break;
+        }
+    }
+    char peek() { skip(); return pos_ < s_.size() ? s_[pos_] : '\0'; }
+    bool accept(char c) { if (peek() == c) { ++pos_; return true; } return false; }
+    bool starts(const char* w) const { return s_.compare(pos_, std::strlen(w), w) == 0; }
+
+    std::string command_name() {
+        size_t j = pos_;
+        while (j < s_.size() && std::isalpha((unsigned char)s_[j])) ++j;
+        std::string c = s_.substr(pos_, j - pos_);
+        pos_ = j;
+        return c;
+    }
+    double number() {
+        size_t j = pos_;
+        while (j < s_.size() && (std::isdigit((unsigned char)s_[j]) || s_[j] == '.')) ++j;
+        if (j < s_.size() && (s_[j] == 'e' || s_[j] == 'E')) {
+            ++j;
+            if (j < s_.size() && (s_[j] == '+' || s_[j] == '-')) ++j;
+            while (j < s_.size() && std::isdigit((unsigned char)s_[j])) ++j;
+        }
+        std::string t = s_.substr(pos_, j - pos_);
+        char* end = nullptr;
+        double v = std::strtod(t.c_str(), &end);
+        if (t.empty() || end == t.c_str()) fail("bad number: '" + t + "'");
+        pos_ = j;
+        return v;
+    }
+    std::string identifier() {
+        size_t j = pos_;
+        while (j < s_.size() && (std::isalnum((unsigned char)s_[j]) || s_[j] == '_')) ++j;
+        std::string id = s_.substr(pos_, j - pos_);
+        pos_ = j;
+        return id;
+    }
+
+    P expr() {
+        P n = term();
+        for (;;) {
+            char c = peek();
+            if (c == '+') { ++pos_; n = bin(K::Add, n, term()); }
+            else if (c == '-') { ++pos_; n = bin(K::Sub, n, term()); }
+            else return n;
+        }
+    }
+    bool primary_follows() {
+        skip();
+        if (pos_ >= s_.size()) return false;
+        char c = s_[pos_];
+        if (c == '(' || c == '{' || c == '.' || c == '_') return true;
+        if (std::isalnum((unsigned char)c)) return true;
+        if (c == '\\') return !(starts("\\cdot") || starts("\\right"));
+        return false;
+    }
+    P term() {
+        P n = power();
+        for (;;) {
+            if (latex_) {
+                skip();
+                if (starts("\\cdot")) { pos_ += 5; n = bin(K::Mul, n, power()); continue; }
+            }
+            char c = peek();
+            if (c == '*') { ++pos_; n = bin(K::Mul, n, power()); }
+            else if (c == '/') { ++pos_; n = bin(K::Div, n, power()); }
+            else if (latex_ && primary_follows()) { n = bin(K::Mul, n, power()); }
+            else return n;
+        }
+    }
+    P power() {
+        P base = unary();
+        if (peek() == '^') {
+            ++pos_;
+            return bin(K::Pow, base, latex_ ? braced() : power());
+        }
+        return base;
+    }
+    P unary() {
+        char c = peek();
+        if (c == '-') { ++pos_; return neg(unary()); }
+        if (c == '+') { ++pos_; return unary(); }
+        return primary();
+    }
+    // LaTeX argument: {expr} or a single token.
+    P braced() {
+        skip();
+        if (pos_ < s_.size() && s_[pos_] == '{') {
+            ++pos_;
+            P e = expr();
+            if (!accept('}')) fail("expected }");
+            return e;
+        }
+        return single();
+    }
+    P single() {
+        skip();
+        char c = pos_ < s_.size() ? s_[pos_] : '\0';
+        if (std::isdigit((unsigned char)c) || c == '.') return num(number());
+        if (latex_ && c == '\\') {
+            ++pos_;
+            std::string cmd = command_name();
+            auto it = latex_letters().find(cmd);
+            if (it != latex_letters().end()) return sym(it->second);
+            fail("unexpected latex command \\" + cmd + " as single arg");
+        }
+        if (std::isalpha((unsigned char)c) || c == '_') return sym(identifier());
+        return primary();
+    }
+    P primary() {
+        skip();
+        char c = pos_ < s_.size() ? s_[pos_] : '\0';
+        if (c == '(') {
+            ++pos_;
+            P e = expr();
+            if (!accept(')')) fail("expected )");
+            return e;
+        }
+        if (latex_) {
+            if (starts("\\left")) {
+                pos_ += 5;
+                skip();
+                if (pos_ < s_.size() && (s_[pos_] == '(' || s_[pos_] == '[')) ++pos_;
+                P e = expr();
+                skip();
+                if (starts("\\right")) {
+                    pos_ += 6;
+                    skip();
+                    if (pos_ < s_.size() && (s_[pos_] == ')' || s_[pos_] == ']')) ++pos_;
+                }
+                return e;
+            }
+            if (c == '{') {
+                ++pos_;
+                P e = expr();
+                if (!accept('}')) fail("expected }");
+                return e;
+            }
+            if (c == '\\') {
+                ++pos_;
+                std::string cmd = command_name();
+                if (cmd == "frac") {
+                    P n = braced();
+                    P d = braced();
+                    return bin(K::Div, n, d);
+                }
+                auto f = latex_functions().find(cmd);
+                if (f != latex_functions().end()) {
+                    P exponent;
+                    if (peek() == '^') { ++pos_; exponent = braced(); }
+                    P fn = call(f->second, {braced()});
+                    return exponent ? bin(K::Pow, fn, exponent) : fn;
+                }
+                auto l = latex_letters().find(cmd);
+                if (l != latex_letters().end()) return sym(l->second);
+                fail("unknown latex command \\" + cmd);
+            }
+        }
+        if (std::isdigit((unsigned char)c) || c == '.') return num(number());
+        if (std::isalpha((unsigned char)c) || c == '_') {
+            std::string id = identifier();
+            if (peek() == '(') {
+                if (!plain_functions().count(id)) fail("unknown function: " + id);
+                ++pos_;
+                std::vector<P> args;
+                if (peek() != ')') {
+                    args.push_back(expr());
+                    while (accept(',')) args.push_back(expr());
+                }
+                if (!accept(')')) fail("expected ) after " + id);
+                return call(id, std::move(args));
+            }
+            return sym(id);
+        }
+        fail(std::string("unexpected char '") + c + "' at pos " + std::to_string(pos_));
+    }
+};
+
+// Replaces symbol names by state / parameter / time references.
+P resolve(const P& n, const System& s) {
+    switch (n->k) {
+    case K::Const: {
+        for (size_t i = 0; i < s.vars.size(); ++i)
+            if (s.vars[i] == n->name) { auto r = std::make_shared<Node>(); r->k = K::Var; r->idx = (int)i; r->name = n->name; return r; }
+        for (size_t j = 0; j < s.params.size(); ++j)
+            if (s.params[j] == n->name) { auto r = std::make_shared<Node>(); r->k = K::Par; r->idx = 1 + (int)j; r->name = n->name; return r; }
+        if (!s.time_symbol.empty() && n->name == s.time_symbol) { auto r = std::make_shared<Node>(); r->k = K::Time; return r; }
+        if (n->name == "pi") return n;
+        throw std::runtime_error("unknown symbol: " + n->name);
+    }
+    case K::Num: case K::Var: case K::Par: case K::Time: return n;
+    case K::Neg: return neg(resolve(n->a, s));
+    case K::Call: {
+        std::vector<P> a;
+        for (auto& x : n->args) a.push_back(resolve(x, s));
+        return call(n->name, std::move(a));
+    }
+    default: return bin(n->k, resolve(n->a, s), resolve(n->b, s));
+    }
+}
+
+std::vector<P> parse_system(const System& s) {
+    if (s.vars.size() != s.rhs.size()) throw std::runtime_error("vars/rhs size mismatch");
+    std::vector<P> out;
+    for (const auto& r : s.rhs) out.push_back(resolve(Parser(r, s.latex).parse(), s));
+    return out;
+}
+
+// --------------------------------------------------------------- printing --
+int precedence(K k) {
+    switch (k) {
+    case K::Add: case K::Sub: return 1;
+    case K::Mul: case K::Div: return 2;
+    case K::Neg: return 3;
+    case K::Pow: return 4;
+    default: return 5;
+    }
+}
+
+// "N.0" for integers below 1e12, otherwise the shortest "%.{p}g" that
+// round-trips, with ".0" appended when it has neither '.' nor an exponent.
+std::string format_number(double v) {
+    if (v == (double)(long long)v) {
+        long long iv = (long long)v;
+        if (iv > -1000000000000LL && iv < 1000000000000LL) return std::to_string(iv) + ".0";
+    }
+    char buf[64];
+    for (int p = 1; p <= 17; ++p) {
+        std::snprintf(buf, sizeof buf, "%.*g", p, v);
+        if (std::strtod(buf, nullptr) == v) break;
+    }
+    std::string r(buf);
+    if (r.find('.') == std::string::npos && r.find('e') == std::string::npos &&
+        r.find("inf") == std::string::npos && r.find("nan") == std::string::npos)
+        r += ".0";
+    return r;
+}
+
+struct Style {
+    std::string state = "X";       // state array name
+    std::string params = "a";      // parameter array name
+    std::string time = "t";        // time variable name
+    bool typed_literals = false;   // wrap literals as ((T)v)
+    bool c_functions = false;      // abs -> fabs for C
+};
+
+void print(const P& n, const Style& st, std::ostream& o);
+void print_child(const P& n, const Style& st, std::ostream& o, int min_prec) {
+    if (precedence(n->k) < min_prec) { o << "("; print(n, st, o); o << ")"; }
+    else print(n, st, o);
+}
+void print(const P& n, const Style& st, std::ostream& o) {
+    switch (n->k) {
+    case K::Num:
+        if (st.typed_literals) o << "((T)" << format_number(n->num) << ")";
+        else o << format_number(n->num);
+        break;
+    case K::Var: o << st.state << "[" << n->idx << "]"; break;
+    case K::Par: o << st.params << "[" << n->idx << "]"; break;
+    case K::Time: o << st.time; break;
+    case K::Const: o << (st.typed_literals ? "((T)3.14159265358979323846)" : n->name); break;
+    case K::Neg: o << "-"; print_child(n->a, st, o, precedence(K::Neg)); break;
+    case K::Add: print_child(n->a, st, o, 1); o << " + "; print_child(n->b, st, o, 1); break;
+    case K::Sub: print_child(n->a, st, o, 1); o << " - "; print_child(n->b, st, o, 2); break;
+    case K::Mul: print_child(n->a, st, o, 2); o << " * "; print_child(n->b, st, o, 2); break;
+    case K::Div: print_child(n->a, st, o, 2); o << " / "; print_child(n->b, st, o, 3); break;
+    case K::Pow: {
+        if (n->b->k == K::Num) {
+            double e = n->b->num;
+            int ie = (int)e;
+            if (e == (double)ie && ie >= 2 && ie <= 4) {
+                o << "(";
+                for (int i = 0; i < ie; ++i) { if (i) o << " * "; print_child(n->a, st, o, 2); }
+                o << ")";
+                break;
+            }
+        }
+        o << "pow("; print(n->a, st, o); o << ", "; print(n->b, st, o); o << ")";
+        break;
+    }
+    case K::Call: {
+        std::string f = n->name;
+        if (f == "abs") f = "fabs";
+        o << f << "(";
+        for (size_t i = 0; i < n->args.size(); ++i) { if (i) o << ", "; print(n->args[i], st, o); }
+        o << ")";
+        break;
+    }
+    }
+}
+std::string to_text(const P& n, const Style& st) { std::ostringstream o; print(n, st, o); return o.str(); }
+
+// ------------------------------------------------ differentiation (exact) --
+bool is_num(const P& n, double v) { return n->k == K::Num && n->num == v; }
+bool is_zero(const P& n) { return is_num(n, 0.0); }
+
+P s_add(P a, P b) {
+    if (is_zero(a)) return b;
+    if (is_zero(b)) return a;
+    if (a->k == K::Num && b->k == K::Num) return num(a->num + b->num);
+    if (b->k == K::Neg) return bin(K::Sub, a, b->a);
+    return bin(K::Add, a, b);
+}
+P s_neg(P a) {
+    if (a->k == K::Num) return num(-a->num);
+    if (a->k == K::Neg) return a->a;
+    return neg(a);
+}
+P s_sub(P a, P b) {
+    if (is_zero(b)) return a;
+    if (is_zero(a)) return s_neg(b);
+    if (a->k == K::Num && b->k == K::Num) return num(a->num - b->num);
+    if (b->k == K::Neg) return bin(K::Add, a, b->a);
+    return bin(K::Sub, a, b);
+}
+P s_mul(P a, P b) {
+    if (is_zero(a) || is_zero(b)) return num(0.0);
+    if (is_num(a, 1.0)) return b;
+    if (is_num(b, 1.0)) return a;
+    if (is_num(a, -1.0)) return s_neg(b);
+    if (is_num(b, -1.0)) return s_neg(a);
+    if (a->k == K::Num && b->k == K::Num) return num(a->num * b->num);
+    if (a->k == K::Neg && b->k == K::Neg) return s_mul(a->a, b->a);
+    if (a->k == K::Neg) return s_neg(s_mul(a->a, b));
+    if (b->k == K::Neg) return s_neg(s_mul(a, b->a));
+    return bin(K::Mul, a, b);
+}
+P s_div(P a, P b) {
+    if (is_zero(a)) return num(0.0);
+    if (is_num(b, 1.0)) return a;
+    if (a->k == K::Neg) return s_neg(s_div(a->a, b));
+    return bin(K::Div, a, b);
+}
+P f1(const char* f, P x) { return call(f, {std::move(x)}); }
+
+// d n / d X[j]
+P diff(const P& n, int j) {
+    switch (n->k) {
+    case K::Num: case K::Par: case K::Time: case K::Const: return num(0.0);
+    case K::Var: return num(n->idx == j ? 1.0 : 0.0);
+    case K::Neg: return s_neg(diff(n->a, j));
+    case K::Add: return s_add(diff(n->a, j), diff(n->b, j));
+    case K::Sub: return s_sub(diff(n->a, j), diff(n->b, j));
+    case K::Mul: return s_add(s_mul(diff(n->a, j), n->b), s_mul(n->a, diff(n->b, j)));
+    case K::Div: {
+        P da = diff(n->a, j), db = diff(n->b, j);
+        if (is_zero(db)) return s_div(da, n->b);
+        // (da*b - a*db) / (b*b)
+        return s_div(s_sub(s_mul(da, n->b), s_mul(n->a, db)), bin(K::Mul, n->b, n->b));
+    }
+    case K::Pow: {
+        P du = diff(n->a, j);
+        if (n->b->k == K::Num) {
+            double e = n->b->num;
+            if (is_zero(du)) return num(0.0);
+            P d;
+            if (e == 2.0) d = s_mul(num(2.0), n->a);
+            else if (e == 1.0) d = num(1.0);
+            else d = s_mul(num(e), bin(K::Pow, n->a, num(e - 1.0)));
+            return s_mul(d, du);
+        }
+        P dv = diff(n->b, j);
+        // d(u^v) = u^v * (dv*log(u) + v*du/u)
+        P inner = s_add(s_mul(dv, f1("log", n->a)), s_div(s_mul(n->b, du), n->a));
+        if (is_zero(inner)) return num(0.0);
+        return s_mul(n, inner);
+    }
+    case K::Call: {
+        const std::string& f = n->name;
+        if (n->args.size() == 2) {
+            P u = n->args[0], v = n->args[1];
+            P du = diff(u, j), dv = diff(v, j);
+            if (f == "pow") return diff(bin(K::Pow, u, v), j);
+            if (f == "fmod") return s_sub(du, s_mul(f1("trunc", s_div(u, v)), dv));
+            if (f == "atan2") {  // d atan2(u,v) = (v du - u dv)/(u^2+v^2)
+                if (is_zero(du) && is_zero(dv)) return num(0.0);
+                return s_div(s_sub(s_mul(v, du), s_mul(u, dv)), bin(K::Add, bin(K::Mul, u, u), bin(K::Mul, v, v)));
+            }
+            throw std::runtime_error("cannot differentiate " + f);
+        }
+        if (n->args.size() != 1) throw std::runtime_error("bad arity for " + f);
+        P u = n->args[0];
+        P du = diff(u, j);
+        if (is_zero(du)) return num(0.0);
+        P g;
+        if (f == "sin") g = f1("cos", u);
+        else if (f == "cos") g = s_neg(f1("sin", u));
+        else if (f == "tan") { P c = f1("cos", u); g = s_div(num(1.0), bin(K::Mul, c, c)); }
+        else if (f == "asin") g = s_div(num(1.0), f1("sqrt", bin(K::Sub, num(1.0), bin(K::Mul, u, u))));
+        else if (f == "acos") g = s_neg(s_div(num(1.0), f1("sqrt", bin(K::Sub, num(1.0), bin(K::Mul, u, u)))));
+        else if (f == "atan") g = s_div(num(1.0), bin(K::Add, num(1.0), bin(K::Mul, u, u)));
+        else if (f == "sinh") g = f1("cosh", u);
+        else if (f == "cosh") g = f1("sinh", u);
+        else if (f == "tanh") { P t = f1("tanh", u); g = bin(K::Sub, num(1.0), bin(K::Mul, t, t)); }
+        else if (f == "exp") g = n;
+        else if (f == "log") g = s_div(num(1.0), u);
+        else if (f == "log2") g = s_div(num(1.4426950408889634), u);
+        else if (f == "log10") g = s_div(num(0.4342944819032518), u);
+        else if (f == "sqrt") g = s_div(num(0.5), n);
+        else if (f == "cbrt") g = s_div(num(1.0), bin(K::Mul, num(3.0), bin(K::Mul, n, n)));
+        else if (f == "fabs" || f == "abs") g = call("copysign", {num(1.0), u});
+        else if (f == "floor" || f == "ceil") return num(0.0);
+        else throw std::runtime_error("cannot differentiate " + f);
+        return s_mul(g, du);
+    }
+    }
+    return num(0.0);
+}
+
+// ------------------------------------------------------------ op counting --
+void count(const P& n, int& flops, int& calls) {
+    switch (n->k) {
+    case K::Num: case K::Var: case K::Par: case K::Time: case K::Const: return;
+    case K::Neg: count(n->a, flops, calls); return;  // sign flips are free modifiers
+    case K::Call: ++calls; for (auto& a : n->args) count(a, flops, calls); return;
+    case K::Pow:
+        count(n->a, flops, calls);
+        if (n->b->k == K::Num && n->b->num == (int)n->b->num && n->b->num >= 2 && n->b->num <= 4) {
+            flops += (int)n->b->num - 1;
+            // the base is printed once per factor
+            int f2 = 0, c2 = 0; count(n->a, f2, c2); flops += f2 * ((int)n->b->num - 1); calls += c2 * ((int)n->b->num - 1);
+        } else { ++calls; count(n->b, flops, calls); }
+        return;
+    default: ++flops; count(n->a, flops, calls); count(n->b, flops, calls); return;
+    }
+}
+
+double eval(const P& n, const double* X, const double* a, double t) {
+    switch (n->k) {
+    case K::Num: return n->num;
+    case K::Var: return X[n->idx];
+    case K::Par: return a[n->idx];
+    case K::Time: return t;
+    case K::Const: return 3.14159265358979323846;
+    case K::Neg: return -eval(n->a, X, a, t);
+    case K::Add: return eval(n->a, X, a, t) + eval(n->b, X, a, t);
+    case K::Sub: return eval(n->a, X, a, t) - eval(n->b, X, a, t);
+    case K::Mul: return eval(n->a, X, a, t) * eval(n->b, X, a, t);
+    case K::Div: return eval(n->a, X, a, t) / eval(n->b, X, a, t);
+    case K::Pow: {
+        double u = eval(n->a, X, a, t);
+        if (n->b->k == K::Num) {
+            double e = n->b->num; int ie = (int)e;
+            if (e == (double)ie && ie >= 2 && ie <= 4) { double r = u; for (int i = 1; i < ie; ++i) r = r * u; return r; }
+        }
+        return std::pow(u, eval(n->b, X, a, t));
+    }
+    case K::Call: {
+        const std::string& f = n->name;
+        double u = eval(n->args[0], X, a, t);
+        if (n->args.size() == 2) {
+            double v = eval(n->args[1], X, a, t);
+            if (f == "pow") return std::pow(u, v);
+            if (f == "atan2") return std::atan2(u, v);
+            if (f == "fmod") return std::fmod(u, v);
+            if (f == "copysign") return std::copysign(u, v);
+        }
+        if (f == "sin") return std::sin(u); if (f == "cos") return std::cos(u); if (f == "tan") return std::tan(u);
+        if (f == "asin") return std::asin(u); if (f == "acos") return std::acos(u); if (f == "atan") return std::atan(u);
+        if (f == "sinh") return std::sinh(u); if (f == "cosh") return std::cosh(u); if (f == "tanh") return std::tanh(u);
+        if (f == "exp") return std::exp(u); if (f == "log") return std::log(u); if (f == "log2") return std::log2(u);
+        if (f == "log10") return std::log10(u); if (f == "sqrt") return std::sqrt(u); if (f == "cbrt") return std::cbrt(u);
+        if (f == "fabs" || f == "abs") return std::fabs(u); if (f == "floor") return std::floor(u);
+        if (f == "ceil") return std::ceil(u); if (f == "trunc") return std::trunc(u);
+        throw std::runtime_error("unknown function " + f);
+    }
+    }
+    return 0.0;
+}
+
+// ----------------------------------------------------------- KRS schemes --
+// Text layout of each scheme follows U-CUDA/codegen.cpp:579-606 so that
+// callers comparing step bodies (and the CPU oracle, which compiles them)
+// see the same statements.
+std::vector<std::string> rhs_text(const System& s, const std::string& state) {
+    Style st; st.state = state;
+    std::vector<std::string> out;
+    for (auto& t : parse_system(s)) out.push_back(to_text(t, st));
+    return out;
+}
+
+std::string krs_euler(const System& s) {
+    int N = (int)s.vars.size();
+    auto f = rhs_text(s, "X");
+    std::ostringstream o;
+    o << "    numb X1[" << N << "];\n    int i;\n";
+    for (int i = 0; i < N; ++i) o << "    X1[" << i << "] = X[" << i << "] + h * (" << f[i] << ");\n";
+    o << "    for (i = 0; i < " << N << "; i++)\n        X[i] = X1[i];\n";
+    return o.str();
+}
+std::string krs_euler_cromer(const System& s) {
+    auto f = rhs_text(s, "X");
+    std::ostringstream o;
+    for (size_t i = 0; i < f.size(); ++i) o << "    X[" << i << "] = X[" << i << "] + h * (" << f[i] << ");\n";
+    return o.str();
+}
+std::string krs_midpoint(const System& s) {
+    int N = (int)s.vars.size();
+    auto f0 = rhs_text(s, "X"), f1v = rhs_text(s, "X1");
+    std::ostringstream o;
+    o << "    numb X1[" << N << "];\n";
+    for (int i = 0; i < N; ++i) o << "    X1[" << i << "] = X[" << i << "] + 0.5 * h * (" << f0[i] << ");\n";
+    for (int i = 0; i < N; ++i) o << "    X[" << i << "] = X[" << i << "] + h * (" << f1v[i] << ");\n";
+    return o.str();
+}
+std::string krs_rk4(const System& s) {
+    int N = (int)s.vars.size();
+    auto k = rhs_text(s, "X1");
+    std::ostringstream o;
+    o << "    numb X1[" << N << "];\n    numb k[" << N << "][4];\n    int N = " << N << ";\n    int i, j;\n";
+    o << "    for (i = 0; i < N; i++) {\n        X1[i] = X[i];\n    }\n";
+    o << "    for (j = 0; j < 4; j++) {\n";
+    for (int i = 0; i < N; ++i) o << "        k[" << i << "][j] = (" << k[i] << ");\n";
+    o << "        if (j == 3) {\n            for (i = 0; i < N; i++) {\n"
+         "            X[i] = X[i] + h * (k[i][0] + 2 * k[i][1] + 2 * k[i][2] + k[i][3]) / 6;\n"
+         "            }\n        }\n";
+    o << "        else if (j == 2) {\n            for (i = 0; i < N; i++) {\n"
+         "            X1[i] = X[i] + h * k[i][j];\n            }\n        }\n";
+    o << "        else {\n            for (i = 0; i < N; i++) {\n"
+         "                X1[i] = X[i] + 0.5 * h * k[i][j];\n            }\n        }\n    }\n";
+    return o.str();
+}
+
+}  // namespace
+
+// ================================================================= public ==
+std::string codegen_scheme(const System& s, Scheme sch) {
+    switch (sch) {
+    case Scheme::Euler: return krs_euler(s);
+    case Scheme::EulerCromer: return krs_euler_cromer(s);
+    case Scheme::ExplicitMidpoint: return krs_midpoint(s);
+    case Scheme::RK4: return krs_rk4(s);
+    case Scheme::DOPRI78: throw std::runtime_error("DOPRI78 step bodies are not generated by this engine; pass a custom KRS body");
+    case Scheme::CD: throw std::runtime_error("CD step bodies are not generated by this engine; pass a custom KRS body");
+    }
+    throw std::runtime_error("unknown scheme");
+}
+
+Scheme scheme_from_name(const std::string& name) {
+    if (name == "Euler-Cromer") return Scheme::EulerCromer;
+    if (name == "Explicit Midpoint") return Scheme::ExplicitMidpoint;
+    if (name == "RK4") return Scheme::RK4;
+    if (name == "DOPRI78") return Scheme::DOPRI78;
+    if (name == "CD") return Scheme::CD;
+    return Scheme::Euler;
+}
+
+std::string normalize_value(const std::string& value) {
+    size_t a = value.find_first_not_of(" \t\n\r");
+    if (a == std::string::npos) return "";
+    size_t b = value.find_last_not_of(" \t\n\r");
+    P t = Parser(value.substr(a, b - a + 1), false).parse();
+    Style st;
+    std::function<void(const P&)> check = [&](const P& n) {
+        if (n->k == K::Const && n->name != "pi") throw std::runtime_error("unknown symbol: " + n->name);
+        if (n->a) check(n->a);
+        if (n->b) check(n->b);
+        for (auto& x : n->args) check(x);
+    };
+    check(t);
+    return to_text(t, st);
+}
+
+std::string emit_system_source(const System& s, Dialect d) {
+    const int N = (int)s.vars.size();
+    const int M = 1 + (int)s.params.size();
+    if (N <= 0) throw std::runtime_error("system has no state variables");
+    auto trees = parse_system(s);
+
+    Style st;
+    st.typed_literals = true;
+    st.c_functions = (d == Dialect::C);
+
+    bool uses_time = false;
+    std::function<void(const P&)> scan = [&](const P& n) {
+        if (n->k == K::Time) uses_time = true;
+        if (n->a) scan(n->a);
+        if (n->b) scan(n->b);
+        for (auto& x : n->args) scan(x);
+    };
+    for (auto& t : trees) scan(t);
+
+    // Jacobian entries (row i, column j).
+    std::vector<std::vector<P>> J(N, std::vector<P>(N));
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) J[i][j] = diff(trees[i], j);
+
+    std::ostringstream o;
+    o << "#define UC_N " << N << "\n#define UC_M " << M << "\n#define UC_USES_TIME " << (uses_time ? 1 : 0) << "\n";
+    const bool hip = (d == Dialect::Hip);
+    const char* fn = hip ? "template <typename T>\n__device__ __forceinline__ void" : "static inline void";
+    if (!hip) o << "typedef double T;\n";
+
+    // right-hand side
+    o << fn << " uc_rhs(const T* __restrict__ X, const T* __restrict__ a, const T t, T* __restrict__ dX) {\n";
+    o << "    (void)a; (void)t;\n";
+    for (int i = 0; i < N; ++i) o << "    dX[" << i << "] = " << to_text(trees[i], st) << ";\n";
+    o << "}\n";
+
+    // tangent: Jacobian entries once, then JV for each of the K vectors
+    if (hip) o << "template <typename T, int K>\n__device__ __forceinline__ void";
+    else o << "static inline void";
+    o << " uc_tangent(const T* __restrict__ X, const T* __restrict__ a, const T t, const T* __restrict__ V, T* __restrict__ JV"
+      << (hip ? "" : ", const int K") << ") {\n";
+    o << "    (void)X; (void)a; (void)t;\n";
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) {
+            const P& e = J[i][j];
+            if (is_zero(e) || e->k == K::Num || (e->k == K::Neg && e->a->k == K::Num)) continue;
+            o << "    const T J" << i << "_" << j << " = " << to_text(e, st) << ";\n";
+        }
+    if (hip) o << "#pragma unroll\n";
+    o << "    for (int k = 0; k < K; ++k) {\n";
+    for (int i = 0; i < N; ++i) {
+        std::ostringstream terms;
+        bool first = true;
+        for (int j = 0; j < N; ++j) {
+            const P& e = J[i][j];
+            if (is_zero(e)) continue;
+            std::string v = "V[k * " + std::to_string(N) + " + " + std::to_string(j) + "]";
+            std::string term;
+            bool negative = false;
+            if (is_num(e, 1.0)) term = v;
+            else if (is_num(e, -1.0)) { term = v; negative = true; }
+            else if (e->k == K::Num) term = "((T)" + format_number(e->num) + ") * " + v;
+            else if (e->k == K::Neg && e->a->k == K::Num) term = "((T)" + format_number(-e->a->num) + ") * " + v;
+            else term = "J" + std::to_string(i) + "_" + std::to_string(j) + " * " + v;
+            if (first) terms << (negative ? "-" : "") << term;
+            else terms << (negative ? " - " : " + ") << term;
+            first = false;
+        }
+        o << "        JV[k * " << N << " + " << i << "] = " << (first ? std::string("(T)0") : terms.str()) << ";\n";
+    }
+    o << "    }\n}\n";
+    return o.str();
+}
+
+OpCounts count_ops(const System& s) {
+    OpCounts c;
+    auto trees = parse_system(s);
+    const int N = (int)s.vars.size();
+    for (auto& t : trees) count(t, c.rhs_flops, c.rhs_calls);
+    for (int i = 0; i < N; ++i) {
+        int row_terms = 0;
+        for (int j = 0; j < N; ++j) {
+            P e = diff(trees[i], j);
+            if (is_zero(e)) continue;
+            ++c.jac_nonzeros;
+            ++row_terms;
+            if (!(e->k == K::Num || (e->k == K::Neg && e->a->k == K::Num))) count(e, c.jac_flops, c.jac_calls);
+        }
+        // per tangent vector: one multiply per nonzero (unit entries are free) plus adds
+        (void)row_terms;
+    }
+    return c;
+}
+
+void eval_rhs(const System& s, const double* X, const double* a, double t, double* dX) {
+    auto trees = parse_system(s);
+    for (size_t i = 0; i < trees.size(); ++i) dX[i] = eval(trees[i], X, a, t);
+}
+
+}  // namespace ucuda

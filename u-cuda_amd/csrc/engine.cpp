@@ -1,0 +1,521 @@
+// engine.cpp — C ABI over HIP + hipRTC (replaces U-CUDA/nvrtc_engine.cpp and the
+// NVRTC/driver half of U-CUDA/parametric_engine.cpp).
+//
+// * A system (vars/params/rhs) is parsed once; its device functions are
+//   generated by codegen and pasted, with the UcJob descriptor, in front of
+//   the hand-written ensemble kernels (kernels/ensemble.hip, embedded at
+//   build time). hipRTC compiles the program for gfx950 on first use.
+// * Compiled modules are cached per (source, strict) with LRU eviction
+//   (capacity 8), the pattern of NvrtcEngine's cache (nvrtc_engine.h:47-56).
+// * The engine is internally locked; launches go on the caller's stream.
+// * Axis tables live in small device buffers recycled once the event
+//   recorded after the launch that used them has completed, so an enqueue
+//   allocates only when every buffer is still in flight.
+#include "../../include/ucuda.h"
+
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <list>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "codegen.hpp"
+#include "uc_job.h"
+
+namespace {
+
+#include "kernels_embed.inc"  // const char* kEnsembleSource, kJobHeader
+
+constexpr int kMaxVars = 32;      // kMaxAmountOfX, U-CUDA/parametric_engine.cpp:30
+constexpr int kMaxValues = 64;    // kMaxAmountOfValues, parametric_engine.cpp:31
+constexpr int kBlock = 256;
+constexpr size_t kModuleCacheSize = 8;
+
+struct Module {
+    std::string key;
+    hipModule_t mod = nullptr;
+    size_t code_size = 0;
+    std::vector<char> code;
+};
+
+struct TableBuffer {
+    double* dev = nullptr;
+    double* host = nullptr;  // pinned staging, reused only after `done`
+    size_t cap = 0;        // doubles
+    hipEvent_t done = nullptr;
+    bool in_flight = false;
+};
+
+uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ULL) {
+    for (unsigned char c : s) { h ^= c; h *= 1099511628211ULL; }
+    return h;
+}
+
+}  // namespace
+
+struct uc_system {
+    ucuda::System sys;
+    std::string device_source;
+    int n_vars = 0;
+    int n_values = 0;
+    ucuda::OpCounts ops;
+};
+
+struct uc_engine {
+    int device = -1;
+    std::recursive_mutex mu;
+    std::string error;
+    std::list<Module> modules;      // front = most recently used
+    std::vector<TableBuffer> tables;
+
+    int fail(int code, const std::string& msg) { error = msg; return code; }
+    ~uc_engine() {
+        for (auto& m : modules)
+            if (m.mod) (void)hipModuleUnload(m.mod);
+        for (auto& t : tables) {
+            if (t.done) { (void)hipEventSynchronize(t.done); (void)hipEventDestroy(t.done); }
+            if (t.dev) (void)hipFree(t.dev);
+            if (t.host) (void)hipHostFree(t.host);
+        }
+    }
+};
+
+namespace {
+
+std::string hip_msg(hipError_t e, const char* what) {
+    return std::string(what) + ": " + hipGetErrorString(e);
+}
+
+std::string program_source(const uc_system* s, bool strict) {
+    std::string src;
+    src += "#define UC_STRICT ";
+    src += strict ? "1\n" : "0\n";
+    src += kJobHeader;
+    src += "\n";
+    src += s->device_source;
+    src += "\n";
+    src += kEnsembleSource;
+    return src;
+}
+
+// Compiles (or fetches from the cache) the module for a system. With
+// load=false only the code object is produced (works without a device).
+int get_module(uc_engine* e, uc_system* s, bool strict, bool load, Module** out) {
+    const std::string src = program_source(s, strict);
+    const std::string key = std::to_string(fnv1a(src)) + (strict ? ":s" : ":f");
+    for (auto it = e->modules.begin(); it != e->modules.end(); ++it) {
+        if (it->key == key) {
+            e->modules.splice(e->modules.begin(), e->modules, it);
+            Module& m = e->modules.front();
+            if (load && !m.mod) {
+                hipError_t he = hipModuleLoadData(&m.mod, m.code.data());
+                if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLoadData"));
+            }
+            *out = &m;
+            return UC_OK;
+        }
+    }
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "ucuda_ensemble.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return e->fail(UC_ERR_COMPILE, "hiprtcCreateProgram failed");
+    std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
+                                     strict ? "-ffp-contract=off" : "-ffp-contract=fast",
+                                     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        return e->fail(UC_ERR_COMPILE, "hipRTC compile failed:\n" + log);
+    }
+    Module m;
+    m.key = key;
+    hiprtcGetCodeSize(prog, &m.code_size);
+    m.code.resize(m.code_size);
+    hiprtcGetCode(prog, m.code.data());
+    hiprtcDestroyProgram(&prog);
+    if (load) {
+        hipError_t he = hipModuleLoadData(&m.mod, m.code.data());
+        if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLoadData"));
+    }
+    e->modules.push_front(std::move(m));
+    while (e->modules.size() > kModuleCacheSize) {
+        if (e->modules.back().mod) (void)hipModuleUnload(e->modules.back().mod);
+        e->modules.pop_back();
+    }
+    *out = &e->modules.front();
+    return UC_OK;
+}
+
+// Reference grid values (getValueByIdx / getValueByIdx_log,
+// U-CUDA/cudaLibrary.cu:1344-1396), evaluated in double on the host.
+void axis_values(const uc_axis& ax, double* out) {
+    const int n = ax.n_pts;
+    for (int i = 0; i < n; ++i) {
+        if (ax.log_scale) {
+            if (n == 1) { out[i] = std::pow(10.0, std::log10(ax.lo)); continue; }
+            const double l0 = std::log10(ax.lo), l1 = std::log10(ax.hi);
+            out[i] = std::pow(10.0, l0 + (double)i * ((l1 - l0) / (double)(n - 1)));
+        } else {
+            if (n == 1) { out[i] = ax.hi; continue; }
+            const double t = (double)i / (double)(n - 1);
+            out[i] = (1.0 - t) * ax.lo + t * ax.hi;
+        }
+    }
+}
+
+int validate_common(uc_engine* e, const uc_system* s, const uc_common& c, bool allow_step_axis) {
+    if (!c.initial_conditions && c.ic_seed == 0) return e->fail(UC_ERR_ARGUMENT, "initial_conditions is NULL");
+    if (!c.base_values && c.n_values > 0) return e->fail(UC_ERR_ARGUMENT, "base_values is NULL");
+    if (c.n_values > kMaxValues) return e->fail(UC_ERR_ARGUMENT, "too many base_values");
+    if (c.n_values < s->n_values) return e->fail(UC_ERR_ARGUMENT, "base_values shorter than the system's a[] (a[0] + params)");
+    if (c.n_axes < 1 || c.n_axes > 2) return e->fail(UC_ERR_ARGUMENT, "n_axes must be 1 or 2");
+    int step_axes = 0;
+    for (int i = 0; i < c.n_axes; ++i) {
+        const uc_axis& ax = c.axis[i];
+        if (ax.n_pts <= 0) return e->fail(UC_ERR_ARGUMENT, "axis n_pts must be > 0");
+        if (ax.log_scale && !(ax.lo > 0.0 && ax.hi > 0.0)) return e->fail(UC_ERR_ARGUMENT, "log scale needs lo > 0 and hi > 0");
+        switch (ax.kind) {
+        case UC_AXIS_PARAM:
+            if (ax.index <= 0 || ax.index >= c.n_values || ax.index >= s->n_values)
+                return e->fail(UC_ERR_ARGUMENT, "param_index out of range");
+            break;
+        case UC_AXIS_VAR:
+            if (ax.index < 0 || ax.index >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "var_sweep_index out of range");
+            break;
+        case UC_AXIS_STEP:
+            if (!allow_step_axis) return e->fail(UC_ERR_ARGUMENT, "this analysis cannot sweep h");
+            ++step_axes;
+            break;
+        default: return e->fail(UC_ERR_ARGUMENT, "unknown axis kind");
+        }
+    }
+    if (step_axes > 1) return e->fail(UC_ERR_ARGUMENT, "only one axis may sweep h");
+    if (!(c.h > 0.0) && step_axes == 0) return e->fail(UC_ERR_ARGUMENT, "h must be > 0");
+    if (!(c.t_max > 0.0)) return e->fail(UC_ERR_ARGUMENT, "t_max must be > 0");
+    if (c.transient_time < 0.0) return e->fail(UC_ERR_ARGUMENT, "transient_time must be >= 0");
+    if (c.precision != UC_PREC_F64 && c.precision != UC_PREC_F32C) return e->fail(UC_ERR_ARGUMENT, "unknown precision");
+    return UC_OK;
+}
+
+long long cells_of(const uc_common& c) {
+    long long n = 1;
+    for (int i = 0; i < c.n_axes; ++i) n *= c.axis[i].n_pts;
+    return n;
+}
+
+// Uploads axis tables + base params + base IC into a recycled device buffer
+// and fills the descriptor's grid fields.
+int stage_tables(uc_engine* e, const uc_system* s, const uc_common& c, UcJob& j, hipStream_t stream, TableBuffer** used) {
+    const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
+    const size_t need = (size_t)nx + (size_t)ny + (size_t)kMaxValues + (size_t)kMaxVars;
+    TableBuffer* tb = nullptr;
+    for (auto& t : e->tables) {
+        if (t.in_flight && hipEventQuery(t.done) == hipSuccess) t.in_flight = false;
+        if (!t.in_flight && t.cap >= need) { tb = &t; break; }
+    }
+    if (!tb) {
+        e->tables.emplace_back();
+        tb = &e->tables.back();
+        tb->cap = std::max<size_t>(need, 4096);
+        hipError_t he = hipMalloc((void**)&tb->dev, tb->cap * sizeof(double));
+        if (he == hipSuccess) he = hipHostMalloc((void**)&tb->host, tb->cap * sizeof(double), hipHostMallocDefault);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&tb->done, hipEventDisableTiming);
+        if (he != hipSuccess) {
+            if (tb->dev) (void)hipFree(tb->dev);
+            if (tb->host) (void)hipHostFree(tb->host);
+            e->tables.pop_back();
+            return e->fail(UC_ERR_HIP, hip_msg(he, "table buffer allocation"));
+        }
+    }
+    double* ax0 = tb->host;
+    double* ax1 = ax0 + nx;
+    double* par = ax1 + ny;
+    double* ic = par + kMaxValues;
+    std::fill(ax0, ax0 + need, 0.0);
+    axis_values(c.axis[0], ax0);
+    if (c.n_axes > 1) axis_values(c.axis[1], ax1);
+    for (int i = 0; i < c.n_values; ++i) par[i] = c.base_values[i];
+    for (int i = 0; i < s->n_vars; ++i) ic[i] = c.initial_conditions ? c.initial_conditions[i] : 0.0;
+    hipError_t he = hipMemcpyAsync(tb->dev, tb->host, need * sizeof(double), hipMemcpyHostToDevice, stream);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipMemcpyAsync(tables)"));
+
+    std::memset(&j, 0, sizeof j);
+    j.n_cells = (long long)nx * ny;
+    j.cell_offset = 0;
+    j.nx = nx;
+    j.ny = ny;
+    for (int i = 0; i < 2; ++i) {
+        j.axis_kind[i] = i < c.n_axes ? c.axis[i].kind : UC_AXIS_NONE;
+        j.axis_index[i] = i < c.n_axes ? c.axis[i].index : 0;
+    }
+    j.axis_vals[0] = tb->dev;
+    j.axis_vals[1] = tb->dev + nx;
+    j.base_params = tb->dev + nx + ny;
+    j.base_ic = tb->dev + nx + ny + kMaxValues;
+    j.n_values = c.n_values;
+    j.ic_seed = c.ic_seed;
+    j.ic_lo = c.ic_lo;
+    j.ic_hi = c.ic_hi;
+    j.h = c.h;
+    j.max_value = c.max_value;
+    j.transient_time = c.transient_time;
+    j.t_max = c.t_max;
+    j.pre_scaller = 1;
+    *used = tb;
+    return UC_OK;
+}
+
+double step_of(const uc_common& c) {
+    for (int i = 0; i < c.n_axes; ++i)
+        if (c.axis[i].kind == UC_AXIS_STEP) return std::min(c.axis[i].lo, c.axis[i].hi);
+    return c.h;
+}
+
+int launch(uc_engine* e, Module* m, const char* name, UcJob& j, hipStream_t stream, TableBuffer* tb) {
+    hipFunction_t f;
+    hipError_t he = hipModuleGetFunction(&f, m->mod, name);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, name));
+    const long long blocks = (j.n_cells + kBlock - 1) / kBlock;
+    if (blocks > 0x7fffffffLL) return e->fail(UC_ERR_ARGUMENT, "grid too large");
+    void* args[] = {&j};
+    he = hipModuleLaunchKernel(f, (unsigned)blocks, 1, 1, kBlock, 1, 1, 0, stream, args, nullptr);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLaunchKernel"));
+    he = hipEventRecord(tb->done, stream);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipEventRecord"));
+    tb->in_flight = true;
+    return UC_OK;
+}
+
+int enter(uc_engine* e) {
+    if (e->device < 0) return e->fail(UC_ERR_NO_DEVICE, "engine was created without a device");
+    hipError_t he = hipSetDevice(e->device);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipSetDevice"));
+    return UC_OK;
+}
+
+ucuda::System to_system(const uc_system_desc* d) {
+    ucuda::System s;
+    for (int i = 0; i < d->n_vars; ++i) s.vars.push_back(d->vars[i]);
+    for (int i = 0; i < d->n_params; ++i) s.params.push_back(d->params[i]);
+    for (int i = 0; i < d->n_vars; ++i) s.rhs.push_back(d->rhs[i]);
+    s.latex = d->latex != 0;
+    if (d->time_symbol) s.time_symbol = d->time_symbol;
+    return s;
+}
+
+int copy_out(const std::string& text, char* buf, size_t cap, size_t* len) {
+    if (len) *len = text.size() + 1;
+    if (buf && cap > 0) {
+        size_t n = std::min(cap - 1, text.size());
+        std::memcpy(buf, text.data(), n);
+        buf[n] = '\0';
+    }
+    return UC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* uc_version(void) { return "ucuda-mi355x 0.1"; }
+
+int uc_engine_create(int device, uc_engine** out) {
+    if (!out) return UC_ERR_ARGUMENT;
+    auto e = std::make_unique<uc_engine>();
+    e->device = device;
+    if (device >= 0) {
+        hipError_t he = hipSetDevice(device);
+        if (he != hipSuccess) return UC_ERR_HIP;
+    }
+    *out = e.release();
+    return UC_OK;
+}
+
+void uc_engine_destroy(uc_engine* e) { delete e; }
+
+const char* uc_last_error(const uc_engine* e) { return e ? e->error.c_str() : "null engine"; }
+
+int uc_system_create(uc_engine* e, const uc_system_desc* desc, uc_system** out) {
+    if (!e || !desc || !out) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (desc->n_vars <= 0 || desc->n_vars > kMaxVars) return e->fail(UC_ERR_ARGUMENT, "n_vars out of range");
+    if (desc->n_params < 0 || desc->n_params + 1 > kMaxValues) return e->fail(UC_ERR_ARGUMENT, "n_params out of range");
+    auto s = std::make_unique<uc_system>();
+    try {
+        s->sys = to_system(desc);
+        s->device_source = ucuda::emit_system_source(s->sys, ucuda::Dialect::Hip);
+        s->ops = ucuda::count_ops(s->sys);
+    } catch (const std::exception& ex) {
+        return e->fail(UC_ERR_PARSE, ex.what());
+    }
+    s->n_vars = desc->n_vars;
+    s->n_values = desc->n_params + 1;
+    *out = s.release();
+    return UC_OK;
+}
+
+void uc_system_destroy(uc_system* s) { delete s; }
+int uc_system_n_vars(const uc_system* s) { return s ? s->n_vars : 0; }
+int uc_system_n_values(const uc_system* s) { return s ? s->n_values : 0; }
+
+int uc_system_source(const uc_system* s, int dialect, char* buf, size_t cap, size_t* len) {
+    if (!s) return UC_ERR_ARGUMENT;
+    if (dialect == 0) return copy_out(program_source(s, false), buf, cap, len);
+    try {
+        return copy_out(ucuda::emit_system_source(s->sys, ucuda::Dialect::C), buf, cap, len);
+    } catch (...) {
+        return UC_ERR_PARSE;
+    }
+}
+
+int uc_system_op_counts(const uc_system* s, int* rhs_flops, int* rhs_calls, int* jac_nonzeros, int* jac_flops) {
+    if (!s) return UC_ERR_ARGUMENT;
+    if (rhs_flops) *rhs_flops = s->ops.rhs_flops;
+    if (rhs_calls) *rhs_calls = s->ops.rhs_calls;
+    if (jac_nonzeros) *jac_nonzeros = s->ops.jac_nonzeros;
+    if (jac_flops) *jac_flops = s->ops.jac_flops;
+    return UC_OK;
+}
+
+int uc_system_compile(uc_engine* e, uc_system* s, int strict, size_t* code_size) {
+    if (!e || !s) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    Module* m = nullptr;
+    int rc = get_module(e, s, strict != 0, false, &m);
+    if (rc) return rc;
+    if (code_size) *code_size = m->code_size;
+    return UC_OK;
+}
+
+int uc_codegen_scheme(uc_engine* e, const uc_system_desc* desc, int scheme, char* buf, size_t cap, size_t* len) {
+    if (!desc) return UC_ERR_ARGUMENT;
+    try {
+        ucuda::System s = to_system(desc);
+        return copy_out(ucuda::codegen_scheme(s, (ucuda::Scheme)scheme), buf, cap, len);
+    } catch (const std::exception& ex) {
+        if (e) e->error = ex.what();
+        return UC_ERR_PARSE;
+    }
+}
+
+int uc_system_eval_rhs(const uc_system* s, const double* X, const double* a, double t, double* dX) {
+    if (!s || !X || !a || !dX) return UC_ERR_ARGUMENT;
+    try {
+        ucuda::eval_rhs(s->sys, X, a, t, dX);
+    } catch (...) {
+        return UC_ERR_PARSE;
+    }
+    return UC_OK;
+}
+
+long long uc_grid_cells(const uc_common* c) { return c ? cells_of(*c) : 0; }
+
+int uc_axis_values(const uc_axis* ax, double* out) {
+    if (!ax || !out || ax->n_pts <= 0) return UC_ERR_ARGUMENT;
+    axis_values(*ax, out);
+    return UC_OK;
+}
+
+int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, void* stream) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    int rc = enter(e);
+    if (rc) return rc;
+    if ((rc = validate_common(e, s, r->c, true))) return rc;
+    if (r->writable_var < -1 || r->writable_var >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "writable_var out of range");
+    if (r->pre_scaller <= 0) return e->fail(UC_ERR_ARGUMENT, "pre_scaller must be > 0");
+    if (r->max_peaks < 0) return e->fail(UC_ERR_ARGUMENT, "max_peaks must be >= 0");
+    if (!o->flags || (r->max_peaks > 0 && (!o->peaks || !o->intervals)))
+        return e->fail(UC_ERR_ARGUMENT, "bifurcation outputs missing");
+    if (o->traj_len > 0 && !o->traj) return e->fail(UC_ERR_ARGUMENT, "traj_len > 0 but traj is NULL");
+    // Recorded samples: ceil(t_max / h / pre_scaller) (run_bif1d, parametric_engine.cpp:1796),
+    // transient steps: (int)(transient_time / h) (:1797).
+    const double hw = step_of(r->c);
+    if (!(hw > 0.0)) return e->fail(UC_ERR_ARGUMENT, "h must be > 0 (whole swept range)");
+    const long long iters = (long long)std::ceil(r->c.t_max / hw / r->pre_scaller);
+    if (iters <= 0) return e->fail(UC_ERR_ARGUMENT, "computed amountOfPointsInBlock <= 0");
+
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
+    j.skip_steps = r->c.h > 0.0 ? (long long)(r->c.transient_time / r->c.h) : 0;
+    j.n_iters = iters;
+    j.pre_scaller = r->pre_scaller;
+    j.writable_var = r->writable_var;
+    j.max_peaks = r->max_peaks;
+    j.traj_len = o->traj ? o->traj_len : 0;
+    j.flag = o->flags;
+    j.out0 = o->peaks;
+    j.out1 = o->intervals;
+    j.traj = o->traj;
+    j.final_state = o->final_state;
+    return launch(e, m, r->c.precision == UC_PREC_F64 ? "uc_bif_f64" : "uc_bif_f32c", j, st, tb);
+}
+
+static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream, bool spectrum) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    int rc = enter(e);
+    if (rc) return rc;
+    if ((rc = validate_common(e, s, r->c, true))) return rc;
+    if (!(r->NT > 0.0)) return e->fail(UC_ERR_ARGUMENT, "NT must be > 0");
+    if (!(r->eps > 0.0)) return e->fail(UC_ERR_ARGUMENT, "eps must be > 0");
+    if (r->method != UC_METHOD_VARIATIONAL && r->method != UC_METHOD_FINITE_DIFF) return e->fail(UC_ERR_ARGUMENT, "unknown method");
+    if (!o->values || !o->flags) return e->fail(UC_ERR_ARGUMENT, "outputs missing");
+    // NT blocks: (int)(t_max / NT); steps per block (int)(NT / h); transient
+    // (int)(transient_time / h) (run_lle_2d, parametric_engine.cpp:2731-2732, 2865).
+    const long long blocks = (long long)(r->c.t_max / r->NT);
+    if (blocks <= 0) return e->fail(UC_ERR_ARGUMENT, "computed t_max / NT <= 0");
+    bool step_axis = false;
+    for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
+    const long long nts = r->c.h > 0.0 ? (long long)(r->NT / r->c.h) : 0;
+    if (!step_axis && nts <= 0) return e->fail(UC_ERR_ARGUMENT, "computed NT / h <= 0");
+
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
+    j.skip_steps = r->c.h > 0.0 ? (long long)(r->c.transient_time / r->c.h) : 0;
+    j.n_iters = blocks;
+    j.nt_steps = nts;
+    j.nt = r->NT;
+    j.eps = r->eps;
+    j.method = r->method;
+    j.probe_seed = r->probe_seed;
+    j.flag = o->flags;
+    j.out0 = o->values;
+    j.final_state = o->final_state;
+    const char* name = spectrum ? (r->c.precision == UC_PREC_F64 ? "uc_ls_f64" : "uc_ls_f32c")
+                                : (r->c.precision == UC_PREC_F64 ? "uc_lle_f64" : "uc_lle_f32c");
+    return launch(e, m, name, j, st, tb);
+}
+
+int uc_run_lle(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream) {
+    return run_lyap(e, s, r, o, stream, false);
+}
+
+int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream) {
+    return run_lyap(e, s, r, o, stream, true);
+}
+
+int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream) {
+    (void)s; (void)r; (void)o; (void)stream;
+    if (!e) return UC_ERR_ARGUMENT;
+    return e->fail(UC_ERR_ARGUMENT, "basins: not available in this build");
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+/* uc_job.h — launch descriptor shared by the host engine and the run-time
+ * compiled HIP kernels (this text is pasted verbatim into every hipRTC
+ * program, so it must stay plain C with fixed-size members only).
+ *
+ * One descriptor describes one ensemble: a grid of cells, each cell one
+ * trajectory. Axis 0 varies fastest: cell = iy * nx + ix, the row-major
+ * layout of the reference's 2-D results (values[iy * n + ix],
+ * U-CUDA/parametric_engine.h:468-470) and of its kernel index mapping
+ * (getValueByIdx, U-CUDA/cudaLibrary.cu:1344-1361).
+ */
+#ifndef UC_JOB_H
+#define UC_JOB_H
+
+#define UC_AXIS_NONE 0
+#define UC_AXIS_PARAM 1   /* sweeps a[index] (1-based, a[0] reserved)   */
+#define UC_AXIS_VAR 2     /* sweeps the initial condition X0[index]      */
+#define UC_AXIS_STEP 3    /* sweeps the integration step h               */
+
+#define UC_METHOD_VARIATIONAL 0  /* tangent-space variational equations    */
+#define UC_METHOD_FINITE_DIFF 1  /* reference finite-difference Benettin   */
+
+#define UC_LABEL_UNBOUND 0
+
+typedef struct UcJob {
+    /* ---- grid ---- */
+    long long n_cells;            /* cells handled by this launch             */
+    long long cell_offset;        /* global index of the launch's first cell  */
+    int nx, ny;                   /* points on axis 0 / axis 1 (ny = 1: 1-D)  */
+    int axis_kind[2];             /* UC_AXIS_*                                */
+    int axis_index[2];
+    const double* axis_vals[2];   /* device tables, nx resp. ny values        */
+    const double* base_params;    /* a[0..n_values-1]                         */
+    const double* base_ic;        /* X0[0..N-1]                               */
+    int n_values;
+    int pre_scaller;
+    /* per-cell random initial state: X0[i] = ic_lo + (ic_hi - ic_lo) * u    */
+    unsigned long long ic_seed;   /* 0 = use base_ic                          */
+    double ic_lo, ic_hi;
+
+    /* ---- integration ---- */
+    double h;
+    double max_value;             /* 0 disables the magnitude check           */
+    long long skip_steps;         /* transient steps when no axis sweeps h    */
+    long long n_iters;            /* bif: recorded samples; lle/ls: blocks    */
+    long long nt_steps;           /* lle/ls: steps per renormalisation block  */
+    double transient_time, t_max, nt;  /* per-lane recount under an h axis    */
+    double eps;                   /* finite-difference probe distance         */
+    unsigned long long probe_seed;/* tangent / probe basis seed               */
+    int method;                   /* UC_METHOD_*                              */
+    int writable_var;             /* bif: sampled variable (-1: x0+pi*x1+e*x2)*/
+    int max_peaks;                /* bif: stored peaks per cell               */
+    int traj_len;                 /* bif: samples copied to traj (0 = none)   */
+
+    /* ---- adaptive integration / basins ---- */
+    double rtol, atol;            /* Dormand-Prince 5(4) error control        */
+    double h_min, h_max;
+    double section_period;        /* Poincare section every section_period    */
+    int n_sections;               /* sections integrated in total             */
+    int n_record;                 /* trailing sections kept for the signature */
+    int max_period;               /* longest cycle searched for               */
+    int label_vars;               /* state variables used by the signature    */
+    double period_tol;            /* cycle closure tolerance                  */
+    double quantum;               /* signature quantisation step              */
+    long long max_attempts;       /* per-cell cap on attempted steps          */
+
+    /* ---- outputs (device pointers, structure of arrays) ---- */
+    int* flag;                    /* per cell                                 */
+    double* out0;                 /* bif peaks / lle / ls planes / basin sig. */
+    double* out1;                 /* bif intervals / basin cycle point        */
+    double* traj;                 /* bif: traj[i * n_cells + cell]            */
+    double* final_state;          /* final_state[i * n_cells + cell]          */
+    long long* counters;          /* per cell: accepted, rejected steps       */
+    int* label;                   /* basins: int32 attractor signature hash   */
+} UcJob;
+
+#endif

@@ -1,0 +1,389 @@
+"""Python host mirror of the engine's C ABI (include/ucuda.h).
+
+Thin ctypes layer: systems are defined as in the reference (vars, params,
+rhs strings; U-CUDA/codegen.hpp:10-15), requests mirror the reference's
+Request structs (U-CUDA/parametric_engine.h:27-126, 411-482, 581-648) and
+results come back as torch tensors on the GPU. torch only supplies device
+memory and the stream; every computation runs in libucuda.so's HIP kernels.
+There is no fallback: if libucuda.so is missing or a launch fails, the call
+raises.
+"""
+import ctypes as C
+import math
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libucuda.so")
+
+UC_OK = 0
+PREC_F64, PREC_F32C = 0, 1
+AXIS_PARAM, AXIS_VAR, AXIS_STEP = 1, 2, 3
+METHOD_VARIATIONAL, METHOD_FINITE_DIFF = 0, 1
+SCHEME_EULER, SCHEME_EULER_CROMER, SCHEME_MIDPOINT, SCHEME_RK4 = 0, 1, 2, 3
+REFERENCE_PROBE_SEED = 1234567891  # U-CUDA/cudaLibrary.cu:2603
+
+
+class UcError(RuntimeError):
+    pass
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [("vars", C.POINTER(C.c_char_p)), ("n_vars", C.c_int),
+                ("params", C.POINTER(C.c_char_p)), ("n_params", C.c_int),
+                ("rhs", C.POINTER(C.c_char_p)), ("latex", C.c_int),
+                ("time_symbol", C.c_char_p)]
+
+
+class Axis(C.Structure):
+    _fields_ = [("kind", C.c_int), ("index", C.c_int), ("n_pts", C.c_int),
+                ("lo", C.c_double), ("hi", C.c_double), ("log_scale", C.c_int)]
+
+
+class Common(C.Structure):
+    _fields_ = [("initial_conditions", C.POINTER(C.c_double)),
+                ("base_values", C.POINTER(C.c_double)), ("n_values", C.c_int),
+                ("n_axes", C.c_int), ("axis", Axis * 2),
+                ("h", C.c_double), ("transient_time", C.c_double), ("t_max", C.c_double),
+                ("max_value", C.c_double), ("precision", C.c_int), ("strict", C.c_int),
+                ("ic_seed", C.c_ulonglong), ("ic_lo", C.c_double), ("ic_hi", C.c_double)]
+
+
+class BifRequest(C.Structure):
+    _fields_ = [("c", Common), ("writable_var", C.c_int), ("pre_scaller", C.c_int),
+                ("max_peaks", C.c_int)]
+
+
+class BifOutput(C.Structure):
+    _fields_ = [("flags", C.c_void_p), ("peaks", C.c_void_p), ("intervals", C.c_void_p),
+                ("traj", C.c_void_p), ("traj_len", C.c_int), ("final_state", C.c_void_p)]
+
+
+class LyapRequest(C.Structure):
+    _fields_ = [("c", Common), ("NT", C.c_double), ("eps", C.c_double), ("method", C.c_int),
+                ("probe_seed", C.c_ulonglong)]
+
+
+class LyapOutput(C.Structure):
+    _fields_ = [("values", C.c_void_p), ("flags", C.c_void_p), ("final_state", C.c_void_p)]
+
+
+class BasinsRequest(C.Structure):
+    _fields_ = [("c", Common), ("rtol", C.c_double), ("atol", C.c_double),
+                ("h_min", C.c_double), ("h_max", C.c_double), ("section_period", C.c_double),
+                ("n_sections", C.c_int), ("n_record", C.c_int), ("max_period", C.c_int),
+                ("label_vars", C.c_int), ("period_tol", C.c_double), ("quantum", C.c_double),
+                ("max_attempts", C.c_longlong)]
+
+
+class BasinsOutput(C.Structure):
+    _fields_ = [("labels", C.c_void_p), ("periods", C.c_void_p), ("cycle_point", C.c_void_p),
+                ("steps", C.c_void_p)]
+
+
+# Every symbol include/ucuda.h declares, with its ctypes signature.
+_SIGNATURES = {
+    "uc_version": (C.c_char_p, []),
+    "uc_engine_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "uc_engine_destroy": (None, [C.c_void_p]),
+    "uc_last_error": (C.c_char_p, [C.c_void_p]),
+    "uc_system_create": (C.c_int, [C.c_void_p, C.POINTER(SystemDesc), C.POINTER(C.c_void_p)]),
+    "uc_system_destroy": (None, [C.c_void_p]),
+    "uc_system_n_vars": (C.c_int, [C.c_void_p]),
+    "uc_system_n_values": (C.c_int, [C.c_void_p]),
+    "uc_system_source": (C.c_int, [C.c_void_p, C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "uc_system_compile": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_size_t)]),
+    "uc_system_op_counts": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 4),
+    "uc_codegen_scheme": (C.c_int, [C.c_void_p, C.POINTER(SystemDesc), C.c_int, C.c_char_p, C.c_size_t,
+                                    C.POINTER(C.c_size_t)]),
+    "uc_system_eval_rhs": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_double,
+                                     C.POINTER(C.c_double)]),
+    "uc_run_bif": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.POINTER(BifOutput), C.c_void_p]),
+    "uc_run_lle": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.POINTER(LyapOutput), C.c_void_p]),
+    "uc_run_ls": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.POINTER(LyapOutput), C.c_void_p]),
+    "uc_run_basins": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BasinsRequest), C.POINTER(BasinsOutput),
+                                C.c_void_p]),
+    "uc_grid_cells": (C.c_longlong, [C.POINTER(Common)]),
+    "uc_axis_values": (C.c_int, [C.POINTER(Axis), C.POINTER(C.c_double)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads libucuda.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise UcError("libucuda.so not built: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def _strs(items):
+    arr = (C.c_char_p * max(1, len(items)))()
+    for i, s in enumerate(items):
+        arr[i] = s.encode()
+    return arr
+
+
+class Engine:
+    """Wraps uc_engine. device < 0: compile-only engine (no GPU touched)."""
+
+    def __init__(self, device=0):
+        self.device = device
+        h = C.c_void_p()
+        rc = lib().uc_engine_create(device, C.byref(h))
+        if rc != UC_OK:
+            raise UcError("uc_engine_create failed (rc=%d)" % rc)
+        self.h = h
+
+    def error(self):
+        return lib().uc_last_error(self.h).decode(errors="replace")
+
+    def check(self, rc, what):
+        if rc != UC_OK:
+            raise UcError("%s failed (rc=%d): %s" % (what, rc, self.error()))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().uc_engine_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def codegen_scheme(self, vars, params, rhs, scheme=SCHEME_RK4, latex=False):
+        d, keep = _desc(vars, params, rhs, latex, None)
+        n = C.c_size_t()
+        self.check(lib().uc_codegen_scheme(self.h, C.byref(d), scheme, None, 0, C.byref(n)), "codegen_scheme")
+        buf = C.create_string_buffer(n.value)
+        self.check(lib().uc_codegen_scheme(self.h, C.byref(d), scheme, buf, n.value, C.byref(n)), "codegen_scheme")
+        del keep
+        return buf.value.decode()
+
+
+def _desc(vars, params, rhs, latex, time_symbol):
+    v, p, r = _strs(vars), _strs(params), _strs(rhs)
+    d = SystemDesc(v, len(vars), p, len(params), r, 1 if latex else 0,
+                   time_symbol.encode() if time_symbol else None)
+    return d, (v, p, r)
+
+
+class System:
+    """A parsed system (uc_system). values() builds the a[] vector."""
+
+    def __init__(self, engine, vars, params, rhs, latex=False, time_symbol=None, defaults=None, name=""):
+        self.engine = engine
+        self.vars, self.params, self.rhs = list(vars), list(params), list(rhs)
+        self.latex, self.time_symbol = latex, time_symbol
+        self.defaults = dict(defaults or {})
+        self.name = name
+        d, keep = _desc(self.vars, self.params, self.rhs, latex, time_symbol)
+        h = C.c_void_p()
+        engine.check(lib().uc_system_create(engine.h, C.byref(d), C.byref(h)), "uc_system_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().uc_system_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    @property
+    def n_vars(self):
+        return len(self.vars)
+
+    @property
+    def n_values(self):
+        return 1 + len(self.params)
+
+    def values(self, **kw):
+        """a[] with a[0] = 0.5 (the reference sessions' CD symmetry default,
+        U-CUDA/analysis_session.cpp:491-492) and params in alphabet order."""
+        vals = dict(self.defaults)
+        vals.update(kw)
+        return [0.5] + [float(vals[p]) for p in self.params]
+
+    def source(self, dialect=0):
+        n = C.c_size_t()
+        self.engine.check(lib().uc_system_source(self.h, dialect, None, 0, C.byref(n)), "uc_system_source")
+        buf = C.create_string_buffer(n.value)
+        self.engine.check(lib().uc_system_source(self.h, dialect, buf, n.value, C.byref(n)), "uc_system_source")
+        return buf.value.decode()
+
+    def compile(self, strict=False):
+        n = C.c_size_t()
+        self.engine.check(lib().uc_system_compile(self.engine.h, self.h, 1 if strict else 0, C.byref(n)),
+                          "uc_system_compile")
+        return n.value
+
+    def op_counts(self):
+        v = [C.c_int() for _ in range(4)]
+        self.engine.check(lib().uc_system_op_counts(self.h, *[C.byref(x) for x in v]), "op_counts")
+        return dict(zip(("rhs_flops", "rhs_calls", "jac_nonzeros", "jac_flops"), [x.value for x in v]))
+
+    def eval_rhs(self, X, a, t=0.0):
+        Xa = (C.c_double * len(X))(*X)
+        aa = (C.c_double * len(a))(*a)
+        out = (C.c_double * len(X))()
+        self.engine.check(lib().uc_system_eval_rhs(self.h, Xa, aa, float(t), out), "eval_rhs")
+        return list(out)
+
+
+PRESETS = {
+    # library_template/<name>/system.json (alphabet order = param order)
+    "lorenz": dict(vars=["x", "y", "z"], params=["sigma", "rho", "beta"],
+                   rhs=["sigma*(y-x)", "x*(rho-z)-y", "x*y-beta*z"],
+                   defaults=dict(sigma=10.0, rho=28.0, beta=8.0 / 3.0)),
+    "rossler": dict(vars=["x", "y", "z"], params=["a", "b", "c"],
+                    rhs=["-y-z", "x+a*y", "b+z*(x-c)"],
+                    defaults=dict(a=0.2, b=0.2, c=5.7)),
+    "chen": dict(vars=["x", "y", "z"], params=["a", "b", "c"],
+                 rhs=["a*(y-x)", "(c-a)*x-x*z+c*y", "x*y-b*z"],
+                 defaults=dict(a=40.0, b=3.0, c=28.0)),
+    # 4-D hyperchaotic Rossler: dx=-y-z, dy=x+a*y+w, dz=3+x*z, dw=-0.5*z+d*w
+    "hyper_rossler": dict(vars=["x", "y", "z", "w"], params=["a", "d"],
+                          rhs=["-y-z", "x+a*y+w", "3+x*z", "-0.5*z+d*w"],
+                          defaults=dict(a=0.25, d=0.05)),
+    # forced double-well Duffing, x'' + delta x' - x + x^3 = gamma cos(omega t)
+    "duffing": dict(vars=["x", "v"], params=["delta", "gamma", "omega"],
+                    rhs=["v", "-delta*v+x-x^3+gamma*cos(omega*t)"], time_symbol="t",
+                    defaults=dict(delta=0.15, gamma=0.3, omega=1.0)),
+}
+
+
+def preset(name, engine):
+    p = dict(PRESETS[name])
+    return System(engine, p["vars"], p["params"], p["rhs"], time_symbol=p.get("time_symbol"),
+                  defaults=p.get("defaults"), name=name)
+
+
+def axis(kind, index, n_pts, lo, hi, log_scale=False):
+    return dict(kind=kind, index=index, n_pts=int(n_pts), lo=float(lo), hi=float(hi), log_scale=bool(log_scale))
+
+
+def axis_values(ax):
+    a = Axis(ax["kind"], ax["index"], ax["n_pts"], ax["lo"], ax["hi"], int(ax["log_scale"]))
+    out = (C.c_double * ax["n_pts"])()
+    rc = lib().uc_axis_values(C.byref(a), out)
+    if rc != UC_OK:
+        raise UcError("uc_axis_values rc=%d" % rc)
+    return list(out)
+
+
+def _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
+            ic_seed=0, ic_lo=-1.0, ic_hi=1.0):
+    c = Common()
+    ic = list(ic) if ic is not None else [0.0] * sysm.n_vars
+    icarr = (C.c_double * len(ic))(*ic)
+    varr = (C.c_double * len(values))(*values)
+    c.initial_conditions = icarr
+    c.base_values = varr
+    c.n_values = len(values)
+    c.n_axes = len(axes)
+    for i, ax in enumerate(axes):
+        c.axis[i] = Axis(ax["kind"], ax["index"], ax["n_pts"], ax["lo"], ax["hi"], int(ax["log_scale"]))
+    c.h, c.transient_time, c.t_max, c.max_value = float(h), float(transient_time), float(t_max), float(max_value)
+    c.precision, c.strict = int(precision), 1 if strict else 0
+    c.ic_seed, c.ic_lo, c.ic_hi = int(ic_seed), float(ic_lo), float(ic_hi)
+    return c, (icarr, varr)
+
+
+def n_cells(axes):
+    n = 1
+    for ax in axes:
+        n *= ax["n_pts"]
+    return n
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+def _stream(stream):
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def run_bif(sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pre_scaller=1,
+            max_peaks=256, max_value=1e6, precision=PREC_F64, strict=False, traj_len=0,
+            want_final_state=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0, out=None, stream=None, sync=True):
+    """Bifurcation sweep; returns dict(flags, peaks[max_peaks, cells], intervals, traj, final_state)."""
+    import torch
+    eng = sysm.engine
+    n = n_cells(axes)
+    dev = torch.device("cuda", eng.device)
+    if out is None:
+        out = dict(flags=torch.empty(n, dtype=torch.int32, device=dev),
+                   peaks=torch.empty((max(max_peaks, 1), n), dtype=torch.float64, device=dev),
+                   intervals=torch.empty((max(max_peaks, 1), n), dtype=torch.float64, device=dev),
+                   traj=torch.empty((traj_len, n), dtype=torch.float64, device=dev) if traj_len else None,
+                   final_state=torch.empty((sysm.n_vars, n), dtype=torch.float64, device=dev)
+                   if want_final_state else None)
+    r = BifRequest()
+    r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
+                        ic_seed, ic_lo, ic_hi)
+    r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, max_peaks
+    o = BifOutput(_ptr(out["flags"]).value, _ptr(out["peaks"]).value, _ptr(out["intervals"]).value,
+                  _ptr(out.get("traj")).value, traj_len if out.get("traj") is not None else 0,
+                  _ptr(out.get("final_state")).value)
+    eng.check(lib().uc_run_bif(eng.h, sysm.h, C.byref(r), C.byref(o), _stream(stream)), "uc_run_bif")
+    if sync:
+        torch.cuda.synchronize(dev)
+    del keep
+    return out
+
+
+def run_lyap(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=False, eps=1e-4,
+             method=METHOD_VARIATIONAL, probe_seed=REFERENCE_PROBE_SEED, max_value=1e6,
+             precision=PREC_F64, strict=False, want_final_state=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0,
+             out=None, stream=None, sync=True):
+    """LLE map (spectrum=False, values[1, cells]) or full spectrum (values[N, cells])."""
+    import torch
+    eng = sysm.engine
+    n = n_cells(axes)
+    k = sysm.n_vars if spectrum else 1
+    dev = torch.device("cuda", eng.device)
+    if out is None:
+        out = dict(values=torch.empty((k, n), dtype=torch.float64, device=dev),
+                   flags=torch.empty(n, dtype=torch.int32, device=dev),
+                   final_state=torch.empty((sysm.n_vars, n), dtype=torch.float64, device=dev)
+                   if want_final_state else None)
+    r = LyapRequest()
+    r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
+                        ic_seed, ic_lo, ic_hi)
+    r.NT, r.eps, r.method, r.probe_seed = float(NT), float(eps), int(method), int(probe_seed)
+    o = LyapOutput(_ptr(out["values"]).value, _ptr(out["flags"]).value, _ptr(out.get("final_state")).value)
+    fn = lib().uc_run_ls if spectrum else lib().uc_run_lle
+    eng.check(fn(eng.h, sysm.h, C.byref(r), C.byref(o), _stream(stream)), "uc_run_ls" if spectrum else "uc_run_lle")
+    if sync:
+        torch.cuda.synchronize(dev)
+    del keep
+    return out
+
+
+def rk4_flops(ops, n, tangents=0):
+    """Algorithmic flops of one RK4 step: 4 rhs + 12 N for the state
+    (convention +,-,*,/ = 1, FMA = 2), plus 4 JV products + 12 N per tangent."""
+    f = 4 * ops["rhs_flops"] + 12 * n
+    if tangents:
+        jv = ops["jac_flops"] + tangents * (2 * ops["jac_nonzeros"] - n)
+        f += 4 * jv + 12 * n * tangents
+    return f
+
+
+__all__ = [n for n in dir() if not n.startswith("_")]
+_ = math
