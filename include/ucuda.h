@@ -147,10 +147,17 @@ typedef struct uc_basins_request {
 } uc_basins_request;
 
 typedef struct uc_basins_output {
-    int* labels;          /* [cells] canonical attractor label, 0 = unbound        */
-    int* periods;         /* [cells] detected period (0 = none found, -1 unbound)  */
-    double* cycle_point;  /* optional [label_vars][cells] canonical cycle point    */
-    long long* steps;     /* optional [2][cells] accepted / rejected steps         */
+    int* labels;          /* [cells] attractor label: 1..K canonical order, 0 unbound / unresolved */
+    int* periods;         /* [cells] period p >= 1, 0 no cycle found, -1 unbound, -2 unresolved  */
+    double* cycle_point;  /* optional [label_vars][cells] canonical cycle point (device)          */
+    long long* steps;     /* optional [2][cells] accepted / rejected steps (device)               */
+    /* attractor catalogue (HOST arrays, capacity max_attractors): label k+1 is
+     * attractor k with its period and the quantised canonical point of its
+     * lowest key; clusters are ordered by (period, point), no-cycle last. */
+    int max_attractors;
+    int* n_attractors;
+    int* attractor_period;
+    double* attractor_point;  /* [max_attractors][2] */
 } uc_basins_output;
 
 /* ---- engine ---- */
@@ -183,6 +190,13 @@ int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif
 int uc_run_lle(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream);
+
+/* The streaming peak finder over stored series (the reference's
+ * peakFinderCUDA over d_data, U-CUDA/cudaLibrary.cu:1641-1674): device
+ * data[n_samples][n_series]; counts[n_series], peaks / intervals
+ * [max_peaks][n_series]; intervals scaled by time_step. */
+int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_samples, long long n_series,
+                   double time_step, int max_peaks, int* counts, double* peaks, double* intervals, void* stream);
 
 /* Number of cells of a request's grid (product of axis point counts). */
 long long uc_grid_cells(const uc_common* c);

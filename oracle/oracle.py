@@ -290,3 +290,40 @@ def axis_values(log_scale, n, lo, hi):
     out = (C.c_double * n)()
     lib().oc_axis_values(int(log_scale), n, float(lo), float(hi), out)
     return list(out)
+
+
+class OcBasinsJob(C.Structure):
+    _fields_ = [("base", OcJob), ("rtol", C.c_double), ("atol", C.c_double), ("h_min", C.c_double),
+                ("h_max", C.c_double), ("section_period", C.c_double), ("period_tol", C.c_double),
+                ("n_sections", C.c_int), ("n_record", C.c_int), ("max_period", C.c_int), ("label_vars", C.c_int),
+                ("max_attempts", C.c_longlong)]
+
+
+def make_basins_job(n_vars, axes, ic, values, h0, rtol, atol, section_period, n_sections, n_record=16,
+                    max_period=8, label_vars=2, period_tol=1e-4, h_min=1e-12, h_max=None, max_value=1e6,
+                    max_attempts=10 ** 7):
+    bj = OcBasinsJob()
+    base = make_job(n_vars, axes, ic, values, h0, 0.0, 1.0, max_value=max_value)
+    bj._keep = base
+    bj.base = base
+    bj.rtol, bj.atol, bj.h_min = rtol, atol, h_min
+    bj.h_max = h_max if h_max is not None else section_period / 4.0
+    bj.section_period, bj.period_tol = section_period, period_tol
+    bj.n_sections, bj.n_record, bj.max_period, bj.label_vars = n_sections, n_record, max_period, label_vars
+    bj.max_attempts = max_attempts
+    return bj
+
+
+def basins_cells(funcs, bj, cells, threads=1):
+    """Per cell: period (p >= 1 cycle, 0 none found, -1 unbound, -2 unresolved),
+    canonical point [label_vars, n], accepted / rejected steps [2, n]."""
+    cells, cp = _cells(cells)
+    n = len(cells)
+    period = np.zeros(n, dtype=np.int32)
+    point = np.zeros((bj.label_vars, n))
+    steps = np.zeros((2, n), dtype=np.int64)
+    L = lib()
+    L.oc_basins_cells.argtypes = [_P, C.POINTER(OcBasinsJob), _LL, C.c_longlong, _I, _D, _LL, C.c_int]
+    L.oc_basins_cells(funcs.rhs, C.byref(bj), cp, n, period.ctypes.data_as(_I), point.ctypes.data_as(_D),
+                      steps.ctypes.data_as(_LL), threads)
+    return dict(period=period, point=point, steps=steps)

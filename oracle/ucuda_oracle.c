@@ -489,3 +489,149 @@ void oc_lyap_var_cells(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, con
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 1 ? threads : 1)
     for (long long s = 0; s < n_sel; ++s) lyap_var_cell(rhs, tan, K, j, cells[s], out + s, n_sel, &flags[s]);
 }
+
+/* ------------------------------------------------------------- basins ---
+ * New method (the reference clusters DBSCAN features instead,
+ * U-CUDA/parametric_engine.cpp:6035-6514): adaptive Dormand-Prince 5(4)
+ * (Dormand & Prince 1980 tableau, Hairer's error norm and step controller)
+ * with the step clipped to land on every Poincare section t = k*T; the
+ * state is sampled on the last n_record sections, the smallest period
+ * p <= max_period whose samples close within period_tol is found, and the
+ * cycle point with the smallest (x0, x1) is the cell's canonical point
+ * (p = 0: mean of the samples; p = -1: unbound; p = -2: unresolved). */
+static const double DP_C[7] = {0.0, 1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0, 1.0};
+static const double DP_A[7][6] = {
+    {0, 0, 0, 0, 0, 0},
+    {1.0 / 5.0, 0, 0, 0, 0, 0},
+    {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0, 0},
+    {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0, 0},
+    {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0, 0},
+    {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0, 0},
+    {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
+/* e = b5 - b4 */
+static const double DP_E[7] = {71.0 / 57600.0, 0.0, -71.0 / 16695.0, 71.0 / 1920.0, -17253.0 / 339200.0,
+                               22.0 / 525.0, -1.0 / 40.0};
+
+typedef struct oc_basins_job {
+    oc_job base;
+    double rtol, atol, h_min, h_max, section_period, period_tol;
+    int n_sections, n_record, max_period, label_vars;
+    long long max_attempts;
+} oc_basins_job;
+
+/* one DP5(4) attempt: x (n) at local time t with step h; k1 = f(t, x) on
+ * entry (FSAL); writes x5 and k7 = f(t + h, x5); returns the error norm. */
+static double dp5_attempt(oc_rhs_fn rhs, const double* x, const double* a, double t, double h, int n,
+                          const double* k1, double* x5, double* k7, double rtol, double atol) {
+    double k[7][OC_MAXN], xs[OC_MAXN];
+    for (int i = 0; i < n; ++i) k[0][i] = k1[i];
+    for (int s = 1; s < 7; ++s) {
+        for (int i = 0; i < n; ++i) {
+            double acc = 0.0;
+            for (int q = 0; q < s; ++q) acc = acc + DP_A[s][q] * k[q][i];
+            xs[i] = x[i] + h * acc;
+        }
+        rhs(xs, a, t + DP_C[s] * h, k[s]);
+    }
+    /* stage 7 is evaluated at x5 itself (FSAL) */
+    for (int i = 0; i < n; ++i) { x5[i] = xs[i]; k7[i] = k[6][i]; }
+    double err = 0.0;
+    for (int i = 0; i < n; ++i) {
+        double e = 0.0;
+        for (int s = 0; s < 7; ++s) e = e + DP_E[s] * k[s][i];
+        e = h * e;
+        const double sc = atol + rtol * fmax(fabs(x[i]), fabs(x5[i]));
+        err = err + (e / sc) * (e / sc);
+    }
+    return sqrt(err / (double)n);
+}
+
+static int basins_cell(oc_rhs_fn rhs, const oc_basins_job* bj, long long cell, double* point, int* period,
+                       long long* steps) {
+    const oc_job* j = &bj->base;
+    const int n = j->n_vars, D = bj->label_vars, L = bj->n_record;
+    double a[OC_MAXV], x[OC_MAXN], h;
+    double buf[64][4];
+    oc_cell_setup(j, cell, a, x, &h);
+    const double T = bj->section_period;
+    double k1[OC_MAXN], x5[OC_MAXN], k7[OC_MAXN];
+    long long acc_steps = 0, rej_steps = 0, attempts = 0;
+    for (int i = 0; i < D; ++i) point[i] = 0.0;
+    for (int sec = 0; sec < bj->n_sections; ++sec) {
+        double t = 0.0;
+        rhs(x, a, t, k1);
+        for (;;) {
+            if (attempts >= bj->max_attempts) { *period = -2; steps[0] = acc_steps; steps[1] = rej_steps; return 0; }
+            ++attempts;
+            const int last = t + h >= T;
+            const double hs = last ? T - t : h;
+            const double err = dp5_attempt(rhs, x, a, t, hs, n, k1, x5, k7, bj->rtol, bj->atol);
+            double fac = err == 0.0 ? 5.0 : 0.9 * pow(err, -0.2);
+            fac = fac < 0.2 ? 0.2 : (fac > 5.0 ? 5.0 : fac);
+            if (err <= 1.0) {
+                ++acc_steps;
+                for (int i = 0; i < n; ++i) { x[i] = x5[i]; k1[i] = k7[i]; }
+                double chk = 0.0;
+                for (int i = 0; i < n; ++i) chk += fabs(x[i]);
+                if (isnan(chk) || isinf(chk) || (j->max_value != 0.0 && chk > j->max_value)) {
+                    *period = -1; steps[0] = acc_steps; steps[1] = rej_steps; return 0;
+                }
+                if (last) {
+                    /* keep the unclipped step for the next section */
+                    const double hn = hs * fac;
+                    if (hn > h) h = hn;
+                    if (h > bj->h_max) h = bj->h_max;
+                    break;
+                }
+                t = t + hs;
+                h = hs * fac;
+            } else {
+                ++rej_steps;
+                h = hs * fac;
+            }
+            if (h > bj->h_max) h = bj->h_max;
+            if (h < bj->h_min) { *period = -2; steps[0] = acc_steps; steps[1] = rej_steps; return 0; }
+        }
+        for (int r = L - 1; r > 0; --r)
+            for (int i = 0; i < D; ++i) buf[r][i] = buf[r - 1][i];
+        for (int i = 0; i < D; ++i) buf[0][i] = x[i];
+    }
+    steps[0] = acc_steps;
+    steps[1] = rej_steps;
+    int p = 0;
+    for (int q = 1; q <= bj->max_period && q < L && !p; ++q) {
+        int closes = 1;
+        for (int r = 0; r + q < L && closes; ++r)
+            for (int i = 0; i < D; ++i)
+                if (fabs(buf[r][i] - buf[r + q][i]) > bj->period_tol * (1.0 + fabs(buf[r][i]))) { closes = 0; break; }
+        if (closes) p = q;
+    }
+    if (p > 0) {
+        int best = 0;
+        for (int r = 1; r < p; ++r) {
+            if (buf[r][0] < buf[best][0] || (buf[r][0] == buf[best][0] && D > 1 && buf[r][1] < buf[best][1])) best = r;
+        }
+        for (int i = 0; i < D; ++i) point[i] = buf[best][i];
+    } else {
+        for (int i = 0; i < D; ++i) {
+            double m = 0.0;
+            for (int r = 0; r < L; ++r) m += buf[r][i];
+            point[i] = m / (double)L;
+        }
+    }
+    *period = p;
+    return 1;
+}
+
+void oc_basins_cells(oc_rhs_fn rhs, const oc_basins_job* bj, const long long* cells, long long n_sel, int* period,
+                     double* point, long long* steps, int threads) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 1 ? threads : 1)
+    for (long long s = 0; s < n_sel; ++s) {
+        double pt[4];
+        long long st[2];
+        basins_cell(rhs, bj, cells[s], pt, &period[s], st);
+        for (int i = 0; i < bj->label_vars; ++i) point[(long long)i * n_sel + s] = pt[i];
+        steps[s] = st[0];
+        steps[n_sel + s] = st[1];
+    }
+}

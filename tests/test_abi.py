@@ -1,0 +1,105 @@
+"""The C ABI library (CPU only): loads, exports every symbol include/ucuda.h
+declares, validates requests before touching a device, and reports errors
+through status codes + uc_last_error."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import ucuda
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "ucuda.h")).read()
+    return sorted(set(re.findall(r"\b(uc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", ucuda.LIB_PATH], capture_output=True, text=True, check=True)
+    exported = set(line.split()[-1] for line in out.stdout.splitlines() if line.strip())
+    decl = declared_symbols()
+    assert decl, "no declarations parsed"
+    missing = [s for s in decl if s not in exported]
+    assert not missing, missing
+    # the Python binding knows every declared entry point
+    assert set(decl) == set(ucuda.exported_symbols())
+
+
+def test_compile_only_engine_and_errors(cpu_engine):
+    s = ucuda.preset("lorenz", cpu_engine)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 8, 0.0, 10.0)]
+    r = ucuda.BifRequest()
+    r.c, keep = ucuda._common(s, axes, [0.1, 0.1, 0.1], s.values(), 0.01, 1.0, 1.0, 1e6, 0, 0)
+    r.writable_var, r.pre_scaller, r.max_peaks = 0, 1, 8
+    o = ucuda.BifOutput(1, 1, 1, 0, 0, 0)  # never dereferenced: no device
+    L = ucuda.lib()
+    # a valid request on an engine without a device -> UC_ERR_NO_DEVICE
+    assert L.uc_run_bif(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 5
+    # argument errors are reported before any device work
+    r.c.axis[0].index = 9
+    assert L.uc_run_bif(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    assert "param_index" in cpu_engine.error()
+    r.c.axis[0].index = 2
+    r.pre_scaller = 0
+    assert L.uc_run_bif(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    r.pre_scaller = 1
+    r.c.n_axes = 0
+    assert L.uc_run_bif(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    r.c.n_axes = 1
+    r.c.h = -1.0
+    assert L.uc_run_bif(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    r.c.h = 0.01
+    r.max_peaks = 3000
+    assert L.uc_run_bif(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    del keep
+
+
+def test_lyap_validation(cpu_engine):
+    s = ucuda.preset("rossler", cpu_engine)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 4, 0.1, 0.3), ucuda.axis(ucuda.AXIS_STEP, 0, 4, 0.005, 0.01)]
+    r = ucuda.LyapRequest()
+    r.c, keep = ucuda._common(s, axes, [1, 1, 1], s.values(), 0.01, 1.0, 10.0, 1e6, 0, 0)
+    r.NT, r.eps, r.method, r.probe_seed = 0.1, 1e-4, 0, 1234567891
+    o = ucuda.LyapOutput(1, 1, 0)
+    L = ucuda.lib()
+    assert L.uc_run_lle(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 5
+    r.NT = 0.0
+    assert L.uc_run_lle(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    r.NT = 0.1
+    r.method = 7
+    assert L.uc_run_ls(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    r.method = 0
+    r.c.axis[0].kind = ucuda.AXIS_STEP  # two step axes
+    assert L.uc_run_lle(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    del keep
+
+
+def test_basins_validation(cpu_engine):
+    s = ucuda.preset("duffing", cpu_engine)
+    axes = [ucuda.axis(ucuda.AXIS_VAR, 0, 4, -1, 1), ucuda.axis(ucuda.AXIS_VAR, 1, 4, -1, 1)]
+    r = ucuda.BasinsRequest()
+    r.c, keep = ucuda._common(s, axes, [0, 0], s.values(), 0.01, 0.0, 1.0, 1e6, 0, 0)
+    r.rtol = r.atol = 1e-9
+    r.h_min, r.h_max, r.section_period = 1e-12, 1.0, 6.283185307179586
+    r.n_sections, r.n_record, r.max_period, r.label_vars, r.period_tol, r.quantum = 10, 16, 8, 2, 1e-4, 1e-3
+    o = ucuda.BasinsOutput(1, 1, 0, 0, 0, None, None, None)
+    L = ucuda.lib()
+    assert L.uc_run_basins(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 5
+    r.max_period = 16
+    assert L.uc_run_basins(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    r.max_period = 8
+    r.n_record = 40
+    assert L.uc_run_basins(cpu_engine.h, s.h, C.byref(r), C.byref(o), None) == 1
+    del keep
+
+
+def test_version_and_sources(cpu_engine):
+    assert ucuda.lib().uc_version().decode().startswith("ucuda")
+    s = ucuda.preset("duffing", cpu_engine)
+    src = s.source(0)
+    assert "uc_bif_f64" in src and "uc_bas_f32c" in src and "#define UC_USES_TIME 1" in src
+    assert "cos(a[3] * t)" in s.source(1)

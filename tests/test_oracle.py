@@ -1,0 +1,248 @@
+"""Pins of the FP64 CPU oracle (CPU only).
+
+Each check is chosen so a plausible slip in the oracle (dropped term, wrong
+index or sign, swapped operand) fails it: closed forms (RK4 on linear
+systems, exact Lyapunov exponents of diagonal linear systems, trace
+invariants), independent known-answer vectors (generator integers computed
+here from the published splitmix64 / LCG definition), hand-worked peak
+sequences, and algebraic identities of Gram-Schmidt.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import ucuda
+
+D = C.POINTER(C.c_double)
+
+
+def _linear_system(eng, A):
+    n = len(A)
+    names = ["x%d" % i for i in range(n)]
+    rhs = []
+    for i in range(n):
+        terms = ["%r*%s" % (float(A[i][j]), names[j]) for j in range(n) if A[i][j] != 0]
+        rhs.append("+".join(terms) if terms else "0")
+    return ucuda.System(eng, names, [], rhs)
+
+
+# ------------------------------------------------------------------ RNG ----
+def _ref_rng(seed, seq, count):
+    """splitmix64 seeding + LCG draw, written from the published definition
+    (U-CUDA/kernels/lle1d.template.cu:39-57) with Python integers."""
+    M = (1 << 64) - 1
+    z = (seed + seq * 0x9E3779B97F4A7C15) & M
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+    s = z ^ (z >> 31)
+    out = []
+    for _ in range(count):
+        s = (s * 6364136223846793005 + 1442695040888963407) & M
+        out.append(float(np.float32(((s >> 40) & 0xFFFFFF) + 1) / np.float32(16777216.0)))
+    return out
+
+
+def test_rng_known_answers():
+    for seed, seq in ((1234567891, 0), (1234567891, 1), (1234567891, 262143), (20261203, 77)):
+        assert oracle.rng_uniforms(seed, seq, 16) == _ref_rng(seed, seq, 16)
+    # draws lie in (0, 1]
+    u = oracle.rng_uniforms(1234567891, 5, 1000)
+    assert min(u) > 0.0 and max(u) <= 1.0
+
+
+# ----------------------------------------------------------------- grid ----
+def test_axis_values_closed_forms():
+    v = oracle.axis_values(False, 1024, 0.0, 250.0)
+    assert v[0] == 0.0 and v[-1] == 250.0
+    assert v[1] == 250.0 / 1023.0
+    assert oracle.axis_values(False, 1, 3.0, 7.0) == [7.0]  # n == 1 -> hi (kernel convention)
+    lg = oracle.axis_values(True, 5, 1e-3, 10.0)
+    np.testing.assert_allclose(lg, [1e-3, 1e-2, 1e-1, 1.0, 10.0], rtol=1e-14)
+    # engine host tables use the same formula
+    assert ucuda.axis_values(ucuda.axis(1, 1, 1024, 0.0, 250.0)) == v
+    assert ucuda.axis_values(ucuda.axis(1, 1, 5, 1e-3, 10.0, True)) == lg
+
+
+# ------------------------------------------------------------------ RK4 ----
+def test_rk4_step_is_the_taylor_polynomial_on_linear_systems(cpu_engine):
+    A = np.array([[-0.3, 1.1, 0.0], [-1.2, -0.1, 0.4], [0.2, 0.0, -0.7]])
+    s = _linear_system(cpu_engine, A)
+    step = oracle.KrsStep(cpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 1)
+    h = 0.05
+    x0 = np.array([0.3, -1.0, 2.0])
+    x1 = np.array(step.step(list(x0), [0.5], h))
+    hA = h * A
+    P = np.eye(3) + hA + hA @ hA / 2 + hA @ hA @ hA / 6 + hA @ hA @ hA @ hA / 24
+    np.testing.assert_allclose(x1, P @ x0, rtol=1e-14, atol=1e-15)
+    # the engine-order RK4 on the generated right-hand side is the same map
+    funcs = oracle.SystemFuncs(s.source(1))
+    xa = (C.c_double * 3)(*x0)
+    oracle.lib().oc_rk4(funcs.rhs, xa, (C.c_double * 1)(0.5), h, 3)
+    assert list(xa) == list(x1)
+
+
+def test_rk4_tangent_is_the_derivative_of_the_step(cpu_engine):
+    s = ucuda.preset("lorenz", cpu_engine)
+    funcs = oracle.SystemFuncs(s.source(1))
+    a = (C.c_double * 4)(*s.values())
+    x = np.array([1.3, -0.4, 20.0])
+    V = np.array([0.3, 0.5, -0.2])
+    xa, Va = (C.c_double * 3)(*x), (C.c_double * 3)(*V)
+    oracle.lib().oc_rk4_tan(funcs.rhs, funcs.tan, xa, Va, 1, a, 0.01, 3)
+    e = 1e-6
+    xp, xm = (C.c_double * 3)(*(x + e * V)), (C.c_double * 3)(*(x - e * V))
+    oracle.lib().oc_rk4(funcs.rhs, xp, a, 0.01, 3)
+    oracle.lib().oc_rk4(funcs.rhs, xm, a, 0.01, 3)
+    fd = (np.array(list(xp)) - np.array(list(xm))) / (2 * e)
+    np.testing.assert_allclose(list(Va), fd, rtol=1e-7, atol=1e-8)
+
+
+# ----------------------------------------------------------------- loop ----
+def test_loop_flags(cpu_engine):
+    lor = ucuda.preset("lorenz", cpu_engine)
+    step = oracle.KrsStep(cpu_engine.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
+    L = oracle.lib()
+    x = (C.c_double * 3)(0.1, 0.1, 0.1)
+    a = (C.c_double * 4)(*lor.values(rho=0.5))
+    # rho < 1: the origin attracts -> fixed point
+    assert L.oc_loop_model(step.ptr, x, a, 0.01, 20000, 3, 1, 0, 1e6, None) == -1
+    x = (C.c_double * 3)(0.1, 0.1, 0.1)
+    a = (C.c_double * 4)(*lor.values(rho=28.0))
+    assert L.oc_loop_model(step.ptr, x, a, 0.01, 2000, 3, 1, 0, 1e6, None) == 1
+    # magnitude check: sum|x| above max_value -> 0
+    x = (C.c_double * 3)(0.1, 0.1, 0.1)
+    assert L.oc_loop_model(step.ptr, x, a, 0.01, 2000, 3, 1, 0, 5.0, None) == 0
+    # blow-up x' = x^2 -> inf/nan -> 0 even with max_value disabled
+    bl = ucuda.System(cpu_engine, ["x"], [], ["x*x"])
+    sb = oracle.KrsStep(cpu_engine.codegen_scheme(bl.vars, bl.params, bl.rhs), 1, 1)
+    x = (C.c_double * 1)(1.0)
+    assert L.oc_loop_model(sb.ptr, x, (C.c_double * 1)(0.5), 0.1, 1000, 1, 1, 0, 0.0, None) == 0
+
+
+def test_loop_records_before_stepping(cpu_engine):
+    lor = ucuda.preset("lorenz", cpu_engine)
+    step = oracle.KrsStep(cpu_engine.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
+    a = lor.values()
+    data = (C.c_double * 5)()
+    x = (C.c_double * 3)(1.0, 2.0, 3.0)
+    oracle.lib().oc_loop_model(step.ptr, x, (C.c_double * 4)(*a), 0.01, 5, 3, 1, -1, 1e6, data)
+    assert data[0] == 1.0 + math.pi * 2.0 + math.e * 3.0  # x0 + pi*x1 + e*x2 of the initial state
+    x2 = step.step([1.0, 2.0, 3.0], a, 0.01)
+    assert data[1] == x2[0] + math.pi * x2[1] + math.e * x2[2]
+
+
+# ---------------------------------------------------------- peak finder ----
+def test_peak_finder_on_sampled_sine():
+    h = 0.01
+    t = np.arange(20000) * h
+    pk, iv = oracle.peak_finder(np.sin(2.0 * t), h)
+    # maxima at t = pi/4 + k*pi; the first is dropped; parabolic error O(h^2)
+    assert len(pk) == int((t[-1] - math.pi / 4) / math.pi + 1) - 1
+    np.testing.assert_allclose(pk, 1.0, atol=1e-9)
+    np.testing.assert_allclose(iv, math.pi, atol=1e-6)
+
+
+def test_peak_finder_hand_worked_sequences():
+    # plateau of three equal samples: one peak at the LAST plateau sample,
+    # refined with its neighbours (denominator 2 - 2*2 + 1 = -1, delta = 0.5)
+    d = [0, 1, 2, 2, 2, 1, 0, 1, 3, 1, 0, 0, 5, 4, 0, 0]
+    pk, iv = oracle.peak_finder(d, 1.0)
+    # raw peaks: j=4 (value 2, delta=(0.5*(2-1))/(2-4+1)=-0.5 -> value 2-0.25*1*(-0.5)=2.125, t=3-0.5=2.5),
+    #            j=8 (3: delta=0.5*(1-1)/(1-6+1)=0 -> 3, t=7), j=12 (5: delta=0.5*(0-4)/(0-10+4)=1/3 ->
+    #            5-0.25*(-4)*(1/3)=5.3333, t=11+1/3); first dropped
+    np.testing.assert_allclose(pk, [3.0, 5.0 + 1.0 / 3.0])
+    np.testing.assert_allclose(iv, [7.0 - 2.5, 11.0 + 1.0 / 3.0 - 7.0])
+    # rising plateau is not a peak; a single peak gives count 0 after the shift
+    assert len(oracle.peak_finder([0, 1, 1, 2, 3, 2, 1, 0], 1.0)[0]) == 0
+    assert len(oracle.peak_finder([0, 1, 2], 1.0)[0]) == 0
+    # a walk that reaches the end of the record is no peak
+    pk, _ = oracle.peak_finder([0, 1, 5, 1, 0, 1, 2, 2, 2, 2], 1.0)
+    assert len(pk) == 0
+
+
+# -------------------------------------------------------- Gram-Schmidt ----
+@pytest.mark.parametrize("n", [2, 3, 4])
+def test_gram_schmidt_identities(n):
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    B = np.zeros(n * n)
+    den = np.zeros(n)
+    oracle.lib().oc_cgs(np.ascontiguousarray(A.ravel()).ctypes.data_as(D), B.ctypes.data_as(D), n,
+                        den.ctypes.data_as(D))
+    Q = B.reshape(n, n)
+    np.testing.assert_allclose(Q @ Q.T, np.eye(n), atol=1e-12)
+    R = np.linalg.qr(A.T)[1]  # rows of A are the vectors
+    np.testing.assert_allclose(den, np.abs(np.diag(R)), rtol=1e-12)
+    V = np.ascontiguousarray(A.ravel().copy())
+    d2 = np.zeros(n)
+    oracle.lib().oc_mgs(V.ctypes.data_as(D), n, n, d2.ctypes.data_as(D))
+    np.testing.assert_allclose(d2, den, rtol=1e-12)
+    np.testing.assert_allclose(V, B, atol=1e-12)
+
+
+# ---------------------------------------------------- Lyapunov exponents ----
+def _rk4_amp(z):
+    return 1 + z + z * z / 2 + z ** 3 / 6 + z ** 4 / 24
+
+
+def test_lyapunov_on_diagonal_linear_system(cpu_engine):
+    """diag(mu): every block multiplies volume by prod R(h mu)^(nts+1) exactly
+    (R = RK4 amplification), so the SUM of exponents is exact; each exponent
+    converges to its closed form up to the initial-alignment term
+    log|c|/t_max of the random probe basis."""
+    mus = [-0.5, -1.0, -2.0]
+    s = _linear_system(cpu_engine, np.diag(mus))
+    h, NT, t_max = 0.01, 0.1, 200.0
+    ax = [ucuda.axis(ucuda.AXIS_VAR, 0, 2, 1.0, 2.0)]
+    job = oracle.make_job(3, ax, [1.0, 1.0, 1.0], [0.5], h, 1.0, t_max, NT=NT, max_value=0.0)
+    blocks, nts = int(t_max / NT), int(NT / h)
+    expect = np.array([blocks * (nts + 1) * math.log(_rk4_amp(h * m)) / t_max for m in mus])
+    funcs = oracle.SystemFuncs(s.source(1))
+    ls = oracle.lyap_var_cells(funcs, 3, job, [0, 1])["values"]
+    np.testing.assert_allclose(ls.sum(axis=0), expect.sum(), rtol=1e-12)
+    np.testing.assert_allclose(ls[:, 0], expect, atol=0.03)
+    lle = oracle.lyap_var_cells(funcs, 1, job, [0, 1])["values"][0]
+    np.testing.assert_allclose(lle, expect[0], atol=0.03)
+    step = oracle.KrsStep(cpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 1)
+    # the reference finite-difference method is exact on a linear system:
+    # identical to the variational result up to rounding
+    np.testing.assert_allclose(oracle.ls_fd_cells(step, job, [0, 1])["values"], ls, rtol=1e-6, atol=1e-9)
+    np.testing.assert_allclose(oracle.lle_fd_cells(step, job, [0, 1])["values"][0], lle, rtol=1e-6, atol=1e-9)
+
+
+def test_lorenz_spectrum_invariants(cpu_engine):
+    s = ucuda.preset("lorenz", cpu_engine)
+    funcs = oracle.SystemFuncs(s.source(1))
+    h, NT, t_max = 0.01, 0.1, 200.0
+    ax = [ucuda.axis(ucuda.AXIS_PARAM, 2, 1, 28.0, 28.0)]
+    job = oracle.make_job(3, ax, [0.1, 0.1, 0.1], s.values(), h, 20.0, t_max, NT=NT)
+    ls = oracle.lyap_var_cells(funcs, 3, job, [0])["values"][:, 0]
+    factor = int(t_max / NT) * (int(NT / h) + 1) * h / t_max  # steps actually taken / t_max
+    # constant trace: sum of exponents = -(sigma + 1 + beta)
+    np.testing.assert_allclose(ls.sum(), -(10 + 1 + 8 / 3) * factor, rtol=1e-3)
+    # published values (sanity bounds): 0.906, 0, -14.57
+    assert abs(ls[0] / factor - 0.906) < 0.06
+    assert abs(ls[1] / factor) < 0.02
+
+
+def test_variational_is_the_small_eps_limit_of_the_reference_method(cpu_engine):
+    s = ucuda.preset("rossler", cpu_engine)
+    funcs = oracle.SystemFuncs(s.source(1))
+    step = oracle.KrsStep(cpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
+    ax = [ucuda.axis(ucuda.AXIS_PARAM, 3, 6, 4.0, 9.0)]
+    job = oracle.make_job(3, ax, [1.0, 1.0, 1.0], s.values(), 0.01, 50.0, 200.0, NT=0.1)
+    var = oracle.lyap_var_cells(funcs, 1, job, range(6))["values"][0]
+    gaps = []
+    for eps in (1e-4, 1e-5, 1e-6):
+        job.eps = eps
+        gaps.append(np.abs(oracle.lle_fd_cells(step, job, range(6))["values"][0] - var).max())
+    assert gaps[1] < gaps[0] / 3 and gaps[2] < gaps[1] / 3 and gaps[2] < 1e-5
+
+
+def test_krs_index_checker():
+    assert oracle.check_indices("X[0] = X[2] + a[3];", 3, 4) == []
+    d = oracle.check_indices("X[0] = X[3];\n a[4] = 1; // X[9]", 3, 4)
+    assert [line for line, _ in d] == [1, 2]

@@ -658,7 +658,9 @@ std::string normalize_value(const std::string& value) {
     P t = Parser(value.substr(a, b - a + 1), false).parse();
     Style st;
     std::function<void(const P&)> check = [&](const P& n) {
-        if (n->k == K::Const && n->name != "pi") throw std::runtime_error("unknown symbol: " + n->name);
+        // values carry no names: every symbol (pi included) is an error, as in
+        // the reference (it prints through an empty name map)
+        if (n->k == K::Const) throw std::runtime_error("unknown symbol: " + n->name);
         if (n->a) check(n->a);
         if (n->b) check(n->b);
         for (auto& x : n->args) check(x);

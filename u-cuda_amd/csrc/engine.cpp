@@ -19,7 +19,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <list>
+#include <map>
+#include <tuple>
+#include <unordered_map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -428,9 +432,8 @@ int uc_axis_values(const uc_axis* ax, double* out) {
 int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, void* stream) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> lk(e->mu);
-    int rc = enter(e);
+    int rc = validate_common(e, s, r->c, true);
     if (rc) return rc;
-    if ((rc = validate_common(e, s, r->c, true))) return rc;
     if (r->writable_var < -1 || r->writable_var >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "writable_var out of range");
     if (r->pre_scaller <= 0) return e->fail(UC_ERR_ARGUMENT, "pre_scaller must be > 0");
     if (r->max_peaks < 0) return e->fail(UC_ERR_ARGUMENT, "max_peaks must be >= 0");
@@ -443,6 +446,8 @@ int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif
     if (!(hw > 0.0)) return e->fail(UC_ERR_ARGUMENT, "h must be > 0 (whole swept range)");
     const long long iters = (long long)std::ceil(r->c.t_max / hw / r->pre_scaller);
     if (iters <= 0) return e->fail(UC_ERR_ARGUMENT, "computed amountOfPointsInBlock <= 0");
+    if (r->max_peaks > 2500) return e->fail(UC_ERR_ARGUMENT, "max_peaks above the reference cap 2500");
+    if ((rc = enter(e))) return rc;
 
     Module* m = nullptr;
     if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
@@ -467,9 +472,8 @@ int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif
 static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream, bool spectrum) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> lk(e->mu);
-    int rc = enter(e);
+    int rc = validate_common(e, s, r->c, true);
     if (rc) return rc;
-    if ((rc = validate_common(e, s, r->c, true))) return rc;
     if (!(r->NT > 0.0)) return e->fail(UC_ERR_ARGUMENT, "NT must be > 0");
     if (!(r->eps > 0.0)) return e->fail(UC_ERR_ARGUMENT, "eps must be > 0");
     if (r->method != UC_METHOD_VARIATIONAL && r->method != UC_METHOD_FINITE_DIFF) return e->fail(UC_ERR_ARGUMENT, "unknown method");
@@ -482,6 +486,7 @@ static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const 
     for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
     const long long nts = r->c.h > 0.0 ? (long long)(r->NT / r->c.h) : 0;
     if (!step_axis && nts <= 0) return e->fail(UC_ERR_ARGUMENT, "computed NT / h <= 0");
+    if ((rc = enter(e))) return rc;
 
     Module* m = nullptr;
     if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
@@ -513,9 +518,166 @@ int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lya
 }
 
 int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream) {
-    (void)s; (void)r; (void)o; (void)stream;
-    if (!e) return UC_ERR_ARGUMENT;
-    return e->fail(UC_ERR_ARGUMENT, "basins: not available in this build");
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    int rc = UC_OK;
+    uc_common c = r->c;
+    if (c.t_max <= 0.0) c.t_max = 1.0;  // unused by basins
+    if ((rc = validate_common(e, s, c, false))) return rc;
+    const int lbl = s->n_vars < 2 ? s->n_vars : 2;
+    if (!(r->rtol > 0.0) || !(r->atol > 0.0)) return e->fail(UC_ERR_ARGUMENT, "rtol and atol must be > 0");
+    if (!(r->section_period > 0.0)) return e->fail(UC_ERR_ARGUMENT, "section_period must be > 0");
+    if (r->n_sections < 1) return e->fail(UC_ERR_ARGUMENT, "n_sections must be >= 1");
+    if (r->n_record < 2 || r->n_record > 16) return e->fail(UC_ERR_ARGUMENT, "n_record must be in [2, 16]");
+    if (r->max_period < 1 || r->max_period > 8 || r->max_period >= r->n_record)
+        return e->fail(UC_ERR_ARGUMENT, "max_period must be in [1, min(8, n_record - 1)]");
+    if (r->label_vars != 0 && r->label_vars != lbl) return e->fail(UC_ERR_ARGUMENT, "label_vars must be min(n_vars, 2)");
+    if (!(r->period_tol > 0.0) || !(r->quantum > 0.0)) return e->fail(UC_ERR_ARGUMENT, "period_tol and quantum must be > 0");
+    if (!(r->h_min > 0.0) || !(r->h_max >= r->h_min)) return e->fail(UC_ERR_ARGUMENT, "need 0 < h_min <= h_max");
+    if (!o->labels || !o->periods) return e->fail(UC_ERR_ARGUMENT, "labels / periods outputs missing");
+    if ((rc = enter(e))) return rc;
+
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = stage_tables(e, s, c, j, st, &tb))) return rc;
+    const long long n = j.n_cells;
+    j.rtol = r->rtol; j.atol = r->atol; j.h_min = r->h_min; j.h_max = r->h_max;
+    j.section_period = r->section_period; j.n_sections = r->n_sections; j.n_record = r->n_record;
+    j.max_period = r->max_period; j.label_vars = lbl; j.period_tol = r->period_tol; j.quantum = r->quantum;
+    j.max_attempts = r->max_attempts > 0 ? r->max_attempts : 100000000LL;
+    j.flag = o->periods;
+    j.counters = o->steps;
+    j.label = o->labels;
+
+    // temporaries: canonical points (when the caller does not want them) and the key table
+    std::vector<void*> temps;
+    auto cleanup = [&]() { for (void* p : temps) (void)hipFree(p); };
+    auto dev_alloc = [&](void** p, size_t bytes) -> int {
+        hipError_t he = hipMalloc(p, bytes);
+        if (he != hipSuccess) { cleanup(); return e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(basins)")); }
+        temps.push_back(*p);
+        return UC_OK;
+    };
+    double* points = o->cycle_point;
+    if (!points && (rc = dev_alloc((void**)&points, sizeof(double) * (size_t)lbl * (size_t)n))) return rc;
+    j.out0 = points;
+    if ((rc = launch(e, m, c.precision == UC_PREC_F64 ? "uc_bas_f64" : "uc_bas_f32c", j, st, tb))) { cleanup(); return rc; }
+
+    // ---- catalogue of distinct quantised signatures ----
+    hipFunction_t fkeys, flabels;
+    hipError_t he = hipModuleGetFunction(&fkeys, m->mod, "uc_bas_keys");
+    if (he == hipSuccess) he = hipModuleGetFunction(&flabels, m->mod, "uc_bas_labels");
+    if (he != hipSuccess) { cleanup(); return e->fail(UC_ERR_HIP, hip_msg(he, "basins label kernels")); }
+    const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
+    int cap = 1 << 16;
+    std::vector<unsigned long long> keys;
+    unsigned long long* table = nullptr;
+    unsigned long long* overflow = nullptr;
+    if ((rc = dev_alloc((void**)&overflow, sizeof(unsigned long long)))) return rc;
+    for (;;) {
+        if ((rc = dev_alloc((void**)&table, sizeof(unsigned long long) * (size_t)cap))) return rc;
+        he = hipMemsetAsync(table, 0xFF, sizeof(unsigned long long) * (size_t)cap, st);
+        if (he == hipSuccess) he = hipMemsetAsync(overflow, 0, sizeof(unsigned long long), st);
+        void* kargs[] = {&j, &table, &cap, &overflow};
+        if (he == hipSuccess) he = hipModuleLaunchKernel(fkeys, blocks, 1, 1, kBlock, 1, 1, 0, st, kargs, nullptr);
+        unsigned long long ovf = 0;
+        keys.assign((size_t)cap, 0ULL);
+        if (he == hipSuccess) he = hipMemcpyAsync(keys.data(), table, sizeof(unsigned long long) * (size_t)cap, hipMemcpyDeviceToHost, st);
+        if (he == hipSuccess) he = hipMemcpyAsync(&ovf, overflow, sizeof ovf, hipMemcpyDeviceToHost, st);
+        if (he == hipSuccess) he = hipStreamSynchronize(st);
+        if (he != hipSuccess) { cleanup(); return e->fail(UC_ERR_HIP, hip_msg(he, "basins signature keys")); }
+        if (ovf == 0) break;
+        if (cap >= (1 << 24)) { cleanup(); return e->fail(UC_ERR_ARGUMENT, "too many distinct attractor signatures (raise quantum)"); }
+        cap <<= 2;
+    }
+
+    // ---- host: cluster adjacent keys, canonical order, labels 1..K ----
+    struct KeyInfo { int p; long long qx, qv; };
+    auto decode = [](unsigned long long k) {
+        KeyInfo ki;
+        ki.p = (int)(k >> 56);
+        auto sx = [](unsigned long long v) { v &= (1ULL << 28) - 1ULL; return (long long)(v ^ (1ULL << 27)) - (1LL << 27); };
+        ki.qx = sx(k >> 28);
+        ki.qv = sx(k);
+        return ki;
+    };
+    std::vector<int> slots;
+    std::unordered_map<unsigned long long, int> index;
+    for (int i = 0; i < cap; ++i)
+        if (keys[(size_t)i] != ~0ULL) { index[keys[(size_t)i]] = (int)slots.size(); slots.push_back(i); }
+    std::vector<int> parent(slots.size());
+    for (size_t i = 0; i < parent.size(); ++i) parent[i] = (int)i;
+    std::function<int(int)> find = [&](int x) { while (parent[(size_t)x] != x) x = parent[(size_t)x] = parent[(size_t)parent[(size_t)x]]; return x; };
+    auto encode = [](int p, long long qx, long long qv) {
+        const unsigned long long mask = (1ULL << 28) - 1ULL;
+        return ((unsigned long long)(p & 0xFF) << 56) | (((unsigned long long)qx & mask) << 28) | ((unsigned long long)qv & mask);
+    };
+    for (size_t i = 0; i < slots.size(); ++i) {
+        KeyInfo ki = decode(keys[(size_t)slots[i]]);
+        if (ki.p == 0) continue;
+        for (long long dx = -1; dx <= 1; ++dx)
+            for (long long dv = -1; dv <= 1; ++dv) {
+                auto it = index.find(encode(ki.p, ki.qx + dx, ki.qv + dv));
+                if (it != index.end()) { int a = find((int)i), b = find(it->second); if (a != b) parent[(size_t)std::max(a, b)] = std::min(a, b); }
+            }
+    }
+    // representative per cluster: the smallest (qx, qv) key
+    std::map<int, KeyInfo> rep;
+    for (size_t i = 0; i < slots.size(); ++i) {
+        KeyInfo ki = decode(keys[(size_t)slots[i]]);
+        int root = find((int)i);
+        auto it = rep.find(root);
+        if (it == rep.end() || std::make_pair(ki.qx, ki.qv) < std::make_pair(it->second.qx, it->second.qv)) rep[root] = ki;
+    }
+    std::vector<std::pair<std::tuple<int, long long, long long>, int>> order;
+    for (auto& kv : rep) order.push_back({std::make_tuple(kv.second.p == 0 ? 1000 : kv.second.p, kv.second.qx, kv.second.qv), kv.first});
+    std::sort(order.begin(), order.end());
+    std::map<int, int> label_of_root;
+    for (size_t k = 0; k < order.size(); ++k) label_of_root[order[k].second] = (int)k + 1;
+    std::vector<int> slot_label((size_t)cap, 0);
+    for (size_t i = 0; i < slots.size(); ++i) slot_label[(size_t)slots[i]] = label_of_root[find((int)i)];
+    if (o->n_attractors) *o->n_attractors = (int)order.size();
+    for (size_t k = 0; k < order.size() && (int)k < o->max_attractors; ++k) {
+        const KeyInfo& ki = rep[order[k].second];
+        if (o->attractor_period) o->attractor_period[k] = ki.p;
+        if (o->attractor_point) {
+            o->attractor_point[2 * k] = ki.p ? (double)ki.qx * r->quantum : 0.0;
+            o->attractor_point[2 * k + 1] = ki.p ? (double)ki.qv * r->quantum : 0.0;
+        }
+    }
+    int* dlabels = nullptr;
+    if ((rc = dev_alloc((void**)&dlabels, sizeof(int) * (size_t)cap))) return rc;
+    he = hipMemcpyAsync(dlabels, slot_label.data(), sizeof(int) * (size_t)cap, hipMemcpyHostToDevice, st);
+    void* largs[] = {&j, &table, &dlabels, &cap};
+    if (he == hipSuccess) he = hipModuleLaunchKernel(flabels, blocks, 1, 1, kBlock, 1, 1, 0, st, largs, nullptr);
+    if (he == hipSuccess) he = hipStreamSynchronize(st);
+    cleanup();
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "basins labels"));
+    return UC_OK;
+}
+
+int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_samples, long long n_series,
+                   double time_step, int max_peaks, int* counts, double* peaks, double* intervals, void* stream) {
+    if (!e || !s) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (!data || !counts || n_samples < 0 || n_series <= 0 || max_peaks < 0 || max_peaks > 2500)
+        return e->fail(UC_ERR_ARGUMENT, "peak finder: bad arguments");
+    if (max_peaks > 0 && (!peaks || !intervals)) return e->fail(UC_ERR_ARGUMENT, "peak finder: outputs missing");
+    int rc = enter(e);
+    if (rc) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, true, true, &m))) return rc;
+    hipFunction_t f;
+    hipError_t he = hipModuleGetFunction(&f, m->mod, "uc_peaks_series");
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "uc_peaks_series"));
+    const unsigned blocks = (unsigned)((n_series + kBlock - 1) / kBlock);
+    void* args[] = {(void*)&data, &n_samples, &n_series, &time_step, &max_peaks, &counts, &peaks, &intervals};
+    he = hipModuleLaunchKernel(f, blocks, 1, 1, kBlock, 1, 1, 0, (hipStream_t)stream, args, nullptr);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLaunchKernel(peaks)"));
+    return UC_OK;
 }
 
 }  // extern "C"

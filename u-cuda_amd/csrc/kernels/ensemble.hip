@@ -253,13 +253,23 @@ __device__ __forceinline__ void uc_rk4_tan(StateF32C& s, float* V, const float* 
 
 // --------------------------------------------------- precision adaptors ---
 template <class S> struct Prec;
+// Fixed-point threshold of the trailing-step test: the reference's 1e-8
+// (configCUDA.h:84) in double. With float stages the state cannot settle
+// closer to an equilibrium than the float grid allows: one step there still
+// moves it by about h*|J|*eps_f32*|x|, so the threshold is raised to
+// 2^-16 * h * pre * sum|x| (128 float ulps of the state per unit |J|*h),
+// far below the per-step motion of any oscillating orbit.
 template <> struct Prec<StateF64> {
     typedef double T;
     static __device__ __forceinline__ double to_t(double v) { return v; }
+    static __device__ __forceinline__ double fixed_point_eps(double, int, double) { return UC_EPS_FIXED_POINT; }
 };
 template <> struct Prec<StateF32C> {
     typedef float T;
     static __device__ __forceinline__ float to_t(double v) { return (float)v; }
+    static __device__ __forceinline__ double fixed_point_eps(double h, int pre, double sum_abs) {
+        return fmax(UC_EPS_FIXED_POINT, 1.52587890625e-05 * h * (double)pre * sum_abs);
+    }
 };
 
 // ------------------------------------------------------------ step loop ---
@@ -283,11 +293,11 @@ __device__ __forceinline__ bool uc_diverged(const S& s, double max_value) {
 }
 
 template <class S>
-__device__ __forceinline__ int uc_fixed_point_flag(const S& s, const double* prev) {
-    double d = 0.0;
+__device__ __forceinline__ int uc_fixed_point_flag(const S& s, const double* prev, double h, int pre) {
+    double d = 0.0, m = 0.0;
 #pragma unroll
-    for (int i = 0; i < UC_N; ++i) d = d + fabs(s.get(i) - prev[i]);
-    return fabs(d) < UC_EPS_FIXED_POINT ? -1 : 1;
+    for (int i = 0; i < UC_N; ++i) { d = d + fabs(s.get(i) - prev[i]); m = m + fabs(prev[i]); }
+    return fabs(d) < Prec<S>::fixed_point_eps(h, pre, m) ? -1 : 1;
 }
 
 // Plain state loop.
@@ -322,7 +332,7 @@ __device__ int uc_loop(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
     for (int p = 0; p < pre; ++p) uc_rk4(s, a, h, (typename Prec<S>::T)0);
-    return uc_fixed_point_flag(s, prev);
+    return uc_fixed_point_flag(s, prev, (double)h, pre);
 }
 
 // State + K tangent vectors advanced together for `iters` steps plus the
@@ -344,7 +354,7 @@ __device__ int uc_loop_tan(S& s, typename Prec<S>::T* V, const typename Prec<S>:
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
     uc_rk4_tan<K>(s, V, a, h, (typename Prec<S>::T)0);
-    return uc_fixed_point_flag(s, prev);
+    return uc_fixed_point_flag(s, prev, (double)h, 1);
 }
 
 // ------------------------------------------------------ streaming peaks ---
@@ -664,9 +674,357 @@ __device__ void uc_lyap_body(const UcJob& j) {
     }
 }
 
+// The same streaming finder over stored series: data[i * n_series + s].
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_peaks_series(const double* data, long long n_samples,
+                                                                      long long n_series, double time_step,
+                                                                      int max_peaks, int* counts, double* peaks,
+                                                                      double* intervals) {
+    const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= n_series) return;
+    PeakObserver obs;
+    obs.w0 = 0.0; obs.w1 = 0.0; obs.n = n_samples; obs.c_next = 2; obs.mode = 0; obs.raw = 0;
+    obs.last_t = 0.0; obs.time_step = time_step; obs.max_peaks = max_peaks; obs.stride = n_series;
+    obs.peaks = peaks + lane; obs.intervals = intervals + lane; obs.traj = nullptr; obs.traj_len = 0;
+    for (long long i = 0; i < n_samples; ++i) obs.sample(i, data[i * n_series + lane]);
+    counts[lane] = obs.count();
+}
+
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_f64(UcJob j) { uc_bif_body<StateF64>(j); }
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_f32c(UcJob j) { uc_bif_body<StateF32C>(j); }
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_lle_f64(UcJob j) { uc_lyap_body<1, StateF64>(j); }
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_lle_f32c(UcJob j) { uc_lyap_body<1, StateF32C>(j); }
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_ls_f64(UcJob j) { uc_lyap_body<UC_N, StateF64>(j); }
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_ls_f32c(UcJob j) { uc_lyap_body<UC_N, StateF32C>(j); }
+
+// ================================================================ basins ==
+// Adaptive Dormand-Prince 5(4) (Dormand & Prince 1980 coefficients; Hairer's
+// RMS error norm and step controller, FSAL) with every step clipped to land
+// on the Poincare sections t = k*T. The system time seen by the right-hand
+// side is local to the current section period (t in [0, T]), exact for
+// T-periodic forcing. The last UC_REC section points of the first
+// UC_LBL state variables are kept in a shift register; afterwards the
+// smallest period p <= max_period whose points close within period_tol is
+// found and the cycle point with the smallest (x0, x1) becomes the cell's
+// canonical point (no cycle: p = 0 and the mean point; unbound: p = -1;
+// step-size collapse or attempt cap: p = -2).
+#define UC_REC 16
+#define UC_MAXP 8
+#define UC_LBL (UC_N < 2 ? UC_N : 2)
+
+#define DPA21 (1.0 / 5.0)
+#define DPA31 (3.0 / 40.0)
+#define DPA32 (9.0 / 40.0)
+#define DPA41 (44.0 / 45.0)
+#define DPA42 (-56.0 / 15.0)
+#define DPA43 (32.0 / 9.0)
+#define DPA51 (19372.0 / 6561.0)
+#define DPA52 (-25360.0 / 2187.0)
+#define DPA53 (64448.0 / 6561.0)
+#define DPA54 (-212.0 / 729.0)
+#define DPA61 (9017.0 / 3168.0)
+#define DPA62 (-355.0 / 33.0)
+#define DPA63 (46732.0 / 5247.0)
+#define DPA64 (49.0 / 176.0)
+#define DPA65 (-5103.0 / 18656.0)
+#define DPB1 (35.0 / 384.0)
+#define DPB3 (500.0 / 1113.0)
+#define DPB4 (125.0 / 192.0)
+#define DPB5 (-2187.0 / 6784.0)
+#define DPB6 (11.0 / 84.0)
+#define DPE1 (71.0 / 57600.0)
+#define DPE3 (-71.0 / 16695.0)
+#define DPE4 (71.0 / 1920.0)
+#define DPE5 (-17253.0 / 339200.0)
+#define DPE6 (22.0 / 525.0)
+#define DPE7 (-1.0 / 40.0)
+
+// One attempt from (t, x) with step h, k1 = f(t, x) on entry. Produces the
+// 5th-order increment dx (x5 = x + dx), k7 = f(t + h, x5) and the error norm.
+// The stage sums follow the tableau row order left to right, like the oracle.
+template <typename T>
+__device__ __forceinline__ double uc_dp5(const T* x, const T* a, T t, T h, const T* k1, T* dx, T* k7, double rtol,
+                                         double atol) {
+    T k2[UC_N], k3[UC_N], k4[UC_N], k5[UC_N], k6[UC_N], xs[UC_N];
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) xs[i] = x[i] + h * ((T)DPA21 * k1[i]);
+    uc_rhs<T>(xs, a, t + (T)(1.0 / 5.0) * h, k2);
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) xs[i] = x[i] + h * ((T)DPA31 * k1[i] + (T)DPA32 * k2[i]);
+    uc_rhs<T>(xs, a, t + (T)(3.0 / 10.0) * h, k3);
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) xs[i] = x[i] + h * ((T)DPA41 * k1[i] + (T)DPA42 * k2[i] + (T)DPA43 * k3[i]);
+    uc_rhs<T>(xs, a, t + (T)(4.0 / 5.0) * h, k4);
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i)
+        xs[i] = x[i] + h * ((T)DPA51 * k1[i] + (T)DPA52 * k2[i] + (T)DPA53 * k3[i] + (T)DPA54 * k4[i]);
+    uc_rhs<T>(xs, a, t + (T)(8.0 / 9.0) * h, k5);
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i)
+        xs[i] = x[i] + h * ((T)DPA61 * k1[i] + (T)DPA62 * k2[i] + (T)DPA63 * k3[i] + (T)DPA64 * k4[i] +
+                            (T)DPA65 * k5[i]);
+    uc_rhs<T>(xs, a, t + h, k6);
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) {
+        dx[i] = h * ((T)DPB1 * k1[i] + (T)0.0 * k2[i] + (T)DPB3 * k3[i] + (T)DPB4 * k4[i] + (T)DPB5 * k5[i] +
+                     (T)DPB6 * k6[i]);
+        xs[i] = x[i] + dx[i];
+    }
+    uc_rhs<T>(xs, a, t + h, k7);
+    double err = 0.0;
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) {
+        const T e = h * ((T)DPE1 * k1[i] + (T)0.0 * k2[i] + (T)DPE3 * k3[i] + (T)DPE4 * k4[i] + (T)DPE5 * k5[i] +
+                         (T)DPE6 * k6[i] + (T)DPE7 * k7[i]);
+        const double sc = atol + rtol * fmax(fabs((double)x[i]), fabs((double)xs[i]));
+        const double q = (double)e / sc;
+        err = err + q * q;
+    }
+    return sqrt(err / (double)UC_N);
+}
+
+struct BasF64 {
+    double x[UC_N];
+    double t;
+    __device__ __forceinline__ void load(const double* x0) {
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) x[i] = x0[i];
+    }
+    __device__ __forceinline__ double get(int i) const { return x[i]; }
+    __device__ __forceinline__ void reset_time() { t = 0.0; }
+    __device__ __forceinline__ double time() const { return t; }
+    __device__ __forceinline__ double remaining(double T) const { return T - t; }
+    __device__ __forceinline__ void advance_time(double hs) { t = t + hs; }
+};
+
+struct BasF32C {
+    float hi[UC_N], c[UC_N];
+    float thi, tc;  // compensated local time
+    __device__ __forceinline__ void load(const double* x0) {
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) { hi[i] = (float)x0[i]; c[i] = (float)((double)hi[i] - x0[i]); }
+    }
+    __device__ __forceinline__ double get(int i) const { return (double)hi[i] - (double)c[i]; }
+    __device__ __forceinline__ void reset_time() { thi = 0.0f; tc = 0.0f; }
+    __device__ __forceinline__ double time() const { return (double)thi - (double)tc; }
+    __device__ __forceinline__ double remaining(double T) const { return T - time(); }
+    __device__ __forceinline__ void advance_time(double hs) { uc_kahan_add(thi, tc, (float)hs); }
+};
+
+// One accepted-or-rejected attempt for each state policy. Returns the error
+// norm; on acceptance (err <= 1) the state moves and k1 <- k7 (FSAL).
+__device__ __forceinline__ double uc_bas_attempt(BasF64& s, const double* a, double hs, double* k1, double rtol,
+                                                 double atol) {
+    double dx[UC_N], k7[UC_N];
+    const double err = uc_dp5<double>(s.x, a, s.t, hs, k1, dx, k7, rtol, atol);
+    if (err <= 1.0) {
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) { s.x[i] = s.x[i] + dx[i]; k1[i] = k7[i]; }
+    }
+    return err;
+}
+__device__ __forceinline__ double uc_bas_attempt(BasF32C& s, const float* a, double hs, float* k1, double rtol,
+                                                 double atol) {
+    float dx[UC_N], k7[UC_N];
+    const double err = uc_dp5<float>(s.hi, a, s.thi, (float)hs, k1, dx, k7, rtol, atol);
+    if (err <= 1.0) {
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) { uc_kahan_add(s.hi[i], s.c[i], dx[i]); k1[i] = k7[i]; }
+    }
+    return err;
+}
+__device__ __forceinline__ void uc_bas_k1(BasF64& s, const double* a, double* k1) { uc_rhs<double>(s.x, a, s.t, k1); }
+__device__ __forceinline__ void uc_bas_k1(BasF32C& s, const float* a, float* k1) { uc_rhs<float>(s.hi, a, s.thi, k1); }
+
+template <class S> struct BasPrec;
+template <> struct BasPrec<BasF64> {
+    typedef double T;
+    static __device__ __forceinline__ double tol(double v) { return v; }
+};
+// float stages cannot resolve errors below ~1e-6 of the state
+template <> struct BasPrec<BasF32C> {
+    typedef float T;
+    static __device__ __forceinline__ double tol(double v) { return fmax(v, 9.5367431640625e-07); }
+};
+
+template <class S>
+__device__ void uc_basins_body(const UcJob& j) {
+    typedef typename BasPrec<S>::T T;
+    const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= j.n_cells) return;
+    const long long cell = j.cell_offset + lane;
+    double ad[UC_M], x0[UC_N], hd;
+    uc_setup(j, cell, ad, x0, hd);
+    T a[UC_M];
+#pragma unroll
+    for (int i = 0; i < UC_M; ++i) a[i] = (T)ad[i];
+    const double rtol = BasPrec<S>::tol(j.rtol), atol = BasPrec<S>::tol(j.atol);
+    const double T_sec = j.section_period;
+    S s;
+    s.load(x0);
+    double h = hd;
+    double buf[UC_REC][UC_LBL];
+#pragma unroll
+    for (int r = 0; r < UC_REC; ++r)
+#pragma unroll
+        for (int i = 0; i < UC_LBL; ++i) buf[r][i] = 0.0;
+    long long acc = 0, rej = 0;
+    int status = 0;  // 0 running, -1 unbound, -2 unresolved
+    T k1[UC_N];
+    for (int sec = 0; sec < j.n_sections && status == 0; ++sec) {
+        s.reset_time();
+        uc_bas_k1(s, a, k1);
+        for (;;) {
+            if (acc + rej >= j.max_attempts) { status = -2; break; }
+            const double rem = s.remaining(T_sec);
+            const bool last = h >= rem;
+            const double hs = last ? rem : h;
+            const double err = uc_bas_attempt(s, a, hs, k1, rtol, atol);
+            double fac = err == 0.0 ? 5.0 : 0.9 * pow(err, -0.2);
+            fac = fac < 0.2 ? 0.2 : (fac > 5.0 ? 5.0 : fac);
+            if (err <= 1.0) {
+                ++acc;
+                if (uc_diverged(s, j.max_value)) { status = -1; break; }
+                if (last) {
+                    // keep the unclipped step for the next section
+                    const double hn = hs * fac;
+                    if (hn > h) h = hn;
+                    if (h > j.h_max) h = j.h_max;
+                    break;
+                }
+                s.advance_time(hs);
+                h = hs * fac;
+            } else {
+                ++rej;
+                h = hs * fac;
+            }
+            if (h > j.h_max) h = j.h_max;
+            if (h < j.h_min) { status = -2; break; }
+        }
+        if (status != 0) break;
+#pragma unroll
+        for (int r = UC_REC - 1; r > 0; --r)
+#pragma unroll
+            for (int i = 0; i < UC_LBL; ++i) buf[r][i] = buf[r - 1][i];
+#pragma unroll
+        for (int i = 0; i < UC_LBL; ++i) buf[0][i] = s.get(i);
+    }
+    int p = status;
+    double pt[UC_LBL];
+#pragma unroll
+    for (int i = 0; i < UC_LBL; ++i) pt[i] = 0.0;
+    if (status == 0) {
+        const int L = j.n_record < UC_REC ? j.n_record : UC_REC;
+        p = 0;
+#pragma unroll
+        for (int q = 1; q <= UC_MAXP; ++q) {
+            if (p != 0 || q > j.max_period || q >= L) continue;
+            bool closes = true;
+#pragma unroll
+            for (int r = 0; r + q < UC_REC; ++r) {
+                if (r + q >= L) continue;
+#pragma unroll
+                for (int i = 0; i < UC_LBL; ++i)
+                    closes = closes && fabs(buf[r][i] - buf[r + q][i]) <= j.period_tol * (1.0 + fabs(buf[r][i]));
+            }
+            if (closes) p = q;
+        }
+        if (p > 0) {
+#pragma unroll
+            for (int i = 0; i < UC_LBL; ++i) pt[i] = buf[0][i];
+#pragma unroll
+            for (int r = 1; r < UC_MAXP; ++r) {
+                if (r >= p) continue;
+                const bool less = buf[r][0] < pt[0] || (UC_LBL > 1 && buf[r][0] == pt[0] && buf[r][UC_LBL - 1] < pt[UC_LBL - 1]);
+                if (less) {
+#pragma unroll
+                    for (int i = 0; i < UC_LBL; ++i) pt[i] = buf[r][i];
+                }
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < UC_REC; ++r) {
+                if (r >= L) continue;
+#pragma unroll
+                for (int i = 0; i < UC_LBL; ++i) pt[i] = pt[i] + buf[r][i];
+            }
+#pragma unroll
+            for (int i = 0; i < UC_LBL; ++i) pt[i] = pt[i] / (double)L;
+        }
+    }
+    j.flag[lane] = p;
+#pragma unroll
+    for (int i = 0; i < UC_LBL; ++i) j.out0[(long long)i * j.n_cells + lane] = pt[i];
+    if (j.counters) {
+        j.counters[lane] = acc;
+        j.counters[j.n_cells + lane] = rej;
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bas_f64(UcJob j) { uc_basins_body<BasF64>(j); }
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bas_f32c(UcJob j) { uc_basins_body<BasF32C>(j); }
+
+// ---- attractor catalogue: quantised signature keys in an open-addressing
+// hash set (linear probing, bounded by the table size), then a per-cell
+// lookup of the label the host assigned to each distinct key.
+#define UC_KEY_EMPTY 0xFFFFFFFFFFFFFFFFULL
+
+__device__ __forceinline__ u64 uc_sig_key(int p, const double* pt, double quantum) {
+    // p in [0, 255]; two signed 28-bit quantised coordinates
+    long long qx = 0, qv = 0;
+    if (p > 0) {
+        qx = (long long)floor(pt[0] / quantum);
+        qv = UC_LBL > 1 ? (long long)floor(pt[UC_LBL - 1] / quantum) : 0;
+    }
+    const u64 mask = (1ULL << 28) - 1ULL;
+    return ((u64)(p & 0xFF) << 56) | (((u64)qx & mask) << 28) | ((u64)qv & mask);
+}
+
+__device__ __forceinline__ u64 uc_hash(u64 k) {
+    k ^= k >> 33; k *= 0xff51afd7ed558ccdULL; k ^= k >> 33; k *= 0xc4ceb9fe1a85ec53ULL; k ^= k >> 33;
+    return k;
+}
+
+// job fields used: n_cells, flag (periods), out0 (points), quantum;
+// table: `cap` slots (a power of two); keys that find no slot count in *overflow.
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bas_keys(UcJob j, u64* table, int cap, u64* overflow) {
+    const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= j.n_cells) return;
+    const int p = j.flag[lane];
+    if (p < 0) return;
+    double pt[UC_LBL];
+#pragma unroll
+    for (int i = 0; i < UC_LBL; ++i) pt[i] = j.out0[(long long)i * j.n_cells + lane];
+    const u64 key = uc_sig_key(p, pt, j.quantum);
+    unsigned slot = (unsigned)(uc_hash(key) & (u64)(cap - 1));
+    for (int probe = 0; probe < cap; ++probe) {
+        const u64 cur = __atomic_load_n(&table[slot], __ATOMIC_RELAXED);
+        if (cur == key) return;
+        if (cur == UC_KEY_EMPTY) {
+            const u64 old = atomicCAS(&table[slot], UC_KEY_EMPTY, key);
+            if (old == UC_KEY_EMPTY || old == key) return;
+        }
+        slot = (slot + 1) & (unsigned)(cap - 1);
+    }
+    atomicAdd((unsigned long long*)overflow, 1ULL);
+}
+
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bas_labels(UcJob j, const u64* table, const int* slot_label,
+                                                                    int cap) {
+    const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= j.n_cells) return;
+    const int p = j.flag[lane];
+    if (p < 0) { j.label[lane] = UC_LABEL_UNBOUND; return; }
+    double pt[UC_LBL];
+#pragma unroll
+    for (int i = 0; i < UC_LBL; ++i) pt[i] = j.out0[(long long)i * j.n_cells + lane];
+    const u64 key = uc_sig_key(p, pt, j.quantum);
+    unsigned slot = (unsigned)(uc_hash(key) & (u64)(cap - 1));
+    int label = UC_LABEL_UNBOUND;
+    for (int probe = 0; probe < cap; ++probe) {
+        const u64 cur = table[slot];
+        if (cur == key) { label = slot_label[slot]; break; }
+        if (cur == UC_KEY_EMPTY) break;
+        slot = (slot + 1) & (unsigned)(cap - 1);
+    }
+    j.label[lane] = label;
+}

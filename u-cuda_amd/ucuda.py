@@ -77,7 +77,8 @@ class BasinsRequest(C.Structure):
 
 class BasinsOutput(C.Structure):
     _fields_ = [("labels", C.c_void_p), ("periods", C.c_void_p), ("cycle_point", C.c_void_p),
-                ("steps", C.c_void_p)]
+                ("steps", C.c_void_p), ("max_attractors", C.c_int), ("n_attractors", C.POINTER(C.c_int)),
+                ("attractor_period", C.POINTER(C.c_int)), ("attractor_point", C.POINTER(C.c_double))]
 
 
 # Every symbol include/ucuda.h declares, with its ctypes signature.
@@ -102,6 +103,8 @@ _SIGNATURES = {
     "uc_run_ls": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.POINTER(LyapOutput), C.c_void_p]),
     "uc_run_basins": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BasinsRequest), C.POINTER(BasinsOutput),
                                 C.c_void_p]),
+    "uc_peak_finder": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong, C.c_double,
+                                 C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "uc_grid_cells": (C.c_longlong, [C.POINTER(Common)]),
     "uc_axis_values": (C.c_int, [C.POINTER(Axis), C.POINTER(C.c_double)]),
 }
@@ -373,6 +376,58 @@ def run_lyap(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=Fals
         torch.cuda.synchronize(dev)
     del keep
     return out
+
+
+def run_basins(sysm, axes, ic, values, h0=0.01, rtol=1e-9, atol=1e-9, section_period=2 * math.pi,
+               n_sections=200, n_record=16, max_period=8, period_tol=1e-4, quantum=1e-3, h_min=1e-12,
+               h_max=None, max_attempts=0, max_value=1e6, precision=PREC_F64, strict=False,
+               want_points=True, want_steps=True, max_attractors=64, stream=None):
+    """Basins of attraction: labels[cells] (1..K canonical, 0 unbound), periods,
+    canonical points [2, cells], step counts [2, cells], attractor catalogue."""
+    import torch
+    eng = sysm.engine
+    n = n_cells(axes)
+    dev = torch.device("cuda", eng.device)
+    lbl = min(sysm.n_vars, 2)
+    out = dict(labels=torch.empty(n, dtype=torch.int32, device=dev),
+               periods=torch.empty(n, dtype=torch.int32, device=dev),
+               points=torch.empty((lbl, n), dtype=torch.float64, device=dev) if want_points else None,
+               steps=torch.empty((2, n), dtype=torch.int64, device=dev) if want_steps else None)
+    r = BasinsRequest()
+    r.c, keep = _common(sysm, axes, ic, values, h0, 0.0, 1.0, max_value, precision, strict)
+    r.rtol, r.atol, r.h_min = float(rtol), float(atol), float(h_min)
+    r.h_max = float(h_max) if h_max is not None else float(section_period) / 4.0
+    r.section_period, r.n_sections, r.n_record = float(section_period), int(n_sections), int(n_record)
+    r.max_period, r.label_vars, r.period_tol, r.quantum = int(max_period), lbl, float(period_tol), float(quantum)
+    r.max_attempts = int(max_attempts)
+    na = C.c_int(0)
+    ap = (C.c_int * max_attractors)()
+    apt = (C.c_double * (2 * max_attractors))()
+    o = BasinsOutput(_ptr(out["labels"]).value, _ptr(out["periods"]).value, _ptr(out["points"]).value,
+                     _ptr(out["steps"]).value, max_attractors, C.pointer(na), ap, apt)
+    eng.check(lib().uc_run_basins(eng.h, sysm.h, C.byref(r), C.byref(o), _stream(stream)), "uc_run_basins")
+    torch.cuda.synchronize(dev)
+    k = min(na.value, max_attractors)
+    out["attractors"] = [dict(label=i + 1, period=ap[i], point=(apt[2 * i], apt[2 * i + 1])) for i in range(k)]
+    del keep
+    return out
+
+
+def peak_finder(sysm, data, time_step, max_peaks=2500, stream=None):
+    """Streaming peak finder over stored series: data is a [n_samples, n_series]
+    float64 CUDA tensor. Returns counts, peaks[max_peaks, n_series], intervals."""
+    import torch
+    eng = sysm.engine
+    ns, nser = data.shape
+    dev = data.device
+    counts = torch.empty(nser, dtype=torch.int32, device=dev)
+    peaks = torch.full((max(1, max_peaks), nser), float("nan"), dtype=torch.float64, device=dev)
+    intervals = torch.full((max(1, max_peaks), nser), float("nan"), dtype=torch.float64, device=dev)
+    data = data.contiguous()
+    eng.check(lib().uc_peak_finder(eng.h, sysm.h, _ptr(data), ns, nser, float(time_step), max_peaks, _ptr(counts),
+                                   _ptr(peaks), _ptr(intervals), _stream(stream)), "uc_peak_finder")
+    torch.cuda.synchronize(dev)
+    return dict(counts=counts, peaks=peaks, intervals=intervals)
 
 
 def rk4_flops(ops, n, tangents=0):
