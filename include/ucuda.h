@@ -198,6 +198,27 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
 int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_samples, long long n_series,
                    double time_step, int max_peaks, int* counts, double* peaks, double* intervals, void* stream);
 
+/* ---- persistent heterogeneous batch ----
+ * Jobs of any size, system and analysis, added with the same request /
+ * output structs as the uc_run_* calls; jobs that share a compiled kernel
+ * run as ONE persistent launch over a job table (workgroups take chunks of
+ * cells from a global counter, longest jobs first), and all such launches
+ * are captured into one hipGraph that uc_batch_launch replays. The modules
+ * stay cached in the engine while the batch lives (LRU capacity 8). */
+typedef struct uc_batch uc_batch;
+int uc_batch_create(uc_engine* e, uc_batch** out);
+void uc_batch_destroy(uc_batch* b);
+int uc_batch_add_bif(uc_batch* b, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, int* job_id);
+int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int spectrum, const uc_lyap_output* o,
+                      int* job_id);
+/* launch shape: block (64/128/256 lanes) and resident waves per SIMD (1..8) */
+int uc_batch_set_launch(uc_batch* b, int block, int waves_per_simd);
+/* autotuner: times the batch for each candidate shape, keeps the fastest */
+int uc_batch_tune(uc_batch* b, void* stream, int* best_block, int* best_waves_per_simd, double* best_ms);
+/* one graph launch of the whole batch on a non-default stream */
+int uc_batch_launch(uc_batch* b, void* stream);
+int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int* waves_per_simd);
+
 /* Number of cells of a request's grid (product of axis point counts). */
 long long uc_grid_cells(const uc_common* c);
 /* Axis values exactly as the kernels use them (host, n_pts doubles). */

@@ -232,52 +232,63 @@ void oc_mgs(double* v, int K, int n, double* d) {
  * Classic RK4 in the stage order of the reference KRS body
  * (U-CUDA/codegen.cpp:597-606): X1 = X + 0.5*h*k (stages 0, 1), X1 = X + h*k
  * (stage 2), X += h*(k0 + 2*k1 + 2*k2 + k3)/6, summed left to right. */
-void oc_rk4(oc_rhs_fn rhs, double* x, const double* a, double h, int n) {
+static void rk4_at(oc_rhs_fn rhs, double* x, const double* a, double h, double t, int n) {
     double x1[OC_MAXN], k[OC_MAXN], acc[OC_MAXN];
     const double hh = 0.5 * h;
-    rhs(x, a, 0.0, k);
+    rhs(x, a, t, k);
     for (int i = 0; i < n; ++i) { acc[i] = k[i]; x1[i] = x[i] + hh * k[i]; }
-    rhs(x1, a, hh, k);
+    rhs(x1, a, t + hh, k);
     for (int i = 0; i < n; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = x[i] + hh * k[i]; }
-    rhs(x1, a, hh, k);
+    rhs(x1, a, t + hh, k);
     for (int i = 0; i < n; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = x[i] + h * k[i]; }
-    rhs(x1, a, h, k);
+    rhs(x1, a, t + h, k);
     for (int i = 0; i < n; ++i) { acc[i] = acc[i] + k[i]; x[i] = x[i] + h * acc[i] / 6.0; }
 }
 
+void oc_rk4(oc_rhs_fn rhs, double* x, const double* a, double h, int n) { rk4_at(rhs, x, a, h, 0.0, n); }
+
 /* RK4 applied to the variational system V' = J(X) V with the same stages
  * (the exact derivative of the discrete RK4 map). */
-void oc_rk4_tan(oc_rhs_fn rhs, oc_tan_fn tan, double* x, double* V, int K, const double* a, double h, int n) {
+static void rk4_tan_at(oc_rhs_fn rhs, oc_tan_fn tan, double* x, double* V, int K, const double* a, double h, double t,
+                       int n) {
     double x1[OC_MAXN], k[OC_MAXN], acc[OC_MAXN];
     double v1[OC_MAXN * OC_MAXN], l[OC_MAXN * OC_MAXN], lacc[OC_MAXN * OC_MAXN];
     const double hh = 0.5 * h;
     const int m = K * n;
-    rhs(x, a, 0.0, k);
-    tan(x, a, 0.0, V, l, K);
+    rhs(x, a, t, k);
+    tan(x, a, t, V, l, K);
     for (int i = 0; i < n; ++i) { acc[i] = k[i]; x1[i] = x[i] + hh * k[i]; }
     for (int i = 0; i < m; ++i) { lacc[i] = l[i]; v1[i] = V[i] + hh * l[i]; }
-    rhs(x1, a, hh, k);
-    tan(x1, a, hh, v1, l, K);
+    rhs(x1, a, t + hh, k);
+    tan(x1, a, t + hh, v1, l, K);
     for (int i = 0; i < n; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = x[i] + hh * k[i]; }
     for (int i = 0; i < m; ++i) { lacc[i] = lacc[i] + 2.0 * l[i]; v1[i] = V[i] + hh * l[i]; }
-    rhs(x1, a, hh, k);
-    tan(x1, a, hh, v1, l, K);
+    rhs(x1, a, t + hh, k);
+    tan(x1, a, t + hh, v1, l, K);
     for (int i = 0; i < n; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = x[i] + h * k[i]; }
     for (int i = 0; i < m; ++i) { lacc[i] = lacc[i] + 2.0 * l[i]; v1[i] = V[i] + h * l[i]; }
-    rhs(x1, a, h, k);
-    tan(x1, a, h, v1, l, K);
+    rhs(x1, a, t + h, k);
+    tan(x1, a, t + h, v1, l, K);
     for (int i = 0; i < n; ++i) { acc[i] = acc[i] + k[i]; x[i] = x[i] + h * acc[i] / 6.0; }
     for (int i = 0; i < m; ++i) { lacc[i] = lacc[i] + l[i]; V[i] = V[i] + h * lacc[i] / 6.0; }
 }
 
-typedef struct { oc_rhs_fn rhs; oc_tan_fn tan; double* V; int K; int n; } tan_ctx;
+void oc_rk4_tan(oc_rhs_fn rhs, oc_tan_fn tan, double* x, double* V, int K, const double* a, double h, int n) {
+    rk4_tan_at(rhs, tan, x, V, K, a, h, 0.0, n);
+}
+
+/* right-hand-side stepping with the system time t advanced by h per step
+ * (t is read only by systems with a time symbol) */
+typedef struct { oc_rhs_fn rhs; oc_tan_fn tan; double* V; int K; int n; double t; } tan_ctx;
 static void adv_rhs(void* ctx, double* x, const double* a, double h) {
     tan_ctx* c = (tan_ctx*)ctx;
-    oc_rk4(c->rhs, x, a, h, c->n);
+    rk4_at(c->rhs, x, a, h, c->t, c->n);
+    c->t += h;
 }
 static void adv_tan(void* ctx, double* x, const double* a, double h) {
     tan_ctx* c = (tan_ctx*)ctx;
-    oc_rk4_tan(c->rhs, c->tan, x, c->V, c->K, a, h, c->n);
+    rk4_tan_at(c->rhs, c->tan, x, c->V, c->K, a, h, c->t, c->n);
+    c->t += h;
 }
 
 /* Random unit probes (LLEKernelCUDA :2603-2626, LSKernelCUDA :2980-2997):
@@ -458,7 +469,7 @@ static void lyap_var_cell(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, 
     long long skip, nts;
     lyap_counts(j, h, &skip, &nts);
     const long long blocks = (long long)(j->t_max / j->NT);
-    tan_ctx ctx = {rhs, tan, V, K, n};
+    tan_ctx ctx = {rhs, tan, V, K, n, 0.0};
     int alive = loop_generic(adv_rhs, &ctx, x, a, h, skip, n, 1, 0, j->max_value, NULL) != 0;
     for (int k = 0; k < K; ++k) sum[k] = 0.0;
     if (alive) {

@@ -1,0 +1,95 @@
+"""Persistent heterogeneous batch: every job's result from the captured-graph
+batch launch is bit-identical to the same job run alone (same kernel code
+per cell, placement does not matter), across systems, analyses, precisions
+and grid sizes; graph replays reproduce the first launch."""
+import numpy as np
+import pytest
+import torch
+
+import ucuda
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(eng):
+    lor, ros, chen, hr, duf = (ucuda.preset(n, eng) for n in ("lorenz", "rossler", "chen", "hyper_rossler", "duffing"))
+    J = []
+    J.append(("lyap", ros, dict(axes=[ucuda.axis(2, 2, 37, 0.05, 2.0), ucuda.axis(2, 3, 29, 1.0, 20.0)],
+                                ic=[1, 1, 1], values=ros.values(), h=0.01, transient_time=20.0, t_max=60.0, NT=0.1,
+                                precision=ucuda.PREC_F32C)))
+    J.append(("lyap", lor, dict(axes=[ucuda.axis(1, 2, 300, 20.0, 200.0)], ic=[0.1, 0.1, 0.1], values=lor.values(),
+                                h=0.01, transient_time=5.0, t_max=30.0, NT=0.1)))
+    J.append(("lyap", hr, dict(axes=[ucuda.axis(1, 2, 12, 0.0, 0.1), ucuda.axis(1, 1, 11, 0.2, 0.3)], ic=None,
+                               values=hr.values(), h=0.005, transient_time=2.0, t_max=20.0, NT=0.1, spectrum=True,
+                               ic_seed=20261203)))
+    J.append(("lyap", duf, dict(axes=[ucuda.axis(2, 0, 33, -2.5, 2.5), ucuda.axis(2, 1, 17, -2.5, 2.5)],
+                                ic=[0.0, 0.0], values=duf.values(), h=0.01, transient_time=10.0, t_max=40.0, NT=0.1)))
+    J.append(("bif", chen, dict(axes=[ucuda.axis(1, 3, 500, 20.0, 30.0)], ic=[0.1, 0.1, 0.1], values=chen.values(),
+                                h=0.0025, transient_time=5.0, t_max=10.0, max_peaks=32)))
+    J.append(("bif", lor, dict(axes=[ucuda.axis(1, 2, 7, 100.0, 200.0)], ic=[0.1, 0.1, 0.1], values=lor.values(),
+                               h=0.01, transient_time=20.0, t_max=20.0, max_peaks=32, precision=ucuda.PREC_F32C)))
+    return J
+
+
+def _alone(kind, sysm, kw):
+    kw = dict(kw)
+    if kind == "lyap":
+        return ucuda.run_lyap(sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), kw.pop("h"),
+                              kw.pop("transient_time"), kw.pop("t_max"), kw.pop("NT"), **kw)
+    return ucuda.run_bif(sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), kw.pop("h"), kw.pop("transient_time"),
+                         kw.pop("t_max"), **kw)
+
+
+def test_batch_equals_alone_and_replays(gpu_engine):
+    print("case: 6 heterogeneous jobs (5 systems, lle/ls/bif, f64/f32c) in one graph launch", flush=True)
+    jobs = _jobs(gpu_engine)
+    b = ucuda.Batch(gpu_engine)
+    outs = []
+    for kind, sysm, kw in jobs:
+        kw2 = dict(kw)
+        if kind == "lyap":
+            outs.append(b.add_lyap(sysm, kw2.pop("axes"), kw2.pop("ic"), kw2.pop("values"), kw2.pop("h"),
+                                   kw2.pop("transient_time"), kw2.pop("t_max"), kw2.pop("NT"), **kw2))
+        else:
+            outs.append(b.add_bif(sysm, kw2.pop("axes"), kw2.pop("ic"), kw2.pop("values"), kw2.pop("h"),
+                                  kw2.pop("transient_time"), kw2.pop("t_max"), **kw2))
+    info = b.info()
+    assert info["n_jobs"] == 6 and info["n_groups"] >= 5
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        b.launch(stream)
+    first = [{k: v.clone() for k, v in o.items()} for o in outs]
+    for (kind, sysm, kw), o in zip(jobs, outs):
+        ref = _alone(kind, sysm, kw)
+        for key in o:
+            assert torch.equal(o[key].cpu(), ref[key].cpu()), (kind, sysm.name, key)
+    # replays with eager work in between reproduce the first launch
+    for _ in range(3):
+        for o in outs:
+            for v in o.values():
+                v.fill_(-7)
+        with torch.cuda.stream(stream):
+            b.launch(stream)
+        torch.cuda.synchronize()
+        for o, f in zip(outs, first):
+            for key in o:
+                assert torch.equal(o[key], f[key]), key
+
+
+def test_batch_tuner_and_shapes(gpu_engine):
+    print("case: tuner over launch shapes; every shape gives identical results", flush=True)
+    ros = ucuda.preset("rossler", gpu_engine)
+    b = ucuda.Batch(gpu_engine)
+    outs = [b.add_lyap(ros, [ucuda.axis(2, 2, 64, 0.05, 2.0), ucuda.axis(2, 3, n, 1.0, 20.0)], [1, 1, 1],
+                       ros.values(), 0.01, 5.0, 20.0, 0.1, precision=ucuda.PREC_F32C) for n in (3, 50, 200)]
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        best = b.tune(stream)
+        torch.cuda.synchronize()
+        ref = [o["values"].clone() for o in outs]
+        assert best["block"] in (64, 256) and 1 <= best["waves_per_simd"] <= 8 and best["ms"] > 0
+        for blk, w in ((64, 2), (256, 8), (128, 1)):
+            b.set_launch(blk, w)
+            b.launch(stream)
+            for o, r in zip(outs, ref):
+                assert torch.equal(o["values"], r)

@@ -44,12 +44,19 @@ def build_library(verbose=False):
            "-Wall", "-Wno-unused-function",
            "-o", LIB + ".tmp",
            os.path.join(CSRC, "engine.cpp"), os.path.join(CSRC, "codegen.cpp"),
+           os.path.join(CSRC, "parametric_engine.cpp"),
            "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-lamdhip64",
            "-Wl,-rpath," + os.path.join(ROCM, "lib")]
     if verbose:
         print(" ".join(cmd))
     subprocess.run(cmd, check=True)
     os.replace(LIB + ".tmp", LIB)
+    # C++ drop-in facade demo, linked against the library like a session layer would be
+    demo = os.path.join(HERE, "facade_demo")
+    subprocess.run([os.path.join(ROCM, "bin", "hipcc"), "-O2", "-std=c++17", "-o", demo + ".tmp",
+                    os.path.join(CSRC, "facade_demo.cpp"), "-L" + HERE, "-lucuda", "-Wl,-rpath," + HERE,
+                    "-Wl,-rpath," + os.path.join(ROCM, "lib")], check=True)
+    os.replace(demo + ".tmp", demo)
     return LIB
 
 

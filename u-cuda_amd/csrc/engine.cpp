@@ -43,6 +43,7 @@ constexpr size_t kModuleCacheSize = 8;
 
 struct Module {
     std::string key;
+    int pins = 0;  // live batches using the module: never evicted while pinned
     hipModule_t mod = nullptr;
     size_t code_size = 0;
     std::vector<char> code;
@@ -151,9 +152,14 @@ int get_module(uc_engine* e, uc_system* s, bool strict, bool load, Module** out)
         if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLoadData"));
     }
     e->modules.push_front(std::move(m));
-    while (e->modules.size() > kModuleCacheSize) {
-        if (e->modules.back().mod) (void)hipModuleUnload(e->modules.back().mod);
-        e->modules.pop_back();
+    // LRU eviction past the capacity, skipping modules pinned by live batches
+    size_t over = e->modules.size() > kModuleCacheSize ? e->modules.size() - kModuleCacheSize : 0;
+    for (auto it = std::prev(e->modules.end()); over > 0 && it != e->modules.begin();) {
+        auto cur = it--;
+        if (cur->pins > 0) continue;
+        if (cur->mod) (void)hipModuleUnload(cur->mod);
+        e->modules.erase(cur);
+        --over;
     }
     *out = &e->modules.front();
     return UC_OK;
@@ -216,11 +222,57 @@ long long cells_of(const uc_common& c) {
     return n;
 }
 
-// Uploads axis tables + base params + base IC into a recycled device buffer
-// and fills the descriptor's grid fields.
-int stage_tables(uc_engine* e, const uc_system* s, const uc_common& c, UcJob& j, hipStream_t stream, TableBuffer** used) {
+size_t table_doubles(const uc_common& c) {
     const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
-    const size_t need = (size_t)nx + (size_t)ny + (size_t)kMaxValues + (size_t)kMaxVars;
+    return (size_t)nx + (size_t)ny + (size_t)kMaxValues + (size_t)kMaxVars;
+}
+
+// Host image of a launch's input tables: axis values (nx, then ny), 64
+// parameter slots, 32 initial-state slots.
+void fill_tables(const uc_system* s, const uc_common& c, double* host) {
+    const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
+    std::fill(host, host + table_doubles(c), 0.0);
+    double* ax1 = host + nx;
+    double* par = ax1 + ny;
+    double* ic = par + kMaxValues;
+    axis_values(c.axis[0], host);
+    if (c.n_axes > 1) axis_values(c.axis[1], ax1);
+    for (int i = 0; i < c.n_values; ++i) par[i] = c.base_values[i];
+    for (int i = 0; i < s->n_vars; ++i) ic[i] = c.initial_conditions ? c.initial_conditions[i] : 0.0;
+}
+
+// Descriptor fields common to every analysis, pointing at device tables.
+void wire_job(const uc_common& c, const double* dev, UcJob& j) {
+    const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
+    std::memset(&j, 0, sizeof j);
+    j.n_cells = (long long)nx * ny;
+    j.cell_offset = 0;
+    j.stride = j.n_cells;
+    j.nx = nx;
+    j.ny = ny;
+    for (int i = 0; i < 2; ++i) {
+        j.axis_kind[i] = i < c.n_axes ? c.axis[i].kind : UC_AXIS_NONE;
+        j.axis_index[i] = i < c.n_axes ? c.axis[i].index : 0;
+    }
+    j.axis_vals[0] = dev;
+    j.axis_vals[1] = dev + nx;
+    j.base_params = dev + nx + ny;
+    j.base_ic = dev + nx + ny + kMaxValues;
+    j.n_values = c.n_values;
+    j.ic_seed = c.ic_seed;
+    j.ic_lo = c.ic_lo;
+    j.ic_hi = c.ic_hi;
+    j.h = c.h;
+    j.max_value = c.max_value;
+    j.transient_time = c.transient_time;
+    j.t_max = c.t_max;
+    j.pre_scaller = 1;
+}
+
+// Uploads the tables into a recycled device buffer (pinned staging, reused
+// once the event recorded after its launch has completed) and wires `j`.
+int stage_tables(uc_engine* e, const uc_system* s, const uc_common& c, UcJob& j, hipStream_t stream, TableBuffer** used) {
+    const size_t need = table_doubles(c);
     TableBuffer* tb = nullptr;
     for (auto& t : e->tables) {
         if (t.in_flight && hipEventQuery(t.done) == hipSuccess) t.in_flight = false;
@@ -240,40 +292,10 @@ int stage_tables(uc_engine* e, const uc_system* s, const uc_common& c, UcJob& j,
             return e->fail(UC_ERR_HIP, hip_msg(he, "table buffer allocation"));
         }
     }
-    double* ax0 = tb->host;
-    double* ax1 = ax0 + nx;
-    double* par = ax1 + ny;
-    double* ic = par + kMaxValues;
-    std::fill(ax0, ax0 + need, 0.0);
-    axis_values(c.axis[0], ax0);
-    if (c.n_axes > 1) axis_values(c.axis[1], ax1);
-    for (int i = 0; i < c.n_values; ++i) par[i] = c.base_values[i];
-    for (int i = 0; i < s->n_vars; ++i) ic[i] = c.initial_conditions ? c.initial_conditions[i] : 0.0;
+    fill_tables(s, c, tb->host);
     hipError_t he = hipMemcpyAsync(tb->dev, tb->host, need * sizeof(double), hipMemcpyHostToDevice, stream);
     if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipMemcpyAsync(tables)"));
-
-    std::memset(&j, 0, sizeof j);
-    j.n_cells = (long long)nx * ny;
-    j.cell_offset = 0;
-    j.nx = nx;
-    j.ny = ny;
-    for (int i = 0; i < 2; ++i) {
-        j.axis_kind[i] = i < c.n_axes ? c.axis[i].kind : UC_AXIS_NONE;
-        j.axis_index[i] = i < c.n_axes ? c.axis[i].index : 0;
-    }
-    j.axis_vals[0] = tb->dev;
-    j.axis_vals[1] = tb->dev + nx;
-    j.base_params = tb->dev + nx + ny;
-    j.base_ic = tb->dev + nx + ny + kMaxValues;
-    j.n_values = c.n_values;
-    j.ic_seed = c.ic_seed;
-    j.ic_lo = c.ic_lo;
-    j.ic_hi = c.ic_hi;
-    j.h = c.h;
-    j.max_value = c.max_value;
-    j.transient_time = c.transient_time;
-    j.t_max = c.t_max;
-    j.pre_scaller = 1;
+    wire_job(c, tb->dev, j);
     *used = tb;
     return UC_OK;
 }
@@ -429,9 +451,8 @@ int uc_axis_values(const uc_axis* ax, double* out) {
     return UC_OK;
 }
 
-int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, void* stream) {
-    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
-    std::lock_guard<std::recursive_mutex> lk(e->mu);
+// Validates a bifurcation request and fills the analysis-specific fields.
+static int prep_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, long long* iters_out) {
     int rc = validate_common(e, s, r->c, true);
     if (rc) return rc;
     if (r->writable_var < -1 || r->writable_var >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "writable_var out of range");
@@ -447,14 +468,11 @@ int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif
     const long long iters = (long long)std::ceil(r->c.t_max / hw / r->pre_scaller);
     if (iters <= 0) return e->fail(UC_ERR_ARGUMENT, "computed amountOfPointsInBlock <= 0");
     if (r->max_peaks > 2500) return e->fail(UC_ERR_ARGUMENT, "max_peaks above the reference cap 2500");
-    if ((rc = enter(e))) return rc;
+    *iters_out = iters;
+    return UC_OK;
+}
 
-    Module* m = nullptr;
-    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
-    UcJob j;
-    TableBuffer* tb = nullptr;
-    hipStream_t st = (hipStream_t)stream;
-    if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
+static void fill_bif(const uc_bif_request* r, const uc_bif_output* o, long long iters, UcJob& j) {
     j.skip_steps = r->c.h > 0.0 ? (long long)(r->c.transient_time / r->c.h) : 0;
     j.n_iters = iters;
     j.pre_scaller = r->pre_scaller;
@@ -466,12 +484,26 @@ int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif
     j.out1 = o->intervals;
     j.traj = o->traj;
     j.final_state = o->final_state;
+}
+
+int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, void* stream) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    long long iters = 0;
+    int rc = prep_bif(e, s, r, o, &iters);
+    if (rc) return rc;
+    if ((rc = enter(e))) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
+    fill_bif(r, o, iters, j);
     return launch(e, m, r->c.precision == UC_PREC_F64 ? "uc_bif_f64" : "uc_bif_f32c", j, st, tb);
 }
 
-static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream, bool spectrum) {
-    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
-    std::lock_guard<std::recursive_mutex> lk(e->mu);
+static int prep_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o) {
     int rc = validate_common(e, s, r->c, true);
     if (rc) return rc;
     if (!(r->NT > 0.0)) return e->fail(UC_ERR_ARGUMENT, "NT must be > 0");
@@ -486,17 +518,13 @@ static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const 
     for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
     const long long nts = r->c.h > 0.0 ? (long long)(r->NT / r->c.h) : 0;
     if (!step_axis && nts <= 0) return e->fail(UC_ERR_ARGUMENT, "computed NT / h <= 0");
-    if ((rc = enter(e))) return rc;
+    return UC_OK;
+}
 
-    Module* m = nullptr;
-    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
-    UcJob j;
-    TableBuffer* tb = nullptr;
-    hipStream_t st = (hipStream_t)stream;
-    if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
+static void fill_lyap(const uc_lyap_request* r, const uc_lyap_output* o, UcJob& j) {
     j.skip_steps = r->c.h > 0.0 ? (long long)(r->c.transient_time / r->c.h) : 0;
-    j.n_iters = blocks;
-    j.nt_steps = nts;
+    j.n_iters = (long long)(r->c.t_max / r->NT);
+    j.nt_steps = r->c.h > 0.0 ? (long long)(r->NT / r->c.h) : 0;
     j.nt = r->NT;
     j.eps = r->eps;
     j.method = r->method;
@@ -504,9 +532,27 @@ static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const 
     j.flag = o->flags;
     j.out0 = o->values;
     j.final_state = o->final_state;
-    const char* name = spectrum ? (r->c.precision == UC_PREC_F64 ? "uc_ls_f64" : "uc_ls_f32c")
-                                : (r->c.precision == UC_PREC_F64 ? "uc_lle_f64" : "uc_lle_f32c");
-    return launch(e, m, name, j, st, tb);
+}
+
+static const char* lyap_kernel(bool spectrum, int precision) {
+    return spectrum ? (precision == UC_PREC_F64 ? "uc_ls_f64" : "uc_ls_f32c")
+                    : (precision == UC_PREC_F64 ? "uc_lle_f64" : "uc_lle_f32c");
+}
+
+static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream, bool spectrum) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    int rc = prep_lyap(e, s, r, o);
+    if (rc) return rc;
+    if ((rc = enter(e))) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
+    fill_lyap(r, o, j);
+    return launch(e, m, lyap_kernel(spectrum, r->c.precision), j, st, tb);
 }
 
 int uc_run_lle(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream) {
@@ -677,6 +723,308 @@ int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_s
     void* args[] = {(void*)&data, &n_samples, &n_series, &time_step, &max_peaks, &counts, &peaks, &intervals};
     he = hipModuleLaunchKernel(f, blocks, 1, 1, kBlock, 1, 1, 0, (hipStream_t)stream, args, nullptr);
     if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLaunchKernel(peaks)"));
+    return UC_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================
+// Persistent heterogeneous batch: every job of the same (module, kernel)
+// shares one persistent launch over a job table; all groups are captured
+// into one hipGraph and launched once.
+struct uc_batch {
+    uc_engine* e = nullptr;
+    struct Job {
+        Module* m;
+        std::string kernel;  // persistent kernel name
+        UcJob job;
+        double cost;         // steps per cell, for longest-first ordering
+    };
+    std::vector<Job> jobs;
+    std::vector<void*> owned;  // device allocations (tables, job arrays, counters)
+    struct Group {
+        Module* m;
+        std::string kernel;
+        std::vector<UcJob> table;
+        UcJob* d_jobs = nullptr;
+        long long* d_start = nullptr;
+        long long total_chunks = 0;
+    };
+    std::vector<Group> groups;
+    unsigned long long* d_counters = nullptr;
+    int block = 0, waves_per_simd = 0, built_block = -1;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipStream_t> aux;
+    std::vector<hipEvent_t> events;
+
+    ~uc_batch() {
+        for (auto& j : jobs) --j.m->pins;
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto ev : events) (void)hipEventDestroy(ev);
+        for (auto st : aux) (void)hipStreamDestroy(st);
+        for (void* p : owned) (void)hipFree(p);
+    }
+    int fail(int code, const std::string& m) { return e->fail(code, m); }
+    void drop_graph() {
+        if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+        if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+    }
+};
+
+namespace {
+
+int batch_stage(uc_batch* b, const uc_system* s, const uc_common& c, UcJob& j) {
+    const size_t need = table_doubles(c);
+    std::vector<double> host(need);
+    fill_tables(s, c, host.data());
+    double* dev = nullptr;
+    hipError_t he = hipMalloc((void**)&dev, need * sizeof(double));
+    if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(batch tables)"));
+    b->owned.push_back(dev);
+    he = hipMemcpy(dev, host.data(), need * sizeof(double), hipMemcpyHostToDevice);
+    if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "hipMemcpy(batch tables)"));
+    wire_job(c, dev, j);
+    return UC_OK;
+}
+
+int cu_count(int device) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) n = 256;
+    return n;
+}
+
+// Builds the per-group job tables for a chunk size of `block` cells.
+int batch_build(uc_batch* b, int block) {
+    if (b->built_block == block) return UC_OK;
+    b->drop_graph();
+    std::map<std::pair<Module*, std::string>, size_t> gi;
+    b->groups.clear();
+    std::vector<uc_batch::Job> order = b->jobs;
+    std::stable_sort(order.begin(), order.end(), [](const uc_batch::Job& x, const uc_batch::Job& y) { return x.cost > y.cost; });
+    for (auto& jb : order) {
+        auto key = std::make_pair(jb.m, jb.kernel);
+        auto it = gi.find(key);
+        if (it == gi.end()) {
+            gi[key] = b->groups.size();
+            b->groups.push_back(uc_batch::Group{jb.m, jb.kernel, {}, nullptr, nullptr, 0});
+            it = gi.find(key);
+        }
+        b->groups[it->second].table.push_back(jb.job);
+    }
+    for (auto& g : b->groups) {
+        std::vector<long long> start(g.table.size() + 1, 0);
+        for (size_t i = 0; i < g.table.size(); ++i)
+            start[i + 1] = start[i] + (g.table[i].n_cells + block - 1) / block;
+        g.total_chunks = start.back();
+        hipError_t he = hipMalloc((void**)&g.d_jobs, sizeof(UcJob) * g.table.size());
+        if (he == hipSuccess) { b->owned.push_back(g.d_jobs); he = hipMalloc((void**)&g.d_start, sizeof(long long) * start.size()); }
+        if (he == hipSuccess) { b->owned.push_back(g.d_start); he = hipMemcpy(g.d_jobs, g.table.data(), sizeof(UcJob) * g.table.size(), hipMemcpyHostToDevice); }
+        if (he == hipSuccess) he = hipMemcpy(g.d_start, start.data(), sizeof(long long) * start.size(), hipMemcpyHostToDevice);
+        if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "batch job tables"));
+    }
+    if (!b->d_counters) {
+        hipError_t he = hipMalloc((void**)&b->d_counters, sizeof(unsigned long long) * 64);
+        if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "batch counters"));
+        b->owned.push_back(b->d_counters);
+    }
+    if (b->groups.size() > 64) return b->fail(UC_ERR_ARGUMENT, "more than 64 kernel groups in one batch");
+    b->built_block = block;
+    return UC_OK;
+}
+
+// Enqueues the reset kernel and one persistent launch per group; groups
+// run concurrently on auxiliary streams joined back into `st`.
+int batch_enqueue(uc_batch* b, hipStream_t st, int block, int wps) {
+    const int cus = cu_count(b->e->device);
+    const int waves_per_wg = block / 64;
+    long long wgs = (long long)cus * 4 * wps / waves_per_wg;
+    if (wgs < 1) wgs = 1;
+    hipFunction_t freset;
+    hipError_t he = hipModuleGetFunction(&freset, b->groups[0].m->mod, "uc_batch_reset");
+    int ng = (int)b->groups.size();
+    void* rargs[] = {&b->d_counters, &ng};
+    if (he == hipSuccess) he = hipModuleLaunchKernel(freset, 1, 1, 1, 64, 1, 1, 0, st, rargs, nullptr);
+    if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "batch reset"));
+    while ((int)b->aux.size() < ng - 1) {
+        hipStream_t s2;
+        if ((he = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "aux stream"));
+        b->aux.push_back(s2);
+    }
+    while ((int)b->events.size() < 2 * ng) {
+        hipEvent_t ev;
+        if ((he = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "event"));
+        b->events.push_back(ev);
+    }
+    if (ng > 1 && (he = hipEventRecord(b->events[0], st)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "fork"));
+    for (int g = 0; g < ng; ++g) {
+        auto& G = b->groups[(size_t)g];
+        hipStream_t gs = g == 0 ? st : b->aux[(size_t)g - 1];
+        if (g > 0 && (he = hipStreamWaitEvent(gs, b->events[0], 0)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "fork wait"));
+        hipFunction_t f;
+        if ((he = hipModuleGetFunction(&f, G.m->mod, G.kernel.c_str())) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, G.kernel.c_str()));
+        int nj = (int)G.table.size();
+        unsigned long long* ctr = b->d_counters + g;
+        void* args[] = {&G.d_jobs, &G.d_start, &nj, &ctr};
+        const long long grid = std::min<long long>(wgs, std::max<long long>(1, G.total_chunks));
+        if ((he = hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, 0, gs, args, nullptr)) != hipSuccess)
+            return b->fail(UC_ERR_HIP, hip_msg(he, "persistent launch"));
+        if (g > 0) {
+            if ((he = hipEventRecord(b->events[(size_t)(ng + g)], gs)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "join"));
+            if ((he = hipStreamWaitEvent(st, b->events[(size_t)(ng + g)], 0)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "join wait"));
+        }
+    }
+    return UC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int uc_batch_create(uc_engine* e, uc_batch** out) {
+    if (!e || !out) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    int rc = enter(e);
+    if (rc) return rc;
+    auto b = new uc_batch;
+    b->e = e;
+    *out = b;
+    return UC_OK;
+}
+
+void uc_batch_destroy(uc_batch* b) {
+    if (!b) return;
+    std::lock_guard<std::recursive_mutex> lk(b->e->mu);
+    (void)hipSetDevice(b->e->device);
+    delete b;
+}
+
+int uc_batch_add_bif(uc_batch* b, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, int* job_id) {
+    if (!b || !s || !r || !o) return UC_ERR_ARGUMENT;
+    uc_engine* e = b->e;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    long long iters = 0;
+    int rc = prep_bif(e, s, r, o, &iters);
+    if (rc) return rc;
+    if ((rc = enter(e))) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    if ((rc = batch_stage(b, s, r->c, j))) return rc;
+    fill_bif(r, o, iters, j);
+    const double cost = (double)(j.skip_steps + iters * r->pre_scaller);
+    b->jobs.push_back({m, r->c.precision == UC_PREC_F64 ? "uc_bif_f64_q" : "uc_bif_f32c_q", j, cost});
+    ++m->pins;
+    b->built_block = -1;
+    if (job_id) *job_id = (int)b->jobs.size() - 1;
+    return UC_OK;
+}
+
+int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int spectrum, const uc_lyap_output* o,
+                      int* job_id) {
+    if (!b || !s || !r || !o) return UC_ERR_ARGUMENT;
+    uc_engine* e = b->e;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    int rc = prep_lyap(e, s, r, o);
+    if (rc) return rc;
+    if ((rc = enter(e))) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    if ((rc = batch_stage(b, s, r->c, j))) return rc;
+    fill_lyap(r, o, j);
+    const double cost = (double)(j.skip_steps + j.n_iters * (j.nt_steps + 1)) * (spectrum ? 1.0 + s->n_vars : 2.0);
+    b->jobs.push_back({m, std::string(lyap_kernel(spectrum != 0, r->c.precision)) + "_q", j, cost});
+    ++m->pins;
+    b->built_block = -1;
+    if (job_id) *job_id = (int)b->jobs.size() - 1;
+    return UC_OK;
+}
+
+int uc_batch_set_launch(uc_batch* b, int block, int waves_per_simd) {
+    if (!b) return UC_ERR_ARGUMENT;
+    if (block != 64 && block != 128 && block != 256) return b->fail(UC_ERR_ARGUMENT, "block must be 64, 128 or 256");
+    if (waves_per_simd < 1 || waves_per_simd > 8) return b->fail(UC_ERR_ARGUMENT, "waves_per_simd must be in [1, 8]");
+    if (b->block != block || b->waves_per_simd != waves_per_simd) b->drop_graph();
+    b->block = block;
+    b->waves_per_simd = waves_per_simd;
+    return UC_OK;
+}
+
+int uc_batch_tune(uc_batch* b, void* stream, int* best_block, int* best_wps, double* best_ms) {
+    if (!b) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(b->e->mu);
+    if (b->jobs.empty()) return b->fail(UC_ERR_ARGUMENT, "empty batch");
+    int rc = enter(b->e);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t t0, t1;
+    if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) return b->fail(UC_ERR_HIP, "event create");
+    const int blocks[] = {64, 256};
+    const int wpss[] = {2, 4, 8};
+    double best = 1e30;
+    int bb = 256, bw = 4;
+    for (int blk : blocks)
+        for (int w : wpss) {
+            if ((rc = batch_build(b, blk))) break;
+            (void)hipEventRecord(t0, st);
+            if ((rc = batch_enqueue(b, st, blk, w))) break;
+            (void)hipEventRecord(t1, st);
+            if (hipEventSynchronize(t1) != hipSuccess) { rc = b->fail(UC_ERR_HIP, "tune sync"); break; }
+            float ms = 0.0f;
+            (void)hipEventElapsedTime(&ms, t0, t1);
+            if (ms < best) { best = ms; bb = blk; bw = w; }
+        }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    if (rc) return rc;
+    b->drop_graph();
+    b->block = bb;
+    b->waves_per_simd = bw;
+    if (best_block) *best_block = bb;
+    if (best_wps) *best_wps = bw;
+    if (best_ms) *best_ms = best;
+    return UC_OK;
+}
+
+int uc_batch_launch(uc_batch* b, void* stream) {
+    if (!b) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(b->e->mu);
+    if (b->jobs.empty()) return b->fail(UC_ERR_ARGUMENT, "empty batch");
+    int rc = enter(b->e);
+    if (rc) return rc;
+    const int block = b->block ? b->block : 256, wps = b->waves_per_simd ? b->waves_per_simd : 4;
+    hipStream_t st = (hipStream_t)stream;
+    if (!st) return b->fail(UC_ERR_ARGUMENT, "batch launch needs a non-default stream (graph capture)");
+    if ((rc = batch_build(b, block))) return rc;
+    if (!b->exec) {
+        hipError_t he = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "hipStreamBeginCapture"));
+        rc = batch_enqueue(b, st, block, wps);
+        hipGraph_t g = nullptr;
+        he = hipStreamEndCapture(st, &g);
+        if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+        if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "hipStreamEndCapture"));
+        b->graph = g;
+        he = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+        if (he != hipSuccess) { b->exec = nullptr; return b->fail(UC_ERR_HIP, hip_msg(he, "hipGraphInstantiate")); }
+    }
+    hipError_t he = hipGraphLaunch(b->exec, st);
+    if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "hipGraphLaunch"));
+    return UC_OK;
+}
+
+int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int* waves_per_simd) {
+    if (!b) return UC_ERR_ARGUMENT;
+    if (n_jobs) *n_jobs = (int)b->jobs.size();
+    if (n_groups) {
+        std::map<std::pair<Module*, std::string>, int> g;
+        for (auto& j : b->jobs) g[std::make_pair(j.m, j.kernel)] = 1;
+        *n_groups = (int)g.size();
+    }
+    if (block) *block = b->block ? b->block : 256;
+    if (waves_per_simd) *waves_per_simd = b->waves_per_simd ? b->waves_per_simd : 4;
     return UC_OK;
 }
 

@@ -1,0 +1,72 @@
+// facade_demo.cpp — drives the C++ facade the way the reference's session
+// layer drives ParametricEngine (build a Request, run, read the Result),
+// and prints a JSON summary. Used by the GPU tests to check that the facade
+// and the C ABI give the same numbers.
+#include <cmath>
+#include <cstdio>
+
+#include "parametric_engine.hpp"
+
+using namespace ucuda;
+
+int main() {
+    ParametricEngine eng(0);
+    System lorenz{{"x", "y", "z"}, {"sigma", "rho", "beta"}, {"sigma*(y-x)", "x*(rho-z)-y", "x*y-beta*z"}, false, ""};
+    Bifurcation1DRequest b;
+    b.system = lorenz;
+    b.amountOfX = 3;
+    b.initial_conditions = {0.1, 0.1, 0.1};
+    b.base_values = {0.5, 10.0, 28.0, 8.0 / 3.0};
+    b.param_index = 2;
+    b.param_lo = 0.0;
+    b.param_hi = 250.0;
+    b.n_pts = 16;
+    b.h = 0.01;
+    b.transient_time = 20.0;
+    b.t_max = 20.0;
+    b.max_peaks = 64;
+    Bifurcation1DResult br = eng.run_bifurcation_1d(b);
+    if (!br.ok) { std::printf("{\"error\": \"%s\"}\n", br.error.c_str()); return 1; }
+
+    System rossler{{"x", "y", "z"}, {"a", "b", "c"}, {"-y-z", "x+a*y", "b+z*(x-c)"}, false, ""};
+    LLE2DRequest l;
+    l.system = rossler;
+    l.amountOfX = 3;
+    l.initial_conditions = {1.0, 1.0, 1.0};
+    l.base_values = {0.5, 0.2, 0.2, 5.7};
+    l.param_index = 2;
+    l.param_lo = 0.05;
+    l.param_hi = 2.0;
+    l.param_index_2 = 3;
+    l.param_lo_2 = 1.0;
+    l.param_hi_2 = 20.0;
+    l.n_pts = 8;
+    l.h = 0.01;
+    l.transient_time = 5.0;
+    l.t_max = 50.0;
+    l.NT = 0.1;
+    LLE2DResult lr = eng.run_lle_2d(l);
+    if (!lr.ok) { std::printf("{\"error\": \"%s\"}\n", lr.error.c_str()); return 1; }
+
+    LS2DRequest ls = l;
+    ls.n_pts = 4;
+    LS2DResult lsr = eng.run_ls_2d(ls);
+    if (!lsr.ok) { std::printf("{\"error\": \"%s\"}\n", lsr.error.c_str()); return 1; }
+
+    std::printf("{\"bif_flags\": [");
+    for (size_t i = 0; i < br.flags.size(); ++i) std::printf("%s%d", i ? ", " : "", br.flags[i]);
+    std::printf("], \"bif_first_peaks\": [");
+    for (size_t i = 0; i < br.flags.size(); ++i)
+        std::printf("%s%.17g", i ? ", " : "", br.bifurcation_points[i].empty() ? 0.0 : br.bifurcation_points[i][0]);
+    std::printf("], \"lle2d\": [");
+    for (size_t i = 0; i < lr.values.size(); ++i) std::printf("%s%.17g", i ? ", " : "", std::isnan(lr.values[i]) ? 999.0 : lr.values[i]);
+    std::printf("], \"ls2d_sum\": [");
+    const size_t n = lsr.flags.size();
+    for (size_t i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < lsr.n_exponents; ++k) s += lsr.values[(size_t)k * n + i];
+        std::printf("%s%.17g", i ? ", " : "", s);
+    }
+    std::printf("], \"lle_min\": %.17g, \"lle_max\": %.17g}\n", lr.min_val, lr.max_val);
+    return 0;
+}

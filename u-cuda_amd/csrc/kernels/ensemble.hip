@@ -300,10 +300,13 @@ __device__ __forceinline__ int uc_fixed_point_flag(const S& s, const double* pre
     return fabs(d) < Prec<S>::fixed_point_eps(h, pre, m) ? -1 : 1;
 }
 
-// Plain state loop.
+// Plain state loop. `t` is the system time (used only by right-hand sides
+// with a time symbol; dead code otherwise), advanced by h per step.
 template <class S, class Obs>
-__device__ int uc_loop(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, long long iters, int pre,
+__device__ int uc_loop(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t, long long iters, int pre,
                        int wvar, double max_value, Obs& obs) {
+    typedef typename Prec<S>::T T;
+    const double hd = (double)h;
     int until_check = 0;
     for (long long i = 0; i < iters; ++i) {
         if (Obs::records) {
@@ -318,9 +321,10 @@ __device__ int uc_loop(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h
                 for (int q = 1; q < UC_N; ++q) v = (q == wvar) ? s.get(q) : v;
             }
             obs.sample(i, v);
-            for (int p = 0; p < pre; ++p) uc_rk4(s, a, h, (typename Prec<S>::T)0);
+            for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += hd; }
         } else {
-            uc_rk4(s, a, h, (typename Prec<S>::T)0);
+            uc_rk4(s, a, h, (T)t);
+            if (UC_USES_TIME) t += hd;
         }
         if (until_check == 0) {
             if (uc_diverged(s, max_value)) return 0;
@@ -331,8 +335,8 @@ __device__ int uc_loop(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h
     double prev[UC_N];
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
-    for (int p = 0; p < pre; ++p) uc_rk4(s, a, h, (typename Prec<S>::T)0);
-    return uc_fixed_point_flag(s, prev, (double)h, pre);
+    for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += hd; }
+    return uc_fixed_point_flag(s, prev, hd, pre);
 }
 
 // State + K tangent vectors advanced together for `iters` steps plus the
@@ -340,10 +344,13 @@ __device__ int uc_loop(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h
 // counterpart of one reference renormalisation block).
 template <int K, class S>
 __device__ int uc_loop_tan(S& s, typename Prec<S>::T* V, const typename Prec<S>::T* a, typename Prec<S>::T h,
-                           long long iters, double max_value) {
+                           double& t, long long iters, double max_value) {
+    typedef typename Prec<S>::T T;
+    const double hd = (double)h;
     int until_check = 0;
     for (long long i = 0; i < iters; ++i) {
-        uc_rk4_tan<K>(s, V, a, h, (typename Prec<S>::T)0);
+        uc_rk4_tan<K>(s, V, a, h, (T)t);
+        if (UC_USES_TIME) t += hd;
         if (until_check == 0) {
             if (uc_diverged(s, max_value)) return 0;
             until_check = UC_CHECK_INTERVAL;
@@ -353,8 +360,9 @@ __device__ int uc_loop_tan(S& s, typename Prec<S>::T* V, const typename Prec<S>:
     double prev[UC_N];
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
-    uc_rk4_tan<K>(s, V, a, h, (typename Prec<S>::T)0);
-    return uc_fixed_point_flag(s, prev, (double)h, 1);
+    uc_rk4_tan<K>(s, V, a, h, (T)t);
+    if (UC_USES_TIME) t += hd;
+    return uc_fixed_point_flag(s, prev, hd, 1);
 }
 
 // ------------------------------------------------------ streaming peaks ---
@@ -374,7 +382,7 @@ struct PeakObserver {
     double last_t;
     double time_step;
     int max_peaks;
-    long long stride;       // n_cells
+    long long stride;       // output plane stride
     double* peaks;          // peaks[k * stride]
     double* intervals;
     double* traj;
@@ -424,10 +432,8 @@ struct PeakObserver {
 // Bifurcation sweep (calculateDiscreteModelCUDA + peakFinderCUDA fused):
 // transient, recorded run with streaming peaks, flag = peak count / -1 / 0.
 template <class S>
-__device__ void uc_bif_body(const UcJob& j) {
+__device__ void uc_bif_cell(const UcJob& j, long long lane) {
     typedef typename Prec<S>::T T;
-    const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= j.n_cells) return;
     const long long cell = j.cell_offset + lane;
 
     double ad[UC_M], x0[UC_N], hd;
@@ -447,18 +453,19 @@ __device__ void uc_bif_body(const UcJob& j) {
     s.load(x0);
 
     NoObserver none;
-    int flag = uc_loop(s, a, h, skip, j.pre_scaller, 0, j.max_value, none);
+    double t = 0.0;
+    int flag = uc_loop(s, a, h, t, skip, j.pre_scaller, 0, j.max_value, none);
     PeakObserver obs;
     obs.w0 = 0.0; obs.w1 = 0.0; obs.n = iters; obs.c_next = 2; obs.mode = 0; obs.raw = 0;
     obs.last_t = 0.0; obs.time_step = hd * (double)j.pre_scaller; obs.max_peaks = j.max_peaks;
-    obs.stride = j.n_cells; obs.peaks = j.out0 + lane; obs.intervals = j.out1 + lane;
+    obs.stride = j.stride; obs.peaks = j.out0 + lane; obs.intervals = j.out1 + lane;
     obs.traj = j.traj ? j.traj + lane : nullptr; obs.traj_len = j.traj ? j.traj_len : 0;
     if (flag == 1 || flag == -1)
-        flag = uc_loop(s, a, h, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
+        flag = uc_loop(s, a, h, t, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
     j.flag[lane] = (flag == 1) ? obs.count() : flag;
     if (j.final_state) {
 #pragma unroll
-        for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.n_cells + lane] = s.get(i);
+        for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.stride + lane] = s.get(i);
     }
 }
 
@@ -540,19 +547,17 @@ __device__ __forceinline__ void uc_cgs(const double* a, double* b, double* den) 
 // double after every block of nt_steps + 1 steps (the reference block, which
 // ends with the step loop's trailing step). Finite-difference method: the
 // reference's Benettin scheme with probe copies at distance eps.
-// Results: out0[k * n_cells + cell] = sum_k / t_max; flag 1, or -1 with 999
+// Results: out0[k * stride + cell] = sum_k / t_max; flag 1, or -1 with 999
 // (diverged) / NaN (non-positive step).
 template <int K, class S>
-__device__ void uc_lyap_body(const UcJob& j) {
+__device__ void uc_lyap_cell(const UcJob& j, long long lane) {
     typedef typename Prec<S>::T T;
-    const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= j.n_cells) return;
     const long long cell = j.cell_offset + lane;
 
     double ad[UC_M], x0[UC_N], hd;
     if (!uc_setup(j, cell, ad, x0, hd)) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) j.out0[(long long)k * j.n_cells + lane] = __builtin_nan("");
+        for (int k = 0; k < K; ++k) j.out0[(long long)k * j.stride + lane] = __builtin_nan("");
         j.flag[lane] = -1;
         return;
     }
@@ -574,7 +579,8 @@ __device__ void uc_lyap_body(const UcJob& j) {
     bool alive = true;
 
     NoObserver none;
-    if (uc_loop(s, a, h, skip, 1, 0, j.max_value, none) == 0) alive = false;
+    double t = 0.0;
+    if (uc_loop(s, a, h, t, skip, 1, 0, j.max_value, none) == 0) alive = false;
 
     if (alive) {
         double z[K * UC_N];
@@ -586,7 +592,7 @@ __device__ void uc_lyap_body(const UcJob& j) {
 #pragma unroll
             for (int i = 0; i < K * UC_N; ++i) V[i] = Prec<S>::to_t(z[i]);
             for (long long b = 0; b < j.n_iters; ++b) {
-                if (uc_loop_tan<K>(s, V, a, h, nts, j.max_value) == 0) { alive = false; break; }
+                if (uc_loop_tan<K>(s, V, a, h, t, nts, j.max_value) == 0) { alive = false; break; }
                 double v[K * UC_N];
 #pragma unroll
                 for (int i = 0; i < K * UC_N; ++i) v[i] = (double)V[i];
@@ -626,10 +632,13 @@ __device__ void uc_lyap_body(const UcJob& j) {
                 y[k].load(yk);
             }
             for (long long b = 0; b < j.n_iters && alive; ++b) {
-                if (uc_loop(s, a, h, nts, 1, 0, j.max_value, none) == 0) { alive = false; break; }
+                const double t0 = t;
+                if (uc_loop(s, a, h, t, nts, 1, 0, j.max_value, none) == 0) { alive = false; break; }
 #pragma unroll
-                for (int k = 0; k < K; ++k)
-                    if (alive && uc_loop(y[k], a, h, nts, 1, 0, j.max_value, none) == 0) alive = false;
+                for (int k = 0; k < K; ++k) {
+                    double tk = t0;
+                    if (alive && uc_loop(y[k], a, h, tk, nts, 1, 0, j.max_value, none) == 0) alive = false;
+                }
                 if (!alive) break;
                 if (K == 1) {
                     double d = 0.0;
@@ -666,11 +675,11 @@ __device__ void uc_lyap_body(const UcJob& j) {
         }
     }
 #pragma unroll
-    for (int k = 0; k < K; ++k) j.out0[(long long)k * j.n_cells + lane] = alive ? sum[k] / j.t_max : 999.0;
+    for (int k = 0; k < K; ++k) j.out0[(long long)k * j.stride + lane] = alive ? sum[k] / j.t_max : 999.0;
     j.flag[lane] = alive ? 1 : -1;
     if (j.final_state) {
 #pragma unroll
-        for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.n_cells + lane] = s.get(i);
+        for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.stride + lane] = s.get(i);
     }
 }
 
@@ -689,12 +698,62 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_peaks_series(const dou
     counts[lane] = obs.count();
 }
 
-extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_f64(UcJob j) { uc_bif_body<StateF64>(j); }
-extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_f32c(UcJob j) { uc_bif_body<StateF32C>(j); }
-extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_lle_f64(UcJob j) { uc_lyap_body<1, StateF64>(j); }
-extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_lle_f32c(UcJob j) { uc_lyap_body<1, StateF32C>(j); }
-extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_ls_f64(UcJob j) { uc_lyap_body<UC_N, StateF64>(j); }
-extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_ls_f32c(UcJob j) { uc_lyap_body<UC_N, StateF32C>(j); }
+// ------------------------------------------------------- launch shapes ---
+// Direct launch: one lane per cell of one job.
+#define UC_DIRECT(NAME, CELL)                                                   \
+    extern "C" __global__ void __launch_bounds__(UC_BLOCK) NAME(UcJob j) {      \
+        const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x; \
+        if (lane < j.n_cells) CELL(j, lane);                                    \
+    }
+
+// Persistent launch over a job table: each workgroup repeatedly takes the
+// next chunk of blockDim cells from a global counter (chunks of all jobs in
+// one index space, chunk_start = prefix sums, jobs ordered by cost so the
+// longest work is taken first) until the counter passes the end; every
+// workgroup reaches that exit, so the grid drains. A workgroup's lanes
+// always belong to one job.
+#define UC_PERSISTENT(NAME, CELL)                                                                   \
+    extern "C" __global__ void __launch_bounds__(UC_BLOCK) NAME(const UcJob* __restrict__ jobs,     \
+                                                                const long long* __restrict__ start, \
+                                                                int n_jobs, u64* counter) {          \
+        __shared__ long long s_chunk;                                                               \
+        const long long total = start[n_jobs];                                                      \
+        for (;;) {                                                                                  \
+            if (threadIdx.x == 0) s_chunk = (long long)atomicAdd((unsigned long long*)counter, 1ULL); \
+            __syncthreads();                                                                        \
+            const long long c = s_chunk;                                                            \
+            __syncthreads();                                                                        \
+            if (c >= total) break;                                                                  \
+            int lo = 0, hi = n_jobs - 1;                                                            \
+            while (lo < hi) {                                                                       \
+                const int mid = (lo + hi + 1) >> 1;                                                 \
+                if (start[mid] <= c) lo = mid; else hi = mid - 1;                                   \
+            }                                                                                       \
+            lo = __builtin_amdgcn_readfirstlane(lo); /* uniform: job fields become scalar loads */   \
+            const long long lane = (c - start[lo]) * (long long)blockDim.x + threadIdx.x;           \
+            if (lane < jobs[lo].n_cells) CELL(jobs[lo], lane);                                      \
+        }                                                                                           \
+    }
+
+// Zeroes the work counters of a batch (a kernel, not a memset, so the reset
+// can live inside a captured graph).
+extern "C" __global__ void uc_batch_reset(u64* counters, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) counters[i] = 0ULL;
+}
+
+UC_DIRECT(uc_bif_f64, uc_bif_cell<StateF64>)
+UC_DIRECT(uc_bif_f32c, uc_bif_cell<StateF32C>)
+UC_DIRECT(uc_lle_f64, (uc_lyap_cell<1, StateF64>))
+UC_DIRECT(uc_lle_f32c, (uc_lyap_cell<1, StateF32C>))
+UC_DIRECT(uc_ls_f64, (uc_lyap_cell<UC_N, StateF64>))
+UC_DIRECT(uc_ls_f32c, (uc_lyap_cell<UC_N, StateF32C>))
+UC_PERSISTENT(uc_bif_f64_q, uc_bif_cell<StateF64>)
+UC_PERSISTENT(uc_bif_f32c_q, uc_bif_cell<StateF32C>)
+UC_PERSISTENT(uc_lle_f64_q, (uc_lyap_cell<1, StateF64>))
+UC_PERSISTENT(uc_lle_f32c_q, (uc_lyap_cell<1, StateF32C>))
+UC_PERSISTENT(uc_ls_f64_q, (uc_lyap_cell<UC_N, StateF64>))
+UC_PERSISTENT(uc_ls_f32c_q, (uc_lyap_cell<UC_N, StateF32C>))
 
 // ================================================================ basins ==
 // Adaptive Dormand-Prince 5(4) (Dormand & Prince 1980 coefficients; Hairer's
@@ -847,10 +906,8 @@ template <> struct BasPrec<BasF32C> {
 };
 
 template <class S>
-__device__ void uc_basins_body(const UcJob& j) {
+__device__ void uc_basins_cell(const UcJob& j, long long lane) {
     typedef typename BasPrec<S>::T T;
-    const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (lane >= j.n_cells) return;
     const long long cell = j.cell_offset + lane;
     double ad[UC_M], x0[UC_N], hd;
     uc_setup(j, cell, ad, x0, hd);
@@ -953,15 +1010,17 @@ __device__ void uc_basins_body(const UcJob& j) {
     }
     j.flag[lane] = p;
 #pragma unroll
-    for (int i = 0; i < UC_LBL; ++i) j.out0[(long long)i * j.n_cells + lane] = pt[i];
+    for (int i = 0; i < UC_LBL; ++i) j.out0[(long long)i * j.stride + lane] = pt[i];
     if (j.counters) {
         j.counters[lane] = acc;
-        j.counters[j.n_cells + lane] = rej;
+        j.counters[j.stride + lane] = rej;
     }
 }
 
-extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bas_f64(UcJob j) { uc_basins_body<BasF64>(j); }
-extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bas_f32c(UcJob j) { uc_basins_body<BasF32C>(j); }
+UC_DIRECT(uc_bas_f64, uc_basins_cell<BasF64>)
+UC_DIRECT(uc_bas_f32c, uc_basins_cell<BasF32C>)
+UC_PERSISTENT(uc_bas_f64_q, uc_basins_cell<BasF64>)
+UC_PERSISTENT(uc_bas_f32c_q, uc_basins_cell<BasF32C>)
 
 // ---- attractor catalogue: quantised signature keys in an open-addressing
 // hash set (linear probing, bounded by the table size), then a per-cell
@@ -993,7 +1052,7 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bas_keys(UcJob j, u64*
     if (p < 0) return;
     double pt[UC_LBL];
 #pragma unroll
-    for (int i = 0; i < UC_LBL; ++i) pt[i] = j.out0[(long long)i * j.n_cells + lane];
+    for (int i = 0; i < UC_LBL; ++i) pt[i] = j.out0[(long long)i * j.stride + lane];
     const u64 key = uc_sig_key(p, pt, j.quantum);
     unsigned slot = (unsigned)(uc_hash(key) & (u64)(cap - 1));
     for (int probe = 0; probe < cap; ++probe) {
@@ -1016,7 +1075,7 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bas_labels(UcJob j, co
     if (p < 0) { j.label[lane] = UC_LABEL_UNBOUND; return; }
     double pt[UC_LBL];
 #pragma unroll
-    for (int i = 0; i < UC_LBL; ++i) pt[i] = j.out0[(long long)i * j.n_cells + lane];
+    for (int i = 0; i < UC_LBL; ++i) pt[i] = j.out0[(long long)i * j.stride + lane];
     const u64 key = uc_sig_key(p, pt, j.quantum);
     unsigned slot = (unsigned)(uc_hash(key) & (u64)(cap - 1));
     int label = UC_LABEL_UNBOUND;

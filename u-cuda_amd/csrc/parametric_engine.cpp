@@ -1,0 +1,356 @@
+// parametric_engine.cpp — the C++ facade (see the header) over the C ABI.
+#include "parametric_engine.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <limits>
+#include <map>
+#include <mutex>
+
+#include "../../include/ucuda.h"
+
+namespace ucuda {
+namespace {
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    bool alloc(size_t count) {
+        n = count ? count : 1;
+        return hipMalloc((void**)&p, n * sizeof(T)) == hipSuccess;
+    }
+    bool download(std::vector<T>& out, size_t count) const {
+        out.resize(count);
+        return count == 0 || hipMemcpy(out.data(), p, count * sizeof(T), hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+std::string key_of(const System& s) {
+    std::string k = s.latex ? "L|" : "P|";
+    for (auto& v : s.vars) k += v + ",";
+    k += "|";
+    for (auto& v : s.params) k += v + ",";
+    k += "|";
+    for (auto& v : s.rhs) k += v + ";";
+    return k + "|" + s.time_symbol;
+}
+
+uc_axis make_axis(bool over_h, bool over_var, int var_index, int param_index, int n, double lo, double hi, bool lg) {
+    uc_axis a{};
+    a.kind = over_h ? UC_AXIS_STEP : (over_var ? UC_AXIS_VAR : UC_AXIS_PARAM);
+    a.index = over_var ? var_index : param_index;
+    a.n_pts = n;
+    a.lo = lo;
+    a.hi = hi;
+    a.log_scale = lg ? 1 : 0;
+    return a;
+}
+
+bool cancelled(const std::shared_ptr<std::atomic<bool>>& c) { return c && c->load(std::memory_order_relaxed); }
+void done(const std::shared_ptr<std::atomic<float>>& p) { if (p) p->store(1.0f, std::memory_order_relaxed); }
+
+}  // namespace
+
+struct ParametricEngine::Impl {
+    uc_engine* eng = nullptr;
+    std::mutex mu;
+    std::map<std::string, uc_system*> systems;
+
+    ~Impl() {
+        for (auto& kv : systems) uc_system_destroy(kv.second);
+        if (eng) uc_engine_destroy(eng);
+    }
+    std::string err() const { return eng ? uc_last_error(eng) : "no engine"; }
+    uc_system* system(const System& s, std::string& error) {
+        const std::string k = key_of(s);
+        auto it = systems.find(k);
+        if (it != systems.end()) return it->second;
+        std::vector<const char*> v, p, r;
+        for (auto& x : s.vars) v.push_back(x.c_str());
+        for (auto& x : s.params) p.push_back(x.c_str());
+        for (auto& x : s.rhs) r.push_back(x.c_str());
+        uc_system_desc d{v.data(), (int)v.size(), p.data(), (int)p.size(), r.data(), s.latex ? 1 : 0,
+                         s.time_symbol.empty() ? nullptr : s.time_symbol.c_str()};
+        uc_system* out = nullptr;
+        if (uc_system_create(eng, &d, &out) != UC_OK) { error = err(); return nullptr; }
+        systems[k] = out;
+        return out;
+    }
+    void common(uc_common& c, const std::vector<double>& ic, const std::vector<double>& vals, double h,
+                double transient, double t_max, double max_value, int precision) {
+        c.initial_conditions = ic.data();
+        c.base_values = vals.data();
+        c.n_values = (int)vals.size();
+        c.h = h;
+        c.transient_time = transient;
+        c.t_max = t_max;
+        c.max_value = max_value;
+        c.precision = precision;
+        c.strict = 0;
+        c.ic_seed = 0;
+    }
+};
+
+ParametricEngine::ParametricEngine(int device) : impl_(new Impl) {
+    if (uc_engine_create(device, &impl_->eng) != UC_OK) impl_->eng = nullptr;
+}
+ParametricEngine::~ParametricEngine() = default;
+
+Bifurcation1DResult ParametricEngine::run_bifurcation_1d(const Bifurcation1DRequest& req) {
+    Bifurcation1DResult res;
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    if (!impl_->eng) { res.error = "engine not initialised"; return res; }
+    if (req.continuation || req.use_cpu) { res.error = "continuation / CPU sweeps are not part of this engine"; return res; }
+    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
+    uc_system* s = impl_->system(req.system, res.error);
+    if (!s) return res;
+    uc_bif_request r{};
+    impl_->common(r.c, req.initial_conditions, req.base_values, req.h, req.transient_time, req.t_max, req.max_value,
+                  req.precision);
+    r.c.n_axes = 1;
+    r.c.axis[0] = make_axis(req.sweep_over_h, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts,
+                            req.param_lo, req.param_hi, req.log_scale);
+    r.writable_var = req.writable_var;
+    r.pre_scaller = req.pre_scaller;
+    r.max_peaks = req.max_peaks;
+    const size_t n = (size_t)req.n_pts, mp = (size_t)std::max(1, req.max_peaks);
+    DevBuf<int> flags;
+    DevBuf<double> peaks, ivals;
+    if (!flags.alloc(n) || !peaks.alloc(mp * n) || !ivals.alloc(mp * n)) { res.error = "hipMalloc failed"; return res; }
+    uc_bif_output o{flags.p, peaks.p, ivals.p, nullptr, 0, nullptr};
+    if (uc_run_bif(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    std::vector<int> f;
+    std::vector<double> pk, iv;
+    if (hipDeviceSynchronize() != hipSuccess || !flags.download(f, n) || !peaks.download(pk, mp * n) ||
+        !ivals.download(iv, mp * n)) { res.error = "device error"; return res; }
+    res.param_lo = req.param_lo;
+    res.param_hi = req.param_hi;
+    res.n_pts = req.n_pts;
+    const double hw = req.sweep_over_h ? std::min(req.param_lo, req.param_hi) : req.h;
+    res.record_steps = (int)std::ceil(req.t_max / hw / req.pre_scaller);
+    res.flags = f;
+    res.bifurcation_points.assign(n, {});
+    res.peak_times.assign(n, {});
+    for (size_t c = 0; c < n; ++c)
+        for (int k = 0; k < f[c]; ++k) {
+            res.bifurcation_points[c].push_back(pk[(size_t)k * n + c]);
+            res.peak_times[c].push_back(iv[(size_t)k * n + c]);
+        }
+    done(req.progress);
+    res.ok = true;
+    return res;
+}
+
+namespace {
+template <class I, class Req>
+int lyap_common(I* impl, const Req& req, uc_lyap_request& r, int n_axes) {
+    impl->common(r.c, req.initial_conditions, req.base_values, req.h, req.transient_time, req.t_max, req.max_value,
+                 req.precision);
+    r.c.n_axes = n_axes;
+    r.NT = req.NT;
+    r.eps = req.eps;
+    r.method = req.method;
+    r.probe_seed = 1234567891ULL;
+    return 0;
+}
+}  // namespace
+
+LLE1DResult ParametricEngine::run_lle_1d(const LLE1DRequest& req) {
+    LLE1DResult res;
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    if (!impl_->eng) { res.error = "engine not initialised"; return res; }
+    if (req.continuation || req.use_cpu) { res.error = "continuation / CPU sweeps are not part of this engine"; return res; }
+    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
+    uc_system* s = impl_->system(req.system, res.error);
+    if (!s) return res;
+    uc_lyap_request r{};
+    lyap_common(impl_.get(), req, r, 1);
+    r.c.axis[0] = make_axis(req.sweep_over_h, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts,
+                            req.param_lo, req.param_hi, req.log_scale);
+    const size_t n = (size_t)req.n_pts;
+    DevBuf<double> vals;
+    DevBuf<int> flags;
+    if (!vals.alloc(n) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    uc_lyap_output o{vals.p, flags.p, nullptr};
+    if (uc_run_lle(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    std::vector<double> v;
+    if (hipDeviceSynchronize() != hipSuccess || !vals.download(v, n) || !flags.download(res.flags, n)) {
+        res.error = "device error";
+        return res;
+    }
+    res.n_pts = req.n_pts;
+    res.param_lo = req.param_lo;
+    res.param_hi = req.param_hi;
+    res.lyapunov.resize(n);
+    for (size_t i = 0; i < n; ++i)
+        res.lyapunov[i] = (res.flags[i] == 1) ? v[i] : std::numeric_limits<double>::quiet_NaN();
+    done(req.progress);
+    res.ok = true;
+    return res;
+}
+
+LS1DResult ParametricEngine::run_ls_1d(const LS1DRequest& req) {
+    LS1DResult res;
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    if (!impl_->eng) { res.error = "engine not initialised"; return res; }
+    if (req.continuation || req.use_cpu) { res.error = "continuation / CPU sweeps are not part of this engine"; return res; }
+    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
+    uc_system* s = impl_->system(req.system, res.error);
+    if (!s) return res;
+    uc_lyap_request r{};
+    lyap_common(impl_.get(), req, r, 1);
+    r.c.axis[0] = make_axis(req.sweep_over_h, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts,
+                            req.param_lo, req.param_hi, req.log_scale);
+    const size_t n = (size_t)req.n_pts, N = req.system.vars.size();
+    DevBuf<double> vals;
+    DevBuf<int> flags;
+    if (!vals.alloc(n * N) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    uc_lyap_output o{vals.p, flags.p, nullptr};
+    if (uc_run_ls(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    std::vector<double> v;
+    if (hipDeviceSynchronize() != hipSuccess || !vals.download(v, n * N) || !flags.download(res.flags, n)) {
+        res.error = "device error";
+        return res;
+    }
+    res.n_pts = req.n_pts;
+    res.n_exponents = (int)N;
+    res.param_lo = req.param_lo;
+    res.param_hi = req.param_hi;
+    res.spectrum.assign(n, std::vector<double>(N));
+    for (size_t i = 0; i < n; ++i)
+        for (size_t k = 0; k < N; ++k) res.spectrum[i][k] = v[k * n + i];
+    done(req.progress);
+    res.ok = true;
+    return res;
+}
+
+namespace {
+void axes_2d(const LLE2DRequest& req, uc_common& c) {
+    c.n_axes = 2;
+    c.axis[0] = make_axis(req.sweep_over_h, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts,
+                          req.param_lo, req.param_hi, req.log_scale);
+    c.axis[1] = make_axis(req.sweep_over_h_2, req.sweep_over_var_2, req.var_sweep_index_2, req.param_index_2,
+                          req.n_pts, req.param_lo_2, req.param_hi_2, req.log_scale_2);
+}
+void min_max(const double* v, const int* f, size_t n, double& lo, double& hi) {
+    bool any = false;
+    lo = hi = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        if (f[i] != 1 || !std::isfinite(v[i]) || v[i] == 999.0 || v[i] == -999.0) continue;
+        if (!any) { lo = hi = v[i]; any = true; }
+        lo = std::min(lo, v[i]);
+        hi = std::max(hi, v[i]);
+    }
+}
+}  // namespace
+
+LLE2DResult ParametricEngine::run_lle_2d(const LLE2DRequest& req) {
+    LLE2DResult res;
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    if (!impl_->eng) { res.error = "engine not initialised"; return res; }
+    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
+    uc_system* s = impl_->system(req.system, res.error);
+    if (!s) return res;
+    uc_lyap_request r{};
+    lyap_common(impl_.get(), req, r, 2);
+    axes_2d(req, r.c);
+    const size_t n = (size_t)req.n_pts * (size_t)req.n_pts;
+    DevBuf<double> vals;
+    DevBuf<int> flags;
+    if (!vals.alloc(n) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    uc_lyap_output o{vals.p, flags.p, nullptr};
+    if (uc_run_lle(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    if (hipDeviceSynchronize() != hipSuccess || !vals.download(res.values, n) || !flags.download(res.flags, n)) {
+        res.error = "device error";
+        return res;
+    }
+    for (size_t i = 0; i < n; ++i)
+        if (res.flags[i] != 1) res.values[i] = std::numeric_limits<double>::quiet_NaN();
+    min_max(res.values.data(), res.flags.data(), n, res.min_val, res.max_val);
+    res.n_pts = req.n_pts;
+    res.param_lo = req.param_lo; res.param_hi = req.param_hi;
+    res.param_lo_2 = req.param_lo_2; res.param_hi_2 = req.param_hi_2;
+    done(req.progress);
+    res.ok = true;
+    return res;
+}
+
+LS2DResult ParametricEngine::run_ls_2d(const LS2DRequest& req) {
+    LS2DResult res;
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    if (!impl_->eng) { res.error = "engine not initialised"; return res; }
+    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
+    uc_system* s = impl_->system(req.system, res.error);
+    if (!s) return res;
+    uc_lyap_request r{};
+    lyap_common(impl_.get(), req, r, 2);
+    axes_2d(req, r.c);
+    const size_t n = (size_t)req.n_pts * (size_t)req.n_pts, N = req.system.vars.size();
+    DevBuf<double> vals;
+    DevBuf<int> flags;
+    if (!vals.alloc(n * N) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    uc_lyap_output o{vals.p, flags.p, nullptr};
+    if (uc_run_ls(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    if (hipDeviceSynchronize() != hipSuccess || !vals.download(res.values, n * N) || !flags.download(res.flags, n)) {
+        res.error = "device error";
+        return res;
+    }
+    res.n_pts = req.n_pts;
+    res.n_exponents = (int)N;
+    res.min_val.resize(N);
+    res.max_val.resize(N);
+    for (size_t k = 0; k < N; ++k) min_max(res.values.data() + k * n, res.flags.data(), n, res.min_val[k], res.max_val[k]);
+    res.param_lo = req.param_lo; res.param_hi = req.param_hi;
+    res.param_lo_2 = req.param_lo_2; res.param_hi_2 = req.param_hi_2;
+    done(req.progress);
+    res.ok = true;
+    return res;
+}
+
+BasinsResult ParametricEngine::run_basins(const BasinsRequest& req) {
+    BasinsResult res;
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    if (!impl_->eng) { res.error = "engine not initialised"; return res; }
+    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
+    uc_system* s = impl_->system(req.system, res.error);
+    if (!s) return res;
+    uc_basins_request r{};
+    impl_->common(r.c, req.initial_conditions, req.base_values, req.h, 0.0, 1.0, req.max_value, req.precision);
+    r.c.n_axes = 2;
+    r.c.axis[0] = make_axis(false, true, req.axis_x_var, 0, req.n_pts, req.axis_x_lo, req.axis_x_hi, false);
+    r.c.axis[1] = make_axis(false, true, req.axis_y_var, 0, req.n_pts, req.axis_y_lo, req.axis_y_hi, false);
+    r.rtol = req.rtol;
+    r.atol = req.atol;
+    r.h_min = 1e-12;
+    r.h_max = req.section_period / 4.0;
+    r.section_period = req.section_period;
+    r.n_sections = req.n_sections;
+    r.n_record = 16;
+    r.max_period = 8;
+    r.label_vars = 0;
+    r.period_tol = 1e-4;
+    r.quantum = 1e-3;
+    const size_t n = (size_t)req.n_pts * (size_t)req.n_pts;
+    DevBuf<int> labels, periods;
+    if (!labels.alloc(n) || !periods.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    int n_att = 0;
+    uc_basins_output o{labels.p, periods.p, nullptr, nullptr, 0, &n_att, nullptr, nullptr};
+    if (uc_run_basins(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    if (!labels.download(res.basin_idx, n) || !periods.download(res.periods, n)) { res.error = "device error"; return res; }
+    res.helpful_array.resize(n);
+    for (size_t i = 0; i < n; ++i) res.helpful_array[i] = res.periods[i] >= 0 ? 1 : 0;
+    res.n_clusters = n_att;
+    res.min_cluster_idx = 0;
+    res.n_pts = req.n_pts;
+    res.axis_x_lo = req.axis_x_lo; res.axis_x_hi = req.axis_x_hi;
+    res.axis_y_lo = req.axis_y_lo; res.axis_y_hi = req.axis_y_hi;
+    res.axis_x_var = req.axis_x_var; res.axis_y_var = req.axis_y_var;
+    done(req.progress);
+    res.ok = true;
+    return res;
+}
+
+}  // namespace ucuda

@@ -1,0 +1,181 @@
+// parametric_engine.hpp — drop-in C++ facade over the C ABI.
+//
+// Same class, method and field names as the reference facade
+// (U-CUDA/parametric_engine.h:27-126, 211-482, 581-723, 844-889) so a session
+// layer written against it keeps compiling: Request in, Result by value, no
+// exceptions (errors in Result::error), cancel token checked before launch,
+// progress set to 1 on completion. Additions: `system` (the engine needs the
+// right-hand side, not only a step body), `precision`, `method`,
+// `max_peaks`. Results use the reference's sentinels: LLE 999 -> NaN with
+// flag -1; LS keeps 999 with flag -1; bifurcation flags are the peak count,
+// -1 (fixed point / bad step) or 0 (diverged); 2-D maps are row-major
+// values[iy * n + ix], LS planes values[k * n * n + iy * n + ix].
+#pragma once
+#include <atomic>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "codegen.hpp"
+
+namespace ucuda {
+
+struct Bifurcation1DRequest {
+    System system;
+    std::string krs_body;  // informational: the engine integrates `system` with RK4
+    int amountOfX = 0;
+    std::vector<double> initial_conditions;
+    std::vector<double> base_values;
+    int param_index = 0;
+    bool sweep_over_var = false;
+    int var_sweep_index = 0;
+    bool sweep_over_h = false;
+    bool log_scale = false;
+    bool continuation = false;
+    bool continuation_reverse = false;
+    bool use_cpu = false;
+    double param_lo = 0.0, param_hi = 1.0;
+    int n_pts = 1000;
+    int writable_var = 0;
+    double h = 0.01, transient_time = 0.0, t_max = 0.0;
+    int pre_scaller = 1;
+    double max_value = 1.0e6;
+    int max_peaks = 2500;
+    int precision = 0;  // UC_PREC_F64 / UC_PREC_F32C
+    std::shared_ptr<std::atomic<bool>> cancel;
+    std::shared_ptr<std::atomic<float>> progress;
+};
+
+struct Bifurcation1DResult {
+    bool ok = false, cancelled = false;
+    std::string error;
+    double param_lo = 0.0, param_hi = 1.0;
+    int n_pts = 0, record_steps = 0;
+    std::vector<std::vector<double>> bifurcation_points;
+    std::vector<std::vector<double>> peak_times;
+    std::vector<int> flags;
+};
+
+struct LLE1DRequest {
+    System system;
+    int amountOfX = 0;
+    std::vector<double> initial_conditions, base_values;
+    bool sweep_over_var = false;
+    int var_sweep_index = 0;
+    bool sweep_over_h = false, log_scale = false;
+    int param_index = 0;
+    bool continuation = false, continuation_reverse = false, use_cpu = false;
+    double param_lo = 0.0, param_hi = 1.0;
+    int n_pts = 500;
+    double h = 0.01, transient_time = 0.0, t_max = 100.0, NT = 1.0, eps = 1.0e-4, max_value = 1.0e6;
+    int precision = 0, method = 0;
+    std::shared_ptr<std::atomic<bool>> cancel;
+    std::shared_ptr<std::atomic<float>> progress;
+};
+
+struct LLE1DResult {
+    bool ok = false, cancelled = false;
+    std::string error;
+    int n_pts = 0;
+    double param_lo = 0.0, param_hi = 1.0;
+    std::vector<double> lyapunov;  // NaN where the point diverged
+    std::vector<int> flags;        // 1 ok, -1 diverged
+};
+
+using LS1DRequest = LLE1DRequest;
+struct LS1DResult {
+    bool ok = false, cancelled = false;
+    std::string error;
+    int n_pts = 0, n_exponents = 0;
+    double param_lo = 0.0, param_hi = 1.0;
+    std::vector<std::vector<double>> spectrum;  // 999 where the point diverged
+    std::vector<int> flags;
+};
+
+struct LLE2DRequest {
+    System system;
+    int amountOfX = 0;
+    std::vector<double> initial_conditions, base_values;
+    bool sweep_over_var = false, sweep_over_var_2 = false;
+    bool sweep_over_h = false, sweep_over_h_2 = false;
+    bool log_scale = false, log_scale_2 = false;
+    int param_index = 0, var_sweep_index = 0, param_index_2 = 0, var_sweep_index_2 = 0;
+    double param_lo = 0.0, param_hi = 1.0, param_lo_2 = 0.0, param_hi_2 = 1.0;
+    int n_pts = 200;
+    double h = 0.01, transient_time = 0.0, t_max = 100.0, NT = 1.0, eps = 1.0e-4, max_value = 1.0e6;
+    int precision = 0, method = 0;
+    std::shared_ptr<std::atomic<bool>> cancel;
+    std::shared_ptr<std::atomic<float>> progress;
+};
+
+struct LLE2DResult {
+    bool ok = false, cancelled = false;
+    std::string error;
+    int n_pts = 0;
+    double param_lo = 0.0, param_hi = 1.0, param_lo_2 = 0.0, param_hi_2 = 1.0;
+    std::vector<double> values;  // [iy * n + ix], NaN where diverged
+    std::vector<int> flags;
+    double min_val = 0.0, max_val = 0.0;
+};
+
+using LS2DRequest = LLE2DRequest;
+struct LS2DResult {
+    bool ok = false, cancelled = false;
+    std::string error;
+    int n_pts = 0, n_exponents = 0;
+    double param_lo = 0.0, param_hi = 1.0, param_lo_2 = 0.0, param_hi_2 = 1.0;
+    std::vector<double> values;  // [k * n * n + iy * n + ix]
+    std::vector<int> flags;
+    std::vector<double> min_val, max_val;
+};
+
+struct BasinsRequest {
+    System system;
+    int amountOfX = 0;
+    std::vector<double> initial_conditions, base_values;
+    int axis_x_var = 0, axis_y_var = 1;
+    double axis_x_lo = -10.0, axis_x_hi = 10.0, axis_y_lo = -10.0, axis_y_hi = 10.0;
+    int n_pts = 200;
+    double h = 0.01;                 // first step of the adaptive integrator
+    double rtol = 1e-9, atol = 1e-9;
+    double section_period = 6.283185307179586;
+    int n_sections = 200;
+    double max_value = 1.0e6;
+    int precision = 0;
+    std::shared_ptr<std::atomic<bool>> cancel;
+    std::shared_ptr<std::atomic<float>> progress;
+};
+
+struct BasinsResult {
+    bool ok = false, cancelled = false;
+    std::string error;
+    int n_pts = 0;
+    double axis_x_lo = 0.0, axis_x_hi = 0.0, axis_y_lo = 0.0, axis_y_hi = 0.0;
+    int axis_x_var = 0, axis_y_var = 1;
+    std::vector<int> basin_idx;      // >= 1 attractor (canonical order), 0 unbound
+    std::vector<int> helpful_array;  // 1 settled on an attractor, 0 unbound / unresolved
+    std::vector<int> periods;        // detected Poincare period per cell
+    int n_clusters = 0;
+    int min_cluster_idx = 0;
+};
+
+class ParametricEngine {
+public:
+    explicit ParametricEngine(int device = 0);
+    ~ParametricEngine();
+    ParametricEngine(const ParametricEngine&) = delete;
+    ParametricEngine& operator=(const ParametricEngine&) = delete;
+
+    Bifurcation1DResult run_bifurcation_1d(const Bifurcation1DRequest& req);
+    LLE1DResult run_lle_1d(const LLE1DRequest& req);
+    LS1DResult run_ls_1d(const LS1DRequest& req);
+    LLE2DResult run_lle_2d(const LLE2DRequest& req);
+    LS2DResult run_ls_2d(const LS2DRequest& req);
+    BasinsResult run_basins(const BasinsRequest& req);
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
+
+}  // namespace ucuda

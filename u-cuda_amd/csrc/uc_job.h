@@ -23,8 +23,9 @@
 
 typedef struct UcJob {
     /* ---- grid ---- */
-    long long n_cells;            /* cells handled by this launch             */
-    long long cell_offset;        /* global index of the launch's first cell  */
+    long long n_cells;            /* cells handled by this entry              */
+    long long cell_offset;        /* global index of the entry's first cell   */
+    long long stride;             /* plane stride of the SoA outputs          */
     int nx, ny;                   /* points on axis 0 / axis 1 (ny = 1: 1-D)  */
     int axis_kind[2];             /* UC_AXIS_*                                */
     int axis_index[2];

@@ -105,6 +105,17 @@ _SIGNATURES = {
                                 C.c_void_p]),
     "uc_peak_finder": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong, C.c_double,
                                  C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uc_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "uc_batch_destroy": (None, [C.c_void_p]),
+    "uc_batch_add_bif": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.POINTER(BifOutput),
+                                   C.POINTER(C.c_int)]),
+    "uc_batch_add_lyap": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.c_int, C.POINTER(LyapOutput),
+                                    C.POINTER(C.c_int)]),
+    "uc_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "uc_batch_tune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_double)]),
+    "uc_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "uc_batch_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 4),
     "uc_grid_cells": (C.c_longlong, [C.POINTER(Common)]),
     "uc_axis_values": (C.c_int, [C.POINTER(Axis), C.POINTER(C.c_double)]),
 }
@@ -428,6 +439,89 @@ def peak_finder(sysm, data, time_step, max_peaks=2500, stream=None):
                                    _ptr(peaks), _ptr(intervals), _stream(stream)), "uc_peak_finder")
     torch.cuda.synchronize(dev)
     return dict(counts=counts, peaks=peaks, intervals=intervals)
+
+
+class Batch:
+    """Persistent heterogeneous batch (uc_batch): add jobs, tune, launch the
+    captured graph. Outputs are torch tensors allocated at add time."""
+
+    def __init__(self, engine):
+        self.engine = engine
+        h = C.c_void_p()
+        engine.check(lib().uc_batch_create(engine.h, C.byref(h)), "uc_batch_create")
+        self.h = h
+        self.jobs = []
+        self._keep = []
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().uc_batch_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def add_lyap(self, sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=False, eps=1e-4,
+                 method=METHOD_VARIATIONAL, probe_seed=REFERENCE_PROBE_SEED, max_value=1e6, precision=PREC_F64,
+                 strict=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0):
+        import torch
+        n = n_cells(axes)
+        k = sysm.n_vars if spectrum else 1
+        dev = torch.device("cuda", self.engine.device)
+        out = dict(values=torch.empty((k, n), dtype=torch.float64, device=dev),
+                   flags=torch.empty(n, dtype=torch.int32, device=dev))
+        r = LyapRequest()
+        r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
+                            ic_seed, ic_lo, ic_hi)
+        r.NT, r.eps, r.method, r.probe_seed = float(NT), float(eps), int(method), int(probe_seed)
+        o = LyapOutput(_ptr(out["values"]).value, _ptr(out["flags"]).value, None)
+        jid = C.c_int()
+        self.engine.check(lib().uc_batch_add_lyap(self.h, sysm.h, C.byref(r), 1 if spectrum else 0, C.byref(o),
+                                                  C.byref(jid)), "uc_batch_add_lyap")
+        self._keep.append((keep, sysm))
+        self.jobs.append(out)
+        return out
+
+    def add_bif(self, sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pre_scaller=1,
+                max_peaks=64, max_value=1e6, precision=PREC_F64, strict=False):
+        import torch
+        n = n_cells(axes)
+        dev = torch.device("cuda", self.engine.device)
+        out = dict(flags=torch.empty(n, dtype=torch.int32, device=dev),
+                   peaks=torch.empty((max(1, max_peaks), n), dtype=torch.float64, device=dev),
+                   intervals=torch.empty((max(1, max_peaks), n), dtype=torch.float64, device=dev))
+        r = BifRequest()
+        r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict)
+        r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, max_peaks
+        o = BifOutput(_ptr(out["flags"]).value, _ptr(out["peaks"]).value, _ptr(out["intervals"]).value, None, 0, None)
+        jid = C.c_int()
+        self.engine.check(lib().uc_batch_add_bif(self.h, sysm.h, C.byref(r), C.byref(o), C.byref(jid)),
+                          "uc_batch_add_bif")
+        self._keep.append((keep, sysm))
+        self.jobs.append(out)
+        return out
+
+    def set_launch(self, block, waves_per_simd):
+        self.engine.check(lib().uc_batch_set_launch(self.h, block, waves_per_simd), "uc_batch_set_launch")
+
+    def tune(self, stream=None):
+        b, w, ms = C.c_int(), C.c_int(), C.c_double()
+        self.engine.check(lib().uc_batch_tune(self.h, _stream(stream), C.byref(b), C.byref(w), C.byref(ms)),
+                          "uc_batch_tune")
+        return dict(block=b.value, waves_per_simd=w.value, ms=ms.value)
+
+    def launch(self, stream=None, sync=True):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        self.engine.check(lib().uc_batch_launch(self.h, _stream(stream)), "uc_batch_launch")
+        if sync:
+            torch.cuda.synchronize()
+
+    def info(self):
+        v = [C.c_int() for _ in range(4)]
+        self.engine.check(lib().uc_batch_info(self.h, *[C.byref(x) for x in v]), "uc_batch_info")
+        return dict(zip(("n_jobs", "n_groups", "block", "waves_per_simd"), [x.value for x in v]))
 
 
 def rk4_flops(ops, n, tangents=0):
