@@ -219,6 +219,37 @@ int uc_batch_tune(uc_batch* b, void* stream, int* best_block, int* best_waves_pe
 int uc_batch_launch(uc_batch* b, void* stream);
 int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int* waves_per_simd);
 
+/* ---- serving: continuous batching ----
+ * A scheduler thread admits jobs in arrival order while their outputs fit
+ * the HBM budget, then runs rounds over all admitted jobs: each round takes
+ * chunks of cells from every job (oldest first) up to a traj-step budget and
+ * launches the persistent kernels once per (system, analysis) group; jobs
+ * submitted meanwhile join the next round. Results are bit-identical to a
+ * direct uc_run_* of the same request. Outputs stay on the device until
+ * uc_server_fetch copies them out. */
+typedef struct uc_server uc_server;
+typedef struct uc_server_config {
+    double hbm_fraction;       /* admission budget as a fraction of free HBM (0 -> 0.5) */
+    long long max_bytes;       /* explicit budget in bytes (overrides hbm_fraction if > 0) */
+    double round_traj_steps;   /* work per round in trajectory-steps (0 -> 2e9)         */
+    int block;                 /* lanes per workgroup (0 -> 256)                          */
+    int waves_per_simd;        /* resident waves per SIMD (0 -> 4)                        */
+} uc_server_config;
+typedef struct uc_server_stats_t {
+    long long completed, rounds, queued, active;
+    double p50_ms, p99_ms, max_ms, busy_s, wall_s, traj_steps, used_bytes, budget_bytes;
+} uc_server_stats_t;
+int uc_server_create(uc_engine* e, const uc_server_config* cfg, uc_server** out);
+void uc_server_destroy(uc_server* s);
+int uc_server_submit_lyap(uc_server* s, uc_system* sys, const uc_lyap_request* r, int spectrum, long long* job_id);
+int uc_server_submit_bif(uc_server* s, uc_system* sys, const uc_bif_request* r, long long* job_id);
+/* state: 0 queued, 1 running, 2 done, 3 cancelled, 4 failed, 5 fetched */
+int uc_server_poll(uc_server* s, long long job_id, int* state, double* latency_ms);
+int uc_server_wait(uc_server* s, long long job_id, double timeout_s);
+int uc_server_cancel(uc_server* s, long long job_id);
+int uc_server_fetch(uc_server* s, long long job_id, int* flags, double* out0, double* out1);
+int uc_server_stats(uc_server* s, uc_server_stats_t* stats);
+
 /* Number of cells of a request's grid (product of axis point counts). */
 long long uc_grid_cells(const uc_common* c);
 /* Axis values exactly as the kernels use them (host, n_pts doubles). */

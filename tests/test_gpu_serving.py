@@ -1,0 +1,98 @@
+"""Continuous-batching server: jobs submitted while others run are admitted
+into later rounds, share rounds with them, and every result is bit-identical
+to the same request run alone; cancellation, admission by memory budget and
+latency statistics."""
+import time
+
+import numpy as np
+import pytest
+
+import ucuda
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix(eng):
+    ros, lor, hr = ucuda.preset("rossler", eng), ucuda.preset("lorenz", eng), ucuda.preset("hyper_rossler", eng)
+    jobs = []
+    for i in range(6):
+        n = 16 + 13 * i
+        jobs.append(("lyap", ros, dict(axes=[ucuda.axis(1, 2, n, 0.05, 2.0), ucuda.axis(1, 3, n, 1.0, 20.0)],
+                                       ic=[1, 1, 1], values=ros.values(), h=0.01, transient_time=5.0 + i,
+                                       t_max=20.0 + 10 * i, NT=0.1,
+                                       precision=ucuda.PREC_F32C if i % 2 else ucuda.PREC_F64)))
+    jobs.append(("bif", lor, dict(axes=[ucuda.axis(1, 2, 333, 0.0, 250.0)], ic=[0.1, 0.1, 0.1], values=lor.values(),
+                                  h=0.01, transient_time=20.0, t_max=20.0, max_peaks=32)))
+    jobs.append(("lyap", hr, dict(axes=[ucuda.axis(1, 2, 9, 0.0, 0.1), ucuda.axis(1, 1, 7, 0.2, 0.3)], ic=None,
+                                  values=hr.values(), h=0.005, transient_time=2.0, t_max=10.0, NT=0.1, spectrum=True,
+                                  ic_seed=20261203)))
+    return jobs
+
+
+def _submit(srv, kind, sysm, kw):
+    kw = dict(kw)
+    args = (sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), kw.pop("h"), kw.pop("transient_time"),
+            kw.pop("t_max"))
+    if kind == "lyap":
+        return srv.submit_lyap(*args, kw.pop("NT"), **kw)
+    return srv.submit_bif(*args, **kw)
+
+
+def _alone(kind, sysm, kw):
+    kw = dict(kw)
+    args = (sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), kw.pop("h"), kw.pop("transient_time"),
+            kw.pop("t_max"))
+    if kind == "lyap":
+        out = ucuda.run_lyap(*args, kw.pop("NT"), **kw)
+        return dict(values=out["values"].cpu().numpy(), flags=out["flags"].cpu().numpy())
+    out = ucuda.run_bif(*args, **kw)
+    return dict(flags=out["flags"].cpu().numpy(), peaks=out["peaks"].cpu().numpy(),
+                intervals=out["intervals"].cpu().numpy())
+
+
+def test_results_identical_alone_and_in_the_mix(gpu_engine):
+    print("case: 8 staggered submissions (lle f64/f32c, bif, ls) with small rounds", flush=True)
+    jobs = _mix(gpu_engine)
+    alone = [_alone(k, s, kw) for k, s, kw in jobs]
+    srv = ucuda.Server(gpu_engine, round_traj_steps=2e7)
+    ids = []
+    for i, (k, s, kw) in enumerate(jobs):
+        ids.append(_submit(srv, k, s, kw))
+        if i % 3 == 2:
+            time.sleep(0.01)  # later jobs arrive while earlier ones are running
+    for jid in ids:
+        srv.wait(jid, 120.0)
+    st = srv.stats()
+    assert st["completed"] == len(jobs) and st["rounds"] > 1
+    assert st["p50_ms"] > 0 and st["p99_ms"] >= st["p50_ms"]
+    for (k, s, kw), jid, ref in zip(jobs, ids, alone):
+        got = srv.fetch(jid)
+        if k == "lyap":
+            np.testing.assert_array_equal(got["flags"], ref["flags"])
+            np.testing.assert_array_equal(got["values"], ref["values"])
+        else:
+            np.testing.assert_array_equal(got["flags"], ref["flags"])
+            for c in range(len(ref["flags"])):
+                m = ref["flags"][c]
+                if m > 0:
+                    np.testing.assert_array_equal(got["peaks"][:m, c], ref["peaks"][:m, c])
+                    np.testing.assert_array_equal(got["intervals"][:m, c], ref["intervals"][:m, c])
+    srv.close()
+
+
+def test_admission_budget_and_cancel(gpu_engine):
+    print("case: tiny HBM budget admits one job at a time; cancel a queued job", flush=True)
+    ros = ucuda.preset("rossler", gpu_engine)
+    srv = ucuda.Server(gpu_engine, max_bytes=300000, round_traj_steps=1e7)
+    axes = [ucuda.axis(1, 2, 100, 0.05, 2.0), ucuda.axis(1, 3, 100, 1.0, 20.0)]  # 10k cells = 120 kB each
+    ids = [srv.submit_lyap(ros, axes, [1, 1, 1], ros.values(), 0.01, 5.0, 20.0, 0.1) for _ in range(4)]
+    srv.cancel(ids[3])
+    for jid in ids[:3]:
+        srv.wait(jid, 120.0)
+    assert srv.poll(ids[3])[0] == 3
+    st = srv.stats()
+    assert st["completed"] == 3 and st["used_bytes"] <= 300000 * 3
+    a = srv.fetch(ids[0])["values"]
+    b = srv.fetch(ids[1])["values"]
+    np.testing.assert_array_equal(a, b)
+    srv.close()

@@ -17,6 +17,10 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <thread>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -229,7 +233,7 @@ size_t table_doubles(const uc_common& c) {
 
 // Host image of a launch's input tables: axis values (nx, then ny), 64
 // parameter slots, 32 initial-state slots.
-void fill_tables(const uc_system* s, const uc_common& c, double* host) {
+void fill_tables(int n_vars, const uc_common& c, double* host) {
     const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
     std::fill(host, host + table_doubles(c), 0.0);
     double* ax1 = host + nx;
@@ -238,7 +242,7 @@ void fill_tables(const uc_system* s, const uc_common& c, double* host) {
     axis_values(c.axis[0], host);
     if (c.n_axes > 1) axis_values(c.axis[1], ax1);
     for (int i = 0; i < c.n_values; ++i) par[i] = c.base_values[i];
-    for (int i = 0; i < s->n_vars; ++i) ic[i] = c.initial_conditions ? c.initial_conditions[i] : 0.0;
+    for (int i = 0; i < n_vars; ++i) ic[i] = c.initial_conditions ? c.initial_conditions[i] : 0.0;
 }
 
 // Descriptor fields common to every analysis, pointing at device tables.
@@ -292,7 +296,7 @@ int stage_tables(uc_engine* e, const uc_system* s, const uc_common& c, UcJob& j,
             return e->fail(UC_ERR_HIP, hip_msg(he, "table buffer allocation"));
         }
     }
-    fill_tables(s, c, tb->host);
+    fill_tables(s->n_vars, c, tb->host);
     hipError_t he = hipMemcpyAsync(tb->dev, tb->host, need * sizeof(double), hipMemcpyHostToDevice, stream);
     if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipMemcpyAsync(tables)"));
     wire_job(c, tb->dev, j);
@@ -778,7 +782,7 @@ namespace {
 int batch_stage(uc_batch* b, const uc_system* s, const uc_common& c, UcJob& j) {
     const size_t need = table_doubles(c);
     std::vector<double> host(need);
-    fill_tables(s, c, host.data());
+    fill_tables(s->n_vars, c, host.data());
     double* dev = nullptr;
     hipError_t he = hipMalloc((void**)&dev, need * sizeof(double));
     if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(batch tables)"));
@@ -1025,6 +1029,443 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
     }
     if (block) *block = b->block ? b->block : 256;
     if (waves_per_simd) *waves_per_simd = b->waves_per_simd ? b->waves_per_simd : 4;
+    return UC_OK;
+}
+
+}  // extern "C"
+
+// =====================================================================
+// Serving: continuous batching. A scheduler thread admits submitted jobs
+// in arrival order while their output buffers fit in the HBM budget,
+// splits every admitted job into chunks of cells and runs rounds: each
+// round takes chunks from all admitted jobs (oldest first, at least one
+// chunk per job, up to a traj-step budget), launches one persistent kernel
+// per (module, kernel) group over the round's job table, and waits for it.
+// Jobs that arrive while a round runs join the next round. A job's cells
+// compute exactly as in a direct launch (same kernel code per cell; the
+// output plane stride stays the job's own), so results do not depend on
+// what else shares the round. Outputs stay in device memory until fetched.
+struct uc_server {
+    uc_engine* e = nullptr;
+    uc_server_config cfg{};
+    struct Job {
+        long long id = 0;
+        int kind = 0;  // 0 bif, 1 lle, 2 ls
+        Module* m = nullptr;
+        std::string kernel;
+        UcJob tmpl{};
+        double* tables = nullptr;
+        // owned device outputs
+        int* flags = nullptr;
+        double* out0 = nullptr;
+        double* out1 = nullptr;
+        size_t out0_n = 0, out1_n = 0;
+        size_t bytes = 0;
+        long long cells = 0, next = 0, done_cells = 0;
+        double step_cost = 1.0;  // traj-steps per cell
+        int state = 0;           // 0 queued, 1 running, 2 done, 3 cancelled, 4 failed
+        std::chrono::steady_clock::time_point t_submit, t_done;
+        // request copies (pointers inside refer to these)
+        std::vector<double> ic, vals;
+        uc_common c{};
+        uc_bif_request bif{};
+        uc_lyap_request lyap{};
+        int n_exp = 1, max_peaks = 0, n_vars = 0;
+        long long iters = 0;
+        std::string error;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<std::shared_ptr<Job>> pending;
+    std::vector<std::shared_ptr<Job>> active;
+    std::map<long long, std::shared_ptr<Job>> all;
+    long long next_id = 1;
+    size_t used_bytes = 0, budget = 0;
+    bool stop = false;
+    std::thread worker;
+    hipStream_t stream = nullptr;
+    std::vector<double> latencies_ms;
+    double busy_s = 0.0, traj_steps_done = 0.0;
+    long long rounds = 0;
+    std::chrono::steady_clock::time_point t_start;
+    std::string error;
+};
+
+namespace {
+
+void server_free(uc_server::Job& j) {
+    if (j.flags) (void)hipFree(j.flags);
+    if (j.out0) (void)hipFree(j.out0);
+    if (j.out1) (void)hipFree(j.out1);
+    if (j.tables) (void)hipFree(j.tables);
+    j.flags = nullptr; j.out0 = j.out1 = nullptr; j.tables = nullptr;
+}
+
+// admission: allocate outputs + tables, build the descriptor template
+bool server_admit(uc_server* S, uc_server::Job& j) {
+    const size_t need = table_doubles(j.c);
+    std::vector<double> host(need);
+    fill_tables(j.n_vars, j.c, host.data());
+    hipError_t he = hipMalloc((void**)&j.tables, need * sizeof(double));
+    if (he == hipSuccess) he = hipMemcpy(j.tables, host.data(), need * sizeof(double), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMalloc((void**)&j.flags, sizeof(int) * (size_t)j.cells);
+    if (he == hipSuccess) he = hipMalloc((void**)&j.out0, sizeof(double) * std::max<size_t>(1, j.out0_n));
+    if (he == hipSuccess && j.out1_n) he = hipMalloc((void**)&j.out1, sizeof(double) * j.out1_n);
+    if (he != hipSuccess) { server_free(j); j.error = hip_msg(he, "admission"); return false; }
+    wire_job(j.c, j.tables, j.tmpl);
+    if (j.kind == 0) {
+        uc_bif_output o{j.flags, j.out0, j.out1, nullptr, 0, nullptr};
+        fill_bif(&j.bif, &o, j.iters, j.tmpl);
+    } else {
+        uc_lyap_output o{j.out0, j.flags, nullptr};
+        fill_lyap(&j.lyap, &o, j.tmpl);
+    }
+    (void)S;
+    return true;
+}
+
+void server_loop(uc_server* S) {
+    (void)hipSetDevice(S->e->device);
+    using clock = std::chrono::steady_clock;
+    std::vector<unsigned long long*> counters;
+    unsigned long long* d_counters = nullptr;
+    (void)hipMalloc((void**)&d_counters, sizeof(unsigned long long) * 64);
+    for (;;) {
+        std::vector<std::shared_ptr<uc_server::Job>> round_jobs;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            // admission in arrival order while the outputs fit
+            while (!S->pending.empty()) {
+                auto j = S->pending.front();
+                if (j->state == 3) { S->pending.pop_front(); continue; }
+                if (S->used_bytes + j->bytes > S->budget && !S->active.empty()) break;
+                S->pending.pop_front();
+                if (!server_admit(S, *j)) { j->state = 4; j->t_done = clock::now(); continue; }
+                S->used_bytes += j->bytes;
+                j->state = 1;
+                S->active.push_back(j);
+            }
+            if (S->active.empty()) {
+                if (S->stop) break;
+                S->cv.wait_for(lk, std::chrono::milliseconds(20));
+                continue;
+            }
+            round_jobs = S->active;
+        }
+        // ---- build the round: chunk allocation, oldest first ----
+        const int block = S->cfg.block > 0 ? S->cfg.block : 256;
+        const double budget_steps = S->cfg.round_traj_steps > 0 ? S->cfg.round_traj_steps : 2e9;
+        struct Entry { UcJob j; uc_server::Job* job; long long cells; };
+        std::vector<Entry> entries;
+        double spent = 0.0;
+        bool progress = true;
+        std::map<uc_server::Job*, long long> taken;
+        while (progress && spent < budget_steps) {
+            progress = false;
+            for (auto& jp : round_jobs) {
+                auto& J = *jp;
+                const long long pos = J.next + taken[&J];
+                if (J.state != 1 || pos >= J.cells) continue;
+                const long long chunk = std::max<long long>(block, (long long)(budget_steps / 8 / std::max(1.0, J.step_cost)) / block * block);
+                const long long n = std::min(chunk, J.cells - pos);
+                taken[&J] += n;
+                spent += (double)n * J.step_cost;
+                progress = true;
+                if (spent >= budget_steps) break;
+            }
+        }
+        for (auto& jp : round_jobs) {
+            auto& J = *jp;
+            const long long n = taken[&J];
+            if (n <= 0) continue;
+            UcJob e = J.tmpl;
+            e.cell_offset = J.next;
+            e.n_cells = n;
+            e.stride = J.cells;
+            e.flag = J.flags + J.next;
+            e.out0 = J.out0 + J.next;
+            e.out1 = J.out1 ? J.out1 + J.next : nullptr;
+            entries.push_back({e, &J, n});
+        }
+        // ---- group by kernel, upload tables, launch, wait ----
+        std::map<std::pair<Module*, std::string>, std::vector<Entry*>> groups;
+        for (auto& en : entries) groups[std::make_pair(en.job->m, en.job->kernel)].push_back(&en);
+        const auto t0 = clock::now();
+        std::vector<void*> temps;
+        int gidx = 0;
+        bool failed = false;
+        std::string why;
+        {
+            hipFunction_t freset;
+            int ng = (int)groups.size();
+            void* rargs[] = {&d_counters, &ng};
+            if (hipModuleGetFunction(&freset, entries[0].job->m->mod, "uc_batch_reset") != hipSuccess ||
+                hipModuleLaunchKernel(freset, 1, 1, 1, 64, 1, 1, 0, S->stream, rargs, nullptr) != hipSuccess) {
+                failed = true; why = "reset";
+            }
+        }
+        const int cus = cu_count(S->e->device);
+        const int wps = S->cfg.waves_per_simd > 0 ? S->cfg.waves_per_simd : 4;
+        for (auto& kv : groups) {
+            if (failed) break;
+            std::vector<UcJob> table;
+            std::vector<long long> start(1, 0);
+            for (Entry* en : kv.second) {
+                table.push_back(en->j);
+                start.push_back(start.back() + (en->cells + block - 1) / block);
+            }
+            UcJob* d_jobs = nullptr;
+            long long* d_start = nullptr;
+            hipError_t he = hipMalloc((void**)&d_jobs, sizeof(UcJob) * table.size());
+            if (he == hipSuccess) { temps.push_back(d_jobs); he = hipMalloc((void**)&d_start, sizeof(long long) * start.size()); }
+            if (he == hipSuccess) { temps.push_back(d_start); he = hipMemcpyAsync(d_jobs, table.data(), sizeof(UcJob) * table.size(), hipMemcpyHostToDevice, S->stream); }
+            if (he == hipSuccess) he = hipMemcpyAsync(d_start, start.data(), sizeof(long long) * start.size(), hipMemcpyHostToDevice, S->stream);
+            hipFunction_t f;
+            if (he == hipSuccess) he = hipModuleGetFunction(&f, kv.first.first->mod, kv.first.second.c_str());
+            int nj = (int)table.size();
+            unsigned long long* ctr = d_counters + gidx;
+            void* args[] = {&d_jobs, &d_start, &nj, &ctr};
+            const long long grid = std::min<long long>((long long)cus * 4 * wps / (block / 64), std::max<long long>(1, start.back()));
+            if (he == hipSuccess) he = hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, 0, S->stream, args, nullptr);
+            if (he != hipSuccess) { failed = true; why = hip_msg(he, "server round"); }
+            ++gidx;
+        }
+        hipError_t he = hipStreamSynchronize(S->stream);
+        if (he != hipSuccess) { failed = true; why = hip_msg(he, "server round sync"); }
+        for (void* p : temps) (void)hipFree(p);
+        const auto t1 = clock::now();
+        // ---- bookkeeping ----
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->busy_s += std::chrono::duration<double>(t1 - t0).count();
+        ++S->rounds;
+        for (auto& en : entries) {
+            auto& J = *en.job;
+            if (failed) { J.state = 4; J.error = why; J.t_done = t1; continue; }
+            J.next += en.cells;
+            S->traj_steps_done += (double)en.cells * J.step_cost;
+            if (J.next >= J.cells && J.state == 1) {
+                J.state = 2;
+                J.t_done = t1;
+                S->latencies_ms.push_back(std::chrono::duration<double, std::milli>(J.t_done - J.t_submit).count());
+            }
+        }
+        std::vector<std::shared_ptr<uc_server::Job>> still;
+        for (auto& jp : S->active) {
+            if (jp->state == 1) still.push_back(jp);
+            else if (jp->state != 2) { S->used_bytes -= jp->bytes; server_free(*jp); }
+        }
+        S->active.swap(still);
+        S->cv.notify_all();
+    }
+    if (d_counters) (void)hipFree(d_counters);
+}
+
+int server_submit(uc_server* S, std::shared_ptr<uc_server::Job> j, long long* id) {
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (S->stop) return S->e->fail(UC_ERR_ARGUMENT, "server stopped");
+        j->id = S->next_id++;
+        j->t_submit = std::chrono::steady_clock::now();
+        S->all[j->id] = j;
+        S->pending.push_back(j);
+    }
+    S->cv.notify_all();
+    if (id) *id = j->id;
+    return UC_OK;
+}
+
+// copy the request so the job does not depend on caller memory
+void server_copy_common(uc_server::Job& j, const uc_common& c, int n_vars) {
+    j.c = c;
+    j.n_vars = n_vars;
+    j.ic.assign(c.initial_conditions ? c.initial_conditions : nullptr,
+                c.initial_conditions ? c.initial_conditions + n_vars : nullptr);
+    j.vals.assign(c.base_values, c.base_values + c.n_values);
+    j.c.initial_conditions = j.ic.empty() ? nullptr : j.ic.data();
+    j.c.base_values = j.vals.data();
+    j.cells = cells_of(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int uc_server_create(uc_engine* e, const uc_server_config* cfg, uc_server** out) {
+    if (!e || !out) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    int rc = enter(e);
+    if (rc) return rc;
+    auto S = new uc_server;
+    S->e = e;
+    if (cfg) S->cfg = *cfg;
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    const double frac = S->cfg.hbm_fraction > 0.0 ? S->cfg.hbm_fraction : 0.5;
+    S->budget = S->cfg.max_bytes > 0 ? (size_t)S->cfg.max_bytes : (size_t)(frac * (double)free_b);
+    hipError_t he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) { delete S; return e->fail(UC_ERR_HIP, hip_msg(he, "server stream")); }
+    S->t_start = std::chrono::steady_clock::now();
+    S->worker = std::thread(server_loop, S);
+    *out = S;
+    return UC_OK;
+}
+
+void uc_server_destroy(uc_server* S) {
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        for (auto& j : S->pending) j->state = 3;
+    }
+    S->cv.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    (void)hipSetDevice(S->e->device);
+    for (auto& kv : S->all) {
+        if (kv.second->m) --kv.second->m->pins;
+        server_free(*kv.second);
+    }
+    if (S->stream) (void)hipStreamDestroy(S->stream);
+    delete S;
+}
+
+int uc_server_submit_lyap(uc_server* S, uc_system* s, const uc_lyap_request* r, int spectrum, long long* id) {
+    if (!S || !s || !r) return UC_ERR_ARGUMENT;
+    uc_engine* e = S->e;
+    auto j = std::make_shared<uc_server::Job>();
+    {
+        std::lock_guard<std::recursive_mutex> lk(e->mu);
+        double dummy = 0.0;
+        int dflag = 0;
+        uc_lyap_output probe{&dummy, &dflag, nullptr};
+        int rc = prep_lyap(e, s, r, &probe);
+        if (rc) return rc;
+        if ((rc = enter(e))) return rc;
+        if ((rc = get_module(e, s, r->c.strict != 0, true, &j->m))) return rc;
+        ++j->m->pins;
+    }
+    server_copy_common(*j, r->c, s->n_vars);
+    j->lyap = *r;
+    j->lyap.c = j->c;
+    j->kind = spectrum ? 2 : 1;
+    j->n_exp = spectrum ? s->n_vars : 1;
+    j->kernel = std::string(lyap_kernel(spectrum != 0, r->c.precision)) + "_q";
+    j->out0_n = (size_t)j->n_exp * (size_t)j->cells;
+    j->bytes = j->out0_n * sizeof(double) + (size_t)j->cells * sizeof(int);
+    const long long skip = r->c.h > 0 ? (long long)(r->c.transient_time / r->c.h) : 0;
+    const long long blocks = (long long)(r->c.t_max / r->NT), nts = r->c.h > 0 ? (long long)(r->NT / r->c.h) : 1;
+    j->step_cost = (double)(skip + 1 + blocks * (nts + 1));
+    return server_submit(S, j, id);
+}
+
+int uc_server_submit_bif(uc_server* S, uc_system* s, const uc_bif_request* r, long long* id) {
+    if (!S || !s || !r) return UC_ERR_ARGUMENT;
+    uc_engine* e = S->e;
+    auto j = std::make_shared<uc_server::Job>();
+    long long iters = 0;
+    {
+        std::lock_guard<std::recursive_mutex> lk(e->mu);
+        int dflag = 0;
+        double dummy = 0.0;
+        uc_bif_output probe{&dflag, &dummy, &dummy, nullptr, 0, nullptr};
+        int rc = prep_bif(e, s, r, &probe, &iters);
+        if (rc) return rc;
+        if ((rc = enter(e))) return rc;
+        if ((rc = get_module(e, s, r->c.strict != 0, true, &j->m))) return rc;
+        ++j->m->pins;
+    }
+    server_copy_common(*j, r->c, s->n_vars);
+    j->bif = *r;
+    j->bif.c = j->c;
+    j->kind = 0;
+    j->max_peaks = r->max_peaks;
+    j->iters = iters;
+    j->kernel = r->c.precision == UC_PREC_F64 ? "uc_bif_f64_q" : "uc_bif_f32c_q";
+    j->out0_n = j->out1_n = (size_t)std::max(1, r->max_peaks) * (size_t)j->cells;
+    j->bytes = 2 * j->out0_n * sizeof(double) + (size_t)j->cells * sizeof(int);
+    const long long skip = r->c.h > 0 ? (long long)(r->c.transient_time / r->c.h) : 0;
+    j->step_cost = (double)(skip + 1 + iters * r->pre_scaller + r->pre_scaller);
+    return server_submit(S, j, id);
+}
+
+int uc_server_poll(uc_server* S, long long id, int* state, double* latency_ms) {
+    if (!S) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return S->e->fail(UC_ERR_ARGUMENT, "unknown job id");
+    auto& j = *it->second;
+    if (state) *state = j.state;
+    if (latency_ms) *latency_ms = (j.state >= 2) ? std::chrono::duration<double, std::milli>(j.t_done - j.t_submit).count() : -1.0;
+    return UC_OK;
+}
+
+int uc_server_wait(uc_server* S, long long id, double timeout_s) {
+    if (!S) return UC_ERR_ARGUMENT;
+    std::unique_lock<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return S->e->fail(UC_ERR_ARGUMENT, "unknown job id");
+    auto j = it->second;
+    const auto deadline = std::chrono::steady_clock::now() + std::chrono::duration<double>(timeout_s);
+    while (j->state < 2) {
+        if (S->cv.wait_until(lk, deadline) == std::cv_status::timeout && j->state < 2) return UC_ERR_BUSY;
+    }
+    return j->state == 2 ? UC_OK : (j->state == 3 ? UC_ERR_CANCELLED : UC_ERR_HIP);
+}
+
+int uc_server_cancel(uc_server* S, long long id) {
+    if (!S) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) return S->e->fail(UC_ERR_ARGUMENT, "unknown job id");
+    if (it->second->state <= 1) { it->second->state = 3; it->second->t_done = std::chrono::steady_clock::now(); }
+    S->cv.notify_all();
+    return UC_OK;
+}
+
+// Copies a finished job's outputs to host memory and releases them.
+// lyap: values[n_exp][cells], flags[cells]; bif: flags, peaks/intervals [max_peaks][cells].
+int uc_server_fetch(uc_server* S, long long id, int* flags, double* out0, double* out1) {
+    if (!S) return UC_ERR_ARGUMENT;
+    std::shared_ptr<uc_server::Job> j;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->all.find(id);
+        if (it == S->all.end()) return S->e->fail(UC_ERR_ARGUMENT, "unknown job id");
+        j = it->second;
+        if (j->state != 2) return j->state < 2 ? UC_ERR_BUSY : S->e->fail(UC_ERR_ARGUMENT, "job did not complete: " + j->error);
+    }
+    (void)hipSetDevice(S->e->device);
+    hipError_t he = hipSuccess;
+    if (flags) he = hipMemcpy(flags, j->flags, sizeof(int) * (size_t)j->cells, hipMemcpyDeviceToHost);
+    if (he == hipSuccess && out0) he = hipMemcpy(out0, j->out0, sizeof(double) * j->out0_n, hipMemcpyDeviceToHost);
+    if (he == hipSuccess && out1 && j->out1) he = hipMemcpy(out1, j->out1, sizeof(double) * j->out1_n, hipMemcpyDeviceToHost);
+    if (he != hipSuccess) return S->e->fail(UC_ERR_HIP, hip_msg(he, "fetch"));
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->used_bytes -= j->bytes;
+    server_free(*j);
+    if (j->m) { --j->m->pins; j->m = nullptr; }
+    j->state = 5;  // fetched
+    S->cv.notify_all();
+    return UC_OK;
+}
+
+int uc_server_stats(uc_server* S, uc_server_stats_t* st) {
+    if (!S || !st) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(S->mu);
+    std::vector<double> l = S->latencies_ms;
+    std::sort(l.begin(), l.end());
+    auto pct = [&](double q) { return l.empty() ? 0.0 : l[(size_t)std::min<double>((double)l.size() - 1, std::floor(q * (double)l.size()))]; };
+    st->completed = (long long)l.size();
+    st->p50_ms = pct(0.50);
+    st->p99_ms = pct(0.99);
+    st->max_ms = l.empty() ? 0.0 : l.back();
+    st->busy_s = S->busy_s;
+    st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - S->t_start).count();
+    st->traj_steps = S->traj_steps_done;
+    st->rounds = S->rounds;
+    st->queued = (long long)S->pending.size();
+    st->active = (long long)S->active.size();
+    st->used_bytes = (double)S->used_bytes;
+    st->budget_bytes = (double)S->budget;
     return UC_OK;
 }
 

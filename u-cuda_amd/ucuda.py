@@ -67,6 +67,18 @@ class LyapOutput(C.Structure):
     _fields_ = [("values", C.c_void_p), ("flags", C.c_void_p), ("final_state", C.c_void_p)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_fraction", C.c_double), ("max_bytes", C.c_longlong), ("round_traj_steps", C.c_double),
+                ("block", C.c_int), ("waves_per_simd", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("completed", C.c_longlong), ("rounds", C.c_longlong), ("queued", C.c_longlong),
+                ("active", C.c_longlong), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("busy_s", C.c_double), ("wall_s", C.c_double), ("traj_steps", C.c_double),
+                ("used_bytes", C.c_double), ("budget_bytes", C.c_double)]
+
+
 class BasinsRequest(C.Structure):
     _fields_ = [("c", Common), ("rtol", C.c_double), ("atol", C.c_double),
                 ("h_min", C.c_double), ("h_max", C.c_double), ("section_period", C.c_double),
@@ -116,6 +128,16 @@ _SIGNATURES = {
                                 C.POINTER(C.c_double)]),
     "uc_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
     "uc_batch_info": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_int)] * 4),
+    "uc_server_create": (C.c_int, [C.c_void_p, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+    "uc_server_destroy": (None, [C.c_void_p]),
+    "uc_server_submit_lyap": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.c_int,
+                                        C.POINTER(C.c_longlong)]),
+    "uc_server_submit_bif": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.POINTER(C.c_longlong)]),
+    "uc_server_poll": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
+    "uc_server_wait": (C.c_int, [C.c_void_p, C.c_longlong, C.c_double]),
+    "uc_server_cancel": (C.c_int, [C.c_void_p, C.c_longlong]),
+    "uc_server_fetch": (C.c_int, [C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uc_server_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "uc_grid_cells": (C.c_longlong, [C.POINTER(Common)]),
     "uc_axis_values": (C.c_int, [C.POINTER(Axis), C.POINTER(C.c_double)]),
 }
@@ -522,6 +544,85 @@ class Batch:
         v = [C.c_int() for _ in range(4)]
         self.engine.check(lib().uc_batch_info(self.h, *[C.byref(x) for x in v]), "uc_batch_info")
         return dict(zip(("n_jobs", "n_groups", "block", "waves_per_simd"), [x.value for x in v]))
+
+
+class Server:
+    """Continuous-batching server (uc_server). submit_* return a job id;
+    fetch() returns numpy results once the job is done."""
+
+    def __init__(self, engine, hbm_fraction=0.5, max_bytes=0, round_traj_steps=0.0, block=0, waves_per_simd=0):
+        self.engine = engine
+        cfg = ServerConfig(hbm_fraction, max_bytes, round_traj_steps, block, waves_per_simd)
+        h = C.c_void_p()
+        engine.check(lib().uc_server_create(engine.h, C.byref(cfg), C.byref(h)), "uc_server_create")
+        self.h = h
+        self.meta = {}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().uc_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit_lyap(self, sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=False, eps=1e-4,
+                    method=METHOD_VARIATIONAL, probe_seed=REFERENCE_PROBE_SEED, max_value=1e6, precision=PREC_F64,
+                    strict=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0):
+        r = LyapRequest()
+        r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
+                            ic_seed, ic_lo, ic_hi)
+        r.NT, r.eps, r.method, r.probe_seed = float(NT), float(eps), int(method), int(probe_seed)
+        jid = C.c_longlong()
+        self.engine.check(lib().uc_server_submit_lyap(self.h, sysm.h, C.byref(r), 1 if spectrum else 0,
+                                                      C.byref(jid)), "uc_server_submit_lyap")
+        self.meta[jid.value] = ("lyap", n_cells(axes), sysm.n_vars if spectrum else 1, 0)
+        del keep
+        return jid.value
+
+    def submit_bif(self, sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pre_scaller=1,
+                   max_peaks=64, max_value=1e6, precision=PREC_F64, strict=False):
+        r = BifRequest()
+        r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict)
+        r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, max_peaks
+        jid = C.c_longlong()
+        self.engine.check(lib().uc_server_submit_bif(self.h, sysm.h, C.byref(r), C.byref(jid)), "uc_server_submit_bif")
+        self.meta[jid.value] = ("bif", n_cells(axes), max(1, max_peaks), 1)
+        del keep
+        return jid.value
+
+    def poll(self, jid):
+        st, lat = C.c_int(), C.c_double()
+        self.engine.check(lib().uc_server_poll(self.h, jid, C.byref(st), C.byref(lat)), "uc_server_poll")
+        return st.value, lat.value
+
+    def wait(self, jid, timeout_s=120.0):
+        rc = lib().uc_server_wait(self.h, jid, float(timeout_s))
+        self.engine.check(rc, "uc_server_wait")
+
+    def cancel(self, jid):
+        self.engine.check(lib().uc_server_cancel(self.h, jid), "uc_server_cancel")
+
+    def fetch(self, jid):
+        import numpy as np
+        kind, n, k, two = self.meta[jid]
+        flags = np.zeros(n, dtype=np.int32)
+        out0 = np.zeros((k, n))
+        out1 = np.zeros((k, n)) if two else None
+        self.engine.check(lib().uc_server_fetch(self.h, jid, flags.ctypes.data_as(C.c_void_p),
+                                                out0.ctypes.data_as(C.c_void_p),
+                                                out1.ctypes.data_as(C.c_void_p) if two else None), "uc_server_fetch")
+        if kind == "lyap":
+            return dict(values=out0, flags=flags)
+        return dict(flags=flags, peaks=out0, intervals=out1)
+
+    def stats(self):
+        st = ServerStats()
+        self.engine.check(lib().uc_server_stats(self.h, C.byref(st)), "uc_server_stats")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
 
 
 def rk4_flops(ops, n, tangents=0):
