@@ -118,6 +118,173 @@ def lle_errors(g, cpu):
                 flag_mismatch=int(((g == 999.0) != ~ok).sum()))
 
 
+def extra_ls(eng, n=1024, steps_scale=1.0):
+    """hyper-Rossler 4-exponent spectrum map: n x n over d in [0, 0.1] x a in
+    [0.2, 0.3], random X0 in [-1, 1]^4 (seed 20261203), h=0.005, GS every 20
+    steps, 1e5 steps; FP32-compensated and FP64."""
+    import torch
+    import ucuda
+    s = ucuda.preset("hyper_rossler", eng)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, n, 0.0, 0.1), ucuda.axis(ucuda.AXIS_PARAM, 1, n, 0.2, 0.3)]
+    t_max = 500.0 * steps_scale
+    out = {}
+    for name, prec in (("f32c", ucuda.PREC_F32C), ("f64", ucuda.PREC_F64)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = ucuda.run_lyap(s, axes, None, s.values(), 0.005, 0.0, t_max, 0.1, spectrum=True, precision=prec,
+                           ic_seed=20261203, ic_lo=-1.0, ic_hi=1.0)
+        dt = time.perf_counter() - t0
+        blocks, nts = int(t_max / 0.1), int(0.1 / 0.005)
+        steps = n * n * (1 + blocks * (nts + 1))
+        ok = r["flags"].cpu().numpy() == 1
+        out[name] = dict(s=dt, traj_steps_per_s=steps / dt, cells=n * n, steps_per_cell=steps // (n * n),
+                         ok_cells=int(ok.sum()))
+    return out
+
+
+def extra_basins(eng, n=2048):
+    """forced Duffing basins: n x n ICs over [-2.5, 2.5]^2, DP5(4) rtol=atol=1e-9,
+    200 forcing periods, Poincare signatures -> int32 labels."""
+    import numpy as np
+    import torch
+    import ucuda
+    s = ucuda.preset("duffing", eng)
+    axes = [ucuda.axis(ucuda.AXIS_VAR, 0, n, -2.5, 2.5), ucuda.axis(ucuda.AXIS_VAR, 1, n, -2.5, 2.5)]
+    out = {}
+    labels = {}
+    for name, prec in (("f64", ucuda.PREC_F64), ("f32c", ucuda.PREC_F32C)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = ucuda.run_basins(s, axes, [0.0, 0.0], s.values(), n_sections=200, precision=prec, want_points=False)
+        dt = time.perf_counter() - t0
+        st = r["steps"].cpu().numpy()
+        labels[name] = (r["labels"].cpu().numpy(), r["attractors"])
+        out[name] = dict(s=dt, cells=n * n, accepted_steps=int(st[0].sum()), rejected_steps=int(st[1].sum()),
+                         traj_steps_per_s=float(st[0].sum()) / dt, attractors=r["attractors"])
+    # label agreement FP32C vs FP64 (catalogues mapped by period + point)
+    l64, a64 = labels["f64"]
+    l32, a32 = labels["f32c"]
+    m = {}
+    for a in a32:
+        for b in a64:
+            if a["period"] == b["period"] and (a["period"] == 0 or (abs(a["point"][0] - b["point"][0]) <= 3e-3 and
+                                                                    abs(a["point"][1] - b["point"][1]) <= 3e-3)):
+                m[a["label"]] = b["label"]
+    l32m = np.array([m.get(int(v), 0 if v == 0 else -99) for v in l32])
+    out["f32c_label_agreement"] = float((l32m == l64).mean())
+    return out
+
+
+def extra_batch(eng, n_jobs=64, seed=0):
+    """64 heterogeneous LLE-map jobs, systems uniform over Lorenz / Rossler /
+    Chen / Duffing, grid side round(64*16^U), transient round(10^(3+2U))
+    steps, 2e3 averaged steps, job seed 1000+j; one persistent hipGraph launch
+    after the launch-shape tuner."""
+    import numpy as np
+    import torch
+    import ucuda
+    rng = np.random.default_rng(seed)
+    systems = {k: ucuda.preset(k, eng) for k in ("lorenz", "rossler", "chen", "duffing")}
+    boxes = {"lorenz": (2, 20.0, 200.0, 3, 1.0, 4.0), "rossler": (2, 0.05, 2.0, 3, 1.0, 20.0),
+             "chen": (1, 30.0, 45.0, 3, 20.0, 30.0), "duffing": (1, 0.1, 0.4, 2, 0.2, 0.5)}
+    b = ucuda.Batch(eng)
+    total = 0
+    for j in range(n_jobs):
+        name = list(systems)[int(rng.integers(0, 4))]
+        sysm = systems[name]
+        side = int(round(64 * 16 ** rng.uniform()))
+        transient_steps = int(round(10 ** (3 + 2 * rng.uniform())))
+        h = 0.0025 if name == "chen" else 0.01
+        p1, lo1, hi1, p2, lo2, hi2 = boxes[name]
+        axes = [ucuda.axis(ucuda.AXIS_PARAM, p1, side, lo1, hi1), ucuda.axis(ucuda.AXIS_PARAM, p2, side, lo2, hi2)]
+        ic = [0.1] * sysm.n_vars
+        t_max = 2000 * h
+        b.add_lyap(sysm, axes, ic, sysm.values(), h, transient_steps * h, t_max, 10 * h,
+                   precision=ucuda.PREC_F32C, probe_seed=1000 + j)
+        total += side * side * (transient_steps + 1 + int(t_max / (10 * h)) * 11)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        tuned = b.tune(stream)
+        b.launch(stream)  # graph build + first replay
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.launch(stream)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    return dict(jobs=n_jobs, traj_steps=total, s=dt, traj_steps_per_s=total / dt, tuned=tuned, **b.info())
+
+
+def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
+    """Poisson arrivals of mixed bifurcation / LLE-map requests (systems above,
+    side log-uniform in [32, 2048] capped at 1024, steps log-uniform in
+    [1e3, 1e5]); the arrival rate is set for ~70% of the measured capacity.
+    Reports throughput and p50 / p99 job latency, and checks 1% of jobs (at
+    least one) against the FP64 path."""
+    import numpy as np
+    import ucuda
+    rng = np.random.default_rng(seed)
+    ros, lor = ucuda.preset("rossler", eng), ucuda.preset("lorenz", eng)
+    jobs = []
+    t = 0.0
+    mean_steps = None
+    specs = []
+    for _ in range(100000):
+        side = int(round(32 * 64 ** rng.uniform()))
+        side = min(side, 1024)
+        steps = int(round(10 ** (3 + 2 * rng.uniform())))
+        kind = "lle" if rng.uniform() < 0.5 else "bif"
+        specs.append((kind, side, steps))
+        if len(specs) >= 4000:
+            break
+    est = np.mean([side * side * steps for _, side, steps in specs])
+    if rate is None:
+        rate = 0.7 * capacity / est if capacity else 20.0
+    srv = ucuda.Server(eng, round_traj_steps=5e9)
+    ids = []
+    t0 = time.perf_counter()
+    i = 0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration_s:
+            break
+        kind, side, steps = specs[i % len(specs)]
+        i += 1
+        now = time.perf_counter() - t0
+        if t > now:
+            time.sleep(t - now)
+        if kind == "lle":
+            axes = [ucuda.axis(1, 2, side, 0.05, 2.0), ucuda.axis(1, 3, side, 1.0, 20.0)]
+            blocks = max(1, steps // 11)
+            jid = srv.submit_lyap(ros, axes, [1, 1, 1], ros.values(), 0.01, 0.0, blocks * 0.1, 0.1,
+                                  precision=ucuda.PREC_F32C)
+        else:
+            axes = [ucuda.axis(1, 2, side * side, 0.0, 250.0)]
+            jid = srv.submit_bif(lor, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, steps * 0.005, steps * 0.005,
+                                 max_peaks=8, precision=ucuda.PREC_F32C)
+        ids.append((jid, kind, side, steps, axes))
+    for jid, *_ in ids:
+        srv.wait(jid, 600.0)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    # audit: ~1% of jobs recomputed alone in FP64 (reported max |diff| for LLE jobs)
+    audit = []
+    for jid, kind, side, steps, axes in ids[::100][:5]:
+        got = srv.fetch(jid)
+        if kind == "lle":
+            blocks = max(1, steps // 11)
+            ref = ucuda.run_lyap(ros, axes, [1, 1, 1], ros.values(), 0.01, 0.0, blocks * 0.1, 0.1)
+            rv = ref["values"].cpu().numpy()[0]
+            ok = (got["flags"] == 1) & (ref["flags"].cpu().numpy() == 1)
+            audit.append(dict(kind=kind, cells=side * side, median_abs_diff=float(np.median(np.abs(got["values"][0][ok] - rv[ok])))))
+        else:
+            audit.append(dict(kind=kind, cells=side * side))
+    srv.close()
+    return dict(arrival_rate_per_s=rate, offered_utilisation=rate * est / capacity if capacity else None,
+                jobs=len(ids), completed=st["completed"], p50_ms=st["p50_ms"], p99_ms=st["p99_ms"],
+                max_ms=st["max_ms"], traj_steps_per_s=st["traj_steps"] / wall, rounds=st["rounds"],
+                busy_frac=st["busy_s"] / wall, audit_fp64=audit)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,6 +293,8 @@ def main():
     ap.add_argument("--n", type=int, default=512, help="grid side")
     ap.add_argument("--cpu-stride", type=int, default=8, help="CPU subsample: every k-th point per axis")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the LS / basins / batch / serving measurements")
+    ap.add_argument("--serve-seconds", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's path runs on a single MI355X "
@@ -184,6 +353,13 @@ def main():
                      "kernel": "uc_lle_f32c", "traffic": None},
         "wall_s_per_step": f32["wall"] / args.steps,
     }
+    if not args.no_extra:
+        cap = value  # traj-steps/s of the headline kernel sets the serving load
+        line["ls_hyper_rossler_1024"] = extra_ls(eng)
+        line["basins_duffing_2048"] = extra_basins(eng)
+        line["batch_64_jobs"] = extra_batch(eng)
+        line["serving"] = extra_serving(eng, duration_s=args.serve_seconds, capacity=cap)
+        line["p99_job_latency_ms"] = line["serving"]["p99_ms"]
     if cpu:
         line["cpu_baseline"] = {"value": cpu["cells"] * spc / cpu["t1"], "unit": "traj-steps/s", "cores": 1,
                                 "kind": "port",

@@ -169,6 +169,12 @@ const char* uc_version(void);
 
 /* ---- systems / codegen (U-CUDA/codegen.hpp:25-48) ---- */
 int uc_system_create(uc_engine* e, const uc_system_desc* desc, uc_system** out);
+/* A caller-supplied KRS step body (the reference's custom schemes and any
+ * codegen_scheme output, U-CUDA/system_record.h:10-13, krs_cpu.cpp:293-316):
+ * statements over numb X[n_vars], const numb a[n_values], numb h. Usable by
+ * uc_run_bif and by uc_run_lle / uc_run_ls with UC_METHOD_FINITE_DIFF.
+ * Constant out-of-range X[k] / a[k] subscripts are rejected (UC_ERR_PARSE). */
+int uc_system_create_krs(uc_engine* e, const char* krs_body, int n_vars, int n_values, uc_system** out);
 void uc_system_destroy(uc_system* s);
 int uc_system_n_vars(const uc_system* s);
 int uc_system_n_values(const uc_system* s);

@@ -271,3 +271,30 @@ def test_ls_hyper_rossler_random_ics(gpu_engine):
     np.testing.assert_allclose(g[:, ok], r[:, ok], atol=1e-9, rtol=0)
     # the float run flags the same cells; its spectrum sums agree to the FP64 ones
     np.testing.assert_array_equal(g32[0] == 999.0, ~ok)
+
+
+def test_custom_krs_bodies(gpu_engine):
+    print("case: custom KRS step bodies (Euler, midpoint, RK4 text) -> bif + FD LLE vs oracle", flush=True)
+    lor = ucuda.preset("lorenz", gpu_engine)
+    vals = lor.values()
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 16, 10.0, 200.0)]
+    for scheme in (ucuda.SCHEME_EULER, ucuda.SCHEME_MIDPOINT, ucuda.SCHEME_RK4):
+        body = gpu_engine.codegen_scheme(lor.vars, lor.params, lor.rhs, scheme)
+        ks = ucuda.KrsSystem(gpu_engine, body, 3, 4)
+        g = ucuda.run_bif(ks, axes, [0.1, 0.1, 0.1], vals, 0.005, 5.0, 10.0, max_peaks=32, strict=True, traj_len=40)
+        step = oracle.KrsStep(body, 3, 4)
+        job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], vals, 0.005, 5.0, 10.0, max_peaks=32)
+        r = oracle.bif_cells(step, job, range(16), traj_len=40)
+        np.testing.assert_array_equal(g["flags"].cpu().numpy(), r["flags"])
+        ok = r["flags"] != 0
+        np.testing.assert_array_equal(g["traj"].cpu().numpy()[:, ok], r["traj"][:, ok])
+    # the RK4 step text integrates exactly like the built-in RK4 (same operation order)
+    body = gpu_engine.codegen_scheme(lor.vars, lor.params, lor.rhs, ucuda.SCHEME_RK4)
+    ks = ucuda.KrsSystem(gpu_engine, body, 3, 4)
+    a = ucuda.run_lyap(ks, axes, [0.1, 0.1, 0.1], vals, 0.01, 5.0, 20.0, 0.1, method=ucuda.METHOD_FINITE_DIFF,
+                       strict=True)["values"].cpu().numpy()
+    b = ucuda.run_lyap(lor, axes, [0.1, 0.1, 0.1], vals, 0.01, 5.0, 20.0, 0.1, method=ucuda.METHOD_FINITE_DIFF,
+                       strict=True)["values"].cpu().numpy()
+    np.testing.assert_array_equal(a, b)
+    with pytest.raises(ucuda.UcError):
+        ucuda.run_lyap(ks, axes, [0.1, 0.1, 0.1], vals, 0.01, 5.0, 20.0, 0.1)  # variational needs a Jacobian

@@ -694,7 +694,8 @@ std::string emit_system_source(const System& s, Dialect d) {
         for (int j = 0; j < N; ++j) J[i][j] = diff(trees[i], j);
 
     std::ostringstream o;
-    o << "#define UC_N " << N << "\n#define UC_M " << M << "\n#define UC_USES_TIME " << (uses_time ? 1 : 0) << "\n";
+    o << "#define UC_N " << N << "\n#define UC_M " << M << "\n#define UC_USES_TIME " << (uses_time ? 1 : 0)
+      << "\n#define UC_CUSTOM_KRS 0\n";
     const bool hip = (d == Dialect::Hip);
     const char* fn = hip ? "template <typename T>\n__device__ __forceinline__ void" : "static inline void";
     if (!hip) o << "typedef double T;\n";
@@ -741,6 +742,71 @@ std::string emit_system_source(const System& s, Dialect d) {
     }
     o << "    }\n}\n";
     return o.str();
+}
+
+std::string emit_custom_krs_source(const std::string& body, int n_vars, int n_values) {
+    std::ostringstream o;
+    o << "#define UC_N " << n_vars << "\n#define UC_M " << n_values << "\n#define UC_USES_TIME 0\n#define UC_CUSTOM_KRS 1\n";
+    // the step body in the reference prelude's environment (U-CUDA/krs_cpu.cpp:293-316):
+    // numb, pi and euler of that type (a body may shadow them), min / max
+    o << "template <typename numb>\n__device__ __forceinline__ void uc_krs_step(numb* X, const numb* a, const numb h) {\n"
+         "    const numb pi = (numb)3.1415926535897932384626433832795;\n"
+         "    const numb euler = (numb)2.7182818284590452353602874713527;\n"
+         "    (void)pi; (void)euler; (void)a; (void)h;\n    {\n"
+      << body << "\n    }\n}\n";
+    // no right-hand side / Jacobian: the kernels that need them are rejected by the engine
+    o << "template <typename T>\n__device__ __forceinline__ void uc_rhs(const T*, const T*, const T, T* dX) {\n"
+         "    for (int i = 0; i < UC_N; ++i) dX[i] = (T)0;\n}\n";
+    o << "template <typename T, int K>\n__device__ __forceinline__ void uc_tangent(const T*, const T*, const T, const T*, T* JV) {\n"
+         "    for (int i = 0; i < K * UC_N; ++i) JV[i] = (T)0;\n}\n";
+    return o.str();
+}
+
+// Constant subscripts of X[] / a[] must be in range (krs_cpu_check_indices,
+// U-CUDA/krs_cpu.cpp:21-99): comments and literals are skipped, identifiers
+// are taken whole, only integer-literal subscripts are checked.
+std::string check_krs_indices(const std::string& b, int nx, int nv) {
+    std::ostringstream diag;
+    const size_t n = b.size();
+    int line = 1;
+    for (size_t i = 0; i < n;) {
+        const char c = b[i];
+        if (c == '\n') { ++line; ++i; continue; }
+        if (c == '/' && i + 1 < n && b[i + 1] == '/') { while (i < n && b[i] != '\n') ++i; continue; }
+        if (c == '/' && i + 1 < n && b[i + 1] == '*') {
+            i += 2;
+            while (i + 1 < n && !(b[i] == '*' && b[i + 1] == '/')) { if (b[i] == '\n') ++line; ++i; }
+            i = (i + 1 < n) ? i + 2 : n;
+            continue;
+        }
+        if (c == '"' || c == '\'') {
+            const char q = c;
+            ++i;
+            while (i < n && b[i] != q) { if (b[i] == '\\') ++i; else if (b[i] == '\n') ++line; ++i; }
+            ++i;
+            continue;
+        }
+        if (!(std::isalpha((unsigned char)c) || c == '_')) { ++i; continue; }
+        const size_t st = i;
+        while (i < n && (std::isalnum((unsigned char)b[i]) || b[i] == '_')) ++i;
+        const std::string id = b.substr(st, i - st);
+        if (id != "X" && id != "a") continue;
+        size_t j = i;
+        while (j < n && (b[j] == ' ' || b[j] == '\t')) ++j;
+        if (j >= n || b[j] != '[') continue;
+        ++j;
+        while (j < n && (b[j] == ' ' || b[j] == '\t')) ++j;
+        const size_t ds = j;
+        while (j < n && std::isdigit((unsigned char)b[j])) ++j;
+        if (j == ds) continue;
+        const size_t de = j;
+        while (j < n && (b[j] == ' ' || b[j] == '\t')) ++j;
+        if (j >= n || b[j] != ']') continue;
+        const long idx = std::strtol(b.substr(ds, de - ds).c_str(), nullptr, 10);
+        if (id == "X" && (idx < 0 || idx >= nx)) diag << "line " << line << ": X[" << idx << "] out of range (" << nx << " variables)\n";
+        if (id == "a" && (idx < 0 || idx >= nv)) diag << "line " << line << ": a[" << idx << "] out of range (" << nv << " values)\n";
+    }
+    return diag.str();
 }
 
 OpCounts count_ops(const System& s) {

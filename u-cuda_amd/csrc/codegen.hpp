@@ -48,6 +48,13 @@ enum class Dialect {
 // argument. Also emits UC_N / UC_M / UC_USES_TIME macros.
 std::string emit_system_source(const System& s, Dialect d);
 
+// Device source for a caller-supplied KRS step body (custom schemes): the
+// body becomes uc_krs_step<numb>(X, a, h) with the reference prelude's
+// names; right-hand side and Jacobian are absent (zero stubs).
+std::string emit_custom_krs_source(const std::string& body, int n_vars, int n_values);
+// Range check of constant X[k] / a[k] subscripts; empty string when clean.
+std::string check_krs_indices(const std::string& body, int n_vars, int n_values);
+
 // Number of nonzero Jacobian entries and right-hand-side operation counts,
 // used for the roofline accounting (+,-,* and / each count 1; calls counted
 // separately).

@@ -70,6 +70,7 @@ uint64_t fnv1a(const std::string& s, uint64_t h = 1469598103934665603ULL) {
 
 struct uc_system {
     ucuda::System sys;
+    bool custom = false;  // caller-supplied KRS step body instead of a right-hand side
     std::string device_source;
     int n_vars = 0;
     int n_values = 0;
@@ -393,6 +394,22 @@ int uc_system_create(uc_engine* e, const uc_system_desc* desc, uc_system** out) 
     return UC_OK;
 }
 
+int uc_system_create_krs(uc_engine* e, const char* krs_body, int n_vars, int n_values, uc_system** out) {
+    if (!e || !krs_body || !out) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (n_vars <= 0 || n_vars > kMaxVars) return e->fail(UC_ERR_ARGUMENT, "n_vars out of range");
+    if (n_values < 1 || n_values > kMaxValues) return e->fail(UC_ERR_ARGUMENT, "n_values out of range");
+    const std::string diag = ucuda::check_krs_indices(krs_body, n_vars, n_values);
+    if (!diag.empty()) return e->fail(UC_ERR_PARSE, "KRS body:\n" + diag);
+    auto s = std::make_unique<uc_system>();
+    s->custom = true;
+    s->device_source = ucuda::emit_custom_krs_source(krs_body, n_vars, n_values);
+    s->n_vars = n_vars;
+    s->n_values = n_values;
+    *out = s.release();
+    return UC_OK;
+}
+
 void uc_system_destroy(uc_system* s) { delete s; }
 int uc_system_n_vars(const uc_system* s) { return s ? s->n_vars : 0; }
 int uc_system_n_values(const uc_system* s) { return s ? s->n_values : 0; }
@@ -400,6 +417,7 @@ int uc_system_n_values(const uc_system* s) { return s ? s->n_values : 0; }
 int uc_system_source(const uc_system* s, int dialect, char* buf, size_t cap, size_t* len) {
     if (!s) return UC_ERR_ARGUMENT;
     if (dialect == 0) return copy_out(program_source(s, false), buf, cap, len);
+    if (s->custom) return UC_ERR_ARGUMENT;
     try {
         return copy_out(ucuda::emit_system_source(s->sys, ucuda::Dialect::C), buf, cap, len);
     } catch (...) {
@@ -438,7 +456,7 @@ int uc_codegen_scheme(uc_engine* e, const uc_system_desc* desc, int scheme, char
 }
 
 int uc_system_eval_rhs(const uc_system* s, const double* X, const double* a, double t, double* dX) {
-    if (!s || !X || !a || !dX) return UC_ERR_ARGUMENT;
+    if (!s || !X || !a || !dX || s->custom) return UC_ERR_ARGUMENT;
     try {
         ucuda::eval_rhs(s->sys, X, a, t, dX);
     } catch (...) {
@@ -513,6 +531,8 @@ static int prep_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const
     if (!(r->NT > 0.0)) return e->fail(UC_ERR_ARGUMENT, "NT must be > 0");
     if (!(r->eps > 0.0)) return e->fail(UC_ERR_ARGUMENT, "eps must be > 0");
     if (r->method != UC_METHOD_VARIATIONAL && r->method != UC_METHOD_FINITE_DIFF) return e->fail(UC_ERR_ARGUMENT, "unknown method");
+    if (s->custom && r->method != UC_METHOD_FINITE_DIFF)
+        return e->fail(UC_ERR_ARGUMENT, "a custom step body has no Jacobian: use the finite-difference method");
     if (!o->values || !o->flags) return e->fail(UC_ERR_ARGUMENT, "outputs missing");
     // NT blocks: (int)(t_max / NT); steps per block (int)(NT / h); transient
     // (int)(transient_time / h) (run_lle_2d, parametric_engine.cpp:2731-2732, 2865).
@@ -571,6 +591,7 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     int rc = UC_OK;
+    if (s->custom) return e->fail(UC_ERR_ARGUMENT, "basins need a right-hand side (adaptive integrator), not a step body");
     uc_common c = r->c;
     if (c.t_max <= 0.0) c.t_max = 1.0;  // unused by basins
     if ((rc = validate_common(e, s, c, false))) return rc;

@@ -119,6 +119,13 @@ __device__ __forceinline__ void uc_kahan_add(float& hi, float& c, float incr) {
     hi = t;
 }
 
+#if UC_CUSTOM_KRS
+// A caller-supplied step body (the reference's custom KRS schemes,
+// U-CUDA/system_record.h:10-13) is the step map itself: numb = double, or
+// float without compensation (the body forms X + increment internally).
+__device__ __forceinline__ void uc_rk4(StateF64& s, const double* a, double h, double) { uc_krs_step<double>(s.x, a, h); }
+__device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, float h, float) { uc_krs_step<float>(s.hi, a, h); }
+#else
 // Classic RK4 with the reference's stage structure and update order
 // (U-CUDA/codegen.cpp:597-606): X1 = X + 0.5*h*k for stages 0,1, X1 = X + h*k
 // for stage 2, X += h*(k0 + 2*k1 + 2*k2 + k3)/6. Accumulating the bracket
@@ -167,6 +174,8 @@ __device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, float h, fl
         uc_kahan_add(s.hi[i], s.c[i], h6 * acc[i]);
     }
 }
+
+#endif
 
 // RK4 on the state together with K tangent vectors V (row-major K x N): the
 // variational equations V' = J(X) V integrated with the same stages, which

@@ -100,6 +100,7 @@ _SIGNATURES = {
     "uc_engine_destroy": (None, [C.c_void_p]),
     "uc_last_error": (C.c_char_p, [C.c_void_p]),
     "uc_system_create": (C.c_int, [C.c_void_p, C.POINTER(SystemDesc), C.POINTER(C.c_void_p)]),
+    "uc_system_create_krs": (C.c_int, [C.c_void_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "uc_system_destroy": (None, [C.c_void_p]),
     "uc_system_n_vars": (C.c_int, [C.c_void_p]),
     "uc_system_n_values": (C.c_int, [C.c_void_p]),
@@ -275,6 +276,21 @@ class System:
         out = (C.c_double * len(X))()
         self.engine.check(lib().uc_system_eval_rhs(self.h, Xa, aa, float(t), out), "eval_rhs")
         return list(out)
+
+
+class KrsSystem(System):
+    """A system given as a KRS step body (custom scheme) instead of rhs strings."""
+
+    def __init__(self, engine, body, n_vars, n_values, name="krs"):
+        self.engine = engine
+        self.vars = ["x%d" % i for i in range(n_vars)]
+        self.params = ["p%d" % i for i in range(n_values - 1)]
+        self.rhs, self.latex, self.time_symbol, self.defaults, self.name = [], False, None, {}, name
+        self.body = body
+        h = C.c_void_p()
+        engine.check(lib().uc_system_create_krs(engine.h, body.encode(), n_vars, n_values, C.byref(h)),
+                     "uc_system_create_krs")
+        self.h = h
 
 
 PRESETS = {
