@@ -20,6 +20,12 @@ import os
 import sys
 import time
 
+_T0 = time.perf_counter()
+
+
+def progress(msg):
+    print("[bench %6.1fs] %s" % (time.perf_counter() - _T0, msg), file=sys.stderr, flush=True)
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "u-cuda_amd"))
 
@@ -86,24 +92,31 @@ def cpu_baseline(eng, sysm, w, n, stride, threads_all):
     rN = oracle.lle_fd_cells(step, job, cells, threads=threads_all)
     tN = time.perf_counter() - t0
     assert (r1["values"] == rN["values"]).all()
+    progress("cpu baseline: %d cells single-thread %.1fs, %d threads %.1fs" % (len(cells), t1, threads_all, tN))
     funcs = oracle.SystemFuncs(sysm.source(1))
     var = oracle.lyap_var_cells(funcs, 1, job, cells, threads=threads_all)["values"][0]
-    spread = np.zeros_like(var)
-    for d in ((1e-9, 0, 0), (0, 0, 1e-9), (0, -1e-9, 0)):
-        icp = [w["ic"][i] + d[i] for i in range(3)]
-        jobp = oracle.make_job(3, w["axes"], icp, vals, w["h"], w["transient"], w["t_max"], NT=w["NT"])
-        varp = oracle.lyap_var_cells(funcs, 1, jobp, cells, threads=threads_all)["values"][0]
-        spread = np.maximum(spread, np.abs(varp - var))
-    return dict(cells=cells, fd=r1["values"][0], var=var, spread=spread, t1=t1, tN=tN)
+    spreads = {}
+    for mag in (1e-9, 1e-6):
+        spread = np.zeros_like(var)
+        for d in ((mag, 0, 0), (0, 0, mag), (0, -mag, 0)):
+            icp = [w["ic"][i] + d[i] for i in range(3)]
+            jobp = oracle.make_job(3, w["axes"], icp, vals, w["h"], w["transient"], w["t_max"], NT=w["NT"])
+            varp = oracle.lyap_var_cells(funcs, 1, jobp, cells, threads=threads_all)["values"][0]
+            spread = np.maximum(spread, np.abs(varp - var))
+        spreads[mag] = spread
+    progress("oracle variational + perturbations done")
+    return dict(cells=cells, fd=r1["values"][0], var=var, spreads=spreads, t1=t1, tN=tN)
 
 
-def lle_errors(g, cpu):
-    """Error of a GPU LLE map against the FP64 oracle on the sampled cells."""
+def lle_errors(g, cpu, mag):
+    """Error of a GPU LLE map against the FP64 oracle on the sampled cells.
+    Reproducible cells: three perturbations of X0 of size `mag` (1e-9 for
+    FP64, 1e-6 = float rounding size for FP32-compensated) move the FP64
+    exponent by <= 1e-5."""
     import numpy as np
     g = g[cpu["cells"]]
-    var, spread, fd = cpu["var"], cpu["spread"], cpu["fd"]
+    var, spread, fd = cpu["var"], cpu["spreads"][mag], cpu["fd"]
     ok = var != 999.0
-    # reproducible: three 1e-9 perturbations of X0 move the FP64 exponent by <= 1e-5
     repro = ok & (spread <= 1e-5)
     sens = ok & ~repro
     e = np.abs(g - var)
@@ -317,11 +330,12 @@ def main():
     res = {}
     for name, prec in (("f64", ucuda.PREC_F64), ("f32c", ucuda.PREC_F32C)):
         res[name] = run_gpu(sysm, w, prec, args.steps, args.warmup)
+        progress("%s LLE map: %.2f ms per map" % (name, sum(res[name]["kernel_ms"]) / len(res[name]["kernel_ms"])))
 
     cpu = None
     if not args.no_cpu:
         cb = cpu_baseline(eng, sysm, w, args.n, args.cpu_stride, min(16, os.cpu_count() or 1))
-        errs = {name: lle_errors(res[name]["values"], cb) for name in res}
+        errs = {"f64": lle_errors(res["f64"]["values"], cb, 1e-9), "f32c": lle_errors(res["f32c"]["values"], cb, 1e-6)}
         cpu = dict(cells=len(cb["cells"]), t1=cb["t1"], tN=cb["tN"], errs=errs)
 
     f32 = res["f32c"]
@@ -356,9 +370,13 @@ def main():
     if not args.no_extra:
         cap = value  # traj-steps/s of the headline kernel sets the serving load
         line["ls_hyper_rossler_1024"] = extra_ls(eng)
+        progress("LS spectrum map done")
         line["basins_duffing_2048"] = extra_basins(eng)
+        progress("basins done")
         line["batch_64_jobs"] = extra_batch(eng)
+        progress("batch done")
         line["serving"] = extra_serving(eng, duration_s=args.serve_seconds, capacity=cap)
+        progress("serving done")
         line["p99_job_latency_ms"] = line["serving"]["p99_ms"]
     if cpu:
         line["cpu_baseline"] = {"value": cpu["cells"] * spc / cpu["t1"], "unit": "traj-steps/s", "cores": 1,

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 def _jobs(eng):
     lor, ros, chen, hr, duf = (ucuda.preset(n, eng) for n in ("lorenz", "rossler", "chen", "hyper_rossler", "duffing"))
     J = []
-    J.append(("lyap", ros, dict(axes=[ucuda.axis(2, 2, 37, 0.05, 2.0), ucuda.axis(2, 3, 29, 1.0, 20.0)],
+    J.append(("lyap", ros, dict(axes=[ucuda.axis(1, 2, 37, 0.05, 2.0), ucuda.axis(1, 3, 29, 1.0, 20.0)],
                                 ic=[1, 1, 1], values=ros.values(), h=0.01, transient_time=20.0, t_max=60.0, NT=0.1,
                                 precision=ucuda.PREC_F32C)))
     J.append(("lyap", lor, dict(axes=[ucuda.axis(1, 2, 300, 20.0, 200.0)], ic=[0.1, 0.1, 0.1], values=lor.values(),
@@ -80,7 +80,7 @@ def test_batch_tuner_and_shapes(gpu_engine):
     print("case: tuner over launch shapes; every shape gives identical results", flush=True)
     ros = ucuda.preset("rossler", gpu_engine)
     b = ucuda.Batch(gpu_engine)
-    outs = [b.add_lyap(ros, [ucuda.axis(2, 2, 64, 0.05, 2.0), ucuda.axis(2, 3, n, 1.0, 20.0)], [1, 1, 1],
+    outs = [b.add_lyap(ros, [ucuda.axis(1, 2, 64, 0.05, 2.0), ucuda.axis(1, 3, n, 1.0, 20.0)], [1, 1, 1],
                        ros.values(), 0.01, 5.0, 20.0, 0.1, precision=ucuda.PREC_F32C) for n in (3, 50, 200)]
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):

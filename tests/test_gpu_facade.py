@@ -29,7 +29,8 @@ def test_facade_matches_c_abi(gpu_engine):
     ros = ucuda.preset("rossler", gpu_engine)
     axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 8, 0.05, 2.0), ucuda.axis(ucuda.AXIS_PARAM, 3, 8, 1.0, 20.0)]
     l = ucuda.run_lyap(ros, axes, [1.0, 1.0, 1.0], ros.values(), 0.01, 5.0, 50.0, 0.1)["values"].cpu().numpy()[0]
-    assert res["lle2d"] == l.tolist()
+    # separate processes compile the module separately: equal to the last bits
+    np.testing.assert_allclose(res["lle2d"], l, rtol=1e-12, atol=0)
     axes4 = [ucuda.axis(ucuda.AXIS_PARAM, 2, 4, 0.05, 2.0), ucuda.axis(ucuda.AXIS_PARAM, 3, 4, 1.0, 20.0)]
     ls = ucuda.run_lyap(ros, axes4, [1.0, 1.0, 1.0], ros.values(), 0.01, 5.0, 50.0, 0.1, spectrum=True)
     s = ls["values"].cpu().numpy()

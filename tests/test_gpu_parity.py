@@ -10,8 +10,9 @@ Tolerances:
     trajectories). The variational result is the eps -> 0 limit of the
     reference method (their gap shrinks linearly with eps).
   * Lyapunov exponents, fast FP64 and FP32-compensated: 1e-3 absolute on
-    cells whose FP64 exponent is reproducible (three 1e-9 perturbations of X0
-    move it by <= 1e-5). On chaotic cells any non-identical rounding changes
+    cells whose FP64 exponent is reproducible (three perturbations of X0 move
+    it by <= 1e-5; perturbation size 1e-9 for FP64, 1e-6 for FP32, the size
+    of the rounding each mode injects). On chaotic cells any non-identical rounding changes
     the finite-time exponent by ~1e-2 (the FP64 oracle's own spread); the
     error there is reported, not asserted.
   * FP32-compensated: short-horizon trajectories 1e-5 relative, periodic
@@ -35,22 +36,25 @@ def _lorenz_case(eng, n_rho=64, lo=0.0, hi=250.0, transient=100.0, t_max=100.0, 
     return s, axes, vals, step, job
 
 
-def _stable_cells(step, job, cells, rel=1e-8):
-    """Cells whose FP64 oracle peaks are insensitive to a 1e-9 change of X0
-    (periodic windows / fixed points): same count, peaks within `rel`."""
-    import copy
+def _stable_cells(step, job, cells, rel=1e-8, mag=1e-9):
+    """Cells whose FP64 oracle peaks are insensitive to changes of X0 of size
+    `mag` in three directions (periodic windows / fixed points): same count,
+    peaks within `rel`. mag = 1e-9 for FP64 comparisons; 1e-6 (the size of
+    float rounding) for FP32-compensated comparisons."""
     a = oracle.bif_cells(step, job, cells)
-    jp = oracle.make_job(job.n_vars, _axes_of(job), [job.ic[0] + 1e-9] + [job.ic[i] for i in range(1, job.n_vars)],
-                         [job.params[i] for i in range(job.n_values)], job.h, job.transient_time, job.t_max,
-                         max_peaks=job.max_peaks)
-    b = oracle.bif_cells(step, jp, cells)
-    stable = np.zeros(len(a["flags"]), dtype=bool)
-    for c in range(len(stable)):
-        n = a["flags"][c]
-        if n > 8 and b["flags"][c] == n:
-            d = np.abs(a["peaks"][:n, c] - b["peaks"][:n, c]).max()
-            stable[c] = d <= rel * max(1.0, np.abs(a["peaks"][:n, c]).max())
-    _ = copy
+    stable = np.ones(len(a["flags"]), dtype=bool)
+    for d in range(3):
+        ic = [job.ic[i] + (mag if i == d else 0.0) for i in range(job.n_vars)]
+        jp = oracle.make_job(job.n_vars, _axes_of(job), ic, [job.params[i] for i in range(job.n_values)], job.h,
+                             job.transient_time, job.t_max, max_peaks=job.max_peaks)
+        b = oracle.bif_cells(step, jp, cells)
+        for c in range(len(stable)):
+            n = a["flags"][c]
+            if not (n > 8 and b["flags"][c] == n):
+                stable[c] = False
+                continue
+            d_ = np.abs(a["peaks"][:n, c] - b["peaks"][:n, c]).max()
+            stable[c] &= bool(d_ <= rel * max(1.0, np.abs(a["peaks"][:n, c]).max()))
     return a, stable
 
 
@@ -91,7 +95,7 @@ def test_bif_lorenz_f32c(gpu_engine):
     s, axes, vals, step, job = _lorenz_case(gpu_engine)
     gpu = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 100.0, 100.0, max_peaks=256,
                         precision=ucuda.PREC_F32C)
-    ref, stable = _stable_cells(step, job, range(64))
+    ref, stable = _stable_cells(step, job, range(64), mag=1e-6)
     assert stable.sum() >= 4
     # peak values 1e-6; intervals come from the parabolic refinement of float
     # samples (a second difference divided by ~h^2): 1e-5
@@ -111,17 +115,18 @@ def _rossler_lle_case(eng, n=16):
     return s, axes, s.values()
 
 
-def _lle_oracles(eng, s, axes, vals, n_cells, ic=(1.0, 1.0, 1.0)):
-    """FP64 variational oracle at X0 and at three X0 perturbations of 1e-9;
-    a cell is reproducible when all three stay within 1e-5 of the first
-    (chaotic cells move by ~1e-2 under such a perturbation)."""
+def _lle_oracles(eng, s, axes, vals, n_cells, ic=(1.0, 1.0, 1.0), mag=1e-9):
+    """FP64 variational oracle at X0 and at three X0 perturbations of size
+    `mag`; a cell is reproducible when all three stay within 1e-5 of the
+    first (chaotic cells move by ~1e-2 under such a perturbation). mag = 1e-9
+    for FP64 comparisons, 1e-6 (float rounding size) for FP32-compensated."""
     funcs = oracle.SystemFuncs(s.source(1))
     step = oracle.KrsStep(eng.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
     job = oracle.make_job(3, axes, list(ic), vals, 0.01, 50.0, 500.0, NT=0.1)
     cells = range(n_cells)
     r_var = oracle.lyap_var_cells(funcs, 1, job, cells, threads=8)["values"][0]
     spread = np.zeros_like(r_var)
-    for d in ((1e-9, 0, 0), (0, 0, 1e-9), (0, -1e-9, 0)):
+    for d in ((mag, 0, 0), (0, 0, mag), (0, -mag, 0)):
         jp = oracle.make_job(3, axes, [ic[i] + d[i] for i in range(3)], vals, 0.01, 50.0, 500.0, NT=0.1)
         rp = oracle.lyap_var_cells(funcs, 1, jp, cells, threads=8)["values"][0]
         spread = np.maximum(spread, np.abs(rp - r_var))
@@ -158,7 +163,7 @@ def test_lle_rossler_f32c(gpu_engine):
     s, axes, vals = _rossler_lle_case(gpu_engine)
     g = ucuda.run_lyap(s, axes, [1.0, 1.0, 1.0], vals, 0.01, 50.0, 500.0, 0.1,
                        precision=ucuda.PREC_F32C)["values"].cpu().numpy()[0]
-    step, job, r_var, repro, spread = _lle_oracles(gpu_engine, s, axes, vals, 256)
+    step, job, r_var, repro, spread = _lle_oracles(gpu_engine, s, axes, vals, 256, mag=1e-6)
     ok = r_var != 999.0
     np.testing.assert_array_equal(g == 999.0, ~ok)
     err = np.abs(g - r_var)
