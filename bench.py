@@ -47,7 +47,7 @@ def steps_per_cell(w):
     return skip + 1 + blocks * (nts + 1)
 
 
-def run_gpu(sysm, w, precision, steps, warmup):
+def run_gpu(sysm, w, precision, steps, warmup, strict=False):
     import torch
     import ucuda
     n = ucuda.n_cells(w["axes"])
@@ -56,14 +56,14 @@ def run_gpu(sysm, w, precision, steps, warmup):
                flags=torch.empty(n, dtype=torch.int32, device=dev), final_state=None)
     args = (sysm, w["axes"], w["ic"], sysm.values(a=0.2), w["h"], w["transient"], w["t_max"], w["NT"])
     for _ in range(warmup):
-        ucuda.run_lyap(*args, precision=precision, out=out)
+        ucuda.run_lyap(*args, precision=precision, strict=strict, out=out)
     stream = torch.cuda.current_stream()
     events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for ev in events:
         ev[0].record(stream)
-        ucuda.run_lyap(*args, precision=precision, out=out, stream=stream, sync=False)
+        ucuda.run_lyap(*args, precision=precision, strict=strict, out=out, stream=stream, sync=False)
         ev[1].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -74,9 +74,11 @@ def run_gpu(sysm, w, precision, steps, warmup):
 
 def cpu_baseline(eng, sysm, w, n, stride, threads_all):
     """Times the reference CPU path (finite-difference Benettin, FP64, compiled
-    step body) single-threaded and on all cores over a fixed subsample, and
-    computes the FP64 variational oracle on the same cells twice (X0 and X0
-    perturbed by 1e-9) to separate reproducible cells from sensitive ones."""
+    step body) single-threaded and on all cores over a fixed subsample, then
+    computes on the same cells the FP64 variational oracle and its spread
+    under rounding-sized state noise and parameter perturbations, for FP64
+    (1e-15) and FP32 (~1e-7), which separates reproducible cells from
+    sensitive (chaotic) ones."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
@@ -85,37 +87,31 @@ def cpu_baseline(eng, sysm, w, n, stride, threads_all):
     job = oracle.make_job(3, w["axes"], w["ic"], vals, w["h"], w["transient"], w["t_max"], NT=w["NT"])
     idx = np.arange(0, n, stride)
     cells = (idx[:, None] * n + idx[None, :]).ravel()  # iy * n + ix, every stride-th point per axis
+    progress("cpu baseline: %d cells, single thread" % len(cells))
     t0 = time.perf_counter()
     r1 = oracle.lle_fd_cells(step, job, cells, threads=1)
     t1 = time.perf_counter() - t0
+    progress("cpu baseline: single thread %.1fs; now %d threads" % (t1, threads_all))
     t0 = time.perf_counter()
     rN = oracle.lle_fd_cells(step, job, cells, threads=threads_all)
     tN = time.perf_counter() - t0
     assert (r1["values"] == rN["values"]).all()
-    progress("cpu baseline: %d cells single-thread %.1fs, %d threads %.1fs" % (len(cells), t1, threads_all, tN))
+    progress("cpu baseline: %d threads %.1fs" % (threads_all, tN))
     funcs = oracle.SystemFuncs(sysm.source(1))
     var = oracle.lyap_var_cells(funcs, 1, job, cells, threads=threads_all)["values"][0]
-    spreads = {}
-    for mag in (1e-9, 1e-6):
-        spread = np.zeros_like(var)
-        for d in ((mag, 0, 0), (0, 0, mag), (0, -mag, 0)):
-            icp = [w["ic"][i] + d[i] for i in range(3)]
-            jobp = oracle.make_job(3, w["axes"], icp, vals, w["h"], w["transient"], w["t_max"], NT=w["NT"])
-            varp = oracle.lyap_var_cells(funcs, 1, jobp, cells, threads=threads_all)["values"][0]
-            spread = np.maximum(spread, np.abs(varp - var))
-        spreads[mag] = spread
-    progress("oracle variational + perturbations done")
+    spreads = {p: oracle.lyap_rounding_spread(funcs, 1, job, cells, var, p, threads=threads_all)[0]
+               for p in ("f64", "f32")}
+    progress("oracle variational + rounding spreads done")
     return dict(cells=cells, fd=r1["values"][0], var=var, spreads=spreads, t1=t1, tN=tN)
 
 
-def lle_errors(g, cpu, mag):
+def lle_errors(g, cpu, rounding):
     """Error of a GPU LLE map against the FP64 oracle on the sampled cells.
-    Reproducible cells: three perturbations of X0 of size `mag` (1e-9 for
-    FP64, 1e-6 = float rounding size for FP32-compensated) move the FP64
-    exponent by <= 1e-5."""
+    Reproducible cells: the oracle's exponent moves by <= 1e-5 under state
+    noise and parameter perturbations of the `rounding` precision's size."""
     import numpy as np
     g = g[cpu["cells"]]
-    var, spread, fd = cpu["var"], cpu["spreads"][mag], cpu["fd"]
+    var, spread, fd = cpu["var"], cpu["spreads"][rounding], cpu["fd"]
     ok = var != 999.0
     repro = ok & (spread <= 1e-5)
     sens = ok & ~repro
@@ -129,6 +125,16 @@ def lle_errors(g, cpu, mag):
                 median_fp64_spread_sensitive=float(np.median(spread[sens])) if sens.any() else None,
                 vs_reference_fd_eps1e4=dict(max=float(efd[ok].max()), median=float(np.median(efd[ok]))),
                 flag_mismatch=int(((g == 999.0) != ~ok).sum()))
+
+
+def strict_check(g, cpu):
+    """Strict FP64 (no FMA contraction, reference operation order) against the
+    oracle on the sampled cells: expected bit-identical on every cell,
+    chaotic ones included."""
+    import numpy as np
+    g = g[cpu["cells"]]
+    return dict(cells=int(len(g)), bit_identical=int((g == cpu["var"]).sum()),
+                max_abs_err=float(np.abs(g - cpu["var"]).max()))
 
 
 def extra_ls(eng, n=1024, steps_scale=1.0):
@@ -275,8 +281,18 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
             jid = srv.submit_bif(lor, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, steps * 0.005, steps * 0.005,
                                  max_peaks=8, precision=ucuda.PREC_F32C)
         ids.append((jid, kind, side, steps, axes))
-    for jid, *_ in ids:
-        srv.wait(jid, 600.0)
+    progress("serving: %d jobs submitted, draining" % len(ids))
+    pending, last = [jid for jid, *_ in ids], time.perf_counter()
+    deadline = last + 300.0
+    while pending and time.perf_counter() < deadline:
+        pending = [jid for jid in pending if srv.poll(jid)[0] not in (2, 3, 4, 5)]
+        if pending:
+            time.sleep(0.02)
+        if time.perf_counter() - last > 15.0:
+            progress("serving: %d jobs still running" % len(pending))
+            last = time.perf_counter()
+    if pending:
+        raise RuntimeError("serving: %d jobs unfinished after 300 s" % len(pending))
     wall = time.perf_counter() - t0
     st = srv.stats()
     # audit: ~1% of jobs recomputed alone in FP64 (reported max |diff| for LLE jobs)
@@ -308,6 +324,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the LS / basins / batch / serving measurements")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--only", choices=["f32c", "f64"], default=None, help="time one precision only (profiling)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="after timing, run one device-to-device copy of this size (HBM counter calibration)")
+    ap.add_argument("--dump-cells", default="", help="write the sampled GPU and oracle exponents to this .npz")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's path runs on a single MI355X "
@@ -329,15 +349,35 @@ def main():
 
     res = {}
     for name, prec in (("f64", ucuda.PREC_F64), ("f32c", ucuda.PREC_F32C)):
+        if args.only and name != args.only:
+            continue
         res[name] = run_gpu(sysm, w, prec, args.steps, args.warmup)
         progress("%s LLE map: %.2f ms per map" % (name, sum(res[name]["kernel_ms"]) / len(res[name]["kernel_ms"])))
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and not args.only:
+        strict = run_gpu(sysm, w, ucuda.PREC_F64, 1, 0, strict=True)
+        progress("f64 strict LLE map: %.2f ms" % strict["kernel_ms"][0])
         cb = cpu_baseline(eng, sysm, w, args.n, args.cpu_stride, min(16, os.cpu_count() or 1))
-        errs = {"f64": lle_errors(res["f64"]["values"], cb, 1e-9), "f32c": lle_errors(res["f32c"]["values"], cb, 1e-6)}
+        errs = {"f64": lle_errors(res["f64"]["values"], cb, "f64"), "f32c": lle_errors(res["f32c"]["values"], cb, "f32"),
+                "f64_strict": strict_check(strict["values"], cb)}
         cpu = dict(cells=len(cb["cells"]), t1=cb["t1"], tN=cb["tN"], errs=errs)
+        if args.dump_cells:
+            np.savez(args.dump_cells, cells=cb["cells"], var=cb["var"], fd=cb["fd"], spread_f64=cb["spreads"]["f64"],
+                     spread_f32=cb["spreads"]["f32"], f64=res["f64"]["values"][cb["cells"]],
+                     f32c=res["f32c"]["values"][cb["cells"]], f64_strict=strict["values"][cb["cells"]])
 
+    if args.calibrate_copy_mib:
+        src = torch.empty(args.calibrate_copy_mib << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        progress("copy calibration: %d MiB read + %d MiB written" % (args.calibrate_copy_mib, args.calibrate_copy_mib))
+    if args.only:
+        r = res[args.only]
+        print(json.dumps({"only": args.only, "ms_per_map": float(np.mean(r["kernel_ms"])),
+                          "traj_steps_per_s": traj_steps / (float(np.mean(r["kernel_ms"])) / 1e3)}))
+        return
     f32 = res["f32c"]
     ms = float(np.mean(f32["kernel_ms"]))
     value = traj_steps / (ms / 1e3)
@@ -369,12 +409,13 @@ def main():
     }
     if not args.no_extra:
         cap = value  # traj-steps/s of the headline kernel sets the serving load
+        progress("LS spectrum map (hyper-Rossler 1024^2) start")
         line["ls_hyper_rossler_1024"] = extra_ls(eng)
-        progress("LS spectrum map done")
+        progress("LS spectrum map done; basins (Duffing 2048^2) start")
         line["basins_duffing_2048"] = extra_basins(eng)
-        progress("basins done")
+        progress("basins done; batch (64 jobs) start")
         line["batch_64_jobs"] = extra_batch(eng)
-        progress("batch done")
+        progress("batch done; serving (%.0f s of Poisson arrivals) start" % args.serve_seconds)
         line["serving"] = extra_serving(eng, duration_s=args.serve_seconds, capacity=cap)
         progress("serving done")
         line["p99_job_latency_ms"] = line["serving"]["p99_ms"]

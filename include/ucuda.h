@@ -161,7 +161,9 @@ typedef struct uc_basins_output {
 } uc_basins_output;
 
 /* ---- engine ---- */
-/* device < 0 creates a compile-only engine (no HIP device is touched). */
+/* device < 0 creates a compile-only engine (no HIP device is touched).
+ * Batches and servers keep their engine alive: uc_engine_destroy may be
+ * called before them and the engine is freed with the last of them. */
 int uc_engine_create(int device, uc_engine** out);
 void uc_engine_destroy(uc_engine* e);
 const char* uc_last_error(const uc_engine* e);

@@ -69,6 +69,8 @@ def lib():
         L.oc_lle_fd_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
         L.oc_ls_fd_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
         L.oc_lyap_var_cells.argtypes = [_P, _P, C.c_int, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
+        L.oc_lyap_var_cells_noisy.argtypes = [_P, _P, C.c_int, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int,
+                                              C.c_double, C.c_ulonglong]
         _lib = L
     return _lib
 
@@ -270,6 +272,54 @@ def lyap_var_cells(funcs, K, job, cells, threads=1):
     lib().oc_lyap_var_cells(funcs.rhs, funcs.tan, K, C.byref(job), cp, len(cells), out.ctypes.data_as(_D),
                             flags.ctypes.data_as(_I), threads)
     return dict(values=out, flags=flags)
+
+
+def lyap_var_cells_noisy(funcs, K, job, cells, noise, noise_seed, threads=1):
+    """Variational exponents with a relative state kick of size `noise` after
+    every 100 transient steps and every NT block; used to find the cells whose
+    finite-time exponent survives rounding-sized noise."""
+    cells, cp = _cells(cells)
+    out = np.zeros((K, len(cells)))
+    flags = np.zeros(len(cells), dtype=np.int32)
+    lib().oc_lyap_var_cells_noisy(funcs.rhs, funcs.tan, K, C.byref(job), cp, len(cells), out.ctypes.data_as(_D),
+                                  flags.ctypes.data_as(_I), threads, C.c_double(noise), C.c_ulonglong(noise_seed))
+    return dict(values=out, flags=flags)
+
+
+# Relative rounding sizes of the two engine precisions, used as state noise
+# (per 100 transient steps / per NT block) and as parameter perturbation:
+# FP64 ~ a few ulp of double, FP32-compensated ~ one float rounding (the
+# stage arithmetic and the parameters are float there).
+ROUNDING = {"f64": (1e-15, 1e-15), "f32": (1e-7, 6e-8)}
+_PERTURB_SIGNS = ((1, -1, 1, -1, 1), (-1, 1, -1, 1, -1), (1, 1, -1, -1, 1))
+
+
+def perturbed_job(job, dp, signs):
+    """Copy of `job` with the axis ends (lo0, hi0, lo1, hi1) and the base
+    values scaled by (1 + dp * sign)."""
+    axes = [dict(kind=job.axis_kind[i], index=job.axis_index[i], n_pts=job.axis_n[i], log_scale=bool(job.axis_log[i]),
+                 lo=job.axis_lo[i] * (1.0 + dp * signs[2 * i]), hi=job.axis_hi[i] * (1.0 + dp * signs[2 * i + 1]))
+            for i in range(job.n_axes)]
+    values = [job._params[i] * (1.0 + dp * signs[4]) for i in range(job.n_values)]
+    return make_job(job.n_vars, axes, [job._ic[i] for i in range(job.n_vars)], values, job.h, job.transient_time,
+                    job.t_max, max_value=job.max_value, writable_var=job.writable_var, pre_scaller=job.pre_scaller,
+                    max_peaks=job.max_peaks, NT=job.NT, eps=job.eps, probe_seed=job.probe_seed, ic_seed=job.ic_seed,
+                    ic_lo=job.ic_lo, ic_hi=job.ic_hi)
+
+
+def lyap_rounding_spread(funcs, K, job, cells, base, precision, threads=1):
+    """Largest change of the variational exponents `base` over three runs
+    that each add state noise and a parameter perturbation of the given
+    precision's rounding size (ROUNDING), with distinct seeds and signs. A
+    cell whose spread is small has a finite-time exponent that any
+    implementation rounding at that size must reproduce; chaotic cells move
+    by ~1e-2."""
+    noise, dp = ROUNDING[precision]
+    spread = np.zeros_like(np.asarray(base, dtype=np.float64))
+    for k, signs in enumerate(_PERTURB_SIGNS):
+        r = lyap_var_cells_noisy(funcs, K, perturbed_job(job, dp, signs), cells, noise, 1000 + k, threads=threads)
+        spread = np.maximum(spread, np.abs(r["values"] - base))
+    return spread
 
 
 def peak_finder(data, h):

@@ -457,8 +457,24 @@ void oc_ls_fd_cells(oc_step_fn step, const oc_job* j, const long long* cells, lo
  * loop's trailing step; K = 1 renormalises (floor 1e-14 like the FD
  * distance), K > 1 re-orthonormalises by modified Gram-Schmidt; exponents
  * are the log-norm sums over t_max. */
+/* Rounding-noise model for reproducibility classification (noise > 0 only):
+ * after every chunk of steps each state component is scaled by
+ * (1 + noise * u), u uniform in [-1, 1) from a per-cell splitmix64 stream.
+ * noise = 0 leaves the pinned path untouched. */
+static double noise_u(unsigned long long* s) {
+    unsigned long long z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+static void noise_kick(double* x, int n, double noise, unsigned long long* s) {
+    for (int i = 0; i < n; ++i) x[i] = x[i] * (1.0 + noise * noise_u(s));
+}
+
 static void lyap_var_cell(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, long long cell, double* out,
-                          long long stride, int* flag) {
+                          long long stride, int* flag, double noise, unsigned long long noise_seed) {
     const int n = j->n_vars;
     double a[OC_MAXV], x[OC_MAXN], h, V[OC_MAXN * OC_MAXN], d[OC_MAXN], sum[OC_MAXN];
     if (!oc_cell_setup(j, cell, a, x, &h)) {
@@ -470,13 +486,29 @@ static void lyap_var_cell(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, 
     lyap_counts(j, h, &skip, &nts);
     const long long blocks = (long long)(j->t_max / j->NT);
     tan_ctx ctx = {rhs, tan, V, K, n, 0.0};
-    int alive = loop_generic(adv_rhs, &ctx, x, a, h, skip, n, 1, 0, j->max_value, NULL) != 0;
+    unsigned long long ns = noise_seed ^ ((unsigned long long)cell * 0xD1B54A32D192ED03ull);
+    int alive;
+    if (noise > 0.0) {
+        /* chunks of 99 loop steps + the loop's trailing step = 100 steps each,
+         * then the remainder with its trailing step: the same skip + 1 steps
+         * and the same divergence-check steps as the single call below */
+        alive = 1;
+        long long done = 0;
+        for (; skip - done > 100 && alive; done += 100) {
+            alive = loop_generic(adv_rhs, &ctx, x, a, h, 99, n, 1, 0, j->max_value, NULL) != 0;
+            noise_kick(x, n, noise, &ns);
+        }
+        if (alive) alive = loop_generic(adv_rhs, &ctx, x, a, h, skip - done, n, 1, 0, j->max_value, NULL) != 0;
+    } else {
+        alive = loop_generic(adv_rhs, &ctx, x, a, h, skip, n, 1, 0, j->max_value, NULL) != 0;
+    }
     for (int k = 0; k < K; ++k) sum[k] = 0.0;
     if (alive) {
         probe_basis(j->probe_seed, cell, K, n, V);
         if (K > 1) oc_mgs(V, K, n, d);
         for (long long b = 0; b < blocks; ++b) {
             if (loop_generic(adv_tan, &ctx, x, a, h, nts, n, 1, 0, j->max_value, NULL) == 0) { alive = 0; break; }
+            if (noise > 0.0) noise_kick(x, n, noise, &ns);
             if (K == 1) {
                 double nrm = 0.0;
                 for (int i = 0; i < n; ++i) nrm = nrm + V[i] * V[i];
@@ -498,7 +530,15 @@ static void lyap_var_cell(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, 
 void oc_lyap_var_cells(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, const long long* cells, long long n_sel,
                        double* out, int* flags, int threads) {
 #pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 1 ? threads : 1)
-    for (long long s = 0; s < n_sel; ++s) lyap_var_cell(rhs, tan, K, j, cells[s], out + s, n_sel, &flags[s]);
+    for (long long s = 0; s < n_sel; ++s) lyap_var_cell(rhs, tan, K, j, cells[s], out + s, n_sel, &flags[s], 0.0, 0);
+}
+
+void oc_lyap_var_cells_noisy(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, const long long* cells,
+                             long long n_sel, double* out, int* flags, int threads, double noise,
+                             unsigned long long noise_seed) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads > 1 ? threads : 1)
+    for (long long s = 0; s < n_sel; ++s)
+        lyap_var_cell(rhs, tan, K, j, cells[s], out + s, n_sel, &flags[s], noise, noise_seed);
 }
 
 /* ------------------------------------------------------------- basins ---

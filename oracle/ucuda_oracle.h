@@ -77,6 +77,12 @@ void oc_ls_fd_cells(oc_step_fn step, const oc_job* j, const long long* cells, lo
                     int* flags, int threads);
 void oc_lyap_var_cells(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, const long long* cells, long long n_sel,
                        double* out, int* flags, int threads);
+/* same with a relative state kick of size `noise` after every 100 transient
+ * steps and every NT block (models per-step rounding of a lower precision or
+ * of another operation order; used to classify reproducible cells) */
+void oc_lyap_var_cells_noisy(oc_rhs_fn rhs, oc_tan_fn tan, int K, const oc_job* j, const long long* cells,
+                             long long n_sel, double* out, int* flags, int threads, double noise,
+                             unsigned long long noise_seed);
 
 #ifdef __cplusplus
 }

@@ -40,6 +40,23 @@ def _alone(kind, sysm, kw):
                          kw.pop("t_max"), **kw)
 
 
+def _written(o, key):
+    """Host copy of output `key` with the slots a kernel does not write set to
+    zero: bifurcation rows k >= flags[c] of peaks / intervals (a cell writes
+    only its own peak count; the rest of the buffer is whatever the allocator
+    held)."""
+    v = o[key].cpu().clone()
+    if key in ("peaks", "intervals"):
+        n = o["flags"].cpu().clamp(min=0).to(torch.int64)
+        v[torch.arange(v.shape[0])[:, None] >= n[None, :]] = 0
+    return v
+
+
+def _same(a, b):
+    return a.shape == b.shape and bool(torch.all((a == b) | (torch.isnan(a) & torch.isnan(b)))) \
+        if a.is_floating_point() else torch.equal(a, b)
+
+
 def test_batch_equals_alone_and_replays(gpu_engine):
     print("case: 6 heterogeneous jobs (5 systems, lle/ls/bif, f64/f32c) in one graph launch", flush=True)
     jobs = _jobs(gpu_engine)
@@ -58,11 +75,12 @@ def test_batch_equals_alone_and_replays(gpu_engine):
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         b.launch(stream)
-    first = [{k: v.clone() for k, v in o.items()} for o in outs]
-    for (kind, sysm, kw), o in zip(jobs, outs):
+    torch.cuda.synchronize()
+    first = [{k: _written(o, k) for k in o} for o in outs]
+    for (kind, sysm, kw), f in zip(jobs, first):
         ref = _alone(kind, sysm, kw)
-        for key in o:
-            assert torch.equal(o[key].cpu(), ref[key].cpu()), (kind, sysm.name, key)
+        for key in f:
+            assert _same(f[key], _written(ref, key)), (kind, sysm.name, key)
     # replays with eager work in between reproduce the first launch
     for _ in range(3):
         for o in outs:
@@ -73,7 +91,7 @@ def test_batch_equals_alone_and_replays(gpu_engine):
         torch.cuda.synchronize()
         for o, f in zip(outs, first):
             for key in o:
-                assert torch.equal(o[key], f[key]), key
+                assert _same(_written(o, key), f[key]), key
 
 
 def test_batch_tuner_and_shapes(gpu_engine):

@@ -246,3 +246,22 @@ def test_krs_index_checker():
     assert oracle.check_indices("X[0] = X[2] + a[3];", 3, 4) == []
     d = oracle.check_indices("X[0] = X[3];\n a[4] = 1; // X[9]", 3, 4)
     assert [line for line, _ in d] == [1, 2]
+
+
+def test_rounding_noise_model(cpu_engine):
+    """The noisy variant takes exactly the plain path's steps: a kick size
+    that rounds away (1 + 1e-300 == 1) gives bit-identical exponents. Under
+    FP64-sized noise and parameter perturbation, a period-1 Rossler cell
+    (c = 2.5) keeps its exponent to 1e-5 while a chaotic one (c = 5.7) moves
+    by far more (over 550 time units a 1e-15 kick grows by e^(0.07 * 550))."""
+    s = ucuda.preset("rossler", cpu_engine)
+    funcs = oracle.SystemFuncs(s.source(1))
+    ax = [ucuda.axis(ucuda.AXIS_PARAM, 3, 2, 2.5, 5.7)]
+    job = oracle.make_job(3, ax, [1.0, 1.0, 1.0], s.values(), 0.01, 50.0, 500.0, NT=0.1)
+    var = oracle.lyap_var_cells(funcs, 1, job, [0, 1])["values"]
+    same = oracle.lyap_var_cells_noisy(funcs, 1, job, [0, 1], 1e-300, 7)["values"]
+    assert np.array_equal(var, same)
+    assert np.array_equal(oracle.lyap_var_cells(funcs, 1, oracle.perturbed_job(job, 0.0, (1, 1, 1, 1, 1)),
+                                                [0, 1])["values"], var)
+    spread = oracle.lyap_rounding_spread(funcs, 1, job, [0, 1], var, "f64")[0]
+    assert spread[0] <= 1e-5 < 1e-3 < spread[1], spread

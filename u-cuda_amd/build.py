@@ -44,7 +44,7 @@ def build_library(verbose=False):
            "-Wall", "-Wno-unused-function",
            "-o", LIB + ".tmp",
            os.path.join(CSRC, "engine.cpp"), os.path.join(CSRC, "codegen.cpp"),
-           os.path.join(CSRC, "parametric_engine.cpp"),
+           os.path.join(CSRC, "parametric_engine.cpp"), os.path.join(CSRC, "data_export.cpp"),
            "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-lamdhip64",
            "-Wl,-rpath," + os.path.join(ROCM, "lib")]
     if verbose:

@@ -1,0 +1,127 @@
+// data_export.cpp — CSV writers (see the header for the format source).
+#include "data_export.hpp"
+
+#include <iomanip>
+
+namespace ucuda {
+namespace csv {
+
+namespace {
+constexpr int kPrecision = 15;
+
+void list(std::ostream& out, const char* name, const std::vector<double>& v) {
+    out << name << "[" << v.size() << "] = { ";
+    for (size_t k = 0; k < v.size(); ++k) {
+        out << v[k];
+        out << (k + 1 != v.size() ? ", " : " }\n");
+    }
+}
+}  // namespace
+
+void values_and_ic(std::ostream& out, const std::vector<double>& values, const std::vector<double>& ic) {
+    list(out, "a", values);
+    list(out, "X0", ic);
+}
+
+double param_value_at(size_t i, int n_pts, double lo, double hi) {
+    if (n_pts <= 1) return lo;
+    return lo + (hi - lo) * (double)i / (double)(n_pts - 1);
+}
+
+void bif1d_config(std::ostream& out, const std::vector<double>& values, const std::vector<double>& ic, double t_max,
+                  double transient, double h, int pre_scaller, int writable_var, int index_of_mut_var, double lo,
+                  double hi) {
+    out << std::setprecision(kPrecision);
+    out << "1D classical bifurcation\nParameter estimation\n";
+    values_and_ic(out, values, ic);
+    out << "CT = " << t_max << "\nTT = " << transient << "\nh = " << h << "\ndecimator = " << pre_scaller
+        << "\nindexVar for peakfinder = " << writable_var << "\nindexPar for estimation = " << index_of_mut_var
+        << "\nstart value = " << lo << ", stop value = " << hi << "\n";
+}
+
+void bif1d_rows(std::ostream& out, double param, int npeaks, const double* peaks, const double* times) {
+    out << std::setprecision(kPrecision);
+    if (npeaks == 0) out << param << ", " << 0 << ", " << 0 << '\n';
+    else if (npeaks == -1) out << param << ", " << 0 << ", " << -1 << '\n';
+    else
+        for (int j = 0; j < npeaks; ++j) out << param << ", " << peaks[j] << ", " << times[j] << '\n';
+}
+
+void curve1d_config(std::ostream& out, const char* title, const std::vector<double>& values,
+                    const std::vector<double>& ic, double t_max, double NT, double transient, double h, double eps,
+                    int index_of_mut_var, double lo, double hi) {
+    out << std::setprecision(kPrecision);
+    out << title << "\nParameter estimation\n";
+    values_and_ic(out, values, ic);
+    out << "CT = " << t_max << "\nNT = " << NT << "\nTT = " << transient << "\n";
+    out << "h = " << h << "\neps = " << eps << "\n";
+    out << "indexPar = " << index_of_mut_var << "\n";
+    out << "start value = " << lo << ", stop value = " << hi << "\n";
+}
+
+void lle1d_row(std::ostream& out, double param, double lyapunov) {
+    out << std::setprecision(kPrecision) << param << ", " << lyapunov << '\n';
+}
+
+void ls1d_row(std::ostream& out, double param, const double* exponents, int n_exponents) {
+    out << std::setprecision(kPrecision) << param;
+    for (int j = 0; j < n_exponents; ++j) out << ", " << exponents[j];
+    out << '\n';
+}
+
+void map2d_config(std::ostream& out, const char* title, int par_or_var, const std::vector<double>& values,
+                  const std::vector<double>& ic, double t_max, double NT, double transient, double h, double eps,
+                  int index1, int index2, double lo1, double hi1, double lo2, double hi2, int n_pts, int n_exponents) {
+    out << std::setprecision(kPrecision) << title << "\n";
+    if (par_or_var == 1) out << "Parameter estimation\n";
+    else if (par_or_var == 0) out << "Initial conditions estimation\n";
+    else if (par_or_var == 2) out << "Mixed: x=IC, y=parameter\n";
+    values_and_ic(out, values, ic);
+    out << "CT = " << t_max << "\nNT = " << NT << "\nTT = " << transient << "\n";
+    out << "h = " << h << "\neps = " << eps << "\n";
+    out << "indices = " << index1 << ", " << index2 << "\n";
+    out << "axis1: " << lo1 << " .. " << hi1 << "\n";
+    out << "axis2: " << lo2 << " .. " << hi2 << "\n";
+    out << "n_pts = " << n_pts << "x" << n_pts << "\n";
+    if (n_exponents > 0) out << "exponents = " << n_exponents << "\n";
+}
+
+void grid(std::ostream& out, int n_pts, const double* values) {
+    out << std::setprecision(kPrecision);
+    for (int iy = 0; iy < n_pts; ++iy) {
+        for (int ix = 0; ix < n_pts; ++ix) {
+            out << values[(size_t)iy * n_pts + ix];
+            if (ix + 1 < n_pts) out << ", ";
+        }
+        out << '\n';
+    }
+}
+
+void ls2d_cells(std::ostream& out, int n_pts, int n_exponents, const double* values) {
+    out << std::setprecision(kPrecision);
+    const size_t total = (size_t)n_pts * n_pts;
+    for (size_t cell = 0; cell < total; ++cell) {
+        for (int k = 0; k < n_exponents; ++k) {
+            out << values[(size_t)k * total + cell];
+            if (k + 1 < n_exponents) out << ", ";
+        }
+        out << '\n';
+    }
+}
+
+void basins_ranges(std::ostream& out, double xlo, double xhi, double ylo, double yhi) {
+    out << std::setprecision(kPrecision) << xlo << " " << xhi << "\n" << ylo << " " << yhi << "\n";
+}
+
+void grid_int(std::ostream& out, int n_pts, const int* values) {
+    for (int iy = 0; iy < n_pts; ++iy) {
+        for (int ix = 0; ix < n_pts; ++ix) {
+            out << values[(size_t)iy * n_pts + ix];
+            if (ix + 1 < n_pts) out << ", ";
+        }
+        out << '\n';
+    }
+}
+
+}  // namespace csv
+}  // namespace ucuda

@@ -1,0 +1,44 @@
+// data_export.hpp — CSV files in the reference's on-disk format, so results
+// can be diffed against U-CUDA's exports (U-CUDA/data_export.cpp:16-58,
+// 176-294, 300-466, 472-544). Numbers are printed with 15 significant
+// digits (set_precision, U-CUDA/configCUDA.h:137); separators ", ".
+#pragma once
+#include <fstream>
+#include <string>
+#include <vector>
+
+namespace ucuda {
+namespace csv {
+
+// Header block "a[n] = { ... }" / "X0[n] = { ... }".
+void values_and_ic(std::ostream& out, const std::vector<double>& values, const std::vector<double>& ic);
+
+// Sweep value written next to 1-D results: lo + (hi - lo) * i / (n - 1), n <= 1 -> lo.
+double param_value_at(size_t i, int n_pts, double lo, double hi);
+
+void bif1d_config(std::ostream& out, const std::vector<double>& values, const std::vector<double>& ic, double t_max,
+                  double transient, double h, int pre_scaller, int writable_var, int index_of_mut_var, double lo,
+                  double hi);
+// One row per peak "param, peak, interval"; "param, 0, 0" without peaks,
+// "param, 0, -1" for a fixed point.
+void bif1d_rows(std::ostream& out, double param, int npeaks, const double* peaks, const double* times);
+
+void curve1d_config(std::ostream& out, const char* title, const std::vector<double>& values,
+                    const std::vector<double>& ic, double t_max, double NT, double transient, double h, double eps,
+                    int index_of_mut_var, double lo, double hi);
+void lle1d_row(std::ostream& out, double param, double lyapunov);
+void ls1d_row(std::ostream& out, double param, const double* exponents, int n_exponents);
+
+void map2d_config(std::ostream& out, const char* title, int par_or_var, const std::vector<double>& values,
+                  const std::vector<double>& ic, double t_max, double NT, double transient, double h, double eps,
+                  int index1, int index2, double lo1, double hi1, double lo2, double hi2, int n_pts, int n_exponents);
+// n_pts rows of n_pts values (values[iy * n + ix]).
+void grid(std::ostream& out, int n_pts, const double* values);
+// one row per cell (iy * n + ix) with its n_exponents values (plane-major input).
+void ls2d_cells(std::ostream& out, int n_pts, int n_exponents, const double* values);
+// basins: "lo hi" range lines then the integer grid.
+void basins_ranges(std::ostream& out, double xlo, double xhi, double ylo, double yhi);
+void grid_int(std::ostream& out, int n_pts, const int* values);
+
+}  // namespace csv
+}  // namespace ucuda

@@ -17,6 +17,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -79,6 +80,11 @@ struct uc_system {
 
 struct uc_engine {
     int device = -1;
+    // One reference for the caller's handle plus one per live batch / server:
+    // those hold module pins and table buffers of this engine, so the engine
+    // is freed by whichever of uc_engine_destroy / uc_batch_destroy /
+    // uc_server_destroy drops the last reference, in any order.
+    std::atomic<int> refs{1};
     std::recursive_mutex mu;
     std::string error;
     std::list<Module> modules;      // front = most recently used
@@ -371,7 +377,16 @@ int uc_engine_create(int device, uc_engine** out) {
     return UC_OK;
 }
 
-void uc_engine_destroy(uc_engine* e) { delete e; }
+namespace {
+void engine_release(uc_engine* e) {
+    if (e && e->refs.fetch_sub(1, std::memory_order_acq_rel) == 1) {
+        if (e->device >= 0) (void)hipSetDevice(e->device);
+        delete e;
+    }
+}
+}  // namespace
+
+void uc_engine_destroy(uc_engine* e) { engine_release(e); }
 
 const char* uc_last_error(const uc_engine* e) { return e ? e->error.c_str() : "null engine"; }
 
@@ -914,15 +929,20 @@ int uc_batch_create(uc_engine* e, uc_batch** out) {
     if (rc) return rc;
     auto b = new uc_batch;
     b->e = e;
+    e->refs.fetch_add(1, std::memory_order_relaxed);
     *out = b;
     return UC_OK;
 }
 
 void uc_batch_destroy(uc_batch* b) {
     if (!b) return;
-    std::lock_guard<std::recursive_mutex> lk(b->e->mu);
-    (void)hipSetDevice(b->e->device);
-    delete b;
+    uc_engine* e = b->e;
+    {
+        std::lock_guard<std::recursive_mutex> lk(e->mu);
+        (void)hipSetDevice(e->device);
+        delete b;
+    }
+    engine_release(e);
 }
 
 int uc_batch_add_bif(uc_batch* b, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, int* job_id) {
@@ -1326,6 +1346,7 @@ int uc_server_create(uc_engine* e, const uc_server_config* cfg, uc_server** out)
     hipError_t he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
     if (he != hipSuccess) { delete S; return e->fail(UC_ERR_HIP, hip_msg(he, "server stream")); }
     S->t_start = std::chrono::steady_clock::now();
+    e->refs.fetch_add(1, std::memory_order_relaxed);
     S->worker = std::thread(server_loop, S);
     *out = S;
     return UC_OK;
@@ -1340,13 +1361,18 @@ void uc_server_destroy(uc_server* S) {
     }
     S->cv.notify_all();
     if (S->worker.joinable()) S->worker.join();
-    (void)hipSetDevice(S->e->device);
-    for (auto& kv : S->all) {
-        if (kv.second->m) --kv.second->m->pins;
-        server_free(*kv.second);
+    uc_engine* e = S->e;
+    {
+        std::lock_guard<std::recursive_mutex> lk(e->mu);
+        (void)hipSetDevice(e->device);
+        for (auto& kv : S->all) {
+            if (kv.second->m) --kv.second->m->pins;
+            server_free(*kv.second);
+        }
+        if (S->stream) (void)hipStreamDestroy(S->stream);
+        delete S;
     }
-    if (S->stream) (void)hipStreamDestroy(S->stream);
-    delete S;
+    engine_release(e);
 }
 
 int uc_server_submit_lyap(uc_server* S, uc_system* s, const uc_lyap_request* r, int spectrum, long long* id) {

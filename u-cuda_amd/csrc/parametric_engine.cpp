@@ -9,6 +9,9 @@
 #include <mutex>
 
 #include "../../include/ucuda.h"
+#include "data_export.hpp"
+
+#include <fstream>
 
 namespace ucuda {
 namespace {
@@ -139,6 +142,16 @@ Bifurcation1DResult ParametricEngine::run_bifurcation_1d(const Bifurcation1DRequ
             res.bifurcation_points[c].push_back(pk[(size_t)k * n + c]);
             res.peak_times[c].push_back(iv[(size_t)k * n + c]);
         }
+    if (!req.csv_output_path.empty()) {
+        std::ofstream cfg(req.csv_output_path + "_config.csv");
+        csv::bif1d_config(cfg, req.base_values, req.initial_conditions, req.t_max, req.transient_time, req.h,
+                          req.pre_scaller, req.writable_var, req.sweep_over_var ? req.var_sweep_index : req.param_index,
+                          req.param_lo, req.param_hi);
+        std::ofstream out(req.csv_output_path);
+        for (size_t c = 0; c < n; ++c)
+            csv::bif1d_rows(out, csv::param_value_at(c, req.n_pts, req.param_lo, req.param_hi), f[c],
+                            res.bifurcation_points[c].data(), res.peak_times[c].data());
+    }
     done(req.progress);
     res.ok = true;
     return res;
@@ -187,6 +200,14 @@ LLE1DResult ParametricEngine::run_lle_1d(const LLE1DRequest& req) {
     res.lyapunov.resize(n);
     for (size_t i = 0; i < n; ++i)
         res.lyapunov[i] = (res.flags[i] == 1) ? v[i] : std::numeric_limits<double>::quiet_NaN();
+    if (!req.csv_output_path.empty()) {
+        std::ofstream cfg(req.csv_output_path + "_config.csv");
+        csv::curve1d_config(cfg, "1D LLE", req.base_values, req.initial_conditions, req.t_max, req.NT,
+                            req.transient_time, req.h, req.eps, req.param_index, req.param_lo, req.param_hi);
+        std::ofstream out(req.csv_output_path);
+        for (size_t i = 0; i < n; ++i)
+            csv::lle1d_row(out, csv::param_value_at(i, req.n_pts, req.param_lo, req.param_hi), res.lyapunov[i]);
+    }
     done(req.progress);
     res.ok = true;
     return res;
@@ -222,6 +243,15 @@ LS1DResult ParametricEngine::run_ls_1d(const LS1DRequest& req) {
     res.spectrum.assign(n, std::vector<double>(N));
     for (size_t i = 0; i < n; ++i)
         for (size_t k = 0; k < N; ++k) res.spectrum[i][k] = v[k * n + i];
+    if (!req.csv_output_path.empty()) {
+        std::ofstream cfg(req.csv_output_path + "_config.csv");
+        csv::curve1d_config(cfg, "1D LS", req.base_values, req.initial_conditions, req.t_max, req.NT,
+                            req.transient_time, req.h, req.eps, req.param_index, req.param_lo, req.param_hi);
+        std::ofstream out(req.csv_output_path);
+        for (size_t i = 0; i < n; ++i)
+            csv::ls1d_row(out, csv::param_value_at(i, req.n_pts, req.param_lo, req.param_hi), res.spectrum[i].data(),
+                          (int)N);
+    }
     done(req.progress);
     res.ok = true;
     return res;
@@ -273,6 +303,16 @@ LLE2DResult ParametricEngine::run_lle_2d(const LLE2DRequest& req) {
     res.n_pts = req.n_pts;
     res.param_lo = req.param_lo; res.param_hi = req.param_hi;
     res.param_lo_2 = req.param_lo_2; res.param_hi_2 = req.param_hi_2;
+    if (!req.csv_output_path.empty()) {
+        std::ofstream cfg(req.csv_output_path + "_config.csv");
+        const int pov = (req.sweep_over_var == req.sweep_over_var_2) ? (req.sweep_over_var ? 0 : 1) : 2;
+        csv::map2d_config(cfg, "2D LLE", pov, req.base_values, req.initial_conditions, req.t_max, req.NT,
+                          req.transient_time, req.h, req.eps, req.sweep_over_var ? req.var_sweep_index : req.param_index,
+                          req.sweep_over_var_2 ? req.var_sweep_index_2 : req.param_index_2, req.param_lo, req.param_hi,
+                          req.param_lo_2, req.param_hi_2, req.n_pts, 0);
+        std::ofstream out(req.csv_output_path);
+        csv::grid(out, req.n_pts, res.values.data());
+    }
     done(req.progress);
     res.ok = true;
     return res;
@@ -305,6 +345,16 @@ LS2DResult ParametricEngine::run_ls_2d(const LS2DRequest& req) {
     for (size_t k = 0; k < N; ++k) min_max(res.values.data() + k * n, res.flags.data(), n, res.min_val[k], res.max_val[k]);
     res.param_lo = req.param_lo; res.param_hi = req.param_hi;
     res.param_lo_2 = req.param_lo_2; res.param_hi_2 = req.param_hi_2;
+    if (!req.csv_output_path.empty()) {
+        std::ofstream cfg(req.csv_output_path + "_config.csv");
+        const int pov = (req.sweep_over_var == req.sweep_over_var_2) ? (req.sweep_over_var ? 0 : 1) : 2;
+        csv::map2d_config(cfg, "2D LS", pov, req.base_values, req.initial_conditions, req.t_max, req.NT,
+                          req.transient_time, req.h, req.eps, req.sweep_over_var ? req.var_sweep_index : req.param_index,
+                          req.sweep_over_var_2 ? req.var_sweep_index_2 : req.param_index_2, req.param_lo, req.param_hi,
+                          req.param_lo_2, req.param_hi_2, req.n_pts, (int)N);
+        std::ofstream out(req.csv_output_path);
+        csv::ls2d_cells(out, req.n_pts, (int)N, res.values.data());
+    }
     done(req.progress);
     res.ok = true;
     return res;
@@ -348,6 +398,14 @@ BasinsResult ParametricEngine::run_basins(const BasinsRequest& req) {
     res.axis_x_lo = req.axis_x_lo; res.axis_x_hi = req.axis_x_hi;
     res.axis_y_lo = req.axis_y_lo; res.axis_y_hi = req.axis_y_hi;
     res.axis_x_var = req.axis_x_var; res.axis_y_var = req.axis_y_var;
+    if (!req.csv_output_path.empty()) {
+        std::ofstream out(req.csv_output_path);
+        csv::basins_ranges(out, req.axis_x_lo, req.axis_x_hi, req.axis_y_lo, req.axis_y_hi);
+        csv::grid_int(out, req.n_pts, res.basin_idx.data());
+        std::ofstream help(req.csv_output_path + "_3.csv");
+        csv::basins_ranges(help, req.axis_x_lo, req.axis_x_hi, req.axis_y_lo, req.axis_y_hi);
+        csv::grid_int(help, req.n_pts, res.helpful_array.data());
+    }
     done(req.progress);
     res.ok = true;
     return res;

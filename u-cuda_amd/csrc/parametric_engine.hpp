@@ -42,6 +42,7 @@ struct Bifurcation1DRequest {
     double max_value = 1.0e6;
     int max_peaks = 2500;
     int precision = 0;  // UC_PREC_F64 / UC_PREC_F32C
+    std::string csv_output_path;  // non-empty: write <path> and <path>_config.csv in the reference format
     std::shared_ptr<std::atomic<bool>> cancel;
     std::shared_ptr<std::atomic<float>> progress;
 };
@@ -69,6 +70,7 @@ struct LLE1DRequest {
     int n_pts = 500;
     double h = 0.01, transient_time = 0.0, t_max = 100.0, NT = 1.0, eps = 1.0e-4, max_value = 1.0e6;
     int precision = 0, method = 0;
+    std::string csv_output_path;  // non-empty: write <path> and <path>_config.csv in the reference format
     std::shared_ptr<std::atomic<bool>> cancel;
     std::shared_ptr<std::atomic<float>> progress;
 };
@@ -104,6 +106,7 @@ struct LLE2DRequest {
     int n_pts = 200;
     double h = 0.01, transient_time = 0.0, t_max = 100.0, NT = 1.0, eps = 1.0e-4, max_value = 1.0e6;
     int precision = 0, method = 0;
+    std::string csv_output_path;  // non-empty: write <path> and <path>_config.csv in the reference format
     std::shared_ptr<std::atomic<bool>> cancel;
     std::shared_ptr<std::atomic<float>> progress;
 };
@@ -142,6 +145,7 @@ struct BasinsRequest {
     int n_sections = 200;
     double max_value = 1.0e6;
     int precision = 0;
+    std::string csv_output_path;  // non-empty: write <path> and <path>_config.csv in the reference format
     std::shared_ptr<std::atomic<bool>> cancel;
     std::shared_ptr<std::atomic<float>> progress;
 };

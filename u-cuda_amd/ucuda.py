@@ -8,9 +8,11 @@ memory and the stream; every computation runs in libucuda.so's HIP kernels.
 There is no fallback: if libucuda.so is missing or a launch fails, the call
 raises.
 """
+import atexit
 import ctypes as C
 import math
 import os
+import weakref
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libucuda.so")
@@ -562,6 +564,19 @@ class Batch:
         return dict(zip(("n_jobs", "n_groups", "block", "waves_per_simd"), [x.value for x in v]))
 
 
+_LIVE_SERVERS = weakref.WeakSet()
+
+
+@atexit.register
+def _close_servers():
+    # stop scheduler threads while the HIP runtime is still fully up
+    for srv in list(_LIVE_SERVERS):
+        try:
+            srv.close()
+        except Exception:
+            pass
+
+
 class Server:
     """Continuous-batching server (uc_server). submit_* return a job id;
     fetch() returns numpy results once the job is done."""
@@ -573,6 +588,7 @@ class Server:
         engine.check(lib().uc_server_create(engine.h, C.byref(cfg), C.byref(h)), "uc_server_create")
         self.h = h
         self.meta = {}
+        _LIVE_SERVERS.add(self)
 
     def close(self):
         if getattr(self, "h", None):
