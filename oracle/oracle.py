@@ -66,6 +66,8 @@ def lib():
         L.oc_rk4.argtypes = [_P, _D, _D, C.c_double, C.c_int]
         L.oc_rk4_tan.argtypes = [_P, _P, _D, _D, C.c_int, _D, C.c_double, C.c_int]
         L.oc_bif_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _I, _D, _D, _D, C.c_int, C.c_int]
+        L.oc_bif_cells_noisy.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _I, _D, _D, _D, C.c_int, C.c_int,
+                                         C.c_double, C.c_ulonglong]
         L.oc_lle_fd_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
         L.oc_ls_fd_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
         L.oc_lyap_var_cells.argtypes = [_P, _P, C.c_int, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
@@ -233,7 +235,9 @@ def _cells(cells):
     return cells, cells.ctypes.data_as(_LL)
 
 
-def bif_cells(step, job, cells, traj_len=0, threads=1):
+def bif_cells(step, job, cells, traj_len=0, threads=1, noise=0.0, noise_seed=0):
+    """Bifurcation cells; noise > 0 adds a relative state kick of that size
+    after every 100th step (rounding model, see lyap_rounding_spread)."""
     cells, cp = _cells(cells)
     n = len(cells)
     mp = max(1, job.max_peaks)
@@ -241,9 +245,9 @@ def bif_cells(step, job, cells, traj_len=0, threads=1):
     peaks = np.full((mp, n), np.nan)
     intervals = np.full((mp, n), np.nan)
     traj = np.full((traj_len, n), np.nan) if traj_len else None
-    lib().oc_bif_cells(step.ptr, C.byref(job), cp, n, flags.ctypes.data_as(_I), peaks.ctypes.data_as(_D),
-                       intervals.ctypes.data_as(_D), traj.ctypes.data_as(_D) if traj is not None else None,
-                       traj_len, threads)
+    lib().oc_bif_cells_noisy(step.ptr, C.byref(job), cp, n, flags.ctypes.data_as(_I), peaks.ctypes.data_as(_D),
+                             intervals.ctypes.data_as(_D), traj.ctypes.data_as(_D) if traj is not None else None,
+                             traj_len, threads, C.c_double(noise), C.c_ulonglong(noise_seed))
     return dict(flags=flags, peaks=peaks, intervals=intervals, traj=traj)
 
 
@@ -320,6 +324,26 @@ def lyap_rounding_spread(funcs, K, job, cells, base, precision, threads=1):
         r = lyap_var_cells_noisy(funcs, K, perturbed_job(job, dp, signs), cells, noise, 1000 + k, threads=threads)
         spread = np.maximum(spread, np.abs(r["values"] - base))
     return spread
+
+
+def bif_rounding_stable(step, job, cells, base, precision, rel, min_peaks=9, threads=1):
+    """Cells of `base` (bif_cells output) with more than min_peaks - 1 peaks
+    whose peak count and values (within `rel` of the largest |peak|) survive
+    three runs with state noise and parameter perturbations of the given
+    precision's rounding size (ROUNDING)."""
+    noise, dp = ROUNDING[precision]
+    n = base["flags"]
+    stable = n >= min_peaks
+    for k, signs in enumerate(_PERTURB_SIGNS):
+        r = bif_cells(step, perturbed_job(job, dp, signs), cells, threads=threads, noise=noise, noise_seed=2000 + k)
+        for c in np.nonzero(stable)[0]:
+            m = n[c]
+            if r["flags"][c] != m:
+                stable[c] = False
+                continue
+            d = np.abs(base["peaks"][:m, c] - r["peaks"][:m, c]).max()
+            stable[c] = bool(d <= rel * max(1.0, np.abs(base["peaks"][:m, c]).max()))
+    return stable
 
 
 def peak_finder(data, h):

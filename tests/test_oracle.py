@@ -265,3 +265,36 @@ def test_rounding_noise_model(cpu_engine):
                                                 [0, 1])["values"], var)
     spread = oracle.lyap_rounding_spread(funcs, 1, job, [0, 1], var, "f64")[0]
     assert spread[0] <= 1e-5 < 1e-3 < spread[1], spread
+
+
+def test_bif_rounding_model(cpu_engine):
+    """Bifurcation oracle with the rounding model: a kick that rounds away
+    reproduces the plain run bit for bit; the period-1 Lorenz orbit at
+    rho = 240 keeps its peaks to 1e-8 under FP64-sized noise, while the
+    chaotic rho = 28 cell does not."""
+    s = ucuda.preset("lorenz", cpu_engine)
+    step = oracle.KrsStep(cpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
+    ax = [ucuda.axis(ucuda.AXIS_PARAM, 2, 2, 240.0, 28.0)]
+    job = oracle.make_job(3, ax, [0.1, 0.1, 0.1], s.values(), 0.01, 100.0, 100.0, max_peaks=256)
+    a = oracle.bif_cells(step, job, [0, 1])
+    b = oracle.bif_cells(step, job, [0, 1], noise=1e-300, noise_seed=3)
+    for k in ("flags", "peaks", "intervals"):
+        assert np.array_equal(a[k], b[k], equal_nan=True)
+    stable = oracle.bif_rounding_stable(step, job, [0, 1], a, "f64", 1e-8)
+    assert stable.tolist() == [True, False], (stable, a["flags"])
+
+
+def test_hyperchaotic_rossler_spectrum_published(cpu_engine):
+    """Rossler's hyperchaotic system (a, d) = (0.25, 0.05) from (-10, -6, 0,
+    10.1): two positive exponents, one zero. Published spectrum (Wolf, Swift,
+    Swinney & Vastano 1985): 0.112, 0.019, 0, -25.2; the oracle's exponents
+    are sums over (NT/h + 1) steps per block divided by t_max, hence the
+    20/21 factor."""
+    s = ucuda.preset("hyper_rossler", cpu_engine)
+    funcs = oracle.SystemFuncs(s.source(1))
+    job = oracle.make_job(4, [ucuda.axis(ucuda.AXIS_PARAM, 2, 1, 0.05, 0.05)], [-10, -6, 0, 10.1], s.values(),
+                          0.005, 0.0, 2000.0, NT=0.1)
+    r = oracle.lyap_var_cells(funcs, 4, job, [0])
+    assert r["flags"][0] == 1
+    l1, l2, l3, l4 = r["values"][:, 0] * 20 / 21
+    assert abs(l1 - 0.112) < 0.01 and abs(l2 - 0.019) < 0.005 and abs(l3) < 2e-3 and -30 < l4 < -20

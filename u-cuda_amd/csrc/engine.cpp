@@ -1077,16 +1077,22 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
 
 // =====================================================================
 // Serving: continuous batching. A scheduler thread admits submitted jobs
-// in arrival order while their output buffers fit in the HBM budget,
-// splits every admitted job into chunks of cells and runs rounds: each
-// round takes chunks from all admitted jobs (oldest first, at least one
-// chunk per job, up to a traj-step budget), launches one persistent kernel
-// per (module, kernel) group over the round's job table, and waits for it.
-// Jobs that arrive while a round runs join the next round. A job's cells
-// compute exactly as in a direct launch (same kernel code per cell; the
-// output plane stride stays the job's own), so results do not depend on
-// what else shares the round. Outputs stay in device memory until fetched.
+// in arrival order while their output buffers fit in the HBM budget and
+// keeps up to kSlots rounds in flight, one per HIP stream. A round takes
+// cells from the admitted jobs (a fair share for every job first, then
+// oldest first) up to the lanes one slot's share of the GPU holds, and
+// launches one persistent kernel per (module, kernel) group over the
+// round's job table; the next round is built and launched as soon as a
+// slot is free, so a round of long cells does not hold back short jobs and
+// the tails of rounds overlap. Jobs that arrive meanwhile join the next
+// round. A job's cells compute exactly as in a direct launch (same kernel
+// code per cell; the output plane stride stays the job's own), so results
+// do not depend on what else shares a round. Outputs stay in device memory
+// until fetched. Device buffers use the stream-ordered allocator where the
+// device has one (hipFree synchronises the whole device).
 struct uc_server {
+    static constexpr int kSlots = 4;
+    static constexpr int kMaxGroups = 64;  // counters per slot
     uc_engine* e = nullptr;
     uc_server_config cfg{};
     struct Job {
@@ -1102,9 +1108,12 @@ struct uc_server {
         double* out1 = nullptr;
         size_t out0_n = 0, out1_n = 0;
         size_t bytes = 0;
-        long long cells = 0, next = 0, done_cells = 0;
-        double step_cost = 1.0;  // traj-steps per cell
-        int state = 0;           // 0 queued, 1 running, 2 done, 3 cancelled, 4 failed
+        long long cells = 0;
+        long long next = 0;        // cells handed to rounds
+        long long done_cells = 0;  // cells of completed rounds
+        int inflight = 0;          // rounds in flight that hold cells of this job
+        double step_cost = 1.0;    // traj-steps per cell
+        int state = 0;             // 0 queued, 1 running, 2 done, 3 cancelled, 4 failed, 5 fetched
         std::chrono::steady_clock::time_point t_submit, t_done;
         // request copies (pointers inside refer to these)
         std::vector<double> ic, vals;
@@ -1115,6 +1124,22 @@ struct uc_server {
         long long iters = 0;
         std::string error;
     };
+    struct Entry {
+        std::shared_ptr<Job> job;
+        long long first, cells;
+    };
+    struct Slot {
+        hipStream_t stream = nullptr;
+        hipEvent_t done = nullptr;
+        UcJob* d_jobs = nullptr;
+        long long* d_start = nullptr;
+        unsigned long long* d_ctr = nullptr;
+        UcJob* h_jobs = nullptr;      // pinned staging, read by the copy until `done`
+        long long* h_start = nullptr;
+        size_t cap = 0;               // entries (jobs); starts hold cap + kMaxGroups
+        bool busy = false;
+        std::vector<Entry> entries;
+    };
     std::mutex mu;
     std::condition_variable cv;
     std::deque<std::shared_ptr<Job>> pending;
@@ -1123,8 +1148,12 @@ struct uc_server {
     long long next_id = 1;
     size_t used_bytes = 0, budget = 0;
     bool stop = false;
+    bool async_alloc = false;
     std::thread worker;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // admission uploads, fetch copies, frees
+    Slot slots[kSlots];
+    int n_busy = 0;
+    std::chrono::steady_clock::time_point busy_since;
     std::vector<double> latencies_ms;
     double busy_s = 0.0, traj_steps_done = 0.0;
     long long rounds = 0;
@@ -1134,11 +1163,20 @@ struct uc_server {
 
 namespace {
 
-void server_free(uc_server::Job& j) {
-    if (j.flags) (void)hipFree(j.flags);
-    if (j.out0) (void)hipFree(j.out0);
-    if (j.out1) (void)hipFree(j.out1);
-    if (j.tables) (void)hipFree(j.tables);
+hipError_t server_malloc(uc_server* S, void** p, size_t bytes, hipStream_t s) {
+    return S->async_alloc ? hipMallocAsync(p, bytes, s) : hipMalloc(p, bytes);
+}
+
+void server_release(uc_server* S, void* p, hipStream_t s) {
+    if (p) (void)(S->async_alloc ? hipFreeAsync(p, s) : hipFree(p));
+}
+
+// Frees a job's device buffers; no round in flight may hold its cells.
+void server_free(uc_server* S, uc_server::Job& j) {
+    server_release(S, j.flags, S->stream);
+    server_release(S, j.out0, S->stream);
+    server_release(S, j.out1, S->stream);
+    server_release(S, j.tables, S->stream);
     j.flags = nullptr; j.out0 = j.out1 = nullptr; j.tables = nullptr;
 }
 
@@ -1147,12 +1185,14 @@ bool server_admit(uc_server* S, uc_server::Job& j) {
     const size_t need = table_doubles(j.c);
     std::vector<double> host(need);
     fill_tables(j.n_vars, j.c, host.data());
-    hipError_t he = hipMalloc((void**)&j.tables, need * sizeof(double));
-    if (he == hipSuccess) he = hipMemcpy(j.tables, host.data(), need * sizeof(double), hipMemcpyHostToDevice);
-    if (he == hipSuccess) he = hipMalloc((void**)&j.flags, sizeof(int) * (size_t)j.cells);
-    if (he == hipSuccess) he = hipMalloc((void**)&j.out0, sizeof(double) * std::max<size_t>(1, j.out0_n));
-    if (he == hipSuccess && j.out1_n) he = hipMalloc((void**)&j.out1, sizeof(double) * j.out1_n);
-    if (he != hipSuccess) { server_free(j); j.error = hip_msg(he, "admission"); return false; }
+    hipError_t he = server_malloc(S, (void**)&j.tables, need * sizeof(double), S->stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(j.tables, host.data(), need * sizeof(double), hipMemcpyHostToDevice, S->stream);
+    if (he == hipSuccess) he = server_malloc(S, (void**)&j.flags, sizeof(int) * (size_t)j.cells, S->stream);
+    if (he == hipSuccess) he = server_malloc(S, (void**)&j.out0, sizeof(double) * std::max<size_t>(1, j.out0_n), S->stream);
+    if (he == hipSuccess && j.out1_n) he = server_malloc(S, (void**)&j.out1, sizeof(double) * j.out1_n, S->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(S->stream);  // tables uploaded, buffers usable on any stream
+    if (he != hipSuccess) { server_free(S, j); j.error = hip_msg(he, "admission"); return false; }
     wire_job(j.c, j.tables, j.tmpl);
     if (j.kind == 0) {
         uc_bif_output o{j.flags, j.out0, j.out1, nullptr, 0, nullptr};
@@ -1161,144 +1201,235 @@ bool server_admit(uc_server* S, uc_server::Job& j) {
         uc_lyap_output o{j.out0, j.flags, nullptr};
         fill_lyap(&j.lyap, &o, j.tmpl);
     }
-    (void)S;
     return true;
+}
+
+bool slot_reserve(uc_server* S, uc_server::Slot& sl, size_t n_entries) {
+    if (n_entries <= sl.cap) return true;
+    size_t cap = std::max<size_t>(256, sl.cap);
+    while (cap < n_entries) cap *= 2;
+    server_release(S, sl.d_jobs, sl.stream);
+    server_release(S, sl.d_start, sl.stream);
+    if (sl.h_jobs) (void)hipHostFree(sl.h_jobs);
+    if (sl.h_start) (void)hipHostFree(sl.h_start);
+    sl.d_jobs = nullptr; sl.d_start = nullptr; sl.h_jobs = nullptr; sl.h_start = nullptr; sl.cap = 0;
+    const size_t n_start = cap + uc_server::kMaxGroups;
+    if (server_malloc(S, (void**)&sl.d_jobs, sizeof(UcJob) * cap, sl.stream) != hipSuccess ||
+        server_malloc(S, (void**)&sl.d_start, sizeof(long long) * n_start, sl.stream) != hipSuccess ||
+        hipHostMalloc((void**)&sl.h_jobs, sizeof(UcJob) * cap, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&sl.h_start, sizeof(long long) * n_start, hipHostMallocDefault) != hipSuccess)
+        return false;
+    sl.cap = cap;
+    return true;
+}
+
+// Picks the cells of the next round (under S->mu): every job with cells left
+// gets a fair share of the slot's lanes first, then the oldest jobs fill the
+// rest; an optional traj-step cap bounds the round's work.
+std::vector<uc_server::Entry> server_pick(uc_server* S, long long lanes, int block) {
+    std::vector<uc_server::Entry> out;
+    std::vector<uc_server::Job*> open;
+    for (auto& jp : S->active)
+        if (jp->state == 1 && jp->next < jp->cells) open.push_back(jp.get());
+    if (open.empty()) return out;
+    const double cap_steps = S->cfg.round_traj_steps > 0 ? S->cfg.round_traj_steps : 0.0;
+    std::map<uc_server::Job*, long long> take;
+    std::map<std::pair<Module*, std::string>, int> groups;
+    long long cells = 0;
+    double steps = 0.0;
+    const long long share = std::max<long long>(block, (lanes / (long long)open.size()) / block * block);
+    for (int pass = 0; pass < 2 && cells < lanes; ++pass) {
+        for (uc_server::Job* J : open) {
+            if (cells >= lanes || (cap_steps > 0.0 && steps >= cap_steps)) break;
+            const auto g = std::make_pair(J->m, J->kernel);
+            if (!groups.count(g) && (int)groups.size() >= uc_server::kMaxGroups) continue;
+            const long long left = J->cells - J->next - take[J];
+            long long n = std::min(left, pass == 0 ? share : lanes - cells);
+            if (cap_steps > 0.0) n = std::min<long long>(n, std::max<long long>(1, (long long)((cap_steps - steps) / J->step_cost)));
+            if (n <= 0) continue;
+            take[J] += n;
+            cells += n;
+            steps += (double)n * J->step_cost;
+            groups[g] = 1;
+        }
+    }
+    for (auto& jp : S->active) {
+        auto it = take.find(jp.get());
+        if (it == take.end() || it->second <= 0) continue;
+        out.push_back({jp, jp->next, it->second});
+        jp->next += it->second;
+        ++jp->inflight;
+    }
+    return out;
+}
+
+// Uploads the round's job tables and launches it on the slot's stream.
+// Returns an error message, empty on success.
+std::string server_launch(uc_server* S, uc_server::Slot& sl, int block, int wps, int cus) {
+    // group entries by (module, kernel), longest cells first inside a group
+    std::map<std::pair<Module*, std::string>, std::vector<const uc_server::Entry*>> groups;
+    for (auto& en : sl.entries) groups[std::make_pair(en.job->m, en.job->kernel)].push_back(&en);
+    if (!slot_reserve(S, sl, sl.entries.size())) return "server: job-table buffers";
+    size_t jo = 0, so = 0;
+    struct G { Module* m; const std::string* k; size_t jofs, sofs; int nj; long long chunks; };
+    std::vector<G> gl;
+    for (auto& kv : groups) {
+        auto& v = kv.second;
+        std::stable_sort(v.begin(), v.end(), [](const uc_server::Entry* a, const uc_server::Entry* b) {
+            return a->job->step_cost > b->job->step_cost;
+        });
+        G g{kv.first.first, &kv.first.second, jo, so, (int)v.size(), 0};
+        sl.h_start[so++] = 0;
+        for (const uc_server::Entry* en : v) {
+            UcJob j = en->job->tmpl;
+            j.cell_offset = en->first;
+            j.n_cells = en->cells;
+            j.stride = en->job->cells;
+            j.flag = en->job->flags + en->first;
+            j.out0 = en->job->out0 + en->first;
+            j.out1 = en->job->out1 ? en->job->out1 + en->first : nullptr;
+            sl.h_jobs[jo++] = j;
+            g.chunks += (en->cells + block - 1) / block;
+            sl.h_start[so++] = g.chunks;
+        }
+        gl.push_back(g);
+    }
+    hipError_t he = hipMemcpyAsync(sl.d_jobs, sl.h_jobs, sizeof(UcJob) * jo, hipMemcpyHostToDevice, sl.stream);
+    if (he == hipSuccess)
+        he = hipMemcpyAsync(sl.d_start, sl.h_start, sizeof(long long) * so, hipMemcpyHostToDevice, sl.stream);
+    hipFunction_t f;
+    if (he == hipSuccess) he = hipModuleGetFunction(&f, gl[0].m->mod, "uc_batch_reset");
+    int ng = (int)gl.size();
+    void* rargs[] = {&sl.d_ctr, &ng};
+    if (he == hipSuccess) he = hipModuleLaunchKernel(f, 1, 1, 1, 64, 1, 1, 0, sl.stream, rargs, nullptr);
+    const long long resident = (long long)cus * 4 * wps / (block / 64);
+    for (size_t gi = 0; gi < gl.size() && he == hipSuccess; ++gi) {
+        G& g = gl[gi];
+        he = hipModuleGetFunction(&f, g.m->mod, g.k->c_str());
+        UcJob* dj = sl.d_jobs + g.jofs;
+        long long* ds = sl.d_start + g.sofs;
+        unsigned long long* ctr = sl.d_ctr + gi;
+        void* args[] = {&dj, &ds, &g.nj, &ctr};
+        const long long grid = std::min<long long>(resident, std::max<long long>(1, g.chunks));
+        if (he == hipSuccess)
+            he = hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, 0, sl.stream, args, nullptr);
+    }
+    if (he == hipSuccess) he = hipEventRecord(sl.done, sl.stream);
+    return he == hipSuccess ? std::string() : hip_msg(he, "server round");
+}
+
+// Books a finished (or failed) round (under S->mu).
+void server_retire(uc_server* S, uc_server::Slot& sl, const std::string& why) {
+    const auto now = std::chrono::steady_clock::now();
+    for (auto& en : sl.entries) {
+        auto& J = *en.job;
+        --J.inflight;
+        if (!why.empty()) {
+            if (J.state == 1) { J.state = 4; J.error = why; J.t_done = now; }
+            continue;
+        }
+        J.done_cells += en.cells;
+        S->traj_steps_done += (double)en.cells * J.step_cost;
+        if (J.state == 1 && J.done_cells >= J.cells) {
+            J.state = 2;
+            J.t_done = now;
+            S->latencies_ms.push_back(std::chrono::duration<double, std::milli>(J.t_done - J.t_submit).count());
+        }
+    }
+    sl.entries.clear();
+    sl.busy = false;
+    if (--S->n_busy == 0) S->busy_s += std::chrono::duration<double>(now - S->busy_since).count();
+    ++S->rounds;
+}
+
+// Drops finished jobs from the active list and frees cancelled / failed
+// ones once no round holds their cells (under S->mu).
+void server_sweep(uc_server* S) {
+    std::vector<std::shared_ptr<uc_server::Job>> still;
+    for (auto& jp : S->active) {
+        if (jp->state == 1 || jp->inflight > 0) { still.push_back(jp); continue; }
+        if (jp->state != 2) { S->used_bytes -= jp->bytes; server_free(S, *jp); }
+    }
+    S->active.swap(still);
+}
+
+void server_destroy_slots(uc_server* S) {
+    for (auto& sl : S->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        server_release(S, sl.d_jobs, sl.stream);
+        server_release(S, sl.d_start, sl.stream);
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.d_ctr) (void)hipFree(sl.d_ctr);
+        if (sl.h_jobs) (void)hipHostFree(sl.h_jobs);
+        if (sl.h_start) (void)hipHostFree(sl.h_start);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        sl = uc_server::Slot();
+    }
 }
 
 void server_loop(uc_server* S) {
     (void)hipSetDevice(S->e->device);
-    using clock = std::chrono::steady_clock;
-    std::vector<unsigned long long*> counters;
-    unsigned long long* d_counters = nullptr;
-    (void)hipMalloc((void**)&d_counters, sizeof(unsigned long long) * 64);
+    const int block = S->cfg.block > 0 ? S->cfg.block : 256;
+    const int wps = S->cfg.waves_per_simd > 0 ? S->cfg.waves_per_simd : 4;
+    const int cus = cu_count(S->e->device);
+    // lanes one slot's rounds hold: the GPU's resident lanes split over the slots
+    const long long lanes = std::max<long long>(block, (long long)cus * 4 * wps * 64 / uc_server::kSlots / block * block);
+    std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
-        std::vector<std::shared_ptr<uc_server::Job>> round_jobs;
-        {
-            std::unique_lock<std::mutex> lk(S->mu);
-            // admission in arrival order while the outputs fit
-            while (!S->pending.empty()) {
-                auto j = S->pending.front();
-                if (j->state == 3) { S->pending.pop_front(); continue; }
-                if (S->used_bytes + j->bytes > S->budget && !S->active.empty()) break;
-                S->pending.pop_front();
-                if (!server_admit(S, *j)) { j->state = 4; j->t_done = clock::now(); continue; }
-                S->used_bytes += j->bytes;
-                j->state = 1;
-                S->active.push_back(j);
-            }
-            if (S->active.empty()) {
-                if (S->stop) break;
-                S->cv.wait_for(lk, std::chrono::milliseconds(20));
-                continue;
-            }
-            round_jobs = S->active;
+        // retire finished rounds
+        for (auto& sl : S->slots) {
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipErrorNotReady) continue;
+            server_retire(S, sl, q == hipSuccess ? std::string() : hip_msg(q, "server round"));
         }
-        // ---- build the round: chunk allocation, oldest first ----
-        const int block = S->cfg.block > 0 ? S->cfg.block : 256;
-        const double budget_steps = S->cfg.round_traj_steps > 0 ? S->cfg.round_traj_steps : 2e9;
-        struct Entry { UcJob j; uc_server::Job* job; long long cells; };
-        std::vector<Entry> entries;
-        double spent = 0.0;
-        bool progress = true;
-        std::map<uc_server::Job*, long long> taken;
-        while (progress && spent < budget_steps) {
-            progress = false;
-            for (auto& jp : round_jobs) {
-                auto& J = *jp;
-                const long long pos = J.next + taken[&J];
-                if (J.state != 1 || pos >= J.cells) continue;
-                const long long chunk = std::max<long long>(block, (long long)(budget_steps / 8 / std::max(1.0, J.step_cost)) / block * block);
-                const long long n = std::min(chunk, J.cells - pos);
-                taken[&J] += n;
-                spent += (double)n * J.step_cost;
-                progress = true;
-                if (spent >= budget_steps) break;
-            }
-        }
-        for (auto& jp : round_jobs) {
-            auto& J = *jp;
-            const long long n = taken[&J];
-            if (n <= 0) continue;
-            UcJob e = J.tmpl;
-            e.cell_offset = J.next;
-            e.n_cells = n;
-            e.stride = J.cells;
-            e.flag = J.flags + J.next;
-            e.out0 = J.out0 + J.next;
-            e.out1 = J.out1 ? J.out1 + J.next : nullptr;
-            entries.push_back({e, &J, n});
-        }
-        // ---- group by kernel, upload tables, launch, wait ----
-        std::map<std::pair<Module*, std::string>, std::vector<Entry*>> groups;
-        for (auto& en : entries) groups[std::make_pair(en.job->m, en.job->kernel)].push_back(&en);
-        const auto t0 = clock::now();
-        std::vector<void*> temps;
-        int gidx = 0;
-        bool failed = false;
-        std::string why;
-        {
-            hipFunction_t freset;
-            int ng = (int)groups.size();
-            void* rargs[] = {&d_counters, &ng};
-            if (hipModuleGetFunction(&freset, entries[0].job->m->mod, "uc_batch_reset") != hipSuccess ||
-                hipModuleLaunchKernel(freset, 1, 1, 1, 64, 1, 1, 0, S->stream, rargs, nullptr) != hipSuccess) {
-                failed = true; why = "reset";
-            }
-        }
-        const int cus = cu_count(S->e->device);
-        const int wps = S->cfg.waves_per_simd > 0 ? S->cfg.waves_per_simd : 4;
-        for (auto& kv : groups) {
-            if (failed) break;
-            std::vector<UcJob> table;
-            std::vector<long long> start(1, 0);
-            for (Entry* en : kv.second) {
-                table.push_back(en->j);
-                start.push_back(start.back() + (en->cells + block - 1) / block);
-            }
-            UcJob* d_jobs = nullptr;
-            long long* d_start = nullptr;
-            hipError_t he = hipMalloc((void**)&d_jobs, sizeof(UcJob) * table.size());
-            if (he == hipSuccess) { temps.push_back(d_jobs); he = hipMalloc((void**)&d_start, sizeof(long long) * start.size()); }
-            if (he == hipSuccess) { temps.push_back(d_start); he = hipMemcpyAsync(d_jobs, table.data(), sizeof(UcJob) * table.size(), hipMemcpyHostToDevice, S->stream); }
-            if (he == hipSuccess) he = hipMemcpyAsync(d_start, start.data(), sizeof(long long) * start.size(), hipMemcpyHostToDevice, S->stream);
-            hipFunction_t f;
-            if (he == hipSuccess) he = hipModuleGetFunction(&f, kv.first.first->mod, kv.first.second.c_str());
-            int nj = (int)table.size();
-            unsigned long long* ctr = d_counters + gidx;
-            void* args[] = {&d_jobs, &d_start, &nj, &ctr};
-            const long long grid = std::min<long long>((long long)cus * 4 * wps / (block / 64), std::max<long long>(1, start.back()));
-            if (he == hipSuccess) he = hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, 0, S->stream, args, nullptr);
-            if (he != hipSuccess) { failed = true; why = hip_msg(he, "server round"); }
-            ++gidx;
-        }
-        hipError_t he = hipStreamSynchronize(S->stream);
-        if (he != hipSuccess) { failed = true; why = hip_msg(he, "server round sync"); }
-        for (void* p : temps) (void)hipFree(p);
-        const auto t1 = clock::now();
-        // ---- bookkeeping ----
-        std::lock_guard<std::mutex> lk(S->mu);
-        S->busy_s += std::chrono::duration<double>(t1 - t0).count();
-        ++S->rounds;
-        for (auto& en : entries) {
-            auto& J = *en.job;
-            if (failed) { J.state = 4; J.error = why; J.t_done = t1; continue; }
-            J.next += en.cells;
-            S->traj_steps_done += (double)en.cells * J.step_cost;
-            if (J.next >= J.cells && J.state == 1) {
-                J.state = 2;
-                J.t_done = t1;
-                S->latencies_ms.push_back(std::chrono::duration<double, std::milli>(J.t_done - J.t_submit).count());
-            }
-        }
-        std::vector<std::shared_ptr<uc_server::Job>> still;
-        for (auto& jp : S->active) {
-            if (jp->state == 1) still.push_back(jp);
-            else if (jp->state != 2) { S->used_bytes -= jp->bytes; server_free(*jp); }
-        }
-        S->active.swap(still);
+        if (S->stop)
+            for (auto& jp : S->active)
+                if (jp->state == 1) { jp->state = 3; jp->t_done = std::chrono::steady_clock::now(); }
+        server_sweep(S);
         S->cv.notify_all();
+        // admission in arrival order while the outputs fit
+        while (!S->stop && !S->pending.empty()) {
+            auto j = S->pending.front();
+            if (j->state == 3) { S->pending.pop_front(); continue; }
+            if (S->used_bytes + j->bytes > S->budget && !S->active.empty()) break;
+            S->pending.pop_front();
+            if (!server_admit(S, *j)) { j->state = 4; j->t_done = std::chrono::steady_clock::now(); continue; }
+            S->used_bytes += j->bytes;
+            j->state = 1;
+            S->active.push_back(j);
+        }
+        // launch on a free slot
+        uc_server::Slot* free_slot = nullptr;
+        for (auto& sl : S->slots)
+            if (!sl.busy && sl.stream) { free_slot = &sl; break; }
+        bool launched = false;
+        if (free_slot && !S->stop) {
+            auto entries = server_pick(S, lanes, block);
+            if (!entries.empty()) {
+                free_slot->entries = std::move(entries);
+                free_slot->busy = true;
+                if (S->n_busy++ == 0) S->busy_since = std::chrono::steady_clock::now();
+                const std::string why = server_launch(S, *free_slot, block, wps, cus);
+                if (!why.empty()) {
+                    (void)hipStreamSynchronize(free_slot->stream);
+                    server_retire(S, *free_slot, why);
+                }
+                launched = true;
+            }
+        }
+        if (launched) continue;
+        if (S->n_busy == 0) {
+            if (S->stop) break;
+            if (S->active.empty() || S->pending.empty())
+                S->cv.wait_for(lk, std::chrono::milliseconds(20));
+            continue;
+        }
+        // rounds in flight and nothing to launch: poll again shortly (a
+        // submission wakes the loop at once)
+        S->cv.wait_for(lk, std::chrono::microseconds(200));
     }
-    if (d_counters) (void)hipFree(d_counters);
 }
 
 int server_submit(uc_server* S, std::shared_ptr<uc_server::Job> j, long long* id) {
@@ -1343,8 +1474,21 @@ int uc_server_create(uc_engine* e, const uc_server_config* cfg, uc_server** out)
     (void)hipMemGetInfo(&free_b, &total_b);
     const double frac = S->cfg.hbm_fraction > 0.0 ? S->cfg.hbm_fraction : 0.5;
     S->budget = S->cfg.max_bytes > 0 ? (size_t)S->cfg.max_bytes : (size_t)(frac * (double)free_b);
+    int pools = 0;
+    S->async_alloc = hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, e->device) == hipSuccess &&
+                     pools != 0;
     hipError_t he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
-    if (he != hipSuccess) { delete S; return e->fail(UC_ERR_HIP, hip_msg(he, "server stream")); }
+    for (auto& sl : S->slots) {
+        if (he == hipSuccess) he = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
+        if (he == hipSuccess) he = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (he == hipSuccess) he = hipMalloc((void**)&sl.d_ctr, sizeof(unsigned long long) * uc_server::kMaxGroups);
+    }
+    if (he != hipSuccess) {
+        server_destroy_slots(S);
+        if (S->stream) (void)hipStreamDestroy(S->stream);
+        delete S;
+        return e->fail(UC_ERR_HIP, hip_msg(he, "server streams"));
+    }
     S->t_start = std::chrono::steady_clock::now();
     e->refs.fetch_add(1, std::memory_order_relaxed);
     S->worker = std::thread(server_loop, S);
@@ -1367,9 +1511,10 @@ void uc_server_destroy(uc_server* S) {
         (void)hipSetDevice(e->device);
         for (auto& kv : S->all) {
             if (kv.second->m) --kv.second->m->pins;
-            server_free(*kv.second);
+            server_free(S, *kv.second);
         }
-        if (S->stream) (void)hipStreamDestroy(S->stream);
+        server_destroy_slots(S);
+        if (S->stream) { (void)hipStreamSynchronize(S->stream); (void)hipStreamDestroy(S->stream); }
         delete S;
     }
     engine_release(e);
@@ -1482,13 +1627,16 @@ int uc_server_fetch(uc_server* S, long long id, int* flags, double* out0, double
     }
     (void)hipSetDevice(S->e->device);
     hipError_t he = hipSuccess;
-    if (flags) he = hipMemcpy(flags, j->flags, sizeof(int) * (size_t)j->cells, hipMemcpyDeviceToHost);
-    if (he == hipSuccess && out0) he = hipMemcpy(out0, j->out0, sizeof(double) * j->out0_n, hipMemcpyDeviceToHost);
-    if (he == hipSuccess && out1 && j->out1) he = hipMemcpy(out1, j->out1, sizeof(double) * j->out1_n, hipMemcpyDeviceToHost);
+    if (flags) he = hipMemcpyAsync(flags, j->flags, sizeof(int) * (size_t)j->cells, hipMemcpyDeviceToHost, S->stream);
+    if (he == hipSuccess && out0)
+        he = hipMemcpyAsync(out0, j->out0, sizeof(double) * j->out0_n, hipMemcpyDeviceToHost, S->stream);
+    if (he == hipSuccess && out1 && j->out1)
+        he = hipMemcpyAsync(out1, j->out1, sizeof(double) * j->out1_n, hipMemcpyDeviceToHost, S->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(S->stream);
     if (he != hipSuccess) return S->e->fail(UC_ERR_HIP, hip_msg(he, "fetch"));
     std::lock_guard<std::mutex> lk(S->mu);
     S->used_bytes -= j->bytes;
-    server_free(*j);
+    server_free(S, *j);
     if (j->m) { --j->m->pins; j->m = nullptr; }
     j->state = 5;  // fetched
     S->cv.notify_all();
@@ -1505,8 +1653,9 @@ int uc_server_stats(uc_server* S, uc_server_stats_t* st) {
     st->p50_ms = pct(0.50);
     st->p99_ms = pct(0.99);
     st->max_ms = l.empty() ? 0.0 : l.back();
-    st->busy_s = S->busy_s;
-    st->wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - S->t_start).count();
+    const auto now = std::chrono::steady_clock::now();
+    st->busy_s = S->busy_s + (S->n_busy > 0 ? std::chrono::duration<double>(now - S->busy_since).count() : 0.0);
+    st->wall_s = std::chrono::duration<double>(now - S->t_start).count();
     st->traj_steps = S->traj_steps_done;
     st->rounds = S->rounds;
     st->queued = (long long)S->pending.size();
