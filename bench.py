@@ -30,7 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "u-cuda_amd"))
 
 METRIC = "RK4 trajectory-steps/s, FP32+comp vs FP64, at LLE err≤1e-3; p99 job latency"
-PEAK_FP32_VALU = 157.3e12   # MI355X published FP32 vector peak (packed)
+PEAK_FP32_VALU = 157.3e12   # MI355X published FP32 vector peak
 PEAK_FP64_VALU = 78.6e12    # MI355X published FP64 vector peak
 
 
@@ -45,6 +45,18 @@ def steps_per_cell(w):
     blocks = int(w["t_max"] / w["NT"])
     nts = int(w["NT"] / w["h"])
     return skip + 1 + blocks * (nts + 1)
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the headline kernel as measured by the PMC
+    passes recorded in profiles/ (FETCH_SIZE and WRITE_SIZE, each scaled by
+    its reading for a plain copy of known size); None if absent."""
+    import re
+    path = os.path.join(ROOT, "profiles", "lle_map_rossler_512_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*(\d+)", open(path).read())
+    return int(m.group(1)) if m else None
 
 
 def run_gpu(sysm, w, precision, steps, warmup, strict=False):
@@ -422,9 +434,8 @@ def main():
         "fp64": {"value": traj_steps / (f64_ms / 1e3), "ms_per_step": f64_ms},
         "f32c_over_f64": f64_ms / ms,
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": PEAK_FP32_VALU / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU,
-                     "frac_unpacked_fp32": achieved / PEAK_FP64_VALU, "flops_per_traj_step": flops_per_step,
-                     "kernel": "uc_lle_f32c", "traffic": None},
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_VALU, "flops_per_traj_step": flops_per_step,
+                     "kernel": "uc_lle_f32c", "traffic": profiled_traffic()},
         "wall_s_per_step": f32["wall"] / args.steps,
     }
     if not args.no_extra:

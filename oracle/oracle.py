@@ -67,7 +67,7 @@ def lib():
         L.oc_rk4_tan.argtypes = [_P, _P, _D, _D, C.c_int, _D, C.c_double, C.c_int]
         L.oc_bif_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _I, _D, _D, _D, C.c_int, C.c_int]
         L.oc_bif_cells_noisy.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _I, _D, _D, _D, C.c_int, C.c_int,
-                                         C.c_double, C.c_ulonglong]
+                                         C.c_double, C.c_ulonglong, C.c_int]
         L.oc_lle_fd_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
         L.oc_ls_fd_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
         L.oc_lyap_var_cells.argtypes = [_P, _P, C.c_int, C.POINTER(OcJob), _LL, C.c_longlong, _D, _I, C.c_int]
@@ -235,9 +235,10 @@ def _cells(cells):
     return cells, cells.ctypes.data_as(_LL)
 
 
-def bif_cells(step, job, cells, traj_len=0, threads=1, noise=0.0, noise_seed=0):
+def bif_cells(step, job, cells, traj_len=0, threads=1, noise=0.0, noise_seed=0, noise_every=100):
     """Bifurcation cells; noise > 0 adds a relative state kick of that size
-    after every 100th step (rounding model, see lyap_rounding_spread)."""
+    after every noise_every-th step (rounding model, see
+    lyap_rounding_spread)."""
     cells, cp = _cells(cells)
     n = len(cells)
     mp = max(1, job.max_peaks)
@@ -247,7 +248,7 @@ def bif_cells(step, job, cells, traj_len=0, threads=1, noise=0.0, noise_seed=0):
     traj = np.full((traj_len, n), np.nan) if traj_len else None
     lib().oc_bif_cells_noisy(step.ptr, C.byref(job), cp, n, flags.ctypes.data_as(_I), peaks.ctypes.data_as(_D),
                              intervals.ctypes.data_as(_D), traj.ctypes.data_as(_D) if traj is not None else None,
-                             traj_len, threads, C.c_double(noise), C.c_ulonglong(noise_seed))
+                             traj_len, threads, C.c_double(noise), C.c_ulonglong(noise_seed), int(noise_every))
     return dict(flags=flags, peaks=peaks, intervals=intervals, traj=traj)
 
 

@@ -311,25 +311,25 @@ static void probe_basis(u64 seed, long long cell, int K, int n, double* z) {
  * trailing steps (the CPU twin passes pre_scaller = 1 there, :472-475).
  * Recorded run only for transient flag 1 / -1. Under an h axis the counts
  * are recomputed per cell with ceil (cudaLibrary.cu:1028-1030). */
-/* KRS step with a relative state kick after every 100th step (noise > 0);
- * see noise_kick below. */
+/* KRS step with a relative state kick after every `every`-th step
+ * (noise > 0); see noise_kick below. */
 typedef struct noisy_step {
     oc_step_fn step;
     double noise;
     unsigned long long rng;
-    long long count;
+    long long count, every;
     int n;
 } noisy_step;
 static void noise_kick(double* x, int n, double noise, unsigned long long* s);
 static void adv_krs_noisy(void* ctx, double* x, const double* a, double h) {
     noisy_step* c = (noisy_step*)ctx;
     c->step(x, a, h);
-    if (++c->count % 100 == 0) noise_kick(x, c->n, c->noise, &c->rng);
+    if (++c->count % c->every == 0) noise_kick(x, c->n, c->noise, &c->rng);
 }
 
 static void bif_cell(oc_step_fn step, const oc_job* j, long long cell, long long iters_max, double* data,
                      double* pk, double* tm, int* flag_out, double* peaks, double* intervals, long long stride,
-                     double* traj, int traj_len, double noise, unsigned long long noise_seed) {
+                     double* traj, int traj_len, double noise, unsigned long long noise_seed, int every) {
     double a[OC_MAXV], x[OC_MAXN], h;
     if (!oc_cell_setup(j, cell, a, x, &h)) { *flag_out = -1; return; }
     long long skip = j->h > 0.0 ? (long long)(j->transient_time / j->h) : 0, iters = iters_max;
@@ -338,7 +338,8 @@ static void bif_cell(oc_step_fn step, const oc_job* j, long long cell, long long
         long long it = (long long)ceil(j->t_max / h / (double)j->pre_scaller);
         if (it < iters) iters = it;
     }
-    noisy_step ns = {step, noise, noise_seed ^ ((unsigned long long)cell * 0xD1B54A32D192ED03ull), 0, j->n_vars};
+    noisy_step ns = {step, noise, noise_seed ^ ((unsigned long long)cell * 0xD1B54A32D192ED03ull), 0,
+                     every > 0 ? every : 100, j->n_vars};
     const oc_adv_fn adv = noise > 0.0 ? adv_krs_noisy : adv_krs;
     void* ctx = noise > 0.0 ? (void*)&ns : (void*)step;
     int flag = loop_generic(adv, ctx, x, a, h, skip, j->n_vars, j->pre_scaller, 0, j->max_value, NULL);
@@ -359,7 +360,7 @@ static void bif_cell(oc_step_fn step, const oc_job* j, long long cell, long long
 
 void oc_bif_cells_noisy(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int* flags,
                         double* peaks, double* intervals, double* traj, int traj_len, int threads, double noise,
-                        unsigned long long noise_seed) {
+                        unsigned long long noise_seed, int noise_every) {
     const long long iters_max = (long long)ceil(j->t_max / worst_step(j) / j->pre_scaller);
 #pragma omp parallel num_threads(threads > 1 ? threads : 1)
     {
@@ -369,7 +370,7 @@ void oc_bif_cells_noisy(oc_step_fn step, const oc_job* j, const long long* cells
 #pragma omp for schedule(dynamic, 1)
         for (long long s = 0; s < n_sel; ++s)
             bif_cell(step, j, cells[s], iters_max, data, pk, tm, &flags[s], peaks + s, intervals + s, n_sel,
-                     traj ? traj + s : NULL, traj ? traj_len : 0, noise, noise_seed);
+                     traj ? traj + s : NULL, traj ? traj_len : 0, noise, noise_seed, noise_every);
         free(data);
         free(pk);
         free(tm);
@@ -378,7 +379,7 @@ void oc_bif_cells_noisy(oc_step_fn step, const oc_job* j, const long long* cells
 
 void oc_bif_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int* flags,
                   double* peaks, double* intervals, double* traj, int traj_len, int threads) {
-    oc_bif_cells_noisy(step, j, cells, n_sel, flags, peaks, intervals, traj, traj_len, threads, 0.0, 0);
+    oc_bif_cells_noisy(step, j, cells, n_sel, flags, peaks, intervals, traj, traj_len, threads, 0.0, 0, 100);
 }
 
 static void lyap_counts(const oc_job* j, double h, long long* skip, long long* nts) {

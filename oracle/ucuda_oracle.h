@@ -71,10 +71,11 @@ void oc_rk4_tan(oc_rhs_fn rhs, oc_tan_fn tan, double* x, double* V, int K, const
 /* per-cell drivers over a selection of global cell indices (threads <= 1: serial) */
 void oc_bif_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int* flags,
                   double* peaks, double* intervals, double* traj, int traj_len, int threads);
-/* same with a relative state kick of size `noise` after every 100th step */
+/* same with a relative state kick of size `noise` after every
+ * `noise_every`-th step */
 void oc_bif_cells_noisy(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int* flags,
                         double* peaks, double* intervals, double* traj, int traj_len, int threads, double noise,
-                        unsigned long long noise_seed);
+                        unsigned long long noise_seed, int noise_every);
 void oc_lle_fd_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, double* out,
                      int* flags, int threads);
 void oc_ls_fd_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, double* out,

@@ -23,7 +23,9 @@ Tolerances:
     parameter perturbations of the mode's rounding size (oracle.ROUNDING):
     FP64 1e-6 relative; FP32-compensated 1e-6 relative (values) and 1e-5
     (intervals) on the cells that hold 1e-6 under float-sized noise.
-  * FP32-compensated: short-horizon trajectories 1e-5 relative.
+  * FP32-compensated short-horizon trajectories: within 4x the deviation
+    that float-sized per-step noise produces in the FP64 oracle (the float
+    stages are amplified by the flow; ~3e-4 relative at t = 1 for rho = 250).
 """
 import numpy as np
 import pytest
@@ -104,11 +106,27 @@ def test_bif_lorenz_f32c(gpu_engine):
     _check_periodic_peaks(gpu, ref, stable, 1e-6, rel_intervals=1e-5)
     flags = gpu["flags"].cpu().numpy()
     np.testing.assert_array_equal(flags <= 0, ref["flags"] <= 0)
+    # short horizon: float stages inject ~6e-8 relative per step and at rho
+    # ~ 250 the origin's unstable eigenvalue (~45) amplifies that to ~3e-4
+    # relative by t = 1. Bound: the running maximum of |gpu - oracle| stays
+    # within 4x the running maximum deviation that 6e-8 per-step state noise
+    # plus float-sized parameter perturbations produce in the FP64 oracle,
+    # plus 1e-6 of the trajectory's scale (a CPU float emulation of the step
+    # reaches 0.74x of that bound).
     s2 = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 0.0, 1.0, max_peaks=4, traj_len=100,
                        precision=ucuda.PREC_F32C)
     job2 = oracle.make_job(3, axes, [0.1, 0.1, 0.1], vals, 0.01, 0.0, 1.0, max_peaks=4)
-    r2 = oracle.bif_cells(step, job2, range(64), traj_len=100)
-    np.testing.assert_allclose(s2["traj"].cpu().numpy(), r2["traj"], rtol=1e-5, atol=1e-6)
+    r2 = oracle.bif_cells(step, job2, range(64), traj_len=100)["traj"]
+    spread = np.zeros_like(r2)
+    for k, signs in enumerate(oracle._PERTURB_SIGNS):
+        rn = oracle.bif_cells(step, oracle.perturbed_job(job2, 6e-8, signs), range(64), traj_len=100, noise=6e-8,
+                              noise_seed=3000 + k, noise_every=1)["traj"]
+        spread = np.maximum(spread, np.abs(rn - r2))
+    err = np.maximum.accumulate(np.abs(s2["traj"].cpu().numpy() - r2), axis=0)
+    bound = 4.0 * (np.maximum.accumulate(spread, axis=0) + 1e-6 * np.abs(r2).max(axis=0))
+    print("FP32C short horizon: max relative error %.1e; max err / bound %.2f"
+          % ((err / np.abs(r2).max(axis=0)).max(), (err / bound).max()))
+    assert (err <= bound).all()
 
 
 def _rossler_lle_case(eng, n=16):

@@ -112,6 +112,10 @@ struct StateF32C {
     __device__ __forceinline__ double get(int i) const { return (double)hi[i] - (double)c[i]; }
 };
 
+#define UC_LN2 0.6931471805599453
+__device__ __forceinline__ float uc_ldexp(float x, int e) { return __builtin_ldexpf(x, e); }
+__device__ __forceinline__ double uc_ldexp(double x, int e) { return __builtin_ldexp(x, e); }
+
 __device__ __forceinline__ void uc_kahan_add(float& hi, float& c, float incr) {
     const float y = incr - c;
     const float t = hi + y;
@@ -374,6 +378,96 @@ __device__ int uc_loop_tan(S& s, typename Prec<S>::T* V, const typename Prec<S>:
     return uc_fixed_point_flag(s, prev, hd, 1);
 }
 
+// Wave-uniform variants of the two loops above, used when the step counts
+// come from the job descriptor (every lane of the wave runs the same count;
+// only sweeps over h make them per cell). The counters live in scalar
+// registers and there is no per-step branch: iteration i with i % 100 == 0
+// is followed by the magnitude check, then 99 unconditioned iterations run.
+// A lane whose check fails is dead from that check on: its state and time at
+// the check are kept, it records nothing more, and it keeps stepping
+// (harmlessly) until the loop ends, which it does as soon as every lane of
+// the wave is dead. Results equal those of the per-lane loops.
+template <class S>
+__device__ __forceinline__ double uc_sample_value(const S& s, int wvar) {
+    double v;
+    if (wvar < 0) {
+        if (UC_N >= 3) v = s.get(0) + UC_PI * s.get(UC_N >= 2 ? 1 : 0) + UC_E * s.get(UC_N >= 3 ? 2 : 0);
+        else if (UC_N == 2) v = s.get(0) + UC_PI * s.get(UC_N >= 2 ? 1 : 0);
+        else v = s.get(0);
+    } else {
+        v = s.get(0);
+#pragma unroll
+        for (int q = 1; q < UC_N; ++q) v = (q == wvar) ? s.get(q) : v;
+    }
+    return v;
+}
+
+template <class S, class Obs>
+__device__ __forceinline__ void uc_iter(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t,
+                                        long long i, int pre, int wvar, bool dead, Obs& obs) {
+    typedef typename Prec<S>::T T;
+    if (Obs::records) {
+        if (!dead) obs.sample(i, uc_sample_value(s, wvar));
+        for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += (double)h; }
+    } else {
+        uc_rk4(s, a, h, (T)t);
+        if (UC_USES_TIME) t += (double)h;
+    }
+}
+
+__device__ __forceinline__ bool uc_wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0ULL; }
+
+template <class S, class Obs>
+__device__ int uc_loop_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t, long long iters,
+                         int pre, int wvar, double max_value, Obs& obs) {
+    typedef typename Prec<S>::T T;
+    const double hd = (double)h;
+    bool dead = false;
+    S snap = s;
+    double tsnap = t;
+    for (long long i = 0; i < iters;) {
+        uc_iter(s, a, h, t, i, pre, wvar, dead, obs);
+        if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
+        if (uc_wave_all(dead)) break;
+        const long long stop = (iters - i > UC_CHECK_INTERVAL) ? i + UC_CHECK_INTERVAL : iters;
+        for (++i; i < stop; ++i) uc_iter(s, a, h, t, i, pre, wvar, dead, obs);
+    }
+    if (dead) { s = snap; t = tsnap; return 0; }
+    double prev[UC_N];
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
+    for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += hd; }
+    return uc_fixed_point_flag(s, prev, hd, pre);
+}
+
+template <int K, class S>
+__device__ int uc_loop_tan_u(S& s, typename Prec<S>::T* V, const typename Prec<S>::T* a, typename Prec<S>::T h,
+                             double& t, long long iters, double max_value) {
+    typedef typename Prec<S>::T T;
+    const double hd = (double)h;
+    bool dead = false;
+    S snap = s;
+    double tsnap = t;
+    for (long long i = 0; i < iters;) {
+        uc_rk4_tan<K>(s, V, a, h, (T)t);
+        if (UC_USES_TIME) t += hd;
+        if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
+        if (uc_wave_all(dead)) break;
+        const long long stop = (iters - i > UC_CHECK_INTERVAL) ? i + UC_CHECK_INTERVAL : iters;
+        for (++i; i < stop; ++i) {
+            uc_rk4_tan<K>(s, V, a, h, (T)t);
+            if (UC_USES_TIME) t += hd;
+        }
+    }
+    if (dead) { s = snap; t = tsnap; return 0; }
+    double prev[UC_N];
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
+    uc_rk4_tan<K>(s, V, a, h, (T)t);
+    if (UC_USES_TIME) t += hd;
+    return uc_fixed_point_flag(s, prev, hd, 1);
+}
+
 // ------------------------------------------------------ streaming peaks ---
 // The reference peak finder (U-CUDA/cudaLibrary.cu:1503-1633) run as samples
 // arrive instead of over a stored trajectory: candidate c needs d[c-1], d[c],
@@ -463,14 +557,17 @@ __device__ void uc_bif_cell(const UcJob& j, long long lane) {
 
     NoObserver none;
     double t = 0.0;
-    int flag = uc_loop(s, a, h, t, skip, j.pre_scaller, 0, j.max_value, none);
+    const bool uni = !uc_sweeps_step(j);  // step counts are wave-uniform unless the sweep is over h
+    int flag = uni ? uc_loop_u(s, a, h, t, j.skip_steps, j.pre_scaller, 0, j.max_value, none)
+                   : uc_loop(s, a, h, t, skip, j.pre_scaller, 0, j.max_value, none);
     PeakObserver obs;
     obs.w0 = 0.0; obs.w1 = 0.0; obs.n = iters; obs.c_next = 2; obs.mode = 0; obs.raw = 0;
     obs.last_t = 0.0; obs.time_step = hd * (double)j.pre_scaller; obs.max_peaks = j.max_peaks;
     obs.stride = j.stride; obs.peaks = j.out0 + lane; obs.intervals = j.out1 + lane;
     obs.traj = j.traj ? j.traj + lane : nullptr; obs.traj_len = j.traj ? j.traj_len : 0;
     if (flag == 1 || flag == -1)
-        flag = uc_loop(s, a, h, t, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
+        flag = uni ? uc_loop_u(s, a, h, t, j.n_iters, j.pre_scaller, j.writable_var, j.max_value, obs)
+                   : uc_loop(s, a, h, t, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
     j.flag[lane] = (flag == 1) ? obs.count() : flag;
     if (j.final_state) {
 #pragma unroll
@@ -589,7 +686,13 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
 
     NoObserver none;
     double t = 0.0;
-    if (uc_loop(s, a, h, t, skip, 1, 0, j.max_value, none) == 0) alive = false;
+    // step counts are wave-uniform unless the sweep is over h
+    const bool uni = !uc_sweeps_step(j);
+    auto run = [&](S& st, double& tt, long long n_job, long long n_cell) -> int {
+        return uni ? uc_loop_u(st, a, h, tt, n_job, 1, 0, j.max_value, none)
+                   : uc_loop(st, a, h, tt, n_cell, 1, 0, j.max_value, none);
+    };
+    if (run(s, t, j.skip_steps, skip) == 0) alive = false;
 
     if (alive) {
         double z[K * UC_N];
@@ -600,29 +703,93 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
             T V[K * UC_N];
 #pragma unroll
             for (int i = 0; i < K * UC_N; ++i) V[i] = Prec<S>::to_t(z[i]);
+#if !UC_STRICT
+            // K = 1: the tangent equation is linear, so rescaling V between
+            // blocks only telescopes the per-block log norms. V is rescaled
+            // by a power of two (exact) to |V|^2 in [0.5, 2), the exponents
+            // are summed as an integer and one log of the final |V| closes
+            // the sum: sum_b log|v_b| = ksum * ln 2 + log|V_end|. A block
+            // whose growth |V_end| / |V_start| would reach the reference's
+            // 1e-14 floor switches the lane to the reference's unit
+            // renormalisation (with the sum so far) for the rest of the run.
+            // K > 1: Gram-Schmidt every block as the reference; the norms
+            // are multiplied into a mantissa / exponent pair instead of one
+            // log per norm and block.
+            int ksum = 0;
+            bool ref_mode = false;
+            double start2 = 1.0, lsum = 0.0;
+            double prod[K];
+            int esum[K];
+#pragma unroll
+            for (int k = 0; k < K; ++k) { prod[k] = 1.0; esum[k] = 0; }
+#endif
             for (long long b = 0; b < j.n_iters; ++b) {
-                if (uc_loop_tan<K>(s, V, a, h, t, nts, j.max_value) == 0) { alive = false; break; }
+                const int r = uni ? uc_loop_tan_u<K>(s, V, a, h, t, j.nt_steps, j.max_value)
+                                  : uc_loop_tan<K>(s, V, a, h, t, nts, j.max_value);
+                if (r == 0) { alive = false; break; }
                 double v[K * UC_N];
 #pragma unroll
                 for (int i = 0; i < K * UC_N; ++i) v[i] = (double)V[i];
                 if (K == 1) {
+#if !UC_STRICT
+                    double n2 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) n2 = n2 + v[i] * v[i];
+                    if (!ref_mode && n2 > 1e-28 * start2 && n2 <= 1e300) {
+                        int e;
+                        (void)__builtin_frexp(n2, &e);
+                        const int k = e >> 1;  // floor(e / 2)
+#pragma unroll
+                        for (int i = 0; i < UC_N; ++i) V[i] = uc_ldexp(V[i], -k);
+                        ksum += k;
+                        start2 = __builtin_ldexp(n2, -2 * k);
+                        continue;
+                    }
+                    if (!ref_mode) {
+                        // the reference's sum up to this block and its block-start unit vector
+                        lsum = (double)ksum * UC_LN2 + 0.5 * log(start2);
+                        const double inv0 = 1.0 / sqrt(start2);
+#pragma unroll
+                        for (int i = 0; i < UC_N; ++i) v[i] = v[i] * inv0;
+                        ref_mode = true;
+                    }
+#endif
                     double nrm = 0.0;
 #pragma unroll
                     for (int i = 0; i < UC_N; ++i) nrm = nrm + v[i] * v[i];
                     nrm = sqrt(nrm);
                     if (nrm <= 1e-14) nrm = 1e-14;
+#if UC_STRICT
                     sum[0] = sum[0] + log(nrm);
+#else
+                    lsum = lsum + log(nrm);
+#endif
                     const double inv = 1.0 / nrm;
 #pragma unroll
                     for (int i = 0; i < UC_N; ++i) v[i] = v[i] * inv;
                 } else {
                     uc_mgs<K>(v, d);
 #pragma unroll
-                    for (int k = 0; k < K; ++k) sum[k] = sum[k] + log(d[k]);
+                    for (int k = 0; k < K; ++k) {
+#if UC_STRICT
+                        sum[k] = sum[k] + log(d[k]);
+#else
+                        int e;
+                        prod[k] = __builtin_frexp(prod[k] * d[k], &e);
+                        esum[k] += e;
+#endif
+                    }
                 }
 #pragma unroll
                 for (int i = 0; i < K * UC_N; ++i) V[i] = Prec<S>::to_t(v[i]);
             }
+#if !UC_STRICT
+            if (K == 1) sum[0] = ref_mode ? lsum : (double)ksum * UC_LN2 + 0.5 * log(start2);
+            else {
+#pragma unroll
+                for (int k = 0; k < K; ++k) sum[k] = log(prod[k]) + (double)esum[k] * UC_LN2;
+            }
+#endif
         } else {
             // Reference finite-difference Benettin (LLEKernelCUDA / LSKernelCUDA).
             const double eps = j.eps;
@@ -642,11 +809,11 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
             }
             for (long long b = 0; b < j.n_iters && alive; ++b) {
                 const double t0 = t;
-                if (uc_loop(s, a, h, t, nts, 1, 0, j.max_value, none) == 0) { alive = false; break; }
+                if (run(s, t, j.nt_steps, nts) == 0) { alive = false; break; }
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     double tk = t0;
-                    if (alive && uc_loop(y[k], a, h, tk, nts, 1, 0, j.max_value, none) == 0) alive = false;
+                    if (alive && run(y[k], tk, j.nt_steps, nts) == 0) alive = false;
                 }
                 if (!alive) break;
                 if (K == 1) {
