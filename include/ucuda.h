@@ -231,9 +231,9 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
  * A scheduler thread admits jobs in arrival order while their outputs fit
  * the HBM budget and keeps up to four rounds in flight, one per stream:
  * each round takes cells from the admitted jobs (a fair share per job, then
- * oldest first) up to a quarter of the GPU's resident lanes and launches the
- * persistent kernels once per (system, analysis) group; a new round starts
- * whenever one finishes, and jobs submitted meanwhile join it. Results are
+ * oldest first) up to the GPU's resident lanes and launches the persistent
+ * kernels once per (system, analysis) group; a new round is queued whenever
+ * a slot frees, and jobs submitted meanwhile join it. Results are
  * bit-identical to a direct uc_run_* of the same request. Outputs stay on the
  * device until uc_server_fetch copies them out. */
 typedef struct uc_server uc_server;
@@ -242,7 +242,7 @@ typedef struct uc_server_config {
     long long max_bytes;       /* explicit budget in bytes (overrides hbm_fraction if > 0) */
     double round_traj_steps;   /* optional cap on a round's trajectory-steps (0 -> none) */
     int block;                 /* lanes per workgroup (0 -> 256)                          */
-    int waves_per_simd;        /* resident waves per SIMD (0 -> 4)                        */
+    int waves_per_simd;        /* resident waves per SIMD (0 -> 8)                        */
 } uc_server_config;
 typedef struct uc_server_stats_t {
     long long completed, rounds, queued, active;

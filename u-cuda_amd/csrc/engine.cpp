@@ -1080,11 +1080,10 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
 // in arrival order while their output buffers fit in the HBM budget and
 // keeps up to kSlots rounds in flight, one per HIP stream. A round takes
 // cells from the admitted jobs (a fair share for every job first, then
-// oldest first) up to the lanes one slot's share of the GPU holds, and
-// launches one persistent kernel per (module, kernel) group over the
-// round's job table; the next round is built and launched as soon as a
-// slot is free, so a round of long cells does not hold back short jobs and
-// the tails of rounds overlap. Jobs that arrive meanwhile join the next
+// oldest first) up to the GPU's resident lanes, and launches one persistent
+// kernel per (module, kernel) group over the round's job table; the next
+// round is built and launched as soon as a slot is free, so rounds queue
+// back to back on the device and the tail of one overlaps the next. Jobs that arrive meanwhile join the next
 // round. A job's cells compute exactly as in a direct launch (same kernel
 // code per cell; the output plane stride stays the job's own), so results
 // do not depend on what else shares a round. Outputs stay in device memory
@@ -1371,10 +1370,12 @@ void server_destroy_slots(uc_server* S) {
 void server_loop(uc_server* S) {
     (void)hipSetDevice(S->e->device);
     const int block = S->cfg.block > 0 ? S->cfg.block : 256;
-    const int wps = S->cfg.waves_per_simd > 0 ? S->cfg.waves_per_simd : 4;
+    const int wps = S->cfg.waves_per_simd > 0 ? S->cfg.waves_per_simd : 8;
     const int cus = cu_count(S->e->device);
-    // lanes one slot's rounds hold: the GPU's resident lanes split over the slots
-    const long long lanes = std::max<long long>(block, (long long)cus * 4 * wps * 64 / uc_server::kSlots / block * block);
+    // a round holds the GPU's resident lanes: the slots' streams share fewer
+    // hardware queues than there are slots, so each round must fill the GPU
+    // on its own; the rounds queued behind it start as soon as it drains
+    const long long lanes = std::max<long long>(block, (long long)cus * 4 * wps * 64 / block * block);
     std::unique_lock<std::mutex> lk(S->mu);
     for (;;) {
         // retire finished rounds
