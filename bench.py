@@ -267,33 +267,54 @@ def extra_batch(eng, n_jobs=64, seed=0):
 
 
 def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
-    """Poisson arrivals of mixed bifurcation / LLE-map requests (systems above,
-    side log-uniform in [32, 2048], steps log-uniform in [1e3, 1e5]); the
-    arrival rate is set for ~70% of the measured capacity.
-    Reports throughput and p50 / p99 job latency, and checks 1% of jobs (at
-    least one) against the FP64 path."""
+    """Poisson arrivals of mixed requests, a third each: Lorenz bifurcation
+    sweeps (side^2 rho values), Rossler LLE maps (side x side) and forced
+    Duffing basin maps (side x side initial conditions, rtol = atol = 1e-9),
+    side log-uniform in [32, 2048], step count log-uniform in [1e3, 1e5]
+    (basins: steps / 500 forcing periods, at least 16), FP32-compensated.
+    The arrival rate is set for ~70% of `capacity` (LLE traj-steps/s),
+    weighting a bifurcation step 0.5 and a basin section 410 LLE steps (the
+    measured FP32C rates of the three kernels).
+    Reports throughput and p50 / p99 job latency; every 100th job is run
+    again alone in FP64 and compared."""
     import numpy as np
     import ucuda
     rng = np.random.default_rng(seed)
-    ros, lor = ucuda.preset("rossler", eng), ucuda.preset("lorenz", eng)
-    jobs = []
-    t = 0.0
-    mean_steps = None
+    ros, lor, duf = ucuda.preset("rossler", eng), ucuda.preset("lorenz", eng), ucuda.preset("duffing", eng)
     specs = []
-    for _ in range(100000):
+    for _ in range(4000):
         side = int(round(32 * 64 ** rng.uniform()))
         steps = int(round(10 ** (3 + 2 * rng.uniform())))
-        kind = "lle" if rng.uniform() < 0.5 else "bif"
+        kind = ("lle", "bif", "basins")[int(rng.integers(0, 3))]
         specs.append((kind, side, steps))
-        if len(specs) >= 4000:
-            break
-    est = np.mean([side * side * steps for _, side, steps in specs])
+    weight = {"lle": 1.0, "bif": 0.5}
+
+    def cost(kind, side, steps):
+        if kind == "basins":
+            return side * side * max(16, steps // 500) * 410.0
+        return side * side * steps * weight[kind]
+    est = float(np.mean([cost(*sp) for sp in specs]))
     if rate is None:
         rate = 0.7 * capacity / est if capacity else 20.0
     srv = ucuda.Server(eng)
     ids = []
+    t = 0.0
     t0 = time.perf_counter()
     i = 0
+
+    def request(kind, side, steps):
+        if kind == "lle":
+            axes = [ucuda.axis(1, 2, side, 0.05, 2.0), ucuda.axis(1, 3, side, 1.0, 20.0)]
+            blocks = max(1, steps // 11)
+            return axes, dict(fn="lyap", args=(ros, axes, [1, 1, 1], ros.values(), 0.01, 0.0, blocks * 0.1, 0.1))
+        if kind == "bif":
+            axes = [ucuda.axis(1, 2, side * side, 0.0, 250.0)]
+            return axes, dict(fn="bif", args=(lor, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, steps * 0.005,
+                                              steps * 0.005), kw=dict(max_peaks=8))
+        axes = [ucuda.axis(2, 0, side, -2.5, 2.5), ucuda.axis(2, 1, side, -2.5, 2.5)]
+        return axes, dict(fn="basins", args=(duf, axes, [0.0, 0.0], duf.values()),
+                          kw=dict(n_sections=max(16, steps // 500)))
+
     while True:
         t += rng.exponential(1.0 / rate)
         if t > duration_s:
@@ -303,16 +324,15 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
         now = time.perf_counter() - t0
         if t > now:
             time.sleep(t - now)
-        if kind == "lle":
-            axes = [ucuda.axis(1, 2, side, 0.05, 2.0), ucuda.axis(1, 3, side, 1.0, 20.0)]
-            blocks = max(1, steps // 11)
-            jid = srv.submit_lyap(ros, axes, [1, 1, 1], ros.values(), 0.01, 0.0, blocks * 0.1, 0.1,
-                                  precision=ucuda.PREC_F32C)
+        axes, rq = request(kind, side, steps)
+        kw = dict(rq.get("kw", {}), precision=ucuda.PREC_F32C)
+        if rq["fn"] == "lyap":
+            jid = srv.submit_lyap(*rq["args"], **kw)
+        elif rq["fn"] == "bif":
+            jid = srv.submit_bif(*rq["args"], **kw)
         else:
-            axes = [ucuda.axis(1, 2, side * side, 0.0, 250.0)]
-            jid = srv.submit_bif(lor, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, steps * 0.005, steps * 0.005,
-                                 max_peaks=8, precision=ucuda.PREC_F32C)
-        ids.append((jid, kind, side, steps, axes))
+            jid = srv.submit_basins(*rq["args"], **kw)
+        ids.append((jid, kind, side, steps))
     progress("serving: %d jobs submitted, draining" % len(ids))
     pending, last = [jid for jid, *_ in ids], time.perf_counter()
     deadline = last + 300.0
@@ -327,23 +347,41 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
         raise RuntimeError("serving: %d jobs unfinished after 300 s" % len(pending))
     wall = time.perf_counter() - t0
     st = srv.stats()
-    # audit: ~1% of jobs recomputed alone in FP64 (reported max |diff| for LLE jobs)
+    failed = [jid for jid, *_ in ids if srv.poll(jid)[0] != 2]
+    # audit: every 100th job recomputed alone in FP64
     audit = []
-    for jid, kind, side, steps, axes in ids[::100][:5]:
+    for jid, kind, side, steps in ids[::100]:
         got = srv.fetch(jid)
+        axes, rq = request(kind, side, steps)
         if kind == "lle":
-            blocks = max(1, steps // 11)
-            ref = ucuda.run_lyap(ros, axes, [1, 1, 1], ros.values(), 0.01, 0.0, blocks * 0.1, 0.1)
+            ref = ucuda.run_lyap(*rq["args"])
             rv = ref["values"].cpu().numpy()[0]
             ok = (got["flags"] == 1) & (ref["flags"].cpu().numpy() == 1)
-            audit.append(dict(kind=kind, cells=side * side, median_abs_diff=float(np.median(np.abs(got["values"][0][ok] - rv[ok])))))
+            e = np.abs(got["values"][0][ok] - rv[ok])
+            audit.append(dict(kind=kind, cells=side * side, median_abs_diff=float(np.median(e)) if e.size else None,
+                              frac_le_1e3=float((e <= 1e-3).mean()) if e.size else None))
+        elif kind == "bif":
+            ref = ucuda.run_bif(*rq["args"], **rq["kw"])
+            rf = ref["flags"].cpu().numpy()
+            audit.append(dict(kind=kind, cells=side * side, frac_same_peak_count=float((got["flags"] == rf).mean())))
         else:
-            audit.append(dict(kind=kind, cells=side * side))
+            ref = ucuda.run_basins(*rq["args"], **rq["kw"])
+            m = {}
+            for a32 in got["attractors"]:
+                for a64 in ref["attractors"]:
+                    if a32["period"] == a64["period"] and abs(a32["point"][0] - a64["point"][0]) <= 3e-3 and \
+                            abs(a32["point"][1] - a64["point"][1]) <= 3e-3:
+                        m[a32["label"]] = a64["label"]
+            mapped = np.array([m.get(int(v), 0 if v == 0 else -99) for v in got["labels"]])
+            audit.append(dict(kind=kind, cells=side * side,
+                              label_agreement=float((mapped == ref["labels"].cpu().numpy()).mean())))
     srv.close()
-    return dict(arrival_rate_per_s=rate, offered_utilisation=rate * est / capacity if capacity else None,
-                jobs=len(ids), completed=st["completed"], p50_ms=st["p50_ms"], p99_ms=st["p99_ms"],
-                max_ms=st["max_ms"], traj_steps_per_s=st["traj_steps"] / wall, rounds=st["rounds"],
-                busy_frac=st["busy_s"] / wall, audit_fp64=audit)
+    kinds = {k: sum(1 for _, kk, _, _ in ids if kk == k) for k in ("lle", "bif", "basins")}
+    return dict(arrival_rate_per_s=rate, offered_utilisation_est=rate * est / capacity if capacity else None,
+                jobs=len(ids), jobs_by_kind=kinds, completed=st["completed"], failed=len(failed),
+                p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], max_ms=st["max_ms"],
+                fixed_step_traj_steps_per_s=st["traj_steps"] / wall, rounds=st["rounds"],
+                gpu_busy_frac=st["busy_s"] / wall, wall_s=wall, audit_fp64=audit)
 
 
 def main():

@@ -252,11 +252,17 @@ int uc_server_create(uc_engine* e, const uc_server_config* cfg, uc_server** out)
 void uc_server_destroy(uc_server* s);
 int uc_server_submit_lyap(uc_server* s, uc_system* sys, const uc_lyap_request* r, int spectrum, long long* job_id);
 int uc_server_submit_bif(uc_server* s, uc_system* sys, const uc_bif_request* r, long long* job_id);
+/* Basin maps: the adaptive integration runs in the rounds like the other
+ * analyses; the attractor catalogue and the labels are computed when the
+ * job's last cell is done. Fetch with uc_server_fetch_basins. */
+int uc_server_submit_basins(uc_server* s, uc_system* sys, const uc_basins_request* r, long long* job_id);
 /* state: 0 queued, 1 running, 2 done, 3 cancelled, 4 failed, 5 fetched */
 int uc_server_poll(uc_server* s, long long job_id, int* state, double* latency_ms);
 int uc_server_wait(uc_server* s, long long job_id, double timeout_s);
 int uc_server_cancel(uc_server* s, long long job_id);
 int uc_server_fetch(uc_server* s, long long job_id, int* flags, double* out0, double* out1);
+/* host pointers: labels, periods (required), cycle_point, steps, catalogue (optional) */
+int uc_server_fetch_basins(uc_server* s, long long job_id, const uc_basins_output* o);
 int uc_server_stats(uc_server* s, uc_server_stats_t* stats);
 
 /* Number of cells of a request's grid (product of axis point counts). */

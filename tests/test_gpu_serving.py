@@ -1,6 +1,7 @@
 """Continuous-batching server: jobs submitted while others run are admitted
-into later rounds, share rounds with them, and every result is bit-identical
-to the same request run alone; cancellation, admission by memory budget and
+into later rounds, share rounds with them, and every result (bifurcation,
+Lyapunov, basins with their attractor catalogue) is bit-identical to the
+same request run alone; cancellation, admission by memory budget and
 latency statistics."""
 import time
 
@@ -26,11 +27,17 @@ def _mix(eng):
     jobs.append(("lyap", hr, dict(axes=[ucuda.axis(1, 2, 9, 0.0, 0.1), ucuda.axis(1, 1, 7, 0.2, 0.3)], ic=None,
                                   values=hr.values(), h=0.005, transient_time=2.0, t_max=10.0, NT=0.1, spectrum=True,
                                   ic_seed=20261203)))
+    duf = ucuda.preset("duffing", eng)
+    for prec in (ucuda.PREC_F64, ucuda.PREC_F32C):
+        jobs.append(("basins", duf, dict(axes=[ucuda.axis(2, 0, 24, -2.5, 2.5), ucuda.axis(2, 1, 20, -2.5, 2.5)],
+                                         ic=[0.0, 0.0], values=duf.values(), n_sections=40, precision=prec)))
     return jobs
 
 
 def _submit(srv, kind, sysm, kw):
     kw = dict(kw)
+    if kind == "basins":
+        return srv.submit_basins(sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), **kw)
     args = (sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), kw.pop("h"), kw.pop("transient_time"),
             kw.pop("t_max"))
     if kind == "lyap":
@@ -40,6 +47,11 @@ def _submit(srv, kind, sysm, kw):
 
 def _alone(kind, sysm, kw):
     kw = dict(kw)
+    if kind == "basins":
+        out = ucuda.run_basins(sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), **kw)
+        return dict(labels=out["labels"].cpu().numpy(), periods=out["periods"].cpu().numpy(),
+                    points=out["points"].cpu().numpy(), steps=out["steps"].cpu().numpy(),
+                    attractors=out["attractors"])
     args = (sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), kw.pop("h"), kw.pop("transient_time"),
             kw.pop("t_max"))
     if kind == "lyap":
@@ -51,7 +63,7 @@ def _alone(kind, sysm, kw):
 
 
 def test_results_identical_alone_and_in_the_mix(gpu_engine):
-    print("case: 8 staggered submissions (lle f64/f32c, bif, ls) with small rounds", flush=True)
+    print("case: 10 staggered submissions (lle f64/f32c, bif, ls, basins f64/f32c) with small rounds", flush=True)
     jobs = _mix(gpu_engine)
     alone = [_alone(k, s, kw) for k, s, kw in jobs]
     srv = ucuda.Server(gpu_engine, round_traj_steps=2e7)
@@ -70,6 +82,11 @@ def test_results_identical_alone_and_in_the_mix(gpu_engine):
         if k == "lyap":
             np.testing.assert_array_equal(got["flags"], ref["flags"])
             np.testing.assert_array_equal(got["values"], ref["values"])
+        elif k == "basins":
+            for key in ("labels", "periods", "points", "steps"):
+                np.testing.assert_array_equal(got[key], ref[key], err_msg=key)
+            assert got["attractors"] == ref["attractors"]
+            assert (got["labels"] > 0).any()
         else:
             np.testing.assert_array_equal(got["flags"], ref["flags"])
             for c in range(len(ref["flags"])):

@@ -602,15 +602,19 @@ int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lya
     return run_lyap(e, s, r, o, stream, true);
 }
 
-int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream) {
-    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
-    std::lock_guard<std::recursive_mutex> lk(e->mu);
-    int rc = UC_OK;
+}  // extern "C"
+
+namespace {
+
+// Request checks shared by uc_run_basins and the server; returns the
+// engine status and the number of labelled variables.
+int prep_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, uc_common& c, int& lbl) {
     if (s->custom) return e->fail(UC_ERR_ARGUMENT, "basins need a right-hand side (adaptive integrator), not a step body");
-    uc_common c = r->c;
+    c = r->c;
     if (c.t_max <= 0.0) c.t_max = 1.0;  // unused by basins
-    if ((rc = validate_common(e, s, c, false))) return rc;
-    const int lbl = s->n_vars < 2 ? s->n_vars : 2;
+    int rc = validate_common(e, s, c, false);
+    if (rc) return rc;
+    lbl = s->n_vars < 2 ? s->n_vars : 2;
     if (!(r->rtol > 0.0) || !(r->atol > 0.0)) return e->fail(UC_ERR_ARGUMENT, "rtol and atol must be > 0");
     if (!(r->section_period > 0.0)) return e->fail(UC_ERR_ARGUMENT, "section_period must be > 0");
     if (r->n_sections < 1) return e->fail(UC_ERR_ARGUMENT, "n_sections must be >= 1");
@@ -620,51 +624,53 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
     if (r->label_vars != 0 && r->label_vars != lbl) return e->fail(UC_ERR_ARGUMENT, "label_vars must be min(n_vars, 2)");
     if (!(r->period_tol > 0.0) || !(r->quantum > 0.0)) return e->fail(UC_ERR_ARGUMENT, "period_tol and quantum must be > 0");
     if (!(r->h_min > 0.0) || !(r->h_max >= r->h_min)) return e->fail(UC_ERR_ARGUMENT, "need 0 < h_min <= h_max");
-    if (!o->labels || !o->periods) return e->fail(UC_ERR_ARGUMENT, "labels / periods outputs missing");
-    if ((rc = enter(e))) return rc;
+    return UC_OK;
+}
 
-    Module* m = nullptr;
-    if ((rc = get_module(e, s, c.strict != 0, true, &m))) return rc;
-    UcJob j;
-    TableBuffer* tb = nullptr;
-    hipStream_t st = (hipStream_t)stream;
-    if ((rc = stage_tables(e, s, c, j, st, &tb))) return rc;
-    const long long n = j.n_cells;
+void fill_basins(const uc_basins_request* r, int lbl, UcJob& j) {
     j.rtol = r->rtol; j.atol = r->atol; j.h_min = r->h_min; j.h_max = r->h_max;
     j.section_period = r->section_period; j.n_sections = r->n_sections; j.n_record = r->n_record;
     j.max_period = r->max_period; j.label_vars = lbl; j.period_tol = r->period_tol; j.quantum = r->quantum;
     j.max_attempts = r->max_attempts > 0 ? r->max_attempts : 100000000LL;
-    j.flag = o->periods;
-    j.counters = o->steps;
-    j.label = o->labels;
+}
 
-    // temporaries: canonical points (when the caller does not want them) and the key table
+struct BasinsCatalogue {
+    int n_attractors = 0;
+    std::vector<int> period;      // per attractor, canonical order
+    std::vector<double> point;    // [2] per attractor
+};
+
+// Attractor catalogue of a finished basins integration and the per-cell
+// labels: the distinct quantised signatures go into a device hash set
+// (growing it while it overflows), the host clusters keys of equal period
+// whose quantised points are adjacent, orders the clusters by (period,
+// point) with the no-cycle class last and numbers them 1..K, and a kernel
+// writes every cell's label. `j` is the whole-grid descriptor (flag =
+// periods, out0 = canonical points, label = labels). Device temporaries come
+// from alloc / release; returns an error text, empty on success.
+std::string basins_catalogue(Module* m, UcJob& j, double quantum, hipStream_t st,
+                             const std::function<hipError_t(void**, size_t)>& alloc,
+                             const std::function<void(void*)>& release, BasinsCatalogue& cat) {
     std::vector<void*> temps;
-    auto cleanup = [&]() { for (void* p : temps) (void)hipFree(p); };
-    auto dev_alloc = [&](void** p, size_t bytes) -> int {
-        hipError_t he = hipMalloc(p, bytes);
-        if (he != hipSuccess) { cleanup(); return e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(basins)")); }
+    auto cleanup = [&]() { for (void* p : temps) release(p); temps.clear(); };
+    auto dev_alloc = [&](void** p, size_t bytes) -> bool {
+        if (alloc(p, bytes) != hipSuccess) return false;
         temps.push_back(*p);
-        return UC_OK;
+        return true;
     };
-    double* points = o->cycle_point;
-    if (!points && (rc = dev_alloc((void**)&points, sizeof(double) * (size_t)lbl * (size_t)n))) return rc;
-    j.out0 = points;
-    if ((rc = launch(e, m, c.precision == UC_PREC_F64 ? "uc_bas_f64" : "uc_bas_f32c", j, st, tb))) { cleanup(); return rc; }
-
-    // ---- catalogue of distinct quantised signatures ----
+    const long long n = j.n_cells;
     hipFunction_t fkeys, flabels;
     hipError_t he = hipModuleGetFunction(&fkeys, m->mod, "uc_bas_keys");
     if (he == hipSuccess) he = hipModuleGetFunction(&flabels, m->mod, "uc_bas_labels");
-    if (he != hipSuccess) { cleanup(); return e->fail(UC_ERR_HIP, hip_msg(he, "basins label kernels")); }
+    if (he != hipSuccess) return hip_msg(he, "basins label kernels");
     const unsigned blocks = (unsigned)((n + kBlock - 1) / kBlock);
     int cap = 1 << 16;
     std::vector<unsigned long long> keys;
     unsigned long long* table = nullptr;
     unsigned long long* overflow = nullptr;
-    if ((rc = dev_alloc((void**)&overflow, sizeof(unsigned long long)))) return rc;
+    if (!dev_alloc((void**)&overflow, sizeof(unsigned long long))) { cleanup(); return "basins: device memory"; }
     for (;;) {
-        if ((rc = dev_alloc((void**)&table, sizeof(unsigned long long) * (size_t)cap))) return rc;
+        if (!dev_alloc((void**)&table, sizeof(unsigned long long) * (size_t)cap)) { cleanup(); return "basins: device memory"; }
         he = hipMemsetAsync(table, 0xFF, sizeof(unsigned long long) * (size_t)cap, st);
         if (he == hipSuccess) he = hipMemsetAsync(overflow, 0, sizeof(unsigned long long), st);
         void* kargs[] = {&j, &table, &cap, &overflow};
@@ -674,9 +680,9 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
         if (he == hipSuccess) he = hipMemcpyAsync(keys.data(), table, sizeof(unsigned long long) * (size_t)cap, hipMemcpyDeviceToHost, st);
         if (he == hipSuccess) he = hipMemcpyAsync(&ovf, overflow, sizeof ovf, hipMemcpyDeviceToHost, st);
         if (he == hipSuccess) he = hipStreamSynchronize(st);
-        if (he != hipSuccess) { cleanup(); return e->fail(UC_ERR_HIP, hip_msg(he, "basins signature keys")); }
+        if (he != hipSuccess) { cleanup(); return hip_msg(he, "basins signature keys"); }
         if (ovf == 0) break;
-        if (cap >= (1 << 24)) { cleanup(); return e->fail(UC_ERR_ARGUMENT, "too many distinct attractor signatures (raise quantum)"); }
+        if (cap >= (1 << 24)) { cleanup(); return "too many distinct attractor signatures (raise quantum)"; }
         cap <<= 2;
     }
 
@@ -725,23 +731,77 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
     for (size_t k = 0; k < order.size(); ++k) label_of_root[order[k].second] = (int)k + 1;
     std::vector<int> slot_label((size_t)cap, 0);
     for (size_t i = 0; i < slots.size(); ++i) slot_label[(size_t)slots[i]] = label_of_root[find((int)i)];
-    if (o->n_attractors) *o->n_attractors = (int)order.size();
-    for (size_t k = 0; k < order.size() && (int)k < o->max_attractors; ++k) {
+    cat.n_attractors = (int)order.size();
+    cat.period.clear();
+    cat.point.clear();
+    for (size_t k = 0; k < order.size(); ++k) {
         const KeyInfo& ki = rep[order[k].second];
-        if (o->attractor_period) o->attractor_period[k] = ki.p;
-        if (o->attractor_point) {
-            o->attractor_point[2 * k] = ki.p ? (double)ki.qx * r->quantum : 0.0;
-            o->attractor_point[2 * k + 1] = ki.p ? (double)ki.qv * r->quantum : 0.0;
-        }
+        cat.period.push_back(ki.p);
+        cat.point.push_back(ki.p ? (double)ki.qx * quantum : 0.0);
+        cat.point.push_back(ki.p ? (double)ki.qv * quantum : 0.0);
     }
     int* dlabels = nullptr;
-    if ((rc = dev_alloc((void**)&dlabels, sizeof(int) * (size_t)cap))) return rc;
+    if (!dev_alloc((void**)&dlabels, sizeof(int) * (size_t)cap)) { cleanup(); return "basins: device memory"; }
     he = hipMemcpyAsync(dlabels, slot_label.data(), sizeof(int) * (size_t)cap, hipMemcpyHostToDevice, st);
     void* largs[] = {&j, &table, &dlabels, &cap};
     if (he == hipSuccess) he = hipModuleLaunchKernel(flabels, blocks, 1, 1, kBlock, 1, 1, 0, st, largs, nullptr);
     if (he == hipSuccess) he = hipStreamSynchronize(st);
     cleanup();
-    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "basins labels"));
+    return he == hipSuccess ? std::string() : hip_msg(he, "basins labels");
+}
+
+void copy_catalogue(const BasinsCatalogue& cat, const uc_basins_output* o) {
+    if (o->n_attractors) *o->n_attractors = cat.n_attractors;
+    for (int k = 0; k < cat.n_attractors && k < o->max_attractors; ++k) {
+        if (o->attractor_period) o->attractor_period[k] = cat.period[(size_t)k];
+        if (o->attractor_point) {
+            o->attractor_point[2 * k] = cat.point[2 * (size_t)k];
+            o->attractor_point[2 * k + 1] = cat.point[2 * (size_t)k + 1];
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    uc_common c;
+    int lbl = 0;
+    int rc = prep_basins(e, s, r, c, lbl);
+    if (rc) return rc;
+    if (!o->labels || !o->periods) return e->fail(UC_ERR_ARGUMENT, "labels / periods outputs missing");
+    if ((rc = enter(e))) return rc;
+
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = stage_tables(e, s, c, j, st, &tb))) return rc;
+    const long long n = j.n_cells;
+    fill_basins(r, lbl, j);
+    j.flag = o->periods;
+    j.counters = o->steps;
+    j.label = o->labels;
+
+    // canonical points: the caller's buffer, or a temporary
+    double* points = o->cycle_point;
+    if (!points) {
+        hipError_t he = hipMalloc((void**)&points, sizeof(double) * (size_t)lbl * (size_t)n);
+        if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(basins)"));
+    }
+    auto free_points = [&]() { if (points != o->cycle_point) (void)hipFree(points); };
+    j.out0 = points;
+    if ((rc = launch(e, m, c.precision == UC_PREC_F64 ? "uc_bas_f64" : "uc_bas_f32c", j, st, tb))) { free_points(); return rc; }
+    BasinsCatalogue cat;
+    const std::string why = basins_catalogue(
+        m, j, r->quantum, st, [](void** p, size_t b) { return hipMalloc(p, b); }, [](void* p) { (void)hipFree(p); }, cat);
+    free_points();
+    if (!why.empty()) return e->fail(why.rfind("too many", 0) == 0 ? UC_ERR_ARGUMENT : UC_ERR_HIP, why);
+    copy_catalogue(cat, o);
     return UC_OK;
 }
 
@@ -1096,7 +1156,7 @@ struct uc_server {
     uc_server_config cfg{};
     struct Job {
         long long id = 0;
-        int kind = 0;  // 0 bif, 1 lle, 2 ls
+        int kind = 0;  // 0 bif, 1 lle, 2 ls, 3 basins
         Module* m = nullptr;
         std::string kernel;
         UcJob tmpl{};
@@ -1121,6 +1181,14 @@ struct uc_server {
         uc_lyap_request lyap{};
         int n_exp = 1, max_peaks = 0, n_vars = 0;
         long long iters = 0;
+        // basins: out0 = canonical points [lbl][cells], flags = periods,
+        // labels and step counters [2][cells]; the catalogue is built when
+        // the last cell is done
+        uc_basins_request bas{};
+        int* labels = nullptr;
+        long long* steps = nullptr;
+        int lbl = 0;
+        BasinsCatalogue cat;
         std::string error;
     };
     struct Entry {
@@ -1176,7 +1244,9 @@ void server_free(uc_server* S, uc_server::Job& j) {
     server_release(S, j.out0, S->stream);
     server_release(S, j.out1, S->stream);
     server_release(S, j.tables, S->stream);
-    j.flags = nullptr; j.out0 = j.out1 = nullptr; j.tables = nullptr;
+    server_release(S, j.labels, S->stream);
+    server_release(S, j.steps, S->stream);
+    j.flags = nullptr; j.out0 = j.out1 = nullptr; j.tables = nullptr; j.labels = nullptr; j.steps = nullptr;
 }
 
 // admission: allocate outputs + tables, build the descriptor template
@@ -1190,10 +1260,19 @@ bool server_admit(uc_server* S, uc_server::Job& j) {
     if (he == hipSuccess) he = server_malloc(S, (void**)&j.flags, sizeof(int) * (size_t)j.cells, S->stream);
     if (he == hipSuccess) he = server_malloc(S, (void**)&j.out0, sizeof(double) * std::max<size_t>(1, j.out0_n), S->stream);
     if (he == hipSuccess && j.out1_n) he = server_malloc(S, (void**)&j.out1, sizeof(double) * j.out1_n, S->stream);
+    if (he == hipSuccess && j.kind == 3) he = server_malloc(S, (void**)&j.labels, sizeof(int) * (size_t)j.cells, S->stream);
+    if (he == hipSuccess && j.kind == 3)
+        he = server_malloc(S, (void**)&j.steps, 2 * sizeof(long long) * (size_t)j.cells, S->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(S->stream);  // tables uploaded, buffers usable on any stream
     if (he != hipSuccess) { server_free(S, j); j.error = hip_msg(he, "admission"); return false; }
     wire_job(j.c, j.tables, j.tmpl);
-    if (j.kind == 0) {
+    if (j.kind == 3) {
+        fill_basins(&j.bas, j.lbl, j.tmpl);
+        j.tmpl.flag = j.flags;
+        j.tmpl.out0 = j.out0;
+        j.tmpl.counters = j.steps;
+        j.tmpl.label = j.labels;
+    } else if (j.kind == 0) {
         uc_bif_output o{j.flags, j.out0, j.out1, nullptr, 0, nullptr};
         fill_bif(&j.bif, &o, j.iters, j.tmpl);
     } else {
@@ -1287,6 +1366,7 @@ std::string server_launch(uc_server* S, uc_server::Slot& sl, int block, int wps,
             j.flag = en->job->flags + en->first;
             j.out0 = en->job->out0 + en->first;
             j.out1 = en->job->out1 ? en->job->out1 + en->first : nullptr;
+            j.counters = en->job->steps ? en->job->steps + en->first : nullptr;
             sl.h_jobs[jo++] = j;
             g.chunks += (en->cells + block - 1) / block;
             sl.h_start[so++] = g.chunks;
@@ -1328,10 +1408,21 @@ void server_retire(uc_server* S, uc_server::Slot& sl, const std::string& why) {
             continue;
         }
         J.done_cells += en.cells;
-        S->traj_steps_done += (double)en.cells * J.step_cost;
+        if (J.kind != 3) S->traj_steps_done += (double)en.cells * J.step_cost;  // fixed-step work only
         if (J.state == 1 && J.done_cells >= J.cells) {
+            if (J.kind == 3) {
+                UcJob g = J.tmpl;  // whole grid
+                g.cell_offset = 0;
+                g.n_cells = J.cells;
+                g.stride = J.cells;
+                const std::string err = basins_catalogue(
+                    J.m, g, J.bas.quantum, S->stream,
+                    [S](void** ptr, size_t b) { return server_malloc(S, ptr, b, S->stream); },
+                    [S](void* ptr) { server_release(S, ptr, S->stream); }, J.cat);
+                if (!err.empty()) { J.state = 4; J.error = err; J.t_done = std::chrono::steady_clock::now(); continue; }
+            }
             J.state = 2;
-            J.t_done = now;
+            J.t_done = std::chrono::steady_clock::now();
             S->latencies_ms.push_back(std::chrono::duration<double, std::milli>(J.t_done - J.t_submit).count());
         }
     }
@@ -1580,6 +1671,31 @@ int uc_server_submit_bif(uc_server* S, uc_system* s, const uc_bif_request* r, lo
     return server_submit(S, j, id);
 }
 
+int uc_server_submit_basins(uc_server* S, uc_system* s, const uc_basins_request* r, long long* id) {
+    if (!S || !s || !r) return UC_ERR_ARGUMENT;
+    uc_engine* e = S->e;
+    auto j = std::make_shared<uc_server::Job>();
+    uc_common c;
+    {
+        std::lock_guard<std::recursive_mutex> lk(e->mu);
+        int rc = prep_basins(e, s, r, c, j->lbl);
+        if (rc) return rc;
+        if ((rc = enter(e))) return rc;
+        if ((rc = get_module(e, s, c.strict != 0, true, &j->m))) return rc;
+        ++j->m->pins;
+    }
+    server_copy_common(*j, c, s->n_vars);
+    j->bas = *r;
+    j->bas.c = j->c;
+    j->kind = 3;
+    j->kernel = c.precision == UC_PREC_F64 ? "uc_bas_f64_q" : "uc_bas_f32c_q";
+    j->out0_n = (size_t)j->lbl * (size_t)j->cells;
+    j->bytes = (size_t)j->cells * (2 * sizeof(int) + 2 * sizeof(long long) + (size_t)j->lbl * sizeof(double));
+    // scheduling weight only (the integrator is adaptive): ~64 steps per section
+    j->step_cost = 64.0 * (double)r->n_sections;
+    return server_submit(S, j, id);
+}
+
 int uc_server_poll(uc_server* S, long long id, int* state, double* latency_ms) {
     if (!S) return UC_ERR_ARGUMENT;
     std::lock_guard<std::mutex> lk(S->mu);
@@ -1625,6 +1741,7 @@ int uc_server_fetch(uc_server* S, long long id, int* flags, double* out0, double
         if (it == S->all.end()) return S->e->fail(UC_ERR_ARGUMENT, "unknown job id");
         j = it->second;
         if (j->state != 2) return j->state < 2 ? UC_ERR_BUSY : S->e->fail(UC_ERR_ARGUMENT, "job did not complete: " + j->error);
+        if (j->kind == 3) return S->e->fail(UC_ERR_ARGUMENT, "basins job: use uc_server_fetch_basins");
     }
     (void)hipSetDevice(S->e->device);
     hipError_t he = hipSuccess;
@@ -1635,6 +1752,40 @@ int uc_server_fetch(uc_server* S, long long id, int* flags, double* out0, double
         he = hipMemcpyAsync(out1, j->out1, sizeof(double) * j->out1_n, hipMemcpyDeviceToHost, S->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(S->stream);
     if (he != hipSuccess) return S->e->fail(UC_ERR_HIP, hip_msg(he, "fetch"));
+    std::lock_guard<std::mutex> lk(S->mu);
+    S->used_bytes -= j->bytes;
+    server_free(S, *j);
+    if (j->m) { --j->m->pins; j->m = nullptr; }
+    j->state = 5;  // fetched
+    S->cv.notify_all();
+    return UC_OK;
+}
+
+// Copies a finished basins job's results to host memory (labels and periods
+// required; cycle_point [2][cells], steps [2][cells] and the attractor
+// catalogue when given) and releases its device buffers.
+int uc_server_fetch_basins(uc_server* S, long long id, const uc_basins_output* o) {
+    if (!S || !o || !o->labels || !o->periods) return UC_ERR_ARGUMENT;
+    std::shared_ptr<uc_server::Job> j;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->all.find(id);
+        if (it == S->all.end()) return S->e->fail(UC_ERR_ARGUMENT, "unknown job id");
+        j = it->second;
+        if (j->state != 2) return j->state < 2 ? UC_ERR_BUSY : S->e->fail(UC_ERR_ARGUMENT, "job did not complete: " + j->error);
+        if (j->kind != 3) return S->e->fail(UC_ERR_ARGUMENT, "not a basins job: use uc_server_fetch");
+    }
+    (void)hipSetDevice(S->e->device);
+    const size_t n = (size_t)j->cells;
+    hipError_t he = hipMemcpyAsync(o->labels, j->labels, sizeof(int) * n, hipMemcpyDeviceToHost, S->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(o->periods, j->flags, sizeof(int) * n, hipMemcpyDeviceToHost, S->stream);
+    if (he == hipSuccess && o->cycle_point)
+        he = hipMemcpyAsync(o->cycle_point, j->out0, sizeof(double) * j->out0_n, hipMemcpyDeviceToHost, S->stream);
+    if (he == hipSuccess && o->steps)
+        he = hipMemcpyAsync(o->steps, j->steps, 2 * sizeof(long long) * n, hipMemcpyDeviceToHost, S->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(S->stream);
+    if (he != hipSuccess) return S->e->fail(UC_ERR_HIP, hip_msg(he, "fetch"));
+    copy_catalogue(j->cat, o);
     std::lock_guard<std::mutex> lk(S->mu);
     S->used_bytes -= j->bytes;
     server_free(S, *j);

@@ -140,6 +140,8 @@ _SIGNATURES = {
     "uc_server_wait": (C.c_int, [C.c_void_p, C.c_longlong, C.c_double]),
     "uc_server_cancel": (C.c_int, [C.c_void_p, C.c_longlong]),
     "uc_server_fetch": (C.c_int, [C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uc_server_submit_basins": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BasinsRequest), C.POINTER(C.c_longlong)]),
+    "uc_server_fetch_basins": (C.c_int, [C.c_void_p, C.c_longlong, C.POINTER(BasinsOutput)]),
     "uc_server_stats": (C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     "uc_grid_cells": (C.c_longlong, [C.POINTER(Common)]),
     "uc_axis_values": (C.c_int, [C.POINTER(Axis), C.POINTER(C.c_double)]),
@@ -154,6 +156,12 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise UcError("libucuda.so not built: run __graft_entry__.build()")
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7
+        # (same soname as the system one libucuda.so links), so torch is
+        # loaded first and libucuda.so binds to the runtime torch mapped;
+        # loading libucuda.so first would map a second runtime that torch's
+        # own device initialisation then fails on.
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(L, name)
@@ -429,6 +437,19 @@ def run_lyap(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=Fals
     return out
 
 
+def _basins_request(sysm, axes, ic, values, h0, rtol, atol, section_period, n_sections, n_record, max_period,
+                    period_tol, quantum, h_min, h_max, max_attempts, max_value, precision, strict):
+    r = BasinsRequest()
+    r.c, keep = _common(sysm, axes, ic, values, h0, 0.0, 1.0, max_value, precision, strict)
+    r.rtol, r.atol, r.h_min = float(rtol), float(atol), float(h_min)
+    r.h_max = float(h_max) if h_max is not None else float(section_period) / 4.0
+    r.section_period, r.n_sections, r.n_record = float(section_period), int(n_sections), int(n_record)
+    r.max_period, r.label_vars, r.period_tol, r.quantum = (int(max_period), min(sysm.n_vars, 2), float(period_tol),
+                                                           float(quantum))
+    r.max_attempts = int(max_attempts)
+    return r, keep
+
+
 def run_basins(sysm, axes, ic, values, h0=0.01, rtol=1e-9, atol=1e-9, section_period=2 * math.pi,
                n_sections=200, n_record=16, max_period=8, period_tol=1e-4, quantum=1e-3, h_min=1e-12,
                h_max=None, max_attempts=0, max_value=1e6, precision=PREC_F64, strict=False,
@@ -444,13 +465,8 @@ def run_basins(sysm, axes, ic, values, h0=0.01, rtol=1e-9, atol=1e-9, section_pe
                periods=torch.empty(n, dtype=torch.int32, device=dev),
                points=torch.empty((lbl, n), dtype=torch.float64, device=dev) if want_points else None,
                steps=torch.empty((2, n), dtype=torch.int64, device=dev) if want_steps else None)
-    r = BasinsRequest()
-    r.c, keep = _common(sysm, axes, ic, values, h0, 0.0, 1.0, max_value, precision, strict)
-    r.rtol, r.atol, r.h_min = float(rtol), float(atol), float(h_min)
-    r.h_max = float(h_max) if h_max is not None else float(section_period) / 4.0
-    r.section_period, r.n_sections, r.n_record = float(section_period), int(n_sections), int(n_record)
-    r.max_period, r.label_vars, r.period_tol, r.quantum = int(max_period), lbl, float(period_tol), float(quantum)
-    r.max_attempts = int(max_attempts)
+    r, keep = _basins_request(sysm, axes, ic, values, h0, rtol, atol, section_period, n_sections, n_record,
+                              max_period, period_tol, quantum, h_min, h_max, max_attempts, max_value, precision, strict)
     na = C.c_int(0)
     ap = (C.c_int * max_attractors)()
     apt = (C.c_double * (2 * max_attractors))()
@@ -626,6 +642,19 @@ class Server:
         del keep
         return jid.value
 
+    def submit_basins(self, sysm, axes, ic, values, h0=0.01, rtol=1e-9, atol=1e-9, section_period=2 * math.pi,
+                      n_sections=200, n_record=16, max_period=8, period_tol=1e-4, quantum=1e-3, h_min=1e-12,
+                      h_max=None, max_attempts=0, max_value=1e6, precision=PREC_F64, strict=False):
+        r, keep = _basins_request(sysm, axes, ic, values, h0, rtol, atol, section_period, n_sections, n_record,
+                                  max_period, period_tol, quantum, h_min, h_max, max_attempts, max_value, precision,
+                                  strict)
+        jid = C.c_longlong()
+        self.engine.check(lib().uc_server_submit_basins(self.h, sysm.h, C.byref(r), C.byref(jid)),
+                          "uc_server_submit_basins")
+        self.meta[jid.value] = ("basins", n_cells(axes), min(sysm.n_vars, 2), 0)
+        del keep
+        return jid.value
+
     def poll(self, jid):
         st, lat = C.c_int(), C.c_double()
         self.engine.check(lib().uc_server_poll(self.h, jid, C.byref(st), C.byref(lat)), "uc_server_poll")
@@ -638,9 +667,23 @@ class Server:
     def cancel(self, jid):
         self.engine.check(lib().uc_server_cancel(self.h, jid), "uc_server_cancel")
 
-    def fetch(self, jid):
+    def fetch(self, jid, max_attractors=64):
         import numpy as np
         kind, n, k, two = self.meta[jid]
+        if kind == "basins":
+            labels = np.zeros(n, dtype=np.int32)
+            periods = np.zeros(n, dtype=np.int32)
+            points = np.zeros((k, n))
+            steps = np.zeros((2, n), dtype=np.int64)
+            na = C.c_int(0)
+            ap = (C.c_int * max_attractors)()
+            apt = (C.c_double * (2 * max_attractors))()
+            o = BasinsOutput(labels.ctypes.data, periods.ctypes.data, points.ctypes.data, steps.ctypes.data,
+                             max_attractors, C.pointer(na), ap, apt)
+            self.engine.check(lib().uc_server_fetch_basins(self.h, jid, C.byref(o)), "uc_server_fetch_basins")
+            att = [dict(label=i + 1, period=ap[i], point=(apt[2 * i], apt[2 * i + 1]))
+                   for i in range(min(na.value, max_attractors))]
+            return dict(labels=labels, periods=periods, points=points, steps=steps, attractors=att)
         flags = np.zeros(n, dtype=np.int32)
         out0 = np.zeros((k, n))
         out1 = np.zeros((k, n)) if two else None
