@@ -48,6 +48,8 @@ extern "C" {
 #define UC_SCHEME_EULER_CROMER 1
 #define UC_SCHEME_MIDPOINT 2
 #define UC_SCHEME_RK4 3
+#define UC_SCHEME_DOPRI78 4 /* Dormand-Prince 8(7), 13 stages, fixed step */
+#define UC_SCHEME_CD 5      /* composition explicit / diagonally implicit halves, symmetry a[0] */
 
 typedef struct uc_engine uc_engine;
 typedef struct uc_system uc_system;

@@ -38,9 +38,20 @@ SYSTEMS = [
          rhs=["y", "x ( \\rho-z )-y", "x"], latex=True),
     dict(name="latex_misc", vars=["x", "y"], params=["a", "b"],
          rhs=["\\frac{x}{1+y^{2}}-a \\cdot \\sin{y}", "\\left( x-b \\right) \\cdot y+\\exp{-x}"], latex=True),
+    # CD classification cases: coef*var, var*coef, bare var, (a)*(b),
+    # name*(...), powers, var*var, calls, absent variable
+    dict(name="cd_linear", vars=["x", "y", "z"], params=["a", "b", "c"],
+         rhs=["-a*x + y", "y*b - x*z", "z + 2*x - c"], latex=False),
+    dict(name="cd_parens", vars=["x", "y", "z"], params=["a", "b"],
+         rhs=["(a - b)*(y - x)", "a*(x - y + z)", "(x + y)*(z - 1)"], latex=False),
+    dict(name="cd_nonlinear", vars=["u", "v", "w"], params=["k", "m"],
+         rhs=["u^2 - v", "sin(v) + k*u", "w*w - m*exp(w) + u"], latex=False),
+    dict(name="cd_absent", vars=["p", "q"], params=["g"],
+         rhs=["q", "-g*sin(p) - 0.1*q"], latex=False),
+    dict(name="cd_one_var", vars=["x"], params=["r"], rhs=["r*x*(1-x)"], latex=False),
 ]
 NORMALIZE = ["8/3", "1e-5", "2", "8.5", "-0.5", "10", "0.1", "2^3", "3.0e2", "1/7"]
-SCHEMES = [0, 1, 2, 3]  # Euler, Euler-Cromer, Explicit Midpoint, RK4
+SCHEMES = [0, 1, 2, 3, 4, 5]  # Euler, Euler-Cromer, Explicit Midpoint, RK4, DOPRI78, CD
 
 
 def strs(items):

@@ -312,11 +312,13 @@ def test_ls_hyper_rossler_random_ics(gpu_engine):
 
 
 def test_custom_krs_bodies(gpu_engine):
-    print("case: custom KRS step bodies (Euler, midpoint, RK4 text) -> bif + FD LLE vs oracle", flush=True)
+    print("case: custom KRS step bodies (Euler, midpoint, RK4, DOPRI78, CD text) -> bif + FD LLE vs oracle",
+          flush=True)
     lor = ucuda.preset("lorenz", gpu_engine)
     vals = lor.values()
     axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 16, 10.0, 200.0)]
-    for scheme in (ucuda.SCHEME_EULER, ucuda.SCHEME_MIDPOINT, ucuda.SCHEME_RK4):
+    for scheme in (ucuda.SCHEME_EULER, ucuda.SCHEME_MIDPOINT, ucuda.SCHEME_RK4, ucuda.SCHEME_DOPRI78,
+                   ucuda.SCHEME_CD):
         body = gpu_engine.codegen_scheme(lor.vars, lor.params, lor.rhs, scheme)
         ks = ucuda.KrsSystem(gpu_engine, body, 3, 4)
         g = ucuda.run_bif(ks, axes, [0.1, 0.1, 0.1], vals, 0.005, 5.0, 10.0, max_peaks=32, strict=True, traj_len=40)

@@ -298,3 +298,34 @@ def test_hyperchaotic_rossler_spectrum_published(cpu_engine):
     assert r["flags"][0] == 1
     l1, l2, l3, l4 = r["values"][:, 0] * 20 / 21
     assert abs(l1 - 0.112) < 0.01 and abs(l2 - 0.019) < 0.005 and abs(l3) < 2e-3 and -30 < l4 < -20
+
+
+def test_dopri78_and_cd_step_bodies(cpu_engine):
+    """The DOPRI78 and CD step texts (byte-identical to the reference's)
+    integrate as their methods say. Linear decay x' = -k x, y' = -m y:
+    one CD step with symmetry s = a[0] is exactly
+    (1 - s h k) / (1 + (1 - s) h k) (explicit then implicit half step;
+    s = 1/2 is the trapezoidal rule); one DOPRI78 step matches exp(-h k) to
+    the 13 significant digits its printed coefficients carry, far below
+    RK4's h^5 / 120."""
+    vars_, params, rhs = ["x", "y"], ["k", "m"], ["-k*x", "-m*y"]
+    h, k, m = 0.1, 1.3, 0.4
+    for s_sym in (0.5, 0.3):
+        body = cpu_engine.codegen_scheme(vars_, params, rhs, ucuda.SCHEME_CD)
+        step = oracle.KrsStep(body, 2, 3)
+        X = (C.c_double * 2)(1.0, 2.0)
+        step.fn(X, (C.c_double * 3)(s_sym, k, m), C.c_double(h))
+        for got, x0, c in ((X[0], 1.0, k), (X[1], 2.0, m)):
+            expect = x0 * (1 - s_sym * h * c) / (1 + (1 - s_sym) * h * c)
+            assert abs(got - expect) <= 1e-15 * abs(expect)
+    body = cpu_engine.codegen_scheme(vars_, params, rhs, ucuda.SCHEME_DOPRI78)
+    step = oracle.KrsStep(body, 2, 3)
+    X = (C.c_double * 2)(1.0, 2.0)
+    step.fn(X, (C.c_double * 3)(0.5, k, m), C.c_double(h))
+    rk4 = oracle.KrsStep(cpu_engine.codegen_scheme(vars_, params, rhs, ucuda.SCHEME_RK4), 2, 3)
+    Y = (C.c_double * 2)(1.0, 2.0)
+    rk4.fn(Y, (C.c_double * 3)(0.5, k, m), C.c_double(h))
+    for got, ref4, x0, c in ((X[0], Y[0], 1.0, k), (X[1], Y[1], 2.0, m)):
+        exact = x0 * math.exp(-h * c)
+        assert abs(got - exact) < 1e-12 * x0
+        assert abs(ref4 - exact) > 1e3 * abs(got - exact)

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <regex>
 #include <memory>
 #include <set>
 #include <sstream>
@@ -627,6 +628,275 @@ std::string krs_rk4(const System& s) {
     return o.str();
 }
 
+
+// Dormand-Prince 8(7) 13-stage step (Prince & Dormand 1981, RK8(7)13M) in
+// the reference's text form (U-CUDA/codegen.cpp:611-675): the coefficient
+// tables are printed exactly as the reference prints them (13 significant
+// digits), the 8th-order solution y replaces X, the 7th-order z is formed
+// and discarded.
+std::string krs_dopri78(const System& s) {
+    static const char* const kTables =
+        "    static const numb M[13][12] = {\n"
+        "        {0,0,0,0,0,0,0,0,0,0,0,0},\n"
+        "        {0.05555555555556,0,0,0,0,0,0,0,0,0,0,0},\n"
+        "        {0.02083333333333,0.0625,0,0,0,0,0,0,0,0,0,0},\n"
+        "        {0.03125,0,0.09375,0,0,0,0,0,0,0,0,0},\n"
+        "        {0.3125,0,-1.171875,1.171875,0,0,0,0,0,0,0,0},\n"
+        "        {0.0375,0,0,0.1875,0.15,0,0,0,0,0,0,0},\n"
+        "        {0.04791013711111,0,0,0.1122487127778,-0.02550567377778,0.01284682388889,0,0,0,0,0,0},\n"
+        "        {0.01691798978729,0,0,0.387848278486,0.0359773698515,0.1969702142157,-0.1727138523405,0,0,0,0,0},\n"
+        "        {0.06909575335919,0,0,-0.6342479767289,-0.1611975752246,0.1386503094588,0.9409286140358,0.2116363264819,0,0,0,0},\n"
+        "        {0.183556996839,0,0,-2.468768084316,-0.2912868878163,-0.02647302023312,2.847838764193,0.2813873314699,0.1237448998633,0,0,0},\n"
+        "        {-1.215424817396,0,0,16.67260866595,0.9157418284168,-6.056605804357,-16.00357359416,14.8493030863,-13.37157573529,5.13418264818,0,0},\n"
+        "        {0.2588609164383,0,0,-4.774485785489,-0.435093013777,-3.049483332072,5.577920039936,6.155831589861,-5.062104586737,2.193926173181,0.1346279986593,0},\n"
+        "        {0.8224275996265,0,0,-11.65867325728,-0.7576221166909,0.7139735881596,12.07577498689,-2.12765911392,1.990166207049,-0.234286471544,0.1758985777079,0}\n"
+        "    };\n"
+        "    static const numb B[2][13] = {\n"
+        "        {0.04174749114153,0,0,0,0,-0.05545232861124,0.2393128072012,0.7035106694034,-0.7597596138145,0.6605630309223,0.1581874825101,-0.2381095387529,0.25},\n"
+        "        {0.02955321367635,0,0,0,0,-0.8286062764878,0.3112409000511,2.4673451906,-2.546941651842,1.443548583677,0.07941559588113,0.04444444444444,0}\n"
+        "    };\n";
+    const int N = (int)s.vars.size();
+    const auto k = rhs_text(s, "X1");
+    std::ostringstream o;
+    o << kTables;
+    for (const char* arr : {"X1", "X2", "y", "z"}) o << "    numb " << arr << "[" << N << "];\n";
+    o << "    numb k[" << N << "][13];\n    int N = " << N << ";\n    int i, j, l;\n";
+    o << "    for (i = 0; i < N; ++i) X1[i] = X[i];\n    for (i = 0; i < 13; ++i) {\n";
+    for (int v = 0; v < N; ++v) o << "        k[" << v << "][i] = (" << k[v] << ");\n";
+    o << "        if (i != 12) {\n"
+         "            for (l = 0; l < N; ++l) X2[l] = 0;\n"
+         "            for (j = 0; j < i + 1; ++j)\n"
+         "                for (l = 0; l < N; ++l)\n"
+         "                    X2[l] += M[i + 1][j] * k[l][j];\n"
+         "            for (l = 0; l < N; ++l)\n"
+         "                X1[l] = X[l] + h * X2[l];\n"
+         "        }\n    }\n";
+    // weighted sums of the stages: row 0 -> y (order 8), row 1 -> z (order 7)
+    const char* const target[2] = {"y", "z"};
+    for (int row = 0; row < 2; ++row) {
+        o << "    for (l = 0; l < N; ++l) X2[l] = 0;\n"
+             "    for (i = 0; i < 13; ++i)\n"
+             "        for (l = 0; l < N; ++l)\n"
+             "            X2[l] += B[" << row << "][i] * k[l][i];\n"
+             "    for (l = 0; l < N; ++l)\n"
+             "        " << target[row] << "[l] = X[l] + h * X2[l];\n";
+    }
+    o << "    (void)z[0];\n    for (l = 0; l < N; ++l) X[l] = y[l];\n";
+    return o.str();
+}
+
+// ------------------------------------------------------- CD (composition) --
+// Composition of an explicit half step h1 = h*a[0] in forward variable order
+// and a diagonally implicit half step h2 = h*(1 - a[0]) in reverse order
+// (U-CUDA/codegen.cpp:677-754). Each right-hand side is printed in plain form
+// (names as written, powers as x^n) and classified per variable by the
+// reference's textual rules: absent -> explicit update; linear (no x^n with
+// n >= 2, no x*x, x not inside a known function call) -> the analytic
+// solution of x = x0 + h2*(sign*coef*x + rest); otherwise four fixed-point
+// iterations from the explicit value. The rules work on the text, so the
+// same std::regex patterns are used to reproduce the same statements.
+void print_plain(const P& n, std::ostream& o);
+void print_plain_child(const P& n, std::ostream& o, int min_prec) {
+    if (precedence(n->k) < min_prec) { o << "("; print_plain(n, o); o << ")"; }
+    else print_plain(n, o);
+}
+void print_plain(const P& n, std::ostream& o) {
+    switch (n->k) {
+    case K::Num: o << format_number(n->num); break;
+    case K::Neg: o << "-"; print_plain_child(n->a, o, precedence(K::Neg)); break;
+    case K::Add: print_plain_child(n->a, o, 1); o << " + "; print_plain_child(n->b, o, 1); break;
+    case K::Sub: print_plain_child(n->a, o, 1); o << " - "; print_plain_child(n->b, o, 2); break;
+    case K::Mul: print_plain_child(n->a, o, 2); o << " * "; print_plain_child(n->b, o, 2); break;
+    case K::Div: print_plain_child(n->a, o, 2); o << " / "; print_plain_child(n->b, o, 3); break;
+    case K::Pow: print_plain_child(n->a, o, 4); o << "^"; print_plain_child(n->b, o, 4); break;
+    case K::Call:
+        o << n->name << "(";
+        for (size_t i = 0; i < n->args.size(); ++i) { if (i) o << ", "; print_plain(n->args[i], o); }
+        o << ")";
+        break;
+    default: o << n->name; break;  // unresolved symbol
+    }
+}
+
+std::string trim(const std::string& s) {
+    const size_t a = s.find_first_not_of(" \t\n\r");
+    if (a == std::string::npos) return "";
+    return s.substr(a, s.find_last_not_of(" \t\n\r") - a + 1);
+}
+
+// regex helpers: a pattern that fails to compile counts as "no match"
+bool re_search(const std::string& text, const std::string& pattern, std::smatch* m = nullptr) {
+    try {
+        std::smatch local;
+        return std::regex_search(text, m ? *m : local, std::regex(pattern));
+    } catch (const std::regex_error&) { return false; }
+}
+bool re_full(const std::string& text, const std::string& pattern, std::smatch* m = nullptr) {
+    try {
+        std::smatch local;
+        return std::regex_match(text, m ? *m : local, std::regex(pattern));
+    } catch (const std::regex_error&) { return false; }
+}
+std::string re_replace(const std::string& text, const std::string& pattern, const std::string& with) {
+    try { return std::regex_replace(text, std::regex(pattern), with); } catch (const std::regex_error&) { return text; }
+}
+
+enum class Dependence { None, Linear, Nonlinear };
+
+Dependence cd_dependence(const std::string& expr, const std::string& var) {
+    if (!re_search(expr, "\\b" + var + "\\b")) return Dependence::None;
+    std::smatch m;
+    if (re_search(expr, "\\b" + var + "\\s*\\^\\s*(\\d+)", &m) && std::stoi(m[1].str()) >= 2) return Dependence::Nonlinear;
+    if (re_search(expr, "\\b" + var + "\\s*\\*\\s*" + var + "\\b")) return Dependence::Nonlinear;
+    static const char* const kFunctions[] = {"abs", "acos", "asin", "atan", "cos", "cosh", "exp", "fabs",
+                                             "ln", "log", "pow", "sin", "sinh", "sqrt", "tan", "tanh"};
+    for (const char* f : kFunctions)
+        if (re_search(expr, std::string(f) + "\\s*\\([^)]*\\b" + var + "\\b[^)]*\\)")) return Dependence::Nonlinear;
+    return Dependence::Linear;
+}
+
+// Splits at every '+' / '-' after the first character (parentheses are not
+// tracked), except the sign of a number's exponent.
+std::vector<std::string> cd_terms(const std::string& expr) {
+    std::vector<std::string> out;
+    size_t start = 0;
+    auto push = [&](size_t from, size_t to) {
+        std::string t = trim(expr.substr(from, to - from));
+        if (!t.empty()) out.push_back(t);
+    };
+    for (size_t i = 1; i < expr.size(); ++i) {
+        if ((expr[i] == '+' || expr[i] == '-') && expr[i - 1] != 'e' && expr[i - 1] != 'E') {
+            push(start, i);
+            start = i;
+        }
+    }
+    if (start < expr.size()) push(start, expr.size());
+    return out;
+}
+
+void signed_body(const std::string& token, std::string& sign, std::string& body) {
+    sign = "+";
+    body = token;
+    if (!token.empty() && (token[0] == '-' || token[0] == '+')) {
+        if (token[0] == '-') sign = "-";
+        body = trim(token.substr(1));
+    }
+}
+
+std::string join_rest(const std::vector<std::string>& parts) {
+    std::string r;
+    for (const auto& p : parts) r += (r.empty() ? "" : " ") + p;
+    if (!r.empty() && r[0] == '+') r = trim(r.substr(1));
+    return r;
+}
+
+struct LinearForm { std::string sign = "+", coef = "1", rest; };
+
+// outer * (t1 +- t2 ...): the variable's sign inside, every other term
+// multiplied by `outer`
+LinearForm cd_distribute(const std::string& inner_in, const std::string& var, const std::string& outer) {
+    std::string inner = inner_in;
+    if (!inner.empty() && inner[0] != '+' && inner[0] != '-') inner = "+" + inner;
+    LinearForm f;
+    f.coef = outer;
+    std::vector<std::string> rest;
+    for (const auto& tok : cd_terms(inner)) {
+        std::string sign, body;
+        signed_body(tok, sign, body);
+        if (re_full(body, "^" + var + "$")) { f.sign = sign; continue; }
+        rest.push_back((sign == "-" ? "- " : "+ ") + outer + " * " + body);
+    }
+    f.rest = join_rest(rest);
+    return f;
+}
+
+LinearForm cd_linear_form(const std::string& expr_in, const std::string& var) {
+    const std::string expr = trim(expr_in);
+    std::smatch m;
+    // (left) * (right), the variable only on the right
+    if (re_full(expr, R"(^\(([^)]+)\)\s*\*\s*\(([^)]+)\)$)", &m)) {
+        const std::string left = m[1].str(), right = m[2].str();
+        if (re_search(right, "\\b" + var + "\\b") && !re_search(left, "\\b" + var + "\\b"))
+            return cd_distribute(right, var, "(" + left + ")");
+    }
+    // name * (inner) with the variable inside
+    if (re_full(expr, R"(^([a-zA-Z_]\w*)\s*\*\s*\((.+)\)$)", &m)) {
+        const std::string outer = m[1].str(), inner = m[2].str();
+        if (re_search(inner, "\\b" + var + "\\b")) return cd_distribute(inner, var, outer);
+    }
+    // a sum of terms: the first term coef*var / var*coef / var is the linear one
+    std::string e = expr;
+    if (!e.empty() && e[0] != '+' && e[0] != '-') e = "+" + e;
+    LinearForm f;
+    bool found = false;
+    std::vector<std::string> rest;
+    for (const auto& tok : cd_terms(e)) {
+        std::string sign, body;
+        signed_body(tok, sign, body);
+        bool hit = false;
+        if (re_full(body, "^([a-zA-Z_]\\w*)\\s*\\*\\s*" + var + "$", &m)) { hit = true; f.coef = m[1].str(); }
+        else if (re_full(body, "^" + var + "\\s*\\*\\s*([a-zA-Z_]\\w*)$", &m)) { hit = true; f.coef = m[1].str(); }
+        else if (re_full(body, "^" + var + "$")) { hit = true; f.coef = "1"; }
+        if (hit && !found) { found = true; f.sign = sign; }
+        else rest.push_back(tok);
+    }
+    f.rest = join_rest(rest);
+    return f;
+}
+
+// plain text -> C: x^n -> pow(x, n), ln( -> log(, names -> X[i] / a[1+j]
+std::string cd_code(const std::string& expr, const System& s) {
+    std::string c = expr;
+    for (const auto& v : s.vars) c = re_replace(c, "\\b" + v + "\\s*\\^\\s*(\\d+)", "pow(" + v + ", $1)");
+    c = re_replace(c, R"((X\[\d+\])\s*\^\s*(\d+))", "pow($1, $2)");
+    c = re_replace(c, "\\bln\\s*\\(", "log(");
+    for (size_t i = 0; i < s.vars.size(); ++i) c = re_replace(c, "\\b" + s.vars[i] + "\\b", "X[" + std::to_string(i) + "]");
+    for (size_t j = 0; j < s.params.size(); ++j)
+        c = re_replace(c, "\\b" + s.params[j] + "\\b", "a[" + std::to_string(1 + (int)j) + "]");
+    return c;
+}
+
+std::string krs_cd(const System& s) {
+    if (s.vars.size() != s.rhs.size()) throw std::runtime_error("vars/rhs size mismatch");
+    const int N = (int)s.vars.size();
+    if (N < 2) throw std::runtime_error("CD method requires N >= 2");
+    std::vector<std::string> plain(N);
+    for (int i = 0; i < N; ++i) {
+        std::ostringstream p;
+        print_plain(Parser(s.rhs[i], s.latex).parse(), p);
+        plain[i] = p.str();
+    }
+    std::ostringstream o;
+    o << "    numb h1 = h * a[0];\n    numb h2 = h * (1 - a[0]);\n";
+    for (int i = 0; i < N; ++i) o << "    X[" << i << "] = X[" << i << "] + h1 * (" << cd_code(plain[i], s) << ");\n";
+    for (int i = N - 1; i >= 0; --i) {
+        const std::string x = "X[" + std::to_string(i) + "]";
+        switch (cd_dependence(plain[i], s.vars[i])) {
+        case Dependence::None:
+            o << "    " << x << " = " << x << " + h2 * (" << cd_code(plain[i], s) << ");\n";
+            break;
+        case Dependence::Linear: {
+            // f = sign*coef*x + rest  ->  x = (x + h2*rest) / (1 - sign*h2*coef)
+            const LinearForm f = cd_linear_form(plain[i], s.vars[i]);
+            const std::string coef = f.coef == "1" ? std::string("1") : cd_code(f.coef, s);
+            const std::string den = std::string("(1 ") + (f.sign == "-" ? "+" : "-") + " h2" +
+                                    (coef == "1" ? std::string(")") : " * " + coef + ")");
+            if (f.rest.empty()) o << "    " << x << " = " << x << " / " << den << ";\n";
+            else o << "    " << x << " = (" << x << " + h2 * (" << cd_code(f.rest, s) << ")) / " << den << ";\n";
+            break;
+        }
+        case Dependence::Nonlinear: {
+            const std::string tmp = "x" + std::to_string(i) + "_cd";
+            const std::string code = cd_code(plain[i], s);
+            o << "    numb " << tmp << " = " << x << ";\n";
+            for (int it = 0; it < 4; ++it) o << "    " << x << " = " << tmp << " + h2 * (" << code << ");\n";
+            break;
+        }
+        }
+    }
+    return o.str();
+}
+
 }  // namespace
 
 // ================================================================= public ==
@@ -636,8 +906,8 @@ std::string codegen_scheme(const System& s, Scheme sch) {
     case Scheme::EulerCromer: return krs_euler_cromer(s);
     case Scheme::ExplicitMidpoint: return krs_midpoint(s);
     case Scheme::RK4: return krs_rk4(s);
-    case Scheme::DOPRI78: throw std::runtime_error("DOPRI78 step bodies are not generated by this engine; pass a custom KRS body");
-    case Scheme::CD: throw std::runtime_error("CD step bodies are not generated by this engine; pass a custom KRS body");
+    case Scheme::DOPRI78: return krs_dopri78(s);
+    case Scheme::CD: return krs_cd(s);
     }
     throw std::runtime_error("unknown scheme");
 }

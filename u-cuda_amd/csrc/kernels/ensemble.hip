@@ -430,6 +430,7 @@ __device__ int uc_loop_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T
         if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
         if (uc_wave_all(dead)) break;
         const long long stop = (iters - i > UC_CHECK_INTERVAL) ? i + UC_CHECK_INTERVAL : iters;
+#pragma unroll 2
         for (++i; i < stop; ++i) uc_iter(s, a, h, t, i, pre, wvar, dead, obs);
     }
     if (dead) { s = snap; t = tsnap; return 0; }
@@ -454,6 +455,7 @@ __device__ int uc_loop_tan_u(S& s, typename Prec<S>::T* V, const typename Prec<S
         if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
         if (uc_wave_all(dead)) break;
         const long long stop = (iters - i > UC_CHECK_INTERVAL) ? i + UC_CHECK_INTERVAL : iters;
+#pragma unroll 2
         for (++i; i < stop; ++i) {
             uc_rk4_tan<K>(s, V, a, h, (T)t);
             if (UC_USES_TIME) t += hd;

@@ -21,7 +21,7 @@ UC_OK = 0
 PREC_F64, PREC_F32C = 0, 1
 AXIS_PARAM, AXIS_VAR, AXIS_STEP = 1, 2, 3
 METHOD_VARIATIONAL, METHOD_FINITE_DIFF = 0, 1
-SCHEME_EULER, SCHEME_EULER_CROMER, SCHEME_MIDPOINT, SCHEME_RK4 = 0, 1, 2, 3
+SCHEME_EULER, SCHEME_EULER_CROMER, SCHEME_MIDPOINT, SCHEME_RK4, SCHEME_DOPRI78, SCHEME_CD = 0, 1, 2, 3, 4, 5
 REFERENCE_PROBE_SEED = 1234567891  # U-CUDA/cudaLibrary.cu:2603
 
 
