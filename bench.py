@@ -149,6 +149,67 @@ def strict_check(g, cpu):
                 max_abs_err=float(np.abs(g - cpu["var"]).max()))
 
 
+def extra_bif_config1(eng, threads_all):
+    """The reference's CPU-runnable case, whole: Lorenz bifurcation sweep,
+    sigma = 10, beta = 8/3, 1024 rho values on [0, 250], X0 = (0.1, 0.1, 0.1),
+    RK4 h = 0.01, 1e4 transient + 1e4 recorded steps, maxima of x, <= 256
+    peaks. GPU time from HIP events (strict FP64, fast FP64, FP32C), the
+    FP64 oracle (the reference's CPU algorithm) timed single-threaded on the
+    full sweep, and parity: strict must equal the oracle bit for bit; fast
+    FP64 and FP32C peaks are compared on the cells whose oracle peaks
+    survive rounding-sized noise of that precision."""
+    import numpy as np
+    import torch
+    import ucuda
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    lor = ucuda.preset("lorenz", eng)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 1024, 0.0, 250.0)]
+    args = (lor, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 100.0, 100.0)
+    n = 1024
+    traj_steps = n * (10000 + 1 + 10000 + 1)
+    out, gpu = {}, {}
+    stream = torch.cuda.current_stream()
+    for name, kw in (("f64_strict", dict(strict=True)), ("f64", {}), ("f32c", dict(precision=ucuda.PREC_F32C))):
+        ucuda.run_bif(*args, max_peaks=256, **kw)  # compile + warm-up
+        ms = []
+        for _ in range(3):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            r = ucuda.run_bif(*args, max_peaks=256, stream=stream, sync=False, **kw)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            ms.append(e0.elapsed_time(e1))
+        gpu[name] = {k: v.cpu().numpy() for k, v in r.items() if v is not None}
+        out[name] = dict(ms=float(np.mean(ms)), traj_steps_per_s=traj_steps / (np.mean(ms) / 1e3))
+    step = oracle.KrsStep(eng.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
+    job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 100.0, 100.0, max_peaks=256)
+    t0 = time.perf_counter()
+    ref = oracle.bif_cells(step, job, range(n), threads=1)
+    cpu_s = time.perf_counter() - t0
+    out["cpu_oracle"] = dict(s=cpu_s, traj_steps_per_s=traj_steps / cpu_s, cores=1)
+    g = gpu["f64_strict"]
+    same = np.array_equal(g["flags"], ref["flags"])
+    for c in range(n):
+        m = ref["flags"][c]
+        if m > 0:
+            same &= np.array_equal(g["peaks"][:m, c], ref["peaks"][:m, c])
+            same &= np.array_equal(g["intervals"][:m, c], ref["intervals"][:m, c])
+    out["f64_strict"]["identical_to_oracle"] = bool(same)
+    for name, prec, rel in (("f64", "f64", 1e-8), ("f32c", "f32", 1e-6)):
+        stable = oracle.bif_rounding_stable(step, job, range(n), ref, prec, rel, threads=threads_all)
+        g = gpu[name]
+        errs = []
+        for c in np.nonzero(stable)[0]:
+            m = ref["flags"][c]
+            errs.append(np.inf if g["flags"][c] != m else
+                        float(np.abs(g["peaks"][:m, c] - ref["peaks"][:m, c]).max() / np.abs(ref["peaks"][:m, c]).max()))
+        out[name].update(stable_cells=int(stable.sum()), max_rel_peak_err_stable=max(errs) if errs else None,
+                         ok_le_1e6=bool(errs) and max(errs) <= 1e-6,
+                         flag_class_agreement=float(((g["flags"] <= 0) == (ref["flags"] <= 0)).mean()))
+    return out
+
+
 def extra_ls(eng, n=1024, steps_scale=1.0):
     """hyper-Rossler 4-exponent spectrum maps, h=0.005, Gram-Schmidt every 20
     steps, 1e5 steps, FP32-compensated and FP64:
@@ -478,7 +539,9 @@ def main():
     }
     if not args.no_extra:
         cap = value  # traj-steps/s of the headline kernel sets the serving load
-        progress("LS spectrum map (hyper-Rossler 1024^2) start")
+        progress("Lorenz bifurcation, 1024 rho (the reference's CPU case) start")
+        line["bif_lorenz_1024"] = extra_bif_config1(eng, min(16, os.cpu_count() or 1))
+        progress("Lorenz bifurcation done; LS spectrum map (hyper-Rossler 1024^2) start")
         line["ls_hyper_rossler_1024"] = extra_ls(eng)
         progress("LS spectrum map done; basins (Duffing 2048^2) start")
         line["basins_duffing_2048"] = extra_basins(eng)
