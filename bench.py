@@ -249,12 +249,42 @@ def extra_ls(eng, n=1024, steps_scale=1.0):
             progress("LS %s %s: %.2fs, %d bounded cells" % (case, name, dt, ok))
     a32, a64 = out["attractor_f32c"], out["attractor_f64"]
     out["attractor_f32c_over_f64"] = a64["s"] / a32["s"]
+    # full-size parity and CPU baseline on a fixed subsample of the attractor
+    # map: every 64th point per axis, the whole 1e5-step horizon
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    c = cases["attractor"]
+    strict = ucuda.run_lyap(s, c["axes"], c["ic"], s.values(), 0.005, 0.0, t_max, 0.1, spectrum=True,
+                            strict=True)["values"].cpu().numpy()
+    f32 = ucuda.run_lyap(s, c["axes"], c["ic"], s.values(), 0.005, 0.0, t_max, 0.1, spectrum=True,
+                         precision=ucuda.PREC_F32C)["values"].cpu().numpy()
+    idx = np.arange(0, n, 64)
+    cells = (idx[:, None] * n + idx[None, :]).ravel()
+    funcs = oracle.SystemFuncs(s.source(1))
+    job = oracle.make_job(4, c["axes"], c["ic"], s.values(), 0.005, 0.0, t_max, NT=0.1)
+    t0 = time.perf_counter()
+    r1 = oracle.lyap_var_cells(funcs, 4, job, cells[::4], threads=1)
+    cpu_s = time.perf_counter() - t0
+    bounded1 = int((r1["flags"] == 1).sum())
+    ref = oracle.lyap_var_cells(funcs, 4, job, cells, threads=16)["values"]
+    ok = ref[0] != 999.0
+    progress("LS oracle: %d cells single-thread %.1fs, %d cells on 16 threads" % (len(cells) // 4, cpu_s, len(cells)))
+    out["subsample"] = dict(cells=int(len(cells)), bounded=int(ok.sum()),
+                            strict_f64_max_abs_err=float(np.abs(strict[:, cells][:, ok] - ref[:, ok]).max()),
+                            strict_flags_equal=bool(((strict[0, cells] == 999.0) == ~ok).all()),
+                            f32c_median_abs_err=[float(np.median(np.abs(f32[k, cells][ok] - ref[k, ok])))
+                                                 for k in range(4)],
+                            cpu_oracle=dict(s=cpu_s, cells=int(len(cells) // 4), bounded=bounded1,
+                                            traj_steps_per_s_bounded=bounded1 * spc / cpu_s, cores=1,
+                                            kind="port", method="variational, 4 tangents"))
     return out
 
 
 def extra_basins(eng, n=2048):
     """forced Duffing basins: n x n ICs over [-2.5, 2.5]^2, DP5(4) rtol=atol=1e-9,
     200 forcing periods, Poincare signatures -> int32 labels."""
+    import math
     import numpy as np
     import torch
     import ucuda
@@ -285,6 +315,47 @@ def extra_basins(eng, n=2048):
                 m[a["label"]] = b["label"]
     l32m = np.array([m.get(int(v), 0 if v == 0 else -99) for v in l32])
     out["f32c_label_agreement"] = float((l32m == l64).mean())
+    # full-size parity and CPU baseline on every 32nd point per axis
+    # (64 x 64 = 4096 cells) of the same 2048^2 grid; boundary cells are
+    # those whose FP64 label map changes within 2 grid spacings
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    idx = np.arange(0, n, 32)
+    cells = (idx[:, None] * n + idx[None, :]).ravel()
+    funcs = oracle.SystemFuncs(s.source(1))
+    bj = oracle.make_basins_job(2, axes, [0.0, 0.0], s.values(), 0.01, 1e-9, 1e-9, 2 * math.pi, 200)
+    t0 = time.perf_counter()
+    r1 = oracle.basins_cells(funcs, bj, cells[::4], threads=1)
+    cpu_s = time.perf_counter() - t0
+    ref = oracle.basins_cells(funcs, bj, cells, threads=16)
+    progress("basins oracle: %d cells single-thread %.1fs, %d cells on 16 threads" % (len(cells) // 4, cpu_s, len(cells)))
+    lab2 = l64.reshape(n, n)
+    inner = np.ones_like(lab2, dtype=bool)
+    for dy in range(-2, 3):
+        for dx in range(-2, 3):
+            sh = np.roll(np.roll(lab2, dy, axis=0), dx, axis=1)
+            inner &= sh == lab2
+    inner[:2, :] = inner[-2:, :] = False
+    inner[:, :2] = inner[:, -2:] = False
+    inner = inner.ravel()[cells]
+    per_ref, pt_ref = ref["period"], ref["point"]
+    lab_ref = np.zeros(len(cells), dtype=np.int64)
+    for i in range(len(cells)):
+        p = per_ref[i]
+        if p < 0:
+            continue
+        lab_ref[i] = -99
+        for a in labels["f64"][1]:
+            if a["period"] == p and (p == 0 or (abs(pt_ref[0, i] - a["point"][0]) <= 3e-3 and
+                                                abs(pt_ref[1, i] - a["point"][1]) <= 3e-3)):
+                lab_ref[i] = a["label"]
+                break
+    lg = l64[cells]
+    out["subsample"] = dict(cells=int(len(cells)), interior=int(inner.sum()),
+                            f64_label_agreement_interior=float((lg[inner] == lab_ref[inner]).mean()),
+                            f64_label_agreement_all=float((lg == lab_ref).mean()),
+                            cpu_oracle=dict(s=cpu_s, cells=int(len(cells) // 4), cores=1, kind="port",
+                                            accepted_steps_per_s=float(r1["steps"][0].sum()) / cpu_s))
     return out
 
 
