@@ -35,3 +35,57 @@ def test_facade_matches_c_abi(gpu_engine):
     ls = ucuda.run_lyap(ros, axes4, [1.0, 1.0, 1.0], ros.values(), 0.01, 5.0, 50.0, 0.1, spectrum=True)
     s = ls["values"].cpu().numpy()
     np.testing.assert_allclose(res["ls2d_sum"], s[0] + s[1] + s[2], rtol=0, atol=1e-15)
+
+
+def _rows(path):
+    with open(path) as f:
+        return [line.rstrip("\n") for line in f]
+
+
+def test_facade_csv_files(gpu_engine, tmp_path):
+    print("case: facade_demo writes the reference's CSV files (bif1d, lle2d, ls2d, basins)", flush=True)
+    out = subprocess.run([os.path.join(ROOT, "u-cuda_amd", "facade_demo"), str(tmp_path)], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    g15 = lambda v: "%.15g" % v  # the reference prints with 15 significant digits
+    # bifurcation config: exact lines
+    assert _rows(tmp_path / "bif1d.csv_config.csv") == [
+        "1D classical bifurcation", "Parameter estimation",
+        "a[4] = { 0.5, 10, 28, %s }" % g15(8.0 / 3.0), "X0[3] = { 0.1, 0.1, 0.1 }",
+        "CT = 20", "TT = 20", "h = 0.01", "decimator = 1", "indexVar for peakfinder = 0",
+        "indexPar for estimation = 2", "start value = 0, stop value = 250"]
+    # bifurcation rows: "param, peak, interval" per peak, "param, 0, -1" fixed point, "param, 0, 0" none
+    lor = ucuda.preset("lorenz", gpu_engine)
+    g = ucuda.run_bif(lor, [ucuda.axis(ucuda.AXIS_PARAM, 2, 16, 0.0, 250.0)], [0.1, 0.1, 0.1], lor.values(), 0.01,
+                      20.0, 20.0, max_peaks=64)
+    flags = g["flags"].cpu().numpy()
+    pk, iv = g["peaks"].cpu().numpy(), g["intervals"].cpu().numpy()
+    expect = []
+    for c in range(16):
+        p = g15(0.0 + 250.0 * c / 15)
+        if flags[c] > 0:
+            expect += ["%s, %s, %s" % (p, g15(pk[k, c]), g15(iv[k, c])) for k in range(flags[c])]
+        else:
+            expect.append("%s, 0, %s" % (p, "-1" if flags[c] == -1 else "0"))
+    assert _rows(tmp_path / "bif1d.csv") == expect
+    # LLE 2-D grid: 8 rows of 8 values, row = iy (same numbers as the JSON summary)
+    grid = np.array([[float(x) for x in r.split(", ")] for r in _rows(tmp_path / "lle2d.csv")])
+    assert grid.shape == (8, 8)
+    grid = np.where(np.isnan(grid), 999.0, grid)  # the summary writes diverged cells as 999
+    np.testing.assert_allclose(grid.ravel(), res["lle2d"], rtol=1e-14, atol=0)
+    cfg = _rows(tmp_path / "lle2d.csv_config.csv")
+    assert cfg[:2] == ["2D LLE", "Parameter estimation"] and "n_pts = 8x8" in cfg
+    # LS 2-D: one row per cell with its 3 exponents
+    ls = np.array([[float(x) for x in r.split(", ")] for r in _rows(tmp_path / "ls2d.csv")])
+    assert ls.shape == (16, 3)
+    np.testing.assert_allclose(ls.sum(axis=1), res["ls2d_sum"], rtol=1e-13, atol=1e-13)
+    assert "exponents = 3" in _rows(tmp_path / "ls2d.csv_config.csv")
+    # basins: range lines, then the label grid; _1, _2, _3 and _config alongside
+    b = _rows(tmp_path / "basins.csv")
+    assert b[:2] == ["-2.5 2.5", "-2.5 2.5"]
+    labels = [int(x) for r in b[2:] for x in r.split(", ")]
+    assert labels == res["basins"]
+    for suffix in ("_1.csv", "_2.csv", "_3.csv", "_config.csv"):
+        assert (tmp_path / ("basins.csv" + suffix)).exists(), suffix
+    assert _rows(tmp_path / "basins.csv_config.csv")[0] == "Basins of attraction"

@@ -1,6 +1,7 @@
 // data_export.cpp — CSV writers (see the header for the format source).
 #include "data_export.hpp"
 
+#include <cmath>
 #include <iomanip>
 
 namespace ucuda {
@@ -121,6 +122,31 @@ void grid_int(std::ostream& out, int n_pts, const int* values) {
         }
         out << '\n';
     }
+}
+
+void grid_finite(std::ostream& out, int n_pts, const double* values) {
+    out << std::setprecision(kPrecision);
+    for (int iy = 0; iy < n_pts; ++iy) {
+        for (int ix = 0; ix < n_pts; ++ix) {
+            const double v = values[(size_t)iy * n_pts + ix];
+            out << (std::isfinite(v) ? v : 999.0);
+            if (ix + 1 < n_pts) out << ", ";
+        }
+        out << '\n';
+    }
+}
+
+void basins_config(std::ostream& out, const std::vector<double>& values, const std::vector<double>& ic, double t_max,
+                   double transient, double h, int pre_scaller, double eps, int writable_var, int axis_x_var,
+                   int axis_y_var, double xlo, double xhi, double ylo, double yhi) {
+    out << std::setprecision(kPrecision) << "Basins of attraction\n";
+    values_and_ic(out, values, ic);
+    out << "CT = " << t_max << "\nTT = " << transient << "\nh = " << h << "\n";
+    out << "decimator = " << pre_scaller << "\neps_DBSCAN = " << eps << "\n";
+    out << "indexVar for peakfinder = " << writable_var << "\n";
+    out << "indexVar for estimation = " << axis_x_var << ", " << axis_y_var << "\n";
+    out << "start value_1 = " << xlo << ", stop value_1 = " << xhi << "\n";
+    out << "start value_2 = " << ylo << ", stop value_2 = " << yhi << "\n";
 }
 
 }  // namespace csv

@@ -39,6 +39,11 @@ void ls2d_cells(std::ostream& out, int n_pts, int n_exponents, const double* val
 // basins: "lo hi" range lines then the integer grid.
 void basins_ranges(std::ostream& out, double xlo, double xhi, double ylo, double yhi);
 void grid_int(std::ostream& out, int n_pts, const int* values);
+// n_pts rows of doubles with non-finite values written as 999.
+void grid_finite(std::ostream& out, int n_pts, const double* values);
+void basins_config(std::ostream& out, const std::vector<double>& values, const std::vector<double>& ic, double t_max,
+                   double transient, double h, int pre_scaller, double eps, int writable_var, int axis_x_var,
+                   int axis_y_var, double xlo, double xhi, double ylo, double yhi);
 
 }  // namespace csv
 }  // namespace ucuda

@@ -1,15 +1,19 @@
 // facade_demo.cpp — drives the C++ facade the way the reference's session
 // layer drives ParametricEngine (build a Request, run, read the Result),
 // and prints a JSON summary. Used by the GPU tests to check that the facade
-// and the C ABI give the same numbers.
+// and the C ABI give the same numbers. With a directory argument every run
+// also writes the reference's CSV files there (csv_output_path).
 #include <cmath>
 #include <cstdio>
+#include <string>
 
 #include "parametric_engine.hpp"
 
 using namespace ucuda;
 
-int main() {
+int main(int argc, char** argv) {
+    const std::string dir = argc > 1 ? std::string(argv[1]) + "/" : std::string();
+    auto csv = [&](const char* name) { return dir.empty() ? std::string() : dir + name; };
     ParametricEngine eng(0);
     System lorenz{{"x", "y", "z"}, {"sigma", "rho", "beta"}, {"sigma*(y-x)", "x*(rho-z)-y", "x*y-beta*z"}, false, ""};
     Bifurcation1DRequest b;
@@ -25,6 +29,7 @@ int main() {
     b.transient_time = 20.0;
     b.t_max = 20.0;
     b.max_peaks = 64;
+    b.csv_output_path = csv("bif1d.csv");
     Bifurcation1DResult br = eng.run_bifurcation_1d(b);
     if (!br.ok) { std::printf("{\"error\": \"%s\"}\n", br.error.c_str()); return 1; }
 
@@ -45,13 +50,29 @@ int main() {
     l.transient_time = 5.0;
     l.t_max = 50.0;
     l.NT = 0.1;
+    l.csv_output_path = csv("lle2d.csv");
     LLE2DResult lr = eng.run_lle_2d(l);
     if (!lr.ok) { std::printf("{\"error\": \"%s\"}\n", lr.error.c_str()); return 1; }
 
     LS2DRequest ls = l;
     ls.n_pts = 4;
+    ls.csv_output_path = csv("ls2d.csv");
     LS2DResult lsr = eng.run_ls_2d(ls);
     if (!lsr.ok) { std::printf("{\"error\": \"%s\"}\n", lsr.error.c_str()); return 1; }
+
+    System duffing{{"x", "v"}, {"delta", "gamma", "omega"}, {"v", "-delta*v+x-x^3+gamma*cos(omega*t)"}, false, "t"};
+    BasinsRequest bq;
+    bq.system = duffing;
+    bq.amountOfX = 2;
+    bq.initial_conditions = {0.0, 0.0};
+    bq.base_values = {0.5, 0.15, 0.3, 1.0};
+    bq.n_pts = 16;
+    bq.axis_x_lo = bq.axis_y_lo = -2.5;
+    bq.axis_x_hi = bq.axis_y_hi = 2.5;
+    bq.n_sections = 40;
+    bq.csv_output_path = csv("basins.csv");
+    BasinsResult bres = eng.run_basins(bq);
+    if (!bres.ok) { std::printf("{\"error\": \"%s\"}\n", bres.error.c_str()); return 1; }
 
     std::printf("{\"bif_flags\": [");
     for (size_t i = 0; i < br.flags.size(); ++i) std::printf("%s%d", i ? ", " : "", br.flags[i]);
@@ -67,6 +88,8 @@ int main() {
         for (int k = 0; k < lsr.n_exponents; ++k) s += lsr.values[(size_t)k * n + i];
         std::printf("%s%.17g", i ? ", " : "", s);
     }
+    std::printf("], \"basins\": [");
+    for (size_t i = 0; i < bres.basin_idx.size(); ++i) std::printf("%s%d", i ? ", " : "", bres.basin_idx[i]);
     std::printf("], \"lle_min\": %.17g, \"lle_max\": %.17g}\n", lr.min_val, lr.max_val);
     return 0;
 }

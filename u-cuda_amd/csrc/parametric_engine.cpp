@@ -384,12 +384,21 @@ BasinsResult ParametricEngine::run_basins(const BasinsRequest& req) {
     r.period_tol = 1e-4;
     r.quantum = 1e-3;
     const size_t n = (size_t)req.n_pts * (size_t)req.n_pts;
+    const int lbl = req.system.vars.size() < 2 ? (int)req.system.vars.size() : 2;
     DevBuf<int> labels, periods;
-    if (!labels.alloc(n) || !periods.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    DevBuf<double> points;
+    if (!labels.alloc(n) || !periods.alloc(n) || !points.alloc(n * (size_t)lbl)) { res.error = "hipMalloc failed"; return res; }
     int n_att = 0;
-    uc_basins_output o{labels.p, periods.p, nullptr, nullptr, 0, &n_att, nullptr, nullptr};
+    uc_basins_output o{labels.p, periods.p, points.p, nullptr, 0, &n_att, nullptr, nullptr};
     if (uc_run_basins(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
-    if (!labels.download(res.basin_idx, n) || !periods.download(res.periods, n)) { res.error = "device error"; return res; }
+    std::vector<double> pts;
+    if (!labels.download(res.basin_idx, n) || !periods.download(res.periods, n) || !points.download(pts, n * (size_t)lbl)) {
+        res.error = "device error";
+        return res;
+    }
+    res.cycle_x.assign(pts.begin(), pts.begin() + (std::ptrdiff_t)n);
+    for (size_t i = 0; i < n; ++i)
+        if (res.periods[i] <= 0) res.cycle_x[i] = std::nan("");
     res.helpful_array.resize(n);
     for (size_t i = 0; i < n; ++i) res.helpful_array[i] = res.periods[i] >= 0 ? 1 : 0;
     res.n_clusters = n_att;
@@ -399,12 +408,25 @@ BasinsResult ParametricEngine::run_basins(const BasinsRequest& req) {
     res.axis_y_lo = req.axis_y_lo; res.axis_y_hi = req.axis_y_hi;
     res.axis_x_var = req.axis_x_var; res.axis_y_var = req.axis_y_var;
     if (!req.csv_output_path.empty()) {
-        std::ofstream out(req.csv_output_path);
-        csv::basins_ranges(out, req.axis_x_lo, req.axis_x_hi, req.axis_y_lo, req.axis_y_hi);
-        csv::grid_int(out, req.n_pts, res.basin_idx.data());
-        std::ofstream help(req.csv_output_path + "_3.csv");
-        csv::basins_ranges(help, req.axis_x_lo, req.axis_x_hi, req.axis_y_lo, req.axis_y_hi);
-        csv::grid_int(help, req.n_pts, res.helpful_array.data());
+        // the reference's five files (U-CUDA/data_export.cpp:472-544); its
+        // DBSCAN features (_1 mean peak, _2 mean interval) have no equivalent
+        // here: _1 holds the canonical cycle point's first coordinate and _2
+        // the Poincare period, non-finite / no cycle written as 999
+        std::ofstream cfg(req.csv_output_path + "_config.csv");
+        csv::basins_config(cfg, req.base_values, req.initial_conditions, req.section_period * req.n_sections, 0.0,
+                           req.h, 1, 1e-3, 0, req.axis_x_var, req.axis_y_var, req.axis_x_lo, req.axis_x_hi,
+                           req.axis_y_lo, req.axis_y_hi);
+        auto with_ranges = [&](const std::string& path) {
+            auto f = std::make_unique<std::ofstream>(path);
+            csv::basins_ranges(*f, req.axis_x_lo, req.axis_x_hi, req.axis_y_lo, req.axis_y_hi);
+            return f;
+        };
+        csv::grid_int(*with_ranges(req.csv_output_path), req.n_pts, res.basin_idx.data());
+        csv::grid_finite(*with_ranges(req.csv_output_path + "_1.csv"), req.n_pts, res.cycle_x.data());
+        std::vector<double> per(res.periods.begin(), res.periods.end());
+        for (auto& v : per) v = v > 0 ? v : std::nan("");
+        csv::grid_finite(*with_ranges(req.csv_output_path + "_2.csv"), req.n_pts, per.data());
+        csv::grid_int(*with_ranges(req.csv_output_path + "_3.csv"), req.n_pts, res.helpful_array.data());
     }
     done(req.progress);
     res.ok = true;

@@ -159,6 +159,7 @@ struct BasinsResult {
     std::vector<int> basin_idx;      // >= 1 attractor (canonical order), 0 unbound
     std::vector<int> helpful_array;  // 1 settled on an attractor, 0 unbound / unresolved
     std::vector<int> periods;        // detected Poincare period per cell
+    std::vector<double> cycle_x;     // canonical cycle point, first labelled variable (NaN: none)
     int n_clusters = 0;
     int min_cluster_idx = 0;
 };
