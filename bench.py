@@ -269,12 +269,14 @@ def extra_ls(eng, n=1024, steps_scale=1.0):
     bounded1 = int((r1["flags"] == 1).sum())
     ref = oracle.lyap_var_cells(funcs, 4, job, cells, threads=16)["values"]
     ok = ref[0] != 999.0
+    spread32 = oracle.lyap_rounding_spread(funcs, 4, job, cells, ref, "f32", threads=16)
     progress("LS oracle: %d cells single-thread %.1fs, %d cells on 16 threads" % (len(cells) // 4, cpu_s, len(cells)))
     out["subsample"] = dict(cells=int(len(cells)), bounded=int(ok.sum()),
                             strict_f64_max_abs_err=float(np.abs(strict[:, cells][:, ok] - ref[:, ok]).max()),
                             strict_flags_equal=bool(((strict[0, cells] == 999.0) == ~ok).all()),
                             f32c_median_abs_err=[float(np.median(np.abs(f32[k, cells][ok] - ref[k, ok])))
                                                  for k in range(4)],
+                            fp64_median_spread_under_f32_rounding=[float(np.median(spread32[k, ok])) for k in range(4)],
                             cpu_oracle=dict(s=cpu_s, cells=int(len(cells) // 4), bounded=bounded1,
                                             traj_steps_per_s_bounded=bounded1 * spc / cpu_s, cores=1,
                                             kind="port", method="variational, 4 tangents"))
