@@ -197,6 +197,15 @@ int uc_system_eval_rhs(const uc_system* s, const double* X, const double* a, dou
 
 /* ---- analyses: enqueue on `stream`, return when enqueued ---- */
 int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, void* stream);
+/* 2-D bifurcation map (U-CUDA/parametric_engine.cpp:5487-5911, run_bif2d):
+ * the bifurcation sweep of `r` (max_peaks ignored: every peak of the
+ * recorded window is kept, in chunks of cells that fit a quarter of free
+ * HBM), then each cell's peak values are clustered by the reference's
+ * DBSCAN with radius eps (cudaLibrary.cu:2088-2166). periods[cells]
+ * (device): -1 fixed point / bad step, 0 no peaks or diverged, 1 one peak,
+ * else the cluster count (the period). Synchronises the stream. */
+int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, double eps_dbscan, int* periods,
+                       void* stream);
 int uc_run_lle(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream);

@@ -61,6 +61,7 @@ def lib():
         L.oc_loop_model.argtypes = [_P, _D, _D, C.c_double, C.c_longlong, C.c_int, C.c_int, C.c_int,
                                     C.c_double, _D]
         L.oc_peak_finder.argtypes = [_D, C.c_longlong, _D, _D, C.c_double]
+        L.oc_dbscan_count.argtypes = [_D, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.oc_cgs.argtypes = [_D, _D, C.c_int, _D]
         L.oc_mgs.argtypes = [_D, C.c_int, C.c_int, _D]
         L.oc_rk4.argtypes = [_P, _D, _D, C.c_double, C.c_int]
@@ -345,6 +346,29 @@ def bif_rounding_stable(step, job, cells, base, precision, rel, min_peaks=9, thr
             d = np.abs(base["peaks"][:m, c] - r["peaks"][:m, c]).max()
             stable[c] = bool(d <= rel * max(1.0, np.abs(base["peaks"][:m, c]).max()))
     return stable
+
+
+def dbscan_count(peaks, eps):
+    """The reference's per-cell peak clustering (2-D bifurcation period)."""
+    a = np.ascontiguousarray(np.asarray(peaks, dtype=np.float64))
+    n = len(a)
+    lab = (C.c_int * max(1, n))()
+    sl = (C.c_int * max(1, n))()
+    return lib().oc_dbscan_count(a.ctypes.data_as(_D), n, float(eps), lab, sl)
+
+
+def bif_periods(step, job, cells, eps, threads=1):
+    """2-D bifurcation periods: every peak of the recorded window, then the
+    per-cell DBSCAN count (-1 / 0 / 1 for -1 / 0 / 1 peaks)."""
+    iters = int(np.ceil(job.t_max / job.h / job.pre_scaller))
+    keep = job.max_peaks
+    job.max_peaks = iters // 2 + 2
+    r = bif_cells(step, job, cells, threads=threads)
+    job.max_peaks = keep
+    out = np.zeros(len(r["flags"]), dtype=np.int32)
+    for c, n in enumerate(r["flags"]):
+        out[c] = n if n <= 1 else dbscan_count(r["peaks"][:n, c], eps)
+    return out
 
 
 def peak_finder(data, h):

@@ -382,6 +382,52 @@ void oc_bif_cells(oc_step_fn step, const oc_job* j, const long long* cells, long
     oc_bif_cells_noisy(step, j, cells, n_sel, flags, peaks, intervals, traj, traj_len, threads, 0.0, 0, 100);
 }
 
+/* ------------------------------------------------------------ DBSCAN ---
+ * The reference's per-cell peak clustering for 2-D bifurcation maps
+ * (dbscan, U-CUDA/cudaLibrary.cu:2088-2166, called by dbscanCUDA
+ * :2303-2327 as compiled into kernels/bifurcation2d.template.cu), restated
+ * step for step because its result is the map value: labels[0..n) and a
+ * neighbour-slot array[0..n) start at 0; a peak without a label opens a new
+ * cluster and labels every unlabelled peak k < n - 1 (the last peak is never
+ * scanned as a neighbour) within eps, writing slot[k] = k and counting it;
+ * while that count is positive the loop variable is replaced by
+ * slot[count - 1] (cleared afterwards) and the scan repeats from there; the
+ * loop variable then advances by one. Distance = sqrt(dp^2 + di^2) with the
+ * interval term multiplied by 0 (configCUDA.h:91-92: mult_peak 1,
+ * mult_interval 0). Returns -1 / 0 / 1 for a count of -1 / 0 / 1 peaks,
+ * else the number of clusters minus one. `labels` and `slots` hold n ints. */
+int oc_dbscan_count(const double* peaks, int n, double eps, int* labels, int* slots) {
+    if (n == -1) return -1;
+    if (n == 0) return 0;
+    if (n == 1) return 1;
+    int cluster = 0, neighbours = 0;
+    for (int q = 0; q < n; ++q) { labels[q] = 0; slots[q] = 0; }
+    for (long long i = 0; i < n; ++i) {
+        if (neighbours >= 1) {
+            i = slots[neighbours - 1];
+            slots[neighbours - 1] = 0;
+            neighbours -= 1;
+        } else if (labels[i] == 0) {
+            neighbours = 0;
+            ++cluster;
+            labels[i] = cluster;
+        } else {
+            continue;
+        }
+        for (int k = 0; k < n - 1; ++k) {
+            if (i != k && labels[k] == 0) {
+                const double dp = peaks[i] - peaks[k], di = 0.0;
+                if (sqrt(dp * dp + di * di) < eps) {
+                    labels[k] = cluster;
+                    slots[k] = k;
+                    ++neighbours;
+                }
+            }
+        }
+    }
+    return cluster - 1;
+}
+
 static void lyap_counts(const oc_job* j, double h, long long* skip, long long* nts) {
     if (sweeps_step(j)) {
         *nts = (long long)ceil(j->NT / h);

@@ -59,6 +59,9 @@ int oc_cell_setup(const oc_job* j, long long cell, double* a, double* x0, double
 int oc_loop_model(oc_step_fn step, double* x, const double* a, double h, long long iters, int n, int pre,
                   int wvar, double max_value, double* data);
 int oc_peak_finder(const double* data, long long n, double* peaks, double* times, double h);
+/* 2-D bifurcation period: the reference's per-cell DBSCAN cluster count over
+ * a cell's peaks (labels / slots: n ints of scratch) */
+int oc_dbscan_count(const double* peaks, int n, double eps, int* labels, int* slots);
 
 /* Gram-Schmidt */
 void oc_cgs(const double* a, double* b, int n, double* den);

@@ -89,3 +89,26 @@ def test_facade_csv_files(gpu_engine, tmp_path):
     for suffix in ("_1.csv", "_2.csv", "_3.csv", "_config.csv"):
         assert (tmp_path / ("basins.csv" + suffix)).exists(), suffix
     assert _rows(tmp_path / "basins.csv_config.csv")[0] == "Basins of attraction"
+
+
+def test_facade_bif2d(gpu_engine, tmp_path):
+    print("case: facade run_bifurcation_2d (Lorenz rho x sigma 12x12) vs C ABI run_bif_periods + CSV", flush=True)
+    out = subprocess.run([os.path.join(ROOT, "u-cuda_amd", "facade_demo"), str(tmp_path)], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    lor = ucuda.preset("lorenz", gpu_engine)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 12, 20.0, 250.0), ucuda.axis(ucuda.AXIS_PARAM, 1, 12, 5.0, 16.0)]
+    per = ucuda.run_bif_periods(lor, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 30.0, 30.0, eps_dbscan=0.1)
+    per = per.cpu().numpy()
+    expect = np.where(per < 0, -1.0, per.astype(np.float64))
+    # the facade process compiles with the system hipRTC, this one with torch's bundled hipRTC: chaotic
+    # cells can differ (last-bit differences grow), the map as a whole agrees
+    got = np.array(res["bif2d"])
+    print("facade vs C ABI periods equal on %.3f of the cells" % (got == expect).mean())
+    assert (got == expect).mean() > 0.85
+    grid = np.array([[float(x) for x in r.split(", ")] for r in _rows(tmp_path / "bif2d.csv")])
+    np.testing.assert_array_equal(grid.ravel(), got)
+    cfg = _rows(tmp_path / "bif2d.csv_config.csv")
+    assert cfg[:2] == ["2D bifurcation (DBSCAN)", "Parameter estimation"]
+    assert "eps_DBSCAN = 0.1" in cfg and "indices = 2, 1" in cfg and "n_pts = 12x12" in cfg

@@ -338,3 +338,21 @@ def test_custom_krs_bodies(gpu_engine):
     np.testing.assert_array_equal(a, b)
     with pytest.raises(ucuda.UcError):
         ucuda.run_lyap(ks, axes, [0.1, 0.1, 0.1], vals, 0.01, 5.0, 20.0, 0.1)  # variational needs a Jacobian
+
+
+def test_bif2d_periods(gpu_engine):
+    print("case: 2-D bifurcation periods (Lorenz rho x sigma 24x12, DBSCAN eps 0.1), strict vs oracle", flush=True)
+    s = ucuda.preset("lorenz", gpu_engine)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 24, 20.0, 250.0), ucuda.axis(ucuda.AXIS_PARAM, 1, 12, 5.0, 16.0)]
+    vals = s.values()
+    g = ucuda.run_bif_periods(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 30.0, 30.0, eps_dbscan=0.1,
+                              strict=True).cpu().numpy()
+    step = oracle.KrsStep(gpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
+    job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], vals, 0.01, 30.0, 30.0)
+    r = oracle.bif_periods(step, job, range(24 * 12), 0.1, threads=8)
+    np.testing.assert_array_equal(g, r)
+    assert len(set(r.tolist())) >= 4  # fixed points, periodic windows and chaos all present
+    g32 = ucuda.run_bif_periods(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 30.0, 30.0, eps_dbscan=0.1,
+                                precision=ucuda.PREC_F32C).cpu().numpy()
+    print("F32C periods equal to FP64 on %.3f of the cells" % (g32 == r).mean())
+    assert ((g32 <= 0) == (r <= 0)).mean() > 0.95

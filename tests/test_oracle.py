@@ -329,3 +329,19 @@ def test_dopri78_and_cd_step_bodies(cpu_engine):
         exact = x0 * math.exp(-h * c)
         assert abs(got - exact) < 1e-12 * x0
         assert abs(ref4 - exact) > 1e3 * abs(got - exact)
+
+
+def test_bif2d_dbscan_counts():
+    """2-D bifurcation period = the reference's DBSCAN cluster count of a
+    cell's peaks: a clean period-p peak sequence (p levels further apart than
+    eps) gives p for every length, a constant sequence 1, levels closer than
+    eps merge (also through a chain of eps-neighbours), 0 / 1 peaks give 0 / 1."""
+    for p in range(1, 7):
+        levels = 3.0 + 0.7 * np.arange(p)
+        for n in range(2 * p, 2 * p + 13):
+            assert oracle.dbscan_count(np.tile(levels, 20)[:n], 0.1) == p, (p, n)
+    assert all(oracle.dbscan_count([5.0] * n, 0.1) == 1 for n in range(2, 9))
+    assert oracle.dbscan_count([1.0, 1.05] * 5, 0.1) == 1
+    assert oracle.dbscan_count([1.0, 1.5] * 5, 0.1) == 2
+    assert oracle.dbscan_count([0.0, 0.06, 0.12] * 3, 0.1) == 1
+    assert oracle.dbscan_count([], 0.1) == 0 and oracle.dbscan_count([2.0], 0.1) == 1

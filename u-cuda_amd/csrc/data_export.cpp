@@ -87,6 +87,23 @@ void map2d_config(std::ostream& out, const char* title, int par_or_var, const st
     if (n_exponents > 0) out << "exponents = " << n_exponents << "\n";
 }
 
+void bif2d_config(std::ostream& out, int par_or_var, const std::vector<double>& values, const std::vector<double>& ic,
+                  double t_max, double transient, double h, int pre_scaller, double eps_dbscan, int writable_var,
+                  int index1, int index2, double lo1, double hi1, double lo2, double hi2, int n_pts) {
+    out << std::setprecision(kPrecision) << "2D bifurcation (DBSCAN)\n";
+    if (par_or_var == 1) out << "Parameter estimation\n";
+    else if (par_or_var == 0) out << "Initial conditions estimation\n";
+    else if (par_or_var == 2) out << "Mixed: x=IC, y=parameter\n";
+    values_and_ic(out, values, ic);
+    out << "CT = " << t_max << "\nTT = " << transient << "\nh = " << h << "\ndecimator = " << pre_scaller << "\n";
+    out << "eps_DBSCAN = " << eps_dbscan << "\n";
+    out << "indexVar for peakfinder = " << writable_var << "\n";
+    out << "indices = " << index1 << ", " << index2 << "\n";
+    out << "axis1: " << lo1 << " .. " << hi1 << "\n";
+    out << "axis2: " << lo2 << " .. " << hi2 << "\n";
+    out << "n_pts = " << n_pts << "x" << n_pts << "\n";
+}
+
 void grid(std::ostream& out, int n_pts, const double* values) {
     out << std::setprecision(kPrecision);
     for (int iy = 0; iy < n_pts; ++iy) {

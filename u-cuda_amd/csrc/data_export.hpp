@@ -32,6 +32,9 @@ void ls1d_row(std::ostream& out, double param, const double* exponents, int n_ex
 void map2d_config(std::ostream& out, const char* title, int par_or_var, const std::vector<double>& values,
                   const std::vector<double>& ic, double t_max, double NT, double transient, double h, double eps,
                   int index1, int index2, double lo1, double hi1, double lo2, double hi2, int n_pts, int n_exponents);
+void bif2d_config(std::ostream& out, int par_or_var, const std::vector<double>& values, const std::vector<double>& ic,
+                  double t_max, double transient, double h, int pre_scaller, double eps_dbscan, int writable_var,
+                  int index1, int index2, double lo1, double hi1, double lo2, double hi2, int n_pts);
 // n_pts rows of n_pts values (values[iy * n + ix]).
 void grid(std::ostream& out, int n_pts, const double* values);
 // one row per cell (iy * n + ix) with its n_exponents values (plane-major input).

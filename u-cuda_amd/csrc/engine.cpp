@@ -805,6 +805,80 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
     return UC_OK;
 }
 
+// 2-D bifurcation periods (run_bif2d, U-CUDA/parametric_engine.cpp:5487-5911):
+// the bifurcation sweep keeping every peak of the recorded window (the
+// reference clusters all of them), then the per-cell DBSCAN kernel. Cells
+// run in chunks sized so that the peaks and the clustering scratch fit in a
+// quarter of the free HBM.
+int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, double eps, int* periods, void* stream) {
+    if (!e || !s || !r) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (!periods) return e->fail(UC_ERR_ARGUMENT, "periods output missing");
+    if (!(eps > 0.0)) return e->fail(UC_ERR_ARGUMENT, "eps_dbscan must be > 0");
+    uc_bif_request rq = *r;
+    rq.max_peaks = 0;
+    int dflag = 0;
+    uc_bif_output probe{&dflag, nullptr, nullptr, nullptr, 0, nullptr};
+    long long iters = 0;
+    int rc = prep_bif(e, s, &rq, &probe, &iters);
+    if (rc) return rc;
+    if ((rc = enter(e))) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
+    fill_bif(&rq, &probe, iters, j);
+    // raw peaks of `iters` samples: candidates are >= 2 apart
+    const long long cap = iters / 2 + 2;
+    if (cap > 0x7fffffffLL) return e->fail(UC_ERR_ARGUMENT, "recorded window too long");
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    const size_t per_cell = (size_t)cap * (sizeof(double) + 2 * sizeof(int)) + sizeof(int);
+    const long long total = j.n_cells;
+    long long chunk = (long long)std::min<size_t>((size_t)total, std::max<size_t>(kBlock, free_b / 4 / per_cell));
+    double* peaks = nullptr;
+    int *flags = nullptr, *labels = nullptr, *slots = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(st);
+        for (void* p : {(void*)peaks, (void*)flags, (void*)labels, (void*)slots})
+            if (p) (void)hipFree(p);
+    };
+    hipError_t he = hipMalloc((void**)&peaks, sizeof(double) * (size_t)cap * (size_t)chunk);
+    if (he == hipSuccess) he = hipMalloc((void**)&flags, sizeof(int) * (size_t)chunk);
+    if (he == hipSuccess) he = hipMalloc((void**)&labels, sizeof(int) * (size_t)cap * (size_t)chunk);
+    if (he == hipSuccess) he = hipMalloc((void**)&slots, sizeof(int) * (size_t)cap * (size_t)chunk);
+    if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(bif periods)")); }
+    hipFunction_t fdb;
+    he = hipModuleGetFunction(&fdb, m->mod, "uc_bif_dbscan");
+    if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan")); }
+    const char* kname = r->c.precision == UC_PREC_F64 ? "uc_bif_f64" : "uc_bif_f32c";
+    for (long long off = 0; off < total; off += chunk) {
+        const long long mcells = std::min(chunk, total - off);
+        UcJob jc = j;
+        jc.cell_offset = off;
+        jc.n_cells = mcells;
+        jc.stride = mcells;
+        jc.flag = flags;
+        jc.out0 = peaks;
+        jc.out1 = nullptr;
+        jc.max_peaks = (int)cap;
+        jc.traj = nullptr;
+        jc.traj_len = 0;
+        jc.final_state = nullptr;
+        if ((rc = launch(e, m, kname, jc, st, tb))) { release(); return rc; }
+        int* out = periods + off;
+        long long stride = mcells, count = mcells;
+        void* args[] = {&peaks, &flags, &stride, &count, &eps, &labels, &slots, &out};
+        he = hipModuleLaunchKernel(fdb, (unsigned)((mcells + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, args,
+                                   nullptr);
+        if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan launch")); }
+    }
+    release();
+    return UC_OK;
+}
+
 int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_samples, long long n_series,
                    double time_step, int max_peaks, int* counts, double* peaks, double* intervals, void* stream) {
     if (!e || !s) return UC_ERR_ARGUMENT;

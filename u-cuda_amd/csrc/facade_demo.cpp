@@ -60,6 +60,26 @@ int main(int argc, char** argv) {
     LS2DResult lsr = eng.run_ls_2d(ls);
     if (!lsr.ok) { std::printf("{\"error\": \"%s\"}\n", lsr.error.c_str()); return 1; }
 
+    Bifurcation2DRequest b2;
+    b2.system = lorenz;
+    b2.amountOfX = 3;
+    b2.initial_conditions = {0.1, 0.1, 0.1};
+    b2.base_values = {0.5, 10.0, 28.0, 8.0 / 3.0};
+    b2.param_index = 2;
+    b2.param_lo = 20.0;
+    b2.param_hi = 250.0;
+    b2.param_index_2 = 1;
+    b2.param_lo_2 = 5.0;
+    b2.param_hi_2 = 16.0;
+    b2.n_pts = 12;
+    b2.h = 0.01;
+    b2.transient_time = 30.0;
+    b2.t_max = 30.0;
+    b2.eps_dbscan = 0.1;
+    b2.csv_output_path = csv("bif2d.csv");
+    Bifurcation2DResult b2r = eng.run_bifurcation_2d(b2);
+    if (!b2r.ok) { std::printf("{\"error\": \"%s\"}\n", b2r.error.c_str()); return 1; }
+
     System duffing{{"x", "v"}, {"delta", "gamma", "omega"}, {"v", "-delta*v+x-x^3+gamma*cos(omega*t)"}, false, "t"};
     BasinsRequest bq;
     bq.system = duffing;
@@ -88,6 +108,8 @@ int main(int argc, char** argv) {
         for (int k = 0; k < lsr.n_exponents; ++k) s += lsr.values[(size_t)k * n + i];
         std::printf("%s%.17g", i ? ", " : "", s);
     }
+    std::printf("], \"bif2d\": [");
+    for (size_t i = 0; i < b2r.values.size(); ++i) std::printf("%s%.17g", i ? ", " : "", b2r.values[i]);
     std::printf("], \"basins\": [");
     for (size_t i = 0; i < bres.basin_idx.size(); ++i) std::printf("%s%d", i ? ", " : "", bres.basin_idx[i]);
     std::printf("], \"lle_min\": %.17g, \"lle_max\": %.17g}\n", lr.min_val, lr.max_val);

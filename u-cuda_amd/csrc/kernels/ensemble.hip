@@ -501,7 +501,7 @@ struct PeakObserver {
         const double tpk = (double)(j - 1) + delta;
         if (raw >= 1 && raw - 1 < max_peaks) {
             peaks[(long long)(raw - 1) * stride] = value;
-            intervals[(long long)(raw - 1) * stride] = (tpk - last_t) * time_step;
+            if (intervals) intervals[(long long)(raw - 1) * stride] = (tpk - last_t) * time_step;
         }
         last_t = tpk;
         ++raw;
@@ -565,7 +565,7 @@ __device__ void uc_bif_cell(const UcJob& j, long long lane) {
     PeakObserver obs;
     obs.w0 = 0.0; obs.w1 = 0.0; obs.n = iters; obs.c_next = 2; obs.mode = 0; obs.raw = 0;
     obs.last_t = 0.0; obs.time_step = hd * (double)j.pre_scaller; obs.max_peaks = j.max_peaks;
-    obs.stride = j.stride; obs.peaks = j.out0 + lane; obs.intervals = j.out1 + lane;
+    obs.stride = j.stride; obs.peaks = j.out0 + lane; obs.intervals = j.out1 ? j.out1 + lane : nullptr;
     obs.traj = j.traj ? j.traj + lane : nullptr; obs.traj_len = j.traj ? j.traj_len : 0;
     if (flag == 1 || flag == -1)
         flag = uni ? uc_loop_u(s, a, h, t, j.n_iters, j.pre_scaller, j.writable_var, j.max_value, obs)
@@ -912,6 +912,53 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_peaks_series(const dou
             if (lane < jobs[lo].n_cells) CELL(jobs[lo], lane);                                      \
         }                                                                                           \
     }
+
+// Period of a 2-D bifurcation cell: the reference's DBSCAN over the cell's
+// peak values (dbscan, U-CUDA/cudaLibrary.cu:2088-2166, as compiled into its
+// 2-D bifurcation template), one lane per cell, the same steps in the same
+// order because the cluster count is the map value. labels / slots start at
+// 0; an unlabelled peak opens a cluster; every unlabelled peak k < n - 1
+// within eps (|dp|, the interval term being multiplied by 0) joins it with
+// slot[k] = k; while the neighbour count is positive the loop variable is
+// replaced by slot[count - 1], which is cleared, and the scan repeats from
+// there. Result: -1 / 0 / 1 for -1 / 0 / 1 peaks, else clusters - 1.
+// Peaks and scratch are peak-major ([k * stride + cell]).
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan(const double* __restrict__ peaks,
+                                                                       const int* __restrict__ counts, long long stride,
+                                                                       long long n_cells, double eps, int* labels,
+                                                                       int* slots, int* out) {
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_cells) return;
+    const int n = counts[c];
+    if (n <= 1) { out[c] = n; return; }
+    int cluster = 0, neighbours = 0;
+    for (int q = 0; q < n; ++q) { labels[(long long)q * stride + c] = 0; slots[(long long)q * stride + c] = 0; }
+    for (long long i = 0; i < n; ++i) {
+        if (neighbours >= 1) {
+            i = slots[(long long)(neighbours - 1) * stride + c];
+            slots[(long long)(neighbours - 1) * stride + c] = 0;
+            neighbours -= 1;
+        } else if (labels[i * stride + c] == 0) {
+            neighbours = 0;
+            ++cluster;
+            labels[i * stride + c] = cluster;
+        } else {
+            continue;
+        }
+        const double pi = peaks[i * stride + c];
+        for (int k = 0; k < n - 1; ++k) {
+            if (i != k && labels[(long long)k * stride + c] == 0) {
+                const double dp = pi - peaks[(long long)k * stride + c], di = 0.0;
+                if (sqrt(dp * dp + di * di) < eps) {
+                    labels[(long long)k * stride + c] = cluster;
+                    slots[(long long)k * stride + c] = k;
+                    ++neighbours;
+                }
+            }
+        }
+    }
+    out[c] = cluster - 1;
+}
 
 // Zeroes the work counters of a batch (a kernel, not a memset, so the reset
 // can live inside a captured graph).

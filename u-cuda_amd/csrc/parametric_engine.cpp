@@ -258,7 +258,8 @@ LS1DResult ParametricEngine::run_ls_1d(const LS1DRequest& req) {
 }
 
 namespace {
-void axes_2d(const LLE2DRequest& req, uc_common& c) {
+template <class Req>
+void axes_2d(const Req& req, uc_common& c) {
     c.n_axes = 2;
     c.axis[0] = make_axis(req.sweep_over_h, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts,
                           req.param_lo, req.param_hi, req.log_scale);
@@ -276,6 +277,58 @@ void min_max(const double* v, const int* f, size_t n, double& lo, double& hi) {
     }
 }
 }  // namespace
+
+Bifurcation2DResult ParametricEngine::run_bifurcation_2d(const Bifurcation2DRequest& req) {
+    Bifurcation2DResult res;
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    if (!impl_->eng) { res.error = "engine not initialised"; return res; }
+    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
+    uc_system* s = impl_->system(req.system, res.error);
+    if (!s) return res;
+    uc_bif_request r{};
+    impl_->common(r.c, req.initial_conditions, req.base_values, req.h, req.transient_time, req.t_max, req.max_value,
+                  req.precision);
+    axes_2d(req, r.c);
+    r.writable_var = req.writable_var;
+    r.pre_scaller = req.pre_scaller;
+    const size_t n = (size_t)req.n_pts * (size_t)req.n_pts;
+    DevBuf<int> periods;
+    if (!periods.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    if (uc_run_bif_periods(impl_->eng, s, &r, req.eps_dbscan, periods.p, nullptr) != UC_OK) {
+        res.error = impl_->err();
+        return res;
+    }
+    std::vector<int> per;
+    if (!periods.download(per, n)) { res.error = "device error"; return res; }
+    res.values.resize(n);
+    res.flags.resize(n);
+    double lo = std::numeric_limits<double>::infinity(), hi = -lo;
+    for (size_t i = 0; i < n; ++i) {
+        res.values[i] = per[i] < 0 ? -1.0 : (double)per[i];
+        res.flags[i] = per[i] < 0 ? -1 : 1;
+        // colour range over flagged-ok, non-negative cells (run_bif2d, parametric_engine.cpp:5897-5906)
+        if (res.flags[i] >= 0 && res.values[i] >= 0.0) { lo = std::min(lo, res.values[i]); hi = std::max(hi, res.values[i]); }
+    }
+    res.min_val = std::isfinite(lo) ? lo : 0.0;
+    res.max_val = std::isfinite(hi) ? hi : 0.0;
+    res.n_pts = req.n_pts;
+    res.param_lo = req.param_lo; res.param_hi = req.param_hi;
+    res.param_lo_2 = req.param_lo_2; res.param_hi_2 = req.param_hi_2;
+    if (!req.csv_output_path.empty()) {
+        std::ofstream cfg(req.csv_output_path + "_config.csv");
+        const int pov = (req.sweep_over_var == req.sweep_over_var_2) ? (req.sweep_over_var ? 0 : 1) : 2;
+        csv::bif2d_config(cfg, pov, req.base_values, req.initial_conditions, req.t_max, req.transient_time, req.h,
+                          req.pre_scaller, req.eps_dbscan, req.writable_var,
+                          req.sweep_over_var ? req.var_sweep_index : req.param_index,
+                          req.sweep_over_var_2 ? req.var_sweep_index_2 : req.param_index_2, req.param_lo, req.param_hi,
+                          req.param_lo_2, req.param_hi_2, req.n_pts);
+        std::ofstream out(req.csv_output_path);
+        csv::grid(out, req.n_pts, res.values.data());
+    }
+    done(req.progress);
+    res.ok = true;
+    return res;
+}
 
 LLE2DResult ParametricEngine::run_lle_2d(const LLE2DRequest& req) {
     LLE2DResult res;

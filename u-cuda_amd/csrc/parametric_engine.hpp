@@ -57,6 +57,40 @@ struct Bifurcation1DResult {
     std::vector<int> flags;
 };
 
+// 2-D bifurcation map: period = DBSCAN cluster count of each cell's peaks
+// (U-CUDA/parametric_engine.h:499-569).
+struct Bifurcation2DRequest {
+    System system;
+    std::string krs_body;  // informational: the engine integrates `system` with RK4
+    int amountOfX = 0;
+    std::vector<double> initial_conditions, base_values;
+    bool sweep_over_var = false, sweep_over_h = false, log_scale = false;
+    int var_sweep_index = 0, param_index = 0;
+    bool sweep_over_var_2 = false, sweep_over_h_2 = false, log_scale_2 = false;
+    int var_sweep_index_2 = 0, param_index_2 = 0;
+    double param_lo = 0.0, param_hi = 1.0, param_lo_2 = 0.0, param_hi_2 = 1.0;
+    int n_pts = 200;
+    int writable_var = 0;
+    double h = 0.01, transient_time = 0.0, t_max = 100.0;
+    int pre_scaller = 1;
+    double max_value = 1.0e6;
+    double eps_dbscan = 0.1;
+    int precision = 0;
+    std::string csv_output_path;  // non-empty: write <path> and <path>_config.csv in the reference format
+    std::shared_ptr<std::atomic<bool>> cancel;
+    std::shared_ptr<std::atomic<float>> progress;
+};
+
+struct Bifurcation2DResult {
+    bool ok = false, cancelled = false;
+    std::string error;
+    int n_pts = 0;
+    double param_lo = 0.0, param_hi = 1.0, param_lo_2 = 0.0, param_hi_2 = 1.0;
+    std::vector<double> values;  // [iy * n + ix] period; -1 fixed point / bad step, 0 no peaks / diverged
+    std::vector<int> flags;      // 1 ok, -1 when the period is negative
+    double min_val = 0.0, max_val = 0.0;
+};
+
 struct LLE1DRequest {
     System system;
     int amountOfX = 0;
@@ -172,6 +206,7 @@ public:
     ParametricEngine& operator=(const ParametricEngine&) = delete;
 
     Bifurcation1DResult run_bifurcation_1d(const Bifurcation1DRequest& req);
+    Bifurcation2DResult run_bifurcation_2d(const Bifurcation2DRequest& req);
     LLE1DResult run_lle_1d(const LLE1DRequest& req);
     LS1DResult run_ls_1d(const LS1DRequest& req);
     LLE2DResult run_lle_2d(const LLE2DRequest& req);

@@ -118,6 +118,8 @@ _SIGNATURES = {
     "uc_run_ls": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.POINTER(LyapOutput), C.c_void_p]),
     "uc_run_basins": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BasinsRequest), C.POINTER(BasinsOutput),
                                 C.c_void_p]),
+    "uc_run_bif_periods": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.c_double, C.c_void_p,
+                                     C.c_void_p]),
     "uc_peak_finder": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong, C.c_double,
                                  C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "uc_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -407,6 +409,25 @@ def run_bif(sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pr
         torch.cuda.synchronize(dev)
     del keep
     return out
+
+
+def run_bif_periods(sysm, axes, ic, values, h, transient_time, t_max, eps_dbscan=0.1, writable_var=0,
+                    pre_scaller=1, max_value=1e6, precision=PREC_F64, strict=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0,
+                    stream=None):
+    """2-D bifurcation map: periods[cells] (int32 CUDA tensor) = DBSCAN cluster
+    count of each cell's peaks (-1 fixed point, 0 none / diverged)."""
+    import torch
+    eng = sysm.engine
+    n = n_cells(axes)
+    periods = torch.empty(n, dtype=torch.int32, device=torch.device("cuda", eng.device))
+    r = BifRequest()
+    r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
+                        ic_seed, ic_lo, ic_hi)
+    r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, 0
+    eng.check(lib().uc_run_bif_periods(eng.h, sysm.h, C.byref(r), float(eps_dbscan), _ptr(periods).value,
+                                       _stream(stream)), "uc_run_bif_periods")
+    del keep
+    return periods
 
 
 def run_lyap(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=False, eps=1e-4,
