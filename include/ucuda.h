@@ -206,6 +206,31 @@ int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif
  * else the cluster count (the period). Synchronises the stream. */
 int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, double eps_dbscan, int* periods,
                        void* stream);
+
+/* 1-D DFT sweep (U-CUDA/parametric_engine.h:140-226, run_dft1d_classical
+ * parametric_engine.cpp:4809-5120): per cell a transient of
+ * (int)(transient_time / h) steps, a block of N = (int)(t_max / h / pre)
+ * samples of the sampled variable, a window, and the coefficients
+ * a_k + i b_k = (1/N) sum_n w_n x_n (cos, sin)(2 pi (h pre) f_k n) at n_freq
+ * frequencies (DFT_custom, cudaLibrary.cu:1698-1777, including its
+ * float-precision phase resync every 1000 samples). Axes may sweep
+ * parameters or initial conditions (not h). Cells run in chunks whose blocks
+ * fit a quarter of free HBM; the call synchronises the stream. */
+typedef struct uc_dft_request {
+    uc_common c;
+    int writable_var;    /* sampled variable (-1: x0 + pi x1 + e x2)            */
+    int pre_scaller;     /* steps per sample                                      */
+    int n_freq;          /* >= 2                                                  */
+    double freq_lo, freq_hi;
+    int freq_log_scale;  /* 1: log-spaced frequencies (freq_lo, freq_hi > 0)      */
+    int window_type;     /* 0 none, 1 Hanning, 2 Hamming                          */
+} uc_dft_request;
+typedef struct uc_dft_output {
+    double* ak_cos;      /* [cells][n_freq] device; -1 / 0 for flagged blocks     */
+    double* bk_sin;
+    int* flags;          /* [cells] 1 ok, -1 fixed point / bad step, 0 diverged   */
+} uc_dft_output;
+int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft_output* o, void* stream);
 int uc_run_lle(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream);

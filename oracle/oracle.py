@@ -62,6 +62,10 @@ def lib():
                                     C.c_double, _D]
         L.oc_peak_finder.argtypes = [_D, C.c_longlong, _D, _D, C.c_double]
         L.oc_dbscan_count.argtypes = [_D, C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.oc_dft_window.argtypes = [C.c_int, C.c_int, _D]
+        L.oc_dft_block.argtypes = [_D, C.c_int, _D, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _D, _D]
+        L.oc_dft_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, C.c_int, C.c_double, C.c_double, C.c_int,
+                                   C.c_int, _D, _D, _I, C.c_int]
         L.oc_cgs.argtypes = [_D, _D, C.c_int, _D]
         L.oc_mgs.argtypes = [_D, C.c_int, C.c_int, _D]
         L.oc_rk4.argtypes = [_P, _D, _D, C.c_double, C.c_int]
@@ -369,6 +373,33 @@ def bif_periods(step, job, cells, eps, threads=1):
     for c, n in enumerate(r["flags"]):
         out[c] = n if n <= 1 else dbscan_count(r["peaks"][:n, c], eps)
     return out
+
+
+def dft_window(window_type, N):
+    w = np.zeros(max(1, N))
+    lib().oc_dft_window(int(window_type), int(N), w.ctypes.data_as(_D))
+    return w[:N]
+
+
+def dft_block(x, w, n_freq, f_lo, f_hi, dt, log_axis=False):
+    x = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    w = np.ascontiguousarray(np.asarray(w, dtype=np.float64))
+    ak, bk = np.zeros(n_freq), np.zeros(n_freq)
+    lib().oc_dft_block(x.ctypes.data_as(_D), len(x), w.ctypes.data_as(_D), int(n_freq), float(f_lo), float(f_hi),
+                       int(log_axis), float(dt), ak.ctypes.data_as(_D), bk.ctypes.data_as(_D))
+    return ak, bk
+
+
+def dft_cells(step, job, cells, n_freq, f_lo, f_hi, log_axis=False, window_type=1, threads=1):
+    """1-D DFT sweep: ak, bk [cells, n_freq] and the block's loop flag."""
+    cells, cp = _cells(cells)
+    n = len(cells)
+    ak, bk = np.zeros((n, n_freq)), np.zeros((n, n_freq))
+    flags = np.zeros(n, dtype=np.int32)
+    lib().oc_dft_cells(step.ptr, C.byref(job), cp, n, int(n_freq), float(f_lo), float(f_hi), int(log_axis),
+                       int(window_type), ak.ctypes.data_as(_D), bk.ctypes.data_as(_D), flags.ctypes.data_as(_I),
+                       int(threads))
+    return dict(ak=ak, bk=bk, flags=flags)
 
 
 def peak_finder(data, h):

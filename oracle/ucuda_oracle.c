@@ -428,6 +428,95 @@ int oc_dbscan_count(const double* peaks, int n, double eps, int* labels, int* sl
     return cluster - 1;
 }
 
+/* --------------------------------------------------------------- DFT ---
+ * 1-D DFT sweep (run_dft1d_classical, U-CUDA/parametric_engine.cpp:4809-5120;
+ * kernels calculateDiscreteModelCUDA + DFT_custom, cudaLibrary.cu:1698-1777):
+ * transient of (int)(transient / h) steps (with the loop's trailing pre
+ * steps), a block of (int)(t_max / h / pre) samples of the sampled
+ * variable; checker = the block's loop flag: -1 -> every a_k, b_k = -1,
+ * 0 -> 0, else for each frequency f_k (linear, or the log axis formula)
+ * a_k + i b_k = (1/N) sum_n w_n x_n (cos, sin)(2 pi dt f_k n) with dt =
+ * h * pre, the phase advanced by a rotation and resynchronised every 1000
+ * samples from cosf / sinf of the exact angle (float precision, as the
+ * kernel's cosf). Window (build_window, :4787-4801): 0 none, 1 Hanning
+ * 0.5 (1 - cos(2 pi n / (N - 1))), 2 Hamming 0.53836 - 0.46164 cos(...). */
+void oc_dft_window(int window_type, int N, double* w) {
+    if (window_type == 0) { for (int n = 0; n < N; ++n) w[n] = 1.0; return; }
+    const double gamma = 2.0 * OC_PI / (double)(N - 1);
+    for (int n = 0; n < N; ++n)
+        w[n] = window_type == 2 ? 0.53836 - 0.46164 * cos(gamma * (double)n) : 0.5 * (1.0 - cos(gamma * (double)n));
+}
+
+void oc_dft_block(const double* x, int N, const double* w, int n_freq, double f_lo, double f_hi, int log_axis,
+                  double dt, double* ak, double* bk) {
+    const double f_step = (f_hi - f_lo) / (double)(n_freq - 1);
+    const double psi = 2.0 * OC_PI * dt;
+    for (int k = 0; k < n_freq; ++k) {
+        double f_k;
+        if (log_axis) {
+            const double l0 = log10(f_lo), l1 = log10(f_hi);
+            f_k = pow(10.0, l0 + (double)k * ((l1 - l0) / (double)(n_freq - 1)));
+        } else {
+            f_k = f_lo + (double)k * f_step;
+        }
+        const double c_t = cos(2.0 * OC_PI * dt * f_k), s_t = sin(2.0 * OC_PI * dt * f_k);
+        double a = 0.0, b = 0.0, c_n = 1.0, s_n = 0.0;
+        int reset = 1000;
+        for (int n = 0; n < N; ++n) {
+            const double wd = w[n] * x[n];
+            a = a + wd * c_n;
+            b = b + wd * s_n;
+            const double nc = c_n * c_t - s_n * s_t;
+            const double ns = s_n * c_t + c_n * s_t;
+            c_n = nc;
+            s_n = ns;
+            if (--reset == 0) {
+                reset = 1000;
+                if (n + 1 < N) {
+                    const double exact = psi * f_k * ((double)n + 1.0);
+                    c_n = (double)cosf((float)exact);
+                    s_n = (double)sinf((float)exact);
+                }
+            }
+        }
+        ak[k] = a / (double)N;
+        bk[k] = b / (double)N;
+    }
+}
+
+void oc_dft_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int n_freq, double f_lo,
+                  double f_hi, int log_axis, int window_type, double* ak, double* bk, int* flags, int threads) {
+    const long long N = (long long)(j->t_max / j->h / (double)j->pre_scaller);
+    double* w = (double*)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+    oc_dft_window(window_type, (int)N, w);
+#pragma omp parallel num_threads(threads > 1 ? threads : 1)
+    {
+        double* data = (double*)malloc(sizeof(double) * (size_t)(N > 0 ? N : 1));
+#pragma omp for schedule(dynamic, 1)
+        for (long long s = 0; s < n_sel; ++s) {
+            double a[OC_MAXV], x[OC_MAXN], h;
+            double* A = ak + s * n_freq;
+            double* B = bk + s * n_freq;
+            int flag = -1;
+            if (oc_cell_setup(j, cells[s], a, x, &h)) {
+                const long long skip = (long long)(j->transient_time / j->h);
+                flag = oc_loop_model(step, x, a, h, skip, j->n_vars, j->pre_scaller, 0, j->max_value, NULL);
+                if (flag == 1 || flag == -1)
+                    flag = oc_loop_model(step, x, a, h, N, j->n_vars, j->pre_scaller, j->writable_var, j->max_value,
+                                         data);
+            }
+            flags[s] = flag;
+            if (flag == -1 || flag == 0) {
+                for (int k = 0; k < n_freq; ++k) A[k] = B[k] = (double)flag;
+                continue;
+            }
+            oc_dft_block(data, (int)N, w, n_freq, f_lo, f_hi, log_axis, h * (double)j->pre_scaller, A, B);
+        }
+        free(data);
+    }
+    free(w);
+}
+
 static void lyap_counts(const oc_job* j, double h, long long* skip, long long* nts) {
     if (sweeps_step(j)) {
         *nts = (long long)ceil(j->NT / h);

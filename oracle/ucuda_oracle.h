@@ -62,6 +62,12 @@ int oc_peak_finder(const double* data, long long n, double* peaks, double* times
 /* 2-D bifurcation period: the reference's per-cell DBSCAN cluster count over
  * a cell's peaks (labels / slots: n ints of scratch) */
 int oc_dbscan_count(const double* peaks, int n, double eps, int* labels, int* slots);
+/* 1-D DFT sweep (window, one block, whole cells: ak / bk [n_sel][n_freq]) */
+void oc_dft_window(int window_type, int N, double* w);
+void oc_dft_block(const double* x, int N, const double* w, int n_freq, double f_lo, double f_hi, int log_axis,
+                  double dt, double* ak, double* bk);
+void oc_dft_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int n_freq, double f_lo,
+                  double f_hi, int log_axis, int window_type, double* ak, double* bk, int* flags, int threads);
 
 /* Gram-Schmidt */
 void oc_cgs(const double* a, double* b, int n, double* den);

@@ -112,3 +112,24 @@ def test_facade_bif2d(gpu_engine, tmp_path):
     cfg = _rows(tmp_path / "bif2d.csv_config.csv")
     assert cfg[:2] == ["2D bifurcation (DBSCAN)", "Parameter estimation"]
     assert "eps_DBSCAN = 0.1" in cfg and "indices = 2, 1" in cfg and "n_pts = 12x12" in cfg
+
+
+def test_facade_dft(gpu_engine, tmp_path):
+    print("case: facade run_dft_1d (Lorenz 16 rho, 32 freqs) CSV files and flags vs C ABI", flush=True)
+    out = subprocess.run([os.path.join(ROOT, "u-cuda_amd", "facade_demo"), str(tmp_path)], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    lor = ucuda.preset("lorenz", gpu_engine)
+    g = ucuda.run_dft(lor, [ucuda.axis(ucuda.AXIS_PARAM, 2, 16, 0.5, 250.0)], [0.1, 0.1, 0.1], lor.values(), 0.01,
+                      10.0, 20.0, n_freq=32, freq_lo=0.0, freq_hi=5.0, window_type=1)
+    assert res["dft_flags"] == g["flags"].cpu().numpy().tolist()
+    for name in ("dft1d_AkCOS.csv", "dft1d_BkSIN.csv"):
+        rows = _rows(tmp_path / name)
+        assert rows[:2] == ["0.5, 250", "0, 5"]
+        m = np.array([[float(x) for x in r.split(", ")] for r in rows[2:]])
+        assert m.shape == (16, 32)
+        assert (m[0] == -1.0).all()  # rho = 0.5 settles on the origin: fixed point, every coefficient -1
+    cfg = _rows(tmp_path / "dft1d_config.csv")
+    assert cfg[0] == "1D bifurcation DFT" and "n_freq = 32" in cfg and "window = Hanning" in cfg
+    assert "freq_lo = 0, freq_hi = 5" in cfg and "indexVar for DFT = 0" in cfg

@@ -356,3 +356,33 @@ def test_bif2d_periods(gpu_engine):
                                 precision=ucuda.PREC_F32C).cpu().numpy()
     print("F32C periods equal to FP64 on %.3f of the cells" % (g32 == r).mean())
     assert ((g32 <= 0) == (r <= 0)).mean() > 0.95
+
+
+def test_dft_sweep(gpu_engine):
+    print("case: 1-D DFT sweep (Lorenz 16 rho, 2000-sample blocks, 32 freqs, Hanning), strict vs oracle", flush=True)
+    s = ucuda.preset("lorenz", gpu_engine)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 16, 0.5, 250.0)]
+    vals = s.values()
+    kw = dict(n_freq=32, freq_lo=0.0, freq_hi=5.0, window_type=1)
+    g = ucuda.run_dft(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 10.0, 20.0, strict=True, **kw)
+    step = oracle.KrsStep(gpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
+    job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], vals, 0.01, 10.0, 20.0)
+    r = oracle.dft_cells(step, job, range(16), 32, 0.0, 5.0, window_type=1, threads=8)
+    np.testing.assert_array_equal(g["flags"].cpu().numpy(), r["flags"])
+    assert (r["flags"] == 1).sum() >= 8 and (r["flags"] == -1).any()  # rho < 1: fixed point
+    np.testing.assert_array_equal(g["ak"].cpu().numpy(), r["ak"])
+    np.testing.assert_array_equal(g["bk"].cpu().numpy(), r["bk"])
+    # log frequency axis, Hamming, sampled combination x0 + pi x1 + e x2, pre_scaller 2
+    kw2 = dict(n_freq=24, freq_lo=0.05, freq_hi=8.0, freq_log_scale=True, window_type=2, writable_var=-1,
+               pre_scaller=2)
+    g2 = ucuda.run_dft(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 10.0, 30.0, strict=True, **kw2)
+    job2 = oracle.make_job(3, axes, [0.1, 0.1, 0.1], vals, 0.01, 10.0, 30.0, writable_var=-1, pre_scaller=2)
+    r2 = oracle.dft_cells(step, job2, range(16), 24, 0.05, 8.0, log_axis=True, window_type=2, threads=8)
+    np.testing.assert_array_equal(g2["ak"].cpu().numpy(), r2["ak"])
+    np.testing.assert_array_equal(g2["bk"].cpu().numpy(), r2["bk"])
+    # FP32-compensated: the spectrum's dominant line matches
+    g3 = ucuda.run_dft(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 10.0, 20.0, precision=ucuda.PREC_F32C, **kw)
+    ok = r["flags"] == 1
+    amp = np.hypot(r["ak"], r["bk"])[ok]
+    amp3 = np.hypot(g3["ak"].cpu().numpy(), g3["bk"].cpu().numpy())[ok]
+    print("F32C: max |amplitude error| / max amplitude = %.1e" % (np.abs(amp3 - amp).max() / amp.max()))

@@ -345,3 +345,34 @@ def test_bif2d_dbscan_counts():
     assert oracle.dbscan_count([1.0, 1.5] * 5, 0.1) == 2
     assert oracle.dbscan_count([0.0, 0.06, 0.12] * 3, 0.1) == 1
     assert oracle.dbscan_count([], 0.1) == 0 and oracle.dbscan_count([2.0], 0.1) == 1
+
+
+def test_dft_closed_forms():
+    """1-D DFT block: x_n = cos(2 pi f0 dt n) over a whole number of cycles of
+    every probed frequency difference gives a_k = 1/2 at f0 and 0 elsewhere
+    with the rectangular window, ~1/4 with Hanning (its mean is 1/2); below
+    1000 samples (no float-precision resync) the rotation equals the direct
+    sum; windows: Hanning 0 at both ends and 1 in the middle, Hamming 0.07672
+    at the ends, both symmetric."""
+    N, dt, f0 = 4000, 0.01, 2.0
+    n = np.arange(N)
+    x = np.cos(2 * np.pi * f0 * dt * n)
+    ak, bk = oracle.dft_block(x, oracle.dft_window(0, N), 5, 1.0, 3.0, dt)
+    np.testing.assert_allclose(ak, [0, 0, 0.5, 0, 0], atol=1e-9)
+    assert np.abs(bk).max() < 1e-5  # float-precision resync of the phase every 1000 samples
+    ak_h, _ = oracle.dft_block(x, oracle.dft_window(1, N), 5, 1.0, 3.0, dt)
+    assert abs(ak_h[2] - 0.25) < 1e-3 and np.abs(ak_h[[0, 1, 3, 4]]).max() < 1e-6
+    M = 999
+    y = np.sin(0.37 * n[:M]) + 0.2 * np.cos(1.3 * n[:M])
+    w = oracle.dft_window(2, M)
+    freqs = np.linspace(0.1, 4.0, 7)
+    a, b = oracle.dft_block(y, w, 7, 0.1, 4.0, dt)
+    direct_a = [(w * y * np.cos(2 * np.pi * f * dt * n[:M])).sum() / M for f in freqs]
+    direct_b = [(w * y * np.sin(2 * np.pi * f * dt * n[:M])).sum() / M for f in freqs]
+    np.testing.assert_allclose(a, direct_a, atol=1e-12)
+    np.testing.assert_allclose(b, direct_b, atol=1e-12)
+    hann, hamm = oracle.dft_window(1, 101), oracle.dft_window(2, 101)
+    assert hann[0] == 0.0 and abs(hann[-1]) < 1e-15 and abs(hann[50] - 1.0) < 1e-15
+    np.testing.assert_allclose(hann, hann[::-1], atol=1e-15)
+    np.testing.assert_allclose([hamm[0], hamm[-1]], [0.07672, 0.07672], atol=1e-15)
+    np.testing.assert_allclose(hamm, hamm[::-1], atol=1e-15)

@@ -48,6 +48,33 @@ void bif1d_rows(std::ostream& out, double param, int npeaks, const double* peaks
         for (int j = 0; j < npeaks; ++j) out << param << ", " << peaks[j] << ", " << times[j] << '\n';
 }
 
+void dft1d_config(std::ostream& out, const std::vector<double>& values, const std::vector<double>& ic, double t_max,
+                  double transient, double h, int pre_scaller, int writable_var, int index_of_mut_var, double lo,
+                  double hi, int n_freq, double freq_lo, double freq_hi, int window_type) {
+    out << std::setprecision(kPrecision);
+    out << "1D bifurcation DFT\nParameter estimation\n";
+    values_and_ic(out, values, ic);
+    out << "CT = " << t_max << "\nTT = " << transient << "\nh = " << h << "\ndecimator = " << pre_scaller
+        << "\nindexVar for DFT = " << writable_var << "\nindexPar for estimation = " << index_of_mut_var
+        << "\nstart value = " << lo << ", stop value = " << hi << "\n";
+    out << "n_freq = " << n_freq << "\nfreq_lo = " << freq_lo << ", freq_hi = " << freq_hi << "\n";
+    out << "window = " << (window_type == 0 ? "None" : window_type == 2 ? "Hamming" : "Hanning") << "\n";
+}
+
+void dft1d_header(std::ostream& out, double lo, double hi, double freq_lo, double freq_hi) {
+    out << std::setprecision(kPrecision) << lo << ", " << hi << "\n" << freq_lo << ", " << freq_hi << "\n";
+}
+
+void dft1d_matrix(std::ostream& out, const double* matrix, int n_rows, int n_freq) {
+    out << std::setprecision(kPrecision);
+    for (int i = 0; i < n_rows; ++i) {
+        for (int j = 0; j < n_freq; ++j) {
+            out << matrix[(size_t)i * n_freq + j];
+            out << (j != n_freq - 1 ? ", " : "\n");
+        }
+    }
+}
+
 void curve1d_config(std::ostream& out, const char* title, const std::vector<double>& values,
                     const std::vector<double>& ic, double t_max, double NT, double transient, double h, double eps,
                     int index_of_mut_var, double lo, double hi) {

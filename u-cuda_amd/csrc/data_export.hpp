@@ -23,6 +23,14 @@ void bif1d_config(std::ostream& out, const std::vector<double>& values, const st
 // "param, 0, -1" for a fixed point.
 void bif1d_rows(std::ostream& out, double param, int npeaks, const double* peaks, const double* times);
 
+// DFT: config, then per file the two range lines "lo, hi" and one row of
+// n_freq coefficients per sweep point.
+void dft1d_config(std::ostream& out, const std::vector<double>& values, const std::vector<double>& ic, double t_max,
+                  double transient, double h, int pre_scaller, int writable_var, int index_of_mut_var, double lo,
+                  double hi, int n_freq, double freq_lo, double freq_hi, int window_type);
+void dft1d_header(std::ostream& out, double lo, double hi, double freq_lo, double freq_hi);
+void dft1d_matrix(std::ostream& out, const double* matrix, int n_rows, int n_freq);
+
 void curve1d_config(std::ostream& out, const char* title, const std::vector<double>& values,
                     const std::vector<double>& ic, double t_max, double NT, double transient, double h, double eps,
                     int index_of_mut_var, double lo, double hi);

@@ -879,6 +879,121 @@ int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, doub
     return UC_OK;
 }
 
+// 1-D DFT sweep: the bifurcation kernel records each cell's block (loop
+// flag kept raw), the DFT kernel turns it into n_freq coefficients. The
+// window, the per-frequency rotation and the float-precision resync values
+// are computed here with the formulas of run_dft1d_classical / DFT_custom,
+// so the device work is multiply-add only.
+int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft_output* o, void* stream) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    int rc = validate_common(e, s, r->c, true);
+    if (rc) return rc;
+    for (int i = 0; i < r->c.n_axes; ++i)
+        if (r->c.axis[i].kind == UC_AXIS_STEP) return e->fail(UC_ERR_ARGUMENT, "DFT sweeps parameters or initial conditions, not h");
+    if (!o->ak_cos || !o->bk_sin || !o->flags) return e->fail(UC_ERR_ARGUMENT, "DFT outputs missing");
+    if (r->writable_var < -1 || r->writable_var >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "writable_var out of range");
+    if (r->pre_scaller <= 0) return e->fail(UC_ERR_ARGUMENT, "pre_scaller must be > 0");
+    if (r->n_freq < 2) return e->fail(UC_ERR_ARGUMENT, "n_freq must be >= 2");
+    if (r->window_type < 0 || r->window_type > 2) return e->fail(UC_ERR_ARGUMENT, "window_type must be 0, 1 or 2");
+    if (r->freq_log_scale && !(r->freq_lo > 0.0 && r->freq_hi > 0.0))
+        return e->fail(UC_ERR_ARGUMENT, "log frequency axis needs freq_lo, freq_hi > 0");
+    if (!(r->c.h > 0.0)) return e->fail(UC_ERR_ARGUMENT, "h must be > 0");
+    const long long N = (long long)(r->c.t_max / r->c.h / (double)r->pre_scaller);
+    if (N < 2) return e->fail(UC_ERR_ARGUMENT, "block of t_max / h / pre_scaller samples must hold >= 2");
+    if (N > 0x7fffffffLL) return e->fail(UC_ERR_ARGUMENT, "block too long");
+    if ((rc = enter(e))) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
+
+    // host tables (formulas of build_window and DFT_custom)
+    const double kPi = 3.14159265358979323846;
+    const double dt = r->c.h * (double)r->pre_scaller;
+    std::vector<double> window((size_t)N), tab(2 * (size_t)r->n_freq);
+    if (r->window_type == 0) std::fill(window.begin(), window.end(), 1.0);
+    else {
+        const double gamma = 2.0 * kPi / (double)(N - 1);
+        for (long long n = 0; n < N; ++n)
+            window[(size_t)n] = r->window_type == 2 ? 0.53836 - 0.46164 * std::cos(gamma * (double)n)
+                                                    : 0.5 * (1.0 - std::cos(gamma * (double)n));
+    }
+    const int n_res = (int)((N - 1) / 1000);
+    std::vector<double> resync(2 * (size_t)r->n_freq * (size_t)std::max(1, n_res));
+    const double f_step = (r->freq_hi - r->freq_lo) / (double)(r->n_freq - 1);
+    const double psi = 2.0 * kPi * dt;
+    for (int k = 0; k < r->n_freq; ++k) {
+        double fk;
+        if (r->freq_log_scale) {
+            const double l0 = std::log10(r->freq_lo), l1 = std::log10(r->freq_hi);
+            fk = std::pow(10.0, l0 + (double)k * ((l1 - l0) / (double)(r->n_freq - 1)));
+        } else {
+            fk = r->freq_lo + (double)k * f_step;
+        }
+        tab[2 * (size_t)k] = std::cos(2.0 * kPi * dt * fk);
+        tab[2 * (size_t)k + 1] = std::sin(2.0 * kPi * dt * fk);
+        for (int q = 0; q < n_res; ++q) {
+            const double exact = psi * fk * ((double)(1000LL * (q + 1) - 1) + 1.0);
+            resync[2 * ((size_t)k * n_res + q)] = (double)std::cos((float)exact);
+            resync[2 * ((size_t)k * n_res + q) + 1] = (double)std::sin((float)exact);
+        }
+    }
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    const long long total = j.n_cells;
+    const long long chunk = (long long)std::min<size_t>((size_t)total, std::max<size_t>(kBlock, free_b / 4 / ((size_t)N * sizeof(double))));
+    double *traj = nullptr, *dwin = nullptr, *dtab = nullptr, *dres = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(st);
+        for (void* p : {(void*)traj, (void*)dwin, (void*)dtab, (void*)dres})
+            if (p) (void)hipFree(p);
+    };
+    hipError_t he = hipMalloc((void**)&traj, sizeof(double) * (size_t)N * (size_t)chunk);
+    if (he == hipSuccess) he = hipMalloc((void**)&dwin, sizeof(double) * window.size());
+    if (he == hipSuccess) he = hipMalloc((void**)&dtab, sizeof(double) * tab.size());
+    if (he == hipSuccess) he = hipMalloc((void**)&dres, sizeof(double) * resync.size());
+    if (he == hipSuccess) he = hipMemcpyAsync(dwin, window.data(), sizeof(double) * window.size(), hipMemcpyHostToDevice, st);
+    if (he == hipSuccess) he = hipMemcpyAsync(dtab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice, st);
+    if (he == hipSuccess) he = hipMemcpyAsync(dres, resync.data(), sizeof(double) * resync.size(), hipMemcpyHostToDevice, st);
+    if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "DFT buffers")); }
+    hipFunction_t fdft;
+    he = hipModuleGetFunction(&fdft, m->mod, "uc_dft");
+    if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_dft")); }
+    const char* kname = r->c.precision == UC_PREC_F64 ? "uc_bif_f64" : "uc_bif_f32c";
+    for (long long off = 0; off < total; off += chunk) {
+        long long mcells = std::min(chunk, total - off);
+        UcJob jc = j;
+        jc.cell_offset = off;
+        jc.n_cells = mcells;
+        jc.stride = mcells;
+        jc.skip_steps = (long long)(r->c.transient_time / r->c.h);
+        jc.n_iters = N;
+        jc.pre_scaller = r->pre_scaller;
+        jc.writable_var = r->writable_var;
+        jc.max_peaks = 0;
+        jc.raw_flags = 1;
+        jc.flag = o->flags + off;
+        jc.traj = traj;
+        jc.traj_len = (int)N;
+        if ((rc = launch(e, m, kname, jc, st, tb))) { release(); return rc; }
+        int Ni = (int)N, nf = r->n_freq, nr = n_res;
+        long long stride = mcells;
+        int* fl = o->flags + off;
+        double* A = o->ak_cos + off * nf;
+        double* B = o->bk_sin + off * nf;
+        void* args[] = {&traj, &stride, &mcells, &Ni, &fl, &dwin, &nf, &dtab, &dres, &nr, &A, &B};
+        const long long lanes = mcells * nf;
+        he = hipModuleLaunchKernel(fdft, (unsigned)((lanes + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, args,
+                                   nullptr);
+        if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_dft launch")); }
+    }
+    release();
+    return UC_OK;
+}
+
 int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_samples, long long n_series,
                    double time_step, int max_peaks, int* counts, double* peaks, double* intervals, void* stream) {
     if (!e || !s) return UC_ERR_ARGUMENT;

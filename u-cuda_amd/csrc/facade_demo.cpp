@@ -80,6 +80,25 @@ int main(int argc, char** argv) {
     Bifurcation2DResult b2r = eng.run_bifurcation_2d(b2);
     if (!b2r.ok) { std::printf("{\"error\": \"%s\"}\n", b2r.error.c_str()); return 1; }
 
+    Dft1DRequest dq;
+    dq.system = lorenz;
+    dq.amountOfX = 3;
+    dq.initial_conditions = {0.1, 0.1, 0.1};
+    dq.base_values = {0.5, 10.0, 28.0, 8.0 / 3.0};
+    dq.param_index = 2;
+    dq.param_lo = 0.5;
+    dq.param_hi = 250.0;
+    dq.n_pts = 16;
+    dq.h = 0.01;
+    dq.transient_time = 10.0;
+    dq.t_max = 20.0;
+    dq.n_freq = 32;
+    dq.freq_lo = 0.0;
+    dq.freq_hi = 5.0;
+    dq.csv_output_path = csv("dft1d");
+    Dft1DResult dr = eng.run_dft_1d(dq);
+    if (!dr.ok) { std::printf("{\"error\": \"%s\"}\n", dr.error.c_str()); return 1; }
+
     System duffing{{"x", "v"}, {"delta", "gamma", "omega"}, {"v", "-delta*v+x-x^3+gamma*cos(omega*t)"}, false, "t"};
     BasinsRequest bq;
     bq.system = duffing;
@@ -110,6 +129,8 @@ int main(int argc, char** argv) {
     }
     std::printf("], \"bif2d\": [");
     for (size_t i = 0; i < b2r.values.size(); ++i) std::printf("%s%.17g", i ? ", " : "", b2r.values[i]);
+    std::printf("], \"dft_flags\": [");
+    for (size_t i = 0; i < dr.flags.size(); ++i) std::printf("%s%d", i ? ", " : "", dr.flags[i]);
     std::printf("], \"basins\": [");
     for (size_t i = 0; i < bres.basin_idx.size(); ++i) std::printf("%s%d", i ? ", " : "", bres.basin_idx[i]);
     std::printf("], \"lle_min\": %.17g, \"lle_max\": %.17g}\n", lr.min_val, lr.max_val);

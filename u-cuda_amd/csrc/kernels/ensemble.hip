@@ -570,7 +570,7 @@ __device__ void uc_bif_cell(const UcJob& j, long long lane) {
     if (flag == 1 || flag == -1)
         flag = uni ? uc_loop_u(s, a, h, t, j.n_iters, j.pre_scaller, j.writable_var, j.max_value, obs)
                    : uc_loop(s, a, h, t, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
-    j.flag[lane] = (flag == 1) ? obs.count() : flag;
+    j.flag[lane] = (flag == 1 && !j.raw_flags) ? obs.count() : flag;
     if (j.final_state) {
 #pragma unroll
         for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.stride + lane] = s.get(i);
@@ -958,6 +958,55 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan(const doubl
         }
     }
     out[c] = cluster - 1;
+}
+
+// 1-D DFT of recorded blocks (DFT_custom, U-CUDA/cudaLibrary.cu:1698-1777),
+// one lane per (cell, frequency): a_k + i b_k = (1/N) sum_n w_n x_n
+// (cos, sin)(phase_n), the phase advanced by the per-sample rotation
+// tab[2k], tab[2k+1] (cos, sin) and resynchronised after every 1000 samples
+// (while samples remain) from resync[2 (k * n_res + r)], the float-precision
+// cos / sin of the exact phase. The host computes tab and resync, so the
+// kernel itself only multiplies and adds. Blocks flagged -1 / 0 give -1 / 0
+// for every coefficient. traj is sample-major ([n * stride + cell]); the
+// coefficients are cell-major ([cell * n_freq + k]).
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __restrict__ traj, long long stride,
+                                                                long long n_cells, int N, const int* __restrict__ flags,
+                                                                const double* __restrict__ window, int n_freq,
+                                                                const double* __restrict__ tab,
+                                                                const double* __restrict__ resync, int n_res,
+                                                                double* ak, double* bk) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n_cells * n_freq) return;
+    const long long cell = idx / n_freq;
+    const int k = (int)(idx - cell * n_freq);
+    const int flag = flags[cell];
+    if (flag == -1 || flag == 0) {
+        ak[idx] = (double)flag;
+        bk[idx] = (double)flag;
+        return;
+    }
+    const double ct = tab[2 * k], st = tab[2 * k + 1];
+    double a = 0.0, b = 0.0, cn = 1.0, sn = 0.0;
+    int until = 1000, r = 0;
+    for (int n = 0; n < N; ++n) {
+        const double wd = window[n] * traj[(long long)n * stride + cell];
+        a = a + wd * cn;
+        b = b + wd * sn;
+        const double nc = cn * ct - sn * st;
+        const double ns = sn * ct + cn * st;
+        cn = nc;
+        sn = ns;
+        if (--until == 0) {
+            until = 1000;
+            if (n + 1 < N) {
+                cn = resync[2 * ((long long)k * n_res + r)];
+                sn = resync[2 * ((long long)k * n_res + r) + 1];
+                ++r;
+            }
+        }
+    }
+    ak[idx] = a / (double)N;
+    bk[idx] = b / (double)N;
 }
 
 // Zeroes the work counters of a batch (a kernel, not a memset, so the reset

@@ -278,6 +278,58 @@ void min_max(const double* v, const int* f, size_t n, double& lo, double& hi) {
 }
 }  // namespace
 
+Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
+    Dft1DResult res;
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    if (!impl_->eng) { res.error = "engine not initialised"; return res; }
+    if (req.continuation || req.use_cpu) { res.error = "continuation / CPU sweeps are not part of this engine"; return res; }
+    if (req.sweep_over_h) { res.error = "the DFT sweep runs over parameters or initial conditions, not h"; return res; }
+    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
+    uc_system* s = impl_->system(req.system, res.error);
+    if (!s) return res;
+    uc_dft_request r{};
+    impl_->common(r.c, req.initial_conditions, req.base_values, req.h, req.transient_time, req.t_max, req.max_value,
+                  req.precision);
+    r.c.n_axes = 1;
+    r.c.axis[0] = make_axis(false, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts, req.param_lo,
+                            req.param_hi, req.log_scale);
+    r.writable_var = req.writable_var;
+    r.pre_scaller = req.pre_scaller;
+    r.n_freq = req.n_freq;
+    r.freq_lo = req.freq_lo;
+    r.freq_hi = req.freq_hi;
+    r.freq_log_scale = req.freq_log_scale ? 1 : 0;
+    r.window_type = req.window_type;
+    const size_t n = (size_t)req.n_pts, nf = (size_t)std::max(1, req.n_freq);
+    DevBuf<double> ak, bk;
+    DevBuf<int> flags;
+    if (!ak.alloc(n * nf) || !bk.alloc(n * nf) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    uc_dft_output o{ak.p, bk.p, flags.p};
+    if (uc_run_dft(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    if (!ak.download(res.ak_cos, n * nf) || !bk.download(res.bk_sin, n * nf) || !flags.download(res.flags, n)) {
+        res.error = "device error";
+        return res;
+    }
+    res.n_pts = req.n_pts;
+    res.n_freq = req.n_freq;
+    res.param_lo = req.param_lo; res.param_hi = req.param_hi;
+    res.freq_lo = req.freq_lo; res.freq_hi = req.freq_hi;
+    if (!req.csv_output_path.empty()) {
+        std::ofstream cfg(req.csv_output_path + "_config.csv");
+        csv::dft1d_config(cfg, req.base_values, req.initial_conditions, req.t_max, req.transient_time, req.h,
+                          req.pre_scaller, req.writable_var, req.sweep_over_var ? req.var_sweep_index : req.param_index,
+                          req.param_lo, req.param_hi, req.n_freq, req.freq_lo, req.freq_hi, req.window_type);
+        std::ofstream a(req.csv_output_path + "_AkCOS.csv"), b(req.csv_output_path + "_BkSIN.csv");
+        csv::dft1d_header(a, req.param_lo, req.param_hi, req.freq_lo, req.freq_hi);
+        csv::dft1d_header(b, req.param_lo, req.param_hi, req.freq_lo, req.freq_hi);
+        csv::dft1d_matrix(a, res.ak_cos.data(), req.n_pts, req.n_freq);
+        csv::dft1d_matrix(b, res.bk_sin.data(), req.n_pts, req.n_freq);
+    }
+    done(req.progress);
+    res.ok = true;
+    return res;
+}
+
 Bifurcation2DResult ParametricEngine::run_bifurcation_2d(const Bifurcation2DRequest& req) {
     Bifurcation2DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);

@@ -91,6 +91,44 @@ struct Bifurcation2DResult {
     double min_val = 0.0, max_val = 0.0;
 };
 
+// 1-D DFT sweep (U-CUDA/parametric_engine.h:140-226).
+struct Dft1DRequest {
+    System system;
+    std::string krs_body;  // informational: the engine integrates `system` with RK4
+    int amountOfX = 0;
+    std::vector<double> initial_conditions, base_values;
+    int param_index = 0;
+    bool sweep_over_var = false;
+    int var_sweep_index = 0;
+    bool sweep_over_h = false, log_scale = false;
+    bool continuation = false, continuation_reverse = false, use_cpu = false;
+    double param_lo = 0.0, param_hi = 1.0;
+    int n_pts = 1000;
+    int writable_var = 0;
+    double h = 0.01, transient_time = 0.0, t_max = 0.0;
+    int pre_scaller = 1;
+    double max_value = 1.0e6;
+    int n_freq = 200;
+    double freq_lo = 0.0, freq_hi = 10.0;
+    bool freq_log_scale = false;
+    int window_type = 1;  // 0 none, 1 Hanning, 2 Hamming
+    int precision = 0;
+    std::string csv_output_path;  // non-empty: <path>_config.csv, <path>_AkCOS.csv, <path>_BkSIN.csv
+    std::shared_ptr<std::atomic<bool>> cancel;
+    std::shared_ptr<std::atomic<float>> progress;
+};
+
+struct Dft1DResult {
+    bool ok = false, cancelled = false;
+    std::string error;
+    bool continuation_reverse = false;
+    double param_lo = 0.0, param_hi = 1.0;
+    int n_pts = 0, n_freq = 0;
+    double freq_lo = 0.0, freq_hi = 0.0;
+    std::vector<double> ak_cos, bk_sin;  // [pt * n_freq + f]
+    std::vector<int> flags;              // 1 ok, 0 diverged, -1 fixed point
+};
+
 struct LLE1DRequest {
     System system;
     int amountOfX = 0;
@@ -207,6 +245,7 @@ public:
 
     Bifurcation1DResult run_bifurcation_1d(const Bifurcation1DRequest& req);
     Bifurcation2DResult run_bifurcation_2d(const Bifurcation2DRequest& req);
+    Dft1DResult run_dft_1d(const Dft1DRequest& req);
     LLE1DResult run_lle_1d(const LLE1DRequest& req);
     LS1DResult run_ls_1d(const LS1DRequest& req);
     LLE2DResult run_lle_2d(const LLE2DRequest& req);

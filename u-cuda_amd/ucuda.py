@@ -69,6 +69,16 @@ class LyapOutput(C.Structure):
     _fields_ = [("values", C.c_void_p), ("flags", C.c_void_p), ("final_state", C.c_void_p)]
 
 
+class DftRequest(C.Structure):
+    _fields_ = [("c", Common), ("writable_var", C.c_int), ("pre_scaller", C.c_int), ("n_freq", C.c_int),
+                ("freq_lo", C.c_double), ("freq_hi", C.c_double), ("freq_log_scale", C.c_int),
+                ("window_type", C.c_int)]
+
+
+class DftOutput(C.Structure):
+    _fields_ = [("ak_cos", C.c_void_p), ("bk_sin", C.c_void_p), ("flags", C.c_void_p)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_fraction", C.c_double), ("max_bytes", C.c_longlong), ("round_traj_steps", C.c_double),
                 ("block", C.c_int), ("waves_per_simd", C.c_int)]
@@ -120,6 +130,7 @@ _SIGNATURES = {
                                 C.c_void_p]),
     "uc_run_bif_periods": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.c_double, C.c_void_p,
                                      C.c_void_p]),
+    "uc_run_dft": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DftRequest), C.POINTER(DftOutput), C.c_void_p]),
     "uc_peak_finder": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong, C.c_double,
                                  C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "uc_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -428,6 +439,28 @@ def run_bif_periods(sysm, axes, ic, values, h, transient_time, t_max, eps_dbscan
                                        _stream(stream)), "uc_run_bif_periods")
     del keep
     return periods
+
+
+def run_dft(sysm, axes, ic, values, h, transient_time, t_max, n_freq=200, freq_lo=0.0, freq_hi=10.0,
+            freq_log_scale=False, window_type=1, writable_var=0, pre_scaller=1, max_value=1e6, precision=PREC_F64,
+            strict=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0, stream=None):
+    """1-D DFT sweep: dict(ak, bk [cells, n_freq], flags [cells]) CUDA tensors."""
+    import torch
+    eng = sysm.engine
+    n = n_cells(axes)
+    dev = torch.device("cuda", eng.device)
+    out = dict(ak=torch.empty((n, n_freq), dtype=torch.float64, device=dev),
+               bk=torch.empty((n, n_freq), dtype=torch.float64, device=dev),
+               flags=torch.empty(n, dtype=torch.int32, device=dev))
+    r = DftRequest()
+    r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
+                        ic_seed, ic_lo, ic_hi)
+    r.writable_var, r.pre_scaller, r.n_freq = writable_var, pre_scaller, n_freq
+    r.freq_lo, r.freq_hi, r.freq_log_scale, r.window_type = float(freq_lo), float(freq_hi), int(freq_log_scale), window_type
+    o = DftOutput(_ptr(out["ak"]).value, _ptr(out["bk"]).value, _ptr(out["flags"]).value)
+    eng.check(lib().uc_run_dft(eng.h, sysm.h, C.byref(r), C.byref(o), _stream(stream)), "uc_run_dft")
+    del keep
+    return out
 
 
 def run_lyap(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=False, eps=1e-4,
