@@ -210,6 +210,103 @@ def extra_bif_config1(eng, threads_all):
     return out
 
 
+def _timed_calls(fn, reps=3):
+    """Wall time of whole calls (host set-up, temporary buffers, every
+    kernel), synchronised on both sides; returns (ms list, last result)."""
+    import torch
+    fn()  # compile + warm-up
+    torch.cuda.synchronize()
+    ms, r = [], None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        ms.append((time.perf_counter() - t0) * 1e3)
+    return ms, r
+
+
+def extra_bif2d(eng, n=512, stride=8):
+    """2-D bifurcation period map: Lorenz rho in [20, 250] x sigma in [5, 16]
+    (n x n), beta = 8/3, X0 = (0.1, 0.1, 0.1), RK4 h = 0.01, 1e4 transient +
+    1e4 recorded steps, maxima of x clustered by DBSCAN with eps 0.1 (the
+    reference's default eps). GPU: whole-call wall time (sweep + per-cell
+    DBSCAN). Parity: strict FP64 against the oracle on every stride-th point
+    per axis, whose single-thread time is the CPU rate."""
+    import numpy as np
+    import ucuda
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    lor = ucuda.preset("lorenz", eng)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, n, 20.0, 250.0), ucuda.axis(ucuda.AXIS_PARAM, 1, n, 5.0, 16.0)]
+    args = (lor, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 100.0, 100.0)
+    traj_steps = n * n * (10000 + 1 + 10000 + 1)
+    out, maps = {}, {}
+    for name, kw in (("f64_strict", dict(strict=True)), ("f64", {}), ("f32c", dict(precision=ucuda.PREC_F32C))):
+        ms, r = _timed_calls(lambda: ucuda.run_bif_periods(*args, eps_dbscan=0.1, **kw))
+        maps[name] = r.cpu().numpy()
+        out[name] = dict(ms=float(np.mean(ms)), traj_steps_per_s=traj_steps / (np.mean(ms) / 1e3))
+        progress("bif2d %s: %.1f ms per map" % (name, np.mean(ms)))
+    idx = np.arange(0, n, stride)
+    cells = (idx[:, None] * n + idx[None, :]).ravel()
+    step = oracle.KrsStep(eng.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
+    job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 100.0, 100.0)
+    t0 = time.perf_counter()
+    ref = oracle.bif_periods(step, job, cells, 0.1, threads=1)
+    cpu_s = time.perf_counter() - t0
+    sample_steps = len(cells) * (10000 + 1 + 10000 + 1)
+    out["cpu_oracle"] = dict(cells=int(len(cells)), s=cpu_s, traj_steps_per_s=sample_steps / cpu_s, cores=1)
+    out["f64_strict"]["identical_to_oracle"] = bool(np.array_equal(maps["f64_strict"][cells], ref))
+    for name in ("f64", "f32c"):
+        out[name]["period_agreement_vs_oracle"] = float((maps[name][cells] == ref).mean())
+    out["periods_histogram"] = {int(k): int(v) for k, v in zip(*np.unique(maps["f64_strict"], return_counts=True))
+                                if k <= 8 or v > n * n // 100}
+    return out
+
+
+def extra_dft(eng, n=16384, n_freq=200, stride=64):
+    """1-D DFT sweep: Lorenz rho in [0.5, 250] (n points), X0 = (0.1, 0.1,
+    0.1), RK4 h = 0.01, 1e4 transient steps, blocks of 1e4 samples of x,
+    Hanning window, n_freq frequencies on [0, 10] (the reference's default
+    frequency axis). GPU: whole-call wall time (sweep recording the blocks,
+    then the DFT kernel). Parity: strict FP64 coefficients against the oracle
+    on every stride-th point, whose single-thread time is the CPU rate."""
+    import numpy as np
+    import ucuda
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    lor = ucuda.preset("lorenz", eng)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, n, 0.5, 250.0)]
+    args = (lor, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 100.0, 100.0)
+    kw = dict(n_freq=n_freq, freq_lo=0.0, freq_hi=10.0, window_type=1)
+    N = 10000
+    out, res = {}, {}
+    for name, extra in (("f64_strict", dict(strict=True)), ("f64", {}), ("f32c", dict(precision=ucuda.PREC_F32C))):
+        ms, r = _timed_calls(lambda: ucuda.run_dft(*args, **kw, **extra))
+        res[name] = {k: v.cpu().numpy() for k, v in r.items()}
+        out[name] = dict(ms=float(np.mean(ms)), cells_per_s=n / (np.mean(ms) / 1e3),
+                         dft_terms_per_s=n * n_freq * N / (np.mean(ms) / 1e3))
+        progress("dft %s: %.1f ms per sweep" % (name, np.mean(ms)))
+    cells = np.arange(0, n, stride)
+    step = oracle.KrsStep(eng.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
+    job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 100.0, 100.0)
+    t0 = time.perf_counter()
+    ref = oracle.dft_cells(step, job, cells, n_freq, 0.0, 10.0, window_type=1, threads=1)
+    cpu_s = time.perf_counter() - t0
+    out["cpu_oracle"] = dict(cells=int(len(cells)), s=cpu_s, cells_per_s=len(cells) / cpu_s, cores=1)
+    g = res["f64_strict"]
+    out["f64_strict"]["identical_to_oracle"] = bool(
+        np.array_equal(g["flags"][cells], ref["flags"]) and np.array_equal(g["ak"][cells], ref["ak"])
+        and np.array_equal(g["bk"][cells], ref["bk"]))
+    ok = ref["flags"] == 1
+    amp = np.hypot(ref["ak"], ref["bk"])[ok]
+    for name in ("f64", "f32c"):
+        a = np.hypot(res[name]["ak"][cells], res[name]["bk"][cells])[ok]
+        out[name]["flag_agreement"] = float((res[name]["flags"][cells] == ref["flags"]).mean())
+        out[name]["median_amp_err_rel_max"] = float(np.median(np.abs(a - amp).max(axis=1) / amp.max(axis=1)))
+    out["config"] = dict(cells=n, n_freq=n_freq, samples_per_block=N, window="hanning")
+    return out
+
+
 def extra_ls(eng, n=1024, steps_scale=1.0):
     """hyper-Rossler 4-exponent spectrum maps, h=0.005, Gram-Schmidt every 20
     steps, 1e5 steps, FP32-compensated and FP64:
@@ -528,6 +625,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the LS / basins / batch / serving measurements")
     ap.add_argument("--serve-seconds", type=float, default=8.0)
+    ap.add_argument("--extras", default="all",
+                    help="comma-separated subset of the extra measurements (keys of the JSON line), or 'all'")
     ap.add_argument("--only", choices=["f32c", "f64"], default=None, help="time one precision only (profiling)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="after timing, run one device-to-device copy of this size (HBM counter calibration)")
@@ -612,18 +711,27 @@ def main():
     }
     if not args.no_extra:
         cap = value  # traj-steps/s of the headline kernel sets the serving load
-        progress("Lorenz bifurcation, 1024 rho (the reference's CPU case) start")
-        line["bif_lorenz_1024"] = extra_bif_config1(eng, min(16, os.cpu_count() or 1))
-        progress("Lorenz bifurcation done; LS spectrum map (hyper-Rossler 1024^2) start")
-        line["ls_hyper_rossler_1024"] = extra_ls(eng)
-        progress("LS spectrum map done; basins (Duffing 2048^2) start")
-        line["basins_duffing_2048"] = extra_basins(eng)
-        progress("basins done; batch (64 jobs) start")
-        line["batch_64_jobs"] = extra_batch(eng)
-        progress("batch done; serving (%.0f s of Poisson arrivals) start" % args.serve_seconds)
-        line["serving"] = extra_serving(eng, duration_s=args.serve_seconds, capacity=cap)
-        progress("serving done")
-        line["p99_job_latency_ms"] = line["serving"]["p99_ms"]
+        threads_all = min(16, os.cpu_count() or 1)
+        extras = [
+            ("bif_lorenz_1024", "Lorenz bifurcation, 1024 rho (the reference's CPU case)",
+             lambda: extra_bif_config1(eng, threads_all)),
+            ("bif2d_lorenz_512", "2-D period map (Lorenz 512^2)", lambda: extra_bif2d(eng)),
+            ("dft_lorenz_16384", "DFT sweep (Lorenz 16384 rho x 200 freqs)", lambda: extra_dft(eng)),
+            ("ls_hyper_rossler_1024", "LS spectrum map (hyper-Rossler 1024^2)", lambda: extra_ls(eng)),
+            ("basins_duffing_2048", "basins (Duffing 2048^2)", lambda: extra_basins(eng)),
+            ("batch_64_jobs", "batch (64 jobs)", lambda: extra_batch(eng)),
+            ("serving", "serving (%.0f s of Poisson arrivals)" % args.serve_seconds,
+             lambda: extra_serving(eng, duration_s=args.serve_seconds, capacity=cap)),
+        ]
+        wanted = None if args.extras == "all" else set(args.extras.split(","))
+        for key, what, fn in extras:
+            if wanted is not None and key not in wanted:
+                continue
+            progress("%s start" % what)
+            line[key] = fn()
+            progress("%s done" % what)
+        if "serving" in line:
+            line["p99_job_latency_ms"] = line["serving"]["p99_ms"]
     if cpu:
         line["cpu_baseline"] = {"value": cpu["cells"] * spc / cpu["t1"], "unit": "traj-steps/s", "cores": 1,
                                 "kind": "port",

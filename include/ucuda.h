@@ -242,6 +242,15 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
 int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_samples, long long n_series,
                    double time_step, int max_peaks, int* counts, double* peaks, double* intervals, void* stream);
 
+/* Periods from stored peaks (the reference's dbscanCUDA over its peak
+ * buffers, U-CUDA/cudaLibrary.cu:2321-2345, with dbscan :2088-2166): device
+ * peaks[max_peaks][n_cells] (peak-major, as uc_run_bif writes them) and
+ * counts[n_cells] (-1 / 0 / number of peaks; a count above max_peaks is read
+ * as max_peaks); periods[n_cells] as uc_run_bif_periods. Synchronises the
+ * stream when max_peaks > 1024 (scratch for those cells is allocated). */
+int uc_bif_dbscan_periods(uc_engine* e, uc_system* s, const double* peaks, const int* counts, long long n_cells,
+                          int max_peaks, double eps, int* periods, void* stream);
+
 /* ---- persistent heterogeneous batch ----
  * Jobs of any size, system and analysis, added with the same request /
  * output structs as the uc_run_* calls; jobs that share a compiled kernel

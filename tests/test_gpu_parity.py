@@ -358,6 +358,49 @@ def test_bif2d_periods(gpu_engine):
     assert ((g32 <= 0) == (r <= 0)).mean() > 0.95
 
 
+def _dbscan_cases(rng):
+    """Peak sets for the clustering kernels: sizes around the 64-lane words
+    and the 1024-peak wave / lane boundary, uniform values, levels on a grid
+    of eps (distances at the eps boundary in floating point), tight chains,
+    repeated values, period tiles, and the -1 / 0 / 1 counts."""
+    sizes = [2, 3, 5, 63, 64, 65, 66, 127, 128, 129, 500, 1023, 1024, 1025, 1026, 1300]
+    cases = []
+    for n in sizes:
+        cases.append(rng.uniform(0.0, 3.0, n))
+        cases.append(rng.integers(0, 9, n) * 0.1 + 2.0)
+        cases.append(np.cumsum(rng.uniform(0.0, 0.11, n)))
+        cases.append(np.round(rng.normal(5.0, 0.2, n), 2))
+        cases.append(np.tile(3.0 + 0.5 * np.arange(1 + n % 7), n)[:n])
+    return cases
+
+
+def test_dbscan_periods_stored_peaks(gpu_engine):
+    print("case: DBSCAN periods of stored peak sets (wave kernel n <= 1024, lane kernel above) vs oracle", flush=True)
+    import torch
+    s = ucuda.preset("lorenz", gpu_engine)
+    rng = np.random.default_rng(20261203)
+    cases = _dbscan_cases(rng)
+    max_peaks = max(len(c) for c in cases)
+    n_cells = len(cases) + 3
+    peaks = np.full((max_peaks, n_cells), np.nan)
+    counts = np.zeros(n_cells, dtype=np.int32)
+    for c, v in enumerate(cases):
+        peaks[:len(v), c] = v
+        counts[c] = len(v)
+    counts[-3:] = [-1, 0, 1]
+    peaks[0, -1] = 4.2
+    expect = np.array([oracle.dbscan_count(peaks[:counts[c], c], 0.1) if counts[c] > 1 else counts[c]
+                       for c in range(n_cells)], dtype=np.int32)
+    assert len(set(expect.tolist())) > 20
+    for mp in (max_peaks, 1024):  # all cells; then only the wave kernel (longer counts read as 1024)
+        g = ucuda.dbscan_periods(s, torch.from_numpy(np.ascontiguousarray(peaks[:mp])).cuda(),
+                                 torch.from_numpy(counts).cuda(), 0.1).cpu().numpy()
+        exp = expect if mp == max_peaks else np.array(
+            [oracle.dbscan_count(peaks[:min(counts[c], mp), c], 0.1) if counts[c] > 1 else counts[c]
+             for c in range(n_cells)], dtype=np.int32)
+        np.testing.assert_array_equal(g, exp)
+
+
 def test_dft_sweep(gpu_engine):
     print("case: 1-D DFT sweep (Lorenz 16 rho, 2000-sample blocks, 32 freqs, Hanning), strict vs oracle", flush=True)
     s = ucuda.preset("lorenz", gpu_engine)

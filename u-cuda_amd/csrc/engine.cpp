@@ -805,6 +805,33 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
     return UC_OK;
 }
 
+// Peak clustering launches shared by uc_run_bif_periods and
+// uc_bif_dbscan_periods: the wave-per-cell kernel for cells with at most
+// kDbWaveMax peaks, then the lane-per-cell kernel (HBM scratch labels /
+// slots of max_n rows) for the others; the latter is skipped when no cell
+// can have more.
+static const int kDbWaveMax = 1024;  // UC_DB_WAVE_MAX in the kernel source
+
+static int launch_dbscan(uc_engine* e, Module* m, const double* peaks, const int* counts, long long stride,
+                         long long n_cells, int max_n, double eps, int* labels, int* slots, int* out, hipStream_t st) {
+    hipFunction_t fw, fl;
+    hipError_t he = hipModuleGetFunction(&fw, m->mod, "uc_bif_dbscan_wave");
+    if (he == hipSuccess) he = hipModuleGetFunction(&fl, m->mod, "uc_bif_dbscan");
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan"));
+    void* wargs[] = {(void*)&peaks, (void*)&counts, &stride, &n_cells, &max_n, &eps, &out};
+    const long long waves_per_block = kBlock / 64;
+    he = hipModuleLaunchKernel(fw, (unsigned)((n_cells + waves_per_block - 1) / waves_per_block), 1, 1, kBlock, 1, 1,
+                               0, st, wargs, nullptr);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan_wave launch"));
+    if (max_n <= kDbWaveMax) return UC_OK;
+    int min_n = kDbWaveMax;
+    void* largs[] = {(void*)&peaks, (void*)&counts, &stride, &n_cells, &max_n, &eps, &min_n, &labels, &slots, &out};
+    he = hipModuleLaunchKernel(fl, (unsigned)((n_cells + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, largs,
+                               nullptr);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan launch"));
+    return UC_OK;
+}
+
 // 2-D bifurcation periods (run_bif2d, U-CUDA/parametric_engine.cpp:5487-5911):
 // the bifurcation sweep keeping every peak of the recorded window (the
 // reference clusters all of them), then the per-cell DBSCAN kernel. Cells
@@ -847,12 +874,9 @@ int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, doub
     };
     hipError_t he = hipMalloc((void**)&peaks, sizeof(double) * (size_t)cap * (size_t)chunk);
     if (he == hipSuccess) he = hipMalloc((void**)&flags, sizeof(int) * (size_t)chunk);
-    if (he == hipSuccess) he = hipMalloc((void**)&labels, sizeof(int) * (size_t)cap * (size_t)chunk);
-    if (he == hipSuccess) he = hipMalloc((void**)&slots, sizeof(int) * (size_t)cap * (size_t)chunk);
+    if (he == hipSuccess && cap > kDbWaveMax) he = hipMalloc((void**)&labels, sizeof(int) * (size_t)cap * (size_t)chunk);
+    if (he == hipSuccess && cap > kDbWaveMax) he = hipMalloc((void**)&slots, sizeof(int) * (size_t)cap * (size_t)chunk);
     if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(bif periods)")); }
-    hipFunction_t fdb;
-    he = hipModuleGetFunction(&fdb, m->mod, "uc_bif_dbscan");
-    if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan")); }
     const char* kname = r->c.precision == UC_PREC_F64 ? "uc_bif_f64" : "uc_bif_f32c";
     for (long long off = 0; off < total; off += chunk) {
         const long long mcells = std::min(chunk, total - off);
@@ -868,12 +892,10 @@ int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, doub
         jc.traj_len = 0;
         jc.final_state = nullptr;
         if ((rc = launch(e, m, kname, jc, st, tb))) { release(); return rc; }
-        int* out = periods + off;
-        long long stride = mcells, count = mcells;
-        void* args[] = {&peaks, &flags, &stride, &count, &eps, &labels, &slots, &out};
-        he = hipModuleLaunchKernel(fdb, (unsigned)((mcells + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, args,
-                                   nullptr);
-        if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan launch")); }
+        if ((rc = launch_dbscan(e, m, peaks, flags, mcells, mcells, (int)cap, eps, labels, slots, periods + off, st))) {
+            release();
+            return rc;
+        }
     }
     release();
     return UC_OK;
@@ -1013,6 +1035,37 @@ int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_s
     he = hipModuleLaunchKernel(f, blocks, 1, 1, kBlock, 1, 1, 0, (hipStream_t)stream, args, nullptr);
     if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLaunchKernel(peaks)"));
     return UC_OK;
+}
+
+// Periods from stored peaks (dbscanCUDA, U-CUDA/cudaLibrary.cu:2321-2345).
+int uc_bif_dbscan_periods(uc_engine* e, uc_system* s, const double* peaks, const int* counts, long long n_cells,
+                          int max_peaks, double eps, int* periods, void* stream) {
+    if (!e || !s) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (!counts || !periods || n_cells <= 0 || max_peaks < 0 || (max_peaks > 0 && !peaks))
+        return e->fail(UC_ERR_ARGUMENT, "dbscan periods: bad arguments");
+    if (!(eps > 0.0)) return e->fail(UC_ERR_ARGUMENT, "eps_dbscan must be > 0");
+    int rc = enter(e);
+    if (rc) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, true, true, &m))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    int *labels = nullptr, *slots = nullptr;
+    hipError_t he = hipSuccess;
+    if (max_peaks > kDbWaveMax) {
+        he = hipMalloc((void**)&labels, sizeof(int) * (size_t)max_peaks * (size_t)n_cells);
+        if (he == hipSuccess) he = hipMalloc((void**)&slots, sizeof(int) * (size_t)max_peaks * (size_t)n_cells);
+    }
+    if (he == hipSuccess)
+        rc = launch_dbscan(e, m, peaks, counts, n_cells, n_cells, max_peaks, eps, labels, slots, periods, st);
+    else
+        rc = e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(dbscan scratch)"));
+    if (labels || slots) {
+        (void)hipStreamSynchronize(st);
+        if (labels) (void)hipFree(labels);
+        if (slots) (void)hipFree(slots);
+    }
+    return rc;
 }
 
 }  // extern "C"

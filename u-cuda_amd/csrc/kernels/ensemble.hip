@@ -915,21 +915,113 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_peaks_series(const dou
 
 // Period of a 2-D bifurcation cell: the reference's DBSCAN over the cell's
 // peak values (dbscan, U-CUDA/cudaLibrary.cu:2088-2166, as compiled into its
-// 2-D bifurcation template), one lane per cell, the same steps in the same
-// order because the cluster count is the map value. labels / slots start at
-// 0; an unlabelled peak opens a cluster; every unlabelled peak k < n - 1
-// within eps (|dp|, the interval term being multiplied by 0) joins it with
-// slot[k] = k; while the neighbour count is positive the loop variable is
-// replaced by slot[count - 1], which is cleared, and the scan repeats from
-// there. Result: -1 / 0 / 1 for -1 / 0 / 1 peaks, else clusters - 1.
-// Peaks and scratch are peak-major ([k * stride + cell]).
+// 2-D bifurcation template), the same steps in the same order because the
+// cluster count is the map value. labels / slots start at 0; an unlabelled
+// peak opens a cluster; every unlabelled peak k < n - 1 within eps (|dp|, the
+// interval term being multiplied by 0) joins it with slot[k] = k; while the
+// neighbour count is positive the loop variable is replaced by
+// slot[count - 1], which is cleared, and the scan repeats from there.
+// Result: -1 / 0 / 1 for -1 / 0 / 1 peaks, else clusters - 1.
+// Peaks are peak-major ([k * stride + cell]).
+//
+// uc_bif_dbscan_wave: one wave per cell with n <= UC_DB_WAVE_MAX peaks.
+// Within one scan the set of peaks that join does not depend on the order
+// of k, so the 64 lanes test k = 64 j + lane at once (j < 16, peaks held in
+// registers) and the wave adds the joins with ballots. Only whether a label
+// or a slot is non-zero is ever read (slot[k] is k or 0, and slot 0 means
+// k = 0 either way), so both are bit masks: bit j of `lab` / `slot` in lane
+// l stands for peak 64 j + l. Walking over labelled peaks (neighbour count
+// 0) jumps to the next unlabelled one by ballot. Cells with more peaks are
+// left to uc_bif_dbscan (one lane per cell, scratch in HBM), launched with
+// min_n = UC_DB_WAVE_MAX so that it takes exactly those.
+#define UC_DB_WAVE_MAX 1024
+
+__device__ __forceinline__ double uc_readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan_wave(const double* __restrict__ peaks,
+                                                                            const int* __restrict__ counts,
+                                                                            long long stride, long long n_cells,
+                                                                            int max_n, double eps, int* out) {
+    const int lane = threadIdx.x & 63;
+    const long long c = (long long)blockIdx.x * (UC_BLOCK / 64) + (threadIdx.x >> 6);
+    if (c >= n_cells) return;  // wave-uniform
+    const int n = __builtin_amdgcn_readfirstlane(min(counts[c], max_n));
+    if (n <= 1) {
+        if (lane == 0) out[c] = n;
+        return;
+    }
+    if (n > UC_DB_WAVE_MAX) return;
+    const int J = (n + 63) >> 6;
+    double p[16];
+    unsigned cand = 0;  // bit j: peak 64 j + lane exists and is < n - 1 (scanned as a neighbour)
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int k = 64 * j + lane;
+        p[j] = (j < J && k < n) ? peaks[(long long)k * stride + c] : 0.0;
+        if (k < n - 1) cand |= 1u << j;
+    }
+    unsigned lab = 0, slot = 0;
+    int cluster = 0, nb = 0, i = 0;
+    while (i < n) {
+        if (nb >= 1) {
+            const int idx = nb - 1;
+            const unsigned w = (unsigned)__builtin_amdgcn_readlane((int)slot, idx & 63);
+            if (lane == (idx & 63)) slot &= ~(1u << (idx >> 6));
+            i = ((w >> (idx >> 6)) & 1u) ? idx : 0;
+            nb -= 1;
+        } else {
+            int next = n;
+            for (int j = i >> 6; j < J; ++j) {
+                const int k = 64 * j + lane;
+                const u64 m = __builtin_amdgcn_ballot_w64(k >= i && k < n && !((lab >> j) & 1u));
+                if (m) {
+                    next = 64 * j + (int)__builtin_ctzll(m);
+                    break;
+                }
+            }
+            if (next >= n) break;
+            i = next;
+            ++cluster;
+            if (lane == (i & 63)) lab |= 1u << (i >> 6);
+        }
+        double pi = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            if (j == (i >> 6)) pi = uc_readlane_f64(p[j], i & 63);
+        const unsigned open = cand & ~lab;
+        unsigned joined = 0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j < J) {
+                const double dp = pi - p[j], di = 0.0;
+                const bool hit = ((open >> j) & 1u) && (64 * j + lane) != i && sqrt(dp * dp + di * di) < eps;
+                nb += __builtin_popcountll(__builtin_amdgcn_ballot_w64(hit));
+                if (hit) joined |= 1u << j;
+            }
+        }
+        lab |= joined;
+        slot |= joined;
+        ++i;
+    }
+    if (lane == 0) out[c] = cluster - 1;
+}
+
+// uc_bif_dbscan: one lane per cell with more than min_n peaks (the others
+// are left alone); labels / slots are its scratch, peak-major like peaks.
+// Both kernels read a count above max_n (the rows of peaks) as max_n.
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan(const double* __restrict__ peaks,
                                                                        const int* __restrict__ counts, long long stride,
-                                                                       long long n_cells, double eps, int* labels,
-                                                                       int* slots, int* out) {
+                                                                       long long n_cells, int max_n, double eps,
+                                                                       int min_n, int* labels, int* slots, int* out) {
     const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n_cells) return;
-    const int n = counts[c];
+    const int n = min(counts[c], max_n);
+    if (n <= min_n) return;
     if (n <= 1) { out[c] = n; return; }
     int cluster = 0, neighbours = 0;
     for (int q = 0; q < n; ++q) { labels[(long long)q * stride + c] = 0; slots[(long long)q * stride + c] = 0; }
@@ -969,6 +1061,9 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan(const doubl
 // kernel itself only multiplies and adds. Blocks flagged -1 / 0 give -1 / 0
 // for every coefficient. traj is sample-major ([n * stride + cell]); the
 // coefficients are cell-major ([cell * n_freq + k]).
+#define UC_DFT_U 8
+static_assert(1000 % UC_DFT_U == 0, "DFT sample blocks must tile the 1000-sample resync period");
+
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __restrict__ traj, long long stride,
                                                                 long long n_cells, int N, const int* __restrict__ flags,
                                                                 const double* __restrict__ window, int n_freq,
@@ -987,22 +1082,36 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __re
     }
     const double ct = tab[2 * k], st = tab[2 * k + 1];
     double a = 0.0, b = 0.0, cn = 1.0, sn = 0.0;
-    int until = 1000, r = 0;
-    for (int n = 0; n < N; ++n) {
-        const double wd = window[n] * traj[(long long)n * stride + cell];
-        a = a + wd * cn;
-        b = b + wd * sn;
-        const double nc = cn * ct - sn * st;
-        const double ns = sn * ct + cn * st;
-        cn = nc;
-        sn = ns;
-        if (--until == 0) {
-            until = 1000;
-            if (n + 1 < N) {
-                cn = resync[2 * ((long long)k * n_res + r)];
-                sn = resync[2 * ((long long)k * n_res + r) + 1];
-                ++r;
+    int r = 0;
+    // Samples in blocks of UC_DFT_U, all loads of a block issued before its
+    // arithmetic. UC_DFT_U divides 1000, so a resync (after sample n with
+    // (n + 1) % 1000 == 0 and n + 1 < N) can only fall on a block's last
+    // sample; the per-sample operations and their order are unchanged.
+    const double* x = traj + cell;
+    for (int n0 = 0; n0 < N; n0 += UC_DFT_U) {
+        double xv[UC_DFT_U], wv[UC_DFT_U];
+#pragma unroll
+        for (int u = 0; u < UC_DFT_U; ++u) {
+            const bool in = n0 + u < N;
+            xv[u] = in ? x[(long long)(n0 + u) * stride] : 0.0;
+            wv[u] = in ? window[n0 + u] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < UC_DFT_U; ++u) {
+            if (n0 + u < N) {
+                const double wd = wv[u] * xv[u];
+                a = a + wd * cn;
+                b = b + wd * sn;
+                const double nc = cn * ct - sn * st;
+                const double ns = sn * ct + cn * st;
+                cn = nc;
+                sn = ns;
             }
+        }
+        if ((n0 + UC_DFT_U) % 1000 == 0 && n0 + UC_DFT_U < N) {
+            cn = resync[2 * ((long long)k * n_res + r)];
+            sn = resync[2 * ((long long)k * n_res + r) + 1];
+            ++r;
         }
     }
     ak[idx] = a / (double)N;

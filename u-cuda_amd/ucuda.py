@@ -133,6 +133,8 @@ _SIGNATURES = {
     "uc_run_dft": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DftRequest), C.POINTER(DftOutput), C.c_void_p]),
     "uc_peak_finder": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong, C.c_double,
                                  C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "uc_bif_dbscan_periods": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int,
+                                        C.c_double, C.c_void_p, C.c_void_p]),
     "uc_batch_create": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "uc_batch_destroy": (None, [C.c_void_p]),
     "uc_batch_add_bif": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.POINTER(BifOutput),
@@ -549,6 +551,22 @@ def peak_finder(sysm, data, time_step, max_peaks=2500, stream=None):
                                    _ptr(peaks), _ptr(intervals), _stream(stream)), "uc_peak_finder")
     torch.cuda.synchronize(dev)
     return dict(counts=counts, peaks=peaks, intervals=intervals)
+
+
+def dbscan_periods(sysm, peaks, counts, eps_dbscan=0.1, stream=None):
+    """Periods from stored peaks: peaks [max_peaks, n_cells] float64 and
+    counts [n_cells] int32 CUDA tensors (peak-major, as run_bif returns them).
+    Returns periods [n_cells] int32 as run_bif_periods."""
+    import torch
+    eng = sysm.engine
+    peaks = peaks.contiguous()
+    counts = counts.contiguous()
+    max_peaks, n = peaks.shape
+    assert counts.shape == (n,) and counts.dtype == torch.int32 and peaks.dtype == torch.float64
+    periods = torch.empty(n, dtype=torch.int32, device=peaks.device)
+    eng.check(lib().uc_bif_dbscan_periods(eng.h, sysm.h, _ptr(peaks), _ptr(counts), n, max_peaks, float(eps_dbscan),
+                                          _ptr(periods), _stream(stream)), "uc_bif_dbscan_periods")
+    return periods
 
 
 class Batch:
