@@ -901,6 +901,9 @@ int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, doub
     return UC_OK;
 }
 
+static const int kDftCells = 4;    // UC_DFT_C in the kernel source: cells per DFT lane
+static const int kDftSamples = 4;  // UC_DFT_U: samples per load block (traj / window rows padded to it)
+
 // 1-D DFT sweep: the bifurcation kernel records each cell's block (loop
 // flag kept raw), the DFT kernel turns it into n_freq coefficients. The
 // window, the per-frequency rotation and the float-precision resync values
@@ -935,8 +938,9 @@ int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft
     // host tables (formulas of build_window and DFT_custom)
     const double kPi = 3.14159265358979323846;
     const double dt = r->c.h * (double)r->pre_scaller;
-    std::vector<double> window((size_t)N), tab(2 * (size_t)r->n_freq);
-    if (r->window_type == 0) std::fill(window.begin(), window.end(), 1.0);
+    const long long n_rows = (N + kDftSamples - 1) / kDftSamples * kDftSamples;
+    std::vector<double> window((size_t)n_rows, 0.0), tab(2 * (size_t)r->n_freq);
+    if (r->window_type == 0) std::fill(window.begin(), window.begin() + N, 1.0);
     else {
         const double gamma = 2.0 * kPi / (double)(N - 1);
         for (long long n = 0; n < N; ++n)
@@ -973,7 +977,7 @@ int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft
         for (void* p : {(void*)traj, (void*)dwin, (void*)dtab, (void*)dres})
             if (p) (void)hipFree(p);
     };
-    hipError_t he = hipMalloc((void**)&traj, sizeof(double) * (size_t)N * (size_t)chunk);
+    hipError_t he = hipMalloc((void**)&traj, sizeof(double) * ((size_t)n_rows * (size_t)chunk + kDftCells));
     if (he == hipSuccess) he = hipMalloc((void**)&dwin, sizeof(double) * window.size());
     if (he == hipSuccess) he = hipMalloc((void**)&dtab, sizeof(double) * tab.size());
     if (he == hipSuccess) he = hipMalloc((void**)&dres, sizeof(double) * resync.size());
@@ -1007,7 +1011,7 @@ int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft
         double* A = o->ak_cos + off * nf;
         double* B = o->bk_sin + off * nf;
         void* args[] = {&traj, &stride, &mcells, &Ni, &fl, &dwin, &nf, &dtab, &dres, &nr, &A, &B};
-        const long long lanes = mcells * nf;
+        const long long lanes = (mcells + kDftCells - 1) / kDftCells * ((nf + 63) / 64) * 64;
         he = hipModuleLaunchKernel(fdft, (unsigned)((lanes + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, args,
                                    nullptr);
         if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_dft launch")); }

@@ -1052,16 +1052,29 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan(const doubl
     out[c] = cluster - 1;
 }
 
-// 1-D DFT of recorded blocks (DFT_custom, U-CUDA/cudaLibrary.cu:1698-1777),
-// one lane per (cell, frequency): a_k + i b_k = (1/N) sum_n w_n x_n
-// (cos, sin)(phase_n), the phase advanced by the per-sample rotation
-// tab[2k], tab[2k+1] (cos, sin) and resynchronised after every 1000 samples
-// (while samples remain) from resync[2 (k * n_res + r)], the float-precision
-// cos / sin of the exact phase. The host computes tab and resync, so the
-// kernel itself only multiplies and adds. Blocks flagged -1 / 0 give -1 / 0
-// for every coefficient. traj is sample-major ([n * stride + cell]); the
-// coefficients are cell-major ([cell * n_freq + k]).
-#define UC_DFT_U 8
+// 1-D DFT of recorded blocks (DFT_custom, U-CUDA/cudaLibrary.cu:1698-1777):
+// a_k + i b_k = (1/N) sum_n w_n x_n (cos, sin)(phase_n), the phase advanced
+// by the per-sample rotation tab[2k], tab[2k+1] (cos, sin) and
+// resynchronised after every 1000 samples (while samples remain) from
+// resync[2 (k * n_res + r)], the float-precision cos / sin of the exact
+// phase. The host computes tab and resync, so the kernel only multiplies
+// and adds. Blocks flagged -1 / 0 give -1 / 0 for every coefficient. traj
+// is sample-major ([n * stride + cell]); the coefficients are cell-major
+// ([cell * n_freq + k]).
+//
+// Work split: one wave per (group of UC_DFT_C consecutive cells, 64
+// consecutive frequencies). The rotation depends on the frequency alone, so
+// each lane advances it once per sample for the whole group; each cell's
+// sums see the same operations in the same order as a lane of its own
+// would. The group's samples are the same for every lane, so they are read
+// with scalar loads (UC_DFT_C doubles per sample), which keeps the vector
+// memory path, whose per-lane address work would otherwise bound the
+// kernel, out of the sample loop. The loads are unguarded: the caller pads
+// traj to ceil(N / UC_DFT_U) * UC_DFT_U rows plus UC_DFT_C doubles and
+// window to the same row count; samples beyond N or cells beyond n_cells
+// are read but never used.
+#define UC_DFT_U 4
+#define UC_DFT_C 4
 static_assert(1000 % UC_DFT_U == 0, "DFT sample blocks must tile the 1000-sample resync period");
 
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __restrict__ traj, long long stride,
@@ -1069,43 +1082,50 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __re
                                                                 const double* __restrict__ window, int n_freq,
                                                                 const double* __restrict__ tab,
                                                                 const double* __restrict__ resync, int n_res,
-                                                                double* ak, double* bk) {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n_cells * n_freq) return;
-    const long long cell = idx / n_freq;
-    const int k = (int)(idx - cell * n_freq);
-    const int flag = flags[cell];
-    if (flag == -1 || flag == 0) {
-        ak[idx] = (double)flag;
-        bk[idx] = (double)flag;
-        return;
-    }
+                                                                double* __restrict__ ak, double* __restrict__ bk) {
+    const long long n_groups = (n_cells + UC_DFT_C - 1) / UC_DFT_C;
+    const int kchunks = (n_freq + 63) >> 6;
+    const long long wave = __builtin_amdgcn_readfirstlane(
+        (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const long long g = wave / kchunks;
+    if (g >= n_groups) return;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    const int k_raw = (int)(wave - g * kchunks) * 64 + lane;
+    const int k = min(k_raw, n_freq - 1);  // spare lanes repeat the last frequency and store nothing
+    const long long c0 = g * UC_DFT_C;
+    const int nc = (int)min((long long)UC_DFT_C, n_cells - c0);
     const double ct = tab[2 * k], st = tab[2 * k + 1];
-    double a = 0.0, b = 0.0, cn = 1.0, sn = 0.0;
+    double a[UC_DFT_C], b[UC_DFT_C];
+#pragma unroll
+    for (int c = 0; c < UC_DFT_C; ++c) a[c] = b[c] = 0.0;
+    double cn = 1.0, sn = 0.0;
     int r = 0;
     // Samples in blocks of UC_DFT_U, all loads of a block issued before its
     // arithmetic. UC_DFT_U divides 1000, so a resync (after sample n with
     // (n + 1) % 1000 == 0 and n + 1 < N) can only fall on a block's last
-    // sample; the per-sample operations and their order are unchanged.
-    const double* x = traj + cell;
+    // sample.
+    const double* x = traj + c0;
     for (int n0 = 0; n0 < N; n0 += UC_DFT_U) {
-        double xv[UC_DFT_U], wv[UC_DFT_U];
+        double xv[UC_DFT_U][UC_DFT_C], wv[UC_DFT_U];
 #pragma unroll
         for (int u = 0; u < UC_DFT_U; ++u) {
-            const bool in = n0 + u < N;
-            xv[u] = in ? x[(long long)(n0 + u) * stride] : 0.0;
-            wv[u] = in ? window[n0 + u] : 0.0;
+            wv[u] = window[n0 + u];
+#pragma unroll
+            for (int c = 0; c < UC_DFT_C; ++c) xv[u][c] = x[(long long)(n0 + u) * stride + c];
         }
 #pragma unroll
         for (int u = 0; u < UC_DFT_U; ++u) {
             if (n0 + u < N) {
-                const double wd = wv[u] * xv[u];
-                a = a + wd * cn;
-                b = b + wd * sn;
-                const double nc = cn * ct - sn * st;
-                const double ns = sn * ct + cn * st;
-                cn = nc;
-                sn = ns;
+#pragma unroll
+                for (int c = 0; c < UC_DFT_C; ++c) {
+                    const double wd = wv[u] * xv[u][c];
+                    a[c] = a[c] + wd * cn;
+                    b[c] = b[c] + wd * sn;
+                }
+                const double nc_ = cn * ct - sn * st;
+                const double ns_ = sn * ct + cn * st;
+                cn = nc_;
+                sn = ns_;
             }
         }
         if ((n0 + UC_DFT_U) % 1000 == 0 && n0 + UC_DFT_U < N) {
@@ -1114,8 +1134,22 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __re
             ++r;
         }
     }
-    ak[idx] = a / (double)N;
-    bk[idx] = b / (double)N;
+    if (k_raw >= n_freq) return;
+#pragma unroll
+    for (int c = 0; c < UC_DFT_C; ++c) {
+        if (c < nc) {
+            const long long cell = c0 + c;
+            const int flag = flags[cell];
+            const long long o = cell * n_freq + k;
+            if (flag == -1 || flag == 0) {
+                ak[o] = (double)flag;
+                bk[o] = (double)flag;
+            } else {
+                ak[o] = a[c] / (double)N;
+                bk[o] = b[c] / (double)N;
+            }
+        }
+    }
 }
 
 // Zeroes the work counters of a batch (a kernel, not a memset, so the reset
