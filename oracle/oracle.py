@@ -66,6 +66,7 @@ def lib():
         L.oc_dft_block.argtypes = [_D, C.c_int, _D, C.c_int, C.c_double, C.c_double, C.c_int, C.c_double, _D, _D]
         L.oc_dft_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, C.c_int, C.c_double, C.c_double, C.c_int,
                                    C.c_int, _D, _D, _I, C.c_int]
+        L.oc_dft_hsweep_cells.argtypes = L.oc_dft_cells.argtypes
         L.oc_cgs.argtypes = [_D, _D, C.c_int, _D]
         L.oc_mgs.argtypes = [_D, C.c_int, C.c_int, _D]
         L.oc_rk4.argtypes = [_P, _D, _D, C.c_double, C.c_int]
@@ -399,6 +400,19 @@ def dft_cells(step, job, cells, n_freq, f_lo, f_hi, log_axis=False, window_type=
     lib().oc_dft_cells(step.ptr, C.byref(job), cp, n, int(n_freq), float(f_lo), float(f_hi), int(log_axis),
                        int(window_type), ak.ctypes.data_as(_D), bk.ctypes.data_as(_D), flags.ctypes.data_as(_I),
                        int(threads))
+    return dict(ak=ak, bk=bk, flags=flags)
+
+
+def dft_hsweep_cells(step, job, cells, n_freq, f_lo, f_hi, log_axis=False, window_type=1, threads=1):
+    """1-D DFT sweep over h (per-cell block length, window and dt): ak, bk
+    [cells, n_freq] and flags."""
+    cells, cp = _cells(cells)
+    n = len(cells)
+    ak, bk = np.zeros((n, n_freq)), np.zeros((n, n_freq))
+    flags = np.zeros(n, dtype=np.int32)
+    lib().oc_dft_hsweep_cells(step.ptr, C.byref(job), cp, n, int(n_freq), float(f_lo), float(f_hi), int(log_axis),
+                              int(window_type), ak.ctypes.data_as(_D), bk.ctypes.data_as(_D),
+                              flags.ctypes.data_as(_I), int(threads))
     return dict(ak=ak, bk=bk, flags=flags)
 
 

@@ -517,6 +517,95 @@ void oc_dft_cells(oc_step_fn step, const oc_job* j, const long long* cells, long
     free(w);
 }
 
+/* DFT under an h sweep (run_dft1d_cpu with sweep_over_h, no continuation,
+ * U-CUDA/parametric_engine.cpp:1088-1225; its GPU twin dft1dHSweepKernel,
+ * kernels/dft1d_hsweep.template.cu:85-205): per cell h from the grid,
+ * block = (int)(t_max / h / pre) capped at ceil(t_max / min(h lo, h hi) /
+ * pre); h <= 0 or a block of <= 2 samples -> -1; transient of
+ * (int)(transient / h) single steps plus one trailing step, an escape there
+ * (flag 0) -> -1; the block's loop flag -1 / 0 fills -1 / 0; otherwise the
+ * block's DFT (cpu_dft_block, :811-851) with the window of the block's own
+ * length (cpu_build_window, :794-803), dt = h * pre and the frequency axis
+ * f_k = lo + k (hi - lo) / (n - 1), log: 10^(l0 + (l1 - l0) k / (n - 1))
+ * (a single frequency sits at lo). */
+static void dft_block_cpu(const double* x, int N, const double* w, int n_freq, double f_lo, double f_hi, int log_axis,
+                          double dt, double* ak, double* bk) {
+    const double f_step = n_freq > 1 ? (f_hi - f_lo) / (double)(n_freq - 1) : 0.0;
+    const double psi = 2.0 * OC_PI * dt;
+    for (int k = 0; k < n_freq; ++k) {
+        const double f_k = log_axis
+            ? pow(10.0, log10(f_lo) + (n_freq > 1 ? (log10(f_hi) - log10(f_lo)) * (double)k / (double)(n_freq - 1) : 0.0))
+            : f_lo + (double)k * f_step;
+        const double c_t = cos(2.0 * OC_PI * dt * f_k), s_t = sin(2.0 * OC_PI * dt * f_k);
+        double a = 0.0, b = 0.0, c_n = 1.0, s_n = 0.0;
+        int reset = 1000;
+        for (int n = 0; n < N; ++n) {
+            const double wd = w[n] * x[n];
+            a = a + wd * c_n;
+            b = b + wd * s_n;
+            const double nc = c_n * c_t - s_n * s_t;
+            const double ns = s_n * c_t + c_n * s_t;
+            c_n = nc;
+            s_n = ns;
+            if (--reset == 0) {
+                reset = 1000;
+                if (n + 1 < N) {
+                    const double exact = psi * f_k * (double)(n + 1);
+                    c_n = (double)cosf((float)exact);
+                    s_n = (double)sinf((float)exact);
+                }
+            }
+        }
+        ak[k] = a / (double)N;
+        bk[k] = b / (double)N;
+    }
+}
+
+void oc_dft_hsweep_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int n_freq,
+                         double f_lo, double f_hi, int log_axis, int window_type, double* ak, double* bk, int* flags,
+                         int threads) {
+    int sax = 0;
+    for (int ax = 0; ax < j->n_axes; ++ax)
+        if (j->axis_kind[ax] == OC_AXIS_STEP) sax = ax;
+    const double worst = j->axis_lo[sax] < j->axis_hi[sax] ? j->axis_lo[sax] : j->axis_hi[sax];
+    const long long max_n = (long long)ceil(j->t_max / worst / (double)j->pre_scaller);
+#pragma omp parallel num_threads(threads > 1 ? threads : 1)
+    {
+        double* data = (double*)malloc(sizeof(double) * (size_t)(max_n > 0 ? max_n : 1));
+        double* w = (double*)malloc(sizeof(double) * (size_t)(max_n > 0 ? max_n : 1));
+#pragma omp for schedule(dynamic, 1)
+        for (long long s = 0; s < n_sel; ++s) {
+            double a[OC_MAXV], x[OC_MAXN], h;
+            double* A = ak + s * n_freq;
+            double* B = bk + s * n_freq;
+            int flag = -1;
+            long long n = 0;
+            if (oc_cell_setup(j, cells[s], a, x, &h)) {
+                n = (long long)(j->t_max / h / (double)j->pre_scaller);
+                if (n > max_n) n = max_n;
+                if (n > 2) {
+                    flag = oc_loop_model(step, x, a, h, (long long)(j->transient_time / h), j->n_vars, 1, 0,
+                                         j->max_value, NULL);
+                    if (flag == 0)
+                        flag = -1;
+                    else
+                        flag = oc_loop_model(step, x, a, h, n, j->n_vars, j->pre_scaller, j->writable_var,
+                                             j->max_value, data);
+                }
+            }
+            flags[s] = flag;
+            if (flag == -1 || flag == 0) {
+                for (int k = 0; k < n_freq; ++k) A[k] = B[k] = (double)flag;
+                continue;
+            }
+            oc_dft_window(window_type, (int)n, w);
+            dft_block_cpu(data, (int)n, w, n_freq, f_lo, f_hi, log_axis, h * (double)j->pre_scaller, A, B);
+        }
+        free(data);
+        free(w);
+    }
+}
+
 static void lyap_counts(const oc_job* j, double h, long long* skip, long long* nts) {
     if (sweeps_step(j)) {
         *nts = (long long)ceil(j->NT / h);

@@ -68,6 +68,10 @@ void oc_dft_block(const double* x, int N, const double* w, int n_freq, double f_
                   double dt, double* ak, double* bk);
 void oc_dft_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int n_freq, double f_lo,
                   double f_hi, int log_axis, int window_type, double* ak, double* bk, int* flags, int threads);
+/* the same under an h sweep (per-cell block length, window and dt) */
+void oc_dft_hsweep_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int n_freq,
+                         double f_lo, double f_hi, int log_axis, int window_type, double* ak, double* bk, int* flags,
+                         int threads);
 
 /* Gram-Schmidt */
 void oc_cgs(const double* a, double* b, int n, double* den);

@@ -29,6 +29,17 @@ def test_library_exports_every_declared_symbol():
     assert set(decl) == set(ucuda.exported_symbols())
 
 
+def test_no_fused_sincos_in_host_tables():
+    """The DFT rotation / resync tables (engine) and their restatement
+    (oracle) must call the C library's sin and cos separately: glibc's sincos
+    rounds differently from cos on ~0.1% of arguments, so a compiler that
+    fuses the pair in one library and not the other breaks bit-exact parity."""
+    for path in (ucuda.LIB_PATH, os.path.join(ROOT, "oracle", "liboracle.so")):
+        out = subprocess.run(["nm", "-D", "--undefined-only", path], capture_output=True, text=True, check=True)
+        imported = set(line.split()[-1].split("@")[0] for line in out.stdout.splitlines() if line.strip())
+        assert not imported & {"sincos", "sincosf"}, (path, imported & {"sincos", "sincosf"})
+
+
 def test_compile_only_engine_and_errors(cpu_engine):
     s = ucuda.preset("lorenz", cpu_engine)
     axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 8, 0.0, 10.0)]

@@ -358,6 +358,30 @@ def test_bif2d_periods(gpu_engine):
     assert ((g32 <= 0) == (r <= 0)).mean() > 0.95
 
 
+def test_dft_hsweep(gpu_engine):
+    print("case: DFT over h (Lorenz, 40 steps in [0.002, 0.06], per-cell blocks), strict vs oracle", flush=True)
+    s = ucuda.preset("lorenz", gpu_engine)
+    step = oracle.KrsStep(gpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
+    vals = s.values()
+    for axes, kw, pre in (([ucuda.axis(ucuda.AXIS_STEP, 0, 40, 0.002, 0.06)],
+                           dict(n_freq=48, freq_lo=0.0, freq_hi=4.0, window_type=1), 1),
+                          ([ucuda.axis(ucuda.AXIS_STEP, 0, 24, 0.004, 0.03, True),
+                            ucuda.axis(ucuda.AXIS_PARAM, 2, 3, 20.0, 160.0)],
+                           dict(n_freq=70, freq_lo=0.05, freq_hi=6.0, freq_log_scale=True, window_type=2), 2)):
+        g = ucuda.run_dft(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 5.0, 12.0, strict=True, pre_scaller=pre, **kw)
+        job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], vals, 0.01, 5.0, 12.0, pre_scaller=pre)
+        n = ucuda.n_cells(axes)
+        r = oracle.dft_hsweep_cells(step, job, range(n), kw["n_freq"], kw["freq_lo"], kw["freq_hi"],
+                                    log_axis=kw.get("freq_log_scale", False), window_type=kw["window_type"], threads=8)
+        np.testing.assert_array_equal(g["flags"].cpu().numpy(), r["flags"])
+        assert (r["flags"] == 1).sum() >= n // 2
+        np.testing.assert_array_equal(g["ak"].cpu().numpy(), r["ak"])
+        np.testing.assert_array_equal(g["bk"].cpu().numpy(), r["bk"])
+    g3 = ucuda.run_dft(s, [ucuda.axis(ucuda.AXIS_STEP, 0, 40, 0.002, 0.06)], [0.1, 0.1, 0.1], vals, 0.01, 5.0, 12.0,
+                       precision=ucuda.PREC_F32C, n_freq=48, freq_lo=0.0, freq_hi=4.0)
+    print("F32C h sweep flags equal on %.2f of the cells" % (g3["flags"].cpu().numpy() == 1).mean())
+
+
 def _dbscan_cases(rng):
     """Peak sets for the clustering kernels: sizes around the 64-lane words
     and the 1024-peak wave / lane boundary, uniform values, levels on a grid

@@ -376,3 +376,34 @@ def test_dft_closed_forms():
     np.testing.assert_allclose(hann, hann[::-1], atol=1e-15)
     np.testing.assert_allclose([hamm[0], hamm[-1]], [0.07672, 0.07672], atol=1e-15)
     np.testing.assert_allclose(hamm, hamm[::-1], atol=1e-15)
+
+
+def test_dft_hsweep_pins(cpu_engine):
+    """DFT under an h sweep: x'' = -(2 pi)^2 x from (1, 0) is cos(2 pi t), so
+    every step of the sweep (each with its own block length, window and dt)
+    puts amplitude 1/2 at 1 Hz and ~0 at the other whole-cycle frequencies
+    of the 10 s block (rectangular window; the truncated block leaks
+    O(h / t_max)); a block of <= 2 samples is unusable (-1); and a one-point
+    h axis with pre_scaller 1 reproduces the fixed-h sweep bit for bit."""
+    w = 2.0 * np.pi
+    s = _linear_system(cpu_engine, np.array([[0.0, 1.0], [-w * w, 0.0]]))
+    step = oracle.KrsStep(cpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 2, 1)
+    axes = [ucuda.axis(ucuda.AXIS_STEP, 0, 5, 0.004, 0.01)]
+    job = oracle.make_job(2, axes, [1.0, 0.0], [0.5], 0.01, 1.0, 10.0)
+    r = oracle.dft_hsweep_cells(step, job, range(5), 4, 0.5, 2.0, window_type=0)
+    assert (r["flags"] == 1).all()
+    amp = np.hypot(r["ak"], r["bk"])
+    np.testing.assert_allclose(amp[:, 1], 0.5, atol=5e-3)
+    assert amp[:, [0, 2, 3]].max() < 5e-3
+    # h = 0.2 (50 samples), 2.6 (3 samples; RK4 unstable at omega h = 16: escapes, 0), 5 (2 samples: -1)
+    job2 = oracle.make_job(2, [ucuda.axis(ucuda.AXIS_STEP, 0, 3, 0.2, 5.0)], [1.0, 0.0], [0.5], 0.01, 1.0, 10.0)
+    r2 = oracle.dft_hsweep_cells(step, job2, range(3), 4, 0.5, 2.0)
+    assert r2["flags"].tolist() == [1, 0, -1]
+    assert (r2["ak"][1] == 0).all() and (r2["ak"][2] == -1).all() and (r2["bk"][2] == -1).all()
+    h = 0.0075
+    jh = oracle.make_job(2, [ucuda.axis(ucuda.AXIS_STEP, 0, 1, h, h)], [1.0, 0.0], [0.5], 0.01, 1.0, 10.0)
+    jc = oracle.make_job(2, [ucuda.axis(ucuda.AXIS_VAR, 0, 1, 1.0, 1.0)], [1.0, 0.0], [0.5], h, 1.0, 10.0)
+    rh = oracle.dft_hsweep_cells(step, jh, [0], 6, 0.2, 3.0, window_type=1)
+    rc = oracle.dft_cells(step, jc, [0], 6, 0.2, 3.0, window_type=1)
+    np.testing.assert_array_equal(rh["ak"], rc["ak"])
+    np.testing.assert_array_equal(rh["bk"], rc["bk"])

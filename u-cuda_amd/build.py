@@ -40,8 +40,12 @@ def write_embed():
 
 def build_library(verbose=False):
     write_embed()
+    # host sin / cos stay separate calls (a fused sincos rounds differently
+    # from cos on some arguments; the DFT tables must match the C library's
+    # sin and cos)
     cmd = [os.path.join(ROCM, "bin", "hipcc"), "-O2", "-std=c++17", "-fPIC", "-shared",
            "-Wall", "-Wno-unused-function",
+           "-fno-builtin-sin", "-fno-builtin-cos", "-fno-builtin-sinf", "-fno-builtin-cosf",
            "-o", LIB + ".tmp",
            os.path.join(CSRC, "engine.cpp"), os.path.join(CSRC, "codegen.cpp"),
            os.path.join(CSRC, "parametric_engine.cpp"), os.path.join(CSRC, "data_export.cpp"),

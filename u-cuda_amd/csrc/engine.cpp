@@ -904,25 +904,182 @@ int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, doub
 static const int kDftCells = 4;    // UC_DFT_C in the kernel source: cells per DFT lane
 static const int kDftSamples = 4;  // UC_DFT_U: samples per load block (traj / window rows padded to it)
 
+// DFT under an h sweep (run_dft1d_hsweep_gpu,
+// U-CUDA/parametric_engine.cpp:4272-4520, and its kernel
+// dft1d_hsweep.template.cu): every cell has its own step, block length
+// (int)(t_max / h / pre) capped at ceil(t_max / min(h lo, h hi) / pre),
+// window and sampling interval. The recording kernel runs with raw_flags 2;
+// the host computes each cell's window (one per distinct block length),
+// rotation (cos, sin)(2 pi h pre f_k) and float-precision resync values with
+// the formulas of cpu_dft_block / cpu_build_window, and uc_dft_cells does
+// the sums.
+static int run_dft_hsweep(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft_output* o, int step_ax,
+                          hipStream_t st) {
+    const uc_common& c = r->c;
+    const uc_axis& sa = c.axis[step_ax];
+    if (!(sa.lo > 0.0 && sa.hi > 0.0)) return e->fail(UC_ERR_ARGUMENT, "h lo / hi must be > 0 for an h sweep");
+    if (r->n_freq < 1) return e->fail(UC_ERR_ARGUMENT, "n_freq must be > 0");
+    if (!(r->freq_hi > r->freq_lo)) return e->fail(UC_ERR_ARGUMENT, "freq_hi must be > freq_lo");
+    const double max_d = std::ceil(c.t_max / std::min(sa.lo, sa.hi) / (double)r->pre_scaller);
+    if (!(max_d >= 1.0)) return e->fail(UC_ERR_ARGUMENT, "block of t_max / h / pre_scaller samples is empty");
+    if (max_d > 2.0e9) return e->fail(UC_ERR_ARGUMENT, "h lo too small: block longer than 2e9 samples");
+    const long long max_n = (long long)max_d;
+    const int nf = r->n_freq;
+    const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
+    const long long total = (long long)nx * ny;
+    std::vector<double> hv((size_t)sa.n_pts);
+    axis_values(sa, hv.data());
+
+    // per-cell block length, windows (shared by equal lengths), rotation and
+    // resync tables
+    const double kPi = 3.14159265358979323846;
+    std::vector<int> Ns((size_t)total);
+    std::vector<long long> win_off((size_t)total, 0), res_off((size_t)total, 0);
+    std::vector<double> win, tab(2 * (size_t)total * (size_t)nf), res;
+    std::map<long long, long long> win_of_len;
+    std::vector<double> fk((size_t)nf);
+    for (int k = 0; k < nf; ++k) {
+        if (r->freq_log_scale) {
+            const double l0 = std::log10(r->freq_lo), l1 = std::log10(r->freq_hi);
+            fk[(size_t)k] = std::pow(10.0, l0 + (nf > 1 ? (l1 - l0) * (double)k / (double)(nf - 1) : 0.0));
+        } else {
+            const double f_step = nf > 1 ? (r->freq_hi - r->freq_lo) / (double)(nf - 1) : 0.0;
+            fk[(size_t)k] = r->freq_lo + (double)k * f_step;
+        }
+    }
+    for (long long cell = 0; cell < total; ++cell) {
+        const double h = hv[(size_t)(step_ax == 0 ? cell % nx : (cell / nx) % ny)];
+        long long n = h > 0.0 ? (long long)(c.t_max / h / (double)r->pre_scaller) : 0;
+        if (n > max_n) n = max_n;
+        Ns[(size_t)cell] = (int)n;
+        if (n <= 2) continue;  // unusable: the recording kernel flags it -1
+        auto it = win_of_len.find(n);
+        if (it == win_of_len.end()) {
+            const long long off = (long long)win.size();
+            const long long padded = (n + kDftSamples - 1) / kDftSamples * kDftSamples;
+            win.resize((size_t)(off + padded), 0.0);
+            if (r->window_type == 0) {
+                std::fill(win.begin() + off, win.begin() + off + n, 1.0);
+            } else {
+                const double gamma = 2.0 * kPi / (double)(n - 1);
+                for (long long q = 0; q < n; ++q)
+                    win[(size_t)(off + q)] = r->window_type == 2 ? 0.53836 - 0.46164 * std::cos(gamma * (double)q)
+                                                                 : 0.5 * (1.0 - std::cos(gamma * (double)q));
+            }
+            it = win_of_len.emplace(n, off).first;
+        }
+        win_off[(size_t)cell] = it->second;
+        const double dt = h * (double)r->pre_scaller;
+        const double psi = 2.0 * kPi * dt;
+        const int n_res = (int)((n - 1) / 1000);
+        res_off[(size_t)cell] = (long long)res.size();
+        res.resize(res.size() + 2 * (size_t)nf * (size_t)n_res);
+        double* rs = res.data() + res_off[(size_t)cell];
+        for (int k = 0; k < nf; ++k) {
+            const double f = fk[(size_t)k];
+            tab[2 * ((size_t)cell * nf + k)] = std::cos(2.0 * kPi * dt * f);
+            tab[2 * ((size_t)cell * nf + k) + 1] = std::sin(2.0 * kPi * dt * f);
+            for (int q = 0; q < n_res; ++q) {
+                const double exact = psi * f * (double)(1000LL * (q + 1));
+                rs[2 * ((size_t)k * n_res + q)] = (double)std::cos((float)exact);
+                rs[2 * ((size_t)k * n_res + q) + 1] = (double)std::sin((float)exact);
+            }
+        }
+    }
+    if (win.empty()) win.assign(kDftSamples, 0.0);
+    if (res.empty()) res.assign(2, 0.0);
+
+    int rc = enter(e);
+    if (rc) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    if ((rc = stage_tables(e, s, c, j, st, &tb))) return rc;
+    const long long rows = (max_n + kDftSamples - 1) / kDftSamples * kDftSamples;
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    const long long chunk =
+        (long long)std::min<size_t>((size_t)total, std::max<size_t>(kBlock, free_b / 4 / ((size_t)rows * sizeof(double))));
+    double *traj = nullptr, *dwin = nullptr, *dtab = nullptr, *dres = nullptr;
+    int* dN = nullptr;
+    long long *dwo = nullptr, *dro = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(st);
+        for (void* p : {(void*)traj, (void*)dwin, (void*)dtab, (void*)dres, (void*)dN, (void*)dwo, (void*)dro})
+            if (p) (void)hipFree(p);
+    };
+    auto up = [&](void** d, const void* h, size_t bytes) {
+        hipError_t he = hipMalloc(d, bytes);
+        if (he == hipSuccess) he = hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, st);
+        return he;
+    };
+    hipError_t he = hipMalloc((void**)&traj, sizeof(double) * (size_t)rows * (size_t)chunk);
+    if (he == hipSuccess) he = up((void**)&dwin, win.data(), sizeof(double) * win.size());
+    if (he == hipSuccess) he = up((void**)&dtab, tab.data(), sizeof(double) * tab.size());
+    if (he == hipSuccess) he = up((void**)&dres, res.data(), sizeof(double) * res.size());
+    if (he == hipSuccess) he = up((void**)&dN, Ns.data(), sizeof(int) * Ns.size());
+    if (he == hipSuccess) he = up((void**)&dwo, win_off.data(), sizeof(long long) * win_off.size());
+    if (he == hipSuccess) he = up((void**)&dro, res_off.data(), sizeof(long long) * res_off.size());
+    if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "DFT h-sweep buffers")); }
+    hipFunction_t fdft;
+    he = hipModuleGetFunction(&fdft, m->mod, "uc_dft_cells");
+    if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_dft_cells")); }
+    const char* kname = c.precision == UC_PREC_F64 ? "uc_bif_f64" : "uc_bif_f32c";
+    for (long long off = 0; off < total; off += chunk) {
+        long long mcells = std::min(chunk, total - off);
+        UcJob jc = j;
+        jc.cell_offset = off;
+        jc.n_cells = mcells;
+        jc.stride = mcells;
+        jc.n_iters = max_n;
+        jc.pre_scaller = r->pre_scaller;
+        jc.writable_var = r->writable_var;
+        jc.max_peaks = 0;
+        jc.raw_flags = 2;
+        jc.flag = o->flags + off;
+        jc.traj = traj;
+        jc.traj_len = 0;
+        if ((rc = launch(e, m, kname, jc, st, tb))) { release(); return rc; }
+        long long stride = mcells;
+        int nfa = nf;
+        const int* Np = dN + off;
+        int* fl = o->flags + off;
+        const long long* wo = dwo + off;
+        const double* tp = dtab + 2 * off * nf;
+        const long long* ro = dro + off;
+        double* A = o->ak_cos + off * nf;
+        double* B = o->bk_sin + off * nf;
+        void* args[] = {&traj, &stride, &mcells, &Np, &fl, &dwin, &wo, &nfa, &tp, &dres, &ro, &A, &B};
+        const long long lanes = mcells * ((nf + 63) / 64) * 64;
+        he = hipModuleLaunchKernel(fdft, (unsigned)((lanes + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, args,
+                                   nullptr);
+        if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_dft_cells launch")); }
+    }
+    release();
+    return UC_OK;
+}
+
 // 1-D DFT sweep: the bifurcation kernel records each cell's block (loop
 // flag kept raw), the DFT kernel turns it into n_freq coefficients. The
 // window, the per-frequency rotation and the float-precision resync values
 // are computed here with the formulas of run_dft1d_classical / DFT_custom,
-// so the device work is multiply-add only.
+// so the device work is multiply-add only. An axis over h takes the h-sweep
+// path above.
 int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft_output* o, void* stream) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     int rc = validate_common(e, s, r->c, true);
     if (rc) return rc;
-    for (int i = 0; i < r->c.n_axes; ++i)
-        if (r->c.axis[i].kind == UC_AXIS_STEP) return e->fail(UC_ERR_ARGUMENT, "DFT sweeps parameters or initial conditions, not h");
     if (!o->ak_cos || !o->bk_sin || !o->flags) return e->fail(UC_ERR_ARGUMENT, "DFT outputs missing");
     if (r->writable_var < -1 || r->writable_var >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "writable_var out of range");
     if (r->pre_scaller <= 0) return e->fail(UC_ERR_ARGUMENT, "pre_scaller must be > 0");
-    if (r->n_freq < 2) return e->fail(UC_ERR_ARGUMENT, "n_freq must be >= 2");
     if (r->window_type < 0 || r->window_type > 2) return e->fail(UC_ERR_ARGUMENT, "window_type must be 0, 1 or 2");
     if (r->freq_log_scale && !(r->freq_lo > 0.0 && r->freq_hi > 0.0))
         return e->fail(UC_ERR_ARGUMENT, "log frequency axis needs freq_lo, freq_hi > 0");
+    for (int i = 0; i < r->c.n_axes; ++i)
+        if (r->c.axis[i].kind == UC_AXIS_STEP) return run_dft_hsweep(e, s, r, o, i, (hipStream_t)stream);
+    if (r->n_freq < 2) return e->fail(UC_ERR_ARGUMENT, "n_freq must be >= 2");
     if (!(r->c.h > 0.0)) return e->fail(UC_ERR_ARGUMENT, "h must be > 0");
     const long long N = (long long)(r->c.t_max / r->c.h / (double)r->pre_scaller);
     if (N < 2) return e->fail(UC_ERR_ARGUMENT, "block of t_max / h / pre_scaller samples must hold >= 2");

@@ -559,6 +559,25 @@ __device__ void uc_bif_cell(const UcJob& j, long long lane) {
 
     NoObserver none;
     double t = 0.0;
+    if (j.raw_flags == 2) {
+        // DFT block under an h sweep (dft1dHSweepKernel,
+        // U-CUDA/kernels/dft1d_hsweep.template.cu:125-168): block of
+        // (int)(t_max / h / pre) samples capped at n_iters, unusable (-1)
+        // when it holds <= 2; transient of (int)(transient / h) single steps
+        // plus one trailing step, an escape there (0) marks the cell -1; the
+        // block's own loop flag otherwise.
+        long long blk = (long long)(j.t_max / hd / (double)j.pre_scaller);
+        if (blk > j.n_iters) blk = j.n_iters;
+        if (blk <= 2) { j.flag[lane] = -1; return; }
+        int f = uc_loop(s, a, h, t, (long long)(j.transient_time / hd), 1, 0, j.max_value, none);
+        if (f == 0) { j.flag[lane] = -1; return; }
+        PeakObserver rec;
+        rec.w0 = 0.0; rec.w1 = 0.0; rec.n = blk; rec.c_next = 2; rec.mode = 2; rec.raw = 0;
+        rec.last_t = 0.0; rec.time_step = 0.0; rec.max_peaks = 0; rec.stride = j.stride;
+        rec.peaks = nullptr; rec.intervals = nullptr; rec.traj = j.traj + lane; rec.traj_len = (int)blk;
+        j.flag[lane] = uc_loop(s, a, h, t, blk, j.pre_scaller, j.writable_var, j.max_value, rec);
+        return;
+    }
     const bool uni = !uc_sweeps_step(j);  // step counts are wave-uniform unless the sweep is over h
     int flag = uni ? uc_loop_u(s, a, h, t, j.skip_steps, j.pre_scaller, 0, j.max_value, none)
                    : uc_loop(s, a, h, t, skip, j.pre_scaller, 0, j.max_value, none);
@@ -1150,6 +1169,73 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __re
             }
         }
     }
+}
+
+// DFT of recorded blocks of per-cell length (the DFT under an h sweep,
+// U-CUDA/kernels/dft1d_hsweep.template.cu:168-204): the arithmetic of
+// uc_dft with everything per cell: N[cell] samples, its window at
+// win[win_off[cell]], rotation tab[2 (cell * n_freq + k)] and resync values
+// res[res_off[cell] + 2 (k * n_res + r)] with n_res = (N - 1) / 1000. One
+// wave per (cell, 64 frequencies); samples and window through scalar loads.
+// Loads are unguarded: traj holds ceil(max N / UC_DFT_U) * UC_DFT_U rows and
+// each window row is padded to a multiple of UC_DFT_U.
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft_cells(
+    const double* __restrict__ traj, long long stride, long long n_cells, const int* __restrict__ Ns,
+    const int* __restrict__ flags, const double* __restrict__ win, const long long* __restrict__ win_off, int n_freq,
+    const double* __restrict__ tab, const double* __restrict__ res, const long long* __restrict__ res_off,
+    double* __restrict__ ak, double* __restrict__ bk) {
+    const int kchunks = (n_freq + 63) >> 6;
+    const long long wave = __builtin_amdgcn_readfirstlane(
+        (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+    const long long cell = wave / kchunks;
+    if (cell >= n_cells) return;  // wave-uniform
+    const int lane = threadIdx.x & 63;
+    const int k_raw = (int)(wave - cell * kchunks) * 64 + lane;
+    const int k = min(k_raw, n_freq - 1);
+    const int flag = flags[cell];
+    if (flag == -1 || flag == 0) {
+        if (k_raw < n_freq) {
+            ak[cell * n_freq + k] = (double)flag;
+            bk[cell * n_freq + k] = (double)flag;
+        }
+        return;
+    }
+    const int N = Ns[cell];
+    const int n_res = (N - 1) / 1000;
+    const double ct = tab[2 * (cell * n_freq + k)], st = tab[2 * (cell * n_freq + k) + 1];
+    const double* w = win + win_off[cell];
+    const double* x = traj + cell;
+    const double* rs = res + res_off[cell] + 2LL * k * n_res;
+    double a = 0.0, b = 0.0, cn = 1.0, sn = 0.0;
+    int r = 0;
+    for (int n0 = 0; n0 < N; n0 += UC_DFT_U) {
+        double xv[UC_DFT_U], wv[UC_DFT_U];
+#pragma unroll
+        for (int u = 0; u < UC_DFT_U; ++u) {
+            wv[u] = w[n0 + u];
+            xv[u] = x[(long long)(n0 + u) * stride];
+        }
+#pragma unroll
+        for (int u = 0; u < UC_DFT_U; ++u) {
+            if (n0 + u < N) {
+                const double wd = wv[u] * xv[u];
+                a = a + wd * cn;
+                b = b + wd * sn;
+                const double nc_ = cn * ct - sn * st;
+                const double ns_ = sn * ct + cn * st;
+                cn = nc_;
+                sn = ns_;
+            }
+        }
+        if ((n0 + UC_DFT_U) % 1000 == 0 && n0 + UC_DFT_U < N) {
+            cn = rs[2 * r];
+            sn = rs[2 * r + 1];
+            ++r;
+        }
+    }
+    if (k_raw >= n_freq) return;
+    ak[cell * n_freq + k] = a / (double)N;
+    bk[cell * n_freq + k] = b / (double)N;
 }
 
 // Zeroes the work counters of a batch (a kernel, not a memset, so the reset

@@ -283,7 +283,6 @@ Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
     if (req.continuation || req.use_cpu) { res.error = "continuation / CPU sweeps are not part of this engine"; return res; }
-    if (req.sweep_over_h) { res.error = "the DFT sweep runs over parameters or initial conditions, not h"; return res; }
     if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
     uc_system* s = impl_->system(req.system, res.error);
     if (!s) return res;
@@ -291,8 +290,8 @@ Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
     impl_->common(r.c, req.initial_conditions, req.base_values, req.h, req.transient_time, req.t_max, req.max_value,
                   req.precision);
     r.c.n_axes = 1;
-    r.c.axis[0] = make_axis(false, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts, req.param_lo,
-                            req.param_hi, req.log_scale);
+    r.c.axis[0] = make_axis(req.sweep_over_h, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts,
+                            req.param_lo, req.param_hi, req.log_scale);
     r.writable_var = req.writable_var;
     r.pre_scaller = req.pre_scaller;
     r.n_freq = req.n_freq;
@@ -317,7 +316,8 @@ Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
     if (!req.csv_output_path.empty()) {
         std::ofstream cfg(req.csv_output_path + "_config.csv");
         csv::dft1d_config(cfg, req.base_values, req.initial_conditions, req.t_max, req.transient_time, req.h,
-                          req.pre_scaller, req.writable_var, req.sweep_over_var ? req.var_sweep_index : req.param_index,
+                          req.pre_scaller, req.writable_var,
+                          req.sweep_over_h ? -1 : req.sweep_over_var ? req.var_sweep_index : req.param_index,
                           req.param_lo, req.param_hi, req.n_freq, req.freq_lo, req.freq_hi, req.window_type);
         std::ofstream a(req.csv_output_path + "_AkCOS.csv"), b(req.csv_output_path + "_BkSIN.csv");
         csv::dft1d_header(a, req.param_lo, req.param_hi, req.freq_lo, req.freq_hi);

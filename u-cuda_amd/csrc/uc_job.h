@@ -51,7 +51,9 @@ typedef struct UcJob {
     int writable_var;             /* bif: sampled variable (-1: x0+pi*x1+e*x2)*/
     int max_peaks;                /* bif: stored peaks per cell               */
     int traj_len;                 /* bif: samples copied to traj (0 = none)   */
-    int raw_flags;                /* bif: flag = loop flag (1/-1/0), no count */
+    int raw_flags;                /* bif: 0 flag = peak count; 1 flag = loop  */
+                                  /* flag (1/-1/0); 2 the same under the DFT  */
+                                  /* h-sweep rules (dft1dHSweepKernel)        */
 
     /* ---- adaptive integration / basins ---- */
     double rtol, atol;            /* Dormand-Prince 5(4) error control        */
