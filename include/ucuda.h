@@ -242,6 +242,61 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
 int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_samples, long long n_series,
                    double time_step, int max_peaks, int* counts, double* peaks, double* intervals, void* stream);
 
+/* Fast synchronisation (U-CUDA/parametric_engine.h:767-842 FastSyncRequest /
+ * FastSyncResult, run_fastsync parametric_engine.cpp:6833-7255): a slave copy
+ * of the system is pulled towards a master by N = K (master - slave), over a
+ * window forward with k_forward and back with k_backward, iter_of_synchr
+ * times. FP64; `strict` as elsewhere.
+ *  mode 0 (on attractor): the master is integrated from ic_master for
+ *    (int)(transient / h) steps, then kept for (int)(t_max / h) +
+ *    (int)(window / h) samples; point idx = 0 .. n_out - 1 with n_out =
+ *    (int)(t_max / h) / pre_scaller synchronises the slave (from ic_slave)
+ *    over the (int)(window / h) samples that start at sample idx *
+ *    pre_scaller. sync_error[n_out]; traj (optional) [n_out][n_vars] the
+ *    master at those starts.
+ *  mode 1 (on grid): n_pts x n_pts cells (iy * n_pts + ix) set state
+ *    variables axis_x_var = grid x, axis_y_var = grid y of the master
+ *    (grid_swap_master_slave 0) or of the slave (1); the other, fixed
+ *    initial state first runs (int)(transient / h) uncoupled steps; windows
+ *    of (int)(window / h / pre_scaller) samples. sync_error[n_pts * n_pts].
+ *  error_estim: 0 RMS over the last forward pass, 1 passes until the error
+ *  at the window's end is <= fs_error_trs, 2 error at the window's end after
+ *  the last pass, 3 time until the error is <= fs_error_trs; 0 / 2 report
+ *  log10 (fs_error_trs floor, 3 for errors >= 1e3 or non-finite).
+ *  type_of_synch: 0 unidirectional (stored master), 1 bidirectional. */
+typedef struct uc_fastsync_request {
+    const double* values;        /* a[] */
+    int n_values;
+    int mode;
+    double h;
+    int iter_of_synchr;
+    int pre_scaller;
+    const double* k_forward;     /* n_vars each */
+    const double* k_backward;
+    const double* ic_master;
+    const double* ic_slave;
+    double t_max;                /* mode 0 */
+    double transient_time;
+    double window;
+    int axis_x_var, axis_y_var;  /* mode 1 */
+    double axis_x_lo, axis_x_hi, axis_y_lo, axis_y_hi;
+    int n_pts;
+    int grid_swap_master_slave;
+    int type_of_synch;
+    int error_estim;
+    double fs_error_trs;
+    int strict;
+} uc_fastsync_request;
+typedef struct uc_fastsync_output {
+    double* sync_error;
+    double* traj;                /* mode 0 only, may be NULL */
+} uc_fastsync_output;
+/* Number of sync_error values a request produces (mode 0: (int)(t_max / h)
+ * / pre_scaller; mode 1: n_pts^2); < 0 for an invalid request. */
+long long uc_fastsync_points(const uc_fastsync_request* r);
+int uc_run_fastsync(uc_engine* e, uc_system* s, const uc_fastsync_request* r, const uc_fastsync_output* o,
+                    void* stream);
+
 /* Periods from stored peaks (the reference's dbscanCUDA over its peak
  * buffers, U-CUDA/cudaLibrary.cu:2321-2345, with dbscan :2088-2166): device
  * peaks[max_peaks][n_cells] (peak-major, as uc_run_bif writes them) and

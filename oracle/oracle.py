@@ -27,6 +27,13 @@ NUMB_BYTES = 8               # numb = double
 AXIS_NONE, AXIS_PARAM, AXIS_VAR, AXIS_STEP = 0, 1, 2, 3
 
 
+class OcFs(C.Structure):
+    _fields_ = [("n_vars", C.c_int), ("a", C.POINTER(C.c_double)), ("h", C.c_double),
+                ("iter_of_synchr", C.c_int), ("pre_scaller", C.c_int), ("type_of_synch", C.c_int),
+                ("error_estim", C.c_int), ("fs_error_trs", C.c_double), ("k_forward", C.POINTER(C.c_double)),
+                ("k_backward", C.POINTER(C.c_double)), ("n_iters", C.c_int)]
+
+
 class OcJob(C.Structure):
     _fields_ = [("n_vars", C.c_int), ("n_values", C.c_int),
                 ("ic", C.POINTER(C.c_double)), ("params", C.POINTER(C.c_double)),
@@ -67,6 +74,12 @@ def lib():
         L.oc_dft_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, C.c_int, C.c_double, C.c_double, C.c_int,
                                    C.c_int, _D, _D, _I, C.c_int]
         L.oc_dft_hsweep_cells.argtypes = L.oc_dft_cells.argtypes
+        L.oc_fs_attractor.argtypes = [_P, C.POINTER(OcFs), _D, _D, C.c_double, C.c_double, C.c_double, _LL,
+                                      C.c_longlong, _D, _D, C.c_int]
+        L.oc_fs_attractor.restype = C.c_longlong
+        L.oc_fs_grid.argtypes = [_P, C.POINTER(OcFs), _D, _D, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                 C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _LL, C.c_longlong, _D,
+                                 C.c_int]
         L.oc_cgs.argtypes = [_D, _D, C.c_int, _D]
         L.oc_mgs.argtypes = [_D, C.c_int, C.c_int, _D]
         L.oc_rk4.argtypes = [_P, _D, _D, C.c_double, C.c_int]
@@ -414,6 +427,56 @@ def dft_hsweep_cells(step, job, cells, n_freq, f_lo, f_hi, log_axis=False, windo
                               int(window_type), ak.ctypes.data_as(_D), bk.ctypes.data_as(_D),
                               flags.ctypes.data_as(_I), int(threads))
     return dict(ak=ak, bk=bk, flags=flags)
+
+
+def _fs(n_vars, values, h, iter_of_synchr, pre_scaller, type_of_synch, error_estim, fs_error_trs, kf, kb):
+    f = OcFs()
+    f._a = (C.c_double * max(1, len(values)))(*values)
+    f._kf = (C.c_double * n_vars)(*kf)
+    f._kb = (C.c_double * n_vars)(*kb)
+    f.n_vars, f.a, f.h = n_vars, f._a, h
+    f.iter_of_synchr, f.pre_scaller, f.type_of_synch, f.error_estim = iter_of_synchr, pre_scaller, type_of_synch, error_estim
+    f.fs_error_trs, f.k_forward, f.k_backward = fs_error_trs, f._kf, f._kb
+    return f
+
+
+def _dvec(v):
+    a = np.ascontiguousarray(np.asarray(v, dtype=np.float64))
+    return a, a.ctypes.data_as(_D)
+
+
+def fs_attractor(step, n_vars, values, ic_master, ic_slave, k_forward, k_backward, idx, h=0.01, iter_of_synchr=100,
+                 pre_scaller=1, t_max=100.0, transient_time=0.0, window=50.0, type_of_synch=0, error_estim=2,
+                 fs_error_trs=1e-12, threads=1):
+    """Fast synchronisation on the attractor: sync error and master point of
+    the selected window starts idx; returns (n_points, errors, traj)."""
+    f = _fs(n_vars, values, h, iter_of_synchr, pre_scaller, type_of_synch, error_estim, fs_error_trs, k_forward,
+            k_backward)
+    idx, ip = _cells(idx)
+    out = np.zeros(len(idx))
+    traj = np.zeros((len(idx), n_vars))
+    m, mp = _dvec(ic_master)
+    sl, sp = _dvec(ic_slave)
+    n = lib().oc_fs_attractor(step.ptr, C.byref(f), mp, sp, float(t_max), float(transient_time), float(window), ip,
+                              len(idx), out.ctypes.data_as(_D), traj.ctypes.data_as(_D), int(threads))
+    return n, out, traj
+
+
+def fs_grid(step, n_vars, values, ic_master, ic_slave, k_forward, k_backward, cells, h=0.01, iter_of_synchr=100,
+            pre_scaller=1, transient_time=0.0, window=50.0, axis_x_var=0, axis_y_var=1, axis_x=(-10.0, 10.0),
+            axis_y=(-10.0, 10.0), n_pts=200, grid_swap_master_slave=False, type_of_synch=0, error_estim=2,
+            fs_error_trs=1e-12, threads=1):
+    """Fast synchronisation on an initial-state grid: errors of the selected cells."""
+    f = _fs(n_vars, values, h, iter_of_synchr, pre_scaller, type_of_synch, error_estim, fs_error_trs, k_forward,
+            k_backward)
+    cells, cp = _cells(cells)
+    out = np.zeros(len(cells))
+    m, mp = _dvec(ic_master)
+    sl, sp = _dvec(ic_slave)
+    lib().oc_fs_grid(step.ptr, C.byref(f), mp, sp, float(transient_time), float(window), int(n_pts), int(axis_x_var),
+                     int(axis_y_var), float(axis_x[0]), float(axis_x[1]), float(axis_y[0]), float(axis_y[1]),
+                     int(grid_swap_master_slave), cp, len(cells), out.ctypes.data_as(_D), int(threads))
+    return out
 
 
 def peak_finder(data, h):

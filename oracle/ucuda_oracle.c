@@ -606,6 +606,155 @@ void oc_dft_hsweep_cells(oc_step_fn step, const oc_job* j, const long long* cell
     }
 }
 
+/* ------------------------------------------------------- fast synchro ---
+ * FastSynchro (U-CUDA/cudaLibrary.cu): the coupled step
+ * calculateDiscreteModelforFastSynchro (:424-452): N = K (S1 - X) from the
+ * state before the step, X <- step(X, h forward / -h backward), X += h N;
+ * the synchronisation loops loopCalculateDiscreteModelForFastSynchro_2
+ * (:547-720, on grid) and loopCalculateDiscreteModelForFastSynchro
+ * (:3630-3800, on attractor), which share the passes and estimators; the
+ * master fill fillFSMasterTrajectory (kernels/fastsync_attr.template.cu:
+ * 75-98) and the fixed-state transient fillFSTransientIC (:454-474), both
+ * with K = 0 and S1 = X; run_fastsync's counts
+ * (parametric_engine.cpp:6906-6916, 7075). */
+static void fs_step(oc_step_fn step, int n, double* X, const double* S1, const double* K, const double* a, double h,
+                    int fwd) {
+    double N[OC_MAXN];
+    for (int i = 0; i < n; ++i) N[i] = K[i] * (S1[i] - X[i]);
+    step(X, a, fwd ? h : -h);
+    for (int i = 0; i < n; ++i) X[i] = X[i] + h * N[i];
+}
+
+static double fs_sqdist(int n, const double* a, const double* b) {
+    double r = 0.0;
+    for (int j = 0; j < n; ++j) r = r + (a[j] - b[j]) * (a[j] - b[j]);
+    return r;
+}
+
+/* master: rows [i * n + j] for the unidirectional mode; Xm: initial master
+ * for the bidirectional one */
+static double fs_sync(oc_step_fn step, const oc_fs* f, const double* master, double* Xm, double* Xs) {
+    const int n = f->n_vars, N = f->n_iters;
+    const double h = f->h;
+    double rms = 0.0, prev[OC_MAXN];
+    for (int m = 0; m < f->iter_of_synchr; ++m) {
+        for (int i = 0; i < N - 1; ++i) {
+            if (f->type_of_synch == 0)
+                for (int j = 0; j < n; ++j) Xm[j] = master[(long long)i * n + j];
+            if (f->error_estim == 0) {
+                if (m == f->iter_of_synchr - 1) rms = rms + fs_sqdist(n, Xm, Xs);
+            } else if (f->error_estim == 1) {
+                if (i == N - 2) rms = fs_sqdist(n, Xm, Xs);
+            } else if (f->error_estim == 2) {
+                if (m == f->iter_of_synchr - 1 && i == N - 2) rms = fs_sqdist(n, Xm, Xs);
+            } else {
+                rms = sqrt(fs_sqdist(n, Xm, Xs));
+                if (rms <= f->fs_error_trs) return (double)(i + 1) * h;
+            }
+            if (f->type_of_synch == 1) memcpy(prev, Xs, sizeof(double) * (size_t)n);
+            fs_step(step, n, Xs, Xm, f->k_forward, f->a, h, 1);
+            if (f->type_of_synch == 1) fs_step(step, n, Xm, prev, f->k_forward, f->a, h, 1);
+        }
+        if (f->error_estim == 1) {
+            rms = sqrt(rms);
+            if (rms <= f->fs_error_trs) return (double)(m + 1);
+            rms = 0.0;
+        }
+        for (int i = N - 1; i > 0; --i) {
+            if (f->type_of_synch == 0)
+                for (int j = 0; j < n; ++j) Xm[j] = master[(long long)i * n + j];
+            if (f->type_of_synch == 1) memcpy(prev, Xs, sizeof(double) * (size_t)n);
+            fs_step(step, n, Xs, Xm, f->k_backward, f->a, h, 0);
+            if (f->type_of_synch == 1) fs_step(step, n, Xm, prev, f->k_backward, f->a, h, 0);
+        }
+    }
+    if (f->error_estim == 0) rms = sqrt(rms / (double)(N - 1));
+    if (f->error_estim == 2) rms = sqrt(rms);
+    rms = rms <= f->fs_error_trs ? log10(f->fs_error_trs) : log10(rms);
+    if (isinf(rms) || isnan(rms) || rms >= 3.0) rms = 3.0;
+    if (f->error_estim == 1) return (double)f->iter_of_synchr;
+    if (f->error_estim == 3) return (double)N * h;
+    return rms;
+}
+
+static void fs_free_self(oc_step_fn step, int n, double* X, const double* a, double h) {
+    double s1[OC_MAXN], z[OC_MAXN];
+    for (int i = 0; i < n; ++i) { s1[i] = X[i]; z[i] = 0.0; }
+    fs_step(step, n, X, s1, z, a, h, 1);
+}
+
+long long oc_fs_attractor(oc_step_fn step, oc_fs* f, const double* ic_master, const double* ic_slave, double t_max,
+                          double transient, double window, const long long* idx, long long n_sel, double* out,
+                          double* traj_out, int threads) {
+    const int n = f->n_vars;
+    const int nt = (int)(window / f->h), ct = (int)(t_max / f->h), skip = (int)(transient / f->h);
+    const long long n_pts = ct / f->pre_scaller, len = (long long)ct + nt;
+    double* master = (double*)malloc(sizeof(double) * (size_t)(len > 0 ? len : 1) * (size_t)n);
+    double X[OC_MAXN];
+    for (int j = 0; j < n; ++j) X[j] = ic_master[j];
+    for (int i = 0; i < skip; ++i) fs_free_self(step, n, X, f->a, f->h);
+    for (long long i = 0; i < len; ++i) {
+        for (int j = 0; j < n; ++j) master[i * n + j] = X[j];
+        fs_free_self(step, n, X, f->a, f->h);
+    }
+    f->n_iters = nt;
+#pragma omp parallel for num_threads(threads > 1 ? threads : 1) schedule(dynamic, 1)
+    for (long long s = 0; s < n_sel; ++s) {
+        const double* w = master + idx[s] * f->pre_scaller * n;
+        double Xm[OC_MAXN], Xs[OC_MAXN];
+        for (int j = 0; j < n; ++j) { Xs[j] = ic_slave[j]; Xm[j] = w[j]; }
+        if (traj_out)
+            for (int j = 0; j < n; ++j) traj_out[s * n + j] = w[j];
+        out[s] = fs_sync(step, f, w, Xm, Xs);
+    }
+    free(master);
+    return n_pts;
+}
+
+void oc_fs_grid(oc_step_fn step, oc_fs* f, const double* ic_master, const double* ic_slave, double transient,
+                double window, int n_pts, int axis_x_var, int axis_y_var, double x_lo, double x_hi, double y_lo,
+                double y_hi, int swap, const long long* cells, long long n_sel, double* out, int threads) {
+    const int n = f->n_vars;
+    double icm[OC_MAXN], ics[OC_MAXN];
+    for (int j = 0; j < n; ++j) { icm[j] = ic_master[j]; ics[j] = ic_slave[j]; }
+    const int skip = (int)(transient / f->h);
+    if (transient > 0.0 && skip > 0) {
+        double* fixed = swap ? icm : ics;
+        for (int i = 0; i < skip; ++i) fs_free_self(step, n, fixed, f->a, f->h);
+    }
+    const int nb = (int)(window / f->h / (double)f->pre_scaller);
+    f->n_iters = nb;
+    double* xv = (double*)malloc(sizeof(double) * (size_t)n_pts);
+    double* yv = (double*)malloc(sizeof(double) * (size_t)n_pts);
+    oc_axis_values(0, n_pts, x_lo, x_hi, xv);
+    oc_axis_values(0, n_pts, y_lo, y_hi, yv);
+#pragma omp parallel num_threads(threads > 1 ? threads : 1)
+    {
+        double* master = (double*)malloc(sizeof(double) * (size_t)(nb > 0 ? nb : 1) * (size_t)n);
+#pragma omp for schedule(dynamic, 1)
+        for (long long s = 0; s < n_sel; ++s) {
+            const long long c = cells[s];
+            double Xm[OC_MAXN], Xs[OC_MAXN], z[OC_MAXN];
+            for (int j = 0; j < n; ++j) { Xm[j] = icm[j]; Xs[j] = ics[j]; z[j] = 0.0; }
+            double* X = swap ? Xs : Xm;
+            X[axis_x_var] = xv[c % n_pts];
+            X[axis_y_var] = yv[(c / n_pts) % n_pts];
+            if (f->type_of_synch == 0) {
+                double M[OC_MAXN];
+                for (int j = 0; j < n; ++j) { master[j] = Xm[j]; M[j] = Xm[j]; }
+                for (int i = 1; i < nb; ++i) {
+                    fs_step(step, n, M, z, z, f->a, f->h, 1);
+                    for (int j = 0; j < n; ++j) master[(long long)i * n + j] = M[j];
+                }
+            }
+            out[s] = fs_sync(step, f, master, Xm, Xs);
+        }
+        free(master);
+    }
+    free(xv);
+    free(yv);
+}
+
 static void lyap_counts(const oc_job* j, double h, long long* skip, long long* nts) {
     if (sweeps_step(j)) {
         *nts = (long long)ceil(j->NT / h);

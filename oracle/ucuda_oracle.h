@@ -68,6 +68,24 @@ void oc_dft_block(const double* x, int N, const double* w, int n_freq, double f_
                   double dt, double* ak, double* bk);
 void oc_dft_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int n_freq, double f_lo,
                   double f_hi, int log_axis, int window_type, double* ak, double* bk, int* flags, int threads);
+/* fast synchronisation: on attractor (returns the number of points; out /
+ * traj_out for the selected ones) and on grid (selected cells) */
+typedef struct oc_fs {
+    int n_vars;
+    const double* a;
+    double h;
+    int iter_of_synchr, pre_scaller, type_of_synch, error_estim;
+    double fs_error_trs;
+    const double* k_forward;
+    const double* k_backward;
+    int n_iters;   /* set by the calls */
+} oc_fs;
+long long oc_fs_attractor(oc_step_fn step, oc_fs* f, const double* ic_master, const double* ic_slave, double t_max,
+                          double transient, double window, const long long* idx, long long n_sel, double* out,
+                          double* traj_out, int threads);
+void oc_fs_grid(oc_step_fn step, oc_fs* f, const double* ic_master, const double* ic_slave, double transient,
+                double window, int n_pts, int axis_x_var, int axis_y_var, double x_lo, double x_hi, double y_lo,
+                double y_hi, int swap, const long long* cells, long long n_sel, double* out, int threads);
 /* the same under an h sweep (per-cell block length, window and dt) */
 void oc_dft_hsweep_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int n_freq,
                          double f_lo, double f_hi, int log_axis, int window_type, double* ak, double* bk, int* flags,

@@ -407,3 +407,65 @@ def test_dft_hsweep_pins(cpu_engine):
     rc = oracle.dft_cells(step, jc, [0], 6, 0.2, 3.0, window_type=1)
     np.testing.assert_array_equal(rh["ak"], rc["ak"])
     np.testing.assert_array_equal(rh["bk"], rc["bk"])
+
+
+def _rk4_R(z):
+    return 1.0 + z + z * z / 2.0 + z ** 3 / 6.0 + z ** 4 / 24.0
+
+
+def test_fastsync_closed_forms(cpu_engine):
+    """Fast synchronisation on x' = c1 x, y' = c2 y (RK4): the coupled step
+    N = K (Xm - Xs) before the step, then X += h N, makes the error e = Xm - Xs
+    of each component obey e <- (R(h c) - h K) e with a stored master
+    (unidirectional) and e <- (R(h c) - 2 h K) e when the master is coupled
+    back (bidirectional). With one pass: estimator 2 = log10 |e| at the
+    window's second-to-last sample, estimator 0 = log10 of the RMS over the
+    pass (geometric sum), estimator 3 = (first i with |e_i| <= trs + 1) h,
+    estimator 1 = 1 when the end error is below trs. On the attractor the
+    window starting at sample idx * pre has e0 = R^(skip + idx pre) m0 - s0;
+    on the grid the cell's value replaces the swept variable of the master
+    (or of the slave with swap), and the fixed state first runs `skip`
+    uncoupled steps."""
+    c = np.array([-0.3, 0.2])
+    s = _linear_system(cpu_engine, np.diag(c))
+    step = oracle.KrsStep(cpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 2, 1)
+    h, K = 0.01, np.array([4.0, 6.0])
+    R = _rk4_R(h * c)
+    q1, q2 = R - h * K, R - 2 * h * K
+    m0, s0 = np.array([1.0, -2.0]), np.array([0.3, 0.5])
+    kw = dict(h=h, iter_of_synchr=1, pre_scaller=3, t_max=1.0, transient_time=0.05, window=0.5, fs_error_trs=1e-12)
+    idx = [0, 7, 20]
+    skip, N = 5, 50
+    e0 = np.array([R ** (skip + i * 3) * m0 - s0 for i in idx])
+    n, e2, traj = oracle.fs_attractor(step, 2, [0.5], m0, s0, K, K, idx, error_estim=2, **kw)
+    assert n == 100 // 3
+    np.testing.assert_allclose(traj, e0 + s0, rtol=1e-13)
+    want = np.log10(np.sqrt(((q1 ** (N - 2) * e0) ** 2).sum(axis=1)))
+    np.testing.assert_allclose(e2, want, rtol=1e-11)
+    _, e0v, _ = oracle.fs_attractor(step, 2, [0.5], m0, s0, K, K, idx, error_estim=0, **kw)
+    geo = ((e0 ** 2) * (1 - q1 ** (2 * (N - 1))) / (1 - q1 ** 2)).sum(axis=1)
+    np.testing.assert_allclose(e0v, np.log10(np.sqrt(geo / (N - 1))), rtol=1e-11)
+    _, eb, _ = oracle.fs_attractor(step, 2, [0.5], m0, s0, K, K, idx, error_estim=2, type_of_synch=1, **kw)
+    np.testing.assert_allclose(eb, np.log10(np.sqrt(((q2 ** (N - 2) * e0) ** 2).sum(axis=1))), rtol=1e-11)
+    trs = 0.5
+    kw3 = dict(kw, fs_error_trs=trs)
+    _, e3, _ = oracle.fs_attractor(step, 2, [0.5], m0, s0, K, K, idx, error_estim=3, **kw3)
+    first = [next(i for i in range(N - 1) if np.sqrt(((q1 ** i * e) ** 2).sum()) <= trs) for e in e0]
+    np.testing.assert_allclose(e3, [(i + 1) * h for i in first], rtol=1e-15)
+    _, e1, _ = oracle.fs_attractor(step, 2, [0.5], m0, s0, K, K, idx, error_estim=1, **kw3)
+    assert (e1 == 1.0).all()
+    # on grid: 5 x 5 cells over (x, y) of the master, the slave after 5 uncoupled steps; swap sweeps the slave
+    gkw = dict(h=h, iter_of_synchr=1, pre_scaller=2, transient_time=0.05, window=0.6, axis_x=(-1.0, 1.0),
+               axis_y=(0.5, 2.5), n_pts=5, error_estim=2, fs_error_trs=1e-12)
+    Ng = int(0.6 / h / 2)
+    xv = np.array(oracle.axis_values(False, 5, -1.0, 1.0))
+    yv = np.array(oracle.axis_values(False, 5, 0.5, 2.5))
+    for swap in (False, True):
+        g = oracle.fs_grid(step, 2, [0.5], m0, s0, K, K, range(25), grid_swap_master_slave=swap, **gkw)
+        fixed = (R ** 5) * (m0 if swap else s0)
+        grid = np.array([[xv[cc % 5], yv[cc // 5]] for cc in range(25)])
+        e = (fixed - grid) if swap else (grid - fixed)
+        np.testing.assert_allclose(g, np.log10(np.sqrt(((q1 ** (Ng - 2) * e) ** 2).sum(axis=1))), rtol=1e-11)
+    # a slave started on the master's own window stays on it: zero error, the trs floor
+    _, ez, _ = oracle.fs_attractor(step, 2, [0.5], m0, m0, K, K, [0], error_estim=0, **dict(kw, transient_time=0.0))
+    assert ez[0] == -12.0

@@ -1198,6 +1198,185 @@ int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_s
     return UC_OK;
 }
 
+// ---- fast synchronisation ----
+static int fs_validate(const uc_fastsync_request* r, std::string& why) {
+    if (!r) { why = "request missing"; return UC_ERR_ARGUMENT; }
+    if (!(r->h > 0.0)) { why = "h must be > 0"; return UC_ERR_ARGUMENT; }
+    if (r->iter_of_synchr <= 0) { why = "iter_of_synchr must be > 0"; return UC_ERR_ARGUMENT; }
+    if (r->pre_scaller <= 0) { why = "pre_scaller must be > 0"; return UC_ERR_ARGUMENT; }
+    if (r->type_of_synch < 0 || r->type_of_synch > 1) { why = "type_of_synch must be 0 or 1"; return UC_ERR_ARGUMENT; }
+    if (r->error_estim < 0 || r->error_estim > 3) { why = "error_estim must be 0..3"; return UC_ERR_ARGUMENT; }
+    if (r->transient_time < 0.0) { why = "transient_time must be >= 0"; return UC_ERR_ARGUMENT; }
+    if (r->mode == 0) {
+        if (!(r->t_max > 0.0)) { why = "t_max must be > 0"; return UC_ERR_ARGUMENT; }
+        if (!(r->window > 0.0)) { why = "window must be > 0"; return UC_ERR_ARGUMENT; }
+        if ((int)(r->t_max / r->h) / r->pre_scaller <= 0) { why = "t_max / h / pre_scaller gives no points"; return UC_ERR_ARGUMENT; }
+    } else if (r->mode == 1) {
+        if (r->n_pts <= 0) { why = "n_pts must be > 0"; return UC_ERR_ARGUMENT; }
+        if ((int)(r->window / r->h / (double)r->pre_scaller) <= 0) { why = "window / h / pre_scaller gives no samples"; return UC_ERR_ARGUMENT; }
+    } else {
+        why = "mode must be 0 (on attractor) or 1 (on grid)";
+        return UC_ERR_ARGUMENT;
+    }
+    return UC_OK;
+}
+
+long long uc_fastsync_points(const uc_fastsync_request* r) {
+    std::string why;
+    if (fs_validate(r, why) != UC_OK) return -1;
+    if (r->mode == 0) return (long long)((int)(r->t_max / r->h) / r->pre_scaller);
+    return (long long)r->n_pts * r->n_pts;
+}
+
+// On attractor: one-lane master fill, then one lane per window. On grid: a
+// one-lane transient of the fixed initial state, then chunks of cells (each
+// unidirectional cell stores its master window, n_iters x n_vars doubles).
+int uc_run_fastsync(uc_engine* e, uc_system* s, const uc_fastsync_request* r, const uc_fastsync_output* o,
+                    void* stream) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    std::string why;
+    int rc = fs_validate(r, why);
+    if (rc) return e->fail(rc, why);
+    const int N = s->n_vars;
+    if (!r->k_forward || !r->k_backward || !r->ic_master || !r->ic_slave)
+        return e->fail(UC_ERR_ARGUMENT, "k_forward / k_backward / ic_master / ic_slave missing");
+    if (!r->values && r->n_values > 0) return e->fail(UC_ERR_ARGUMENT, "values is NULL");
+    if (r->n_values > kMaxValues) return e->fail(UC_ERR_ARGUMENT, "too many values");
+    if (r->n_values < s->n_values) return e->fail(UC_ERR_ARGUMENT, "values shorter than the system's a[] (a[0] + params)");
+    if (!o->sync_error) return e->fail(UC_ERR_ARGUMENT, "sync_error output missing");
+    if (s->device_source.find("#define UC_USES_TIME 1") != std::string::npos)
+        return e->fail(UC_ERR_ARGUMENT, "fast synchronisation needs an autonomous system (no time symbol)");
+    if (r->mode == 1) {
+        if (r->axis_x_var < 0 || r->axis_x_var >= N || r->axis_y_var < 0 || r->axis_y_var >= N)
+            return e->fail(UC_ERR_ARGUMENT, "axis_x_var / axis_y_var out of range");
+        if (r->axis_x_var == r->axis_y_var) return e->fail(UC_ERR_ARGUMENT, "axis_x_var == axis_y_var");
+    }
+    if ((rc = enter(e))) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, r->strict != 0, true, &m))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+
+    // small input tables: a[] (kMaxValues slots), ic_master, ic_slave,
+    // k_forward, k_backward, grid axes
+    const int np = r->mode == 1 ? r->n_pts : 1;
+    std::vector<double> host((size_t)kMaxValues + 4 * (size_t)N + 2 * (size_t)np, 0.0);
+    for (int i = 0; i < r->n_values; ++i) host[(size_t)i] = r->values[i];
+    double* hp = host.data() + kMaxValues;
+    for (int i = 0; i < N; ++i) {
+        hp[i] = r->ic_master[i];
+        hp[N + i] = r->ic_slave[i];
+        hp[2 * N + i] = r->k_forward[i];
+        hp[3 * N + i] = r->k_backward[i];
+    }
+    if (r->mode == 1) {
+        uc_axis ax{};
+        ax.n_pts = np;
+        ax.lo = r->axis_x_lo;
+        ax.hi = r->axis_x_hi;
+        axis_values(ax, hp + 4 * N);
+        ax.lo = r->axis_y_lo;
+        ax.hi = r->axis_y_hi;
+        axis_values(ax, hp + 4 * N + np);
+    }
+    double *tabs = nullptr, *buf = nullptr;
+    auto release = [&]() {
+        (void)hipStreamSynchronize(st);
+        if (tabs) (void)hipFree(tabs);
+        if (buf) (void)hipFree(buf);
+    };
+    hipError_t he = hipMalloc((void**)&tabs, sizeof(double) * host.size());
+    if (he == hipSuccess) he = hipMemcpyAsync(tabs, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice, st);
+    if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "fast synchro tables")); }
+    UcFsJob J;
+    std::memset(&J, 0, sizeof J);
+    J.params = tabs;
+    J.ic_master = tabs + kMaxValues;
+    J.ic_slave = tabs + kMaxValues + N;
+    J.k_forward = tabs + kMaxValues + 2 * N;
+    J.k_backward = tabs + kMaxValues + 3 * N;
+    J.h = r->h;
+    J.iter_of_synchr = r->iter_of_synchr;
+    J.pre_scaller = r->pre_scaller;
+    J.type_of_synch = r->type_of_synch;
+    J.error_estim = r->error_estim;
+    J.fs_error_trs = r->fs_error_trs;
+    auto get = [&](const char* name, hipFunction_t* f) {
+        hipError_t x = hipModuleGetFunction(f, m->mod, name);
+        if (x != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(x, name)); }
+        return (int)UC_OK;
+    };
+    auto go = [&](hipFunction_t f, long long lanes, void** args, const char* what) {
+        const long long blocks = (lanes + kBlock - 1) / kBlock;
+        hipError_t x = hipModuleLaunchKernel(f, (unsigned)blocks, 1, 1, lanes == 1 ? 1 : kBlock, 1, 1, 0, st, args, nullptr);
+        if (x != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(x, what)); }
+        return (int)UC_OK;
+    };
+    const int skip = (int)(r->transient_time / r->h);
+    if (r->mode == 0) {
+        const int nt = (int)(r->window / r->h), ct = (int)(r->t_max / r->h);
+        const long long n_out = ct / r->pre_scaller;
+        const long long len = (long long)ct + nt;
+        he = hipMalloc((void**)&buf, sizeof(double) * (size_t)std::max<long long>(len, 1) * (size_t)N);
+        if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(master trajectory)")); }
+        hipFunction_t ff, fa;
+        if ((rc = get("uc_fs_fill", &ff)) || (rc = get("uc_fs_attr", &fa))) return rc;
+        int skip_a = skip, len_a = (int)len;
+        void* fargs[] = {&J, &skip_a, &len_a, &buf};
+        if ((rc = go(ff, 1, fargs, "uc_fs_fill"))) return rc;
+        J.n_cells = n_out;
+        J.n_iters = nt;
+        J.master = buf;
+        J.out = o->sync_error;
+        J.traj_out = o->traj;
+        void* aargs[] = {&J};
+        if ((rc = go(fa, n_out, aargs, "uc_fs_attr"))) return rc;
+        release();
+        return UC_OK;
+    }
+    // on grid
+    const int nb = (int)(r->window / r->h / (double)r->pre_scaller);
+    const long long total = (long long)np * np;
+    double* fixed = const_cast<double*>(r->grid_swap_master_slave ? J.ic_master : J.ic_slave);
+    if (r->transient_time > 0.0 && skip > 0) {
+        hipFunction_t ft;
+        if ((rc = get("uc_fs_transient", &ft))) return rc;
+        int skip_a = skip;
+        void* targs[] = {&J, &skip_a, &fixed};
+        if ((rc = go(ft, 1, targs, "uc_fs_transient"))) return rc;
+    }
+    long long chunk = total;
+    if (r->type_of_synch == 0) {
+        size_t free_b = 0, total_b = 0;
+        (void)hipMemGetInfo(&free_b, &total_b);
+        const size_t per_cell = (size_t)nb * (size_t)N * sizeof(double);
+        chunk = (long long)std::min<size_t>((size_t)total, std::max<size_t>(kBlock, free_b / 4 / per_cell));
+        he = hipMalloc((void**)&buf, per_cell * (size_t)chunk);
+        if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(master windows)")); }
+    }
+    hipFunction_t fg;
+    if ((rc = get("uc_fs_grid", &fg))) return rc;
+    J.n_pts = np;
+    J.axis_var[0] = r->axis_x_var;
+    J.axis_var[1] = r->axis_y_var;
+    J.axis_vals[0] = tabs + kMaxValues + 4 * N;
+    J.axis_vals[1] = tabs + kMaxValues + 4 * N + np;
+    J.swap = r->grid_swap_master_slave ? 1 : 0;
+    J.n_iters = nb;
+    J.data = buf;
+    for (long long off = 0; off < total; off += chunk) {
+        UcFsJob Jc = J;
+        Jc.cell_offset = off;
+        Jc.n_cells = std::min(chunk, total - off);
+        Jc.stride = Jc.n_cells;
+        Jc.out = o->sync_error + off;
+        void* gargs[] = {&Jc};
+        if ((rc = go(fg, Jc.n_cells, gargs, "uc_fs_grid"))) return rc;
+    }
+    release();
+    return UC_OK;
+}
+
 // Periods from stored peaks (dbscanCUDA, U-CUDA/cudaLibrary.cu:2321-2345).
 int uc_bif_dbscan_periods(uc_engine* e, uc_system* s, const double* peaks, const int* counts, long long n_cells,
                           int max_peaks, double eps, int* periods, void* stream) {

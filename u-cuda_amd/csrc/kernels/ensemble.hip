@@ -1238,6 +1238,198 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft_cells(
     bk[cell * n_freq + k] = b / (double)N;
 }
 
+// ========================================================== fast synchro ==
+// Fast synchronisation (the reference's FastSynchro,
+// U-CUDA/cudaLibrary.cu:424-452 step, :547-720 grid loop, :3630-3800
+// attractor loop), FP64. One coupled step: N = K (S1 - X) from the state
+// before the step, X <- step(X, +h forward / -h backward), X += h N (the
+// coupling always with +h). The loops follow the reference line by line,
+// because the error estimators read the state at fixed points of them.
+__device__ __forceinline__ void uc_fs_step(StateF64& X, const double* S1, const double* K, const double* a, double h,
+                                           bool fwd) {
+    double n[UC_N];
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) n[i] = K[i] * (S1[i] - X.x[i]);
+    uc_rk4(X, a, fwd ? h : -h, 0.0);
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) X.x[i] = X.x[i] + h * n[i];
+}
+
+// Uncoupled step as the reference writes it: K = 0 with S1 = X itself
+// (self) or S1 = 0 (the grid's master fill); kept literal so that signed
+// zeros and non-finite states propagate the same way.
+__device__ __forceinline__ void uc_fs_free(StateF64& X, const double* a, double h, bool self) {
+    double s1[UC_N], z[UC_N];
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) { s1[i] = self ? X.x[i] : 0.0; z[i] = 0.0; }
+    uc_fs_step(X, s1, z, a, h, true);
+}
+
+template <class Master>
+__device__ double uc_fs_sync(const UcFsJob& J, const double* a, StateF64& Xm, StateF64& Xs, Master master) {
+    const int n = J.n_iters;
+    const double h = J.h;
+    double kf[UC_N], kb[UC_N];
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) { kf[i] = J.k_forward[i]; kb[i] = J.k_backward[i]; }
+    double rms = 0.0;
+    double prev[UC_N];
+    for (int m = 0; m < J.iter_of_synchr; ++m) {
+        for (int i = 0; i < n - 1; ++i) {
+            if (J.type_of_synch == 0) master(i, Xm);
+            if (J.error_estim == 0) {
+                if (m == J.iter_of_synchr - 1)
+#pragma unroll
+                    for (int j = 0; j < UC_N; ++j) rms = rms + (Xm.x[j] - Xs.x[j]) * (Xm.x[j] - Xs.x[j]);
+            } else if (J.error_estim == 1) {
+                if (i == n - 2) {
+                    rms = 0.0;
+#pragma unroll
+                    for (int j = 0; j < UC_N; ++j) rms = rms + (Xm.x[j] - Xs.x[j]) * (Xm.x[j] - Xs.x[j]);
+                }
+            } else if (J.error_estim == 2) {
+                if (m == J.iter_of_synchr - 1 && i == n - 2) {
+                    rms = 0.0;
+#pragma unroll
+                    for (int j = 0; j < UC_N; ++j) rms = rms + (Xm.x[j] - Xs.x[j]) * (Xm.x[j] - Xs.x[j]);
+                }
+            } else {
+                rms = 0.0;
+#pragma unroll
+                for (int j = 0; j < UC_N; ++j) rms = rms + (Xm.x[j] - Xs.x[j]) * (Xm.x[j] - Xs.x[j]);
+                rms = sqrt(rms);
+                if (rms <= J.fs_error_trs) return (double)(i + 1) * h;
+            }
+            if (J.type_of_synch == 1)
+#pragma unroll
+                for (int j = 0; j < UC_N; ++j) prev[j] = Xs.x[j];
+            uc_fs_step(Xs, Xm.x, kf, a, h, true);
+            if (J.type_of_synch == 1) uc_fs_step(Xm, prev, kf, a, h, true);
+        }
+        if (J.error_estim == 1) {
+            rms = sqrt(rms);
+            if (rms <= J.fs_error_trs) return (double)(m + 1);
+            rms = 0.0;
+        }
+        for (int i = n - 1; i > 0; --i) {
+            if (J.type_of_synch == 0) master(i, Xm);
+            if (J.type_of_synch == 1)
+#pragma unroll
+                for (int j = 0; j < UC_N; ++j) prev[j] = Xs.x[j];
+            uc_fs_step(Xs, Xm.x, kb, a, h, false);
+            if (J.type_of_synch == 1) uc_fs_step(Xm, prev, kb, a, h, false);
+        }
+    }
+    if (J.error_estim == 0) rms = sqrt(rms / (double)(n - 1));
+    if (J.error_estim == 2) rms = sqrt(rms);
+    rms = rms <= J.fs_error_trs ? log10(J.fs_error_trs) : log10(rms);
+    if (isinf(rms) || isnan(rms) || rms >= 3.0) rms = 3.0;
+    if (J.error_estim == 1) return (double)J.iter_of_synchr;
+    if (J.error_estim == 3) return (double)n * h;
+    return rms;
+}
+
+__device__ __forceinline__ void uc_fs_params(const UcFsJob& J, double* a) {
+#pragma unroll
+    for (int i = 0; i < UC_M; ++i) a[i] = J.params[i];
+}
+
+// Master trajectory of the on-attractor mode (fillFSMasterTrajectory,
+// U-CUDA/kernels/fastsync_attr.template.cu:75-98): one lane, `skip`
+// uncoupled steps, then the state before each of `len` steps.
+extern "C" __global__ void uc_fs_fill(UcFsJob J, int skip, int len, double* traj) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double a[UC_M];
+    uc_fs_params(J, a);
+    StateF64 X;
+    X.load(J.ic_master);
+    for (int i = 0; i < skip; ++i) uc_fs_free(X, a, J.h, true);
+    for (int i = 0; i < len; ++i) {
+#pragma unroll
+        for (int j = 0; j < UC_N; ++j) traj[(long long)i * UC_N + j] = X.x[j];
+        uc_fs_free(X, a, J.h, true);
+    }
+}
+
+// Transient of the grid's fixed initial state (fillFSTransientIC,
+// U-CUDA/cudaLibrary.cu:454-474): one lane, in place.
+extern "C" __global__ void uc_fs_transient(UcFsJob J, int skip, double* x0) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double a[UC_M];
+    uc_fs_params(J, a);
+    StateF64 X;
+    X.load(x0);
+    for (int i = 0; i < skip; ++i) uc_fs_free(X, a, J.h, true);
+#pragma unroll
+    for (int j = 0; j < UC_N; ++j) x0[j] = X.x[j];
+}
+
+// On attractor (calculateDiscreteModelforFastSynchroCUDA,
+// U-CUDA/cudaLibrary.cu:3586-3626): lane idx synchronises the slave, from
+// ic_slave, over the master window starting at sample idx * pre.
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_fs_attr(UcFsJob J) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= J.n_cells) return;
+    double a[UC_M];
+    uc_fs_params(J, a);
+    const double* w = J.master + idx * (long long)J.pre_scaller * UC_N;
+    StateF64 Xm, Xs;
+    Xs.load(J.ic_slave);
+    if (J.type_of_synch == 1) Xm.load(w);
+    if (J.traj_out)
+#pragma unroll
+        for (int j = 0; j < UC_N; ++j) J.traj_out[idx * UC_N + j] = w[j];
+    J.out[idx] = uc_fs_sync(J, a, Xm, Xs, [&](int i, StateF64& m) {
+#pragma unroll
+        for (int j = 0; j < UC_N; ++j) m.x[j] = w[(long long)i * UC_N + j];
+    });
+}
+
+// On grid (calculateDiscreteModelICCforFastSynchro,
+// U-CUDA/cudaLibrary.cu:476-545): cell = iy * n_pts + ix sets state
+// variables axis_var[0] / axis_var[1] of the master (swap 0) or the slave
+// (swap 1); unidirectional runs first store the cell's uncoupled master
+// window (S1 = 0 steps) in data.
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_fs_grid(UcFsJob J) {
+    const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane >= J.n_cells) return;
+    const long long cell = J.cell_offset + lane;
+    double a[UC_M];
+    uc_fs_params(J, a);
+    double xm[UC_N], xs[UC_N];
+#pragma unroll
+    for (int j = 0; j < UC_N; ++j) { xm[j] = J.ic_master[j]; xs[j] = J.ic_slave[j]; }
+    const double v0 = J.axis_vals[0][cell % J.n_pts], v1 = J.axis_vals[1][(cell / J.n_pts) % J.n_pts];
+#pragma unroll
+    for (int j = 0; j < UC_N; ++j) {
+        double* x = J.swap ? xs : xm;
+        if (j == J.axis_var[0]) x[j] = v0;
+    }
+#pragma unroll
+    for (int j = 0; j < UC_N; ++j) {
+        double* x = J.swap ? xs : xm;
+        if (j == J.axis_var[1]) x[j] = v1;
+    }
+    StateF64 Xm, Xs;
+    Xm.load(xm);
+    Xs.load(xs);
+    double* d = J.data ? J.data + lane : nullptr;  // unidirectional runs only
+    if (J.type_of_synch == 0) {
+#pragma unroll
+        for (int j = 0; j < UC_N; ++j) d[(long long)j * J.stride] = Xm.x[j];
+        StateF64 M = Xm;
+        for (int i = 1; i < J.n_iters; ++i) {
+            uc_fs_free(M, a, J.h, false);
+#pragma unroll
+            for (int j = 0; j < UC_N; ++j) d[((long long)i * UC_N + j) * J.stride] = M.x[j];
+        }
+    }
+    J.out[lane] = uc_fs_sync(J, a, Xm, Xs, [&](int i, StateF64& m) {
+#pragma unroll
+        for (int j = 0; j < UC_N; ++j) m.x[j] = d[((long long)i * UC_N + j) * J.stride];
+    });
+}
+
 // Zeroes the work counters of a batch (a kernel, not a memset, so the reset
 // can live inside a captured graph).
 extern "C" __global__ void uc_batch_reset(u64* counters, int n) {

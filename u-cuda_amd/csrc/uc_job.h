@@ -77,4 +77,36 @@ typedef struct UcJob {
     int* label;                   /* basins: int32 attractor signature hash   */
 } UcJob;
 
+/* Fast synchronisation (the reference's FastSynchro): a slave copy of the
+ * system is pulled towards a master by N = K (master - slave), integrated
+ * forward with k_forward over a window and back with k_backward,
+ * iter_of_synchr times; the sync error is the map value. */
+typedef struct UcFsJob {
+    long long n_cells;            /* windows (on attractor) / grid cells      */
+    long long cell_offset;        /* global index of the first grid cell      */
+    long long stride;             /* plane stride of the per-cell master data */
+    int n_pts;                    /* grid side                                */
+    int axis_var[2];              /* state variable swept by each grid axis   */
+    const double* axis_vals[2];   /* device tables, n_pts values each         */
+    int swap;                     /* grid: 0 sweeps the master IC, 1 slave IC */
+    const double* params;         /* a[] (UC_M slots)                         */
+    const double* ic_master;      /* N values each                            */
+    const double* ic_slave;
+    const double* k_forward;
+    const double* k_backward;
+    double h;
+    int n_iters;                  /* window samples (amountOfIterations)      */
+    int iter_of_synchr;
+    int pre_scaller;              /* on attractor: window start = idx * pre   */
+    int type_of_synch;            /* 0 unidirectional, 1 bidirectional        */
+    int error_estim;              /* 0 RMS on the last pass, 1 passes to      */
+                                  /* reach fs_error_trs, 2 error at the last  */
+                                  /* point, 3 time to reach fs_error_trs      */
+    double fs_error_trs;
+    const double* master;         /* on attractor: master[i * N + j]          */
+    double* data;                 /* grid: master[(i * N + j) * stride + lane]*/
+    double* out;                  /* sync error per cell                      */
+    double* traj_out;             /* on attractor: decimated master [idx * N + j], optional */
+} UcFsJob;
+
 #endif

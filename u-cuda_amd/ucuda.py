@@ -79,6 +79,21 @@ class DftOutput(C.Structure):
     _fields_ = [("ak_cos", C.c_void_p), ("bk_sin", C.c_void_p), ("flags", C.c_void_p)]
 
 
+class FastSyncRequest(C.Structure):
+    _fields_ = [("values", C.c_void_p), ("n_values", C.c_int), ("mode", C.c_int), ("h", C.c_double),
+                ("iter_of_synchr", C.c_int), ("pre_scaller", C.c_int), ("k_forward", C.c_void_p),
+                ("k_backward", C.c_void_p), ("ic_master", C.c_void_p), ("ic_slave", C.c_void_p),
+                ("t_max", C.c_double), ("transient_time", C.c_double), ("window", C.c_double),
+                ("axis_x_var", C.c_int), ("axis_y_var", C.c_int), ("axis_x_lo", C.c_double),
+                ("axis_x_hi", C.c_double), ("axis_y_lo", C.c_double), ("axis_y_hi", C.c_double),
+                ("n_pts", C.c_int), ("grid_swap_master_slave", C.c_int), ("type_of_synch", C.c_int),
+                ("error_estim", C.c_int), ("fs_error_trs", C.c_double), ("strict", C.c_int)]
+
+
+class FastSyncOutput(C.Structure):
+    _fields_ = [("sync_error", C.c_void_p), ("traj", C.c_void_p)]
+
+
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_fraction", C.c_double), ("max_bytes", C.c_longlong), ("round_traj_steps", C.c_double),
                 ("block", C.c_int), ("waves_per_simd", C.c_int)]
@@ -131,6 +146,9 @@ _SIGNATURES = {
     "uc_run_bif_periods": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.c_double, C.c_void_p,
                                      C.c_void_p]),
     "uc_run_dft": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DftRequest), C.POINTER(DftOutput), C.c_void_p]),
+    "uc_fastsync_points": (C.c_longlong, [C.POINTER(FastSyncRequest)]),
+    "uc_run_fastsync": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FastSyncRequest), C.POINTER(FastSyncOutput),
+                                  C.c_void_p]),
     "uc_peak_finder": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_longlong, C.c_double,
                                  C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "uc_bif_dbscan_periods": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_longlong, C.c_int,
@@ -461,6 +479,47 @@ def run_dft(sysm, axes, ic, values, h, transient_time, t_max, n_freq=200, freq_l
     r.freq_lo, r.freq_hi, r.freq_log_scale, r.window_type = float(freq_lo), float(freq_hi), int(freq_log_scale), window_type
     o = DftOutput(_ptr(out["ak"]).value, _ptr(out["bk"]).value, _ptr(out["flags"]).value)
     eng.check(lib().uc_run_dft(eng.h, sysm.h, C.byref(r), C.byref(o), _stream(stream)), "uc_run_dft")
+    del keep
+    return out
+
+
+def fastsync_request(sysm, values, ic_master, ic_slave, k_forward, k_backward, mode=0, h=0.01, iter_of_synchr=100,
+                     pre_scaller=1, t_max=100.0, transient_time=0.0, window=50.0, axis_x_var=0, axis_y_var=1,
+                     axis_x=(-10.0, 10.0), axis_y=(-10.0, 10.0), n_pts=200, grid_swap_master_slave=False,
+                     type_of_synch=0, error_estim=2, fs_error_trs=1e-12, strict=False):
+    """uc_fastsync_request with its host arrays kept alive (returns (request, keep))."""
+    n = sysm.n_vars
+    arrs = [(C.c_double * max(1, len(v)))(*v) for v in (values, k_forward, k_backward, ic_master, ic_slave)]
+    for v in (k_forward, k_backward, ic_master, ic_slave):
+        assert len(v) == n, "k_forward / k_backward / ic_master / ic_slave need n_vars values"
+    r = FastSyncRequest()
+    r.values, r.k_forward, r.k_backward, r.ic_master, r.ic_slave = [C.cast(a, C.c_void_p) for a in arrs]
+    r.n_values, r.mode, r.h, r.iter_of_synchr, r.pre_scaller = len(values), mode, h, iter_of_synchr, pre_scaller
+    r.t_max, r.transient_time, r.window = t_max, transient_time, window
+    r.axis_x_var, r.axis_y_var, r.n_pts = axis_x_var, axis_y_var, n_pts
+    r.axis_x_lo, r.axis_x_hi = axis_x
+    r.axis_y_lo, r.axis_y_hi = axis_y
+    r.grid_swap_master_slave, r.type_of_synch, r.error_estim = int(grid_swap_master_slave), type_of_synch, error_estim
+    r.fs_error_trs, r.strict = fs_error_trs, int(strict)
+    return r, arrs
+
+
+def run_fastsync(sysm, values, ic_master, ic_slave, k_forward, k_backward, with_traj=True, stream=None, **kw):
+    """Fast synchronisation; returns dict(sync_error, traj) CUDA tensors
+    (traj: on-attractor master points, [n_out, n_vars])."""
+    import torch
+    eng = sysm.engine
+    r, keep = fastsync_request(sysm, values, ic_master, ic_slave, k_forward, k_backward, **kw)
+    n = lib().uc_fastsync_points(C.byref(r))
+    if n < 0:
+        raise UcError("invalid fast synchro request")
+    dev = torch.device("cuda", eng.device)
+    out = dict(sync_error=torch.empty(n, dtype=torch.float64, device=dev),
+               traj=torch.empty((n, sysm.n_vars), dtype=torch.float64, device=dev)
+               if (with_traj and r.mode == 0) else None)
+    o = FastSyncOutput(_ptr(out["sync_error"]).value, _ptr(out.get("traj")).value)
+    eng.check(lib().uc_run_fastsync(eng.h, sysm.h, C.byref(r), C.byref(o), _stream(stream)), "uc_run_fastsync")
+    torch.cuda.synchronize(dev)
     del keep
     return out
 
