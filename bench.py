@@ -307,6 +307,63 @@ def extra_dft(eng, n=16384, n_freq=200, stride=64):
     return out
 
 
+def extra_fastsync(eng, n=1024, stride=64):
+    """Fast synchronisation, Lorenz (10, 28, 8/3), RK4 h = 0.01, coupling
+    k_forward = (1, 0, 0), k_backward = (15, 15, 15), 10 forward/backward
+    passes, unidirectional, error at the window's end (estimator 2):
+    * on grid: n x n master initial states x in [-15, 15], z in [5, 40],
+      slave (3, -2, 20) after 1 s uncoupled, windows of 500 samples ->
+      500 + 10 * 2 * 499 coupled steps per cell;
+    * on attractor: 10,000 windows of 500 samples along a master trajectory
+      of 100 s (the reference's default t_max) after a 2 s transient
+      (pre_scaller 1); one lane per window, so 157 waves: latency-bound.
+    GPU: whole-call wall time. Parity: strict FP64 against the oracle on a
+    sample (every stride-th point per axis / every 100th window, 1e-15
+    relative: the states are bit-identical, log10 may differ in the last
+    bit); the oracle's single-thread time on that sample is the CPU rate."""
+    import numpy as np
+    import ucuda
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    lor = ucuda.preset("lorenz", eng)
+    vals = lor.values()
+    m0, s0, kf, kb = [0.1, 0.1, 0.1], [3.0, -2.0, 20.0], [1.0, 0.0, 0.0], [15.0, 15.0, 15.0]
+    step = oracle.KrsStep(eng.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
+    out = {}
+    gkw = dict(h=0.01, iter_of_synchr=10, pre_scaller=1, transient_time=1.0, window=5.0, axis_x_var=0, axis_y_var=2,
+               axis_x=(-15.0, 15.0), axis_y=(5.0, 40.0), n_pts=n, error_estim=2)
+    steps_cell = 499 + 10 * 2 * 499
+    res = {}
+    for name, kw in (("f64_strict", dict(strict=True)), ("f64", {})):
+        ms, r = _timed_calls(lambda: ucuda.run_fastsync(lor, vals, m0, s0, kf, kb, mode=1, **gkw, **kw))
+        res[name] = r["sync_error"].cpu().numpy()
+        out["grid_" + name] = dict(ms=float(np.mean(ms)), coupled_steps_per_s=n * n * steps_cell / (np.mean(ms) / 1e3))
+        progress("fastsync grid %s: %.1f ms" % (name, np.mean(ms)))
+    idx = np.arange(0, n, stride)
+    cells = (idx[:, None] * n + idx[None, :]).ravel()
+    t0 = time.perf_counter()
+    ref = oracle.fs_grid(step, 3, vals, m0, s0, kf, kb, cells, threads=1, **gkw)
+    cpu_s = time.perf_counter() - t0
+    out["grid_cpu_oracle"] = dict(cells=int(len(cells)), s=cpu_s, coupled_steps_per_s=len(cells) * steps_cell / cpu_s,
+                                  cores=1)
+    g = res["f64_strict"][cells]
+    out["grid_f64_strict"]["max_rel_err_vs_oracle"] = float(np.max(np.abs(g - ref) / np.maximum(np.abs(ref), 1e-300)))
+    out["grid_f64"]["median_abs_diff_vs_strict"] = float(np.median(np.abs(res["f64"] - res["f64_strict"])))
+    akw = dict(h=0.01, iter_of_synchr=10, pre_scaller=1, t_max=100.0, transient_time=2.0, window=5.0, error_estim=2)
+    ms, r = _timed_calls(lambda: ucuda.run_fastsync(lor, vals, m0, s0, kf, kb, mode=0, strict=True, **akw))
+    a = r["sync_error"].cpu().numpy()
+    out["attractor_f64_strict"] = dict(ms=float(np.mean(ms)), windows=int(len(a)),
+                                       coupled_steps_per_s=len(a) * steps_cell / (np.mean(ms) / 1e3))
+    sel = np.arange(0, len(a), 100)
+    t0 = time.perf_counter()
+    _, ra, _ = oracle.fs_attractor(step, 3, vals, m0, s0, kf, kb, sel, threads=1, **akw)
+    out["attractor_cpu_oracle"] = dict(windows=int(len(sel)), s=time.perf_counter() - t0, cores=1)
+    out["attractor_f64_strict"]["max_rel_err_vs_oracle"] = float(
+        np.max(np.abs(a[sel] - ra) / np.maximum(np.abs(ra), 1e-300)))
+    progress("fastsync on attractor: %.1f ms" % np.mean(ms))
+    return out
+
+
 def extra_ls(eng, n=1024, steps_scale=1.0):
     """hyper-Rossler 4-exponent spectrum maps, h=0.005, Gram-Schmidt every 20
     steps, 1e5 steps, FP32-compensated and FP64:
@@ -717,6 +774,8 @@ def main():
              lambda: extra_bif_config1(eng, threads_all)),
             ("bif2d_lorenz_512", "2-D period map (Lorenz 512^2)", lambda: extra_bif2d(eng)),
             ("dft_lorenz_16384", "DFT sweep (Lorenz 16384 rho x 200 freqs)", lambda: extra_dft(eng)),
+            ("fastsync_lorenz", "fast synchro (Lorenz grid 1024^2, 10000 windows on the attractor)",
+             lambda: extra_fastsync(eng)),
             ("ls_hyper_rossler_1024", "LS spectrum map (hyper-Rossler 1024^2)", lambda: extra_ls(eng)),
             ("basins_duffing_2048", "basins (Duffing 2048^2)", lambda: extra_basins(eng)),
             ("batch_64_jobs", "batch (64 jobs)", lambda: extra_batch(eng)),

@@ -133,3 +133,29 @@ def test_facade_dft(gpu_engine, tmp_path):
     cfg = _rows(tmp_path / "dft1d_config.csv")
     assert cfg[0] == "1D bifurcation DFT" and "n_freq = 32" in cfg and "window = Hanning" in cfg
     assert "freq_lo = 0, freq_hi = 5" in cfg and "indexVar for DFT = 0" in cfg
+
+
+def test_facade_fastsync(gpu_engine, tmp_path):
+    print("case: facade run_fastsync (grid 10x10 and on attractor) CSV files vs C ABI", flush=True)
+    out = subprocess.run([os.path.join(ROOT, "u-cuda_amd", "facade_demo"), str(tmp_path)], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    lor = ucuda.preset("lorenz", gpu_engine)
+    kw = dict(h=0.01, iter_of_synchr=3, transient_time=1.0, axis_x_var=0, axis_y_var=2, axis_x=(-15.0, 15.0),
+              axis_y=(5.0, 40.0), n_pts=10)
+    g = ucuda.run_fastsync(lor, lor.values(), [0.1, 0.1, 0.1], [3.0, -2.0, 20.0], [1.0, 0.0, 0.0], [15.0] * 3,
+                           mode=1, window=1.2, **kw)["sync_error"].cpu().numpy()
+    # facade (system hipRTC) and this process (torch's hipRTC) differ in the last bits of fast-mode code
+    np.testing.assert_allclose(res["fs_grid"], g, atol=1e-6)
+    rows = _rows(tmp_path / "fastsync_grid.csv")
+    assert rows[:2] == ["-15 15", "5 40"] and len(rows) == 12
+    np.testing.assert_allclose([float(v) for r in rows[2:] for v in r.split(",")], res["fs_grid"], rtol=1e-14)
+    cfg = _rows(tmp_path / "fastsync_grid.csv_config.csv")
+    assert cfg[0] == "Fast Synchro (On Grid)" and "IC_master[3] = { 0.1, 0.1, 0.1 }" in cfg
+    assert "n_pts = 10x10" in cfg and "grid_swap_master_slave = 0" in cfg
+    assert "type_of_synch = Unidirectional, error_estim = RMS at last point, fs_error_trs = 1e-12" in cfg
+    a = _rows(tmp_path / "fastsync_attr.csv")
+    assert a[0] == "x,y,z,sync_error" and len(a) == 1 + 200 // 4
+    assert len(res["fs_attr"]) == 50
+    assert _rows(tmp_path / "fastsync_attr.csv_config.csv")[0] == "Fast Synchro (On Attractor)"

@@ -193,5 +193,65 @@ void basins_config(std::ostream& out, const std::vector<double>& values, const s
     out << "start value_2 = " << ylo << ", stop value_2 = " << yhi << "\n";
 }
 
+static void fs_vec(std::ostream& out, const char* label, const std::vector<double>& v) {
+    out << label << "[" << v.size() << "] = { ";
+    for (size_t k = 0; k < v.size(); ++k) out << v[k] << (k + 1 != v.size() ? ", " : " }\n");
+}
+
+void fastsync_config(std::ostream& out, int mode, const std::vector<double>& values, const std::vector<double>& ic_master,
+                     const std::vector<double>& ic_slave, const std::vector<double>& k_forward,
+                     const std::vector<double>& k_backward, double h, int iter_of_synchr, int pre_scaller,
+                     double window, int type_of_synch, int error_estim, double fs_error_trs, double t_max,
+                     double transient, int axis_x_var, int axis_y_var, double x_lo, double x_hi, double y_lo,
+                     double y_hi, int n_pts, bool swap) {
+    out << std::setprecision(kPrecision);
+    out << (mode == 0 ? "Fast Synchro (On Attractor)\n" : "Fast Synchro (On Grid)\n");
+    out << "a[" << values.size() << "] = { ";
+    for (size_t k = 0; k < values.size(); ++k) out << values[k] << (k + 1 != values.size() ? ", " : " }\n");
+    fs_vec(out, "IC_master", ic_master);
+    fs_vec(out, "IC_slave", ic_slave);
+    fs_vec(out, "k_forward", k_forward);
+    fs_vec(out, "k_backward", k_backward);
+    out << "h = " << h << "\niter_of_synchr = " << iter_of_synchr << "\ndecimator = " << pre_scaller
+        << "\nwindow = " << window << "\n";
+    const char* estim = error_estim == 1 ? "# iters to reach FS_error_trs"
+                        : error_estim == 2 ? "RMS at last point" : "RMS on last iter";
+    out << "type_of_synch = " << (type_of_synch == 1 ? "Bidirectional" : "Unidirectional")
+        << ", error_estim = " << estim << ", fs_error_trs = " << fs_error_trs << "\n";
+    if (mode == 0) {
+        out << "CT = " << t_max << "\nTT = " << transient << "\n";
+    } else {
+        out << "TT = " << transient << "\n";
+        out << "axis_x_var = " << axis_x_var << ", axis_y_var = " << axis_y_var << "\n";
+        out << "axis_x: " << x_lo << " .. " << x_hi << "\n";
+        out << "axis_y: " << y_lo << " .. " << y_hi << "\n";
+        out << "n_pts = " << n_pts << "x" << n_pts << "\n";
+        out << "grid_swap_master_slave = " << (swap ? 1 : 0) << "\n";
+    }
+}
+
+void fastsync_attractor(std::ostream& out, const double* traj, const double* err, int n_pts, int n_vars,
+                        const std::vector<std::string>& var_names) {
+    out << std::setprecision(kPrecision);
+    for (int j = 0; j < n_vars; ++j) {
+        if (j < (int)var_names.size()) out << var_names[(size_t)j];
+        else out << "x" << j;
+        out << ",";
+    }
+    out << "sync_error\n";
+    for (int i = 0; i < n_pts; ++i) {
+        for (int j = 0; j < n_vars; ++j) out << traj[(size_t)i * n_vars + j] << ",";
+        out << err[i] << "\n";
+    }
+}
+
+void fastsync_grid(std::ostream& out, const double* heatmap, int n, double x_lo, double x_hi, double y_lo,
+                   double y_hi) {
+    out << std::setprecision(kPrecision);
+    out << x_lo << " " << x_hi << "\n" << y_lo << " " << y_hi << "\n";
+    for (int iy = 0; iy < n; ++iy)
+        for (int ix = 0; ix < n; ++ix) out << heatmap[(size_t)iy * n + ix] << (ix + 1 < n ? "," : "\n");
+}
+
 }  // namespace csv
 }  // namespace ucuda

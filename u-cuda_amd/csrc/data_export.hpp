@@ -56,5 +56,18 @@ void basins_config(std::ostream& out, const std::vector<double>& values, const s
                    double transient, double h, int pre_scaller, double eps, int writable_var, int axis_x_var,
                    int axis_y_var, double xlo, double xhi, double ylo, double yhi);
 
+// Fast synchronisation (write_fastsync_config / _attractor / _grid,
+// U-CUDA/data_export.cpp:553-657).
+void fastsync_config(std::ostream& out, int mode, const std::vector<double>& values, const std::vector<double>& ic_master,
+                     const std::vector<double>& ic_slave, const std::vector<double>& k_forward,
+                     const std::vector<double>& k_backward, double h, int iter_of_synchr, int pre_scaller,
+                     double window, int type_of_synch, int error_estim, double fs_error_trs, double t_max,
+                     double transient, int axis_x_var, int axis_y_var, double x_lo, double x_hi, double y_lo,
+                     double y_hi, int n_pts, bool swap);
+void fastsync_attractor(std::ostream& out, const double* traj, const double* err, int n_pts, int n_vars,
+                        const std::vector<std::string>& var_names);
+void fastsync_grid(std::ostream& out, const double* heatmap, int n, double x_lo, double x_hi, double y_lo,
+                   double y_hi);
+
 }  // namespace csv
 }  // namespace ucuda

@@ -1233,11 +1233,12 @@ long long uc_fastsync_points(const uc_fastsync_request* r) {
 // unidirectional cell stores its master window, n_iters x n_vars doubles).
 int uc_run_fastsync(uc_engine* e, uc_system* s, const uc_fastsync_request* r, const uc_fastsync_output* o,
                     void* stream) {
-    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    if (!e || !s) return UC_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     std::string why;
     int rc = fs_validate(r, why);
     if (rc) return e->fail(rc, why);
+    if (!o) return e->fail(UC_ERR_ARGUMENT, "output missing");
     const int N = s->n_vars;
     if (!r->k_forward || !r->k_backward || !r->ic_master || !r->ic_slave)
         return e->fail(UC_ERR_ARGUMENT, "k_forward / k_backward / ic_master / ic_slave missing");

@@ -113,6 +113,36 @@ int main(int argc, char** argv) {
     BasinsResult bres = eng.run_basins(bq);
     if (!bres.ok) { std::printf("{\"error\": \"%s\"}\n", bres.error.c_str()); return 1; }
 
+    FastSyncRequest fq;
+    fq.system = lorenz;
+    fq.amountOfX = 3;
+    fq.values = {0.5, 10.0, 28.0, 8.0 / 3.0};
+    fq.mode = 1;
+    fq.h = 0.01;
+    fq.iter_of_synchr = 3;
+    fq.k_forward = {1.0, 0.0, 0.0};
+    fq.k_backward = {15.0, 15.0, 15.0};
+    fq.ic_master = {0.1, 0.1, 0.1};
+    fq.ic_slave = {3.0, -2.0, 20.0};
+    fq.transient_time = 1.0;
+    fq.window = 1.2;
+    fq.axis_x_var = 0;
+    fq.axis_y_var = 2;
+    fq.axis_x_lo = -15.0; fq.axis_x_hi = 15.0;
+    fq.axis_y_lo = 5.0; fq.axis_y_hi = 40.0;
+    fq.n_pts = 10;
+    fq.csv_output_path = csv("fastsync_grid.csv");
+    FastSyncResult fr = eng.run_fastsync(fq);
+    if (!fr.ok) { std::printf("{\"error\": \"%s\"}\n", fr.error.c_str()); return 1; }
+    fq.mode = 0;
+    fq.t_max = 2.0;
+    fq.pre_scaller = 4;
+    fq.window = 1.0;
+    fq.var_names = {"x", "y", "z"};
+    fq.csv_output_path = csv("fastsync_attr.csv");
+    FastSyncResult fa = eng.run_fastsync(fq);
+    if (!fa.ok) { std::printf("{\"error\": \"%s\"}\n", fa.error.c_str()); return 1; }
+
     std::printf("{\"bif_flags\": [");
     for (size_t i = 0; i < br.flags.size(); ++i) std::printf("%s%d", i ? ", " : "", br.flags[i]);
     std::printf("], \"bif_first_peaks\": [");
@@ -133,6 +163,10 @@ int main(int argc, char** argv) {
     for (size_t i = 0; i < dr.flags.size(); ++i) std::printf("%s%d", i ? ", " : "", dr.flags[i]);
     std::printf("], \"basins\": [");
     for (size_t i = 0; i < bres.basin_idx.size(); ++i) std::printf("%s%d", i ? ", " : "", bres.basin_idx[i]);
+    std::printf("], \"fs_grid\": [");
+    for (size_t i = 0; i < fr.heatmap.size(); ++i) std::printf("%s%.17g", i ? ", " : "", fr.heatmap[i]);
+    std::printf("], \"fs_attr\": [");
+    for (size_t i = 0; i < fa.sync_error.size(); ++i) std::printf("%s%.17g", i ? ", " : "", fa.sync_error[i]);
     std::printf("], \"lle_min\": %.17g, \"lle_max\": %.17g}\n", lr.min_val, lr.max_val);
     return 0;
 }

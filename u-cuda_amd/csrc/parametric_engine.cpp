@@ -330,6 +330,99 @@ Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
     return res;
 }
 
+FastSyncResult ParametricEngine::run_fastsync(const FastSyncRequest& req) {
+    FastSyncResult res;
+    res.mode = req.mode;
+    std::lock_guard<std::mutex> lk(impl_->mu);
+    if (!impl_->eng) { res.error = "engine not initialised"; return res; }
+    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
+    uc_system* s = impl_->system(req.system, res.error);
+    if (!s) return res;
+    const size_t n = req.system.vars.size();
+    if (req.k_forward.size() != n || req.k_backward.size() != n || req.ic_master.size() != n || req.ic_slave.size() != n) {
+        res.error = "k_forward / k_backward / ic_master / ic_slave must hold amountOfX values";
+        return res;
+    }
+    uc_fastsync_request r{};
+    r.values = req.values.data();
+    r.n_values = (int)req.values.size();
+    r.mode = req.mode;
+    r.h = req.h;
+    r.iter_of_synchr = req.iter_of_synchr;
+    r.pre_scaller = req.pre_scaller;
+    r.k_forward = req.k_forward.data();
+    r.k_backward = req.k_backward.data();
+    r.ic_master = req.ic_master.data();
+    r.ic_slave = req.ic_slave.data();
+    r.t_max = req.t_max;
+    r.transient_time = req.transient_time;
+    r.window = req.window;
+    r.axis_x_var = req.axis_x_var;
+    r.axis_y_var = req.axis_y_var;
+    r.axis_x_lo = req.axis_x_lo; r.axis_x_hi = req.axis_x_hi;
+    r.axis_y_lo = req.axis_y_lo; r.axis_y_hi = req.axis_y_hi;
+    r.n_pts = req.n_pts;
+    r.grid_swap_master_slave = req.grid_swap_master_slave ? 1 : 0;
+    r.type_of_synch = req.type_of_synch;
+    r.error_estim = req.error_estim;
+    r.fs_error_trs = req.fs_error_trs;
+    const long long count = uc_fastsync_points(&r);
+    if (count < 0) {
+        // the C ABI names the first violated rule
+        (void)uc_run_fastsync(impl_->eng, s, &r, nullptr, nullptr);
+        res.error = impl_->err();
+        if (res.error.empty()) res.error = "invalid fast synchro request";
+        return res;
+    }
+    DevBuf<double> err, traj;
+    if (!err.alloc((size_t)count) || (req.mode == 0 && !traj.alloc((size_t)count * n))) {
+        res.error = "hipMalloc failed";
+        return res;
+    }
+    uc_fastsync_output o{err.p, req.mode == 0 ? traj.p : nullptr};
+    if (uc_run_fastsync(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    std::vector<double> values;
+    if (!err.download(values, (size_t)count)) { res.error = "device error"; return res; }
+    double vmin = std::numeric_limits<double>::infinity(), vmax = -vmin;
+    for (double v : values) {
+        if (!std::isfinite(v)) continue;
+        vmin = std::min(vmin, v);
+        vmax = std::max(vmax, v);
+    }
+    res.min_val = std::isfinite(vmin) ? vmin : 0.0;
+    res.max_val = std::isfinite(vmax) ? vmax : 0.0;
+    if (req.mode == 0) {
+        if (!traj.download(res.traj_full, (size_t)count * n)) { res.error = "device error"; return res; }
+        res.sync_error = std::move(values);
+        res.n_pts_traj = (int)count;
+        res.amountOfX_traj = (int)n;
+    } else {
+        res.heatmap = std::move(values);
+        res.n_pts_grid = req.n_pts;
+        res.axis_x_lo = req.axis_x_lo; res.axis_x_hi = req.axis_x_hi;
+        res.axis_y_lo = req.axis_y_lo; res.axis_y_hi = req.axis_y_hi;
+        res.axis_x_var = req.axis_x_var; res.axis_y_var = req.axis_y_var;
+    }
+    if (!req.csv_output_path.empty()) {
+        std::ofstream cfg(req.csv_output_path + "_config.csv");
+        csv::fastsync_config(cfg, req.mode, req.values, req.ic_master, req.ic_slave, req.k_forward, req.k_backward,
+                             req.h, req.iter_of_synchr, req.pre_scaller, req.window, req.type_of_synch,
+                             req.error_estim, req.fs_error_trs, req.t_max, req.transient_time, req.axis_x_var,
+                             req.axis_y_var, req.axis_x_lo, req.axis_x_hi, req.axis_y_lo, req.axis_y_hi, req.n_pts,
+                             req.grid_swap_master_slave);
+        std::ofstream data(req.csv_output_path);
+        if (req.mode == 0)
+            csv::fastsync_attractor(data, res.traj_full.data(), res.sync_error.data(), res.n_pts_traj, (int)n,
+                                    req.var_names);
+        else
+            csv::fastsync_grid(data, res.heatmap.data(), req.n_pts, req.axis_x_lo, req.axis_x_hi, req.axis_y_lo,
+                               req.axis_y_hi);
+    }
+    done(req.progress);
+    res.ok = true;
+    return res;
+}
+
 Bifurcation2DResult ParametricEngine::run_bifurcation_2d(const Bifurcation2DRequest& req) {
     Bifurcation2DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);

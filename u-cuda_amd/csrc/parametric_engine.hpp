@@ -236,6 +236,47 @@ struct BasinsResult {
     int min_cluster_idx = 0;
 };
 
+// Fast synchronisation (U-CUDA/parametric_engine.h:767-842).
+struct FastSyncRequest {
+    System system;
+    std::string krs_body;  // informational: the engine steps `system` with RK4
+    int amountOfX = 0;
+    std::vector<double> values;  // a[]
+    int mode = 0;                // 0 on attractor, 1 on grid
+    double h = 0.01;
+    int iter_of_synchr = 100;
+    int pre_scaller = 1;
+    double max_value = 1e6;      // kept for the request's shape; the reference's loops do not test it
+    std::vector<double> k_forward, k_backward, ic_master, ic_slave;
+    double t_max = 100.0, transient_time = 0.0;
+    double window = 50.0;
+    int axis_x_var = 0, axis_y_var = 1;
+    double axis_x_lo = -10.0, axis_x_hi = 10.0, axis_y_lo = -10.0, axis_y_hi = 10.0;
+    int n_pts = 200;
+    bool grid_swap_master_slave = false;
+    int type_of_synch = 0;
+    int error_estim = 2;
+    double fs_error_trs = 1e-12;
+    std::string csv_output_path;  // non-empty: <path> and <path>_config.csv
+    std::vector<std::string> var_names;
+    std::shared_ptr<std::atomic<bool>> cancel;
+    std::shared_ptr<std::atomic<float>> progress;
+};
+
+struct FastSyncResult {
+    bool ok = false, cancelled = false;
+    std::string error;
+    int mode = 0;
+    int n_pts_traj = 0, amountOfX_traj = 0;
+    std::vector<double> traj_full;   // mode 0: [point * amountOfX + var]
+    std::vector<double> sync_error;  // mode 0: per point
+    int n_pts_grid = 0;
+    double axis_x_lo = 0.0, axis_x_hi = 0.0, axis_y_lo = 0.0, axis_y_hi = 0.0;
+    int axis_x_var = 0, axis_y_var = 1;
+    std::vector<double> heatmap;     // mode 1: [iy * n + ix]
+    double min_val = 0.0, max_val = 0.0;
+};
+
 class ParametricEngine {
 public:
     explicit ParametricEngine(int device = 0);
@@ -251,6 +292,7 @@ public:
     LLE2DResult run_lle_2d(const LLE2DRequest& req);
     LS2DResult run_ls_2d(const LS2DRequest& req);
     BasinsResult run_basins(const BasinsRequest& req);
+    FastSyncResult run_fastsync(const FastSyncRequest& req);
 
 private:
     struct Impl;
