@@ -231,6 +231,36 @@ typedef struct uc_dft_output {
     int* flags;          /* [cells] 1 ok, -1 fixed point / bad step, 0 diverged   */
 } uc_dft_output;
 int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft_output* o, void* stream);
+/* Continuation sweeps (the reference's continuation mode of run_bifurcation_1d
+ * / run_lle_1d / run_ls_1d / run_dft_1d, U-CUDA/parametric_engine.cpp:
+ * 3983-4742; kernels/bifurcation1d_cont, lle1d_cont, ls1d_cont,
+ * dft1d_cont .template.cu): axis 0 (a parameter or h) is visited point by
+ * point, each point starting from the state (and probes) the previous one
+ * left; t = j / (n - 1), point j = lo + (hi - lo) t, or hi - (hi - lo) t with
+ * `reverse` (log: in log10). Outputs are indexed by visiting order, cell =
+ * chain * n_pts + j. An optional axis 1 gives independent chains (one GPU
+ * lane each; the reference runs one chain). FP64 only; synchronises.
+ *  bif: block (int)(t_max / h / pre) samples (capped at ceil(t_max / h_min
+ *    / pre)), transient (int)(transient / h) steps; flags = peak count, or -1
+ *    for an empty block, an escaping transient (the state is carried on) or
+ *    a block ending on a fixed point, 0 for a block that escapes part way;
+ *    peak intervals use the base h.
+ *  lle / ls: the reference's finite-difference Benettin chain with probes
+ *    carried between points (method and precision fields are ignored);
+ *    values [n_exponents][cells] = sum / t_max, NaN with flag -1 when the
+ *    point escaped (the chain then restarts from the initial state).
+ *  dft: a point whose block holds <= 2 samples, whose transient escapes or
+ *    whose block ends on a fixed point is -1 everywhere and restarts the
+ *    chain; a block that escapes part way is 0 everywhere and restarts it;
+ *    the others get flag 1 and the coefficients of their block. */
+int uc_run_bif_cont(uc_engine* e, uc_system* s, const uc_bif_request* r, int reverse, const uc_bif_output* o,
+                    void* stream);
+int uc_run_lle_cont(uc_engine* e, uc_system* s, const uc_lyap_request* r, int reverse, const uc_lyap_output* o,
+                    void* stream);
+int uc_run_ls_cont(uc_engine* e, uc_system* s, const uc_lyap_request* r, int reverse, const uc_lyap_output* o,
+                   void* stream);
+int uc_run_dft_cont(uc_engine* e, uc_system* s, const uc_dft_request* r, int reverse, const uc_dft_output* o,
+                    void* stream);
 int uc_run_lle(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream);

@@ -74,6 +74,10 @@ def lib():
         L.oc_dft_cells.argtypes = [_P, C.POINTER(OcJob), _LL, C.c_longlong, C.c_int, C.c_double, C.c_double, C.c_int,
                                    C.c_int, _D, _D, _I, C.c_int]
         L.oc_dft_hsweep_cells.argtypes = L.oc_dft_cells.argtypes
+        L.oc_cont_bif.argtypes = [_P, C.POINTER(OcJob), C.c_int, C.c_int, _I, _D, _D, _D, _I, C.c_int]
+        L.oc_cont_dft.argtypes = [_P, C.POINTER(OcJob), C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                  _D, _D, _I, C.c_int]
+        L.oc_cont_lyap.argtypes = [_P, C.POINTER(OcJob), C.c_int, C.c_int, _D, _I, C.c_int]
         L.oc_fs_attractor.argtypes = [_P, C.POINTER(OcFs), _D, _D, C.c_double, C.c_double, C.c_double, _LL,
                                       C.c_longlong, _D, _D, C.c_int]
         L.oc_fs_attractor.restype = C.c_longlong
@@ -427,6 +431,45 @@ def dft_hsweep_cells(step, job, cells, n_freq, f_lo, f_hi, log_axis=False, windo
                               int(window_type), ak.ctypes.data_as(_D), bk.ctypes.data_as(_D),
                               flags.ctypes.data_as(_I), int(threads))
     return dict(ak=ak, bk=bk, flags=flags)
+
+
+def _n_cells(job):
+    n = 1
+    for i in range(job.n_axes):
+        n *= job.axis_n[i]
+    return n
+
+
+def cont_bif(step, job, reverse=False, threads=1):
+    """Continuation bifurcation chains: flags, peaks / intervals [max_peaks, cells]."""
+    n = _n_cells(job)
+    mp = max(1, job.max_peaks)
+    flags = np.zeros(n, dtype=np.int32)
+    peaks, intervals = np.full((mp, n), np.nan), np.full((mp, n), np.nan)
+    lib().oc_cont_bif(step.ptr, C.byref(job), int(reverse), 0, flags.ctypes.data_as(_I), peaks.ctypes.data_as(_D),
+                      intervals.ctypes.data_as(_D), None, None, int(threads))
+    return dict(flags=flags, peaks=peaks, intervals=intervals)
+
+
+def cont_dft(step, job, n_freq, f_lo, f_hi, reverse=False, log_axis=False, window_type=1, threads=1):
+    """Continuation DFT chains: ak, bk [cells, n_freq], flags."""
+    n = _n_cells(job)
+    ak, bk = np.zeros((n, n_freq)), np.zeros((n, n_freq))
+    flags = np.zeros(n, dtype=np.int32)
+    lib().oc_cont_dft(step.ptr, C.byref(job), int(reverse), int(n_freq), float(f_lo), float(f_hi), int(log_axis),
+                      int(window_type), ak.ctypes.data_as(_D), bk.ctypes.data_as(_D), flags.ctypes.data_as(_I),
+                      int(threads))
+    return dict(ak=ak, bk=bk, flags=flags)
+
+
+def cont_lyap(step, job, K=1, reverse=False, threads=1):
+    """Continuation finite-difference Lyapunov chains: values [K, cells], flags."""
+    n = _n_cells(job)
+    values = np.zeros((K, n))
+    flags = np.zeros(n, dtype=np.int32)
+    lib().oc_cont_lyap(step.ptr, C.byref(job), int(reverse), int(K), values.ctypes.data_as(_D),
+                       flags.ctypes.data_as(_I), int(threads))
+    return dict(values=values, flags=flags)
 
 
 def _fs(n_vars, values, h, iter_of_synchr, pre_scaller, type_of_synch, error_estim, fs_error_trs, kf, kb):

@@ -606,6 +606,243 @@ void oc_dft_hsweep_cells(oc_step_fn step, const oc_job* j, const long long* cell
     }
 }
 
+/* ------------------------------------------------------- continuation ---
+ * Continuation sweeps (kernels/bifurcation1d_cont.template.cu:56-140,
+ * lle1d_cont.template.cu:71-183, ls1d_cont.template.cu:57-180,
+ * dft1d_cont.template.cu:72-175; host counts in
+ * parametric_engine.cpp:4563-4568, 4163-4167): axis 0 (parameter or h) is
+ * walked point by point with cont_sweep_value (:177-186), each point
+ * starting from the previous point's state; axis 1, when present, gives
+ * independent chains (cell = chain * n0 + point). Where a block escapes
+ * part way, the reference goes on to read samples it never wrote; this
+ * restatement (and the engine) report such a block as 0 instead, and the
+ * DFT chain restarts after it. */
+static double cont_value(const oc_job* j, int jj, int reverse) {
+    const int n = j->axis_n[0];
+    const double t = (double)jj / (double)(n > 1 ? n - 1 : 1);
+    const double lo = j->axis_lo[0], hi = j->axis_hi[0];
+    if (j->axis_log[0]) {
+        const double l0 = log10(lo), l1 = log10(hi);
+        return pow(10.0, reverse ? (l1 - (l1 - l0) * t) : (l0 + (l1 - l0) * t));
+    }
+    return reverse ? (hi - (hi - lo) * t) : (lo + (hi - lo) * t);
+}
+
+/* a[], x0 and h of chain `chain`: the base values with axis 1 applied */
+static void chain_setup(const oc_job* j, long long chain, double* a, double* x0, double* h) {
+    oc_job one = *j;
+    one.n_axes = 0;
+    oc_cell_setup(&one, chain, a, x0, h);  /* base values; random x0 keyed by the chain */
+    if (j->n_axes > 1) {
+        const double v = axis_at(j, 1, chain % j->axis_n[1]);
+        if (j->axis_kind[1] == OC_AXIS_PARAM) a[j->axis_index[1]] = v;
+        else if (j->axis_kind[1] == OC_AXIS_VAR) x0[j->axis_index[1]] = v;
+        else *h = v;
+    }
+}
+
+static double chain_point(const oc_job* j, int jj, int reverse, double* a, double h) {
+    const double p = cont_value(j, jj, reverse);
+    if (j->axis_kind[0] == OC_AXIS_STEP) return p;
+    a[j->axis_index[0]] = p;
+    return h;
+}
+
+static long long chains_of(const oc_job* j) { return j->n_axes > 1 ? j->axis_n[1] : 1; }
+
+/* dft: 0 bifurcation chain (flags = peak counts, peaks / intervals
+ * [max_peaks][cells]), 1 DFT chain (flags 1 / 0 / -1, blocks[cell][max_n],
+ * block lengths lens[cell]) */
+void oc_cont_bif(oc_step_fn step, const oc_job* j, int reverse, int dft, int* flags, double* peaks,
+                 double* intervals, double* blocks, int* lens, int threads) {
+    const int n0 = j->axis_n[0];
+    const long long ncell = (long long)n0 * chains_of(j);
+    double hmin = j->h;
+    if (j->axis_kind[0] == OC_AXIS_STEP) hmin = j->axis_lo[0] < j->axis_hi[0] ? j->axis_lo[0] : j->axis_hi[0];
+    else if (j->n_axes > 1 && j->axis_kind[1] == OC_AXIS_STEP) hmin = j->axis_lo[1] < j->axis_hi[1] ? j->axis_lo[1] : j->axis_hi[1];
+    const long long max_n = (long long)ceil(j->t_max / hmin / (double)j->pre_scaller);
+#pragma omp parallel num_threads(threads > 1 ? threads : 1)
+    {
+        double* data = (double*)malloc(sizeof(double) * (size_t)(max_n + 8));
+        double* pk = (double*)malloc(sizeof(double) * (size_t)(max_n + 8));
+        double* tm = (double*)malloc(sizeof(double) * (size_t)(max_n + 8));
+#pragma omp for schedule(dynamic, 1)
+        for (long long chain = 0; chain < chains_of(j); ++chain) {
+            double a[OC_MAXV], x0[OC_MAXN], x[OC_MAXN], hb;
+            chain_setup(j, chain, a, x0, &hb);
+            memcpy(x, x0, sizeof(double) * (size_t)j->n_vars);
+            for (int jj = 0; jj < n0; ++jj) {
+                const long long cell = chain * n0 + jj;
+                const double h = chain_point(j, jj, reverse, a, hb);
+                long long blk = h > 0.0 ? (long long)(j->t_max / h / (double)j->pre_scaller) : 0;
+                if (blk > max_n) blk = max_n;
+                const long long skip = h > 0.0 ? (long long)(j->transient_time / h) : 0;
+                int flag;
+                if (lens) lens[cell] = (int)blk;
+                if (h <= 0.0 || blk <= (dft ? 2 : 0)) {
+                    flag = -1;
+                } else if (oc_loop_model(step, x, a, h, skip, j->n_vars, 1, 0, j->max_value, NULL) == 0) {
+                    flag = -1;
+                } else {
+                    const int f = oc_loop_model(step, x, a, h, blk, j->n_vars, j->pre_scaller, j->writable_var,
+                                                j->max_value, data);
+                    if (f == -1 || f == 0) {
+                        flag = f;
+                    } else if (dft) {
+                        flag = 1;
+                        memcpy(blocks + cell * max_n, data, sizeof(double) * (size_t)blk);
+                    } else {
+                        int cnt = oc_peak_finder(data, blk, pk, tm, j->h * (double)j->pre_scaller);
+                        if (cnt > j->max_peaks) cnt = j->max_peaks;
+                        flag = cnt;
+                        for (int k = 0; k < cnt; ++k) {
+                            peaks[(long long)k * ncell + cell] = pk[k];
+                            intervals[(long long)k * ncell + cell] = tm[k];
+                        }
+                    }
+                }
+                flags[cell] = flag;
+                if (dft && flag != 1) memcpy(x, x0, sizeof(double) * (size_t)j->n_vars);
+            }
+        }
+        free(data);
+        free(pk);
+        free(tm);
+    }
+}
+
+/* DFT chain: the blocks of oc_cont_bif(dft = 1), then each block's DFT
+ * (cpu_dft_block with the window of its length and dt = h pre) */
+void oc_cont_dft(oc_step_fn step, const oc_job* j, int reverse, int n_freq, double f_lo, double f_hi, int log_axis,
+                 int window_type, double* ak, double* bk, int* flags, int threads) {
+    const int n0 = j->axis_n[0];
+    const long long ncell = (long long)n0 * chains_of(j);
+    double hmin = j->h;
+    if (j->axis_kind[0] == OC_AXIS_STEP) hmin = j->axis_lo[0] < j->axis_hi[0] ? j->axis_lo[0] : j->axis_hi[0];
+    else if (j->n_axes > 1 && j->axis_kind[1] == OC_AXIS_STEP) hmin = j->axis_lo[1] < j->axis_hi[1] ? j->axis_lo[1] : j->axis_hi[1];
+    const long long max_n = (long long)ceil(j->t_max / hmin / (double)j->pre_scaller);
+    double* blocks = (double*)malloc(sizeof(double) * (size_t)(ncell * max_n + 1));
+    int* lens = (int*)malloc(sizeof(int) * (size_t)ncell);
+    oc_cont_bif(step, j, reverse, 1, flags, NULL, NULL, blocks, lens, threads);
+#pragma omp parallel num_threads(threads > 1 ? threads : 1)
+    {
+        double* w = (double*)malloc(sizeof(double) * (size_t)(max_n + 1));
+#pragma omp for schedule(dynamic, 1)
+        for (long long cell = 0; cell < ncell; ++cell) {
+            double* A = ak + cell * n_freq;
+            double* B = bk + cell * n_freq;
+            if (flags[cell] != 1) {
+                for (int k = 0; k < n_freq; ++k) A[k] = B[k] = (double)flags[cell];
+                continue;
+            }
+            double a[OC_MAXV], x0[OC_MAXN], hb;
+            chain_setup(j, cell / n0, a, x0, &hb);
+            const double h = chain_point(j, (int)(cell % n0), reverse, a, hb);
+            oc_dft_window(window_type, lens[cell], w);
+            dft_block_cpu(blocks + cell * max_n, lens[cell], w, n_freq, f_lo, f_hi, log_axis,
+                          h * (double)j->pre_scaller, A, B);
+        }
+        free(w);
+    }
+    free(blocks);
+    free(lens);
+}
+
+/* Lyapunov chain, K = 1 or K = n_vars: values[K][cells], flags (1 / -1) */
+static void cont_probes(unsigned long long* st, int K, int n, double* y) {
+    for (int k = 0; k < K; ++k) {
+        double zp = 0.0;
+        for (int i = 0; i < n; ++i) {
+            y[k * n + i] = (double)oc_rng_uniform(st) - 0.5;
+            zp = zp + y[k * n + i] * y[k * n + i];
+        }
+        zp = sqrt(zp);
+        for (int i = 0; i < n; ++i) y[k * n + i] = (zp == 0.0) ? 0.0 : y[k * n + i] / zp;
+    }
+}
+
+void oc_cont_lyap(oc_step_fn step, const oc_job* j, int reverse, int K, double* values, int* flags, int threads) {
+    const int n0 = j->axis_n[0], n = j->n_vars;
+    const long long ncell = (long long)n0 * chains_of(j);
+    const long long n_blocks = (long long)(j->t_max / j->NT);
+#pragma omp parallel for num_threads(threads > 1 ? threads : 1) schedule(dynamic, 1)
+    for (long long chain = 0; chain < chains_of(j); ++chain) {
+        double a[OC_MAXV], x0[OC_MAXN], x[OC_MAXN], hb;
+        double y[OC_MAXN * OC_MAXN], z[OC_MAXN * OC_MAXN], den[OC_MAXN], sum[OC_MAXN];
+        chain_setup(j, chain, a, x0, &hb);
+        memcpy(x, x0, sizeof(double) * (size_t)n);
+        unsigned long long st;
+        oc_rng_init(j->probe_seed, 0ULL, 0ULL, &st);
+        cont_probes(&st, K, n, y);
+        int attached = 0;
+        for (int jj = 0; jj < n0; ++jj) {
+            const long long cell = chain * n0 + jj;
+            const double h = chain_point(j, jj, reverse, a, hb);
+            const long long nts = (h > 0.0 && n_blocks > 0) ? (long long)(j->NT / h) : 0;
+            if (h <= 0.0 || n_blocks <= 0 || nts <= 0) {
+                for (int k = 0; k < K; ++k) values[(long long)k * ncell + cell] = NAN;
+                flags[cell] = -1;
+                continue;
+            }
+            int alive = 1;
+            long long settle = (long long)(j->transient_time / j->NT);
+            if (!attached) {
+                settle = 0;
+                if (oc_loop_model(step, x, a, h, (long long)(j->transient_time / h), n, 1, 0, j->max_value, NULL) == 0) {
+                    alive = 0;
+                } else {
+                    if (K == 1) {
+                        for (int i = 0; i < n; ++i) y[i] = y[i] * j->eps + x[i];
+                    } else {
+                        oc_cgs(y, z, n, den);
+                        for (int k = 0; k < K; ++k)
+                            for (int i = 0; i < n; ++i) y[k * n + i] = z[k * n + i] * j->eps + x[i];
+                    }
+                    attached = 1;
+                }
+            }
+            for (int k = 0; k < K; ++k) sum[k] = 0.0;
+            for (long long b = 0; alive && b < settle + n_blocks; ++b) {
+                if (oc_loop_model(step, x, a, h, nts, n, 1, 0, j->max_value, NULL) == 0) { alive = 0; break; }
+                for (int k = 0; k < K; ++k)
+                    if (oc_loop_model(step, y + k * n, a, h, nts, n, 1, 0, j->max_value, NULL) == 0) { alive = 0; break; }
+                if (!alive) break;
+                if (K == 1) {
+                    double d = 0.0;
+                    for (int l = 0; l < n; ++l) {
+                        const double q = (x[l] - y[l]) / j->eps;
+                        d = d + q * q;
+                    }
+                    d = sqrt(d);
+                    if (d <= 1e-14) d = 1e-14;
+                    if (b >= settle) sum[0] = sum[0] + log(d);
+                    const double inv = 1.0 / d;
+                    for (int l = 0; l < n; ++l) y[l] = x[l] - ((x[l] - y[l] + 1e-14) * inv);
+                } else {
+                    for (int k = 0; k < K; ++k)
+                        for (int l = 0; l < n; ++l) y[k * n + l] = y[k * n + l] - x[l];
+                    oc_cgs(y, z, n, den);
+                    for (int k = 0; k < K; ++k) {
+                        if (b >= settle) sum[k] = sum[k] + log(den[k] / j->eps);
+                        for (int i = 0; i < n; ++i) y[k * n + i] = x[i] + z[k * n + i] * j->eps;
+                    }
+                }
+            }
+            if (alive) {
+                for (int k = 0; k < K; ++k) values[(long long)k * ncell + cell] = sum[k] / j->t_max;
+                flags[cell] = 1;
+            } else {
+                for (int k = 0; k < K; ++k) values[(long long)k * ncell + cell] = NAN;
+                flags[cell] = -1;
+                attached = 0;
+                memcpy(x, x0, sizeof(double) * (size_t)n);
+                oc_rng_init(j->probe_seed, (unsigned long long)jj + 1ULL, 0ULL, &st);
+                cont_probes(&st, K, n, y);
+            }
+        }
+    }
+}
+
 /* ------------------------------------------------------- fast synchro ---
  * FastSynchro (U-CUDA/cudaLibrary.cu): the coupled step
  * calculateDiscreteModelforFastSynchro (:424-452): N = K (S1 - X) from the

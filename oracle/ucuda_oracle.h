@@ -68,6 +68,12 @@ void oc_dft_block(const double* x, int N, const double* w, int n_freq, double f_
                   double dt, double* ak, double* bk);
 void oc_dft_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int n_freq, double f_lo,
                   double f_hi, int log_axis, int window_type, double* ak, double* bk, int* flags, int threads);
+/* continuation chains along axis 0 (axis 1: independent chains) */
+void oc_cont_bif(oc_step_fn step, const oc_job* j, int reverse, int dft, int* flags, double* peaks,
+                 double* intervals, double* blocks, int* lens, int threads);
+void oc_cont_dft(oc_step_fn step, const oc_job* j, int reverse, int n_freq, double f_lo, double f_hi, int log_axis,
+                 int window_type, double* ak, double* bk, int* flags, int threads);
+void oc_cont_lyap(oc_step_fn step, const oc_job* j, int reverse, int K, double* values, int* flags, int threads);
 /* fast synchronisation: on attractor (returns the number of points; out /
  * traj_out for the selected ones) and on grid (selected cells) */
 typedef struct oc_fs {

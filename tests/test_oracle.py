@@ -469,3 +469,80 @@ def test_fastsync_closed_forms(cpu_engine):
     # a slave started on the master's own window stays on it: zero error, the trs floor
     _, ez, _ = oracle.fs_attractor(step, 2, [0.5], m0, m0, K, K, [0], error_estim=0, **dict(kw, transient_time=0.0))
     assert ez[0] == -12.0
+
+
+def _equilibrium(r, branch):
+    """Real roots of r + x - x^3 = 0; branch -1 lowest, +1 highest."""
+    roots = np.roots([-1.0, 0.0, 1.0, r])
+    real = np.sort(roots[np.abs(roots.imag) < 1e-9].real)
+    return real[0] if branch < 0 else real[-1]
+
+
+def test_continuation_chains(cpu_engine):
+    """Continuation chains. (1) x' = r + x - x^3 is bistable for |r| <
+    2/(3 sqrt 3): walking r upwards the state stays on the lower branch,
+    downwards on the upper one, so at r in {-0.25, 0, 0.25} the forward and
+    reverse chains report the exponents 1 - 3 x*^2 of different equilibria
+    (times the reference block factor (NT/h + 1) h / NT); outside the fold
+    both agree. (2) The spectrum chain's exponents sum to the trace of the
+    Jacobian at the equilibrium the chain sits on (pitchfork x' = r x - x^3
+    plus y' = -y). (3) x'' = -w^2 x with w swept: the chain carries (x, v)
+    from point to point, so every peak of point j is the amplitude
+    sqrt(x^2 + (v / w)^2) of the RK4 map's matrix power over the steps the
+    reference spends per point (transient + 1, block * pre + pre). (4) A
+    one-point LLE chain is the ordinary finite-difference sweep of that
+    point (same probe subsequence 0)."""
+    # (1) hysteresis of the largest exponent
+    s = ucuda.System(cpu_engine, ["x"], ["r"], ["r+x-x^3"])
+    step = oracle.KrsStep(cpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 1, 2)
+    rs = np.linspace(-1.0, 1.0, 9)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 1, 9, -1.0, 1.0)]
+    job = oracle.make_job(1, axes, [0.0], [0.5, 0.0], 0.01, 20.0, 20.0, NT=0.1, eps=1e-4)
+    fwd = oracle.cont_lyap(step, job, 1)["values"][0]
+    rev = oracle.cont_lyap(step, job, 1, reverse=True)["values"][0][::-1]  # back in r order
+    factor = 11 * 0.01 / 0.1
+    for i, r in enumerate(rs):
+        lo, hi = 1 - 3 * _equilibrium(r, -1) ** 2, 1 - 3 * _equilibrium(r, +1) ** 2
+        if abs(r) < 2 / (3 * np.sqrt(3)):
+            np.testing.assert_allclose([fwd[i], rev[i]], [lo * factor, hi * factor], rtol=2e-3)
+        else:
+            np.testing.assert_allclose(fwd[i], rev[i], rtol=1e-6)
+    # (4) a one-point chain is the ordinary finite-difference sweep
+    one = [ucuda.axis(ucuda.AXIS_PARAM, 1, 1, 0.3, 0.3)]
+    j1 = oracle.make_job(1, one, [0.4], [0.5, 0.0], 0.01, 5.0, 10.0, NT=0.1, eps=1e-4)
+    np.testing.assert_array_equal(oracle.cont_lyap(step, j1, 1)["values"][0], oracle.lle_fd_cells(step, j1, [0])["values"][0])
+    # (2) spectrum chain: sum of exponents = trace at the chain's equilibrium
+    s2 = ucuda.System(cpu_engine, ["x", "y"], ["r"], ["r*x-x^3", "-y"])
+    step2 = oracle.KrsStep(cpu_engine.codegen_scheme(s2.vars, s2.params, s2.rhs), 2, 2)
+    j2 = oracle.make_job(2, [ucuda.axis(ucuda.AXIS_PARAM, 1, 5, -1.0, 1.0)], [0.3, 0.2], [0.5, 0.0], 0.01, 30.0, 20.0,
+                         NT=0.1, eps=1e-4)
+    # walked downwards: r > 0 settles on x = sqrt(r) (trace -2 r - 1), r < 0 on 0 (trace r - 1); walked
+    # upwards the chain would still sit on the unstable origin at r = 0.5 (x ~ 1e-33 after the r < 0
+    # points: a delayed bifurcation, trace r - 1)
+    ls = oracle.cont_lyap(step2, j2, 2, reverse=True)["values"].sum(axis=0)
+    rv = np.linspace(1.0, -1.0, 5)
+    trace = np.array([r - 1.0 if r < 0 else -2.0 * r - 1.0 for r in rv])
+    keep = rv != 0.0  # r = 0: x decays algebraically
+    np.testing.assert_allclose(ls[keep], trace[keep] * factor, rtol=2e-3)
+    up = oracle.cont_lyap(step2, j2, 2)["values"].sum(axis=0)
+    np.testing.assert_allclose(up[3], (0.5 - 1.0) * factor, rtol=2e-3)
+    # (3) oscillator amplitudes carried along the chain
+    s3 = ucuda.System(cpu_engine, ["x", "v"], ["w"], ["v", "-(w^2)*x"])  # the grammar reads -w^2 as (-w)^2
+    step3 = oracle.KrsStep(cpu_engine.codegen_scheme(s3.vars, s3.params, s3.rhs), 2, 2)
+    h, T0, T, pre = 0.01, 3.0, 25.0, 2
+    for reverse in (False, True):
+        ws = np.linspace(0.8, 2.0, 5)[::-1] if reverse else np.linspace(0.8, 2.0, 5)
+        j3 = oracle.make_job(2, [ucuda.axis(ucuda.AXIS_PARAM, 1, 5, 0.8, 2.0)], [1.0, 0.0], [0.5, 0.0], h, T0, T,
+                             pre_scaller=pre, max_peaks=64)
+        r3 = oracle.cont_bif(step3, j3, reverse=reverse)
+        st = np.array([1.0, 0.0])
+        for jj, w in enumerate(ws):
+            hA = h * np.array([[0.0, 1.0], [-w * w, 0.0]])
+            P = np.eye(2) + hA + hA @ hA / 2 + hA @ hA @ hA / 6 + hA @ hA @ hA @ hA / 24
+            st = np.linalg.matrix_power(P, int(T0 / h) + 1) @ st
+            amp = np.hypot(st[0], st[1] / w)
+            blk = int(T / h / pre)
+            m = r3["flags"][jj]
+            assert m >= int(w * T / (2 * np.pi)) - 1
+            np.testing.assert_allclose(r3["peaks"][:m, jj], amp, rtol=1e-6)
+            st = np.linalg.matrix_power(P, (blk + 1) * pre) @ st

@@ -904,15 +904,129 @@ int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, doub
 static const int kDftCells = 4;    // UC_DFT_C in the kernel source: cells per DFT lane
 static const int kDftSamples = 4;  // UC_DFT_U: samples per load block (traj / window rows padded to it)
 
+// Per-cell DFT tables for blocks of per-cell length (the h-sweep and the
+// continuation DFT, dft1d_hsweep.template.cu / dft1d_cont.template.cu): block
+// (int)(t_max / h / pre) capped at max_n, the window of that length (one per
+// distinct length, cpu_build_window), rotation (cos, sin)(2 pi h pre f_k)
+// and float-precision resync values (cpu_dft_block's formulas, C library).
+struct DftCellTables {
+    std::vector<int> Ns;
+    std::vector<long long> win_off, res_off;
+    std::vector<double> win, tab, res;
+};
+
+static void dft_cell_tables(const uc_dft_request* r, const std::vector<double>& h_cell, long long max_n,
+                            DftCellTables& T) {
+    const double kPi = 3.14159265358979323846;
+    const int nf = r->n_freq;
+    const size_t total = h_cell.size();
+    T.Ns.assign(total, 0);
+    T.win_off.assign(total, 0);
+    T.res_off.assign(total, 0);
+    T.win.clear();
+    T.res.clear();
+    T.tab.assign(2 * total * (size_t)nf, 0.0);
+    std::map<long long, long long> win_of_len;
+    std::vector<double> fk((size_t)nf);
+    for (int k = 0; k < nf; ++k) {
+        if (r->freq_log_scale) {
+            const double l0 = std::log10(r->freq_lo), l1 = std::log10(r->freq_hi);
+            fk[(size_t)k] = std::pow(10.0, l0 + (nf > 1 ? (l1 - l0) * (double)k / (double)(nf - 1) : 0.0));
+        } else {
+            const double f_step = nf > 1 ? (r->freq_hi - r->freq_lo) / (double)(nf - 1) : 0.0;
+            fk[(size_t)k] = r->freq_lo + (double)k * f_step;
+        }
+    }
+    for (size_t cell = 0; cell < total; ++cell) {
+        const double h = h_cell[cell];
+        long long n = h > 0.0 ? (long long)(r->c.t_max / h / (double)r->pre_scaller) : 0;
+        if (n > max_n) n = max_n;
+        T.Ns[cell] = (int)n;
+        if (n <= 2) continue;  // unusable: the recording kernel flags it -1
+        auto it = win_of_len.find(n);
+        if (it == win_of_len.end()) {
+            const long long off = (long long)T.win.size();
+            const long long padded = (n + kDftSamples - 1) / kDftSamples * kDftSamples;
+            T.win.resize((size_t)(off + padded), 0.0);
+            if (r->window_type == 0) {
+                std::fill(T.win.begin() + off, T.win.begin() + off + n, 1.0);
+            } else {
+                const double gamma = 2.0 * kPi / (double)(n - 1);
+                for (long long q = 0; q < n; ++q)
+                    T.win[(size_t)(off + q)] = r->window_type == 2 ? 0.53836 - 0.46164 * std::cos(gamma * (double)q)
+                                                                   : 0.5 * (1.0 - std::cos(gamma * (double)q));
+            }
+            it = win_of_len.emplace(n, off).first;
+        }
+        T.win_off[cell] = it->second;
+        const double dt = h * (double)r->pre_scaller;
+        const double psi = 2.0 * kPi * dt;
+        const int n_res = (int)((n - 1) / 1000);
+        T.res_off[cell] = (long long)T.res.size();
+        T.res.resize(T.res.size() + 2 * (size_t)nf * (size_t)n_res);
+        double* rs = T.res.data() + T.res_off[cell];
+        for (int k = 0; k < nf; ++k) {
+            const double f = fk[(size_t)k];
+            T.tab[2 * (cell * nf + k)] = std::cos(2.0 * kPi * dt * f);
+            T.tab[2 * (cell * nf + k) + 1] = std::sin(2.0 * kPi * dt * f);
+            for (int q = 0; q < n_res; ++q) {
+                const double exact = psi * f * (double)(1000LL * (q + 1));
+                rs[2 * ((size_t)k * n_res + q)] = (double)std::cos((float)exact);
+                rs[2 * ((size_t)k * n_res + q) + 1] = (double)std::sin((float)exact);
+            }
+        }
+    }
+    if (T.win.empty()) T.win.assign(kDftSamples, 0.0);
+    if (T.res.empty()) T.res.assign(2, 0.0);
+}
+
+// Device copies of the tables; `sums` launches uc_dft_cells over cells
+// [off, off + n) of a traj buffer with plane stride `stride`.
+struct DftCellTablesDev {
+    double *win = nullptr, *tab = nullptr, *res = nullptr;
+    int* N = nullptr;
+    long long *wo = nullptr, *ro = nullptr;
+    void free_all() {
+        for (void* p : {(void*)win, (void*)tab, (void*)res, (void*)N, (void*)wo, (void*)ro})
+            if (p) (void)hipFree(p);
+    }
+    hipError_t upload(const DftCellTables& T, hipStream_t st) {
+        auto up = [&](void** d, const void* h, size_t bytes) {
+            hipError_t he = hipMalloc(d, bytes);
+            if (he == hipSuccess) he = hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, st);
+            return he;
+        };
+        hipError_t he = up((void**)&win, T.win.data(), sizeof(double) * T.win.size());
+        if (he == hipSuccess) he = up((void**)&tab, T.tab.data(), sizeof(double) * T.tab.size());
+        if (he == hipSuccess) he = up((void**)&res, T.res.data(), sizeof(double) * T.res.size());
+        if (he == hipSuccess) he = up((void**)&N, T.Ns.data(), sizeof(int) * T.Ns.size());
+        if (he == hipSuccess) he = up((void**)&wo, T.win_off.data(), sizeof(long long) * T.win_off.size());
+        if (he == hipSuccess) he = up((void**)&ro, T.res_off.data(), sizeof(long long) * T.res_off.size());
+        return he;
+    }
+    hipError_t sums(hipFunction_t f, double* traj, long long stride, long long off, long long n, int nf,
+                    const uc_dft_output* o, hipStream_t st) {
+        const int* Np = N + off;
+        int* fl = o->flags + off;
+        const long long* wop = wo + off;
+        const double* tp = tab + 2 * off * nf;
+        const long long* rop = ro + off;
+        double* A = o->ak_cos + off * nf;
+        double* B = o->bk_sin + off * nf;
+        int nfa = nf;
+        void* args[] = {&traj, &stride, &n, &Np, &fl, &win, &wop, &nfa, &tp, &res, &rop, &A, &B};
+        const long long lanes = n * ((nf + 63) / 64) * 64;
+        return hipModuleLaunchKernel(f, (unsigned)((lanes + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, args,
+                                     nullptr);
+    }
+};
+
 // DFT under an h sweep (run_dft1d_hsweep_gpu,
 // U-CUDA/parametric_engine.cpp:4272-4520, and its kernel
 // dft1d_hsweep.template.cu): every cell has its own step, block length
 // (int)(t_max / h / pre) capped at ceil(t_max / min(h lo, h hi) / pre),
-// window and sampling interval. The recording kernel runs with raw_flags 2;
-// the host computes each cell's window (one per distinct block length),
-// rotation (cos, sin)(2 pi h pre f_k) and float-precision resync values with
-// the formulas of cpu_dft_block / cpu_build_window, and uc_dft_cells does
-// the sums.
+// window and sampling interval. The recording kernel runs with raw_flags 2,
+// the sums with uc_dft_cells over the per-cell tables.
 static int run_dft_hsweep(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft_output* o, int step_ax,
                           hipStream_t st) {
     const uc_common& c = r->c;
@@ -927,67 +1041,12 @@ static int run_dft_hsweep(uc_engine* e, uc_system* s, const uc_dft_request* r, c
     const int nf = r->n_freq;
     const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
     const long long total = (long long)nx * ny;
-    std::vector<double> hv((size_t)sa.n_pts);
+    std::vector<double> hv((size_t)sa.n_pts), h_cell((size_t)total);
     axis_values(sa, hv.data());
-
-    // per-cell block length, windows (shared by equal lengths), rotation and
-    // resync tables
-    const double kPi = 3.14159265358979323846;
-    std::vector<int> Ns((size_t)total);
-    std::vector<long long> win_off((size_t)total, 0), res_off((size_t)total, 0);
-    std::vector<double> win, tab(2 * (size_t)total * (size_t)nf), res;
-    std::map<long long, long long> win_of_len;
-    std::vector<double> fk((size_t)nf);
-    for (int k = 0; k < nf; ++k) {
-        if (r->freq_log_scale) {
-            const double l0 = std::log10(r->freq_lo), l1 = std::log10(r->freq_hi);
-            fk[(size_t)k] = std::pow(10.0, l0 + (nf > 1 ? (l1 - l0) * (double)k / (double)(nf - 1) : 0.0));
-        } else {
-            const double f_step = nf > 1 ? (r->freq_hi - r->freq_lo) / (double)(nf - 1) : 0.0;
-            fk[(size_t)k] = r->freq_lo + (double)k * f_step;
-        }
-    }
-    for (long long cell = 0; cell < total; ++cell) {
-        const double h = hv[(size_t)(step_ax == 0 ? cell % nx : (cell / nx) % ny)];
-        long long n = h > 0.0 ? (long long)(c.t_max / h / (double)r->pre_scaller) : 0;
-        if (n > max_n) n = max_n;
-        Ns[(size_t)cell] = (int)n;
-        if (n <= 2) continue;  // unusable: the recording kernel flags it -1
-        auto it = win_of_len.find(n);
-        if (it == win_of_len.end()) {
-            const long long off = (long long)win.size();
-            const long long padded = (n + kDftSamples - 1) / kDftSamples * kDftSamples;
-            win.resize((size_t)(off + padded), 0.0);
-            if (r->window_type == 0) {
-                std::fill(win.begin() + off, win.begin() + off + n, 1.0);
-            } else {
-                const double gamma = 2.0 * kPi / (double)(n - 1);
-                for (long long q = 0; q < n; ++q)
-                    win[(size_t)(off + q)] = r->window_type == 2 ? 0.53836 - 0.46164 * std::cos(gamma * (double)q)
-                                                                 : 0.5 * (1.0 - std::cos(gamma * (double)q));
-            }
-            it = win_of_len.emplace(n, off).first;
-        }
-        win_off[(size_t)cell] = it->second;
-        const double dt = h * (double)r->pre_scaller;
-        const double psi = 2.0 * kPi * dt;
-        const int n_res = (int)((n - 1) / 1000);
-        res_off[(size_t)cell] = (long long)res.size();
-        res.resize(res.size() + 2 * (size_t)nf * (size_t)n_res);
-        double* rs = res.data() + res_off[(size_t)cell];
-        for (int k = 0; k < nf; ++k) {
-            const double f = fk[(size_t)k];
-            tab[2 * ((size_t)cell * nf + k)] = std::cos(2.0 * kPi * dt * f);
-            tab[2 * ((size_t)cell * nf + k) + 1] = std::sin(2.0 * kPi * dt * f);
-            for (int q = 0; q < n_res; ++q) {
-                const double exact = psi * f * (double)(1000LL * (q + 1));
-                rs[2 * ((size_t)k * n_res + q)] = (double)std::cos((float)exact);
-                rs[2 * ((size_t)k * n_res + q) + 1] = (double)std::sin((float)exact);
-            }
-        }
-    }
-    if (win.empty()) win.assign(kDftSamples, 0.0);
-    if (res.empty()) res.assign(2, 0.0);
+    for (long long cell = 0; cell < total; ++cell)
+        h_cell[(size_t)cell] = hv[(size_t)(step_ax == 0 ? cell % nx : (cell / nx) % ny)];
+    DftCellTables T;
+    dft_cell_tables(r, h_cell, max_n, T);
 
     int rc = enter(e);
     if (rc) return rc;
@@ -1001,26 +1060,15 @@ static int run_dft_hsweep(uc_engine* e, uc_system* s, const uc_dft_request* r, c
     (void)hipMemGetInfo(&free_b, &total_b);
     const long long chunk =
         (long long)std::min<size_t>((size_t)total, std::max<size_t>(kBlock, free_b / 4 / ((size_t)rows * sizeof(double))));
-    double *traj = nullptr, *dwin = nullptr, *dtab = nullptr, *dres = nullptr;
-    int* dN = nullptr;
-    long long *dwo = nullptr, *dro = nullptr;
+    double* traj = nullptr;
+    DftCellTablesDev D;
     auto release = [&]() {
         (void)hipStreamSynchronize(st);
-        for (void* p : {(void*)traj, (void*)dwin, (void*)dtab, (void*)dres, (void*)dN, (void*)dwo, (void*)dro})
-            if (p) (void)hipFree(p);
-    };
-    auto up = [&](void** d, const void* h, size_t bytes) {
-        hipError_t he = hipMalloc(d, bytes);
-        if (he == hipSuccess) he = hipMemcpyAsync(*d, h, bytes, hipMemcpyHostToDevice, st);
-        return he;
+        if (traj) (void)hipFree(traj);
+        D.free_all();
     };
     hipError_t he = hipMalloc((void**)&traj, sizeof(double) * (size_t)rows * (size_t)chunk);
-    if (he == hipSuccess) he = up((void**)&dwin, win.data(), sizeof(double) * win.size());
-    if (he == hipSuccess) he = up((void**)&dtab, tab.data(), sizeof(double) * tab.size());
-    if (he == hipSuccess) he = up((void**)&dres, res.data(), sizeof(double) * res.size());
-    if (he == hipSuccess) he = up((void**)&dN, Ns.data(), sizeof(int) * Ns.size());
-    if (he == hipSuccess) he = up((void**)&dwo, win_off.data(), sizeof(long long) * win_off.size());
-    if (he == hipSuccess) he = up((void**)&dro, res_off.data(), sizeof(long long) * res_off.size());
+    if (he == hipSuccess) he = D.upload(T, st);
     if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "DFT h-sweep buffers")); }
     hipFunction_t fdft;
     he = hipModuleGetFunction(&fdft, m->mod, "uc_dft_cells");
@@ -1041,19 +1089,7 @@ static int run_dft_hsweep(uc_engine* e, uc_system* s, const uc_dft_request* r, c
         jc.traj = traj;
         jc.traj_len = 0;
         if ((rc = launch(e, m, kname, jc, st, tb))) { release(); return rc; }
-        long long stride = mcells;
-        int nfa = nf;
-        const int* Np = dN + off;
-        int* fl = o->flags + off;
-        const long long* wo = dwo + off;
-        const double* tp = dtab + 2 * off * nf;
-        const long long* ro = dro + off;
-        double* A = o->ak_cos + off * nf;
-        double* B = o->bk_sin + off * nf;
-        void* args[] = {&traj, &stride, &mcells, &Np, &fl, &dwin, &wo, &nfa, &tp, &dres, &ro, &A, &B};
-        const long long lanes = mcells * ((nf + 63) / 64) * 64;
-        he = hipModuleLaunchKernel(fdft, (unsigned)((lanes + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, args,
-                                   nullptr);
+        he = D.sums(fdft, traj, mcells, off, mcells, nf, o, st);
         if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_dft_cells launch")); }
     }
     release();
@@ -1175,6 +1211,205 @@ int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft
     }
     release();
     return UC_OK;
+}
+
+// ---- continuation sweeps ----
+// Chain points of axis 0 in visiting order (cont_sweep_value,
+// U-CUDA/parametric_engine.cpp:177-186, the formula the continuation kernels
+// inline): t = j / (n - 1), lo + (hi - lo) t, from hi when reversed; log:
+// 10^(l0 + (l1 - l0) t).
+static void cont_values(const uc_axis& ax, int reverse, double* out) {
+    const int n = ax.n_pts;
+    const double denom = (double)(n > 1 ? n - 1 : 1);
+    for (int j = 0; j < n; ++j) {
+        const double t = (double)j / denom;
+        if (ax.log_scale) {
+            const double l0 = std::log10(ax.lo), l1 = std::log10(ax.hi);
+            out[j] = std::pow(10.0, reverse ? (l1 - (l1 - l0) * t) : (l0 + (l1 - l0) * t));
+        } else {
+            out[j] = reverse ? (ax.hi - (ax.hi - ax.lo) * t) : (ax.lo + (ax.hi - ax.lo) * t);
+        }
+    }
+}
+
+// Shared set-up: checks, module, tables, the chain table of axis 0 (owned
+// by the caller: freed after the stream is synchronised), and the smallest
+// step any chain point can take.
+struct ContRun {
+    Module* m = nullptr;
+    UcJob j;
+    TableBuffer* tb = nullptr;
+    double* ax0 = nullptr;
+    double min_h = 0.0;
+};
+
+static int prep_cont(uc_engine* e, uc_system* s, const uc_common& c, int reverse, hipStream_t st, ContRun& R) {
+    int rc = validate_common(e, s, c, true);
+    if (rc) return rc;
+    const uc_axis& a0 = c.axis[0];
+    if (a0.kind != UC_AXIS_PARAM && a0.kind != UC_AXIS_STEP)
+        return e->fail(UC_ERR_ARGUMENT, "continuation sweeps a parameter or h (axis 0), not an initial state");
+    if (c.precision != UC_PREC_F64) return e->fail(UC_ERR_ARGUMENT, "continuation sweeps run in FP64");
+    if (c.ic_seed != 0ULL && c.n_axes < 2) return e->fail(UC_ERR_ARGUMENT, "random initial states need a chain axis");
+    auto hmin = [&](const uc_axis& ax) { return std::min(ax.lo, ax.hi); };
+    R.min_h = a0.kind == UC_AXIS_STEP ? hmin(a0) : (c.n_axes > 1 && c.axis[1].kind == UC_AXIS_STEP ? hmin(c.axis[1]) : c.h);
+    if (!(R.min_h > 0.0)) return e->fail(UC_ERR_ARGUMENT, "h must be > 0 (every point of an h sweep)");
+    if ((rc = enter(e))) return rc;
+    if ((rc = get_module(e, s, c.strict != 0, true, &R.m))) return rc;
+    if ((rc = stage_tables(e, s, c, R.j, st, &R.tb))) return rc;
+    std::vector<double> v((size_t)a0.n_pts);
+    cont_values(a0, reverse, v.data());
+    hipError_t he = hipMalloc((void**)&R.ax0, sizeof(double) * v.size());
+    if (he == hipSuccess) he = hipMemcpyAsync(R.ax0, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice, st);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "continuation table"));
+    R.j.axis_vals[0] = R.ax0;
+    return UC_OK;
+}
+
+static int launch_chains(uc_engine* e, ContRun& R, const char* name, hipStream_t st) {
+    hipFunction_t f;
+    hipError_t he = hipModuleGetFunction(&f, R.m->mod, name);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, name));
+    void* args[] = {&R.j};
+    he = hipModuleLaunchKernel(f, (unsigned)((R.j.ny + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, args, nullptr);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, name));
+    he = hipEventRecord(R.tb->done, st);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipEventRecord"));
+    R.tb->in_flight = true;
+    return UC_OK;
+}
+
+static void cont_done(ContRun& R, hipStream_t st) {
+    (void)hipStreamSynchronize(st);
+    if (R.ax0) (void)hipFree(R.ax0);
+    R.ax0 = nullptr;
+}
+
+int uc_run_bif_cont(uc_engine* e, uc_system* s, const uc_bif_request* r, int reverse, const uc_bif_output* o,
+                    void* stream) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (!o->flags) return e->fail(UC_ERR_ARGUMENT, "flags output missing");
+    if (r->max_peaks < 0) return e->fail(UC_ERR_ARGUMENT, "max_peaks must be >= 0");
+    if (r->max_peaks > 0 && (!o->peaks || !o->intervals)) return e->fail(UC_ERR_ARGUMENT, "peaks / intervals missing");
+    if (r->writable_var < -1 || r->writable_var >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "writable_var out of range");
+    if (r->pre_scaller <= 0) return e->fail(UC_ERR_ARGUMENT, "pre_scaller must be > 0");
+    hipStream_t st = (hipStream_t)stream;
+    ContRun R;
+    int rc = prep_cont(e, s, r->c, reverse, st, R);
+    if (!rc) {
+        const double max_d = std::ceil(r->c.t_max / R.min_h / (double)r->pre_scaller);
+        if (!(max_d >= 1.0) || max_d > 2.0e9) rc = e->fail(UC_ERR_ARGUMENT, "block of t_max / h / pre_scaller samples out of range");
+        R.j.n_iters = (long long)max_d;
+        R.j.pre_scaller = r->pre_scaller;
+        R.j.writable_var = r->writable_var;
+        R.j.max_peaks = r->max_peaks;
+        R.j.flag = o->flags;
+        R.j.out0 = r->max_peaks > 0 ? o->peaks : nullptr;
+        R.j.out1 = r->max_peaks > 0 ? o->intervals : nullptr;
+        R.j.raw_flags = 0;
+        if (!rc) rc = launch_chains(e, R, "uc_cont_bif", st);
+    }
+    cont_done(R, st);
+    return rc;
+}
+
+static int run_lyap_cont(uc_engine* e, uc_system* s, const uc_lyap_request* r, int reverse, const uc_lyap_output* o,
+                         void* stream, bool spectrum) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (!o->values || !o->flags) return e->fail(UC_ERR_ARGUMENT, "outputs missing");
+    if (!(r->NT > 0.0)) return e->fail(UC_ERR_ARGUMENT, "NT must be > 0");
+    if (!(r->eps > 0.0)) return e->fail(UC_ERR_ARGUMENT, "eps must be > 0");
+    hipStream_t st = (hipStream_t)stream;
+    ContRun R;
+    int rc = prep_cont(e, s, r->c, reverse, st, R);
+    if (!rc) {
+        R.j.nt = r->NT;
+        R.j.eps = r->eps;
+        R.j.probe_seed = r->probe_seed;
+        R.j.flag = o->flags;
+        R.j.out0 = o->values;
+        rc = launch_chains(e, R, spectrum ? "uc_cont_ls" : "uc_cont_lle", st);
+    }
+    cont_done(R, st);
+    return rc;
+}
+
+int uc_run_lle_cont(uc_engine* e, uc_system* s, const uc_lyap_request* r, int reverse, const uc_lyap_output* o,
+                    void* stream) {
+    return run_lyap_cont(e, s, r, reverse, o, stream, false);
+}
+
+int uc_run_ls_cont(uc_engine* e, uc_system* s, const uc_lyap_request* r, int reverse, const uc_lyap_output* o,
+                   void* stream) {
+    return run_lyap_cont(e, s, r, reverse, o, stream, true);
+}
+
+int uc_run_dft_cont(uc_engine* e, uc_system* s, const uc_dft_request* r, int reverse, const uc_dft_output* o,
+                    void* stream) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (!o->ak_cos || !o->bk_sin || !o->flags) return e->fail(UC_ERR_ARGUMENT, "DFT outputs missing");
+    if (r->writable_var < -1 || r->writable_var >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "writable_var out of range");
+    if (r->pre_scaller <= 0) return e->fail(UC_ERR_ARGUMENT, "pre_scaller must be > 0");
+    if (r->n_freq < 1) return e->fail(UC_ERR_ARGUMENT, "n_freq must be > 0");
+    if (r->window_type < 0 || r->window_type > 2) return e->fail(UC_ERR_ARGUMENT, "window_type must be 0, 1 or 2");
+    if (!(r->freq_hi > r->freq_lo)) return e->fail(UC_ERR_ARGUMENT, "freq_hi must be > freq_lo");
+    if (r->freq_log_scale && !(r->freq_lo > 0.0 && r->freq_hi > 0.0))
+        return e->fail(UC_ERR_ARGUMENT, "log frequency axis needs freq_lo, freq_hi > 0");
+    hipStream_t st = (hipStream_t)stream;
+    ContRun R;
+    int rc = prep_cont(e, s, r->c, reverse, st, R);
+    if (rc) { cont_done(R, st); return rc; }
+    const uc_common& c = r->c;
+    const double max_d = std::ceil(c.t_max / R.min_h / (double)r->pre_scaller);
+    if (!(max_d >= 1.0) || max_d > 2.0e9) {
+        cont_done(R, st);
+        return e->fail(UC_ERR_ARGUMENT, "block of t_max / h / pre_scaller samples out of range");
+    }
+    const long long max_n = (long long)max_d;
+    const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
+    const long long total = (long long)nx * ny;
+    // every cell's step: the chain point (h axis 0), the chain (h axis 1) or h
+    std::vector<double> v0((size_t)nx), v1((size_t)ny, c.h), h_cell((size_t)total);
+    cont_values(c.axis[0], reverse, v0.data());
+    if (c.n_axes > 1 && c.axis[1].kind == UC_AXIS_STEP) axis_values(c.axis[1], v1.data());
+    for (long long cell = 0; cell < total; ++cell)
+        h_cell[(size_t)cell] = c.axis[0].kind == UC_AXIS_STEP ? v0[(size_t)(cell % nx)] : v1[(size_t)(cell / nx)];
+    DftCellTables T;
+    dft_cell_tables(r, h_cell, max_n, T);
+    const long long rows = (max_n + kDftSamples - 1) / kDftSamples * kDftSamples;
+    size_t free_b = 0, total_b = 0;
+    (void)hipMemGetInfo(&free_b, &total_b);
+    if ((double)rows * (double)total * sizeof(double) > (double)free_b / 2) {
+        cont_done(R, st);
+        return e->fail(UC_ERR_ARGUMENT, "continuation DFT blocks exceed half of the free HBM");
+    }
+    double* traj = nullptr;
+    DftCellTablesDev D;
+    hipError_t he = hipMalloc((void**)&traj, sizeof(double) * ((size_t)rows * (size_t)total + kDftCells));
+    if (he == hipSuccess) he = D.upload(T, st);
+    if (he == hipSuccess) {
+        R.j.n_iters = max_n;
+        R.j.pre_scaller = r->pre_scaller;
+        R.j.writable_var = r->writable_var;
+        R.j.max_peaks = 0;
+        R.j.flag = o->flags;
+        R.j.traj = traj;
+        R.j.raw_flags = 1;
+        rc = launch_chains(e, R, "uc_cont_bif", st);
+        hipFunction_t fdft;
+        if (!rc) {
+            he = hipModuleGetFunction(&fdft, R.m->mod, "uc_dft_cells");
+            if (he == hipSuccess) he = D.sums(fdft, traj, total, 0, total, r->n_freq, o, st);
+        }
+    }
+    if (!rc && he != hipSuccess) rc = e->fail(UC_ERR_HIP, hip_msg(he, "continuation DFT"));
+    cont_done(R, st);
+    if (traj) (void)hipFree(traj);
+    D.free_all();
+    return rc;
 }
 
 int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_samples, long long n_series,

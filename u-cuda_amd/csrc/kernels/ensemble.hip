@@ -1238,6 +1238,239 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft_cells(
     bk[cell * n_freq + k] = b / (double)N;
 }
 
+// ========================================================== continuation ==
+// Continuation sweeps (the reference's bifurcation1d_cont / lle1d_cont /
+// ls1d_cont / dft1d_cont templates, U-CUDA/kernels/*_cont.template.cu): the
+// points of axis 0 are visited in order and each starts from the state the
+// previous one left. One lane per chain: axis 1 (optional) gives independent
+// chains, cell = chain * nx + point. axis_vals[0] holds the chain's points
+// in visiting order (the host applies the reverse direction). FP64.
+__device__ __forceinline__ void uc_chain_setup(const UcJob& J, long long chain, double* a, double* x0, double& h) {
+#pragma unroll
+    for (int i = 0; i < UC_M; ++i) a[i] = (i < J.n_values) ? J.base_params[i] : 0.0;
+    if (J.ic_seed != 0ULL) {
+        UcRng r;
+        r.init(J.ic_seed, (u64)chain, 0ULL);
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) x0[i] = J.ic_lo + (J.ic_hi - J.ic_lo) * (double)r.uniform();
+    } else {
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) x0[i] = J.base_ic[i];
+    }
+    h = J.h;
+    if (J.axis_kind[1] != UC_AXIS_NONE) {
+        const double v = J.axis_vals[1][chain % J.ny];
+        const int idx = J.axis_index[1];
+        if (J.axis_kind[1] == UC_AXIS_PARAM) {
+#pragma unroll
+            for (int i = 0; i < UC_M; ++i) a[i] = (i == idx) ? v : a[i];
+        } else if (J.axis_kind[1] == UC_AXIS_VAR) {
+#pragma unroll
+            for (int i = 0; i < UC_N; ++i) x0[i] = (i == idx) ? v : x0[i];
+        } else {
+            h = v;
+        }
+    }
+}
+
+// The chain's point jj: the parameter a[axis_index[0]] or the step.
+__device__ __forceinline__ double uc_chain_point(const UcJob& J, long long jj, double* a, double h) {
+    const double p = J.axis_vals[0][jj];
+    if (J.axis_kind[0] == UC_AXIS_STEP) return p;
+    const int idx = J.axis_index[0];
+#pragma unroll
+    for (int i = 0; i < UC_M; ++i) a[i] = (i == idx) ? p : a[i];
+    return h;
+}
+
+// Bifurcation chain (bifurcation1dContinuationKernel, :56-140, then
+// peakFinderCUDA over each point's block with the base step h * pre as the
+// time step): block of (int)(t_max / h / pre) samples capped at n_iters,
+// -1 when empty or h <= 0; transient of (int)(transient / h) single steps
+// plus one trailing step, an escape there -> -1 with the state carried on;
+// flag = -1 when the block ends on a fixed point, else its peak count.
+// raw_flags 1: the DFT chain (dft1dContinuationKernel, :72-175): a block of
+// <= 2 samples, an escaping transient or a block ending on a fixed point
+// marks the point -1 and restarts the chain from the initial state; other
+// points get flag 1 and their block in traj. A block that escapes part way
+// gets flag 0 in both (the reference goes on to read the block's unwritten
+// samples there, which has no reproducible value); the DFT chain restarts
+// after it as after its other unusable points.
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_cont_bif(UcJob J) {
+    const long long chain = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (chain >= J.ny) return;
+    double ad[UC_M], x0[UC_N], hb;
+    uc_chain_setup(J, chain, ad, x0, hb);
+    StateF64 s;
+    s.load(x0);
+    double t = 0.0;
+    NoObserver none;
+    const bool dft = J.raw_flags == 1;
+    for (long long jj = 0; jj < J.nx; ++jj) {
+        const long long cell = chain * J.nx + jj;
+        const double h = uc_chain_point(J, jj, ad, hb);
+        long long blk = h > 0.0 ? (long long)(J.t_max / h / (double)J.pre_scaller) : 0;
+        if (blk > J.n_iters) blk = J.n_iters;
+        const long long skip = h > 0.0 ? (long long)(J.transient_time / h) : 0;
+        int flag;
+        if (h <= 0.0 || blk <= (dft ? 2 : 0)) {
+            flag = -1;
+        } else {
+            flag = uc_loop(s, ad, h, t, skip, 1, 0, J.max_value, none);
+            if (flag == 0) {
+                flag = -1;
+            } else {
+                PeakObserver obs;
+                obs.w0 = 0.0; obs.w1 = 0.0; obs.n = blk; obs.c_next = 2; obs.mode = dft ? 2 : 0; obs.raw = 0;
+                obs.last_t = 0.0; obs.time_step = J.h * (double)J.pre_scaller; obs.max_peaks = J.max_peaks;
+                obs.stride = J.stride; obs.peaks = J.out0 ? J.out0 + cell : nullptr;
+                obs.intervals = J.out1 ? J.out1 + cell : nullptr;
+                obs.traj = J.traj ? J.traj + cell : nullptr; obs.traj_len = J.traj ? (int)blk : 0;
+                const int f = uc_loop(s, ad, h, t, blk, J.pre_scaller, J.writable_var, J.max_value, obs);
+                flag = (f == -1) ? -1 : (f == 0) ? 0 : (dft ? 1 : obs.count());
+            }
+        }
+        J.flag[cell] = flag;
+        if (dft && flag != 1) s.load(x0);
+    }
+}
+
+// Finite-difference Lyapunov chain, K = 1 (lle1dContinuationKernel,
+// :71-183) or K = N (ls1dContinuationKernel, :57-180): the probes y
+// (absolute coordinates) ride with the state from point to point. A point
+// that has no attached probes first runs the transient on the state alone
+// and places the probes (K = 1: y eps + x; K = N: the classical
+// Gram-Schmidt of the random basis, times eps, plus x); a point with probes
+// attached runs (int)(transient / NT) extra blocks whose growth is not
+// summed. Block: nt = (int)(NT / h) steps plus the trailing step for the
+// state and each probe, an escape ends the point (NaN, flag -1, chain
+// restarts from the initial state with a new probe basis from subsequence
+// point + 1). K = 1: d = |(x - y) / eps| (floored at 1e-14), sum log d,
+// y = x - (x - y + 1e-14) / d. K = N: y -= x, Gram-Schmidt, sum
+// log(|b_k| / eps), y = x + eps b_k. Values sum / t_max.
+template <int K>
+__device__ void uc_cont_probes(UcRng& r, double* y) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double zp = 0.0;
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) {
+            y[k * UC_N + i] = (double)r.uniform() - 0.5;
+            zp = zp + y[k * UC_N + i] * y[k * UC_N + i];
+        }
+        zp = sqrt(zp);
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) y[k * UC_N + i] = (zp == 0.0) ? 0.0 : y[k * UC_N + i] / zp;
+    }
+}
+
+template <int K>
+__device__ void uc_cont_lyap_chain(const UcJob& J) {
+    const long long chain = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (chain >= J.ny) return;
+    double ad[UC_M], x0[UC_N], hb;
+    uc_chain_setup(J, chain, ad, x0, hb);
+    StateF64 s;
+    s.load(x0);
+    double y[K * UC_N], z[K * UC_N], den[K], sum[K];
+    UcRng r;
+    r.init(J.probe_seed, 0ULL, 0ULL);
+    uc_cont_probes<K>(r, y);
+    bool attached = false;
+    double t = 0.0;
+    NoObserver none;
+    const long long n_blocks = (long long)(J.t_max / J.nt);
+    for (long long jj = 0; jj < J.nx; ++jj) {
+        const long long cell = chain * J.nx + jj;
+        const double h = uc_chain_point(J, jj, ad, hb);
+        const long long nts = (h > 0.0 && n_blocks > 0) ? (long long)(J.nt / h) : 0;
+        if (h <= 0.0 || n_blocks <= 0 || nts <= 0) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) J.out0[(long long)k * J.stride + cell] = __builtin_nan("");
+            J.flag[cell] = -1;
+            continue;
+        }
+        bool alive = true;
+        long long settle = (long long)(J.transient_time / J.nt);
+        if (!attached) {
+            settle = 0;
+            if (uc_loop(s, ad, h, t, (long long)(J.transient_time / h), 1, 0, J.max_value, none) == 0) {
+                alive = false;
+            } else {
+                if (K == 1) {
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) y[i] = y[i] * J.eps + s.x[i];
+                } else {
+                    uc_cgs<K>(y, z, den);
+#pragma unroll
+                    for (int k = 0; k < K; ++k)
+#pragma unroll
+                        for (int i = 0; i < UC_N; ++i) y[k * UC_N + i] = z[k * UC_N + i] * J.eps + s.x[i];
+                }
+                attached = true;
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) sum[k] = 0.0;
+        for (long long b = 0; alive && b < settle + n_blocks; ++b) {
+            const double t0 = t;  // the probes cover the same interval as the state
+            if (uc_loop(s, ad, h, t, nts, 1, 0, J.max_value, none) == 0) { alive = false; break; }
+            for (int k = 0; k < K && alive; ++k) {
+                StateF64 p;
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) p.x[i] = y[k * UC_N + i];
+                double tp = t0;
+                if (uc_loop(p, ad, h, tp, nts, 1, 0, J.max_value, none) == 0) alive = false;
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) y[k * UC_N + i] = p.x[i];
+            }
+            if (!alive) break;
+            if (K == 1) {
+                double d = 0.0;
+#pragma unroll
+                for (int l = 0; l < UC_N; ++l) {
+                    const double q = (s.x[l] - y[l]) / J.eps;
+                    d = d + q * q;
+                }
+                d = sqrt(d);
+                if (d <= 1e-14) d = 1e-14;
+                if (b >= settle) sum[0] = sum[0] + log(d);
+                const double inv = 1.0 / d;
+#pragma unroll
+                for (int l = 0; l < UC_N; ++l) y[l] = s.x[l] - ((s.x[l] - y[l] + 1e-14) * inv);
+            } else {
+#pragma unroll
+                for (int k = 0; k < K; ++k)
+#pragma unroll
+                    for (int l = 0; l < UC_N; ++l) y[k * UC_N + l] = y[k * UC_N + l] - s.x[l];
+                uc_cgs<K>(y, z, den);
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if (b >= settle) sum[k] = sum[k] + log(den[k] / J.eps);
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) y[k * UC_N + i] = s.x[i] + z[k * UC_N + i] * J.eps;
+                }
+            }
+        }
+        if (alive) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) J.out0[(long long)k * J.stride + cell] = sum[k] / J.t_max;
+            J.flag[cell] = 1;
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) J.out0[(long long)k * J.stride + cell] = __builtin_nan("");
+            J.flag[cell] = -1;
+            attached = false;
+            s.load(x0);
+            r.init(J.probe_seed, (u64)jj + 1ULL, 0ULL);
+            uc_cont_probes<K>(r, y);
+        }
+    }
+}
+
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_cont_lle(UcJob J) { uc_cont_lyap_chain<1>(J); }
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_cont_ls(UcJob J) { uc_cont_lyap_chain<UC_N>(J); }
+
 // ========================================================== fast synchro ==
 // Fast synchronisation (the reference's FastSynchro,
 // U-CUDA/cudaLibrary.cu:424-452 step, :547-720 grid loop, :3630-3800

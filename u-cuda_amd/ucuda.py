@@ -146,6 +146,14 @@ _SIGNATURES = {
     "uc_run_bif_periods": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.c_double, C.c_void_p,
                                      C.c_void_p]),
     "uc_run_dft": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DftRequest), C.POINTER(DftOutput), C.c_void_p]),
+    "uc_run_bif_cont": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.c_int, C.POINTER(BifOutput),
+                                  C.c_void_p]),
+    "uc_run_lle_cont": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.c_int, C.POINTER(LyapOutput),
+                                  C.c_void_p]),
+    "uc_run_ls_cont": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.c_int, C.POINTER(LyapOutput),
+                                 C.c_void_p]),
+    "uc_run_dft_cont": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DftRequest), C.c_int, C.POINTER(DftOutput),
+                                  C.c_void_p]),
     "uc_fastsync_points": (C.c_longlong, [C.POINTER(FastSyncRequest)]),
     "uc_run_fastsync": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(FastSyncRequest), C.POINTER(FastSyncOutput),
                                   C.c_void_p]),
@@ -479,6 +487,71 @@ def run_dft(sysm, axes, ic, values, h, transient_time, t_max, n_freq=200, freq_l
     r.freq_lo, r.freq_hi, r.freq_log_scale, r.window_type = float(freq_lo), float(freq_hi), int(freq_log_scale), window_type
     o = DftOutput(_ptr(out["ak"]).value, _ptr(out["bk"]).value, _ptr(out["flags"]).value)
     eng.check(lib().uc_run_dft(eng.h, sysm.h, C.byref(r), C.byref(o), _stream(stream)), "uc_run_dft")
+    del keep
+    return out
+
+
+def run_bif_cont(sysm, axes, ic, values, h, transient_time, t_max, reverse=False, writable_var=0, pre_scaller=1,
+                 max_peaks=256, max_value=1e6, strict=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0):
+    """Continuation bifurcation sweep along axes[0] (axes[1]: independent
+    chains); dict(flags, peaks[max_peaks, cells], intervals)."""
+    import torch
+    eng = sysm.engine
+    n = n_cells(axes)
+    dev = torch.device("cuda", eng.device)
+    out = dict(flags=torch.empty(n, dtype=torch.int32, device=dev),
+               peaks=torch.empty((max(max_peaks, 1), n), dtype=torch.float64, device=dev),
+               intervals=torch.empty((max(max_peaks, 1), n), dtype=torch.float64, device=dev))
+    r = BifRequest()
+    r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, PREC_F64, strict,
+                        ic_seed, ic_lo, ic_hi)
+    r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, max_peaks
+    o = BifOutput(_ptr(out["flags"]).value, _ptr(out["peaks"]).value, _ptr(out["intervals"]).value, None, 0, None)
+    eng.check(lib().uc_run_bif_cont(eng.h, sysm.h, C.byref(r), int(reverse), C.byref(o), None), "uc_run_bif_cont")
+    del keep
+    return out
+
+
+def run_lyap_cont(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=False, reverse=False, eps=1e-4,
+                  probe_seed=REFERENCE_PROBE_SEED, max_value=1e6, strict=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0):
+    """Continuation LLE (spectrum=False, values[1, cells]) or spectrum
+    (values[N, cells]) along axes[0]; dict(values, flags)."""
+    import torch
+    eng = sysm.engine
+    n = n_cells(axes)
+    k = sysm.n_vars if spectrum else 1
+    dev = torch.device("cuda", eng.device)
+    out = dict(values=torch.empty((k, n), dtype=torch.float64, device=dev),
+               flags=torch.empty(n, dtype=torch.int32, device=dev))
+    r = LyapRequest()
+    r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, PREC_F64, strict,
+                        ic_seed, ic_lo, ic_hi)
+    r.NT, r.eps, r.method, r.probe_seed = float(NT), float(eps), METHOD_FINITE_DIFF, int(probe_seed)
+    o = LyapOutput(_ptr(out["values"]).value, _ptr(out["flags"]).value, None)
+    fn = lib().uc_run_ls_cont if spectrum else lib().uc_run_lle_cont
+    eng.check(fn(eng.h, sysm.h, C.byref(r), int(reverse), C.byref(o), None), "uc_run_lyap_cont")
+    del keep
+    return out
+
+
+def run_dft_cont(sysm, axes, ic, values, h, transient_time, t_max, reverse=False, n_freq=200, freq_lo=0.0,
+                 freq_hi=10.0, freq_log_scale=False, window_type=1, writable_var=0, pre_scaller=1, max_value=1e6,
+                 strict=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0):
+    """Continuation DFT sweep along axes[0]; dict(ak, bk [cells, n_freq], flags)."""
+    import torch
+    eng = sysm.engine
+    n = n_cells(axes)
+    dev = torch.device("cuda", eng.device)
+    out = dict(ak=torch.empty((n, n_freq), dtype=torch.float64, device=dev),
+               bk=torch.empty((n, n_freq), dtype=torch.float64, device=dev),
+               flags=torch.empty(n, dtype=torch.int32, device=dev))
+    r = DftRequest()
+    r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, PREC_F64, strict,
+                        ic_seed, ic_lo, ic_hi)
+    r.writable_var, r.pre_scaller, r.n_freq = writable_var, pre_scaller, n_freq
+    r.freq_lo, r.freq_hi, r.freq_log_scale, r.window_type = float(freq_lo), float(freq_hi), int(freq_log_scale), window_type
+    o = DftOutput(_ptr(out["ak"]).value, _ptr(out["bk"]).value, _ptr(out["flags"]).value)
+    eng.check(lib().uc_run_dft_cont(eng.h, sysm.h, C.byref(r), int(reverse), C.byref(o), None), "uc_run_dft_cont")
     del keep
     return out
 
