@@ -159,3 +159,18 @@ def test_facade_fastsync(gpu_engine, tmp_path):
     assert a[0] == "x,y,z,sync_error" and len(a) == 1 + 200 // 4
     assert len(res["fs_attr"]) == 50
     assert _rows(tmp_path / "fastsync_attr.csv_config.csv")[0] == "Fast Synchro (On Attractor)"
+
+
+def test_facade_continuation(gpu_engine, tmp_path):
+    print("case: facade run_lle_1d with continuation (fold x' = r + x - x^3, both directions) vs C ABI", flush=True)
+    out = subprocess.run([os.path.join(ROOT, "u-cuda_amd", "facade_demo"), str(tmp_path)], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    s = ucuda.System(gpu_engine, ["x"], ["r"], ["r+x-x^3"])
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 1, 9, -1.0, 1.0)]
+    for key, rev in (("cont_fwd", False), ("cont_rev", True)):
+        g = ucuda.run_lyap_cont(s, axes, [0.0], [0.5, 0.0], 0.01, 20.0, 20.0, 0.1, reverse=rev)["values"].cpu().numpy()[0]
+        np.testing.assert_allclose(res[key], g, atol=1e-9)
+    assert res["cont_rev_flag"] == 1
+    assert abs(res["cont_fwd"][3] - res["cont_rev"][5]) > 0.5  # r = -0.25 on different branches

@@ -143,6 +143,26 @@ int main(int argc, char** argv) {
     FastSyncResult fa = eng.run_fastsync(fq);
     if (!fa.ok) { std::printf("{\"error\": \"%s\"}\n", fa.error.c_str()); return 1; }
 
+    System fold{{"x"}, {"r"}, {"r+x-x^3"}, false, ""};
+    LLE1DRequest cq;
+    cq.system = fold;
+    cq.amountOfX = 1;
+    cq.initial_conditions = {0.0};
+    cq.base_values = {0.5, 0.0};
+    cq.param_index = 1;
+    cq.param_lo = -1.0;
+    cq.param_hi = 1.0;
+    cq.n_pts = 9;
+    cq.h = 0.01;
+    cq.transient_time = 20.0;
+    cq.t_max = 20.0;
+    cq.NT = 0.1;
+    cq.continuation = true;
+    LLE1DResult cf = eng.run_lle_1d(cq);
+    cq.continuation_reverse = true;
+    LLE1DResult cr = eng.run_lle_1d(cq);
+    if (!cf.ok || !cr.ok) { std::printf("{\"error\": \"%s\"}\n", (cf.ok ? cr : cf).error.c_str()); return 1; }
+
     std::printf("{\"bif_flags\": [");
     for (size_t i = 0; i < br.flags.size(); ++i) std::printf("%s%d", i ? ", " : "", br.flags[i]);
     std::printf("], \"bif_first_peaks\": [");
@@ -167,6 +187,11 @@ int main(int argc, char** argv) {
     for (size_t i = 0; i < fr.heatmap.size(); ++i) std::printf("%s%.17g", i ? ", " : "", fr.heatmap[i]);
     std::printf("], \"fs_attr\": [");
     for (size_t i = 0; i < fa.sync_error.size(); ++i) std::printf("%s%.17g", i ? ", " : "", fa.sync_error[i]);
-    std::printf("], \"lle_min\": %.17g, \"lle_max\": %.17g}\n", lr.min_val, lr.max_val);
+    std::printf("], \"cont_fwd\": [");
+    for (size_t i = 0; i < cf.lyapunov.size(); ++i) std::printf("%s%.17g", i ? ", " : "", cf.lyapunov[i]);
+    std::printf("], \"cont_rev\": [");
+    for (size_t i = 0; i < cr.lyapunov.size(); ++i) std::printf("%s%.17g", i ? ", " : "", cr.lyapunov[i]);
+    std::printf("], \"cont_rev_flag\": %d", cr.continuation_reverse ? 1 : 0);
+    std::printf(", \"lle_min\": %.17g, \"lle_max\": %.17g}\n", lr.min_val, lr.max_val);
     return 0;
 }

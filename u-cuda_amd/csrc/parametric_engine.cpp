@@ -106,7 +106,7 @@ Bifurcation1DResult ParametricEngine::run_bifurcation_1d(const Bifurcation1DRequ
     Bifurcation1DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (req.continuation || req.use_cpu) { res.error = "continuation / CPU sweeps are not part of this engine"; return res; }
+    if (req.use_cpu) { res.error = "CPU sweeps are not part of this engine (use_cpu); the GPU path runs the same algorithm"; return res; }
     if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
     uc_system* s = impl_->system(req.system, res.error);
     if (!s) return res;
@@ -119,12 +119,18 @@ Bifurcation1DResult ParametricEngine::run_bifurcation_1d(const Bifurcation1DRequ
     r.writable_var = req.writable_var;
     r.pre_scaller = req.pre_scaller;
     r.max_peaks = req.max_peaks;
-    const size_t n = (size_t)req.n_pts, mp = (size_t)std::max(1, req.max_peaks);
+    const double hw = req.sweep_over_h ? std::min(req.param_lo, req.param_hi) : req.h;
+    // continuation keeps every peak of the block, as the reference does
+    if (req.continuation) r.max_peaks = (int)std::ceil(req.t_max / hw / req.pre_scaller) / 2 + 2;
+    const size_t n = (size_t)req.n_pts, mp = (size_t)std::max(1, r.max_peaks);
     DevBuf<int> flags;
     DevBuf<double> peaks, ivals;
     if (!flags.alloc(n) || !peaks.alloc(mp * n) || !ivals.alloc(mp * n)) { res.error = "hipMalloc failed"; return res; }
     uc_bif_output o{flags.p, peaks.p, ivals.p, nullptr, 0, nullptr};
-    if (uc_run_bif(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    const int rc = req.continuation
+                       ? uc_run_bif_cont(impl_->eng, s, &r, req.continuation_reverse ? 1 : 0, &o, nullptr)
+                       : uc_run_bif(impl_->eng, s, &r, &o, nullptr);
+    if (rc != UC_OK) { res.error = impl_->err(); return res; }
     std::vector<int> f;
     std::vector<double> pk, iv;
     if (hipDeviceSynchronize() != hipSuccess || !flags.download(f, n) || !peaks.download(pk, mp * n) ||
@@ -132,17 +138,36 @@ Bifurcation1DResult ParametricEngine::run_bifurcation_1d(const Bifurcation1DRequ
     res.param_lo = req.param_lo;
     res.param_hi = req.param_hi;
     res.n_pts = req.n_pts;
-    const double hw = req.sweep_over_h ? std::min(req.param_lo, req.param_hi) : req.h;
     res.record_steps = (int)std::ceil(req.t_max / hw / req.pre_scaller);
+    res.continuation_reverse = req.continuation && req.continuation_reverse;
     res.flags = f;
     res.bifurcation_points.assign(n, {});
     res.peak_times.assign(n, {});
-    for (size_t c = 0; c < n; ++c)
+    for (size_t c = 0; c < n; ++c) {
+        // continuation over h: the chain reports peak times in units of the
+        // base step; the reference rescales them to the point's own step
+        // (run_bif1d_continuation, U-CUDA/parametric_engine.cpp:4718-4727)
+        double scale = 1.0;
+        if (req.continuation && req.sweep_over_h) {
+            const double t = (double)c / (double)(req.n_pts > 1 ? req.n_pts - 1 : 1);
+            double hp;
+            if (req.log_scale) {
+                const double l0 = std::log10(req.param_lo), l1 = std::log10(req.param_hi);
+                hp = std::pow(10.0, req.continuation_reverse ? (l1 - (l1 - l0) * t) : (l0 + (l1 - l0) * t));
+            } else {
+                hp = req.continuation_reverse ? (req.param_hi - (req.param_hi - req.param_lo) * t)
+                                              : (req.param_lo + (req.param_hi - req.param_lo) * t);
+            }
+            scale = hp / req.h;
+        }
         for (int k = 0; k < f[c]; ++k) {
             res.bifurcation_points[c].push_back(pk[(size_t)k * n + c]);
-            res.peak_times[c].push_back(iv[(size_t)k * n + c]);
+            double v = iv[(size_t)k * n + c];
+            if (scale != 1.0) v *= scale;
+            res.peak_times[c].push_back(v);
         }
-    if (!req.csv_output_path.empty()) {
+    }
+    if (!req.csv_output_path.empty() && !req.continuation) {  // the reference writes no file for continuation
         std::ofstream cfg(req.csv_output_path + "_config.csv");
         csv::bif1d_config(cfg, req.base_values, req.initial_conditions, req.t_max, req.transient_time, req.h,
                           req.pre_scaller, req.writable_var, req.sweep_over_var ? req.var_sweep_index : req.param_index,
@@ -175,7 +200,7 @@ LLE1DResult ParametricEngine::run_lle_1d(const LLE1DRequest& req) {
     LLE1DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (req.continuation || req.use_cpu) { res.error = "continuation / CPU sweeps are not part of this engine"; return res; }
+    if (req.use_cpu) { res.error = "CPU sweeps are not part of this engine (use_cpu); the GPU path runs the same algorithm"; return res; }
     if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
     uc_system* s = impl_->system(req.system, res.error);
     if (!s) return res;
@@ -188,19 +213,23 @@ LLE1DResult ParametricEngine::run_lle_1d(const LLE1DRequest& req) {
     DevBuf<int> flags;
     if (!vals.alloc(n) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
     uc_lyap_output o{vals.p, flags.p, nullptr};
-    if (uc_run_lle(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    const int rc = req.continuation
+                       ? uc_run_lle_cont(impl_->eng, s, &r, req.continuation_reverse ? 1 : 0, &o, nullptr)
+                       : uc_run_lle(impl_->eng, s, &r, &o, nullptr);
+    if (rc != UC_OK) { res.error = impl_->err(); return res; }
     std::vector<double> v;
     if (hipDeviceSynchronize() != hipSuccess || !vals.download(v, n) || !flags.download(res.flags, n)) {
         res.error = "device error";
         return res;
     }
     res.n_pts = req.n_pts;
+    res.continuation_reverse = req.continuation && req.continuation_reverse;
     res.param_lo = req.param_lo;
     res.param_hi = req.param_hi;
     res.lyapunov.resize(n);
     for (size_t i = 0; i < n; ++i)
         res.lyapunov[i] = (res.flags[i] == 1) ? v[i] : std::numeric_limits<double>::quiet_NaN();
-    if (!req.csv_output_path.empty()) {
+    if (!req.csv_output_path.empty() && !req.continuation) {  // the reference writes no file for continuation
         std::ofstream cfg(req.csv_output_path + "_config.csv");
         csv::curve1d_config(cfg, "1D LLE", req.base_values, req.initial_conditions, req.t_max, req.NT,
                             req.transient_time, req.h, req.eps, req.param_index, req.param_lo, req.param_hi);
@@ -217,7 +246,7 @@ LS1DResult ParametricEngine::run_ls_1d(const LS1DRequest& req) {
     LS1DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (req.continuation || req.use_cpu) { res.error = "continuation / CPU sweeps are not part of this engine"; return res; }
+    if (req.use_cpu) { res.error = "CPU sweeps are not part of this engine (use_cpu); the GPU path runs the same algorithm"; return res; }
     if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
     uc_system* s = impl_->system(req.system, res.error);
     if (!s) return res;
@@ -230,20 +259,24 @@ LS1DResult ParametricEngine::run_ls_1d(const LS1DRequest& req) {
     DevBuf<int> flags;
     if (!vals.alloc(n * N) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
     uc_lyap_output o{vals.p, flags.p, nullptr};
-    if (uc_run_ls(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    const int rc = req.continuation
+                       ? uc_run_ls_cont(impl_->eng, s, &r, req.continuation_reverse ? 1 : 0, &o, nullptr)
+                       : uc_run_ls(impl_->eng, s, &r, &o, nullptr);
+    if (rc != UC_OK) { res.error = impl_->err(); return res; }
     std::vector<double> v;
     if (hipDeviceSynchronize() != hipSuccess || !vals.download(v, n * N) || !flags.download(res.flags, n)) {
         res.error = "device error";
         return res;
     }
     res.n_pts = req.n_pts;
+    res.continuation_reverse = req.continuation && req.continuation_reverse;
     res.n_exponents = (int)N;
     res.param_lo = req.param_lo;
     res.param_hi = req.param_hi;
     res.spectrum.assign(n, std::vector<double>(N));
     for (size_t i = 0; i < n; ++i)
         for (size_t k = 0; k < N; ++k) res.spectrum[i][k] = v[k * n + i];
-    if (!req.csv_output_path.empty()) {
+    if (!req.csv_output_path.empty() && !req.continuation) {  // the reference writes no file for continuation
         std::ofstream cfg(req.csv_output_path + "_config.csv");
         csv::curve1d_config(cfg, "1D LS", req.base_values, req.initial_conditions, req.t_max, req.NT,
                             req.transient_time, req.h, req.eps, req.param_index, req.param_lo, req.param_hi);
@@ -282,7 +315,7 @@ Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
     Dft1DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (req.continuation || req.use_cpu) { res.error = "continuation / CPU sweeps are not part of this engine"; return res; }
+    if (req.use_cpu) { res.error = "CPU sweeps are not part of this engine (use_cpu); the GPU path runs the same algorithm"; return res; }
     if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
     uc_system* s = impl_->system(req.system, res.error);
     if (!s) return res;
@@ -304,7 +337,11 @@ Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
     DevBuf<int> flags;
     if (!ak.alloc(n * nf) || !bk.alloc(n * nf) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
     uc_dft_output o{ak.p, bk.p, flags.p};
-    if (uc_run_dft(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
+    const int rc = req.continuation
+                       ? uc_run_dft_cont(impl_->eng, s, &r, req.continuation_reverse ? 1 : 0, &o, nullptr)
+                       : uc_run_dft(impl_->eng, s, &r, &o, nullptr);
+    if (rc != UC_OK) { res.error = impl_->err(); return res; }
+    res.continuation_reverse = req.continuation && req.continuation_reverse;
     if (!ak.download(res.ak_cos, n * nf) || !bk.download(res.bk_sin, n * nf) || !flags.download(res.flags, n)) {
         res.error = "device error";
         return res;

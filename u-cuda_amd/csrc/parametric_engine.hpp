@@ -52,6 +52,7 @@ struct Bifurcation1DResult {
     std::string error;
     double param_lo = 0.0, param_hi = 1.0;
     int n_pts = 0, record_steps = 0;
+    bool continuation_reverse = false;  // rows in visiting order: hi first when reversed
     std::vector<std::vector<double>> bifurcation_points;
     std::vector<std::vector<double>> peak_times;
     std::vector<int> flags;
@@ -154,6 +155,7 @@ struct LLE1DResult {
     double param_lo = 0.0, param_hi = 1.0;
     std::vector<double> lyapunov;  // NaN where the point diverged
     std::vector<int> flags;        // 1 ok, -1 diverged
+    bool continuation_reverse = false;
 };
 
 using LS1DRequest = LLE1DRequest;
@@ -162,8 +164,9 @@ struct LS1DResult {
     std::string error;
     int n_pts = 0, n_exponents = 0;
     double param_lo = 0.0, param_hi = 1.0;
-    std::vector<std::vector<double>> spectrum;  // 999 where the point diverged
+    std::vector<std::vector<double>> spectrum;  // 999 where the point diverged (NaN in continuation)
     std::vector<int> flags;
+    bool continuation_reverse = false;
 };
 
 struct LLE2DRequest {
