@@ -358,6 +358,24 @@ def test_bif2d_periods(gpu_engine):
     assert ((g32 <= 0) == (r <= 0)).mean() > 0.95
 
 
+def test_bif2d_periods_many_peaks(gpu_engine):
+    print("case: 2-D periods with > 1024 peaks per cell (two oscillators, x + pi v + e y, 250 s): full-capacity path",
+          flush=True)
+    s = ucuda.System(gpu_engine, ["x", "v", "y", "u"], ["w", "c"], ["v", "-(w^2)*x", "u", "-(c*w^2)*y"])
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 1, 4, 2.0, 31.4), ucuda.axis(ucuda.AXIS_PARAM, 2, 2, 2.25, 6.25)]
+    ic, vals = [1.0, 0.0, 3.0, 0.0], [0.5, 1.0, 1.0]
+    g = ucuda.run_bif_periods(s, axes, ic, vals, 0.01, 1.0, 250.0, eps_dbscan=0.1, writable_var=-1,
+                              strict=True).cpu().numpy()
+    step = oracle.KrsStep(gpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 4, 3)
+    r = oracle.bif_periods(step, oracle.make_job(4, axes, ic, vals, 0.01, 1.0, 250.0, writable_var=-1), range(8),
+                           0.1, threads=8)
+    np.testing.assert_array_equal(g, r)
+    fl = oracle.bif_cells(step, oracle.make_job(4, axes, ic, vals, 0.01, 1.0, 250.0, writable_var=-1,
+                                                max_peaks=20000), range(8))
+    assert fl["flags"].max() > 1024 and fl["flags"].min() < 1024  # both kernels took part
+    assert len(set(r.tolist())) >= 6
+
+
 def test_dft_hsweep(gpu_engine):
     print("case: DFT over h (Lorenz, 40 steps in [0.002, 0.06], per-cell blocks), strict vs oracle", flush=True)
     s = ucuda.preset("lorenz", gpu_engine)

@@ -857,52 +857,94 @@ int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, doub
     hipStream_t st = (hipStream_t)stream;
     if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
     fill_bif(&rq, &probe, iters, j);
-    // raw peaks of `iters` samples: candidates are >= 2 apart
+    // Raw peaks of `iters` samples are >= 2 apart: at most cap per cell. The
+    // first pass keeps up to cap1 = kDbWaveMax + 1 (the wave kernel's limit
+    // plus one); a chunk where some cell reports cap1 peaks may hold more,
+    // and is swept again with the full capacity, in sub-chunks, for the lane
+    // kernel (the sweep is deterministic, so its peaks are the same). This
+    // keeps the common case at ~8 KB of peaks per cell instead of ~80 KB.
     const long long cap = iters / 2 + 2;
     if (cap > 0x7fffffffLL) return e->fail(UC_ERR_ARGUMENT, "recorded window too long");
+    const long long cap1 = std::min<long long>(cap, kDbWaveMax + 1);
     size_t free_b = 0, total_b = 0;
     (void)hipMemGetInfo(&free_b, &total_b);
-    const size_t per_cell = (size_t)cap * (sizeof(double) + 2 * sizeof(int)) + sizeof(int);
     const long long total = j.n_cells;
-    long long chunk = (long long)std::min<size_t>((size_t)total, std::max<size_t>(kBlock, free_b / 4 / per_cell));
-    double* peaks = nullptr;
+    const long long chunk = (long long)std::min<size_t>(
+        (size_t)total, std::max<size_t>(kBlock, free_b / 4 / ((size_t)cap1 * sizeof(double) + sizeof(int))));
+    double *peaks = nullptr, *peaks_full = nullptr;
     int *flags = nullptr, *labels = nullptr, *slots = nullptr;
+    long long full_chunk = 0;
     auto release = [&]() {
         (void)hipStreamSynchronize(st);
-        for (void* p : {(void*)peaks, (void*)flags, (void*)labels, (void*)slots})
+        for (void* p : {(void*)peaks, (void*)peaks_full, (void*)flags, (void*)labels, (void*)slots})
             if (p) (void)hipFree(p);
     };
-    hipError_t he = hipMalloc((void**)&peaks, sizeof(double) * (size_t)cap * (size_t)chunk);
+    hipError_t he = hipMalloc((void**)&peaks, sizeof(double) * (size_t)cap1 * (size_t)chunk);
     if (he == hipSuccess) he = hipMalloc((void**)&flags, sizeof(int) * (size_t)chunk);
-    if (he == hipSuccess && cap > kDbWaveMax) he = hipMalloc((void**)&labels, sizeof(int) * (size_t)cap * (size_t)chunk);
-    if (he == hipSuccess && cap > kDbWaveMax) he = hipMalloc((void**)&slots, sizeof(int) * (size_t)cap * (size_t)chunk);
     if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(bif periods)")); }
     const char* kname = r->c.precision == UC_PREC_F64 ? "uc_bif_f64" : "uc_bif_f32c";
-    for (long long off = 0; off < total; off += chunk) {
-        const long long mcells = std::min(chunk, total - off);
+    auto sweep = [&](long long off, long long n, double* pk, long long mp) {
         UcJob jc = j;
         jc.cell_offset = off;
-        jc.n_cells = mcells;
-        jc.stride = mcells;
+        jc.n_cells = n;
+        jc.stride = n;
         jc.flag = flags;
-        jc.out0 = peaks;
+        jc.out0 = pk;
         jc.out1 = nullptr;
-        jc.max_peaks = (int)cap;
+        jc.max_peaks = (int)mp;
         jc.traj = nullptr;
         jc.traj_len = 0;
         jc.final_state = nullptr;
-        if ((rc = launch(e, m, kname, jc, st, tb))) { release(); return rc; }
-        if ((rc = launch_dbscan(e, m, peaks, flags, mcells, mcells, (int)cap, eps, labels, slots, periods + off, st))) {
-            release();
-            return rc;
+        return launch(e, m, kname, jc, st, tb);
+    };
+    std::vector<int> hflags;
+    for (long long off = 0; off < total; off += chunk) {
+        const long long mcells = std::min(chunk, total - off);
+        if ((rc = sweep(off, mcells, peaks, cap1))) { release(); return rc; }
+        // a count above kDbWaveMax (only possible when cap1 > kDbWaveMax) needs
+        // the lane kernel and its scratch: that chunk takes the full path
+        bool overflow = false;
+        if (cap1 > kDbWaveMax) {
+            hflags.resize((size_t)mcells);
+            he = hipMemcpyAsync(hflags.data(), flags, sizeof(int) * (size_t)mcells, hipMemcpyDeviceToHost, st);
+            if (he == hipSuccess) he = hipStreamSynchronize(st);
+            if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "bif periods counts")); }
+            for (int f : hflags) overflow |= f > kDbWaveMax;
+        }
+        if (!overflow) {
+            // every count <= kDbWaveMax here, so only the wave kernel runs
+            const int max_n = (int)std::min<long long>(cap1, kDbWaveMax);
+            if ((rc = launch_dbscan(e, m, peaks, flags, mcells, mcells, max_n, eps, nullptr, nullptr, periods + off,
+                                    st))) {
+                release();
+                return rc;
+            }
+            continue;
+        }
+        if (!peaks_full) {
+            const size_t per_cell = (size_t)cap * (sizeof(double) + 2 * sizeof(int));
+            full_chunk = (long long)std::min<size_t>((size_t)chunk, std::max<size_t>(kBlock, free_b / 4 / per_cell));
+            he = hipMalloc((void**)&peaks_full, sizeof(double) * (size_t)cap * (size_t)full_chunk);
+            if (he == hipSuccess) he = hipMalloc((void**)&labels, sizeof(int) * (size_t)cap * (size_t)full_chunk);
+            if (he == hipSuccess) he = hipMalloc((void**)&slots, sizeof(int) * (size_t)cap * (size_t)full_chunk);
+            if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(bif periods, full)")); }
+        }
+        for (long long sub = off; sub < off + mcells; sub += full_chunk) {
+            const long long ns = std::min(full_chunk, off + mcells - sub);
+            if ((rc = sweep(sub, ns, peaks_full, cap))) { release(); return rc; }
+            if ((rc = launch_dbscan(e, m, peaks_full, flags, ns, ns, (int)cap, eps, labels, slots, periods + sub,
+                                    st))) {
+                release();
+                return rc;
+            }
         }
     }
     release();
     return UC_OK;
 }
 
-static const int kDftCells = 4;    // UC_DFT_C in the kernel source: cells per DFT lane
-static const int kDftSamples = 4;  // UC_DFT_U: samples per load block (traj / window rows padded to it)
+static const int kDftCells = 8;    // UC_DFT_C in the kernel source: cells per DFT workgroup
+static const int kDftSamples = 4;  // samples per load block of uc_dft_cells (traj / window rows padded to it)
 
 // Per-cell DFT tables for blocks of per-cell length (the h-sweep and the
 // continuation DFT, dft1d_hsweep.template.cu / dft1d_cont.template.cu): block
@@ -1204,9 +1246,8 @@ int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft
         double* A = o->ak_cos + off * nf;
         double* B = o->bk_sin + off * nf;
         void* args[] = {&traj, &stride, &mcells, &Ni, &fl, &dwin, &nf, &dtab, &dres, &nr, &A, &B};
-        const long long lanes = (mcells + kDftCells - 1) / kDftCells * ((nf + 63) / 64) * 64;
-        he = hipModuleLaunchKernel(fdft, (unsigned)((lanes + kBlock - 1) / kBlock), 1, 1, kBlock, 1, 1, 0, st, args,
-                                   nullptr);
+        const long long wgs = (mcells + kDftCells - 1) / kDftCells * ((nf + kBlock - 1) / kBlock);
+        he = hipModuleLaunchKernel(fdft, (unsigned)wgs, 1, 1, kBlock, 1, 1, 0, st, args, nullptr);
         if (he != hipSuccess) { release(); return e->fail(UC_ERR_HIP, hip_msg(he, "uc_dft launch")); }
     }
     release();

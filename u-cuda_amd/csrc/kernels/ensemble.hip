@@ -1081,20 +1081,16 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan(const doubl
 // is sample-major ([n * stride + cell]); the coefficients are cell-major
 // ([cell * n_freq + k]).
 //
-// Work split: one wave per (group of UC_DFT_C consecutive cells, 64
-// consecutive frequencies). The rotation depends on the frequency alone, so
-// each lane advances it once per sample for the whole group; each cell's
-// sums see the same operations in the same order as a lane of its own
-// would. The group's samples are the same for every lane, so they are read
-// with scalar loads (UC_DFT_C doubles per sample), which keeps the vector
-// memory path, whose per-lane address work would otherwise bound the
-// kernel, out of the sample loop. The loads are unguarded: the caller pads
-// traj to ceil(N / UC_DFT_U) * UC_DFT_U rows plus UC_DFT_C doubles and
-// window to the same row count; samples beyond N or cells beyond n_cells
-// are read but never used.
-#define UC_DFT_U 4
-#define UC_DFT_C 4
-static_assert(1000 % UC_DFT_U == 0, "DFT sample blocks must tile the 1000-sample resync period");
+// Work split: one workgroup per (group of UC_DFT_C consecutive cells,
+// UC_BLOCK consecutive frequencies), one thread per frequency. The rotation
+// depends on the frequency alone, so each thread advances it once per
+// sample for the whole group; each cell's sums see the same operations in
+// the same order as a thread of its own would. Blocks of UC_DFT_U samples of
+// the group's cells (and the window) are staged in LDS by one coalesced
+// load per thread, then every thread reads them as LDS broadcasts.
+#define UC_DFT_U 32
+#define UC_DFT_C 8
+static_assert(UC_DFT_U * UC_DFT_C <= UC_BLOCK, "one staged sample per thread");
 
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __restrict__ traj, long long stride,
                                                                 long long n_cells, int N, const int* __restrict__ flags,
@@ -1102,15 +1098,12 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __re
                                                                 const double* __restrict__ tab,
                                                                 const double* __restrict__ resync, int n_res,
                                                                 double* __restrict__ ak, double* __restrict__ bk) {
-    const long long n_groups = (n_cells + UC_DFT_C - 1) / UC_DFT_C;
-    const int kchunks = (n_freq + 63) >> 6;
-    const long long wave = __builtin_amdgcn_readfirstlane(
-        (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-    const long long g = wave / kchunks;
-    if (g >= n_groups) return;  // wave-uniform
-    const int lane = threadIdx.x & 63;
-    const int k_raw = (int)(wave - g * kchunks) * 64 + lane;
-    const int k = min(k_raw, n_freq - 1);  // spare lanes repeat the last frequency and store nothing
+    __shared__ double xs[UC_DFT_U][UC_DFT_C];
+    __shared__ double ws[UC_DFT_U];
+    const int kblocks = (n_freq + UC_BLOCK - 1) / UC_BLOCK;
+    const long long g = (long long)blockIdx.x / kblocks;
+    const int k_raw = (int)((long long)blockIdx.x - g * kblocks) * UC_BLOCK + (int)threadIdx.x;
+    const int k = min(k_raw, n_freq - 1);  // spare threads repeat the last frequency and store nothing
     const long long c0 = g * UC_DFT_C;
     const int nc = (int)min((long long)UC_DFT_C, n_cells - c0);
     const double ct = tab[2 * k], st = tab[2 * k + 1];
@@ -1118,39 +1111,38 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __re
 #pragma unroll
     for (int c = 0; c < UC_DFT_C; ++c) a[c] = b[c] = 0.0;
     double cn = 1.0, sn = 0.0;
-    int r = 0;
-    // Samples in blocks of UC_DFT_U, all loads of a block issued before its
-    // arithmetic. UC_DFT_U divides 1000, so a resync (after sample n with
-    // (n + 1) % 1000 == 0 and n + 1 < N) can only fall on a block's last
-    // sample.
-    const double* x = traj + c0;
+    int until = 1000, r = 0;
+    // this thread's staging slot: sample lu of the block, cell lc of the group
+    const int lu = (int)threadIdx.x / UC_DFT_C, lc = (int)threadIdx.x % UC_DFT_C;
     for (int n0 = 0; n0 < N; n0 += UC_DFT_U) {
-        double xv[UC_DFT_U][UC_DFT_C], wv[UC_DFT_U];
-#pragma unroll
-        for (int u = 0; u < UC_DFT_U; ++u) {
-            wv[u] = window[n0 + u];
-#pragma unroll
-            for (int c = 0; c < UC_DFT_C; ++c) xv[u][c] = x[(long long)(n0 + u) * stride + c];
+        __syncthreads();  // the previous block has been consumed
+        if (lu < UC_DFT_U) {
+            const int n = n0 + lu;
+            xs[lu][lc] = (n < N && lc < nc) ? traj[(long long)n * stride + c0 + lc] : 0.0;
+            if (lc == 0) ws[lu] = n < N ? window[n] : 0.0;
         }
+        __syncthreads();
+        const int m = min(UC_DFT_U, N - n0);
+        for (int u = 0; u < m; ++u) {
+            const double w = ws[u];
 #pragma unroll
-        for (int u = 0; u < UC_DFT_U; ++u) {
-            if (n0 + u < N) {
-#pragma unroll
-                for (int c = 0; c < UC_DFT_C; ++c) {
-                    const double wd = wv[u] * xv[u][c];
-                    a[c] = a[c] + wd * cn;
-                    b[c] = b[c] + wd * sn;
-                }
-                const double nc_ = cn * ct - sn * st;
-                const double ns_ = sn * ct + cn * st;
-                cn = nc_;
-                sn = ns_;
+            for (int c = 0; c < UC_DFT_C; ++c) {
+                const double wd = w * xs[u][c];
+                a[c] = a[c] + wd * cn;
+                b[c] = b[c] + wd * sn;
             }
-        }
-        if ((n0 + UC_DFT_U) % 1000 == 0 && n0 + UC_DFT_U < N) {
-            cn = resync[2 * ((long long)k * n_res + r)];
-            sn = resync[2 * ((long long)k * n_res + r) + 1];
-            ++r;
+            const double nc_ = cn * ct - sn * st;
+            const double ns_ = sn * ct + cn * st;
+            cn = nc_;
+            sn = ns_;
+            if (--until == 0) {
+                until = 1000;
+                if (n0 + u + 1 < N) {
+                    cn = resync[2 * ((long long)k * n_res + r)];
+                    sn = resync[2 * ((long long)k * n_res + r) + 1];
+                    ++r;
+                }
+            }
         }
     }
     if (k_raw >= n_freq) return;
@@ -1177,8 +1169,12 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft(const double* __re
 // win[win_off[cell]], rotation tab[2 (cell * n_freq + k)] and resync values
 // res[res_off[cell] + 2 (k * n_res + r)] with n_res = (N - 1) / 1000. One
 // wave per (cell, 64 frequencies); samples and window through scalar loads.
-// Loads are unguarded: traj holds ceil(max N / UC_DFT_U) * UC_DFT_U rows and
-// each window row is padded to a multiple of UC_DFT_U.
+// Loads are unguarded: traj holds ceil(max N / UC_DFTC_U) * UC_DFTC_U rows and
+// each window row is padded to a multiple of UC_DFTC_U. UC_DFTC_U divides
+// 1000, so a resync can only fall on a block's last sample.
+#define UC_DFTC_U 4
+static_assert(1000 % UC_DFTC_U == 0, "DFT sample blocks must tile the 1000-sample resync period");
+
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft_cells(
     const double* __restrict__ traj, long long stride, long long n_cells, const int* __restrict__ Ns,
     const int* __restrict__ flags, const double* __restrict__ win, const long long* __restrict__ win_off, int n_freq,
@@ -1208,15 +1204,15 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft_cells(
     const double* rs = res + res_off[cell] + 2LL * k * n_res;
     double a = 0.0, b = 0.0, cn = 1.0, sn = 0.0;
     int r = 0;
-    for (int n0 = 0; n0 < N; n0 += UC_DFT_U) {
-        double xv[UC_DFT_U], wv[UC_DFT_U];
+    for (int n0 = 0; n0 < N; n0 += UC_DFTC_U) {
+        double xv[UC_DFTC_U], wv[UC_DFTC_U];
 #pragma unroll
-        for (int u = 0; u < UC_DFT_U; ++u) {
+        for (int u = 0; u < UC_DFTC_U; ++u) {
             wv[u] = w[n0 + u];
             xv[u] = x[(long long)(n0 + u) * stride];
         }
 #pragma unroll
-        for (int u = 0; u < UC_DFT_U; ++u) {
+        for (int u = 0; u < UC_DFTC_U; ++u) {
             if (n0 + u < N) {
                 const double wd = wv[u] * xv[u];
                 a = a + wd * cn;
@@ -1227,7 +1223,7 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_dft_cells(
                 sn = ns_;
             }
         }
-        if ((n0 + UC_DFT_U) % 1000 == 0 && n0 + UC_DFT_U < N) {
+        if ((n0 + UC_DFTC_U) % 1000 == 0 && n0 + UC_DFTC_U < N) {
             cn = rs[2 * r];
             sn = rs[2 * r + 1];
             ++r;
