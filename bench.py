@@ -256,8 +256,16 @@ def extra_bif2d(eng, n=512, stride=8):
     sample_steps = len(cells) * (10000 + 1 + 10000 + 1)
     out["cpu_oracle"] = dict(cells=int(len(cells)), s=cpu_s, traj_steps_per_s=sample_steps / cpu_s, cores=1)
     out["f64_strict"]["identical_to_oracle"] = bool(np.array_equal(maps["f64_strict"][cells], ref))
-    for name in ("f64", "f32c"):
+    # cells whose first 256 peaks survive rounding-sized noise of the mode (the
+    # rest are chaotic: any rounding difference changes their peak set)
+    threads_all = min(16, os.cpu_count() or 1)
+    jobp = oracle.make_job(3, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 100.0, 100.0, max_peaks=256)
+    base = oracle.bif_cells(step, jobp, cells, threads=threads_all)
+    for name, prec in (("f64", "f64"), ("f32c", "f32")):
+        st = oracle.bif_rounding_stable(step, jobp, cells, base, prec, 1e-6, threads=threads_all)
         out[name]["period_agreement_vs_oracle"] = float((maps[name][cells] == ref).mean())
+        out[name]["stable_cells"] = int(st.sum())
+        out[name]["period_agreement_on_stable_cells"] = float((maps[name][cells][st] == ref[st]).mean()) if st.any() else None
     out["periods_histogram"] = {int(k): int(v) for k, v in zip(*np.unique(maps["f64_strict"], return_counts=True))
                                 if k <= 8 or v > n * n // 100}
     return out
@@ -298,11 +306,18 @@ def extra_dft(eng, n=16384, n_freq=200, stride=64):
         np.array_equal(g["flags"][cells], ref["flags"]) and np.array_equal(g["ak"][cells], ref["ak"])
         and np.array_equal(g["bk"][cells], ref["bk"]))
     ok = ref["flags"] == 1
-    amp = np.hypot(ref["ak"], ref["bk"])[ok]
-    for name in ("f64", "f32c"):
-        a = np.hypot(res[name]["ak"][cells], res[name]["bk"][cells])[ok]
+    amp = np.hypot(ref["ak"], ref["bk"])
+    threads_all = min(16, os.cpu_count() or 1)
+    jobp = oracle.make_job(3, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 100.0, 100.0, max_peaks=256)
+    base = oracle.bif_cells(step, jobp, cells, threads=threads_all)
+    for name, prec in (("f64", "f64"), ("f32c", "f32")):
+        a = np.hypot(res[name]["ak"][cells], res[name]["bk"][cells])
+        err = np.abs(a - amp).max(axis=1) / np.maximum(amp.max(axis=1), 1e-300)
+        st = ok & oracle.bif_rounding_stable(step, jobp, cells, base, prec, 1e-6, threads=threads_all)
         out[name]["flag_agreement"] = float((res[name]["flags"][cells] == ref["flags"]).mean())
-        out[name]["median_amp_err_rel_max"] = float(np.median(np.abs(a - amp).max(axis=1) / amp.max(axis=1)))
+        out[name]["median_amp_err_rel_max"] = float(np.median(err[ok]))
+        out[name]["stable_cells"] = int(st.sum())
+        out[name]["max_amp_err_rel_on_stable_cells"] = float(err[st].max()) if st.any() else None
     out["config"] = dict(cells=n, n_freq=n_freq, samples_per_block=N, window="hanning")
     return out
 
