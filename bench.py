@@ -379,6 +379,46 @@ def extra_fastsync(eng, n=1024, stride=64):
     return out
 
 
+def extra_continuation(eng, n_pts=128, n_chains=1024):
+    """Continuation LLE chains, Lorenz (sigma, 8/3), rho walked upwards over
+    [10, 200] in n_pts points, X0 = (0.1, 0.1, 0.1), RK4 h = 0.01, transient
+    5 (first point) then (int)(5 / NT) settle blocks per carried point,
+    t_max 20, NT = 0.1, finite-difference probe eps 1e-4. One chain is the
+    reference's case (one GPU thread there, one lane here: latency-bound);
+    n_chains independent chains over sigma in [6, 14] is the GPU-shaped use.
+    GPU: whole-call wall time, strict FP64; parity of the single chain against
+    the oracle (1e-12: the device log may round the last bit differently);
+    CPU: the oracle's single chain, one thread."""
+    import numpy as np
+    import ucuda
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    lor = ucuda.preset("lorenz", eng)
+    vals = lor.values()
+    one = [ucuda.axis(ucuda.AXIS_PARAM, 2, n_pts, 10.0, 200.0)]
+    many = one + [ucuda.axis(ucuda.AXIS_PARAM, 1, n_chains, 6.0, 14.0)]
+    args = ([0.1, 0.1, 0.1], vals, 0.01, 5.0, 20.0, 0.1)
+    # state + probe steps per chain: 500 + 1 transient, then per point (settle + 200 blocks) x 11 x 2
+    steps = 501 + n_pts * 200 * 11 * 2 + (n_pts - 1) * 50 * 11 * 2
+    out = {}
+    ms, r1 = _timed_calls(lambda: ucuda.run_lyap_cont(lor, one, *args, strict=True), reps=1)
+    out["one_chain"] = dict(ms=float(np.mean(ms)), steps_per_s=steps / (np.mean(ms) / 1e3))
+    progress("continuation: one chain %.1f ms" % np.mean(ms))
+    ms, rm = _timed_calls(lambda: ucuda.run_lyap_cont(lor, many, *args, strict=True), reps=1)
+    out["%d_chains" % n_chains] = dict(ms=float(np.mean(ms)), steps_per_s=n_chains * steps / (np.mean(ms) / 1e3))
+    progress("continuation: %d chains %.1f ms" % (n_chains, np.mean(ms)))
+    step = oracle.KrsStep(eng.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
+    job = oracle.make_job(3, one, [0.1, 0.1, 0.1], vals, 0.01, 5.0, 20.0, NT=0.1, eps=1e-4)
+    t0 = time.perf_counter()
+    ref = oracle.cont_lyap(step, job, 1, threads=1)
+    cpu_s = time.perf_counter() - t0
+    out["cpu_oracle_one_chain"] = dict(s=cpu_s, steps_per_s=steps / cpu_s, cores=1)
+    g = r1["values"].cpu().numpy()[0]
+    out["one_chain"]["max_abs_err_vs_oracle"] = float(np.nanmax(np.abs(g - ref["values"][0])))
+    out["one_chain"]["flags_equal"] = bool(np.array_equal(r1["flags"].cpu().numpy(), ref["flags"]))
+    return out
+
+
 def extra_ls(eng, n=1024, steps_scale=1.0):
     """hyper-Rossler 4-exponent spectrum maps, h=0.005, Gram-Schmidt every 20
     steps, 1e5 steps, FP32-compensated and FP64:
@@ -791,6 +831,8 @@ def main():
             ("dft_lorenz_16384", "DFT sweep (Lorenz 16384 rho x 200 freqs)", lambda: extra_dft(eng)),
             ("fastsync_lorenz", "fast synchro (Lorenz grid 1024^2, 10000 windows on the attractor)",
              lambda: extra_fastsync(eng)),
+            ("continuation_lorenz", "continuation LLE chains (Lorenz, 128 rho points; 1 and 1024 chains)",
+             lambda: extra_continuation(eng)),
             ("ls_hyper_rossler_1024", "LS spectrum map (hyper-Rossler 1024^2)", lambda: extra_ls(eng)),
             ("basins_duffing_2048", "basins (Duffing 2048^2)", lambda: extra_basins(eng)),
             ("batch_64_jobs", "batch (64 jobs)", lambda: extra_batch(eng)),
