@@ -1360,6 +1360,35 @@ __device__ void uc_cont_probes(UcRng& r, double* y) {
     }
 }
 
+// M independent trajectories stepped together for `iters` steps plus the
+// trailing step, each with the step loop's magnitude checks (after
+// iterations 0, 100, 200, ...): 0 when any of them escapes, else 1. The
+// same operations per trajectory as M separate uc_loop calls; advancing
+// them together gives a lone chain M independent dependency chains to
+// interleave. (Only the escape matters to the chains, not the fixed-point
+// test of the trailing step.)
+template <int M>
+__device__ int uc_loop_multi(StateF64* S, const double* a, double h, double& t, long long iters, double max_value) {
+    int until_check = 0;
+    for (long long i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int m = 0; m < M; ++m) uc_rk4(S[m], a, h, t);
+        if (UC_USES_TIME) t += h;
+        if (until_check == 0) {
+            bool esc = false;
+#pragma unroll
+            for (int m = 0; m < M; ++m) esc = esc || uc_diverged(S[m], max_value);
+            if (esc) return 0;
+            until_check = UC_CHECK_INTERVAL;
+        }
+        --until_check;
+    }
+#pragma unroll
+    for (int m = 0; m < M; ++m) uc_rk4(S[m], a, h, t);
+    if (UC_USES_TIME) t += h;
+    return 1;
+}
+
 template <int K>
 __device__ void uc_cont_lyap_chain(const UcJob& J) {
     const long long chain = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1409,18 +1438,19 @@ __device__ void uc_cont_lyap_chain(const UcJob& J) {
 #pragma unroll
         for (int k = 0; k < K; ++k) sum[k] = 0.0;
         for (long long b = 0; alive && b < settle + n_blocks; ++b) {
-            const double t0 = t;  // the probes cover the same interval as the state
-            if (uc_loop(s, ad, h, t, nts, 1, 0, J.max_value, none) == 0) { alive = false; break; }
-            for (int k = 0; k < K && alive; ++k) {
-                StateF64 p;
+            // the state and its K probes over the same block, together
+            StateF64 S[K + 1];
+            S[0] = s;
 #pragma unroll
-                for (int i = 0; i < UC_N; ++i) p.x[i] = y[k * UC_N + i];
-                double tp = t0;
-                if (uc_loop(p, ad, h, tp, nts, 1, 0, J.max_value, none) == 0) alive = false;
+            for (int k = 0; k < K; ++k)
 #pragma unroll
-                for (int i = 0; i < UC_N; ++i) y[k * UC_N + i] = p.x[i];
-            }
-            if (!alive) break;
+                for (int i = 0; i < UC_N; ++i) S[k + 1].x[i] = y[k * UC_N + i];
+            if (uc_loop_multi<K + 1>(S, ad, h, t, nts, J.max_value) == 0) { alive = false; break; }
+            s = S[0];
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) y[k * UC_N + i] = S[k + 1].x[i];
             if (K == 1) {
                 double d = 0.0;
 #pragma unroll
