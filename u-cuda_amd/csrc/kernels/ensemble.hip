@@ -115,6 +115,8 @@ struct StateF32C {
 #define UC_LN2 0.6931471805599453
 __device__ __forceinline__ float uc_ldexp(float x, int e) { return __builtin_ldexpf(x, e); }
 __device__ __forceinline__ double uc_ldexp(double x, int e) { return __builtin_ldexp(x, e); }
+__device__ __forceinline__ int uc_frexp_exp(float x) { int e; (void)__builtin_frexpf(x, &e); return e; }
+__device__ __forceinline__ int uc_frexp_exp(double x) { int e; (void)__builtin_frexp(x, &e); return e; }
 
 __device__ __forceinline__ void uc_kahan_add(float& hi, float& c, float incr) {
     const float y = incr - c;
@@ -305,6 +307,17 @@ __device__ __forceinline__ bool uc_diverged(const S& s, double max_value) {
     return max_value != 0.0 && fabs(chk) > max_value;
 }
 
+// F32C: the magnitude test sums the float state hi. The compensation c is
+// below half an ulp of hi, so the sum moves by < 2^-24 relative, and the
+// test stays in FP32 (no conversions to double on the step path).
+__device__ __forceinline__ bool uc_diverged(const StateF32C& s, double max_value) {
+    float chk = 0.0f;
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) chk = chk + fabsf(s.hi[i]);
+    if (!(chk <= __FLT_MAX__)) return true;  // NaN or Inf
+    return max_value != 0.0 && (double)chk > max_value;
+}
+
 template <class S>
 __device__ __forceinline__ int uc_fixed_point_flag(const S& s, const double* prev, double h, int pre) {
     double d = 0.0, m = 0.0;
@@ -468,6 +481,32 @@ __device__ int uc_loop_tan_u(S& s, typename Prec<S>::T* V, const typename Prec<S
     uc_rk4_tan<K>(s, V, a, h, (T)t);
     if (UC_USES_TIME) t += hd;
     return uc_fixed_point_flag(s, prev, hd, 1);
+}
+
+// One block of uc_loop_tan_u for 1 <= iters <= UC_CHECK_INTERVAL, where the
+// magnitude check falls on iteration 0 only: that step, the check, then
+// `iters` more steps (the remaining iterations and the trailing step) in
+// one loop with a 32-bit counter, run by the live lanes only, so a lane
+// found diverged keeps its state and time at the check. Returns 0 for such
+// a lane, else 1: callers that only need the divergence verdict skip the
+// trailing fixed-point test, which cannot return 0. Same steps, states and
+// dead-lane rule as uc_loop_tan_u.
+template <int K, class S>
+__device__ __forceinline__ int uc_block_tan_u(S& s, typename Prec<S>::T* V, const typename Prec<S>::T* a,
+                                              typename Prec<S>::T h, double& t, int iters, double max_value) {
+    typedef typename Prec<S>::T T;
+    uc_rk4_tan<K>(s, V, a, h, (T)t);
+    if (UC_USES_TIME) t += (double)h;
+    const bool dead = uc_diverged(s, max_value);
+    if (uc_wave_all(dead)) return 0;
+    if (!dead) {
+#pragma unroll 2
+        for (int i = 0; i < iters; ++i) {
+            uc_rk4_tan<K>(s, V, a, h, (T)t);
+            if (UC_USES_TIME) t += (double)h;
+        }
+    }
+    return dead ? 0 : 1;
 }
 
 // ------------------------------------------------------ streaming peaks ---
@@ -733,47 +772,71 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
             // whose growth |V_end| / |V_start| would reach the reference's
             // 1e-14 floor switches the lane to the reference's unit
             // renormalisation (with the sum so far) for the rest of the run.
+            // The norm and the exponent are taken in the state's precision
+            // (FP32 for F32C): a power-of-two rescale is exact in either,
+            // and every double-precision quantity of the sum is formed from
+            // the rescaled vector itself, at the end or at the switch.
             // K > 1: Gram-Schmidt every block as the reference; the norms
             // are multiplied into a mantissa / exponent pair instead of one
             // log per norm and block.
             int ksum = 0;
             bool ref_mode = false;
-            double start2 = 1.0, lsum = 0.0;
+            T start2 = (T)1.0;        // |V|^2 at the block start, for the floor test
+            T Vs[UC_N];               // V at the block start, for the switch
+            double lsum = 0.0;
+            const T big = sizeof(T) == 4 ? (T)__FLT_MAX__ : (T)1e300;
+#pragma unroll
+            for (int i = 0; i < UC_N; ++i) Vs[i] = V[i];
             double prod[K];
             int esum[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) { prod[k] = 1.0; esum[k] = 0; }
+            // one check per block (iteration 0) when the block is no longer
+            // than the check interval and the counts are wave-uniform
+            const bool one_check = K == 1 && uni && j.nt_steps >= 1 && j.nt_steps <= UC_CHECK_INTERVAL;
+#else
+            const bool one_check = false;
 #endif
             for (long long b = 0; b < j.n_iters; ++b) {
-                const int r = uni ? uc_loop_tan_u<K>(s, V, a, h, t, j.nt_steps, j.max_value)
-                                  : uc_loop_tan<K>(s, V, a, h, t, nts, j.max_value);
+                const int r = one_check ? uc_block_tan_u<K>(s, V, a, h, t, (int)j.nt_steps, j.max_value)
+                              : uni     ? uc_loop_tan_u<K>(s, V, a, h, t, j.nt_steps, j.max_value)
+                                        : uc_loop_tan<K>(s, V, a, h, t, nts, j.max_value);
                 if (r == 0) { alive = false; break; }
                 double v[K * UC_N];
-#pragma unroll
-                for (int i = 0; i < K * UC_N; ++i) v[i] = (double)V[i];
                 if (K == 1) {
 #if !UC_STRICT
-                    double n2 = 0.0;
+                    T n2 = (T)0.0;
 #pragma unroll
-                    for (int i = 0; i < UC_N; ++i) n2 = n2 + v[i] * v[i];
-                    if (!ref_mode && n2 > 1e-28 * start2 && n2 <= 1e300) {
-                        int e;
-                        (void)__builtin_frexp(n2, &e);
-                        const int k = e >> 1;  // floor(e / 2)
+                    for (int i = 0; i < UC_N; ++i) n2 = n2 + V[i] * V[i];
+                    if (!ref_mode && n2 > (T)1e-28 * start2 && n2 <= big) {
+                        const int k = uc_frexp_exp(n2) >> 1;  // floor(e / 2)
 #pragma unroll
-                        for (int i = 0; i < UC_N; ++i) V[i] = uc_ldexp(V[i], -k);
+                        for (int i = 0; i < UC_N; ++i) {
+                            V[i] = uc_ldexp(V[i], -k);
+                            Vs[i] = V[i];
+                        }
                         ksum += k;
-                        start2 = __builtin_ldexp(n2, -2 * k);
+                        start2 = uc_ldexp(n2, -2 * k);
                         continue;
                     }
+                    double inv0 = 1.0;
                     if (!ref_mode) {
                         // the reference's sum up to this block and its block-start unit vector
-                        lsum = (double)ksum * UC_LN2 + 0.5 * log(start2);
-                        const double inv0 = 1.0 / sqrt(start2);
+                        double s2 = 1.0;
+                        if (b > 0) {
+                            s2 = 0.0;
 #pragma unroll
-                        for (int i = 0; i < UC_N; ++i) v[i] = v[i] * inv0;
+                            for (int i = 0; i < UC_N; ++i) s2 = s2 + (double)Vs[i] * (double)Vs[i];
+                        }
+                        lsum = (double)ksum * UC_LN2 + 0.5 * log(s2);
+                        inv0 = 1.0 / sqrt(s2);
                         ref_mode = true;
                     }
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) v[i] = (double)V[i] * inv0;
+#else
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) v[i] = (double)V[i];
 #endif
                     double nrm = 0.0;
 #pragma unroll
@@ -789,6 +852,8 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
 #pragma unroll
                     for (int i = 0; i < UC_N; ++i) v[i] = v[i] * inv;
                 } else {
+#pragma unroll
+                    for (int i = 0; i < K * UC_N; ++i) v[i] = (double)V[i];
                     uc_mgs<K>(v, d);
 #pragma unroll
                     for (int k = 0; k < K; ++k) {
@@ -805,8 +870,15 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
                 for (int i = 0; i < K * UC_N; ++i) V[i] = Prec<S>::to_t(v[i]);
             }
 #if !UC_STRICT
-            if (K == 1) sum[0] = ref_mode ? lsum : (double)ksum * UC_LN2 + 0.5 * log(start2);
-            else {
+            if (K == 1) {
+                double s2 = 1.0;
+                if (j.n_iters > 0) {
+                    s2 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) s2 = s2 + (double)V[i] * (double)V[i];
+                }
+                sum[0] = ref_mode ? lsum : (double)ksum * UC_LN2 + 0.5 * log(s2);
+            } else {
 #pragma unroll
                 for (int k = 0; k < K; ++k) sum[k] = log(prod[k]) + (double)esum[k] * UC_LN2;
             }
