@@ -20,6 +20,13 @@ import os
 import sys
 import time
 
+# The serving measurement keeps four rounds in flight on four HIP streams.
+# HIP maps a process's streams onto at most GPU_MAX_HW_QUEUES hardware queues
+# (default 4, some taken by other streams), and rounds sharing a queue run
+# one after another; eight queues give each round stream its own. Must be
+# set before the HIP runtime starts; a value set by the caller wins.
+os.environ.setdefault("GPU_MAX_HW_QUEUES", "8")
+
 _T0 = time.perf_counter()
 
 
@@ -618,7 +625,9 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
     The arrival rate is set for ~70% of `capacity` (LLE traj-steps/s),
     weighting a bifurcation step 0.5 and a basin section 410 LLE steps (the
     measured FP32C rates of the three kernels).
-    Reports throughput and p50 / p99 job latency; every 100th job is run
+    Reports throughput and p50 / p99 job latency, the slowdown (latency over
+    the job's estimated time alone at the measured rates) and p99 latency of
+    the smaller and larger half of the jobs by size; every 100th job is run
     again alone in FP64 and compared."""
     import numpy as np
     import ucuda
@@ -718,11 +727,25 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
             mapped = np.array([m.get(int(v), 0 if v == 0 else -99) for v in got["labels"]])
             audit.append(dict(kind=kind, cells=side * side,
                               label_agreement=float((mapped == ref["labels"].cpu().numpy()).mean())))
+    # per-job latency against the job's own size: the estimated time it
+    # would take alone on the whole GPU at the measured rates
+    lat = np.array([srv.poll(jid)[1] for jid, *_ in ids])
     srv.close()
+    costs = np.array([cost(kind, side, steps) for _, kind, side, steps in ids])
+    sizes = {}
+    if capacity and len(ids):
+        solo_ms = costs / capacity * 1e3
+        slow = lat / solo_ms
+        small = costs <= np.median(costs)
+        sizes = dict(solo_est_ms_p50=float(np.percentile(solo_ms, 50)),
+                     solo_est_ms_p99=float(np.percentile(solo_ms, 99)),
+                     slowdown_p50=float(np.percentile(slow, 50)), slowdown_p99=float(np.percentile(slow, 99)),
+                     p99_ms_smaller_half=float(np.percentile(lat[small], 99)),
+                     p99_ms_larger_half=float(np.percentile(lat[~small], 99)))
     kinds = {k: sum(1 for _, kk, _, _ in ids if kk == k) for k in ("lle", "bif", "basins")}
     return dict(arrival_rate_per_s=rate, offered_utilisation_est=rate * est / capacity if capacity else None,
                 jobs=len(ids), jobs_by_kind=kinds, completed=st["completed"], failed=len(failed),
-                p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], max_ms=st["max_ms"],
+                p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], max_ms=st["max_ms"], **sizes,
                 fixed_step_traj_steps_per_s=st["traj_steps"] / wall, rounds=st["rounds"],
                 gpu_busy_frac=st["busy_s"] / wall, wall_s=wall, audit_fp64=audit)
 

@@ -2010,6 +2010,7 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
 // device has one (hipFree synchronises the whole device).
 struct uc_server {
     static constexpr int kSlots = 4;
+    static constexpr int kLongClass = 4;   // cells of >= 1e4 steps: at most kSlots - 1 slots
     static constexpr int kMaxGroups = 64;  // counters per slot
     uc_engine* e = nullptr;
     uc_server_config cfg{};
@@ -2064,6 +2065,7 @@ struct uc_server {
         long long* h_start = nullptr;
         size_t cap = 0;               // entries (jobs); starts hold cap + kMaxGroups
         bool busy = false;
+        int cls = -1;                 // duration class of the round in flight
         std::vector<Entry> entries;
     };
     std::mutex mu;
@@ -2075,6 +2077,7 @@ struct uc_server {
     size_t used_bytes = 0, budget = 0;
     bool stop = false;
     bool async_alloc = false;
+    hipMemPool_t pool = nullptr;   // the server's own pool; keeps up to the budget mapped
     std::thread worker;
     hipStream_t stream = nullptr;  // admission uploads, fetch copies, frees
     Slot slots[kSlots];
@@ -2083,6 +2086,7 @@ struct uc_server {
     std::vector<double> latencies_ms;
     double busy_s = 0.0, traj_steps_done = 0.0;
     long long rounds = 0;
+    int last_class = -1;  // duration class of the last round picked
     std::chrono::steady_clock::time_point t_start;
     std::string error;
 };
@@ -2090,7 +2094,7 @@ struct uc_server {
 namespace {
 
 hipError_t server_malloc(uc_server* S, void** p, size_t bytes, hipStream_t s) {
-    return S->async_alloc ? hipMallocAsync(p, bytes, s) : hipMalloc(p, bytes);
+    return S->async_alloc ? hipMallocFromPoolAsync(p, bytes, S->pool, s) : hipMalloc(p, bytes);
 }
 
 void server_release(uc_server* S, void* p, hipStream_t s) {
@@ -2160,15 +2164,39 @@ bool slot_reserve(uc_server* S, uc_server::Slot& sl, size_t n_entries) {
     return true;
 }
 
-// Picks the cells of the next round (under S->mu): every job with cells left
-// gets a fair share of the slot's lanes first, then the oldest jobs fill the
-// rest; an optional traj-step cap bounds the round's work.
-std::vector<uc_server::Entry> server_pick(uc_server* S, long long lanes, int block) {
+// Duration class of a job's cells: the decade of its per-cell step count.
+int server_class(const uc_server::Job& J) { return (int)std::floor(std::log10(std::max(1.0, J.step_cost))); }
+
+// Picks the cells of the next round (under S->mu) and returns its duration
+// class in *cls_out. A round holds cells of one duration class only, so it
+// lasts about as long as its cells do and a short job does not wait for
+// another job's long cells. The shortest class with cells left goes first
+// when no round of it is in flight; otherwise the classes take turns (the
+// next class after the last round's, cyclically). Classes of long cells
+// (>= kLongClass) are skipped when `allow_long` is false, which keeps a slot
+// free for short work. Inside the class every job gets a fair share of the
+// slot's lanes first, then the oldest jobs fill the rest; an optional
+// traj-step cap bounds the round's work.
+std::vector<uc_server::Entry> server_pick(uc_server* S, long long lanes, int block, bool allow_long, int* cls_out) {
     std::vector<uc_server::Entry> out;
     std::vector<uc_server::Job*> open;
+    int cls = -1, lowest = -1;
+    for (auto& jp : S->active) {
+        if (jp->state != 1 || jp->next >= jp->cells) continue;
+        const int c = server_class(*jp);
+        if (!allow_long && c >= uc_server::kLongClass) continue;
+        if (lowest < 0 || c < lowest) lowest = c;
+        if (c > S->last_class && (cls < 0 || c < cls)) cls = c;
+    }
+    if (lowest < 0) return out;
+    bool lowest_running = false;
+    for (auto& sl : S->slots)
+        if (sl.busy && sl.cls == lowest) lowest_running = true;
+    if (cls < 0 || !lowest_running) cls = lowest;
+    S->last_class = cls;
+    *cls_out = cls;
     for (auto& jp : S->active)
-        if (jp->state == 1 && jp->next < jp->cells) open.push_back(jp.get());
-    if (open.empty()) return out;
+        if (jp->state == 1 && jp->next < jp->cells && server_class(*jp) == cls) open.push_back(jp.get());
     const double cap_steps = S->cfg.round_traj_steps > 0 ? S->cfg.round_traj_steps : 0.0;
     std::map<uc_server::Job*, long long> take;
     std::map<std::pair<Module*, std::string>, int> groups;
@@ -2357,10 +2385,14 @@ void server_loop(uc_server* S) {
             if (!sl.busy && sl.stream) { free_slot = &sl; break; }
         bool launched = false;
         if (free_slot && !S->stop) {
-            auto entries = server_pick(S, lanes, block);
+            int long_busy = 0, cls = -1;
+            for (auto& sl : S->slots)
+                if (sl.busy && sl.cls >= uc_server::kLongClass) ++long_busy;
+            auto entries = server_pick(S, lanes, block, long_busy < uc_server::kSlots - 1, &cls);
             if (!entries.empty()) {
                 free_slot->entries = std::move(entries);
                 free_slot->busy = true;
+                free_slot->cls = cls;
                 if (S->n_busy++ == 0) S->busy_since = std::chrono::steady_clock::now();
                 const std::string why = server_launch(S, *free_slot, block, wps, cus);
                 if (!why.empty()) {
@@ -2428,6 +2460,23 @@ int uc_server_create(uc_engine* e, const uc_server_config* cfg, uc_server** out)
     int pools = 0;
     S->async_alloc = hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, e->device) == hipSuccess &&
                      pools != 0;
+    if (S->async_alloc) {
+        // A pool of its own whose release threshold is the budget: memory a
+        // finished job frees stays mapped for the next job instead of going
+        // back to the driver at every synchronisation (the default pool's
+        // threshold is 0, and re-mapping stalls admissions under load).
+        hipMemPoolProps props{};
+        props.allocType = hipMemAllocationTypePinned;
+        props.location.type = hipMemLocationTypeDevice;
+        props.location.id = e->device;
+        uint64_t keep = (uint64_t)S->budget;
+        if (hipMemPoolCreate(&S->pool, &props) != hipSuccess ||
+            hipMemPoolSetAttribute(S->pool, hipMemPoolAttrReleaseThreshold, &keep) != hipSuccess) {
+            if (S->pool) (void)hipMemPoolDestroy(S->pool);
+            S->pool = nullptr;
+            S->async_alloc = false;
+        }
+    }
     hipError_t he = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking);
     for (auto& sl : S->slots) {
         if (he == hipSuccess) he = hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking);
@@ -2437,6 +2486,7 @@ int uc_server_create(uc_engine* e, const uc_server_config* cfg, uc_server** out)
     if (he != hipSuccess) {
         server_destroy_slots(S);
         if (S->stream) (void)hipStreamDestroy(S->stream);
+        if (S->pool) (void)hipMemPoolDestroy(S->pool);
         delete S;
         return e->fail(UC_ERR_HIP, hip_msg(he, "server streams"));
     }
@@ -2466,6 +2516,7 @@ void uc_server_destroy(uc_server* S) {
         }
         server_destroy_slots(S);
         if (S->stream) { (void)hipStreamSynchronize(S->stream); (void)hipStreamDestroy(S->stream); }
+        if (S->pool) (void)hipMemPoolDestroy(S->pool);  // after every free above has completed
         delete S;
     }
     engine_release(e);
