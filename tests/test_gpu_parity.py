@@ -189,6 +189,39 @@ def test_lle_rossler_f32c(gpu_engine):
     assert err[repro].max() < 1e-3
 
 
+def test_lle_two_cells_per_lane(gpu_engine):
+    """A 511 x 511 map (odd cell count, 1.99 pair-lane waves per SIMD) takes
+    the two-cells-per-lane kernels in fast FP64 and FP32C. Over a short
+    horizon chaotic growth stays small, so both agree cell by cell with the
+    strict one-cell-per-lane FP64 kernel, including the cells that fail the
+    magnitude check (max_value 6, ~2.5% of the cells by the oracle on every
+    61st cell): same flags, state at the check."""
+    print("case: Rossler LLE 511x511 two cells per lane vs strict one cell per lane", flush=True)
+    s = ucuda.preset("rossler", gpu_engine)
+    n = 511
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, n, 0.05, 2.0), ucuda.axis(ucuda.AXIS_PARAM, 3, n, 1.0, 20.0)]
+    args = (s, axes, [1.0, 1.0, 1.0], s.values(a=0.2), 0.01, 2.0, 5.0, 0.1)
+    ref = ucuda.run_lyap(*args, strict=True, max_value=6.0, want_final_state=True)
+    rv, rf = ref["values"].cpu().numpy()[0], ref["flags"].cpu().numpy()
+    rs = ref["final_state"].cpu().numpy()
+    assert 100 < (rf == -1).sum() < rf.size // 2
+    for prec, tol, stol in ((ucuda.PREC_F64, 1e-9, 1e-9), (ucuda.PREC_F32C, 1e-3, 1e-3)):
+        g = ucuda.run_lyap(*args, precision=prec, max_value=6.0, want_final_state=True)
+        gv, gf = g["values"].cpu().numpy()[0], g["flags"].cpu().numpy()
+        gs = g["final_state"].cpu().numpy()
+        same = gf == rf
+        live = same & (rf == 1)
+        dead = same & (rf == -1)
+        err = np.abs(gv[live] - rv[live])
+        serr = np.abs(gs - rs).max(axis=0) / np.maximum(1.0, np.abs(rs).max(axis=0))
+        print("precision %d: flags equal on %.5f, max |LLE err| %.2e, max state err live %.2e dead %.2e"
+              % (prec, same.mean(), err.max(), serr[live].max(), serr[dead].max()))
+        assert same.mean() > 0.999
+        assert err.max() < tol
+        assert np.all(gv[dead] == 999.0)
+        assert serr[live].max() < stol and serr[dead].max() < stol
+
+
 def test_ls_lorenz_trace_and_oracle(gpu_engine):
     print("case: Lorenz LS 8 rho values, F64 strict vs oracle; sum = -(sigma+1+beta)", flush=True)
     s = ucuda.preset("lorenz", gpu_engine)

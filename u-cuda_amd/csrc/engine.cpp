@@ -46,6 +46,8 @@ constexpr int kMaxValues = 64;    // kMaxAmountOfValues, parametric_engine.cpp:3
 constexpr int kBlock = 256;
 constexpr size_t kModuleCacheSize = 8;
 
+int cu_count(int device);
+
 struct Module {
     std::string key;
     int pins = 0;  // live batches using the module: never evicted while pinned
@@ -317,11 +319,14 @@ double step_of(const uc_common& c) {
     return c.h;
 }
 
-int launch(uc_engine* e, Module* m, const char* name, UcJob& j, hipStream_t stream, TableBuffer* tb) {
+// Launches `name` over `lanes` lanes (default: one per cell).
+int launch(uc_engine* e, Module* m, const char* name, UcJob& j, hipStream_t stream, TableBuffer* tb,
+           long long lanes = -1) {
     hipFunction_t f;
     hipError_t he = hipModuleGetFunction(&f, m->mod, name);
     if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, name));
-    const long long blocks = (j.n_cells + kBlock - 1) / kBlock;
+    if (lanes < 0) lanes = j.n_cells;
+    const long long blocks = (lanes + kBlock - 1) / kBlock;
     if (blocks > 0x7fffffffLL) return e->fail(UC_ERR_ARGUMENT, "grid too large");
     void* args[] = {&j};
     he = hipModuleLaunchKernel(f, (unsigned)blocks, 1, 1, kBlock, 1, 1, 0, stream, args, nullptr);
@@ -578,6 +583,21 @@ static const char* lyap_kernel(bool spectrum, int precision) {
                     : (precision == UC_PREC_F64 ? "uc_lle_f64" : "uc_lle_f32c");
 }
 
+// Whether a LLE map runs two cells per lane (uc_lle_*_x2): fast mode, one
+// tangent vector by the variational method, wave-uniform counts (no sweep
+// over h), blocks of 1..100 steps, and a grid shape where it measured
+// faster. With pw = pair-lane waves per SIMD: FP32C only for 1.5 < pw <= 2
+// (-6% at 512^2; 4-9% slower at pw 2.75..4.9, even at 4), FP64 for
+// pw >= 0.95 (-3..-7% from 362^2 to 1024^2). Below one wave per SIMD a lone
+// wave cannot hide latency even with two cells (1.6x slower at 256^2).
+static bool pair_lle(uc_engine* e, bool spectrum, const uc_lyap_request& r, bool step_axis, long long cells) {
+    if (spectrum || r.c.strict || r.method != UC_METHOD_VARIATIONAL || step_axis) return false;
+    const long long nts = r.c.h > 0.0 ? (long long)(r.NT / r.c.h) : 0;
+    if (nts < 1 || nts > 100) return false;
+    const double pw = (double)((cells + 1) / 2) / ((double)cu_count(e->device) * 4.0 * 64.0);
+    return r.c.precision == UC_PREC_F64 ? pw >= 0.95 : (pw > 1.5 && pw <= 2.0);
+}
+
 static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream, bool spectrum) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> lk(e->mu);
@@ -591,6 +611,14 @@ static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const 
     hipStream_t st = (hipStream_t)stream;
     if ((rc = stage_tables(e, s, r->c, j, st, &tb))) return rc;
     fill_lyap(r, o, j);
+    bool step_axis = false;
+    for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
+    // Two cells per lane where the single-cell grid leaves few waves per SIMD
+    // (fast single-vector variational maps with blocks of <= 100 steps).
+    if (pair_lle(e, spectrum, *r, step_axis, j.n_cells)) {
+        const std::string name = std::string(lyap_kernel(false, r->c.precision)) + "_x2";
+        return launch(e, m, name.c_str(), j, st, tb, (j.n_cells + 1) / 2);
+    }
     return launch(e, m, lyap_kernel(spectrum, r->c.precision), j, st, tb);
 }
 

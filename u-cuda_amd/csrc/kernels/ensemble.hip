@@ -952,6 +952,163 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
     }
 }
 
+#if !UC_STRICT
+// Two cells per lane, cells `lane` and `lane + half`, for the fast
+// single-vector variational LLE with wave-uniform counts and blocks of at
+// most the check interval (the host picks this kernel only then). Each cell
+// runs exactly the steps, magnitude checks, rescales and switch of
+// uc_lyap_cell's fast path; the two cells' instruction streams are
+// independent, which doubles the work a wave has in flight where the grid
+// gives few waves per SIMD. A cell found diverged keeps stepping with the
+// other (its results are discarded) and gets its state at the check back.
+template <class S>
+__device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
+    typedef typename Prec<S>::T T;
+    long long cell[2] = {j.cell_offset + lane, j.cell_offset + lane + half};
+    const bool has1 = lane + half < j.n_cells;
+    if (!has1) cell[1] = cell[0];  // a stand-in copy of cell 0, never written
+    double ad[UC_M], x0[UC_N], hd = 0.0;
+    T a[2][UC_M];
+    S s[2], snap[2];
+    bool dead[2], ok[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        ok[c] = uc_setup(j, cell[c], ad, x0, hd);
+#pragma unroll
+        for (int i = 0; i < UC_M; ++i) a[c][i] = Prec<S>::to_t(ad[i]);
+        s[c].load(x0);
+        snap[c] = s[c];
+        dead[c] = !ok[c] || (c == 1 && !has1);
+    }
+    const T h = Prec<S>::to_t(j.h);
+    const double hdd = (double)h;
+    double t = 0.0;
+
+    // transient: uc_loop_u with pre = 1 and no observer, per cell
+    for (long long i = 0; i < j.skip_steps;) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) uc_rk4(s[c], a[c], h, (T)t);
+        if (UC_USES_TIME) t += hdd;
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            if (!dead[c] && uc_diverged(s[c], j.max_value)) { dead[c] = true; snap[c] = s[c]; }
+        if (uc_wave_all(dead[0] && dead[1])) break;
+        const long long stop = (j.skip_steps - i > UC_CHECK_INTERVAL) ? i + UC_CHECK_INTERVAL : j.skip_steps;
+#pragma unroll 2
+        for (++i; i < stop; ++i) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) uc_rk4(s[c], a[c], h, (T)t);
+            if (UC_USES_TIME) t += hdd;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) uc_rk4(s[c], a[c], h, (T)t);  // the trailing step
+    if (UC_USES_TIME) t += hdd;
+
+    T V[2][UC_N], Vs[2][UC_N], start2[2];
+    int ksum[2];
+    bool ref_mode[2];
+    double lsum[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        double z[UC_N];
+        uc_probe_basis<1>(j.probe_seed, cell[c], z);
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) { V[c][i] = Prec<S>::to_t(z[i]); Vs[c][i] = V[c][i]; }
+        start2[c] = (T)1.0;
+        ksum[c] = 0;
+        ref_mode[c] = false;
+        lsum[c] = 0.0;
+    }
+    const T big = sizeof(T) == 4 ? (T)__FLT_MAX__ : (T)1e300;
+    const int nts = (int)j.nt_steps;
+    bool all_dead = false;
+    for (long long b = 0; b < j.n_iters && !all_dead; ++b) {
+        // uc_block_tan_u for both cells
+#pragma unroll
+        for (int c = 0; c < 2; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
+        if (UC_USES_TIME) t += hdd;
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+            if (!dead[c] && uc_diverged(s[c], j.max_value)) { dead[c] = true; snap[c] = s[c]; }
+        if (uc_wave_all(dead[0] && dead[1])) { all_dead = true; break; }
+#pragma unroll 2
+        for (int i = 0; i < nts; ++i) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
+            if (UC_USES_TIME) t += hdd;
+        }
+        // the fast path's renormalisation, per live cell
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            if (dead[c]) continue;
+            T n2 = (T)0.0;
+#pragma unroll
+            for (int i = 0; i < UC_N; ++i) n2 = n2 + V[c][i] * V[c][i];
+            if (!ref_mode[c] && n2 > (T)1e-28 * start2[c] && n2 <= big) {
+                const int k = uc_frexp_exp(n2) >> 1;
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) {
+                    V[c][i] = uc_ldexp(V[c][i], -k);
+                    Vs[c][i] = V[c][i];
+                }
+                ksum[c] += k;
+                start2[c] = uc_ldexp(n2, -2 * k);
+                continue;
+            }
+            double inv0 = 1.0;
+            if (!ref_mode[c]) {
+                double s2 = 1.0;
+                if (b > 0) {
+                    s2 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) s2 = s2 + (double)Vs[c][i] * (double)Vs[c][i];
+                }
+                lsum[c] = (double)ksum[c] * UC_LN2 + 0.5 * log(s2);
+                inv0 = 1.0 / sqrt(s2);
+                ref_mode[c] = true;
+            }
+            double v[UC_N], nrm = 0.0;
+#pragma unroll
+            for (int i = 0; i < UC_N; ++i) { v[i] = (double)V[c][i] * inv0; nrm = nrm + v[i] * v[i]; }
+            nrm = sqrt(nrm);
+            if (nrm <= 1e-14) nrm = 1e-14;
+            lsum[c] = lsum[c] + log(nrm);
+            const double inv = 1.0 / nrm;
+#pragma unroll
+            for (int i = 0; i < UC_N; ++i) V[c][i] = Prec<S>::to_t(v[i] * inv);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        if (c == 1 && !has1) break;
+        const long long out = lane + (c == 1 ? half : 0);
+        if (!ok[c]) {
+            j.out0[out] = __builtin_nan("");
+            j.flag[out] = -1;
+            continue;
+        }
+        if (dead[c]) s[c] = snap[c];
+        double sum = lsum[c];
+        if (!ref_mode[c]) {
+            double s2 = 1.0;
+            if (j.n_iters > 0) {
+                s2 = 0.0;
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) s2 = s2 + (double)V[c][i] * (double)V[c][i];
+            }
+            sum = (double)ksum[c] * UC_LN2 + 0.5 * log(s2);
+        }
+        j.out0[out] = dead[c] ? 999.0 : sum / j.t_max;
+        j.flag[out] = dead[c] ? -1 : 1;
+        if (j.final_state) {
+#pragma unroll
+            for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.stride + out] = s[c].get(i);
+        }
+    }
+}
+#endif
+
 // The same streaming finder over stored series: data[i * n_series + s].
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_peaks_series(const double* data, long long n_samples,
                                                                       long long n_series, double time_step,
@@ -1780,6 +1937,17 @@ UC_PERSISTENT(uc_lle_f64_q, (uc_lyap_cell<1, StateF64>))
 UC_PERSISTENT(uc_lle_f32c_q, (uc_lyap_cell<1, StateF32C>))
 UC_PERSISTENT(uc_ls_f64_q, (uc_lyap_cell<UC_N, StateF64>))
 UC_PERSISTENT(uc_ls_f32c_q, (uc_lyap_cell<UC_N, StateF32C>))
+#if !UC_STRICT
+// Two cells per lane (uc_lle_pair): lane l runs cells l and l + ceil(n / 2).
+#define UC_DIRECT_PAIR(NAME, CELL)                                              \
+    extern "C" __global__ void __launch_bounds__(UC_BLOCK) NAME(UcJob j) {      \
+        const long long half = (j.n_cells + 1) / 2;                             \
+        const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x; \
+        if (lane < half) CELL(j, lane, half);                                   \
+    }
+UC_DIRECT_PAIR(uc_lle_f64_x2, uc_lle_pair<StateF64>)
+UC_DIRECT_PAIR(uc_lle_f32c_x2, uc_lle_pair<StateF32C>)
+#endif
 
 // ================================================================ basins ==
 // Adaptive Dormand-Prince 5(4) (Dormand & Prince 1980 coefficients; Hairer's
