@@ -630,9 +630,11 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
     Duffing basin maps (side x side initial conditions, rtol = atol = 1e-9),
     side log-uniform in [32, 2048], step count log-uniform in [1e3, 1e5]
     (basins: steps / 500 forcing periods, at least 16), FP32-compensated.
-    The arrival rate is set for ~70% of `capacity` (LLE traj-steps/s),
+    Arrivals at 50 jobs/s, the survey's serving rate. The offered
+    utilisation is estimated against `capacity` (LLE traj-steps/s),
     weighting a bifurcation step 0.5 and a basin section 410 LLE steps (the
-    measured FP32C rates of the three kernels).
+    measured FP32C rates of the three kernels); if it would reach 0.8 the
+    rate is lowered to 70% of capacity.
     Reports throughput and p50 / p99 job latency, the slowdown (latency over
     the job's estimated time alone at the measured rates) and p99 latency of
     the smaller and larger half of the jobs by size; every 100th job is run
@@ -655,7 +657,9 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
         return side * side * steps * weight[kind]
     est = float(np.mean([cost(*sp) for sp in specs]))
     if rate is None:
-        rate = 0.7 * capacity / est if capacity else 20.0
+        rate = 50.0
+        if capacity and rate * est / capacity >= 0.8:
+            rate = 0.7 * capacity / est
     srv = ucuda.Server(eng)
     ids = []
     t = 0.0
