@@ -114,3 +114,31 @@ def test_version_and_sources(cpu_engine):
     src = s.source(0)
     assert "uc_bif_f64" in src and "uc_bas_f32c" in src and "#define UC_USES_TIME 1" in src
     assert "cos(a[3] * t)" in s.source(1)
+
+
+def test_code_object_cache(tmp_path, monkeypatch):
+    """Compiled modules are kept on disk under UC_CACHE_DIR: one entry per
+    (program, options, hipRTC version) with its header; a fresh engine loads
+    it instead of compiling; a damaged entry is recompiled and rewritten;
+    UC_CACHE_DIR set but empty writes nothing."""
+    cache = tmp_path / "kcache"
+    monkeypatch.setenv("UC_CACHE_DIR", str(cache))
+
+    def compile_once():
+        eng = ucuda.Engine(device=-1)  # own in-memory module cache
+        return ucuda.preset("lorenz", eng).compile(strict=True)
+
+    n = compile_once()
+    files = sorted(cache.iterdir())
+    assert len(files) == 1 and files[0].suffix == ".hsaco"
+    raw = files[0].read_bytes()
+    assert raw.startswith(b"UCKC1 ") and raw[57:61] == b"\x7fELF" and len(raw) == 57 + n
+    assert int(raw[23:39], 16) == n
+    assert compile_once() == n and files[0].read_bytes() == raw  # loaded, not rewritten
+    for bad in (raw[: len(raw) // 2], raw[:-1] + bytes([raw[-1] ^ 1])):  # truncated, one bit flipped
+        files[0].write_bytes(bad)
+        assert compile_once() == n
+        assert files[0].read_bytes() == raw  # recompiled and rewritten
+    monkeypatch.setenv("UC_CACHE_DIR", "")
+    other = tmp_path / "unused"
+    assert compile_once() == n and not other.exists() and len(list(cache.iterdir())) == 1

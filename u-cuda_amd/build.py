@@ -49,7 +49,7 @@ def build_library(verbose=False):
            "-o", LIB + ".tmp",
            os.path.join(CSRC, "engine.cpp"), os.path.join(CSRC, "codegen.cpp"),
            os.path.join(CSRC, "parametric_engine.cpp"), os.path.join(CSRC, "data_export.cpp"),
-           "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-lamdhip64",
+           "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-lamdhip64", "-ldl",
            "-Wl,-rpath," + os.path.join(ROCM, "lib")]
     if verbose:
         print(" ".join(cmd))
@@ -65,14 +65,29 @@ def build_library(verbose=False):
 
 
 def check_kernels():
-    """hipRTC-compiles the kernel library for bundled systems (no GPU needed)."""
+    """hipRTC-compiles the kernel library for bundled systems (no GPU needed),
+    one engine per system on its own thread (the compiles release the GIL).
+    The code objects land in the engine's on-disk cache (kcache/ beside the
+    library), so processes that use these systems later load them instead of
+    compiling; ucuda binds torch's HIP runtime first, as a GPU process does,
+    so the entries carry the hipRTC version those processes will look for."""
+    from concurrent.futures import ThreadPoolExecutor
+    import shutil
     import ucuda  # noqa: E402  (sys.path holds this directory)
-    eng = ucuda.Engine(device=-1)
-    sizes = {}
-    for name in ("lorenz", "rossler", "hyper_rossler", "duffing"):
+
+    # every build compiles: drop the entries of earlier builds first
+    shutil.rmtree(os.path.join(HERE, "kcache"), ignore_errors=True)
+
+    def one(name):
+        eng = ucuda.Engine(device=-1)
         sysm = ucuda.preset(name, eng)
-        for strict in (False, True):
-            sizes[(name, strict)] = sysm.compile(strict=strict)
+        return {(name, strict): sysm.compile(strict=strict) for strict in (False, True)}
+
+    sizes = {}
+    names = ("lorenz", "rossler", "hyper_rossler", "duffing")
+    with ThreadPoolExecutor(max_workers=len(names)) as pool:
+        for part in pool.map(one, names):
+            sizes.update(part)
     return sizes
 
 

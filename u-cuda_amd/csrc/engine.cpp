@@ -16,6 +16,10 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <dlfcn.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -124,6 +128,78 @@ std::string program_source(const uc_system* s, bool strict) {
 
 // Compiles (or fetches from the cache) the module for a system. With
 // load=false only the code object is produced (works without a device).
+// ---- persistent code-object cache ----
+// hipRTC takes seconds per module (every analysis family of a system is in
+// one program), so compiled code objects are kept on disk, keyed by the
+// program text, the compile options and the hipRTC version. Directory:
+// $UC_CACHE_DIR (set but empty: no cache), else kcache/ beside this
+// library. A file is "UCKC1 <second hash of the key> <size> <hash of the
+// code>\n" + the code object; any mismatch (collision, truncation, damage)
+// means a recompile. Writes go
+// to a temporary file renamed into place, so concurrent processes never see
+// a partial entry; any I/O failure just means compiling.
+std::string kcache_dir() {
+    const char* env = getenv("UC_CACHE_DIR");
+    if (env) return std::string(env);
+    Dl_info info;
+    if (dladdr((void*)&kcache_dir, &info) && info.dli_fname) {
+        const std::string p(info.dli_fname);
+        const size_t slash = p.find_last_of('/');
+        return (slash == std::string::npos ? std::string(".") : p.substr(0, slash)) + "/kcache";
+    }
+    return std::string();
+}
+
+std::string hex64(uint64_t v) {
+    char b[17];
+    snprintf(b, sizeof b, "%016llx", (unsigned long long)v);
+    return std::string(b);
+}
+
+uint64_t fnv1a_bytes(const std::vector<char>& v) {
+    uint64_t h = 1469598103934665603ULL;
+    for (char c : v) { h ^= (unsigned char)c; h *= 1099511628211ULL; }
+    return h;
+}
+
+std::string kcache_header(const std::string& tag, const std::vector<char>& code) {
+    return tag + " " + hex64((uint64_t)code.size()) + " " + hex64(fnv1a_bytes(code)) + "\n";
+}
+
+// `tag` = "UCKC1 <second hash of the key>"; the rest of the header is
+// checked against the code read back.
+bool kcache_load(const std::string& path, const std::string& tag, std::vector<char>& code) {
+    FILE* f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::string hdr(tag.size() + 35, '\0');  // tag + " " + 16 hex + " " + 16 hex + "\n"
+    bool ok = fread(&hdr[0], 1, hdr.size(), f) == hdr.size() && hdr.compare(0, tag.size(), tag) == 0;
+    if (ok) {
+        char buf[1 << 16];
+        size_t n;
+        code.clear();
+        while ((n = fread(buf, 1, sizeof buf, f)) > 0) code.insert(code.end(), buf, buf + n);
+        ok = !ferror(f) && code.size() > 4 && memcmp(code.data(), "\x7f" "ELF", 4) == 0 &&
+             kcache_header(tag, code) == hdr;
+    }
+    fclose(f);
+    if (!ok) code.clear();
+    return ok;
+}
+
+void kcache_store(const std::string& dir, const std::string& path, const std::string& tag,
+                  const std::vector<char>& code) {
+    (void)mkdir(dir.c_str(), 0755);  // may exist
+    const std::string tmp = path + ".tmp." + std::to_string((long long)getpid()) + "." +
+                            std::to_string((unsigned long long)std::hash<std::thread::id>()(std::this_thread::get_id()));
+    FILE* f = fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    const std::string hdr = kcache_header(tag, code);
+    const bool ok = fwrite(hdr.data(), 1, hdr.size(), f) == hdr.size() &&
+                    fwrite(code.data(), 1, code.size(), f) == code.size();
+    if (fclose(f) == 0 && ok && rename(tmp.c_str(), path.c_str()) == 0) return;
+    (void)unlink(tmp.c_str());
+}
+
 int get_module(uc_engine* e, uc_system* s, bool strict, bool load, Module** out) {
     const std::string src = program_source(s, strict);
     const std::string key = std::to_string(fnv1a(src)) + (strict ? ":s" : ":f");
@@ -139,27 +215,40 @@ int get_module(uc_engine* e, uc_system* s, bool strict, bool load, Module** out)
             return UC_OK;
         }
     }
-    hiprtcProgram prog;
-    if (hiprtcCreateProgram(&prog, src.c_str(), "ucuda_ensemble.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
-        return e->fail(UC_ERR_COMPILE, "hiprtcCreateProgram failed");
     std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
                                      strict ? "-ffp-contract=off" : "-ffp-contract=fast",
                                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-    hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
-    if (rc != HIPRTC_SUCCESS) {
-        size_t n = 0;
-        hiprtcGetProgramLogSize(prog, &n);
-        std::string log(n, '\0');
-        if (n) hiprtcGetProgramLog(prog, &log[0]);
-        hiprtcDestroyProgram(&prog);
-        return e->fail(UC_ERR_COMPILE, "hipRTC compile failed:\n" + log);
-    }
+    int vmaj = 0, vmin = 0;
+    (void)hiprtcVersion(&vmaj, &vmin);
+    std::string ident = src + "\n";
+    for (const char* o : opts) ident += std::string(o) + " ";
+    ident += "\nhiprtc " + std::to_string(vmaj) + "." + std::to_string(vmin);
+    const std::string cdir = kcache_dir();
+    const std::string cpath = cdir.empty() ? std::string() : cdir + "/" + hex64(fnv1a(ident)) + ".hsaco";
+    const std::string ctag = "UCKC1 " + hex64(fnv1a(ident, 0x84222325cbf29ce4ULL));
     Module m;
     m.key = key;
-    hiprtcGetCodeSize(prog, &m.code_size);
-    m.code.resize(m.code_size);
-    hiprtcGetCode(prog, m.code.data());
-    hiprtcDestroyProgram(&prog);
+    if (!cpath.empty() && kcache_load(cpath, ctag, m.code)) {
+        m.code_size = m.code.size();
+    } else {
+        hiprtcProgram prog;
+        if (hiprtcCreateProgram(&prog, src.c_str(), "ucuda_ensemble.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+            return e->fail(UC_ERR_COMPILE, "hiprtcCreateProgram failed");
+        hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
+        if (rc != HIPRTC_SUCCESS) {
+            size_t n = 0;
+            hiprtcGetProgramLogSize(prog, &n);
+            std::string log(n, '\0');
+            if (n) hiprtcGetProgramLog(prog, &log[0]);
+            hiprtcDestroyProgram(&prog);
+            return e->fail(UC_ERR_COMPILE, "hipRTC compile failed:\n" + log);
+        }
+        hiprtcGetCodeSize(prog, &m.code_size);
+        m.code.resize(m.code_size);
+        hiprtcGetCode(prog, m.code.data());
+        hiprtcDestroyProgram(&prog);
+        if (!cpath.empty()) kcache_store(cdir, cpath, ctag, m.code);
+    }
     if (load) {
         hipError_t he = hipModuleLoadData(&m.mod, m.code.data());
         if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLoadData"));
