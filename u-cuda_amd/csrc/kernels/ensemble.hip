@@ -2130,7 +2130,16 @@ __device__ void uc_basins_cell(const UcJob& j, long long lane) {
             const bool last = h >= rem;
             const double hs = last ? rem : h;
             const double err = uc_bas_attempt(s, a, hs, k1, rtol, atol);
+#if UC_STRICT
             double fac = err == 0.0 ? 5.0 : 0.9 * pow(err, -0.2);
+#else
+            // The factor needs a few digits only: err^(-1/5) from the FP32
+            // hardware log2 / exp2 (v_log_f32, v_exp_f32) instead of the FP64
+            // library pow, which costs about as much as the stages' own
+            // arithmetic. Out-of-range err (0, inf) is clamped below as before.
+            double fac = err == 0.0 ? 5.0
+                                    : 0.9 * (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)err));
+#endif
             fac = fac < 0.2 ? 0.2 : (fac > 5.0 ? 5.0 : fac);
             if (err <= 1.0) {
                 ++acc;
