@@ -1991,9 +1991,18 @@ UC_DIRECT_PAIR(uc_lle_f32c_x2, uc_lle_pair<StateF32C>)
 #define DPE6 (22.0 / 525.0)
 #define DPE7 (-1.0 / 40.0)
 
+// Hardware reciprocals for the fast modes' error norm (v_rcp_f32 / v_rcp_f64).
+__device__ __forceinline__ float uc_rcp(float v) { return __builtin_amdgcn_rcpf(v); }
+__device__ __forceinline__ double uc_rcp(double v) { return __builtin_amdgcn_rcp(v); }
+
 // One attempt from (t, x) with step h, k1 = f(t, x) on entry. Produces the
 // 5th-order increment dx (x5 = x + dx), k7 = f(t + h, x5) and the error norm.
 // The stage sums follow the tableau row order left to right, like the oracle.
+// Strict modules return the RMS norm as the oracle computes it. Fast ones
+// return its square, with the scaled errors formed in the stage precision by
+// a hardware reciprocal: the norm only decides acceptance (err^2 <= 1 exactly
+// when err <= 1) and sizes the next step, so it needs no correctly rounded
+// division or square root.
 template <typename T>
 __device__ __forceinline__ double uc_dp5(const T* x, const T* a, T t, T h, const T* k1, T* dx, T* k7, double rtol,
                                          double atol) {
@@ -2028,11 +2037,20 @@ __device__ __forceinline__ double uc_dp5(const T* x, const T* a, T t, T h, const
     for (int i = 0; i < UC_N; ++i) {
         const T e = h * ((T)DPE1 * k1[i] + (T)0.0 * k2[i] + (T)DPE3 * k3[i] + (T)DPE4 * k4[i] + (T)DPE5 * k5[i] +
                          (T)DPE6 * k6[i] + (T)DPE7 * k7[i]);
+#if UC_STRICT
         const double sc = atol + rtol * fmax(fabs((double)x[i]), fabs((double)xs[i]));
         const double q = (double)e / sc;
+#else
+        const T sc = (T)atol + (T)rtol * fmax(fabs(x[i]), fabs(xs[i]));
+        const double q = (double)(e * uc_rcp(sc));
+#endif
         err = err + q * q;
     }
+#if UC_STRICT
     return sqrt(err / (double)UC_N);
+#else
+    return err * (1.0 / (double)UC_N);
+#endif
 }
 
 struct BasF64 {
@@ -2063,8 +2081,9 @@ struct BasF32C {
     __device__ __forceinline__ void advance_time(double hs) { uc_kahan_add(thi, tc, (float)hs); }
 };
 
-// One accepted-or-rejected attempt for each state policy. Returns the error
-// norm; on acceptance (err <= 1) the state moves and k1 <- k7 (FSAL).
+// One accepted-or-rejected attempt for each state policy. Returns uc_dp5's
+// norm (its square in fast modules); on acceptance (<= 1) the state moves
+// and k1 <- k7 (FSAL).
 __device__ __forceinline__ double uc_bas_attempt(BasF64& s, const double* a, double hs, double* k1, double rtol,
                                                  double atol) {
     double dx[UC_N], k7[UC_N];
@@ -2133,12 +2152,13 @@ __device__ void uc_basins_cell(const UcJob& j, long long lane) {
 #if UC_STRICT
             double fac = err == 0.0 ? 5.0 : 0.9 * pow(err, -0.2);
 #else
-            // The factor needs a few digits only: err^(-1/5) from the FP32
-            // hardware log2 / exp2 (v_log_f32, v_exp_f32) instead of the FP64
-            // library pow, which costs about as much as the stages' own
-            // arithmetic. Out-of-range err (0, inf) is clamped below as before.
+            // err is the squared norm here. The factor needs a few digits
+            // only: (err^2)^(-1/10) from the FP32 hardware log2 / exp2
+            // (v_log_f32, v_exp_f32) instead of the FP64 library pow, which
+            // costs about as much as the stages' own arithmetic. Out-of-range
+            // err (0, inf) is clamped below as before.
             double fac = err == 0.0 ? 5.0
-                                    : 0.9 * (double)__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf((float)err));
+                                    : 0.9 * (double)__builtin_amdgcn_exp2f(-0.1f * __builtin_amdgcn_logf((float)err));
 #endif
             fac = fac < 0.2 ? 0.2 : (fac > 5.0 ? 5.0 : fac);
             if (err <= 1.0) {
