@@ -671,8 +671,15 @@ __device__ __forceinline__ void uc_mgs(double* v, double* d) {
         for (int i = 0; i < UC_N; ++i) nrm = nrm + v[k * UC_N + i] * v[k * UC_N + i];
         nrm = sqrt(nrm);
         d[k] = nrm;
+#if UC_STRICT
 #pragma unroll
         for (int i = 0; i < UC_N; ++i) v[k * UC_N + i] = (nrm == 0.0) ? 0.0 : v[k * UC_N + i] / nrm;
+#else
+        // one division per vector (the quotients differ by at most an ulp)
+        const double inv = (nrm == 0.0) ? 0.0 : 1.0 / nrm;
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) v[k * UC_N + i] = v[k * UC_N + i] * inv;
+#endif
     }
 }
 
