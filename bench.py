@@ -20,7 +20,7 @@ import os
 import sys
 import time
 
-# The serving measurement keeps four rounds in flight on four HIP streams.
+# The serving measurement keeps six rounds in flight on six HIP streams.
 # HIP maps a process's streams onto at most GPU_MAX_HW_QUEUES hardware queues
 # (default 4, some taken by other streams), and rounds sharing a queue run
 # one after another; eight queues give each round stream its own. Must be

@@ -359,18 +359,18 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
 
 /* ---- serving: continuous batching ----
  * A scheduler thread admits jobs in arrival order while their outputs fit
- * the HBM budget and keeps up to four rounds in flight, one per stream:
+ * the HBM budget and keeps up to six rounds in flight, one per stream:
  * each round takes cells of one duration class (the decade of the job's
  * per-cell step count) from the admitted jobs (a fair share per job, then
  * oldest first) up to the GPU's resident lanes and launches the persistent
  * kernels once per (system, analysis) group; a new round is queued whenever
  * a slot frees, and jobs submitted meanwhile join it. The shortest class
  * goes first when none of its rounds is in flight, the others take turns,
- * and classes of >= 1e4 steps per cell hold at most three of the four
+ * and classes of >= 1e4 steps per cell hold at most three of the six
  * slots. Results are bit-identical to a direct uc_run_* of the same
  * request. Outputs stay on the device until uc_server_fetch copies them
  * out. Run the process with GPU_MAX_HW_QUEUES=8 (before the HIP runtime
- * starts): with HIP's default of 4 the four round streams share two
+ * starts): with HIP's default of 4 the six round streams share two
  * hardware queues and rounds on one queue run in turn. */
 typedef struct uc_server uc_server;
 typedef struct uc_server_config {

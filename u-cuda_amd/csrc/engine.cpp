@@ -2126,8 +2126,9 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
 // until fetched. Device buffers use the stream-ordered allocator where the
 // device has one (hipFree synchronises the whole device).
 struct uc_server {
-    static constexpr int kSlots = 4;
-    static constexpr int kLongClass = 4;   // cells of >= 1e4 steps: at most kSlots - 1 slots
+    static constexpr int kSlots = 6;
+    static constexpr int kLongClass = 4;   // cells of >= 1e4 steps: at most kLongSlots slots
+    static constexpr int kLongSlots = 3;
     static constexpr int kMaxGroups = 64;  // counters per slot
     uc_engine* e = nullptr;
     uc_server_config cfg{};
@@ -2505,7 +2506,7 @@ void server_loop(uc_server* S) {
             int long_busy = 0, cls = -1;
             for (auto& sl : S->slots)
                 if (sl.busy && sl.cls >= uc_server::kLongClass) ++long_busy;
-            auto entries = server_pick(S, lanes, block, long_busy < uc_server::kSlots - 1, &cls);
+            auto entries = server_pick(S, lanes, block, long_busy < uc_server::kLongSlots, &cls);
             if (!entries.empty()) {
                 free_slot->entries = std::move(entries);
                 free_slot->busy = true;

@@ -800,7 +800,7 @@ def _close_servers():
 class Server:
     """Continuous-batching server (uc_server). submit_* return a job id;
     fetch() returns numpy results once the job is done. The server keeps
-    four rounds in flight on four streams; run the process with
+    six rounds in flight on six streams; run the process with
     GPU_MAX_HW_QUEUES=8 (set before the HIP runtime starts) so that each
     stream has a hardware queue of its own: with HIP's default of 4 the
     round streams share two queues and rounds on one queue run in turn."""
