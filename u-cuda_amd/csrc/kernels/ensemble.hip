@@ -1039,7 +1039,7 @@ __device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
         for (int c = 0; c < 2; ++c)
             if (!dead[c] && uc_diverged(s[c], j.max_value)) { dead[c] = true; snap[c] = s[c]; }
         if (uc_wave_all(dead[0] && dead[1])) { all_dead = true; break; }
-#pragma unroll 2
+#pragma unroll 4
         for (int i = 0; i < nts; ++i) {
 #pragma unroll
             for (int c = 0; c < 2; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
