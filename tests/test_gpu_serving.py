@@ -113,3 +113,32 @@ def test_admission_budget_and_cancel(gpu_engine):
     b = srv.fetch(ids[1])["values"]
     np.testing.assert_array_equal(a, b)
     srv.close()
+
+
+def test_short_jobs_do_not_wait_for_long_cells(gpu_engine):
+    """Rounds hold one duration class (decade of steps per cell), so short
+    jobs submitted right after a job of long cells never share its round:
+    with the GPU not full (the long job is 4,096 cells), they finish long
+    before it. (A long job that fills every wave slot still holds them until
+    its cells finish: rounds are not preemptible.)"""
+    print("case: one 64x64 LLE job of 110,001 steps per cell, then five 16x16 jobs of 111", flush=True)
+    ros = ucuda.preset("rossler", gpu_engine)
+    vals = ros.values()
+    big = [ucuda.axis(1, 2, 64, 0.05, 2.0), ucuda.axis(1, 3, 64, 1.0, 20.0)]
+    small = [ucuda.axis(1, 2, 16, 0.05, 2.0), ucuda.axis(1, 3, 16, 1.0, 20.0)]
+    # warm module and allocator outside the timed part
+    ucuda.run_lyap(ros, small, [1, 1, 1], vals, 0.01, 0.0, 1.0, 0.1, precision=ucuda.PREC_F32C)
+    srv = ucuda.Server(gpu_engine)
+    try:
+        long_id = srv.submit_lyap(ros, big, [1, 1, 1], vals, 0.01, 0.0, 1000.0, 0.1, precision=ucuda.PREC_F32C)
+        short_ids = [srv.submit_lyap(ros, small, [1, 1, 1], vals, 0.01, 0.0, 1.0, 0.1, precision=ucuda.PREC_F32C)
+                     for _ in range(5)]
+        for jid in short_ids + [long_id]:
+            srv.wait(jid, timeout_s=120.0)
+        lat_long = srv.poll(long_id)[1]
+        lat_short = [srv.poll(j)[1] for j in short_ids]
+        print("long job %.1f ms; short jobs %s ms" % (lat_long, " ".join("%.1f" % v for v in lat_short)))
+        assert all(srv.poll(j)[0] == 2 for j in short_ids + [long_id])
+        assert max(lat_short) < 0.5 * lat_long
+    finally:
+        srv.close()
