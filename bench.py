@@ -781,7 +781,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this engine's path runs on a single MI355X "
-              "(cells shard by contiguous ranges across GPUs; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+              "(across GPUs the cells would shard by contiguous ranges, one process per GPU). Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
