@@ -220,6 +220,20 @@ def test_lle_two_cells_per_lane(gpu_engine):
         assert err.max() < tol
         assert np.all(gv[dead] == 999.0)
         assert serr[live].max() < stol and serr[dead].max() < stol
+    # a right-hand side with a time symbol (forced Duffing): the two cells of
+    # a lane share the system time
+    print("case: forced Duffing LLE 511x511 over (x0, v0), two cells per lane vs strict", flush=True)
+    d = ucuda.preset("duffing", gpu_engine)
+    daxes = [ucuda.axis(ucuda.AXIS_VAR, 0, n, -2.5, 2.5), ucuda.axis(ucuda.AXIS_VAR, 1, n, -2.5, 2.5)]
+    dargs = (d, daxes, [0.0, 0.0], d.values(), 0.01, 2.0, 10.0, 0.1)
+    ref = ucuda.run_lyap(*dargs, strict=True)
+    rv, rf = ref["values"].cpu().numpy()[0], ref["flags"].cpu().numpy()
+    assert (rf == 1).all()
+    for prec, tol in ((ucuda.PREC_F64, 1e-9), (ucuda.PREC_F32C, 1e-3)):
+        g = ucuda.run_lyap(*dargs, precision=prec)
+        gv, gf = g["values"].cpu().numpy()[0], g["flags"].cpu().numpy()
+        print("precision %d: max |LLE err| %.2e" % (prec, np.abs(gv - rv).max()))
+        assert (gf == rf).all() and np.abs(gv - rv).max() < tol
 
 
 def test_ls_lorenz_trace_and_oracle(gpu_engine):
