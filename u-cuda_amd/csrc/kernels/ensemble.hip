@@ -535,7 +535,18 @@ struct PeakObserver {
     __device__ __forceinline__ void emit(double dm, double d0, double dp, long long j) {
         const double denom = dm - 2.0 * d0 + dp;
         double delta = 0.0;
+#if UC_STRICT
         if (fabs(denom) > 1e-12) delta = 0.5 * (dm - dp) / denom;
+#else
+        // A wave runs this whenever any of its lanes confirms a peak (most
+        // steps): the reciprocal and one Newton step replace the division
+        // (the quotient moves by an ulp or two).
+        if (fabs(denom) > 1e-12) {
+            double r = __builtin_amdgcn_rcp(denom);
+            r = fma(r, fma(-denom, r, 1.0), r);
+            delta = 0.5 * (dm - dp) * r;
+        }
+#endif
         const double value = d0 - 0.25 * (dm - dp) * delta;
         const double tpk = (double)(j - 1) + delta;
         if (raw >= 1 && raw - 1 < max_peaks) {
