@@ -97,6 +97,13 @@ struct StateF64 {
         for (int i = 0; i < UC_N; ++i) x[i] = x0[i];
     }
     __device__ __forceinline__ double get(int i) const { return x[i]; }
+    // component q (0 <= q < N) selected without indexing the array
+    __device__ __forceinline__ double get_sel(int q) const {
+        double v = x[0];
+#pragma unroll
+        for (int i = 1; i < UC_N; ++i) v = (i == q) ? x[i] : v;
+        return v;
+    }
 };
 
 // F32C state: hi + Kahan compensation c (true value ~ hi - c).
@@ -110,6 +117,16 @@ struct StateF32C {
         }
     }
     __device__ __forceinline__ double get(int i) const { return (double)hi[i] - (double)c[i]; }
+    // the pair is selected in float and converted once (same value as get(q))
+    __device__ __forceinline__ double get_sel(int q) const {
+        float h = hi[0], cc = c[0];
+#pragma unroll
+        for (int i = 1; i < UC_N; ++i) {
+            h = (i == q) ? hi[i] : h;
+            cc = (i == q) ? c[i] : cc;
+        }
+        return (double)h - (double)cc;
+    }
 };
 
 #define UC_LN2 0.6931471805599453
@@ -342,9 +359,7 @@ __device__ int uc_loop(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h
                 else if (UC_N == 2) v = s.get(0) + UC_PI * s.get(UC_N >= 2 ? 1 : 0);
                 else v = s.get(0);
             } else {
-                v = s.get(0);
-#pragma unroll
-                for (int q = 1; q < UC_N; ++q) v = (q == wvar) ? s.get(q) : v;
+                v = s.get_sel(wvar);
             }
             obs.sample(i, v);
             for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += hd; }
@@ -408,9 +423,7 @@ __device__ __forceinline__ double uc_sample_value(const S& s, int wvar) {
         else if (UC_N == 2) v = s.get(0) + UC_PI * s.get(UC_N >= 2 ? 1 : 0);
         else v = s.get(0);
     } else {
-        v = s.get(0);
-#pragma unroll
-        for (int q = 1; q < UC_N; ++q) v = (q == wvar) ? s.get(q) : v;
+        v = s.get_sel(wvar);
     }
     return v;
 }
