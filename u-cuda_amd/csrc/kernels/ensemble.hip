@@ -572,19 +572,20 @@ struct PeakObserver {
 
     __device__ __forceinline__ void sample(long long k, double v) {
         if (k < traj_len) traj[k * stride] = v;
-        if (k >= 2 && mode != 2) {
+        if (k >= 2) {
+            // The search as selects, one branch (emit) left: a candidate
+            // (mode 0 -> 1) can be confirmed or dropped in the same sample,
+            // mode 2 (record end reached inside a plateau) is final.
             const long long c = k - 1;
-            if (mode == 0 && c >= c_next && c < n - 2 &&
-                w1 - w0 > UC_EPS_PEAK_DELTA && w1 > UC_PEAK_THRESHOLD && w1 >= v)
-                mode = 1;
-            if (mode == 1) {
-                if (c < n - 2) {
-                    if (w1 < v) { mode = 0; c_next = c + 2; }
-                    else if (w1 - v > UC_EPS_PEAK_DELTA) { emit(w0, w1, v, c); mode = 0; c_next = c + 2; }
-                } else {
-                    mode = 2;
-                }
-            }
+            const bool inwin = c < n - 2;
+            const bool start = mode == 0 && c >= c_next && inwin && w1 - w0 > UC_EPS_PEAK_DELTA &&
+                               w1 > UC_PEAK_THRESHOLD && w1 >= v;
+            const bool plateau = mode == 1 || start;
+            const bool drop = plateau && inwin && w1 < v;
+            const bool confirm = plateau && inwin && !(w1 < v) && w1 - v > UC_EPS_PEAK_DELTA;
+            if (confirm) emit(w0, w1, v, c);
+            mode = (plateau && !inwin) ? 2 : (drop || confirm) ? 0 : start ? 1 : mode;
+            c_next = (drop || confirm) ? c + 2 : c_next;
         }
         w0 = w1;
         w1 = v;
