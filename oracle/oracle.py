@@ -577,3 +577,34 @@ def basins_cells(funcs, bj, cells, threads=1):
     L.oc_basins_cells(funcs.rhs, C.byref(bj), cp, n, period.ctypes.data_as(_I), point.ctypes.data_as(_D),
                       steps.ctypes.data_as(_LL), threads)
     return dict(period=period, point=point, steps=steps)
+
+
+def dp5_step(rhs_ptr, x, a, t, h, rtol=0.0, atol=1.0):
+    """One Dormand-Prince 5(4) attempt (the basins integrator's step):
+    returns (x5, e, norm) with e the raw embedded error estimate."""
+    L = lib()
+    L.oc_dp5_step.argtypes = [_P, _D, _D, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, _D, _D]
+    L.oc_dp5_step.restype = C.c_double
+    n = len(x)
+    xa = (C.c_double * n)(*x)
+    aa = (C.c_double * max(1, len(a)))(*a)
+    x5 = (C.c_double * n)()
+    e = (C.c_double * n)()
+    nrm = L.oc_dp5_step(rhs_ptr, xa, aa, float(t), float(h), n, float(rtol), float(atol), x5, e)
+    return np.array(list(x5)), np.array(list(e)), nrm
+
+
+def section_signature(samples, max_period=8, tol=1e-4):
+    """Period and canonical point of trailing Poincare samples (row 0 the
+    newest, columns = leading state variables, at most 4)."""
+    s = np.asarray(samples, dtype=np.float64)
+    L_, D_ = s.shape
+    buf = np.zeros((L_, 4))
+    buf[:, :D_] = s
+    buf = np.ascontiguousarray(buf)
+    pt = (C.c_double * 4)()
+    L = lib()
+    L.oc_section_signature.argtypes = [_D, C.c_int, C.c_int, C.c_int, C.c_double, _D]
+    L.oc_section_signature.restype = C.c_int
+    p = L.oc_section_signature(buf.ctypes.data_as(_D), L_, D_, int(max_period), float(tol), pt)
+    return p, np.array(list(pt)[:D_])

@@ -1207,9 +1207,10 @@ typedef struct oc_basins_job {
 } oc_basins_job;
 
 /* one DP5(4) attempt: x (n) at local time t with step h; k1 = f(t, x) on
- * entry (FSAL); writes x5 and k7 = f(t + h, x5); returns the error norm. */
+ * entry (FSAL); writes x5 and k7 = f(t + h, x5); returns the error norm
+ * sqrt(mean_i (e_i / (atol + rtol * max(|x_i|, |x5_i|)))^2), e = h * sum_s E_s k_s. */
 static double dp5_attempt(oc_rhs_fn rhs, const double* x, const double* a, double t, double h, int n,
-                          const double* k1, double* x5, double* k7, double rtol, double atol) {
+                          const double* k1, double* x5, double* k7, double rtol, double atol, double* e_out) {
     double k[7][OC_MAXN], xs[OC_MAXN];
     for (int i = 0; i < n; ++i) k[0][i] = k1[i];
     for (int s = 1; s < 7; ++s) {
@@ -1227,10 +1228,53 @@ static double dp5_attempt(oc_rhs_fn rhs, const double* x, const double* a, doubl
         double e = 0.0;
         for (int s = 0; s < 7; ++s) e = e + DP_E[s] * k[s][i];
         e = h * e;
+        if (e_out) e_out[i] = e;
         const double sc = atol + rtol * fmax(fabs(x[i]), fabs(x5[i]));
         err = err + (e / sc) * (e / sc);
     }
     return sqrt(err / (double)n);
+}
+
+/* Poincare signature of the trailing section samples buf[L][4] (buf[0] the
+ * newest, D <= 4 leading state variables): the smallest period
+ * q <= max_period, q < L, for which every sample r with r + q < L closes
+ * |buf[r] - buf[r + q]| <= tol * (1 + |buf[r]|) in every component; the
+ * cycle point among buf[0 .. q-1] with the smallest (x0, then x1) is the
+ * canonical point. No closing period: 0 and the mean of the L samples. */
+int oc_section_signature(const double* buf, int L, int D, int max_period, double tol, double* point) {
+    int p = 0;
+    for (int q = 1; q <= max_period && q < L && !p; ++q) {
+        int closes = 1;
+        for (int r = 0; r + q < L && closes; ++r)
+            for (int i = 0; i < D; ++i)
+                if (fabs(buf[r * 4 + i] - buf[(r + q) * 4 + i]) > tol * (1.0 + fabs(buf[r * 4 + i]))) { closes = 0; break; }
+        if (closes) p = q;
+    }
+    if (p > 0) {
+        int best = 0;
+        for (int r = 1; r < p; ++r) {
+            if (buf[r * 4] < buf[best * 4] || (buf[r * 4] == buf[best * 4] && D > 1 && buf[r * 4 + 1] < buf[best * 4 + 1]))
+                best = r;
+        }
+        for (int i = 0; i < D; ++i) point[i] = buf[best * 4 + i];
+    } else {
+        for (int i = 0; i < D; ++i) {
+            double m = 0.0;
+            for (int r = 0; r < L; ++r) m += buf[r * 4 + i];
+            point[i] = m / (double)L;
+        }
+    }
+    return p;
+}
+
+/* One DP5(4) attempt from outside (tests): k1 = f(t, x) is evaluated here;
+ * x5 gets the 5th-order solution, err_vec (optional) the raw embedded
+ * error estimate e_i = h * sum_s E_s k_s; returns the controller's norm. */
+double oc_dp5_step(oc_rhs_fn rhs, const double* x, const double* a, double t, double h, int n, double rtol,
+                   double atol, double* x5, double* err_vec) {
+    double k1[OC_MAXN], k7[OC_MAXN];
+    rhs(x, a, t, k1);
+    return dp5_attempt(rhs, x, a, t, h, n, k1, x5, k7, rtol, atol, err_vec);
 }
 
 static int basins_cell(oc_rhs_fn rhs, const oc_basins_job* bj, long long cell, double* point, int* period,
@@ -1252,7 +1296,7 @@ static int basins_cell(oc_rhs_fn rhs, const oc_basins_job* bj, long long cell, d
             ++attempts;
             const int last = t + h >= T;
             const double hs = last ? T - t : h;
-            const double err = dp5_attempt(rhs, x, a, t, hs, n, k1, x5, k7, bj->rtol, bj->atol);
+            const double err = dp5_attempt(rhs, x, a, t, hs, n, k1, x5, k7, bj->rtol, bj->atol, NULL);
             double fac = err == 0.0 ? 5.0 : 0.9 * pow(err, -0.2);
             fac = fac < 0.2 ? 0.2 : (fac > 5.0 ? 5.0 : fac);
             if (err <= 1.0) {
@@ -1285,28 +1329,7 @@ static int basins_cell(oc_rhs_fn rhs, const oc_basins_job* bj, long long cell, d
     }
     steps[0] = acc_steps;
     steps[1] = rej_steps;
-    int p = 0;
-    for (int q = 1; q <= bj->max_period && q < L && !p; ++q) {
-        int closes = 1;
-        for (int r = 0; r + q < L && closes; ++r)
-            for (int i = 0; i < D; ++i)
-                if (fabs(buf[r][i] - buf[r + q][i]) > bj->period_tol * (1.0 + fabs(buf[r][i]))) { closes = 0; break; }
-        if (closes) p = q;
-    }
-    if (p > 0) {
-        int best = 0;
-        for (int r = 1; r < p; ++r) {
-            if (buf[r][0] < buf[best][0] || (buf[r][0] == buf[best][0] && D > 1 && buf[r][1] < buf[best][1])) best = r;
-        }
-        for (int i = 0; i < D; ++i) point[i] = buf[best][i];
-    } else {
-        for (int i = 0; i < D; ++i) {
-            double m = 0.0;
-            for (int r = 0; r < L; ++r) m += buf[r][i];
-            point[i] = m / (double)L;
-        }
-    }
-    *period = p;
+    *period = oc_section_signature(&buf[0][0], L, D, bj->max_period, bj->period_tol, point);
     return 1;
 }
 

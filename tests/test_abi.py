@@ -142,3 +142,44 @@ def test_code_object_cache(tmp_path, monkeypatch):
     monkeypatch.setenv("UC_CACHE_DIR", "")
     other = tmp_path / "unused"
     assert compile_once() == n and not other.exists() and len(list(cache.iterdir())) == 1
+
+
+def test_engine_usable_while_it_compiles(monkeypatch):
+    """A hipRTC compile (seconds) runs outside the engine lock: while one
+    thread compiles a new module, other calls on the same engine return at
+    once, and a compile error on one thread leaves another thread's last
+    error untouched."""
+    import threading
+    import time
+    monkeypatch.setenv("UC_CACHE_DIR", "")  # no disk cache: a real compile
+    eng = ucuda.Engine(device=-1)
+    s = ucuda.System(eng, ["p", "q"], ["k"], ["q", "-k*p - 0.01*q^3"])
+    done = {}
+
+    def compile_it():
+        t0 = time.perf_counter()
+        s.compile(strict=True)
+        done["s"] = time.perf_counter() - t0
+
+    th = threading.Thread(target=compile_it)
+    th.start()
+    time.sleep(0.3)
+    waits = []
+    while th.is_alive() and len(waits) < 5:
+        t0 = time.perf_counter()
+        ucuda.System(eng, ["x"], ["a"], ["-a*x"])     # takes the engine lock
+        waits.append(time.perf_counter() - t0)
+        time.sleep(0.2)
+    th.join()
+    assert done["s"] > 1.0, done                     # the compile really ran
+    assert waits and max(waits) < 0.5, (waits, done)
+    # per-thread error copies
+    with pytest.raises(ucuda.UcError):
+        ucuda.System(eng, ["x"], [], ["x +* 2"])
+    msg = eng.error()
+    assert msg
+    other = {}
+    t = threading.Thread(target=lambda: other.setdefault("m", eng.error()))
+    t.start()
+    t.join()
+    assert other["m"] == msg

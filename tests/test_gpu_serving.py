@@ -98,20 +98,31 @@ def test_results_identical_alone_and_in_the_mix(gpu_engine):
 
 
 def test_admission_budget_and_cancel(gpu_engine):
-    print("case: tiny HBM budget admits one job at a time; cancel a queued job", flush=True)
+    """Outputs held on the device count against the budget until fetched:
+    with room for two 120 kB jobs, the third waits while the first two are
+    done but unfetched, and is admitted once one is fetched."""
+    print("case: 300 kB HBM budget, three 120 kB jobs, fetch frees room; cancel a queued job", flush=True)
+    import time
     ros = ucuda.preset("rossler", gpu_engine)
     srv = ucuda.Server(gpu_engine, max_bytes=300000, round_traj_steps=1e7)
     axes = [ucuda.axis(1, 2, 100, 0.05, 2.0), ucuda.axis(1, 3, 100, 1.0, 20.0)]  # 10k cells = 120 kB each
     ids = [srv.submit_lyap(ros, axes, [1, 1, 1], ros.values(), 0.01, 5.0, 20.0, 0.1) for _ in range(4)]
     srv.cancel(ids[3])
-    for jid in ids[:3]:
+    for jid in ids[:2]:
         srv.wait(jid, 120.0)
+    time.sleep(0.2)
+    assert srv.poll(ids[2])[0] == 0                 # still queued: no room while 0 and 1 hold results
+    assert srv.stats()["used_bytes"] <= 300000
+    a = srv.fetch(ids[0])["values"]                 # frees 120 kB
+    srv.wait(ids[2], 120.0)
+    assert srv.stats()["used_bytes"] <= 300000
+    b = srv.fetch(ids[1])["values"]
+    c = srv.fetch(ids[2])["values"]
     assert srv.poll(ids[3])[0] == 3
     st = srv.stats()
-    assert st["completed"] == 3 and st["used_bytes"] <= 300000 * 3
-    a = srv.fetch(ids[0])["values"]
-    b = srv.fetch(ids[1])["values"]
+    assert st["completed"] == 3 and st["used_bytes"] == 0
     np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(a, c)
     srv.close()
 
 

@@ -546,3 +546,135 @@ def test_continuation_chains(cpu_engine):
             assert m >= int(w * T / (2 * np.pi)) - 1
             np.testing.assert_allclose(r3["peaks"][:m, jj], amp, rtol=1e-6)
             st = np.linalg.matrix_power(P, (blk + 1) * pre) @ st
+
+
+# --------------------------------------------------------------- basins ----
+# The basins integrator is adaptive Dormand-Prince 5(4) (Dormand & Prince
+# 1980) with steps clipped to land on every Poincare section, and a period /
+# canonical-point search over the trailing section samples. Pins: the
+# method's stability polynomials on y' = lam*y (closed forms of the published
+# tableau), order 5 / order 4 of the solution / embedded estimate on a
+# non-autonomous nonlinear problem with a known solution, global error
+# proportional to the tolerance, the period search on synthetic section
+# sequences, and the unforced double well (point attractors (+-1, 0), exact
+# point symmetry).
+def _funcs(eng, vars, params, rhs, time_symbol=None):
+    return oracle.SystemFuncs(ucuda.System(eng, vars, params, rhs, time_symbol=time_symbol).source(1))
+
+
+def test_dp5_step_is_the_stability_polynomial(cpu_engine):
+    f = _funcs(cpu_engine, ["y"], ["lam"], ["lam*y"])
+    for lam, h in ((-1.0, 0.1), (0.7, 0.5), (-3.0, 1.0), (2.0, 0.25), (-0.5, 2.0)):
+        z = lam * h
+        R5 = 1 + z + z**2 / 2 + z**3 / 6 + z**4 / 24 + z**5 / 120 + z**6 / 600
+        # embedded-pair difference: R5 - R4 of the published weights
+        E = -97.0 / 120000 * z**5 + 13.0 / 40000 * z**6 - 1.0 / 24000 * z**7
+        x5, e, _ = oracle.dp5_step(f.rhs, [1.5], [0.5, lam], 0.0, h)
+        assert x5[0] == pytest.approx(1.5 * R5, rel=1e-14, abs=1e-15)
+        assert e[0] == pytest.approx(1.5 * E, rel=1e-12, abs=1e-17)
+
+
+def test_dp5_orders_on_a_nonautonomous_problem(cpu_engine):
+    # u' = 3 cos(3t) u (1 - u), u = 1 / (1 + 2 exp(-sin 3t)): over a fixed
+    # interval the 5th-order solution converges as h^5 and x5 - e (the
+    # embedded 4th-order solution) as h^4; one step's estimate e scales as
+    # h^5. A slipped A, b, c or E entry drops one of these orders.
+    f = _funcs(cpu_engine, ["u", "w"], [], ["3*cos(3*t)*u*(1-u)", "u*w"], time_symbol="t")
+    t0, span = 0.3, 0.8
+
+    def exact(t):
+        return 1.0 / (1.0 + 2.0 * math.exp(-math.sin(3 * t)))
+
+    y0 = [exact(t0), 1.0]
+    e5, e4, est = [], [], []
+    for n in (8, 16, 32, 64):
+        h = span / n
+        y, y4 = list(y0), list(y0)
+        for k in range(n):
+            x5, _, _ = oracle.dp5_step(f.rhs, y, [0.5], t0 + k * h, h)
+            y = list(x5)
+            x5b, eb, _ = oracle.dp5_step(f.rhs, y4, [0.5], t0 + k * h, h)
+            y4 = list(x5b - eb)
+        e5.append(abs(y[0] - exact(t0 + span)))
+        e4.append(abs(y4[0] - exact(t0 + span)))
+        est.append(abs(oracle.dp5_step(f.rhs, y0, [0.5], t0, h)[1][0]))
+    for k in range(3):
+        assert 28 < e5[k] / e5[k + 1] < 42, e5     # 2^5
+        assert 14 < e4[k] / e4[k + 1] < 19, e4     # 2^4
+        assert 28 < est[k] / est[k + 1] < 44, est  # 2^5 (h = 0.1 is not yet asymptotic)
+
+
+def _section_job(n_vars, ic, values, rtol, n_sections, T, **kw):
+    axes = [dict(kind=oracle.AXIS_PARAM, index=1, n_pts=1, lo=values[1], hi=values[1], log_scale=0)]
+    return oracle.make_basins_job(n_vars, axes, ic, values, 0.01, rtol, rtol, T, n_sections, **kw)
+
+
+def test_basins_global_error_proportional_to_tolerance(cpu_engine):
+    # x' = w v, v' = -w x: every section t = k 2pi/w returns to the start, so
+    # the canonical point after 40 sections is the global error at t = 80 pi/w
+    f = _funcs(cpu_engine, ["x", "v"], ["w"], ["w*v", "-w*x"])
+    errs = []
+    for tol in (1e-6, 1e-8, 1e-10):
+        bj = _section_job(2, [1.2, -0.4], [0.5, 1.3], tol, 40, 2 * math.pi / 1.3)
+        r = oracle.basins_cells(f, bj, [0])
+        assert r["period"][0] == 1
+        errs.append(float(np.hypot(r["point"][0, 0] - 1.2, r["point"][1, 0] + 0.4)))
+        assert errs[-1] < 2e3 * tol, (tol, errs[-1])
+    for k in range(2):
+        ratio = errs[k] / errs[k + 1]
+        assert 30 < ratio < 300, errs   # 100x tolerance -> ~100x error
+
+
+def test_section_signature_on_synthetic_sequences():
+    rng = np.random.default_rng(7)
+    for p in range(1, 9):
+        cyc = rng.uniform(-2, 2, size=(p, 2))
+        for phase in range(p):
+            seq = np.array([cyc[(r + phase) % p] for r in range(16)])
+            seq += rng.uniform(-1e-6, 1e-6, size=seq.shape)          # far below the 1e-4 closure
+            got, pt = oracle.section_signature(seq, max_period=8, tol=1e-4)
+            assert got == p, (p, got)
+            best = min(range(p), key=lambda r: (seq[r, 0], seq[r, 1]))
+            assert np.array_equal(pt, seq[best])                     # lexicographic minimum of the cycle
+    # ties in x0 are broken by x1
+    seq = np.array([[0.5, 2.0], [0.5, -1.0], [3.0, 0.0]] * 6)[:16]
+    assert oracle.section_signature(seq)[0] == 3
+    assert np.array_equal(oracle.section_signature(seq)[1], [0.5, -1.0])
+    # period 2 also closes at 4 and 6: the smallest is reported
+    seq = np.array([[1.0, 0.0], [-1.0, 0.0]] * 8)
+    assert oracle.section_signature(seq)[0] == 2
+    # period 9 is beyond max_period 8; no closure -> 0 and the mean
+    cyc = rng.uniform(-2, 2, size=(9, 2))
+    seq = np.array([cyc[r % 9] for r in range(16)])
+    p, pt = oracle.section_signature(seq, max_period=8)
+    assert p == 0 and np.allclose(pt, seq.mean(axis=0), rtol=0, atol=1e-15)
+    # closure is relative: |d| <= tol (1 + |x|); one sample 3 tol off breaks period 1
+    seq = np.ones((16, 2)) * 10.0
+    seq[7, 1] += 3e-4 * 11.0
+    assert oracle.section_signature(seq, tol=1e-4)[0] == 0
+    seq[7, 1] = 10.0 + 0.9e-4 * 11.0
+    assert oracle.section_signature(seq, tol=1e-4)[0] == 1
+
+
+def test_basins_unforced_double_well(cpu_engine):
+    # x'' + 0.15 x' - x + x^3 = 0: point attractors (+-1, 0); the flow is odd,
+    # so mirrored initial states give exactly mirrored sections and steps
+    f = _funcs(cpu_engine, ["x", "v"], ["delta", "gamma", "omega"],
+               ["v", "-delta*v+x-x^3+gamma*cos(omega*t)"], time_symbol="t")
+    rng = np.random.default_rng(11)
+    ics = rng.uniform(-2.5, 2.5, size=(24, 2))
+    for x0, v0 in ics:
+        res = []
+        for sgn in (1.0, -1.0):
+            bj = _section_job(2, [sgn * x0, sgn * v0], [0.5, 0.15, 0.0, 1.0], 1e-9, 60, 2 * math.pi)
+            res.append(oracle.basins_cells(f, bj, [0]))
+        a, b = res
+        assert a["period"][0] == 1 and b["period"][0] == 1
+        assert abs(abs(a["point"][0, 0]) - 1.0) < 1e-6 and abs(a["point"][1, 0]) < 1e-6
+        assert np.array_equal(b["point"][:, 0], -a["point"][:, 0])
+        assert np.array_equal(a["steps"], b["steps"])
+    # inside a well near its bottom the cell stays in that well
+    for x0 in (0.8, 1.2, -0.9):
+        bj = _section_job(2, [x0, 0.05], [0.5, 0.15, 0.0, 1.0], 1e-9, 60, 2 * math.pi)
+        r = oracle.basins_cells(f, bj, [0])
+        assert r["point"][0, 0] == pytest.approx(math.copysign(1.0, x0), abs=1e-6)

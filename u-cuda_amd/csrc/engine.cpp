@@ -92,11 +92,20 @@ struct uc_engine {
     // uc_server_destroy drops the last reference, in any order.
     std::atomic<int> refs{1};
     std::recursive_mutex mu;
+    // `error` has a lock of its own, taken by nothing else, so it can be set
+    // from any thread whatever other locks that thread holds
+    mutable std::mutex err_mu;
     std::string error;
+    // serialises hipRTC compiles, which run without `mu` (they take seconds)
+    std::mutex compile_mu;
     std::list<Module> modules;      // front = most recently used
     std::vector<TableBuffer> tables;
 
-    int fail(int code, const std::string& msg) { error = msg; return code; }
+    int fail(int code, const std::string& msg) {
+        std::lock_guard<std::mutex> lk(err_mu);
+        error = msg;
+        return code;
+    }
     ~uc_engine() {
         for (auto& m : modules)
             if (m.mod) (void)hipModuleUnload(m.mod);
@@ -126,8 +135,6 @@ std::string program_source(const uc_system* s, bool strict) {
     return src;
 }
 
-// Compiles (or fetches from the cache) the module for a system. With
-// load=false only the code object is produced (works without a device).
 // ---- persistent code-object cache ----
 // hipRTC takes seconds per module (every analysis family of a system is in
 // one program), so compiled code objects are kept on disk, keyed by the
@@ -200,21 +207,13 @@ void kcache_store(const std::string& dir, const std::string& path, const std::st
     (void)unlink(tmp.c_str());
 }
 
-int get_module(uc_engine* e, uc_system* s, bool strict, bool load, Module** out) {
-    const std::string src = program_source(s, strict);
-    const std::string key = std::to_string(fnv1a(src)) + (strict ? ":s" : ":f");
-    for (auto it = e->modules.begin(); it != e->modules.end(); ++it) {
-        if (it->key == key) {
-            e->modules.splice(e->modules.begin(), e->modules, it);
-            Module& m = e->modules.front();
-            if (load && !m.mod) {
-                hipError_t he = hipModuleLoadData(&m.mod, m.code.data());
-                if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLoadData"));
-            }
-            *out = &m;
-            return UC_OK;
-        }
-    }
+std::string module_key(const std::string& src, bool strict) {
+    return std::to_string(fnv1a(src)) + (strict ? ":s" : ":f");
+}
+
+// The code object of a program: from the disk cache, else hipRTC for gfx950.
+// Touches no engine state, so it runs without the engine lock.
+int compile_code(const std::string& src, bool strict, std::vector<char>& code, std::string& err) {
     std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
                                      strict ? "-ffp-contract=off" : "-ffp-contract=fast",
                                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
@@ -226,35 +225,49 @@ int get_module(uc_engine* e, uc_system* s, bool strict, bool load, Module** out)
     const std::string cdir = kcache_dir();
     const std::string cpath = cdir.empty() ? std::string() : cdir + "/" + hex64(fnv1a(ident)) + ".hsaco";
     const std::string ctag = "UCKC1 " + hex64(fnv1a(ident, 0x84222325cbf29ce4ULL));
+    if (!cpath.empty() && kcache_load(cpath, ctag, code)) return UC_OK;
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "ucuda_ensemble.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+        err = "hiprtcCreateProgram failed";
+        return UC_ERR_COMPILE;
+    }
+    hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
+    if (rc != HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetProgramLogSize(prog, &n);
+        std::string log(n, '\0');
+        if (n) hiprtcGetProgramLog(prog, &log[0]);
+        hiprtcDestroyProgram(&prog);
+        err = "hipRTC compile failed:\n" + log;
+        return UC_ERR_COMPILE;
+    }
+    size_t size = 0;
+    hiprtcGetCodeSize(prog, &size);
+    code.resize(size);
+    hiprtcGetCode(prog, code.data());
+    hiprtcDestroyProgram(&prog);
+    if (!cpath.empty()) kcache_store(cdir, cpath, ctag, code);
+    return UC_OK;
+}
+
+Module* find_module(uc_engine* e, const std::string& key) {
+    for (auto it = e->modules.begin(); it != e->modules.end(); ++it) {
+        if (it->key == key) {
+            e->modules.splice(e->modules.begin(), e->modules, it);
+            return &e->modules.front();
+        }
+    }
+    return nullptr;
+}
+
+// Inserts a compiled module at the front (under e->mu), with LRU eviction
+// past the capacity, skipping modules pinned by live batches and servers.
+Module* insert_module(uc_engine* e, const std::string& key, std::vector<char>&& code) {
     Module m;
     m.key = key;
-    if (!cpath.empty() && kcache_load(cpath, ctag, m.code)) {
-        m.code_size = m.code.size();
-    } else {
-        hiprtcProgram prog;
-        if (hiprtcCreateProgram(&prog, src.c_str(), "ucuda_ensemble.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
-            return e->fail(UC_ERR_COMPILE, "hiprtcCreateProgram failed");
-        hiprtcResult rc = hiprtcCompileProgram(prog, (int)opts.size(), opts.data());
-        if (rc != HIPRTC_SUCCESS) {
-            size_t n = 0;
-            hiprtcGetProgramLogSize(prog, &n);
-            std::string log(n, '\0');
-            if (n) hiprtcGetProgramLog(prog, &log[0]);
-            hiprtcDestroyProgram(&prog);
-            return e->fail(UC_ERR_COMPILE, "hipRTC compile failed:\n" + log);
-        }
-        hiprtcGetCodeSize(prog, &m.code_size);
-        m.code.resize(m.code_size);
-        hiprtcGetCode(prog, m.code.data());
-        hiprtcDestroyProgram(&prog);
-        if (!cpath.empty()) kcache_store(cdir, cpath, ctag, m.code);
-    }
-    if (load) {
-        hipError_t he = hipModuleLoadData(&m.mod, m.code.data());
-        if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLoadData"));
-    }
+    m.code = std::move(code);
+    m.code_size = m.code.size();
     e->modules.push_front(std::move(m));
-    // LRU eviction past the capacity, skipping modules pinned by live batches
     size_t over = e->modules.size() > kModuleCacheSize ? e->modules.size() - kModuleCacheSize : 0;
     for (auto it = std::prev(e->modules.end()); over > 0 && it != e->modules.begin();) {
         auto cur = it--;
@@ -263,7 +276,52 @@ int get_module(uc_engine* e, uc_system* s, bool strict, bool load, Module** out)
         e->modules.erase(cur);
         --over;
     }
-    *out = &e->modules.front();
+    return &e->modules.front();
+}
+
+// Compiles the module of (s, strict) into the engine's cache WITHOUT holding
+// e->mu, so a seconds-long hipRTC compile does not block the engine's other
+// callers; entry points call it before taking the lock and then find the
+// module in get_module. Failures are left for get_module to report.
+void precompile(uc_engine* e, uc_system* s, bool strict) {
+    if (!e || !s) return;
+    const std::string src = program_source(s, strict);
+    const std::string key = module_key(src, strict);
+    {
+        std::lock_guard<std::recursive_mutex> lk(e->mu);
+        if (find_module(e, key)) return;
+    }
+    std::lock_guard<std::mutex> ck(e->compile_mu);
+    {
+        std::lock_guard<std::recursive_mutex> lk(e->mu);
+        if (find_module(e, key)) return;  // compiled by another caller meanwhile
+    }
+    std::vector<char> code;
+    std::string err;
+    if (compile_code(src, strict, code, err) != UC_OK) return;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    if (!find_module(e, key)) insert_module(e, key, std::move(code));
+}
+
+// The module of (s, strict) from the cache (compiled here, under the lock,
+// if precompile did not run or it was evicted since). With load=false only
+// the code object is produced (works without a device). Caller holds e->mu.
+int get_module(uc_engine* e, uc_system* s, bool strict, bool load, Module** out) {
+    const std::string src = program_source(s, strict);
+    const std::string key = module_key(src, strict);
+    Module* m = find_module(e, key);
+    if (!m) {
+        std::vector<char> code;
+        std::string err;
+        const int rc = compile_code(src, strict, code, err);
+        if (rc != UC_OK) return e->fail(rc, err);
+        m = insert_module(e, key, std::move(code));
+    }
+    if (load && !m->mod) {
+        hipError_t he = hipModuleLoadData(&m->mod, m->code.data());
+        if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLoadData"));
+    }
+    *out = m;
     return UC_OK;
 }
 
@@ -482,7 +540,15 @@ void engine_release(uc_engine* e) {
 
 void uc_engine_destroy(uc_engine* e) { engine_release(e); }
 
-const char* uc_last_error(const uc_engine* e) { return e ? e->error.c_str() : "null engine"; }
+// A copy per calling thread: the engine's message may be rewritten by another
+// thread at any time, the copy only by this thread's next call.
+const char* uc_last_error(const uc_engine* e) {
+    static thread_local std::string copy;
+    if (!e) return "null engine";
+    std::lock_guard<std::mutex> lk(e->err_mu);
+    copy = e->error;
+    return copy.c_str();
+}
 
 int uc_system_create(uc_engine* e, const uc_system_desc* desc, uc_system** out) {
     if (!e || !desc || !out) return UC_ERR_ARGUMENT;
@@ -545,6 +611,7 @@ int uc_system_op_counts(const uc_system* s, int* rhs_flops, int* rhs_calls, int*
 
 int uc_system_compile(uc_engine* e, uc_system* s, int strict, size_t* code_size) {
     if (!e || !s) return UC_ERR_ARGUMENT;
+    precompile(e, s, strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     Module* m = nullptr;
     int rc = get_module(e, s, strict != 0, false, &m);
@@ -559,7 +626,7 @@ int uc_codegen_scheme(uc_engine* e, const uc_system_desc* desc, int scheme, char
         ucuda::System s = to_system(desc);
         return copy_out(ucuda::codegen_scheme(s, (ucuda::Scheme)scheme), buf, cap, len);
     } catch (const std::exception& ex) {
-        if (e) e->error = ex.what();
+        if (e) e->fail(UC_ERR_PARSE, ex.what());
         return UC_ERR_PARSE;
     }
 }
@@ -619,6 +686,7 @@ static void fill_bif(const uc_bif_request* r, const uc_bif_output* o, long long 
 
 int uc_run_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, void* stream) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     long long iters = 0;
     int rc = prep_bif(e, s, r, o, &iters);
@@ -689,6 +757,7 @@ static bool pair_lle(uc_engine* e, bool spectrum, const uc_lyap_request& r, bool
 
 static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream, bool spectrum) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     int rc = prep_lyap(e, s, r, o);
     if (rc) return rc;
@@ -884,6 +953,7 @@ extern "C" {
 
 int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     uc_common c;
     int lbl = 0;
@@ -956,6 +1026,7 @@ static int launch_dbscan(uc_engine* e, Module* m, const double* peaks, const int
 // quarter of the free HBM.
 int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, double eps, int* periods, void* stream) {
     if (!e || !s || !r) return UC_ERR_ARGUMENT;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     if (!periods) return e->fail(UC_ERR_ARGUMENT, "periods output missing");
     if (!(eps > 0.0)) return e->fail(UC_ERR_ARGUMENT, "eps_dbscan must be > 0");
@@ -1263,6 +1334,7 @@ static int run_dft_hsweep(uc_engine* e, uc_system* s, const uc_dft_request* r, c
 // path above.
 int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft_output* o, void* stream) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     int rc = validate_common(e, s, r->c, true);
     if (rc) return rc;
@@ -1446,6 +1518,7 @@ static void cont_done(ContRun& R, hipStream_t st) {
 int uc_run_bif_cont(uc_engine* e, uc_system* s, const uc_bif_request* r, int reverse, const uc_bif_output* o,
                     void* stream) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     if (!o->flags) return e->fail(UC_ERR_ARGUMENT, "flags output missing");
     if (r->max_peaks < 0) return e->fail(UC_ERR_ARGUMENT, "max_peaks must be >= 0");
@@ -1475,6 +1548,7 @@ int uc_run_bif_cont(uc_engine* e, uc_system* s, const uc_bif_request* r, int rev
 static int run_lyap_cont(uc_engine* e, uc_system* s, const uc_lyap_request* r, int reverse, const uc_lyap_output* o,
                          void* stream, bool spectrum) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     if (!o->values || !o->flags) return e->fail(UC_ERR_ARGUMENT, "outputs missing");
     if (!(r->NT > 0.0)) return e->fail(UC_ERR_ARGUMENT, "NT must be > 0");
@@ -1507,6 +1581,7 @@ int uc_run_ls_cont(uc_engine* e, uc_system* s, const uc_lyap_request* r, int rev
 int uc_run_dft_cont(uc_engine* e, uc_system* s, const uc_dft_request* r, int reverse, const uc_dft_output* o,
                     void* stream) {
     if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     if (!o->ak_cos || !o->bk_sin || !o->flags) return e->fail(UC_ERR_ARGUMENT, "DFT outputs missing");
     if (r->writable_var < -1 || r->writable_var >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "writable_var out of range");
@@ -1573,6 +1648,7 @@ int uc_run_dft_cont(uc_engine* e, uc_system* s, const uc_dft_request* r, int rev
 int uc_peak_finder(uc_engine* e, uc_system* s, const double* data, long long n_samples, long long n_series,
                    double time_step, int max_peaks, int* counts, double* peaks, double* intervals, void* stream) {
     if (!e || !s) return UC_ERR_ARGUMENT;
+    precompile(e, s, true);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     if (!data || !counts || n_samples < 0 || n_series <= 0 || max_peaks < 0 || max_peaks > 2500)
         return e->fail(UC_ERR_ARGUMENT, "peak finder: bad arguments");
@@ -1627,6 +1703,7 @@ long long uc_fastsync_points(const uc_fastsync_request* r) {
 int uc_run_fastsync(uc_engine* e, uc_system* s, const uc_fastsync_request* r, const uc_fastsync_output* o,
                     void* stream) {
     if (!e || !s) return UC_ERR_ARGUMENT;
+    if (r) precompile(e, s, r->strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     std::string why;
     int rc = fs_validate(r, why);
@@ -1775,6 +1852,7 @@ int uc_run_fastsync(uc_engine* e, uc_system* s, const uc_fastsync_request* r, co
 int uc_bif_dbscan_periods(uc_engine* e, uc_system* s, const double* peaks, const int* counts, long long n_cells,
                           int max_peaks, double eps, int* periods, void* stream) {
     if (!e || !s) return UC_ERR_ARGUMENT;
+    precompile(e, s, true);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     if (!counts || !periods || n_cells <= 0 || max_peaks < 0 || (max_peaks > 0 && !peaks))
         return e->fail(UC_ERR_ARGUMENT, "dbscan periods: bad arguments");
@@ -1984,6 +2062,7 @@ void uc_batch_destroy(uc_batch* b) {
 int uc_batch_add_bif(uc_batch* b, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, int* job_id) {
     if (!b || !s || !r || !o) return UC_ERR_ARGUMENT;
     uc_engine* e = b->e;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     long long iters = 0;
     int rc = prep_bif(e, s, r, o, &iters);
@@ -2006,6 +2085,7 @@ int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int s
                       int* job_id) {
     if (!b || !s || !r || !o) return UC_ERR_ARGUMENT;
     uc_engine* e = b->e;
+    precompile(e, s, r->c.strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     int rc = prep_lyap(e, s, r, o);
     if (rc) return rc;
@@ -2490,7 +2570,10 @@ void server_loop(uc_server* S) {
         while (!S->stop && !S->pending.empty()) {
             auto j = S->pending.front();
             if (j->state == 3) { S->pending.pop_front(); continue; }
-            if (S->used_bytes + j->bytes > S->budget && !S->active.empty()) break;
+            // over the budget a job waits, unless no admitted job still holds
+            // memory (done but unfetched jobs count): a job larger than the
+            // whole budget then runs alone
+            if (S->used_bytes + j->bytes > S->budget && S->used_bytes > 0) break;
             S->pending.pop_front();
             if (!server_admit(S, *j)) { j->state = 4; j->t_done = std::chrono::steady_clock::now(); continue; }
             S->used_bytes += j->bytes;
@@ -2643,6 +2726,7 @@ void uc_server_destroy(uc_server* S) {
 int uc_server_submit_lyap(uc_server* S, uc_system* s, const uc_lyap_request* r, int spectrum, long long* id) {
     if (!S || !s || !r) return UC_ERR_ARGUMENT;
     uc_engine* e = S->e;
+    precompile(e, s, r->c.strict != 0);
     auto j = std::make_shared<uc_server::Job>();
     {
         std::lock_guard<std::recursive_mutex> lk(e->mu);
@@ -2672,6 +2756,7 @@ int uc_server_submit_lyap(uc_server* S, uc_system* s, const uc_lyap_request* r, 
 int uc_server_submit_bif(uc_server* S, uc_system* s, const uc_bif_request* r, long long* id) {
     if (!S || !s || !r) return UC_ERR_ARGUMENT;
     uc_engine* e = S->e;
+    precompile(e, s, r->c.strict != 0);
     auto j = std::make_shared<uc_server::Job>();
     long long iters = 0;
     {
@@ -2702,6 +2787,7 @@ int uc_server_submit_bif(uc_server* S, uc_system* s, const uc_bif_request* r, lo
 int uc_server_submit_basins(uc_server* S, uc_system* s, const uc_basins_request* r, long long* id) {
     if (!S || !s || !r) return UC_ERR_ARGUMENT;
     uc_engine* e = S->e;
+    precompile(e, s, r->c.strict != 0);
     auto j = std::make_shared<uc_server::Job>();
     uc_common c;
     {
