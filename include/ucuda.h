@@ -185,7 +185,8 @@ int uc_system_n_values(const uc_system* s);
 /* Copies generated source (NUL-terminated) into buf; *len gets the full size.
  * dialect 0: the complete hipRTC program; 1: the system functions as C. */
 int uc_system_source(const uc_system* s, int dialect, char* buf, size_t cap, size_t* len);
-/* Compiles the system's module with hipRTC for gfx950 (no device needed).
+/* Compiles the system's module with hipRTC for gfx950 (no device needed);
+ * on an engine with a device the module is also loaded onto it.
  * *code_size receives the code-object size in bytes. */
 int uc_system_compile(uc_engine* e, uc_system* s, int strict, size_t* code_size);
 /* Algorithmic flop counts of one right-hand side / Jacobian evaluation. */
@@ -264,6 +265,17 @@ int uc_run_dft_cont(uc_engine* e, uc_system* s, const uc_dft_request* r, int rev
 int uc_run_lle(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const uc_basins_output* o, void* stream);
+/* uc_run_lle without allocating. No counterpart in the reference: it is for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The launch tables go into a
+ * static 512 KiB device array per GPU instead of a buffer from the
+ * allocator, so the grid is limited to nx + ny <= 65,440 axis points (else
+ * UC_ERR_ARGUMENT). The system's module must already be loaded on this
+ * engine (uc_system_compile with the same strict flag; else
+ * UC_ERR_ARGUMENT). Same kernels as uc_run_lle, same results. Every call on
+ * one GPU shares that GPU's array, which keeps the last call's tables: calls
+ * that overlap on one GPU race. */
+int uc_run_lle_noalloc(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream);
 
 /* The streaming peak finder over stored series (the reference's
  * peakFinderCUDA over d_data, U-CUDA/cudaLibrary.cu:1641-1674): device

@@ -38,8 +38,22 @@ def write_embed():
             f.write(out)
 
 
+def build_scratch_object(verbose=False):
+    """scratch.hip holds device code (the static scratch array of
+    uc_run_lle_noalloc), so it is compiled for gfx950 on its own."""
+    obj = os.path.join(HERE, "scratch.o")
+    cmd = [os.path.join(ROCM, "bin", "hipcc"), "-x", "hip", "--offload-arch=gfx950", "-O2", "-std=c++17", "-fPIC",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
+           "-c", os.path.join(CSRC, "scratch.hip"), "-o", obj]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    return obj
+
+
 def build_library(verbose=False):
     write_embed()
+    scratch_obj = build_scratch_object(verbose)
     # host sin / cos stay separate calls (a fused sincos rounds differently
     # from cos on some arguments; the DFT tables must match the C library's
     # sin and cos)
@@ -48,7 +62,7 @@ def build_library(verbose=False):
            "-fno-builtin-sin", "-fno-builtin-cos", "-fno-builtin-sinf", "-fno-builtin-cosf",
            "-o", LIB + ".tmp",
            os.path.join(CSRC, "engine.cpp"), os.path.join(CSRC, "codegen.cpp"),
-           os.path.join(CSRC, "parametric_engine.cpp"), os.path.join(CSRC, "data_export.cpp"),
+           os.path.join(CSRC, "parametric_engine.cpp"), os.path.join(CSRC, "data_export.cpp"), "-x", "none", scratch_obj,
            "-L" + os.path.join(ROCM, "lib"), "-lhiprtc", "-lamdhip64", "-ldl",
            "-Wl,-rpath," + os.path.join(ROCM, "lib")]
     if verbose:

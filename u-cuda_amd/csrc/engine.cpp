@@ -39,6 +39,7 @@
 #include <vector>
 
 #include "codegen.hpp"
+#include "scratch.hpp"
 #include "uc_job.h"
 
 namespace {
@@ -100,6 +101,12 @@ struct uc_engine {
     std::mutex compile_mu;
     std::list<Module> modules;      // front = most recently used
     std::vector<TableBuffer> tables;
+    // uc_run_lle_noalloc: pinned host image of the tables (UC_SCRATCH_DOUBLES),
+    // reused once `noalloc_done` (recorded after its copy) has completed;
+    // both made when the engine is created, with the device's scratch address
+    double* noalloc_host = nullptr;
+    hipEvent_t noalloc_done = nullptr;
+    std::string noalloc_error;      // why uc_run_lle_noalloc cannot run, if so
 
     int fail(int code, const std::string& msg) {
         std::lock_guard<std::mutex> lk(err_mu);
@@ -114,6 +121,8 @@ struct uc_engine {
             if (t.dev) (void)hipFree(t.dev);
             if (t.host) (void)hipHostFree(t.host);
         }
+        if (noalloc_done) { (void)hipEventSynchronize(noalloc_done); (void)hipEventDestroy(noalloc_done); }
+        if (noalloc_host) (void)hipHostFree(noalloc_host);
     }
 };
 
@@ -478,6 +487,7 @@ int launch(uc_engine* e, Module* m, const char* name, UcJob& j, hipStream_t stre
     void* args[] = {&j};
     he = hipModuleLaunchKernel(f, (unsigned)blocks, 1, 1, kBlock, 1, 1, 0, stream, args, nullptr);
     if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipModuleLaunchKernel"));
+    if (!tb) return UC_OK;  // tables in the static scratch (uc_run_lle_noalloc)
     he = hipEventRecord(tb->done, stream);
     if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipEventRecord"));
     tb->in_flight = true;
@@ -524,6 +534,15 @@ int uc_engine_create(int device, uc_engine** out) {
     if (device >= 0) {
         hipError_t he = hipSetDevice(device);
         if (he != hipSuccess) return UC_ERR_HIP;
+        // set-up of uc_run_lle_noalloc (its failure leaves the other entry points usable)
+        char msg[256] = {0};
+        if (!uc_scratch_setup(device, msg, sizeof msg)) {
+            e->noalloc_error = msg;
+        } else {
+            he = hipHostMalloc((void**)&e->noalloc_host, sizeof(double) * UC_SCRATCH_DOUBLES, hipHostMallocDefault);
+            if (he == hipSuccess) he = hipEventCreateWithFlags(&e->noalloc_done, hipEventDisableTiming);
+            if (he != hipSuccess) e->noalloc_error = hip_msg(he, "scratch staging set-up");
+        }
     }
     *out = e.release();
     return UC_OK;
@@ -614,7 +633,10 @@ int uc_system_compile(uc_engine* e, uc_system* s, int strict, size_t* code_size)
     precompile(e, s, strict != 0);
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     Module* m = nullptr;
-    int rc = get_module(e, s, strict != 0, false, &m);
+    const bool load = e->device >= 0;  // an engine with a device also loads the module
+    int rc = load ? enter(e) : UC_OK;
+    if (rc) return rc;
+    rc = get_module(e, s, strict != 0, load, &m);
     if (rc) return rc;
     if (code_size) *code_size = m->code_size;
     return UC_OK;
@@ -786,6 +808,51 @@ int uc_run_lle(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_ly
 
 int uc_run_ls(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream) {
     return run_lyap(e, s, r, o, stream, true);
+}
+
+// uc_run_lle with the launch tables in the GPU's static scratch array
+// (scratch.hip) instead of a table buffer from the allocator: no hipMalloc,
+// hipFree, hipHostMalloc or module load happens here. The module must be
+// loaded already (uc_system_compile on this engine, or any earlier run of
+// the system with the same strict flag); the pinned host image was made with
+// the engine. Same kernels and launch shape as uc_run_lle, so the results
+// are identical. Calls that overlap on one GPU share its scratch and race.
+int uc_run_lle_noalloc(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream) {
+    if (!e || !s || !r || !o) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    int rc = prep_lyap(e, s, r, o);
+    if (rc) return rc;
+    const size_t need = table_doubles(r->c);
+    if (need > (size_t)UC_SCRATCH_DOUBLES)
+        return e->fail(UC_ERR_ARGUMENT, "uc_run_lle_noalloc: the axes need " + std::to_string(need) +
+                                            " doubles of scratch, the maximum is " + std::to_string(UC_SCRATCH_DOUBLES) +
+                                            " (nx + ny <= " + std::to_string(UC_SCRATCH_DOUBLES - kMaxValues - kMaxVars) + ")");
+    if ((rc = enter(e))) return rc;
+    if (!e->noalloc_error.empty()) return e->fail(UC_ERR_HIP, "uc_run_lle_noalloc: " + e->noalloc_error);
+    double* dev = (double*)uc_scratch_ptr(e->device);
+    if (!dev || !e->noalloc_host || !e->noalloc_done) return e->fail(UC_ERR_HIP, "uc_run_lle_noalloc: scratch not set up");
+    Module* m = find_module(e, module_key(program_source(s, r->c.strict != 0), r->c.strict != 0));
+    if (!m || !m->mod)
+        return e->fail(UC_ERR_ARGUMENT, "uc_run_lle_noalloc: the system's module is not loaded on this engine "
+                                        "(call uc_system_compile with the same strict flag first)");
+    hipStream_t st = (hipStream_t)stream;
+    // the previous call's copy must have read the host image before it is rewritten
+    hipError_t he = hipEventSynchronize(e->noalloc_done);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipEventSynchronize(scratch staging)"));
+    fill_tables(s->n_vars, r->c, e->noalloc_host);
+    he = hipMemcpyAsync(dev, e->noalloc_host, need * sizeof(double), hipMemcpyHostToDevice, st);
+    if (he == hipSuccess) he = hipEventRecord(e->noalloc_done, st);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "hipMemcpyAsync(scratch tables)"));
+    UcJob j;
+    wire_job(r->c, dev, j);
+    fill_lyap(r, o, j);
+    bool step_axis = false;
+    for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
+    if (pair_lle(e, false, *r, step_axis, j.n_cells)) {
+        const std::string name = std::string(lyap_kernel(false, r->c.precision)) + "_x2";
+        return launch(e, m, name.c_str(), j, st, nullptr, (j.n_cells + 1) / 2);
+    }
+    return launch(e, m, lyap_kernel(false, r->c.precision), j, st, nullptr);
 }
 
 }  // extern "C"

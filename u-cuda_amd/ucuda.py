@@ -140,6 +140,8 @@ _SIGNATURES = {
                                      C.POINTER(C.c_double)]),
     "uc_run_bif": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BifRequest), C.POINTER(BifOutput), C.c_void_p]),
     "uc_run_lle": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.POINTER(LyapOutput), C.c_void_p]),
+    "uc_run_lle_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.POINTER(LyapOutput),
+                                     C.c_void_p]),
     "uc_run_ls": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.POINTER(LyapOutput), C.c_void_p]),
     "uc_run_basins": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BasinsRequest), C.POINTER(BasinsOutput),
                                 C.c_void_p]),
@@ -600,8 +602,10 @@ def run_fastsync(sysm, values, ic_master, ic_slave, k_forward, k_backward, with_
 def run_lyap(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=False, eps=1e-4,
              method=METHOD_VARIATIONAL, probe_seed=REFERENCE_PROBE_SEED, max_value=1e6,
              precision=PREC_F64, strict=False, want_final_state=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0,
-             out=None, stream=None, sync=True):
-    """LLE map (spectrum=False, values[1, cells]) or full spectrum (values[N, cells])."""
+             out=None, stream=None, sync=True, noalloc=False):
+    """LLE map (spectrum=False, values[1, cells]) or full spectrum (values[N, cells]).
+    noalloc=True: uc_run_lle_noalloc (tables in the static device scratch;
+    the system's module must be loaded: sysm.compile(strict) first)."""
     import torch
     eng = sysm.engine
     n = n_cells(axes)
@@ -617,8 +621,13 @@ def run_lyap(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=Fals
                         ic_seed, ic_lo, ic_hi)
     r.NT, r.eps, r.method, r.probe_seed = float(NT), float(eps), int(method), int(probe_seed)
     o = LyapOutput(_ptr(out["values"]).value, _ptr(out["flags"]).value, _ptr(out.get("final_state")).value)
-    fn = lib().uc_run_ls if spectrum else lib().uc_run_lle
-    eng.check(fn(eng.h, sysm.h, C.byref(r), C.byref(o), _stream(stream)), "uc_run_ls" if spectrum else "uc_run_lle")
+    if noalloc:
+        if spectrum:
+            raise ValueError("noalloc covers the LLE map only")
+        fn, name = lib().uc_run_lle_noalloc, "uc_run_lle_noalloc"
+    else:
+        fn, name = (lib().uc_run_ls, "uc_run_ls") if spectrum else (lib().uc_run_lle, "uc_run_lle")
+    eng.check(fn(eng.h, sysm.h, C.byref(r), C.byref(o), _stream(stream)), name)
     if sync:
         torch.cuda.synchronize(dev)
     del keep
