@@ -105,6 +105,13 @@ typedef struct uc_bif_request {
     int writable_var;   /* sampled state index, -1 = x0 + pi*x1 + e*x2 */
     int pre_scaller;
     int max_peaks;      /* stored peaks per cell (reference cap 2500) */
+    /* precision of the recorded run when c.precision is UC_PREC_F32C:
+     * UC_PREC_F64 (0, the default) continues the compensated state in
+     * double for the recorded run, so the peaks are double-precision values
+     * (float stages displace a weakly contracting cycle by ~1e-6 relative);
+     * UC_PREC_F32C records in float stages as well (faster). Ignored for
+     * UC_PREC_F64 requests. */
+    int record_precision;
 } uc_bif_request;
 
 typedef struct uc_bif_output {

@@ -275,6 +275,26 @@ def bif_cells(step, job, cells, traj_len=0, threads=1, noise=0.0, noise_seed=0, 
     return dict(flags=flags, peaks=peaks, intervals=intervals, traj=traj)
 
 
+def bif_cells_phased(step, job, job_rec, cells, noise, noise_rec, noise_seed, threads=1, noise_every=100):
+    """Bifurcation cells whose transient runs `job` with relative kicks of
+    size `noise` and whose recorded run uses `job_rec`'s parameters with
+    kicks of size `noise_rec` (a transient in one precision, the record in
+    another)."""
+    cells, cp = _cells(cells)
+    n = len(cells)
+    mp = max(1, job.max_peaks)
+    flags = np.zeros(n, dtype=np.int32)
+    peaks = np.full((mp, n), np.nan)
+    intervals = np.full((mp, n), np.nan)
+    L = lib()
+    L.oc_bif_cells_phased.argtypes = [_P, C.POINTER(OcJob), C.POINTER(OcJob), _LL, C.c_longlong, _I, _D, _D, _D,
+                                      C.c_int, C.c_int, C.c_double, C.c_double, C.c_ulonglong, C.c_int]
+    L.oc_bif_cells_phased(step.ptr, C.byref(job), C.byref(job_rec), cp, n, flags.ctypes.data_as(_I),
+                          peaks.ctypes.data_as(_D), intervals.ctypes.data_as(_D), None, 0, threads,
+                          float(noise), float(noise_rec), noise_seed, int(noise_every))
+    return dict(flags=flags, peaks=peaks, intervals=intervals)
+
+
 def lle_fd_cells(step, job, cells, threads=1):
     cells, cp = _cells(cells)
     out = np.zeros((1, len(cells)))
@@ -354,12 +374,21 @@ def bif_rounding_stable(step, job, cells, base, precision, rel, min_peaks=9, thr
     """Cells of `base` (bif_cells output) with more than min_peaks - 1 peaks
     whose peak count and values (within `rel` of the largest |peak|) survive
     three runs with state noise and parameter perturbations of the given
-    precision's rounding size (ROUNDING)."""
-    noise, dp = ROUNDING[precision]
+    precision's rounding size (ROUNDING). precision "f32_rec64" models a
+    float transient followed by a double recorded window: float-sized noise
+    and parameter perturbations in the transient, double-sized noise and
+    the exact parameters in the record."""
     n = base["flags"]
     stable = n >= min_peaks
     for k, signs in enumerate(_PERTURB_SIGNS):
-        r = bif_cells(step, perturbed_job(job, dp, signs), cells, threads=threads, noise=noise, noise_seed=2000 + k)
+        if precision == "f32_rec64":
+            noise, dp = ROUNDING["f32"]
+            r = bif_cells_phased(step, perturbed_job(job, dp, signs), job, cells, noise, ROUNDING["f64"][0],
+                                 2000 + k, threads=threads)
+        else:
+            noise, dp = ROUNDING[precision]
+            r = bif_cells(step, perturbed_job(job, dp, signs), cells, threads=threads, noise=noise,
+                          noise_seed=2000 + k)
         for c in np.nonzero(stable)[0]:
             m = n[c]
             if r["flags"][c] != m:

@@ -327,9 +327,12 @@ static void adv_krs_noisy(void* ctx, double* x, const double* a, double h) {
     if (++c->count % c->every == 0) noise_kick(x, c->n, c->noise, &c->rng);
 }
 
+/* jr (optional): the recorded run uses jr's parameters and its own noise
+ * size noise_rec (a transient at one precision, the record at another). */
 static void bif_cell(oc_step_fn step, const oc_job* j, long long cell, long long iters_max, double* data,
                      double* pk, double* tm, int* flag_out, double* peaks, double* intervals, long long stride,
-                     double* traj, int traj_len, double noise, unsigned long long noise_seed, int every) {
+                     double* traj, int traj_len, double noise, unsigned long long noise_seed, int every,
+                     const oc_job* jr, double noise_rec) {
     double a[OC_MAXV], x[OC_MAXN], h;
     if (!oc_cell_setup(j, cell, a, x, &h)) { *flag_out = -1; return; }
     long long skip = j->h > 0.0 ? (long long)(j->transient_time / j->h) : 0, iters = iters_max;
@@ -343,10 +346,17 @@ static void bif_cell(oc_step_fn step, const oc_job* j, long long cell, long long
     const oc_adv_fn adv = noise > 0.0 ? adv_krs_noisy : adv_krs;
     void* ctx = noise > 0.0 ? (void*)&ns : (void*)step;
     int flag = loop_generic(adv, ctx, x, a, h, skip, j->n_vars, j->pre_scaller, 0, j->max_value, NULL);
-    if (flag == 1 || flag == -1)
+    if (!(flag == 1 || flag == -1)) return (void)(*flag_out = flag);
+    if (jr) {
+        double xr[OC_MAXN], hr;
+        if (!oc_cell_setup(jr, cell, a, xr, &hr)) { *flag_out = -1; return; }
+        noisy_step nr = {step, noise_rec, ns.rng ^ 0x9E3779B97F4A7C15ull, 0, every > 0 ? every : 100, j->n_vars};
+        const oc_adv_fn advr = noise_rec > 0.0 ? adv_krs_noisy : adv_krs;
+        void* ctxr = noise_rec > 0.0 ? (void*)&nr : (void*)step;
+        flag = loop_generic(advr, ctxr, x, a, hr, iters, j->n_vars, j->pre_scaller, j->writable_var, j->max_value, data);
+    } else {
         flag = loop_generic(adv, ctx, x, a, h, iters, j->n_vars, j->pre_scaller, j->writable_var, j->max_value, data);
-    else
-        return (void)(*flag_out = flag);
+    }
     for (int i = 0; i < traj_len && i < iters; ++i) traj[(long long)i * stride] = data[i];
     if (flag == -1 || flag == 0) { *flag_out = flag; return; }
     int cnt = oc_peak_finder(data, iters, pk, tm, h * (double)j->pre_scaller);
@@ -358,9 +368,11 @@ static void bif_cell(oc_step_fn step, const oc_job* j, long long cell, long long
     }
 }
 
-void oc_bif_cells_noisy(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int* flags,
-                        double* peaks, double* intervals, double* traj, int traj_len, int threads, double noise,
-                        unsigned long long noise_seed, int noise_every) {
+/* Transient of job j with noise, record with job jr's parameters and
+ * noise_rec (jr NULL: the record continues with j and noise). */
+void oc_bif_cells_phased(oc_step_fn step, const oc_job* j, const oc_job* jr, const long long* cells, long long n_sel,
+                         int* flags, double* peaks, double* intervals, double* traj, int traj_len, int threads,
+                         double noise, double noise_rec, unsigned long long noise_seed, int noise_every) {
     const long long iters_max = (long long)ceil(j->t_max / worst_step(j) / j->pre_scaller);
 #pragma omp parallel num_threads(threads > 1 ? threads : 1)
     {
@@ -370,11 +382,18 @@ void oc_bif_cells_noisy(oc_step_fn step, const oc_job* j, const long long* cells
 #pragma omp for schedule(dynamic, 1)
         for (long long s = 0; s < n_sel; ++s)
             bif_cell(step, j, cells[s], iters_max, data, pk, tm, &flags[s], peaks + s, intervals + s, n_sel,
-                     traj ? traj + s : NULL, traj ? traj_len : 0, noise, noise_seed, noise_every);
+                     traj ? traj + s : NULL, traj ? traj_len : 0, noise, noise_seed, noise_every, jr, noise_rec);
         free(data);
         free(pk);
         free(tm);
     }
+}
+
+void oc_bif_cells_noisy(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int* flags,
+                        double* peaks, double* intervals, double* traj, int traj_len, int threads, double noise,
+                        unsigned long long noise_seed, int noise_every) {
+    oc_bif_cells_phased(step, j, NULL, cells, n_sel, flags, peaks, intervals, traj, traj_len, threads, noise, noise,
+                        noise_seed, noise_every);
 }
 
 void oc_bif_cells(oc_step_fn step, const oc_job* j, const long long* cells, long long n_sel, int* flags,

@@ -22,7 +22,10 @@ Tolerances:
   * Peaks are asserted on cells whose oracle peaks survive state noise and
     parameter perturbations of the mode's rounding size (oracle.ROUNDING):
     FP64 1e-6 relative; FP32-compensated 1e-6 relative (values) and 1e-5
-    (intervals) on the cells that hold 1e-6 under float-sized noise.
+    (intervals): with the recorded window in double (the default) on the
+    whole 1024-rho sweep, on the cells that hold 5e-7 under float-sized noise
+    in the transient and double-sized noise in the record; with float
+    recording on the cells that hold 1e-6 under float-sized noise.
   * FP32-compensated short-horizon trajectories: within 4x the deviation
     that float-sized per-step noise produces in the FP64 oracle (the float
     stages are amplified by the flow; ~3e-4 relative at t = 1 for rho = 250).
@@ -85,11 +88,35 @@ def test_bif_lorenz_f64_fast(gpu_engine):
     np.testing.assert_allclose(s2["traj"].cpu().numpy(), r2["traj"], rtol=1e-10, atol=1e-12)
 
 
-def test_bif_lorenz_f32c(gpu_engine):
-    print("case: Lorenz bif 64 rho, F32 compensated", flush=True)
-    s, axes, vals, step, job = _lorenz_case(gpu_engine)
+def test_bif_lorenz_f32c_full_sweep(gpu_engine):
+    """The reference's CPU case whole (1024 rho on [0, 250], 1e4 + 1e4 RK4
+    steps, <= 256 peaks) in FP32C with the recorded window in double (the
+    default record_precision). Checked cells: those whose FP64 oracle peaks
+    move by <= 5e-7 (half the 1e-6 bar) when the transient gets float-sized
+    state noise and parameter perturbations and the record double-sized
+    noise, i.e. the rounding this method injects; there every peak must
+    match to 1e-6 relative."""
+    print("case: Lorenz bif 1024 rho (config 1), F32C transient + F64 record", flush=True)
+    s, axes, vals, step, job = _lorenz_case(gpu_engine, n_rho=1024)
     gpu = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 100.0, 100.0, max_peaks=256,
                         precision=ucuda.PREC_F32C)
+    ref, stable = _stable_cells(step, job, range(1024), "f32_rec64", 5e-7)
+    assert stable.sum() >= 40, stable.sum()
+    flags = gpu["flags"].cpu().numpy()
+    pk = gpu["peaks"].cpu().numpy()
+    errs = [np.abs(pk[:n, c] - ref["peaks"][:n, c]).max() / np.abs(ref["peaks"][:n, c]).max()
+            for c, n in zip(np.nonzero(stable)[0], ref["flags"][stable]) if flags[c] == n]
+    print("FP32C + F64 record: %d stable cells, worst relative peak error %.2e" % (stable.sum(), max(errs)))
+    _check_periodic_peaks(gpu, ref, stable, 1e-6, rel_intervals=1e-5)
+    # flag classes (periodic / fixed point / diverged) on the whole sweep
+    assert ((flags <= 0) == (ref["flags"] <= 0)).mean() >= 0.99
+
+
+def test_bif_lorenz_f32c(gpu_engine):
+    print("case: Lorenz bif 64 rho, F32 compensated with the record in float too", flush=True)
+    s, axes, vals, step, job = _lorenz_case(gpu_engine)
+    gpu = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 100.0, 100.0, max_peaks=256,
+                        precision=ucuda.PREC_F32C, record_precision=ucuda.PREC_F32C)
     # float-sized noise moves the peaks of most periodic Lorenz cells by
     # 1e-6..1e-5 (weak contraction along the orbit); the cells that hold 1e-6
     # under it are checked at 1e-6, the others are reported
@@ -114,7 +141,7 @@ def test_bif_lorenz_f32c(gpu_engine):
     # plus 1e-6 of the trajectory's scale (a CPU float emulation of the step
     # reaches 0.74x of that bound).
     s2 = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 0.0, 1.0, max_peaks=4, traj_len=100,
-                       precision=ucuda.PREC_F32C)
+                       precision=ucuda.PREC_F32C, record_precision=ucuda.PREC_F32C)
     job2 = oracle.make_job(3, axes, [0.1, 0.1, 0.1], vals, 0.01, 0.0, 1.0, max_peaks=4)
     r2 = oracle.bif_cells(step, job2, range(64), traj_len=100)["traj"]
     spread = np.zeros_like(r2)

@@ -26,6 +26,7 @@
 #include <condition_variable>
 #include <deque>
 #include <thread>
+#include <cfloat>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -410,6 +411,15 @@ void fill_tables(int n_vars, const uc_common& c, double* host) {
     for (int i = 0; i < n_vars; ++i) ic[i] = c.initial_conditions ? c.initial_conditions[i] : 0.0;
 }
 
+// The largest float <= v (FLT_MAX above its range): for a finite float x,
+// x > float_at_most(v) exactly when (double)x > v.
+float float_at_most(double v) {
+    if (v >= (double)FLT_MAX) return FLT_MAX;
+    float f = (float)v;
+    if ((double)f > v) f = std::nextafter(f, -INFINITY);
+    return f;
+}
+
 // Descriptor fields common to every analysis, pointing at device tables.
 void wire_job(const uc_common& c, const double* dev, UcJob& j) {
     const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
@@ -432,7 +442,13 @@ void wire_job(const uc_common& c, const double* dev, UcJob& j) {
     j.ic_lo = c.ic_lo;
     j.ic_hi = c.ic_hi;
     j.h = c.h;
+    j.hh = 0.5 * c.h;
+    j.h6 = c.h * (1.0 / 6.0);
+    j.hf = (float)c.h;
+    j.hhf = 0.5f * j.hf;
+    j.h6f = j.hf * (1.0f / 6.0f);
     j.max_value = c.max_value;
+    j.max_value_f = float_at_most(c.max_value);
     j.transient_time = c.transient_time;
     j.t_max = c.t_max;
     j.pre_scaller = 1;
@@ -688,11 +704,14 @@ static int prep_bif(uc_engine* e, uc_system* s, const uc_bif_request* r, const u
     const long long iters = (long long)std::ceil(r->c.t_max / hw / r->pre_scaller);
     if (iters <= 0) return e->fail(UC_ERR_ARGUMENT, "computed amountOfPointsInBlock <= 0");
     if (r->max_peaks > 2500) return e->fail(UC_ERR_ARGUMENT, "max_peaks above the reference cap 2500");
+    if (r->record_precision != UC_PREC_F64 && r->record_precision != UC_PREC_F32C)
+        return e->fail(UC_ERR_ARGUMENT, "unknown record_precision");
     *iters_out = iters;
     return UC_OK;
 }
 
 static void fill_bif(const uc_bif_request* r, const uc_bif_output* o, long long iters, UcJob& j) {
+    j.record_f64 = r->c.precision == UC_PREC_F32C && r->record_precision == UC_PREC_F64;
     j.skip_steps = r->c.h > 0.0 ? (long long)(r->c.transient_time / r->c.h) : 0;
     j.n_iters = iters;
     j.pre_scaller = r->pre_scaller;

@@ -142,20 +142,47 @@ __device__ __forceinline__ void uc_kahan_add(float& hi, float& c, float incr) {
     hi = t;
 }
 
+// The same update for the increment f * g: y = f * g - c is rounded once
+// (one FMA instead of a multiply and a subtraction).
+__device__ __forceinline__ void uc_kahan_fma(float& hi, float& c, float f, float g) {
+    const float y = fmaf(f, g, -c);
+    const float t = hi + y;
+    c = (t - hi) - y;
+    hi = t;
+}
+
+// Step factors h, h/2, h/6. A loop whose step is the same on every lane
+// passes StepU, read from the job descriptor (host-computed, so they sit in
+// scalar registers and the stage FMAs read one VGPR fewer); other loops pass
+// the plain per-lane h. Both give the same values.
+template <typename T> struct StepU { T h, hh, h6; };
+__device__ __forceinline__ float uc_h(float h) { return h; }
+__device__ __forceinline__ double uc_h(double h) { return h; }
+__device__ __forceinline__ float uc_hh(float h) { return 0.5f * h; }
+__device__ __forceinline__ double uc_hh(double h) { return 0.5 * h; }
+__device__ __forceinline__ float uc_h6(float h) { return h * (1.0f / 6.0f); }
+__device__ __forceinline__ double uc_h6(double h) { return h * (1.0 / 6.0); }
+template <typename T> __device__ __forceinline__ T uc_h(const StepU<T>& s) { return s.h; }
+template <typename T> __device__ __forceinline__ T uc_hh(const StepU<T>& s) { return s.hh; }
+template <typename T> __device__ __forceinline__ T uc_h6(const StepU<T>& s) { return s.h6; }
+__device__ __forceinline__ void uc_job_step(const UcJob& j, StepU<double>& s) { s.h = j.h; s.hh = j.hh; s.h6 = j.h6; }
+__device__ __forceinline__ void uc_job_step(const UcJob& j, StepU<float>& s) { s.h = j.hf; s.hh = j.hhf; s.h6 = j.h6f; }
+
 #if UC_CUSTOM_KRS
 // A caller-supplied step body (the reference's custom KRS schemes,
 // U-CUDA/system_record.h:10-13) is the step map itself: numb = double, or
 // float without compensation (the body forms X + increment internally).
-__device__ __forceinline__ void uc_rk4(StateF64& s, const double* a, double h, double) { uc_krs_step<double>(s.x, a, h); }
-__device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, float h, float) { uc_krs_step<float>(s.hi, a, h); }
+template <class H> __device__ __forceinline__ void uc_rk4(StateF64& s, const double* a, H h, double) { uc_krs_step<double>(s.x, a, uc_h(h)); }
+template <class H> __device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, H h, float) { uc_krs_step<float>(s.hi, a, uc_h(h)); }
 #else
 // Classic RK4 with the reference's stage structure and update order
 // (U-CUDA/codegen.cpp:597-606): X1 = X + 0.5*h*k for stages 0,1, X1 = X + h*k
 // for stage 2, X += h*(k0 + 2*k1 + 2*k2 + k3)/6. Accumulating the bracket
 // left to right keeps the reference's summation order.
-__device__ __forceinline__ void uc_rk4(StateF64& s, const double* a, double h, double t) {
+template <class H>
+__device__ __forceinline__ void uc_rk4(StateF64& s, const double* a, H hs, double t) {
     double x1[UC_N], k[UC_N], acc[UC_N];
-    const double hh = 0.5 * h;
+    const double h = uc_h(hs), hh = uc_hh(hs);
     uc_rhs<double>(s.x, a, t, k);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = k[i]; x1[i] = s.x[i] + hh * k[i]; }
@@ -172,14 +199,15 @@ __device__ __forceinline__ void uc_rk4(StateF64& s, const double* a, double h, d
 #if UC_STRICT
         s.x[i] = s.x[i] + h * acc[i] / 6.0;
 #else
-        s.x[i] = fma(h * (1.0 / 6.0), acc[i], s.x[i]);
+        s.x[i] = fma(uc_h6(hs), acc[i], s.x[i]);
 #endif
     }
 }
 
-__device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, float h, float t) {
+template <class H>
+__device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, H hs, float t) {
     float x1[UC_N], k[UC_N], acc[UC_N];
-    const float hh = 0.5f * h;
+    const float h = uc_h(hs), hh = uc_hh(hs);
     uc_rhs<float>(s.hi, a, t, k);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = k[i]; x1[i] = s.hi[i] + hh * k[i]; }
@@ -190,11 +218,11 @@ __device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, float h, fl
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = acc[i] + 2.0f * k[i]; x1[i] = s.hi[i] + h * k[i]; }
     uc_rhs<float>(x1, a, t + h, k);
-    const float h6 = h * (1.0f / 6.0f);
+    const float h6 = uc_h6(hs);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) {
         acc[i] = acc[i] + k[i];
-        uc_kahan_add(s.hi[i], s.c[i], h6 * acc[i]);
+        uc_kahan_fma(s.hi[i], s.c[i], h6, acc[i]);
     }
 }
 
@@ -203,11 +231,11 @@ __device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, float h, fl
 // RK4 on the state together with K tangent vectors V (row-major K x N): the
 // variational equations V' = J(X) V integrated with the same stages, which
 // is exactly the derivative of the discrete RK4 map.
-template <int K>
-__device__ __forceinline__ void uc_rk4_tan(StateF64& s, double* V, const double* a, double h, double t) {
+template <int K, class H>
+__device__ __forceinline__ void uc_rk4_tan(StateF64& s, double* V, const double* a, H hs, double t) {
     double x1[UC_N], k[UC_N], acc[UC_N];
     double v1[K * UC_N], l[K * UC_N], lacc[K * UC_N];
-    const double hh = 0.5 * h;
+    const double h = uc_h(hs), hh = uc_hh(hs);
     uc_rhs<double>(s.x, a, t, k);
     uc_tangent<double, K>(s.x, a, t, V, l);
 #pragma unroll
@@ -234,7 +262,7 @@ __device__ __forceinline__ void uc_rk4_tan(StateF64& s, double* V, const double*
 #if UC_STRICT
         s.x[i] = s.x[i] + h * acc[i] / 6.0;
 #else
-        s.x[i] = fma(h * (1.0 / 6.0), acc[i], s.x[i]);
+        s.x[i] = fma(uc_h6(hs), acc[i], s.x[i]);
 #endif
     }
 #pragma unroll
@@ -243,16 +271,16 @@ __device__ __forceinline__ void uc_rk4_tan(StateF64& s, double* V, const double*
 #if UC_STRICT
         V[i] = V[i] + h * lacc[i] / 6.0;
 #else
-        V[i] = fma(h * (1.0 / 6.0), lacc[i], V[i]);
+        V[i] = fma(uc_h6(hs), lacc[i], V[i]);
 #endif
     }
 }
 
-template <int K>
-__device__ __forceinline__ void uc_rk4_tan(StateF32C& s, float* V, const float* a, float h, float t) {
+template <int K, class H>
+__device__ __forceinline__ void uc_rk4_tan(StateF32C& s, float* V, const float* a, H hs, float t) {
     float x1[UC_N], k[UC_N], acc[UC_N];
     float v1[K * UC_N], l[K * UC_N], lacc[K * UC_N];
-    const float hh = 0.5f * h;
+    const float h = uc_h(hs), hh = uc_hh(hs);
     uc_rhs<float>(s.hi, a, t, k);
     uc_tangent<float, K>(s.hi, a, t, V, l);
 #pragma unroll
@@ -273,11 +301,11 @@ __device__ __forceinline__ void uc_rk4_tan(StateF32C& s, float* V, const float* 
     for (int i = 0; i < K * UC_N; ++i) { lacc[i] = lacc[i] + 2.0f * l[i]; v1[i] = V[i] + h * l[i]; }
     uc_rhs<float>(x1, a, t + h, k);
     uc_tangent<float, K>(x1, a, t + h, v1, l);
-    const float h6 = h * (1.0f / 6.0f);
+    const float h6 = uc_h6(hs);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) {
         acc[i] = acc[i] + k[i];
-        uc_kahan_add(s.hi[i], s.c[i], h6 * acc[i]);
+        uc_kahan_fma(s.hi[i], s.c[i], h6, acc[i]);
     }
 #pragma unroll
     for (int i = 0; i < K * UC_N; ++i) { lacc[i] = lacc[i] + l[i]; V[i] = fmaf(h6, lacc[i], V[i]); }
@@ -333,6 +361,35 @@ __device__ __forceinline__ bool uc_diverged(const StateF32C& s, double max_value
     for (int i = 0; i < UC_N; ++i) chk = chk + fabsf(s.hi[i]);
     if (!(chk <= __FLT_MAX__)) return true;  // NaN or Inf
     return max_value != 0.0 && (double)chk > max_value;
+}
+
+// The same check for loops whose bound is wave-uniform, without branches:
+// F32C compares the float sum against max_value_f (the largest float <=
+// max_value, so the verdict is the double comparison's).
+__device__ __forceinline__ bool uc_diverged_u(const StateF64& s, const UcJob& j) {
+    double chk = 0.0;
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) chk = chk + fabs(s.x[i]);
+    return !(chk <= __DBL_MAX__) || (j.max_value != 0.0 && chk > j.max_value);
+}
+__device__ __forceinline__ bool uc_diverged_u(const StateF32C& s, const UcJob& j) {
+    float chk = 0.0f;
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) chk = chk + fabsf(s.hi[i]);
+    return !(chk <= __FLT_MAX__) || (j.max_value != 0.0 && chk > j.max_value_f);
+}
+
+// dst = take ? src : dst, member by member (selects, no branch)
+__device__ __forceinline__ void uc_select(StateF64& dst, const StateF64& src, bool take) {
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) dst.x[i] = take ? src.x[i] : dst.x[i];
+}
+__device__ __forceinline__ void uc_select(StateF32C& dst, const StateF32C& src, bool take) {
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) {
+        dst.hi[i] = take ? src.hi[i] : dst.hi[i];
+        dst.c[i] = take ? src.c[i] : dst.c[i];
+    }
 }
 
 template <class S>
@@ -598,8 +655,30 @@ struct PeakObserver {
 };
 
 // ================================================================ kernels ==
+// The recorded run of a bifurcation cell (after a transient that ended with
+// `flag`): streaming peaks, the cell's flag, the optional final state.
+template <class S>
+__device__ __forceinline__ void uc_bif_record(const UcJob& j, long long lane, S& s, const typename Prec<S>::T* a,
+                                              typename Prec<S>::T h, double hd, double& t, bool uni, long long iters,
+                                              int flag) {
+    PeakObserver obs;
+    obs.w0 = 0.0; obs.w1 = 0.0; obs.n = iters; obs.c_next = 2; obs.mode = 0; obs.raw = 0;
+    obs.last_t = 0.0; obs.time_step = hd * (double)j.pre_scaller; obs.max_peaks = j.max_peaks;
+    obs.stride = j.stride; obs.peaks = j.out0 + lane; obs.intervals = j.out1 ? j.out1 + lane : nullptr;
+    obs.traj = j.traj ? j.traj + lane : nullptr; obs.traj_len = j.traj ? j.traj_len : 0;
+    if (flag == 1 || flag == -1)
+        flag = uni ? uc_loop_u(s, a, h, t, j.n_iters, j.pre_scaller, j.writable_var, j.max_value, obs)
+                   : uc_loop(s, a, h, t, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
+    j.flag[lane] = (flag == 1 && !j.raw_flags) ? obs.count() : flag;
+    if (j.final_state) {
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.stride + lane] = s.get(i);
+    }
+}
+
 // Bifurcation sweep (calculateDiscreteModelCUDA + peakFinderCUDA fused):
 // transient, recorded run with streaming peaks, flag = peak count / -1 / 0.
+// F32C with record_f64: the transient in F32C, the recorded run in double.
 template <class S>
 __device__ void uc_bif_cell(const UcJob& j, long long lane) {
     typedef typename Prec<S>::T T;
@@ -645,19 +724,19 @@ __device__ void uc_bif_cell(const UcJob& j, long long lane) {
     const bool uni = !uc_sweeps_step(j);  // step counts are wave-uniform unless the sweep is over h
     int flag = uni ? uc_loop_u(s, a, h, t, j.skip_steps, j.pre_scaller, 0, j.max_value, none)
                    : uc_loop(s, a, h, t, skip, j.pre_scaller, 0, j.max_value, none);
-    PeakObserver obs;
-    obs.w0 = 0.0; obs.w1 = 0.0; obs.n = iters; obs.c_next = 2; obs.mode = 0; obs.raw = 0;
-    obs.last_t = 0.0; obs.time_step = hd * (double)j.pre_scaller; obs.max_peaks = j.max_peaks;
-    obs.stride = j.stride; obs.peaks = j.out0 + lane; obs.intervals = j.out1 ? j.out1 + lane : nullptr;
-    obs.traj = j.traj ? j.traj + lane : nullptr; obs.traj_len = j.traj ? j.traj_len : 0;
-    if (flag == 1 || flag == -1)
-        flag = uni ? uc_loop_u(s, a, h, t, j.n_iters, j.pre_scaller, j.writable_var, j.max_value, obs)
-                   : uc_loop(s, a, h, t, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
-    j.flag[lane] = (flag == 1 && !j.raw_flags) ? obs.count() : flag;
-    if (j.final_state) {
+    if constexpr (sizeof(T) == 4) {
+        if (j.record_f64) {
+            // The recorded window in double from the compensated state, with
+            // the double parameters and step: peaks are the output, and float
+            // stages would displace a weakly contracting cycle by ~1e-6.
+            StateF64 s64;
 #pragma unroll
-        for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.stride + lane] = s.get(i);
+            for (int i = 0; i < UC_N; ++i) s64.x[i] = s.get(i);
+            uc_bif_record(j, lane, s64, ad, hd, hd, t, uni, iters, flag);
+            return;
+        }
     }
+    uc_bif_record(j, lane, s, a, h, hd, t, uni, iters, flag);
 }
 
 // Random unit probe vectors (U-CUDA/cudaLibrary.cu:2603-2626 for one vector,
@@ -1012,8 +1091,10 @@ __device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
         snap[c] = s[c];
         dead[c] = !ok[c] || (c == 1 && !has1);
     }
-    const T h = Prec<S>::to_t(j.h);
-    const double hdd = (double)h;
+    // the step is the same for every lane (no sweep over h on this path)
+    StepU<T> h;
+    uc_job_step(j, h);
+    const double hdd = (double)h.h;
     double t = 0.0;
 
     // transient: uc_loop_u with pre = 1 and no observer, per cell
@@ -1054,62 +1135,75 @@ __device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
     }
     const T big = sizeof(T) == 4 ? (T)__FLT_MAX__ : (T)1e300;
     const int nts = (int)j.nt_steps;
-    bool all_dead = false;
-    for (long long b = 0; b < j.n_iters && !all_dead; ++b) {
-        // uc_block_tan_u for both cells
+    for (long long b = 0; b < j.n_iters; ++b) {
+        // uc_block_tan_u for both cells, with the magnitude check and the
+        // power-of-two rescale as selects: the per-cell branches of the
+        // check cost more than the check at blocks of ~10 steps
 #pragma unroll
         for (int c = 0; c < 2; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
         if (UC_USES_TIME) t += hdd;
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
-            if (!dead[c] && uc_diverged(s[c], j.max_value)) { dead[c] = true; snap[c] = s[c]; }
-        if (uc_wave_all(dead[0] && dead[1])) { all_dead = true; break; }
+        for (int c = 0; c < 2; ++c) {
+            const bool now = !dead[c] && uc_diverged_u(s[c], j);
+            uc_select(snap[c], s[c], now);  // a lane found diverged keeps its state at the check
+            dead[c] = dead[c] || now;
+        }
+        if (uc_wave_all(dead[0] && dead[1])) break;
 #pragma unroll 4
         for (int i = 0; i < nts; ++i) {
 #pragma unroll
             for (int c = 0; c < 2; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
             if (UC_USES_TIME) t += hdd;
         }
-        // the fast path's renormalisation, per live cell
+        // the fast path's renormalisation: |V|^2 back into [0.5, 2) by a
+        // power of two (k = 0 leaves V as it is on the other lanes)
+        bool need[2];
 #pragma unroll
         for (int c = 0; c < 2; ++c) {
-            if (dead[c]) continue;
             T n2 = (T)0.0;
 #pragma unroll
             for (int i = 0; i < UC_N; ++i) n2 = n2 + V[c][i] * V[c][i];
-            if (!ref_mode[c] && n2 > (T)1e-28 * start2[c] && n2 <= big) {
-                const int k = uc_frexp_exp(n2) >> 1;
+            const bool fast = !ref_mode[c] && n2 > (T)1e-28 * start2[c] && n2 <= big;
+            const int k = fast ? (uc_frexp_exp(n2) >> 1) : 0;
 #pragma unroll
-                for (int i = 0; i < UC_N; ++i) {
-                    V[c][i] = uc_ldexp(V[c][i], -k);
-                    Vs[c][i] = V[c][i];
-                }
-                ksum[c] += k;
-                start2[c] = uc_ldexp(n2, -2 * k);
-                continue;
-            }
-            double inv0 = 1.0;
-            if (!ref_mode[c]) {
-                double s2 = 1.0;
-                if (b > 0) {
-                    s2 = 0.0;
-#pragma unroll
-                    for (int i = 0; i < UC_N; ++i) s2 = s2 + (double)Vs[c][i] * (double)Vs[c][i];
-                }
-                lsum[c] = (double)ksum[c] * UC_LN2 + 0.5 * log(s2);
-                inv0 = 1.0 / sqrt(s2);
-                ref_mode[c] = true;
-            }
-            double v[UC_N], nrm = 0.0;
-#pragma unroll
-            for (int i = 0; i < UC_N; ++i) { v[i] = (double)V[c][i] * inv0; nrm = nrm + v[i] * v[i]; }
-            nrm = sqrt(nrm);
-            if (nrm <= 1e-14) nrm = 1e-14;
-            lsum[c] = lsum[c] + log(nrm);
-            const double inv = 1.0 / nrm;
-#pragma unroll
-            for (int i = 0; i < UC_N; ++i) V[c][i] = Prec<S>::to_t(v[i] * inv);
+            for (int i = 0; i < UC_N; ++i) V[c][i] = uc_ldexp(V[c][i], -k);
+            ksum[c] += k;
+            start2[c] = fast ? uc_ldexp(n2, -2 * k) : start2[c];
+            need[c] = !dead[c] && !fast;
         }
+        // the rest switch to (or stay in) the reference's unit
+        // renormalisation in double; Vs still holds their block-start vector
+        if (!uc_wave_all(!need[0] && !need[1])) {
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                if (!need[c]) continue;
+                double inv0 = 1.0;
+                if (!ref_mode[c]) {
+                    double s2 = 1.0;
+                    if (b > 0) {
+                        s2 = 0.0;
+#pragma unroll
+                        for (int i = 0; i < UC_N; ++i) s2 = s2 + (double)Vs[c][i] * (double)Vs[c][i];
+                    }
+                    lsum[c] = (double)ksum[c] * UC_LN2 + 0.5 * log(s2);
+                    inv0 = 1.0 / sqrt(s2);
+                    ref_mode[c] = true;
+                }
+                double v[UC_N], nrm = 0.0;
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) { v[i] = (double)V[c][i] * inv0; nrm = nrm + v[i] * v[i]; }
+                nrm = sqrt(nrm);
+                if (nrm <= 1e-14) nrm = 1e-14;
+                lsum[c] = lsum[c] + log(nrm);
+                const double inv = 1.0 / nrm;
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) V[c][i] = Prec<S>::to_t(v[i] * inv);
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 2; ++c)
+#pragma unroll
+            for (int i = 0; i < UC_N; ++i) Vs[c][i] = V[c][i];
     }
 #pragma unroll
     for (int c = 0; c < 2; ++c) {

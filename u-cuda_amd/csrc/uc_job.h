@@ -40,7 +40,14 @@ typedef struct UcJob {
 
     /* ---- integration ---- */
     double h;
+    /* h / 2 and h / 6 in double, and (float)h, its half and sixth in float,
+     * computed once on the host: kernels whose step is the same on every
+     * lane read them as kernel arguments, i.e. from scalar registers       */
+    double hh, h6;
+    float hf, hhf, h6f;
     double max_value;             /* 0 disables the magnitude check           */
+    float max_value_f;            /* the largest float <= max_value (float    */
+                                  /* x > max_value_f iff (double)x > max_value) */
     long long skip_steps;         /* transient steps when no axis sweeps h    */
     long long n_iters;            /* bif: recorded samples; lle/ls: blocks    */
     long long nt_steps;           /* lle/ls: steps per renormalisation block  */
@@ -51,6 +58,7 @@ typedef struct UcJob {
     int writable_var;             /* bif: sampled variable (-1: x0+pi*x1+e*x2)*/
     int max_peaks;                /* bif: stored peaks per cell               */
     int traj_len;                 /* bif: samples copied to traj (0 = none)   */
+    int record_f64;               /* bif, F32C: the recorded run in double    */
     int raw_flags;                /* bif: 0 flag = peak count; 1 flag = loop  */
                                   /* flag (1/-1/0); 2 the same under the DFT  */
                                   /* h-sweep rules (dft1dHSweepKernel)        */

@@ -52,7 +52,7 @@ class Common(C.Structure):
 
 class BifRequest(C.Structure):
     _fields_ = [("c", Common), ("writable_var", C.c_int), ("pre_scaller", C.c_int),
-                ("max_peaks", C.c_int)]
+                ("max_peaks", C.c_int), ("record_precision", C.c_int)]
 
 
 class BifOutput(C.Structure):
@@ -425,8 +425,11 @@ def _stream(stream):
 
 def run_bif(sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pre_scaller=1,
             max_peaks=256, max_value=1e6, precision=PREC_F64, strict=False, traj_len=0,
-            want_final_state=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0, out=None, stream=None, sync=True):
-    """Bifurcation sweep; returns dict(flags, peaks[max_peaks, cells], intervals, traj, final_state)."""
+            want_final_state=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0, out=None, stream=None, sync=True,
+            record_precision=PREC_F64):
+    """Bifurcation sweep; returns dict(flags, peaks[max_peaks, cells], intervals, traj, final_state).
+    record_precision (F32C sweeps): PREC_F64 records in double after the
+    F32C transient, PREC_F32C records in float stages too."""
     import torch
     eng = sysm.engine
     n = n_cells(axes)
@@ -442,6 +445,7 @@ def run_bif(sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pr
     r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
                         ic_seed, ic_lo, ic_hi)
     r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, max_peaks
+    r.record_precision = record_precision
     o = BifOutput(_ptr(out["flags"]).value, _ptr(out["peaks"]).value, _ptr(out["intervals"]).value,
                   _ptr(out.get("traj")).value, traj_len if out.get("traj") is not None else 0,
                   _ptr(out.get("final_state")).value)
@@ -454,7 +458,7 @@ def run_bif(sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pr
 
 def run_bif_periods(sysm, axes, ic, values, h, transient_time, t_max, eps_dbscan=0.1, writable_var=0,
                     pre_scaller=1, max_value=1e6, precision=PREC_F64, strict=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0,
-                    stream=None):
+                    stream=None, record_precision=PREC_F64):
     """2-D bifurcation map: periods[cells] (int32 CUDA tensor) = DBSCAN cluster
     count of each cell's peaks (-1 fixed point, 0 none / diverged)."""
     import torch
@@ -465,6 +469,7 @@ def run_bif_periods(sysm, axes, ic, values, h, transient_time, t_max, eps_dbscan
     r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
                         ic_seed, ic_lo, ic_hi)
     r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, 0
+    r.record_precision = record_precision
     eng.check(lib().uc_run_bif_periods(eng.h, sysm.h, C.byref(r), float(eps_dbscan), _ptr(periods).value,
                                        _stream(stream)), "uc_run_bif_periods")
     del keep
@@ -752,7 +757,7 @@ class Batch:
         return out
 
     def add_bif(self, sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pre_scaller=1,
-                max_peaks=64, max_value=1e6, precision=PREC_F64, strict=False):
+                max_peaks=64, max_value=1e6, precision=PREC_F64, strict=False, record_precision=PREC_F64):
         import torch
         n = n_cells(axes)
         dev = torch.device("cuda", self.engine.device)
@@ -762,6 +767,7 @@ class Batch:
         r = BifRequest()
         r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict)
         r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, max_peaks
+        r.record_precision = record_precision
         o = BifOutput(_ptr(out["flags"]).value, _ptr(out["peaks"]).value, _ptr(out["intervals"]).value, None, 0, None)
         jid = C.c_int()
         self.engine.check(lib().uc_batch_add_bif(self.h, sysm.h, C.byref(r), C.byref(o), C.byref(jid)),
@@ -849,10 +855,11 @@ class Server:
         return jid.value
 
     def submit_bif(self, sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pre_scaller=1,
-                   max_peaks=64, max_value=1e6, precision=PREC_F64, strict=False):
+                   max_peaks=64, max_value=1e6, precision=PREC_F64, strict=False, record_precision=PREC_F64):
         r = BifRequest()
         r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict)
         r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, max_peaks
+        r.record_precision = record_precision
         jid = C.c_longlong()
         self.engine.check(lib().uc_server_submit_bif(self.h, sysm.h, C.byref(r), C.byref(jid)), "uc_server_submit_bif")
         self.meta[jid.value] = ("bif", n_cells(axes), max(1, max_peaks), 1)
