@@ -514,6 +514,13 @@ def extra_ls(eng, n=1024, steps_scale=1.0):
                             f32c_median_abs_err=[float(np.median(np.abs(f32[k, cells][ok] - ref[k, ok])))
                                                  for k in range(4)],
                             fp64_median_spread_under_f32_rounding=[float(np.median(spread32[k, ok])) for k in range(4)],
+                            # the precision contract: 1e-3 where the oracle
+                            # holds 1e-4 under float-sized rounding, within
+                            # max(6 x its spread, 1e-3) elsewhere
+                            f32c_reproducible_cells=int((ok & (spread32.max(axis=0) <= 1e-4)).sum()),
+                            f32c_frac_within_max_6spread_1e3=[
+                                float((np.abs(f32[k, cells][ok] - ref[k, ok]) <=
+                                       np.maximum(6.0 * spread32[k, ok], 1e-3)).mean()) for k in range(4)],
                             cpu_oracle=dict(s=cpu_s, cells=int(len(cells) // 4), bounded=bounded1,
                                             traj_steps_per_s_bounded=bounded1 * spc / cpu_s, cores=1,
                                             kind="port", method="variational, 4 tangents"))
