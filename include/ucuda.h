@@ -97,6 +97,15 @@ typedef struct uc_common {
     /* optional per-cell random initial state X0 = lo + (hi-lo)*u, seed 0 = off */
     unsigned long long ic_seed;
     double ic_lo, ic_hi;
+    /* optional contiguous range of the grid's cells (cell = iy * nx + ix):
+     * cell_count > 0 runs cells [cell_begin, cell_begin + cell_count) with
+     * the same values, seeds and results they have in the whole grid, and
+     * the outputs hold those cells only (plane stride cell_count). 0 = the
+     * whole grid. Honoured by uc_run_bif, uc_run_lle, uc_run_ls and
+     * uc_run_lle_noalloc (a caller runs a grid in pieces, e.g. to check a
+     * cancel flag or report progress between them); the other entry points
+     * reject cell_count != 0. */
+    long long cell_begin, cell_count;
 } uc_common;
 
 /* Bifurcation sweep (run_bifurcation_1d / _2d, parametric_engine.cpp:1674-2091). */
@@ -420,7 +429,8 @@ int uc_server_fetch(uc_server* s, long long job_id, int* flags, double* out0, do
 int uc_server_fetch_basins(uc_server* s, long long job_id, const uc_basins_output* o);
 int uc_server_stats(uc_server* s, uc_server_stats_t* stats);
 
-/* Number of cells of a request's grid (product of axis point counts). */
+/* Number of cells a request runs: its cell range, else the whole grid
+ * (product of axis point counts). */
 long long uc_grid_cells(const uc_common* c);
 /* Axis values exactly as the kernels use them (host, n_pts doubles). */
 int uc_axis_values(const uc_axis* ax, double* out);

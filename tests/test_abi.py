@@ -183,3 +183,28 @@ def test_engine_usable_while_it_compiles(monkeypatch):
     t.start()
     t.join()
     assert other["m"] == msg
+
+
+def test_session_layer_compiles_and_validates(tmp_path):
+    """A translation unit that builds every request the way the reference's
+    analysis sessions do (krs_body from codegen_scheme, the same fields,
+    snapshot and recluster use) compiles against the facade header, links
+    against the library, and on a device-less engine gets every request
+    through validation to the device stage; the host-only recluster labels
+    synthetic features (three attractors, one spread over two touching boxes,
+    unbound cells) in canonical order."""
+    import json
+    exe = str(tmp_path / "session_fields")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "u-cuda_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "session_fields.cpp"), "-o", exe,
+                    "-L", os.path.join(ROOT, "u-cuda_amd"), "-lucuda",
+                    "-Wl,-rpath," + os.path.join(ROOT, "u-cuda_amd"), "-Wl,-rpath,/opt/rocm/lib",
+                    "-Wl,-rpath-link,/opt/rocm/lib"], check=True, capture_output=True, text=True)
+    out = subprocess.run([exe, "cpu"], capture_output=True, text=True, timeout=120, check=True)
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    for k in ("bif_error", "lle1d_error", "lle2d_error", "basins_error"):
+        assert res[k] == "engine was created without a device", (k, res[k])
+    assert "right-hand side" in res["basins_body_only_error"]
+    assert res["recluster_ok"] == 1
+    assert res["recluster_labels"] == [3, 3, 3, 1, 1, 0, 2, 2, 0]
+    assert res["recluster_n"] == 3 and res["snapshot_eps"] == 0.1

@@ -174,3 +174,46 @@ def test_facade_continuation(gpu_engine, tmp_path):
         np.testing.assert_allclose(res[key], g, atol=1e-9)
     assert res["cont_rev_flag"] == 1
     assert abs(res["cont_fwd"][3] - res["cont_rev"][5]) > 0.5  # r = -0.25 on different branches
+
+
+def test_facade_krs_body_only_dopri78(gpu_engine):
+    """A session that passes only its step body (the DOPRI78 text of
+    codegen_scheme, no right-hand side) gets that step integrated: the
+    bifurcation sweep and the reference's finite-difference LLE equal the
+    oracle stepping the same body (rho in [5, 20]: no chaos, so the fast
+    mode's FMA contraction stays at rounding level)."""
+    import oracle
+    print("case: facade krs_body-only DOPRI78 bif 1-D and LLE 1-D vs the oracle", flush=True)
+    out = subprocess.run([os.path.join(ROOT, "u-cuda_amd", "facade_demo")], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    lor = ucuda.preset("lorenz", gpu_engine)
+    body = gpu_engine.codegen_scheme(lor.vars, lor.params, lor.rhs, ucuda.SCHEME_DOPRI78)
+    step = oracle.KrsStep(body, 3, 4)
+    axes = [dict(kind=oracle.AXIS_PARAM, index=2, n_pts=16, lo=5.0, hi=20.0, log_scale=0)]
+    job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], [0.5, 10.0, 28.0, 8.0 / 3.0], 0.01, 2.0, 10.0, max_peaks=32,
+                          NT=0.1)
+    ref = oracle.bif_cells(step, job, range(16))
+    assert res["krs_bif_flags"] == ref["flags"].tolist()
+    for c in range(16):
+        n = ref["flags"][c]
+        if n > 0:
+            np.testing.assert_allclose(res["krs_bif_peaks"][c], ref["peaks"][:n, c], rtol=1e-9, atol=1e-12)
+    lle = oracle.lle_fd_cells(step, job, range(16))["values"][0]
+    np.testing.assert_allclose(res["krs_lle"], lle, rtol=0, atol=1e-8)
+    assert res["krs_snapshot_index"] == 2
+
+
+def test_facade_cancel_between_pieces(gpu_engine):
+    """A 1024 x 1024 LLE map runs in four pieces of the grid; a cancel set
+    after the first piece reported progress ends the run before the last:
+    the result says cancelled and progress stopped short of 1."""
+    print("case: facade cancel between pieces (Rossler LLE 1024^2)", flush=True)
+    out = subprocess.run([os.path.join(ROOT, "u-cuda_amd", "facade_demo"), "--cancel"], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    print(res)
+    assert res["cancelled"] == 1 and res["ok"] == 0 and res["error"] == "Cancelled by user"
+    assert 0.0 < res["seen"] < 1.0 and res["progress"] < 1.0

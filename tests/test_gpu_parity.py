@@ -598,3 +598,36 @@ def test_dft_sweep(gpu_engine):
     amp = np.hypot(r["ak"], r["bk"])[ok]
     amp3 = np.hypot(g3["ak"].cpu().numpy(), g3["bk"].cpu().numpy())[ok]
     print("F32C: max |amplitude error| / max amplitude = %.1e" % (np.abs(amp3 - amp).max() / amp.max()))
+
+
+def test_cell_ranges_equal_the_whole_grid(gpu_engine):
+    """uc_common.cell_begin / cell_count: a map run in pieces gives exactly
+    the whole run's values (strict FP64, and fast F32C through the same
+    kernels), bifurcation and LLE, for pieces that split rows and pairs."""
+    print("case: Rossler LLE 40x30 and Lorenz bif 50 rho in cell ranges vs whole", flush=True)
+    ros = ucuda.preset("rossler", gpu_engine)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 40, 0.05, 2.0), ucuda.axis(ucuda.AXIS_PARAM, 3, 30, 1.0, 20.0)]
+    for kw in (dict(strict=True), dict(precision=ucuda.PREC_F32C)):
+        args = (ros, axes, [1.0, 1.0, 1.0], ros.values(), 0.01, 2.0, 20.0, 0.1)
+        whole = ucuda.run_lyap(*args, **kw)
+        parts = [ucuda.run_lyap(*args, cells=(b, c), **kw) for b, c in ((0, 7), (7, 500), (507, 693))]
+        v = np.concatenate([p["values"].cpu().numpy()[0] for p in parts])
+        f = np.concatenate([p["flags"].cpu().numpy() for p in parts])
+        np.testing.assert_array_equal(f, whole["flags"].cpu().numpy())
+        np.testing.assert_array_equal(v, whole["values"].cpu().numpy()[0])
+    lor = ucuda.preset("lorenz", gpu_engine)
+    bax = [ucuda.axis(ucuda.AXIS_PARAM, 2, 50, 0.0, 250.0)]
+    bargs = (lor, bax, [0.1, 0.1, 0.1], lor.values(), 0.01, 10.0, 10.0)
+    whole = ucuda.run_bif(*bargs, max_peaks=16, strict=True)
+    parts = [ucuda.run_bif(*bargs, max_peaks=16, strict=True, cells=(b, c)) for b, c in ((0, 13), (13, 37))]
+    np.testing.assert_array_equal(np.concatenate([p["flags"].cpu().numpy() for p in parts]),
+                                  whole["flags"].cpu().numpy())
+    pk = np.concatenate([p["peaks"].cpu().numpy() for p in parts], axis=1)
+    wf = whole["flags"].cpu().numpy()
+    wp = whole["peaks"].cpu().numpy()
+    for c in range(50):
+        if wf[c] > 0:
+            np.testing.assert_array_equal(pk[:min(wf[c], 16), c], wp[:min(wf[c], 16), c])
+    # beyond the grid, and on an entry point that runs whole grids only
+    with pytest.raises(ucuda.UcError, match="beyond the grid"):
+        ucuda.run_lyap(ros, axes, [1.0, 1.0, 1.0], ros.values(), 0.01, 2.0, 20.0, 0.1, cells=(1000, 201))

@@ -8,6 +8,81 @@
 #include <vector>
 
 namespace ucuda {
+
+// Snapshots of the request fields a result's CSV header needs, carried in
+// every *Result (the reference's data_export snapshot structs,
+// U-CUDA/data_export.h:44-210, same member names) so a session or GUI can
+// write the files after the run.
+namespace data_export {
+struct Bif1DSnapshot {
+    std::vector<double> values, initial_conditions;
+    double tMax = 0.0, transientTime = 0.0, h = 0.0;
+    int preScaller = 0, writableVar = 0, indexOfMutVar = 0;
+    double range_lo = 0.0, range_hi = 0.0;
+};
+struct Dft1DSnapshot {
+    std::vector<double> values, initial_conditions;
+    double tMax = 0.0, transientTime = 0.0, h = 0.0;
+    int preScaller = 0, writableVar = 0, indexOfMutVar = 0;
+    double range_lo = 0.0, range_hi = 0.0;
+    int n_freq = 0;
+    double freq_lo = 0.0, freq_hi = 0.0;
+    int window_type = 1;
+};
+struct LLE1DSnapshot {
+    std::vector<double> values, initial_conditions;
+    double tMax = 0.0, NT = 0.0, transientTime = 0.0, h = 0.0, eps = 0.0;
+    int indexOfMutVar = 0;
+    double range_lo = 0.0, range_hi = 0.0;
+};
+using LS1DSnapshot = LLE1DSnapshot;
+struct Bif2DSnapshot {
+    std::vector<double> values, initial_conditions;
+    int par_or_var = 1;
+    double tMax = 0.0, transientTime = 0.0, h = 0.0;
+    int preScaller = 0;
+    double eps_dbscan = 0.0;
+    int writableVar = 0, indexOfMutVar = 0, indexOfMutVar2 = 0;
+    double range1_lo = 0.0, range1_hi = 0.0, range2_lo = 0.0, range2_hi = 0.0;
+    int n_pts = 0;
+};
+struct LLE2DSnapshot {
+    std::vector<double> values, initial_conditions;
+    int par_or_var = 1;
+    double tMax = 0.0, NT = 0.0, transientTime = 0.0, h = 0.0, eps = 0.0;
+    int indexOfMutVar = 0, indexOfMutVar2 = 0;
+    double range1_lo = 0.0, range1_hi = 0.0, range2_lo = 0.0, range2_hi = 0.0;
+    int n_pts = 0;
+};
+struct LS2DSnapshot : LLE2DSnapshot {
+    int n_exponents = 0;
+};
+struct BasinsSnapshot {
+    std::vector<double> values, initial_conditions;
+    double tMax = 0.0, transientTime = 0.0, h = 0.0;
+    int preScaller = 0;
+    double eps_dbscan = 0.0;
+    int writableVar = 0, axis_x_var = 0, axis_y_var = 0;
+    double axis_x_lo = 0.0, axis_x_hi = 0.0, axis_y_lo = 0.0, axis_y_hi = 0.0;
+    int n_pts = 0, feature1 = 0, feature2 = 0;
+    double mult1 = 0.0, mult2 = 0.0;
+};
+struct FastSyncSnapshot {
+    int mode = 0;
+    std::vector<double> values, ic_master, ic_slave, k_forward, k_backward;
+    std::vector<std::string> var_names;
+    double h = 0.0;
+    int iter_of_synchr = 0, preScaller = 0;
+    double window = 0.0;
+    int type_of_synch = 0, error_estim = 0;
+    double fs_error_trs = 0.0, tMax = 0.0, transientTime = 0.0;
+    int axis_x_var = 0, axis_y_var = 0;
+    double axis_x_lo = 0.0, axis_x_hi = 0.0, axis_y_lo = 0.0, axis_y_hi = 0.0;
+    int n_pts = 0;
+    bool grid_swap_master_slave = false;
+};
+}  // namespace data_export
+
 namespace csv {
 
 // Header block "a[n] = { ... }" / "X0[n] = { ... }".

@@ -383,10 +383,24 @@ int validate_common(uc_engine* e, const uc_system* s, const uc_common& c, bool a
     if (!(c.t_max > 0.0)) return e->fail(UC_ERR_ARGUMENT, "t_max must be > 0");
     if (c.transient_time < 0.0) return e->fail(UC_ERR_ARGUMENT, "transient_time must be >= 0");
     if (c.precision != UC_PREC_F64 && c.precision != UC_PREC_F32C) return e->fail(UC_ERR_ARGUMENT, "unknown precision");
+    if (c.cell_count < 0 || c.cell_begin < 0 || (c.cell_count == 0 && c.cell_begin != 0))
+        return e->fail(UC_ERR_ARGUMENT, "cell range: need cell_count >= 0, cell_begin >= 0 (0 when cell_count is 0)");
+    long long total = 1;
+    for (int i = 0; i < c.n_axes; ++i) total *= c.axis[i].n_pts;
+    if (c.cell_count > 0 && c.cell_begin + c.cell_count > total)
+        return e->fail(UC_ERR_ARGUMENT, "cell range beyond the grid");
     return UC_OK;
 }
 
+// For entry points that run whole grids only.
+int whole_grid_only(uc_engine* e, const uc_common& c, const char* what) {
+    if (c.cell_count != 0) return e->fail(UC_ERR_ARGUMENT, std::string(what) + " runs whole grids (cell_count must be 0)");
+    return UC_OK;
+}
+
+// Cells a request runs: its cell range, else the whole grid.
 long long cells_of(const uc_common& c) {
+    if (c.cell_count > 0) return c.cell_count;
     long long n = 1;
     for (int i = 0; i < c.n_axes; ++i) n *= c.axis[i].n_pts;
     return n;
@@ -424,8 +438,8 @@ float float_at_most(double v) {
 void wire_job(const uc_common& c, const double* dev, UcJob& j) {
     const int nx = c.axis[0].n_pts, ny = c.n_axes > 1 ? c.axis[1].n_pts : 1;
     std::memset(&j, 0, sizeof j);
-    j.n_cells = (long long)nx * ny;
-    j.cell_offset = 0;
+    j.n_cells = c.cell_count > 0 ? c.cell_count : (long long)nx * ny;
+    j.cell_offset = c.cell_count > 0 ? c.cell_begin : 0;
     j.stride = j.n_cells;
     j.nx = nx;
     j.ny = ny;
@@ -886,6 +900,7 @@ int prep_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, uc_commo
     if (c.t_max <= 0.0) c.t_max = 1.0;  // unused by basins
     int rc = validate_common(e, s, c, false);
     if (rc) return rc;
+    if ((rc = whole_grid_only(e, c, "basins"))) return rc;
     lbl = s->n_vars < 2 ? s->n_vars : 2;
     if (!(r->rtol > 0.0) || !(r->atol > 0.0)) return e->fail(UC_ERR_ARGUMENT, "rtol and atol must be > 0");
     if (!(r->section_period > 0.0)) return e->fail(UC_ERR_ARGUMENT, "section_period must be > 0");
@@ -1123,6 +1138,7 @@ int uc_run_bif_periods(uc_engine* e, uc_system* s, const uc_bif_request* r, doub
     long long iters = 0;
     int rc = prep_bif(e, s, &rq, &probe, &iters);
     if (rc) return rc;
+    if ((rc = whole_grid_only(e, r->c, "uc_run_bif_periods"))) return rc;
     if ((rc = enter(e))) return rc;
     Module* m = nullptr;
     if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
@@ -1424,6 +1440,7 @@ int uc_run_dft(uc_engine* e, uc_system* s, const uc_dft_request* r, const uc_dft
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     int rc = validate_common(e, s, r->c, true);
     if (rc) return rc;
+    if ((rc = whole_grid_only(e, r->c, "uc_run_dft"))) return rc;
     if (!o->ak_cos || !o->bk_sin || !o->flags) return e->fail(UC_ERR_ARGUMENT, "DFT outputs missing");
     if (r->writable_var < -1 || r->writable_var >= s->n_vars) return e->fail(UC_ERR_ARGUMENT, "writable_var out of range");
     if (r->pre_scaller <= 0) return e->fail(UC_ERR_ARGUMENT, "pre_scaller must be > 0");
@@ -1562,6 +1579,7 @@ struct ContRun {
 static int prep_cont(uc_engine* e, uc_system* s, const uc_common& c, int reverse, hipStream_t st, ContRun& R) {
     int rc = validate_common(e, s, c, true);
     if (rc) return rc;
+    if ((rc = whole_grid_only(e, c, "a continuation sweep"))) return rc;
     const uc_axis& a0 = c.axis[0];
     if (a0.kind != UC_AXIS_PARAM && a0.kind != UC_AXIS_STEP)
         return e->fail(UC_ERR_ARGUMENT, "continuation sweeps a parameter or h (axis 0), not an initial state");
@@ -2153,6 +2171,7 @@ int uc_batch_add_bif(uc_batch* b, uc_system* s, const uc_bif_request* r, const u
     long long iters = 0;
     int rc = prep_bif(e, s, r, o, &iters);
     if (rc) return rc;
+    if ((rc = whole_grid_only(e, r->c, "a batch job"))) return rc;
     if ((rc = enter(e))) return rc;
     Module* m = nullptr;
     if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
@@ -2175,6 +2194,7 @@ int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int s
     std::lock_guard<std::recursive_mutex> lk(e->mu);
     int rc = prep_lyap(e, s, r, o);
     if (rc) return rc;
+    if ((rc = whole_grid_only(e, r->c, "a batch job"))) return rc;
     if ((rc = enter(e))) return rc;
     Module* m = nullptr;
     if ((rc = get_module(e, s, r->c.strict != 0, true, &m))) return rc;
@@ -2821,6 +2841,7 @@ int uc_server_submit_lyap(uc_server* S, uc_system* s, const uc_lyap_request* r, 
         uc_lyap_output probe{&dummy, &dflag, nullptr};
         int rc = prep_lyap(e, s, r, &probe);
         if (rc) return rc;
+        if ((rc = whole_grid_only(e, r->c, "a served job"))) return rc;
         if ((rc = enter(e))) return rc;
         if ((rc = get_module(e, s, r->c.strict != 0, true, &j->m))) return rc;
         ++j->m->pins;
@@ -2852,6 +2873,7 @@ int uc_server_submit_bif(uc_server* S, uc_system* s, const uc_bif_request* r, lo
         uc_bif_output probe{&dflag, &dummy, &dummy, nullptr, 0, nullptr};
         int rc = prep_bif(e, s, r, &probe, &iters);
         if (rc) return rc;
+        if ((rc = whole_grid_only(e, r->c, "a served job"))) return rc;
         if ((rc = enter(e))) return rc;
         if ((rc = get_module(e, s, r->c.strict != 0, true, &j->m))) return rc;
         ++j->m->pins;

@@ -3,15 +3,61 @@
 // and prints a JSON summary. Used by the GPU tests to check that the facade
 // and the C ABI give the same numbers. With a directory argument every run
 // also writes the reference's CSV files there (csv_output_path).
+#include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <string>
+#include <thread>
 
 #include "parametric_engine.hpp"
 
 using namespace ucuda;
 
+namespace {
+// A 1024 x 1024 Rossler LLE map runs in four pieces; a watcher sets the
+// cancel token once the first piece has reported progress. The run must
+// come back cancelled, with progress short of 1.
+int cancel_demo() {
+    ParametricEngine eng(0);
+    System rossler{{"x", "y", "z"}, {"a", "b", "c"}, {"-y-z", "x+a*y", "b+z*(x-c)"}, false, ""};
+    LLE2DRequest l;
+    l.system = rossler;
+    l.amountOfX = 3;
+    l.initial_conditions = {1.0, 1.0, 1.0};
+    l.base_values = {0.5, 0.2, 0.2, 5.7};
+    l.param_index = 2;
+    l.param_lo = 0.05;
+    l.param_hi = 2.0;
+    l.param_index_2 = 3;
+    l.param_lo_2 = 1.0;
+    l.param_hi_2 = 20.0;
+    l.n_pts = 1024;
+    l.h = 0.01;
+    l.transient_time = 5.0;
+    l.t_max = 300.0;
+    l.NT = 0.1;
+    l.cancel = std::make_shared<std::atomic<bool>>(false);
+    l.progress = std::make_shared<std::atomic<float>>(0.0f);
+    float seen = 0.0f;
+    std::thread watcher([&] {
+        const auto t0 = std::chrono::steady_clock::now();
+        while (std::chrono::steady_clock::now() - t0 < std::chrono::seconds(60)) {
+            seen = l.progress->load();
+            if (seen > 0.0f) { l.cancel->store(true); return; }
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+        }
+    });
+    LLE2DResult r = eng.run_lle_2d(l);
+    watcher.join();
+    std::printf("{\"cancelled\": %d, \"ok\": %d, \"progress\": %.6f, \"seen\": %.6f, \"error\": \"%s\"}\n",
+                r.cancelled ? 1 : 0, r.ok ? 1 : 0, l.progress->load(), seen, r.error.c_str());
+    return 0;
+}
+}  // namespace
+
 int main(int argc, char** argv) {
+    if (argc > 1 && std::strcmp(argv[1], "--cancel") == 0) return cancel_demo();
     const std::string dir = argc > 1 ? std::string(argv[1]) + "/" : std::string();
     auto csv = [&](const char* name) { return dir.empty() ? std::string() : dir + name; };
     ParametricEngine eng(0);
@@ -163,7 +209,51 @@ int main(int argc, char** argv) {
     LLE1DResult cr = eng.run_lle_1d(cq);
     if (!cf.ok || !cr.ok) { std::printf("{\"error\": \"%s\"}\n", (cf.ok ? cr : cf).error.c_str()); return 1; }
 
-    std::printf("{\"bif_flags\": [");
+    // the session's step body alone (DOPRI78 text, no right-hand side): the
+    // body itself is stepped, Lyapunov by the reference's finite differences
+    Bifurcation1DRequest kb;
+    kb.krs_body = codegen_scheme(lorenz, Scheme::DOPRI78);
+    kb.amountOfX = 3;
+    kb.initial_conditions = {0.1, 0.1, 0.1};
+    kb.base_values = {0.5, 10.0, 28.0, 8.0 / 3.0};
+    kb.param_index = 2;
+    kb.param_lo = 5.0;
+    kb.param_hi = 20.0;
+    kb.n_pts = 16;
+    kb.h = 0.01;
+    kb.transient_time = 2.0;
+    kb.t_max = 10.0;
+    kb.max_peaks = 32;
+    Bifurcation1DResult kbr = eng.run_bifurcation_1d(kb);
+    LLE1DRequest kl;
+    kl.krs_body = kb.krs_body;
+    kl.amountOfX = 3;
+    kl.initial_conditions = {0.1, 0.1, 0.1};
+    kl.base_values = kb.base_values;
+    kl.param_index = 2;
+    kl.param_lo = 5.0;
+    kl.param_hi = 20.0;
+    kl.n_pts = 16;
+    kl.h = 0.01;
+    kl.transient_time = 2.0;
+    kl.t_max = 10.0;
+    kl.NT = 0.1;
+    LLE1DResult klr = eng.run_lle_1d(kl);
+    if (!kbr.ok || !klr.ok) { std::printf("{\"error\": \"%s\"}\n", (kbr.ok ? klr.error : kbr.error).c_str()); return 1; }
+
+    std::printf("{\"krs_bif_flags\": [");
+    for (size_t i = 0; i < kbr.flags.size(); ++i) std::printf("%s%d", i ? ", " : "", kbr.flags[i]);
+    std::printf("], \"krs_bif_peaks\": [");
+    for (size_t i = 0; i < kbr.flags.size(); ++i) {
+        std::printf("%s[", i ? ", " : "");
+        for (size_t k = 0; k < kbr.bifurcation_points[i].size(); ++k)
+            std::printf("%s%.17g", k ? ", " : "", kbr.bifurcation_points[i][k]);
+        std::printf("]");
+    }
+    std::printf("], \"krs_lle\": [");
+    for (size_t i = 0; i < klr.lyapunov.size(); ++i) std::printf("%s%.17g", i ? ", " : "", klr.lyapunov[i]);
+    std::printf("], \"krs_snapshot_index\": %d, ", kbr.snapshot.indexOfMutVar);
+    std::printf("\"bif_flags\": [");
     for (size_t i = 0; i < br.flags.size(); ++i) std::printf("%s%d", i ? ", " : "", br.flags[i]);
     std::printf("], \"bif_first_peaks\": [");
     for (size_t i = 0; i < br.flags.size(); ++i)

@@ -3,32 +3,51 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <fstream>
+#include <functional>
 #include <limits>
 #include <map>
 #include <mutex>
+#include <numeric>
+#include <tuple>
 
 #include "../../include/ucuda.h"
 #include "data_export.hpp"
 
-#include <fstream>
-
 namespace ucuda {
 namespace {
 
+// Device output buffer. On an engine without a device (compile-only) the
+// buffer is a placeholder address that the C ABI never dereferences: it
+// validates the request and reports UC_ERR_NO_DEVICE before any device work.
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
-    bool alloc(size_t count) {
+    bool owned = false;
+    bool alloc(size_t count, bool device) {
         n = count ? count : 1;
-        return hipMalloc((void**)&p, n * sizeof(T)) == hipSuccess;
+        if (!device) {
+            p = reinterpret_cast<T*>(static_cast<uintptr_t>(256));
+            return true;
+        }
+        owned = hipMalloc((void**)&p, n * sizeof(T)) == hipSuccess;
+        return owned;
     }
     bool download(std::vector<T>& out, size_t count) const {
         out.resize(count);
         return count == 0 || hipMemcpy(out.data(), p, count * sizeof(T), hipMemcpyDeviceToHost) == hipSuccess;
     }
-    ~DevBuf() { if (p) (void)hipFree(p); }
+    // `planes` planes of `count` values (plane stride `count`) into
+    // host[k * total + begin ...]: one piece of a plane-major grid output
+    bool download_piece(T* host, size_t total, size_t begin, size_t count, size_t planes) const {
+        return hipMemcpy2D(host + begin, total * sizeof(T), p, count * sizeof(T), count * sizeof(T), planes,
+                           hipMemcpyDeviceToHost) == hipSuccess;
+    }
+    ~DevBuf() { if (owned) (void)hipFree(p); }
 };
 
 std::string key_of(const System& s) {
@@ -53,21 +72,57 @@ uc_axis make_axis(bool over_h, bool over_var, int var_index, int param_index, in
 }
 
 bool cancelled(const std::shared_ptr<std::atomic<bool>>& c) { return c && c->load(std::memory_order_relaxed); }
-void done(const std::shared_ptr<std::atomic<float>>& p) { if (p) p->store(1.0f, std::memory_order_relaxed); }
+void set_progress(const std::shared_ptr<std::atomic<float>>& p, float v) {
+    if (p) p->store(v, std::memory_order_relaxed);
+}
+template <class R>
+R cancelled_result() {
+    R r;
+    r.cancelled = true;
+    r.error = "Cancelled by user";
+    return r;
+}
+const char* kNoCpu = "CPU sweeps are not part of this engine (use_cpu); the GPU path runs the same algorithm";
+
+// Pieces of a grid run: at least 2^18 cells (a full GPU at the map
+// kernels' occupancy), at most 16.
+constexpr long long kMinPiece = 1LL << 18;
+long long piece_size(long long total) { return std::max(kMinPiece, (total + 15) / 16); }
+
+// Runs `total` cells piece by piece: `run(begin, count)` launches one piece
+// and downloads its outputs (false on error). The cancel token is checked
+// and progress advanced between pieces. 0 done, 1 cancelled, -1 error.
+template <class F>
+int run_pieces(long long total, const std::shared_ptr<std::atomic<bool>>& cancel,
+               const std::shared_ptr<std::atomic<float>>& progress, F run) {
+    const long long piece = piece_size(total);
+    for (long long b = 0; b < total; b += piece) {
+        if (b > 0 && cancelled(cancel)) return 1;
+        const long long count = std::min(piece, total - b);
+        if (!run(b, count)) return -1;
+        set_progress(progress, (float)((double)(b + count) / (double)total));
+    }
+    return 0;
+}
+
+int par_or_var(bool v1, bool v2) { return (v1 == v2) ? (v1 ? 0 : 1) : 2; }
 
 }  // namespace
 
 struct ParametricEngine::Impl {
     uc_engine* eng = nullptr;
+    int device = -1;
     std::mutex mu;
     std::map<std::string, uc_system*> systems;
+    std::map<std::string, std::string> rk4_text;  // system key -> codegen's RK4 step body
 
     ~Impl() {
         for (auto& kv : systems) uc_system_destroy(kv.second);
         if (eng) uc_engine_destroy(eng);
     }
     std::string err() const { return eng ? uc_last_error(eng) : "no engine"; }
-    uc_system* system(const System& s, std::string& error) {
+
+    uc_system* rhs_system(const System& s, std::string& error) {
         const std::string k = key_of(s);
         auto it = systems.find(k);
         if (it != systems.end()) return it->second;
@@ -82,6 +137,45 @@ struct ParametricEngine::Impl {
         systems[k] = out;
         return out;
     }
+    uc_system* krs_system(const std::string& body, int n_vars, int n_values, std::string& error) {
+        const std::string k = "KRS|" + std::to_string(n_vars) + "|" + std::to_string(n_values) + "|" + body;
+        auto it = systems.find(k);
+        if (it != systems.end()) return it->second;
+        uc_system* out = nullptr;
+        if (uc_system_create_krs(eng, body.c_str(), n_vars, n_values, &out) != UC_OK) { error = err(); return nullptr; }
+        systems[k] = out;
+        return out;
+    }
+    const std::string& rk4_of(const System& s) {
+        const std::string k = key_of(s);
+        auto it = rk4_text.find(k);
+        if (it != rk4_text.end()) return it->second;
+        std::string text;
+        try {
+            text = codegen_scheme(s, Scheme::RK4);
+        } catch (...) {
+        }
+        return rk4_text[k] = text;
+    }
+    // The system a request runs: its right-hand side when `system` is set and
+    // the step body is empty or codegen's RK4 text of it (or the analysis
+    // needs the right-hand side), else the step body itself (`stepped`).
+    uc_system* resolve(const System& sys, const std::string& krs, int n_vars, int n_values, bool need_rhs,
+                       std::string& error, bool* stepped = nullptr) {
+        if (stepped) *stepped = false;
+        if (!sys.vars.empty()) {
+            if (need_rhs || krs.empty() || krs == rk4_of(sys)) return rhs_system(sys, error);
+        } else if (need_rhs) {
+            error = "this analysis integrates the right-hand side (adaptive Dormand-Prince): set `system`";
+            return nullptr;
+        }
+        if (krs.empty()) {
+            error = "neither a step body (krs_body) nor a system (vars / params / rhs) was given";
+            return nullptr;
+        }
+        if (stepped) *stepped = true;
+        return krs_system(krs, n_vars, n_values, error);
+    }
     void common(uc_common& c, const std::vector<double>& ic, const std::vector<double>& vals, double h,
                 double transient, double t_max, double max_value, int precision) {
         c.initial_conditions = ic.data();
@@ -94,21 +188,33 @@ struct ParametricEngine::Impl {
         c.precision = precision;
         c.strict = 0;
         c.ic_seed = 0;
+        c.cell_begin = 0;
+        c.cell_count = 0;
     }
 };
 
 ParametricEngine::ParametricEngine(int device) : impl_(new Impl) {
+    impl_->device = device;
     if (uc_engine_create(device, &impl_->eng) != UC_OK) impl_->eng = nullptr;
 }
 ParametricEngine::~ParametricEngine() = default;
+
+namespace {
+template <class Req>
+int vars_of(const Req& req) {
+    return req.amountOfX > 0 ? req.amountOfX : (int)req.system.vars.size();
+}
+}  // namespace
 
 Bifurcation1DResult ParametricEngine::run_bifurcation_1d(const Bifurcation1DRequest& req) {
     Bifurcation1DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (req.use_cpu) { res.error = "CPU sweeps are not part of this engine (use_cpu); the GPU path runs the same algorithm"; return res; }
-    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
-    uc_system* s = impl_->system(req.system, res.error);
+    const bool dev = impl_->device >= 0;
+    if (req.use_cpu) { res.error = kNoCpu; return res; }
+    if (cancelled(req.cancel)) return cancelled_result<Bifurcation1DResult>();
+    uc_system* s = impl_->resolve(req.system, req.krs_body, vars_of(req), (int)req.base_values.size(), false,
+                                  res.error);
     if (!s) return res;
     uc_bif_request r{};
     impl_->common(r.c, req.initial_conditions, req.base_values, req.h, req.transient_time, req.t_max, req.max_value,
@@ -122,19 +228,43 @@ Bifurcation1DResult ParametricEngine::run_bifurcation_1d(const Bifurcation1DRequ
     const double hw = req.sweep_over_h ? std::min(req.param_lo, req.param_hi) : req.h;
     // continuation keeps every peak of the block, as the reference does
     if (req.continuation) r.max_peaks = (int)std::ceil(req.t_max / hw / req.pre_scaller) / 2 + 2;
-    const size_t n = (size_t)req.n_pts, mp = (size_t)std::max(1, r.max_peaks);
-    DevBuf<int> flags;
-    DevBuf<double> peaks, ivals;
-    if (!flags.alloc(n) || !peaks.alloc(mp * n) || !ivals.alloc(mp * n)) { res.error = "hipMalloc failed"; return res; }
-    uc_bif_output o{flags.p, peaks.p, ivals.p, nullptr, 0, nullptr};
-    const int rc = req.continuation
-                       ? uc_run_bif_cont(impl_->eng, s, &r, req.continuation_reverse ? 1 : 0, &o, nullptr)
-                       : uc_run_bif(impl_->eng, s, &r, &o, nullptr);
-    if (rc != UC_OK) { res.error = impl_->err(); return res; }
-    std::vector<int> f;
-    std::vector<double> pk, iv;
-    if (hipDeviceSynchronize() != hipSuccess || !flags.download(f, n) || !peaks.download(pk, mp * n) ||
-        !ivals.download(iv, mp * n)) { res.error = "device error"; return res; }
+    const size_t n = (size_t)std::max(0, req.n_pts), mp = (size_t)std::max(1, r.max_peaks);
+    std::vector<int> f(n);
+    std::vector<double> pk(mp * n), iv(mp * n);
+    if (req.continuation) {
+        DevBuf<int> flags;
+        DevBuf<double> peaks, ivals;
+        if (!flags.alloc(n, dev) || !peaks.alloc(mp * n, dev) || !ivals.alloc(mp * n, dev)) { res.error = "hipMalloc failed"; return res; }
+        uc_bif_output o{flags.p, peaks.p, ivals.p, nullptr, 0, nullptr};
+        if (uc_run_bif_cont(impl_->eng, s, &r, req.continuation_reverse ? 1 : 0, &o, nullptr) != UC_OK) {
+            res.error = impl_->err();
+            return res;
+        }
+        if (hipDeviceSynchronize() != hipSuccess || !flags.download(f, n) || !peaks.download(pk, mp * n) ||
+            !ivals.download(iv, mp * n)) { res.error = "device error"; return res; }
+    } else {
+        const size_t piece = (size_t)std::min<long long>(piece_size((long long)n), (long long)n);
+        DevBuf<int> flags;
+        DevBuf<double> peaks, ivals;
+        if (!flags.alloc(piece, dev) || !peaks.alloc(mp * piece, dev) || !ivals.alloc(mp * piece, dev)) {
+            res.error = "hipMalloc failed";
+            return res;
+        }
+        const int st = run_pieces((long long)n, req.cancel, req.progress, [&](long long b, long long cnt) {
+            r.c.cell_begin = b;
+            r.c.cell_count = cnt;
+            uc_bif_output o{flags.p, peaks.p, ivals.p, nullptr, 0, nullptr};
+            if (uc_run_bif(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return false; }
+            if (hipDeviceSynchronize() != hipSuccess || !flags.download_piece(f.data(), n, b, cnt, 1) ||
+                !peaks.download_piece(pk.data(), n, b, cnt, mp) || !ivals.download_piece(iv.data(), n, b, cnt, mp)) {
+                res.error = "device error";
+                return false;
+            }
+            return true;
+        });
+        if (st == 1) return cancelled_result<Bifurcation1DResult>();
+        if (st < 0) return res;
+    }
     res.param_lo = req.param_lo;
     res.param_hi = req.param_hi;
     res.n_pts = req.n_pts;
@@ -167,32 +297,97 @@ Bifurcation1DResult ParametricEngine::run_bifurcation_1d(const Bifurcation1DRequ
             res.peak_times[c].push_back(v);
         }
     }
+    auto& sn = res.snapshot;
+    sn.values = req.base_values;
+    sn.initial_conditions = req.initial_conditions;
+    sn.tMax = req.t_max;
+    sn.transientTime = req.transient_time;
+    sn.h = req.h;
+    sn.preScaller = req.pre_scaller;
+    sn.writableVar = req.writable_var;
+    sn.indexOfMutVar = req.sweep_over_var ? req.var_sweep_index : req.param_index;
+    sn.range_lo = req.param_lo;
+    sn.range_hi = req.param_hi;
     if (!req.csv_output_path.empty() && !req.continuation) {  // the reference writes no file for continuation
         std::ofstream cfg(req.csv_output_path + "_config.csv");
         csv::bif1d_config(cfg, req.base_values, req.initial_conditions, req.t_max, req.transient_time, req.h,
-                          req.pre_scaller, req.writable_var, req.sweep_over_var ? req.var_sweep_index : req.param_index,
-                          req.param_lo, req.param_hi);
+                          req.pre_scaller, req.writable_var, sn.indexOfMutVar, req.param_lo, req.param_hi);
         std::ofstream out(req.csv_output_path);
         for (size_t c = 0; c < n; ++c)
             csv::bif1d_rows(out, csv::param_value_at(c, req.n_pts, req.param_lo, req.param_hi), f[c],
                             res.bifurcation_points[c].data(), res.peak_times[c].data());
     }
-    done(req.progress);
+    set_progress(req.progress, 1.0f);
     res.ok = true;
     return res;
 }
 
 namespace {
 template <class I, class Req>
-int lyap_common(I* impl, const Req& req, uc_lyap_request& r, int n_axes) {
+void lyap_common(I* impl, const Req& req, uc_lyap_request& r, int n_axes, bool stepped) {
     impl->common(r.c, req.initial_conditions, req.base_values, req.h, req.transient_time, req.t_max, req.max_value,
                  req.precision);
     r.c.n_axes = n_axes;
     r.NT = req.NT;
     r.eps = req.eps;
-    r.method = req.method;
+    // a stepped body has no Jacobian: the reference's finite-difference method
+    r.method = stepped ? UC_METHOD_FINITE_DIFF : req.method;
     r.probe_seed = 1234567891ULL;
-    return 0;
+}
+
+template <class Req, class Snap>
+void lle1d_snapshot(const Req& req, Snap& sn) {
+    sn.values = req.base_values;
+    sn.initial_conditions = req.initial_conditions;
+    sn.tMax = req.t_max;
+    sn.NT = req.NT;
+    sn.transientTime = req.transient_time;
+    sn.h = req.h;
+    sn.eps = req.eps;
+    sn.indexOfMutVar = req.sweep_over_var ? req.var_sweep_index : req.param_index;
+    sn.range_lo = req.param_lo;
+    sn.range_hi = req.param_hi;
+}
+
+// Runs an LLE / LS map (1-D or 2-D, `n` cells, K exponents) into host
+// vectors: piece by piece (cancel + progress), or as one chain.
+template <class Req>
+int run_lyap_map(uc_engine* eng, bool dev, uc_system* s, uc_lyap_request& r, const Req& req, bool spectrum,
+                 bool chain, bool reverse, size_t n, size_t K, std::vector<double>& vals, std::vector<int>& flags,
+                 std::string& error) {
+    vals.assign(n * K, 0.0);
+    flags.assign(n, 0);
+    if (chain) {
+        DevBuf<double> v;
+        DevBuf<int> f;
+        if (!v.alloc(n * K, dev) || !f.alloc(n, dev)) { error = "hipMalloc failed"; return -1; }
+        uc_lyap_output o{v.p, f.p, nullptr};
+        const int rc = spectrum ? uc_run_ls_cont(eng, s, &r, reverse ? 1 : 0, &o, nullptr)
+                                : uc_run_lle_cont(eng, s, &r, reverse ? 1 : 0, &o, nullptr);
+        if (rc != UC_OK) { error = uc_last_error(eng); return -1; }
+        if (hipDeviceSynchronize() != hipSuccess || !v.download(vals, n * K) || !f.download(flags, n)) {
+            error = "device error";
+            return -1;
+        }
+        return 0;
+    }
+    const size_t piece = (size_t)std::min<long long>(piece_size((long long)n), (long long)n);
+    DevBuf<double> v;
+    DevBuf<int> f;
+    if (!v.alloc(piece * K, dev) || !f.alloc(piece, dev)) { error = "hipMalloc failed"; return -1; }
+    return run_pieces((long long)n, req.cancel, req.progress, [&](long long b, long long cnt) {
+        r.c.cell_begin = b;
+        r.c.cell_count = cnt;
+        uc_lyap_output o{v.p, f.p, nullptr};
+        const int rc = spectrum ? uc_run_ls(eng, s, &r, &o, nullptr) : uc_run_lle(eng, s, &r, &o, nullptr);
+        if (rc != UC_OK) { error = uc_last_error(eng); return false; }
+        if (hipDeviceSynchronize() != hipSuccess || !v.download_piece(vals.data(), n, b, cnt, K) ||
+            !f.download_piece(flags.data(), n, b, cnt, 1)) {
+            error = "device error";
+            return false;
+        }
+        return true;
+    });
 }
 }  // namespace
 
@@ -200,28 +395,23 @@ LLE1DResult ParametricEngine::run_lle_1d(const LLE1DRequest& req) {
     LLE1DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (req.use_cpu) { res.error = "CPU sweeps are not part of this engine (use_cpu); the GPU path runs the same algorithm"; return res; }
-    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
-    uc_system* s = impl_->system(req.system, res.error);
+    const bool dev = impl_->device >= 0;
+    if (req.use_cpu) { res.error = kNoCpu; return res; }
+    if (cancelled(req.cancel)) return cancelled_result<LLE1DResult>();
+    bool stepped = false;
+    uc_system* s = impl_->resolve(req.system, req.krs_body, vars_of(req), (int)req.base_values.size(), false,
+                                  res.error, &stepped);
     if (!s) return res;
     uc_lyap_request r{};
-    lyap_common(impl_.get(), req, r, 1);
+    lyap_common(impl_.get(), req, r, 1, stepped);
     r.c.axis[0] = make_axis(req.sweep_over_h, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts,
                             req.param_lo, req.param_hi, req.log_scale);
-    const size_t n = (size_t)req.n_pts;
-    DevBuf<double> vals;
-    DevBuf<int> flags;
-    if (!vals.alloc(n) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
-    uc_lyap_output o{vals.p, flags.p, nullptr};
-    const int rc = req.continuation
-                       ? uc_run_lle_cont(impl_->eng, s, &r, req.continuation_reverse ? 1 : 0, &o, nullptr)
-                       : uc_run_lle(impl_->eng, s, &r, &o, nullptr);
-    if (rc != UC_OK) { res.error = impl_->err(); return res; }
+    const size_t n = (size_t)std::max(0, req.n_pts);
     std::vector<double> v;
-    if (hipDeviceSynchronize() != hipSuccess || !vals.download(v, n) || !flags.download(res.flags, n)) {
-        res.error = "device error";
-        return res;
-    }
+    const int st = run_lyap_map(impl_->eng, dev, s, r, req, false, req.continuation, req.continuation_reverse, n, 1, v, res.flags,
+                                 res.error);
+    if (st == 1) return cancelled_result<LLE1DResult>();
+    if (st < 0) return res;
     res.n_pts = req.n_pts;
     res.continuation_reverse = req.continuation && req.continuation_reverse;
     res.param_lo = req.param_lo;
@@ -229,6 +419,7 @@ LLE1DResult ParametricEngine::run_lle_1d(const LLE1DRequest& req) {
     res.lyapunov.resize(n);
     for (size_t i = 0; i < n; ++i)
         res.lyapunov[i] = (res.flags[i] == 1) ? v[i] : std::numeric_limits<double>::quiet_NaN();
+    lle1d_snapshot(req, res.snapshot);
     if (!req.csv_output_path.empty() && !req.continuation) {  // the reference writes no file for continuation
         std::ofstream cfg(req.csv_output_path + "_config.csv");
         csv::curve1d_config(cfg, "1D LLE", req.base_values, req.initial_conditions, req.t_max, req.NT,
@@ -237,7 +428,7 @@ LLE1DResult ParametricEngine::run_lle_1d(const LLE1DRequest& req) {
         for (size_t i = 0; i < n; ++i)
             csv::lle1d_row(out, csv::param_value_at(i, req.n_pts, req.param_lo, req.param_hi), res.lyapunov[i]);
     }
-    done(req.progress);
+    set_progress(req.progress, 1.0f);
     res.ok = true;
     return res;
 }
@@ -246,28 +437,23 @@ LS1DResult ParametricEngine::run_ls_1d(const LS1DRequest& req) {
     LS1DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (req.use_cpu) { res.error = "CPU sweeps are not part of this engine (use_cpu); the GPU path runs the same algorithm"; return res; }
-    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
-    uc_system* s = impl_->system(req.system, res.error);
+    const bool dev = impl_->device >= 0;
+    if (req.use_cpu) { res.error = kNoCpu; return res; }
+    if (cancelled(req.cancel)) return cancelled_result<LS1DResult>();
+    bool stepped = false;
+    uc_system* s = impl_->resolve(req.system, req.krs_body, vars_of(req), (int)req.base_values.size(), false,
+                                  res.error, &stepped);
     if (!s) return res;
     uc_lyap_request r{};
-    lyap_common(impl_.get(), req, r, 1);
+    lyap_common(impl_.get(), req, r, 1, stepped);
     r.c.axis[0] = make_axis(req.sweep_over_h, req.sweep_over_var, req.var_sweep_index, req.param_index, req.n_pts,
                             req.param_lo, req.param_hi, req.log_scale);
-    const size_t n = (size_t)req.n_pts, N = req.system.vars.size();
-    DevBuf<double> vals;
-    DevBuf<int> flags;
-    if (!vals.alloc(n * N) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
-    uc_lyap_output o{vals.p, flags.p, nullptr};
-    const int rc = req.continuation
-                       ? uc_run_ls_cont(impl_->eng, s, &r, req.continuation_reverse ? 1 : 0, &o, nullptr)
-                       : uc_run_ls(impl_->eng, s, &r, &o, nullptr);
-    if (rc != UC_OK) { res.error = impl_->err(); return res; }
+    const size_t n = (size_t)std::max(0, req.n_pts), N = (size_t)uc_system_n_vars(s);
     std::vector<double> v;
-    if (hipDeviceSynchronize() != hipSuccess || !vals.download(v, n * N) || !flags.download(res.flags, n)) {
-        res.error = "device error";
-        return res;
-    }
+    const int st = run_lyap_map(impl_->eng, dev, s, r, req, true, req.continuation, req.continuation_reverse, n, N, v, res.flags,
+                                 res.error);
+    if (st == 1) return cancelled_result<LS1DResult>();
+    if (st < 0) return res;
     res.n_pts = req.n_pts;
     res.continuation_reverse = req.continuation && req.continuation_reverse;
     res.n_exponents = (int)N;
@@ -276,6 +462,7 @@ LS1DResult ParametricEngine::run_ls_1d(const LS1DRequest& req) {
     res.spectrum.assign(n, std::vector<double>(N));
     for (size_t i = 0; i < n; ++i)
         for (size_t k = 0; k < N; ++k) res.spectrum[i][k] = v[k * n + i];
+    lle1d_snapshot(req, res.snapshot);
     if (!req.csv_output_path.empty() && !req.continuation) {  // the reference writes no file for continuation
         std::ofstream cfg(req.csv_output_path + "_config.csv");
         csv::curve1d_config(cfg, "1D LS", req.base_values, req.initial_conditions, req.t_max, req.NT,
@@ -285,7 +472,7 @@ LS1DResult ParametricEngine::run_ls_1d(const LS1DRequest& req) {
             csv::ls1d_row(out, csv::param_value_at(i, req.n_pts, req.param_lo, req.param_hi), res.spectrum[i].data(),
                           (int)N);
     }
-    done(req.progress);
+    set_progress(req.progress, 1.0f);
     res.ok = true;
     return res;
 }
@@ -309,15 +496,35 @@ void min_max(const double* v, const int* f, size_t n, double& lo, double& hi) {
         hi = std::max(hi, v[i]);
     }
 }
+template <class Req, class Snap>
+void map2d_snapshot(const Req& req, Snap& sn) {
+    sn.values = req.base_values;
+    sn.initial_conditions = req.initial_conditions;
+    sn.par_or_var = par_or_var(req.sweep_over_var, req.sweep_over_var_2);
+    sn.tMax = req.t_max;
+    sn.NT = req.NT;
+    sn.transientTime = req.transient_time;
+    sn.h = req.h;
+    sn.eps = req.eps;
+    sn.indexOfMutVar = req.sweep_over_var ? req.var_sweep_index : req.param_index;
+    sn.indexOfMutVar2 = req.sweep_over_var_2 ? req.var_sweep_index_2 : req.param_index_2;
+    sn.range1_lo = req.param_lo;
+    sn.range1_hi = req.param_hi;
+    sn.range2_lo = req.param_lo_2;
+    sn.range2_hi = req.param_hi_2;
+    sn.n_pts = req.n_pts;
+}
 }  // namespace
 
 Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
     Dft1DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (req.use_cpu) { res.error = "CPU sweeps are not part of this engine (use_cpu); the GPU path runs the same algorithm"; return res; }
-    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
-    uc_system* s = impl_->system(req.system, res.error);
+    const bool dev = impl_->device >= 0;
+    if (req.use_cpu) { res.error = kNoCpu; return res; }
+    if (cancelled(req.cancel)) return cancelled_result<Dft1DResult>();
+    uc_system* s = impl_->resolve(req.system, req.krs_body, vars_of(req), (int)req.base_values.size(), false,
+                                  res.error);
     if (!s) return res;
     uc_dft_request r{};
     impl_->common(r.c, req.initial_conditions, req.base_values, req.h, req.transient_time, req.t_max, req.max_value,
@@ -332,10 +539,10 @@ Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
     r.freq_hi = req.freq_hi;
     r.freq_log_scale = req.freq_log_scale ? 1 : 0;
     r.window_type = req.window_type;
-    const size_t n = (size_t)req.n_pts, nf = (size_t)std::max(1, req.n_freq);
+    const size_t n = (size_t)std::max(0, req.n_pts), nf = (size_t)std::max(1, req.n_freq);
     DevBuf<double> ak, bk;
     DevBuf<int> flags;
-    if (!ak.alloc(n * nf) || !bk.alloc(n * nf) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    if (!ak.alloc(n * nf, dev) || !bk.alloc(n * nf, dev) || !flags.alloc(n, dev)) { res.error = "hipMalloc failed"; return res; }
     uc_dft_output o{ak.p, bk.p, flags.p};
     const int rc = req.continuation
                        ? uc_run_dft_cont(impl_->eng, s, &r, req.continuation_reverse ? 1 : 0, &o, nullptr)
@@ -350,19 +557,33 @@ Dft1DResult ParametricEngine::run_dft_1d(const Dft1DRequest& req) {
     res.n_freq = req.n_freq;
     res.param_lo = req.param_lo; res.param_hi = req.param_hi;
     res.freq_lo = req.freq_lo; res.freq_hi = req.freq_hi;
+    auto& sn = res.snapshot;
+    sn.values = req.base_values;
+    sn.initial_conditions = req.initial_conditions;
+    sn.tMax = req.t_max;
+    sn.transientTime = req.transient_time;
+    sn.h = req.h;
+    sn.preScaller = req.pre_scaller;
+    sn.writableVar = req.writable_var;
+    sn.indexOfMutVar = req.sweep_over_h ? -1 : req.sweep_over_var ? req.var_sweep_index : req.param_index;
+    sn.range_lo = req.param_lo;
+    sn.range_hi = req.param_hi;
+    sn.n_freq = req.n_freq;
+    sn.freq_lo = req.freq_lo;
+    sn.freq_hi = req.freq_hi;
+    sn.window_type = req.window_type;
     if (!req.csv_output_path.empty()) {
         std::ofstream cfg(req.csv_output_path + "_config.csv");
         csv::dft1d_config(cfg, req.base_values, req.initial_conditions, req.t_max, req.transient_time, req.h,
-                          req.pre_scaller, req.writable_var,
-                          req.sweep_over_h ? -1 : req.sweep_over_var ? req.var_sweep_index : req.param_index,
-                          req.param_lo, req.param_hi, req.n_freq, req.freq_lo, req.freq_hi, req.window_type);
+                          req.pre_scaller, req.writable_var, sn.indexOfMutVar, req.param_lo, req.param_hi, req.n_freq,
+                          req.freq_lo, req.freq_hi, req.window_type);
         std::ofstream a(req.csv_output_path + "_AkCOS.csv"), b(req.csv_output_path + "_BkSIN.csv");
         csv::dft1d_header(a, req.param_lo, req.param_hi, req.freq_lo, req.freq_hi);
         csv::dft1d_header(b, req.param_lo, req.param_hi, req.freq_lo, req.freq_hi);
         csv::dft1d_matrix(a, res.ak_cos.data(), req.n_pts, req.n_freq);
         csv::dft1d_matrix(b, res.bk_sin.data(), req.n_pts, req.n_freq);
     }
-    done(req.progress);
+    set_progress(req.progress, 1.0f);
     res.ok = true;
     return res;
 }
@@ -372,10 +593,11 @@ FastSyncResult ParametricEngine::run_fastsync(const FastSyncRequest& req) {
     res.mode = req.mode;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
-    uc_system* s = impl_->system(req.system, res.error);
+    const bool dev = impl_->device >= 0;
+    if (cancelled(req.cancel)) { res = cancelled_result<FastSyncResult>(); res.mode = req.mode; return res; }
+    uc_system* s = impl_->resolve(req.system, req.krs_body, vars_of(req), (int)req.values.size(), false, res.error);
     if (!s) return res;
-    const size_t n = req.system.vars.size();
+    const size_t n = (size_t)uc_system_n_vars(s);
     if (req.k_forward.size() != n || req.k_backward.size() != n || req.ic_master.size() != n || req.ic_slave.size() != n) {
         res.error = "k_forward / k_backward / ic_master / ic_slave must hold amountOfX values";
         return res;
@@ -412,7 +634,7 @@ FastSyncResult ParametricEngine::run_fastsync(const FastSyncRequest& req) {
         return res;
     }
     DevBuf<double> err, traj;
-    if (!err.alloc((size_t)count) || (req.mode == 0 && !traj.alloc((size_t)count * n))) {
+    if (!err.alloc((size_t)count, dev) || (req.mode == 0 && !traj.alloc((size_t)count * n, dev))) {
         res.error = "hipMalloc failed";
         return res;
     }
@@ -440,6 +662,28 @@ FastSyncResult ParametricEngine::run_fastsync(const FastSyncRequest& req) {
         res.axis_y_lo = req.axis_y_lo; res.axis_y_hi = req.axis_y_hi;
         res.axis_x_var = req.axis_x_var; res.axis_y_var = req.axis_y_var;
     }
+    auto& sn = res.snapshot;
+    sn.mode = req.mode;
+    sn.values = req.values;
+    sn.ic_master = req.ic_master;
+    sn.ic_slave = req.ic_slave;
+    sn.k_forward = req.k_forward;
+    sn.k_backward = req.k_backward;
+    sn.var_names = req.var_names;
+    sn.h = req.h;
+    sn.iter_of_synchr = req.iter_of_synchr;
+    sn.preScaller = req.pre_scaller;
+    sn.window = req.window;
+    sn.type_of_synch = req.type_of_synch;
+    sn.error_estim = req.error_estim;
+    sn.fs_error_trs = req.fs_error_trs;
+    sn.tMax = req.t_max;
+    sn.transientTime = req.transient_time;
+    sn.axis_x_var = req.axis_x_var; sn.axis_y_var = req.axis_y_var;
+    sn.axis_x_lo = req.axis_x_lo; sn.axis_x_hi = req.axis_x_hi;
+    sn.axis_y_lo = req.axis_y_lo; sn.axis_y_hi = req.axis_y_hi;
+    sn.n_pts = req.n_pts;
+    sn.grid_swap_master_slave = req.grid_swap_master_slave;
     if (!req.csv_output_path.empty()) {
         std::ofstream cfg(req.csv_output_path + "_config.csv");
         csv::fastsync_config(cfg, req.mode, req.values, req.ic_master, req.ic_slave, req.k_forward, req.k_backward,
@@ -455,7 +699,7 @@ FastSyncResult ParametricEngine::run_fastsync(const FastSyncRequest& req) {
             csv::fastsync_grid(data, res.heatmap.data(), req.n_pts, req.axis_x_lo, req.axis_x_hi, req.axis_y_lo,
                                req.axis_y_hi);
     }
-    done(req.progress);
+    set_progress(req.progress, 1.0f);
     res.ok = true;
     return res;
 }
@@ -464,8 +708,10 @@ Bifurcation2DResult ParametricEngine::run_bifurcation_2d(const Bifurcation2DRequ
     Bifurcation2DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
-    uc_system* s = impl_->system(req.system, res.error);
+    const bool dev = impl_->device >= 0;
+    if (cancelled(req.cancel)) return cancelled_result<Bifurcation2DResult>();
+    uc_system* s = impl_->resolve(req.system, req.krs_body, vars_of(req), (int)req.base_values.size(), false,
+                                  res.error);
     if (!s) return res;
     uc_bif_request r{};
     impl_->common(r.c, req.initial_conditions, req.base_values, req.h, req.transient_time, req.t_max, req.max_value,
@@ -475,7 +721,7 @@ Bifurcation2DResult ParametricEngine::run_bifurcation_2d(const Bifurcation2DRequ
     r.pre_scaller = req.pre_scaller;
     const size_t n = (size_t)req.n_pts * (size_t)req.n_pts;
     DevBuf<int> periods;
-    if (!periods.alloc(n)) { res.error = "hipMalloc failed"; return res; }
+    if (!periods.alloc(n, dev)) { res.error = "hipMalloc failed"; return res; }
     if (uc_run_bif_periods(impl_->eng, s, &r, req.eps_dbscan, periods.p, nullptr) != UC_OK) {
         res.error = impl_->err();
         return res;
@@ -496,18 +742,30 @@ Bifurcation2DResult ParametricEngine::run_bifurcation_2d(const Bifurcation2DRequ
     res.n_pts = req.n_pts;
     res.param_lo = req.param_lo; res.param_hi = req.param_hi;
     res.param_lo_2 = req.param_lo_2; res.param_hi_2 = req.param_hi_2;
+    auto& sn = res.snapshot;
+    sn.values = req.base_values;
+    sn.initial_conditions = req.initial_conditions;
+    sn.par_or_var = par_or_var(req.sweep_over_var, req.sweep_over_var_2);
+    sn.tMax = req.t_max;
+    sn.transientTime = req.transient_time;
+    sn.h = req.h;
+    sn.preScaller = req.pre_scaller;
+    sn.eps_dbscan = req.eps_dbscan;
+    sn.writableVar = req.writable_var;
+    sn.indexOfMutVar = req.sweep_over_var ? req.var_sweep_index : req.param_index;
+    sn.indexOfMutVar2 = req.sweep_over_var_2 ? req.var_sweep_index_2 : req.param_index_2;
+    sn.range1_lo = req.param_lo; sn.range1_hi = req.param_hi;
+    sn.range2_lo = req.param_lo_2; sn.range2_hi = req.param_hi_2;
+    sn.n_pts = req.n_pts;
     if (!req.csv_output_path.empty()) {
         std::ofstream cfg(req.csv_output_path + "_config.csv");
-        const int pov = (req.sweep_over_var == req.sweep_over_var_2) ? (req.sweep_over_var ? 0 : 1) : 2;
-        csv::bif2d_config(cfg, pov, req.base_values, req.initial_conditions, req.t_max, req.transient_time, req.h,
-                          req.pre_scaller, req.eps_dbscan, req.writable_var,
-                          req.sweep_over_var ? req.var_sweep_index : req.param_index,
-                          req.sweep_over_var_2 ? req.var_sweep_index_2 : req.param_index_2, req.param_lo, req.param_hi,
-                          req.param_lo_2, req.param_hi_2, req.n_pts);
+        csv::bif2d_config(cfg, sn.par_or_var, req.base_values, req.initial_conditions, req.t_max, req.transient_time,
+                          req.h, req.pre_scaller, req.eps_dbscan, req.writable_var, sn.indexOfMutVar,
+                          sn.indexOfMutVar2, req.param_lo, req.param_hi, req.param_lo_2, req.param_hi_2, req.n_pts);
         std::ofstream out(req.csv_output_path);
         csv::grid(out, req.n_pts, res.values.data());
     }
-    done(req.progress);
+    set_progress(req.progress, 1.0f);
     res.ok = true;
     return res;
 }
@@ -516,39 +774,36 @@ LLE2DResult ParametricEngine::run_lle_2d(const LLE2DRequest& req) {
     LLE2DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
-    uc_system* s = impl_->system(req.system, res.error);
+    const bool dev = impl_->device >= 0;
+    if (cancelled(req.cancel)) return cancelled_result<LLE2DResult>();
+    bool stepped = false;
+    uc_system* s = impl_->resolve(req.system, req.krs_body, vars_of(req), (int)req.base_values.size(), false,
+                                  res.error, &stepped);
     if (!s) return res;
     uc_lyap_request r{};
-    lyap_common(impl_.get(), req, r, 2);
+    lyap_common(impl_.get(), req, r, 2, stepped);
     axes_2d(req, r.c);
     const size_t n = (size_t)req.n_pts * (size_t)req.n_pts;
-    DevBuf<double> vals;
-    DevBuf<int> flags;
-    if (!vals.alloc(n) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
-    uc_lyap_output o{vals.p, flags.p, nullptr};
-    if (uc_run_lle(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
-    if (hipDeviceSynchronize() != hipSuccess || !vals.download(res.values, n) || !flags.download(res.flags, n)) {
-        res.error = "device error";
-        return res;
-    }
+    const int st = run_lyap_map(impl_->eng, dev, s, r, req, false, false, false, n, 1, res.values, res.flags, res.error);
+    if (st == 1) return cancelled_result<LLE2DResult>();
+    if (st < 0) return res;
     for (size_t i = 0; i < n; ++i)
         if (res.flags[i] != 1) res.values[i] = std::numeric_limits<double>::quiet_NaN();
     min_max(res.values.data(), res.flags.data(), n, res.min_val, res.max_val);
     res.n_pts = req.n_pts;
     res.param_lo = req.param_lo; res.param_hi = req.param_hi;
     res.param_lo_2 = req.param_lo_2; res.param_hi_2 = req.param_hi_2;
+    map2d_snapshot(req, res.snapshot);
     if (!req.csv_output_path.empty()) {
+        const auto& sn = res.snapshot;
         std::ofstream cfg(req.csv_output_path + "_config.csv");
-        const int pov = (req.sweep_over_var == req.sweep_over_var_2) ? (req.sweep_over_var ? 0 : 1) : 2;
-        csv::map2d_config(cfg, "2D LLE", pov, req.base_values, req.initial_conditions, req.t_max, req.NT,
-                          req.transient_time, req.h, req.eps, req.sweep_over_var ? req.var_sweep_index : req.param_index,
-                          req.sweep_over_var_2 ? req.var_sweep_index_2 : req.param_index_2, req.param_lo, req.param_hi,
-                          req.param_lo_2, req.param_hi_2, req.n_pts, 0);
+        csv::map2d_config(cfg, "2D LLE", sn.par_or_var, req.base_values, req.initial_conditions, req.t_max, req.NT,
+                          req.transient_time, req.h, req.eps, sn.indexOfMutVar, sn.indexOfMutVar2, req.param_lo,
+                          req.param_hi, req.param_lo_2, req.param_hi_2, req.n_pts, 0);
         std::ofstream out(req.csv_output_path);
         csv::grid(out, req.n_pts, res.values.data());
     }
-    done(req.progress);
+    set_progress(req.progress, 1.0f);
     res.ok = true;
     return res;
 }
@@ -557,22 +812,19 @@ LS2DResult ParametricEngine::run_ls_2d(const LS2DRequest& req) {
     LS2DResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
-    uc_system* s = impl_->system(req.system, res.error);
+    const bool dev = impl_->device >= 0;
+    if (cancelled(req.cancel)) return cancelled_result<LS2DResult>();
+    bool stepped = false;
+    uc_system* s = impl_->resolve(req.system, req.krs_body, vars_of(req), (int)req.base_values.size(), false,
+                                  res.error, &stepped);
     if (!s) return res;
     uc_lyap_request r{};
-    lyap_common(impl_.get(), req, r, 2);
+    lyap_common(impl_.get(), req, r, 2, stepped);
     axes_2d(req, r.c);
-    const size_t n = (size_t)req.n_pts * (size_t)req.n_pts, N = req.system.vars.size();
-    DevBuf<double> vals;
-    DevBuf<int> flags;
-    if (!vals.alloc(n * N) || !flags.alloc(n)) { res.error = "hipMalloc failed"; return res; }
-    uc_lyap_output o{vals.p, flags.p, nullptr};
-    if (uc_run_ls(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
-    if (hipDeviceSynchronize() != hipSuccess || !vals.download(res.values, n * N) || !flags.download(res.flags, n)) {
-        res.error = "device error";
-        return res;
-    }
+    const size_t n = (size_t)req.n_pts * (size_t)req.n_pts, N = (size_t)uc_system_n_vars(s);
+    const int st = run_lyap_map(impl_->eng, dev, s, r, req, true, false, false, n, N, res.values, res.flags, res.error);
+    if (st == 1) return cancelled_result<LS2DResult>();
+    if (st < 0) return res;
     res.n_pts = req.n_pts;
     res.n_exponents = (int)N;
     res.min_val.resize(N);
@@ -580,28 +832,102 @@ LS2DResult ParametricEngine::run_ls_2d(const LS2DRequest& req) {
     for (size_t k = 0; k < N; ++k) min_max(res.values.data() + k * n, res.flags.data(), n, res.min_val[k], res.max_val[k]);
     res.param_lo = req.param_lo; res.param_hi = req.param_hi;
     res.param_lo_2 = req.param_lo_2; res.param_hi_2 = req.param_hi_2;
+    map2d_snapshot(req, res.snapshot);
+    res.snapshot.n_exponents = (int)N;
     if (!req.csv_output_path.empty()) {
+        const auto& sn = res.snapshot;
         std::ofstream cfg(req.csv_output_path + "_config.csv");
-        const int pov = (req.sweep_over_var == req.sweep_over_var_2) ? (req.sweep_over_var ? 0 : 1) : 2;
-        csv::map2d_config(cfg, "2D LS", pov, req.base_values, req.initial_conditions, req.t_max, req.NT,
-                          req.transient_time, req.h, req.eps, req.sweep_over_var ? req.var_sweep_index : req.param_index,
-                          req.sweep_over_var_2 ? req.var_sweep_index_2 : req.param_index_2, req.param_lo, req.param_hi,
-                          req.param_lo_2, req.param_hi_2, req.n_pts, (int)N);
+        csv::map2d_config(cfg, "2D LS", sn.par_or_var, req.base_values, req.initial_conditions, req.t_max, req.NT,
+                          req.transient_time, req.h, req.eps, sn.indexOfMutVar, sn.indexOfMutVar2, req.param_lo,
+                          req.param_hi, req.param_lo_2, req.param_hi_2, req.n_pts, (int)N);
         std::ofstream out(req.csv_output_path);
         csv::ls2d_cells(out, req.n_pts, (int)N, res.values.data());
     }
-    done(req.progress);
+    set_progress(req.progress, 1.0f);
     res.ok = true;
     return res;
 }
+
+namespace {
+void feature_range(const std::vector<double>& f, const std::vector<int>& ok, double& lo, double& hi) {
+    lo = hi = 0.0;
+    bool any = false;
+    for (size_t i = 0; i < f.size(); ++i) {
+        if (!ok[i] || !std::isfinite(f[i]) || f[i] == 999.0) continue;
+        if (!any) { lo = hi = f[i]; any = true; }
+        lo = std::min(lo, f[i]);
+        hi = std::max(hi, f[i]);
+    }
+}
+
+// Clusters cells by their two features: the features are quantised with
+// step eps, grid boxes that touch (including diagonally) are merged, and
+// clusters are numbered 1..K by their smallest box (feature 1, then 2).
+// Cells with ok == 0 or non-finite features get 0.
+int cluster_features(const std::vector<double>& f1, const std::vector<double>& f2, const std::vector<int>& ok,
+                     double eps, std::vector<int>& label) {
+    const size_t n = f1.size();
+    label.assign(n, 0);
+    std::map<std::pair<long long, long long>, int> box;  // box -> node
+    std::vector<std::pair<long long, long long>> key(n);
+    std::vector<int> node_of(n, -1);
+    std::vector<std::pair<long long, long long>> boxes;
+    for (size_t i = 0; i < n; ++i) {
+        if (!ok[i] || !std::isfinite(f1[i]) || !std::isfinite(f2[i]) || f1[i] == 999.0) continue;
+        key[i] = {(long long)std::floor(f1[i] / eps + 0.5), (long long)std::floor(f2[i] / eps + 0.5)};
+        auto it = box.find(key[i]);
+        if (it == box.end()) {
+            it = box.emplace(key[i], (int)boxes.size()).first;
+            boxes.push_back(key[i]);
+        }
+        node_of[i] = it->second;
+    }
+    std::vector<int> parent(boxes.size());
+    std::iota(parent.begin(), parent.end(), 0);
+    std::function<int(int)> find = [&](int a) { return parent[a] == a ? a : parent[a] = find(parent[a]); };
+    for (size_t b = 0; b < boxes.size(); ++b)
+        for (int dx = -1; dx <= 1; ++dx)
+            for (int dy = -1; dy <= 1; ++dy) {
+                auto it = box.find({boxes[b].first + dx, boxes[b].second + dy});
+                if (it != box.end()) {
+                    const int ra = find((int)b), rb = find(it->second);
+                    if (ra != rb) parent[std::max(ra, rb)] = std::min(ra, rb);
+                }
+            }
+    // canonical order: a cluster's smallest box
+    std::map<int, std::pair<long long, long long>> least;
+    for (size_t b = 0; b < boxes.size(); ++b) {
+        const int r = find((int)b);
+        auto it = least.find(r);
+        if (it == least.end() || boxes[b] < it->second) least[r] = boxes[b];
+    }
+    std::vector<std::pair<std::pair<long long, long long>, int>> order;
+    for (auto& kv : least) order.push_back({kv.second, kv.first});
+    std::sort(order.begin(), order.end());
+    std::map<int, int> id;
+    for (size_t k = 0; k < order.size(); ++k) id[order[k].second] = (int)k + 1;
+    for (size_t i = 0; i < n; ++i)
+        if (node_of[i] >= 0) label[i] = id[find(node_of[i])];
+    return (int)order.size();
+}
+}  // namespace
 
 BasinsResult ParametricEngine::run_basins(const BasinsRequest& req) {
     BasinsResult res;
     std::lock_guard<std::mutex> lk(impl_->mu);
     if (!impl_->eng) { res.error = "engine not initialised"; return res; }
-    if (cancelled(req.cancel)) { res.cancelled = true; res.error = "Cancelled by user"; return res; }
-    uc_system* s = impl_->system(req.system, res.error);
+    const bool dev = impl_->device >= 0;
+    if (cancelled(req.cancel)) return cancelled_result<BasinsResult>();
+    uc_system* s = impl_->resolve(req.system, req.krs_body, vars_of(req), (int)req.base_values.size(), true,
+                                  res.error);
     if (!s) return res;
+    if (!(req.section_period > 0.0)) { res.error = "section_period must be > 0"; return res; }
+    if (!(req.eps_dbscan > 0.0)) { res.error = "eps_dbscan must be > 0"; return res; }
+    int n_sections = req.n_sections;
+    if (n_sections <= 0)
+        n_sections = std::max(17, (int)std::ceil((req.transient_time + req.t_max) / req.section_period));
+    if (req.progress_phase) req.progress_phase->store(1, std::memory_order_relaxed);
+    set_progress(req.progress, 0.0f);
     uc_basins_request r{};
     impl_->common(r.c, req.initial_conditions, req.base_values, req.h, 0.0, 1.0, req.max_value, req.precision);
     r.c.n_axes = 2;
@@ -612,17 +938,17 @@ BasinsResult ParametricEngine::run_basins(const BasinsRequest& req) {
     r.h_min = 1e-12;
     r.h_max = req.section_period / 4.0;
     r.section_period = req.section_period;
-    r.n_sections = req.n_sections;
+    r.n_sections = n_sections;
     r.n_record = 16;
     r.max_period = 8;
     r.label_vars = 0;
     r.period_tol = 1e-4;
-    r.quantum = 1e-3;
+    r.quantum = req.eps_dbscan;
     const size_t n = (size_t)req.n_pts * (size_t)req.n_pts;
-    const int lbl = req.system.vars.size() < 2 ? (int)req.system.vars.size() : 2;
+    const int lbl = std::min(2, uc_system_n_vars(s));
     DevBuf<int> labels, periods;
     DevBuf<double> points;
-    if (!labels.alloc(n) || !periods.alloc(n) || !points.alloc(n * (size_t)lbl)) { res.error = "hipMalloc failed"; return res; }
+    if (!labels.alloc(n, dev) || !periods.alloc(n, dev) || !points.alloc(n * (size_t)lbl, dev)) { res.error = "hipMalloc failed"; return res; }
     int n_att = 0;
     uc_basins_output o{labels.p, periods.p, points.p, nullptr, 0, &n_att, nullptr, nullptr};
     if (uc_run_basins(impl_->eng, s, &r, &o, nullptr) != UC_OK) { res.error = impl_->err(); return res; }
@@ -631,39 +957,80 @@ BasinsResult ParametricEngine::run_basins(const BasinsRequest& req) {
         res.error = "device error";
         return res;
     }
-    res.cycle_x.assign(pts.begin(), pts.begin() + (std::ptrdiff_t)n);
-    for (size_t i = 0; i < n; ++i)
-        if (res.periods[i] <= 0) res.cycle_x[i] = std::nan("");
+    if (req.progress_phase) req.progress_phase->store(2, std::memory_order_relaxed);
     res.helpful_array.resize(n);
-    for (size_t i = 0; i < n; ++i) res.helpful_array[i] = res.periods[i] >= 0 ? 1 : 0;
+    res.avg_peaks.assign(n, 999.0);
+    res.avg_intervals.assign(n, 999.0);
+    res.cycle_x.assign(n, std::nan(""));
+    for (size_t i = 0; i < n; ++i) {
+        const bool settled = res.periods[i] >= 0;
+        res.helpful_array[i] = settled ? 1 : 0;
+        if (!settled) continue;
+        res.avg_peaks[i] = pts[i];
+        res.avg_intervals[i] = lbl > 1 ? pts[n + i] : 0.0;
+        if (res.periods[i] > 0) res.cycle_x[i] = pts[i];
+    }
+    feature_range(res.avg_peaks, res.helpful_array, res.avg_peaks_min, res.avg_peaks_max);
+    feature_range(res.avg_intervals, res.helpful_array, res.avg_intervals_min, res.avg_intervals_max);
     res.n_clusters = n_att;
     res.min_cluster_idx = 0;
     res.n_pts = req.n_pts;
     res.axis_x_lo = req.axis_x_lo; res.axis_x_hi = req.axis_x_hi;
     res.axis_y_lo = req.axis_y_lo; res.axis_y_hi = req.axis_y_hi;
     res.axis_x_var = req.axis_x_var; res.axis_y_var = req.axis_y_var;
+    auto& sn = res.snapshot;
+    sn.values = req.base_values;
+    sn.initial_conditions = req.initial_conditions;
+    sn.tMax = req.t_max;
+    sn.transientTime = req.transient_time;
+    sn.h = req.h;
+    sn.preScaller = req.pre_scaller;
+    sn.eps_dbscan = req.eps_dbscan;
+    sn.writableVar = req.writable_var;
+    sn.axis_x_var = req.axis_x_var; sn.axis_y_var = req.axis_y_var;
+    sn.axis_x_lo = req.axis_x_lo; sn.axis_x_hi = req.axis_x_hi;
+    sn.axis_y_lo = req.axis_y_lo; sn.axis_y_hi = req.axis_y_hi;
+    sn.n_pts = req.n_pts;
+    sn.feature1 = req.feature1; sn.feature2 = req.feature2;
+    sn.mult1 = req.mult1; sn.mult2 = req.mult2;
     if (!req.csv_output_path.empty()) {
-        // the reference's five files (U-CUDA/data_export.cpp:472-544); its
-        // DBSCAN features (_1 mean peak, _2 mean interval) have no equivalent
-        // here: _1 holds the canonical cycle point's first coordinate and _2
-        // the Poincare period, non-finite / no cycle written as 999
+        // the reference's five files (U-CUDA/data_export.cpp:472-544): the
+        // label grid, _1 / _2 the two features (here the canonical cycle
+        // point, 999 where unbound), _3 the settled / unbound array
         std::ofstream cfg(req.csv_output_path + "_config.csv");
-        csv::basins_config(cfg, req.base_values, req.initial_conditions, req.section_period * req.n_sections, 0.0,
-                           req.h, 1, 1e-3, 0, req.axis_x_var, req.axis_y_var, req.axis_x_lo, req.axis_x_hi,
-                           req.axis_y_lo, req.axis_y_hi);
+        csv::basins_config(cfg, req.base_values, req.initial_conditions, req.t_max, req.transient_time, req.h,
+                           req.pre_scaller, req.eps_dbscan, req.writable_var, req.axis_x_var, req.axis_y_var,
+                           req.axis_x_lo, req.axis_x_hi, req.axis_y_lo, req.axis_y_hi);
         auto with_ranges = [&](const std::string& path) {
             auto f = std::make_unique<std::ofstream>(path);
             csv::basins_ranges(*f, req.axis_x_lo, req.axis_x_hi, req.axis_y_lo, req.axis_y_hi);
             return f;
         };
         csv::grid_int(*with_ranges(req.csv_output_path), req.n_pts, res.basin_idx.data());
-        csv::grid_finite(*with_ranges(req.csv_output_path + "_1.csv"), req.n_pts, res.cycle_x.data());
-        std::vector<double> per(res.periods.begin(), res.periods.end());
-        for (auto& v : per) v = v > 0 ? v : std::nan("");
-        csv::grid_finite(*with_ranges(req.csv_output_path + "_2.csv"), req.n_pts, per.data());
+        csv::grid_finite(*with_ranges(req.csv_output_path + "_1.csv"), req.n_pts, res.avg_peaks.data());
+        csv::grid_finite(*with_ranges(req.csv_output_path + "_2.csv"), req.n_pts, res.avg_intervals.data());
         csv::grid_int(*with_ranges(req.csv_output_path + "_3.csv"), req.n_pts, res.helpful_array.data());
     }
-    done(req.progress);
+    set_progress(req.progress, 1.0f);
+    res.ok = true;
+    return res;
+}
+
+BasinsReclusterResult ParametricEngine::run_basins_recluster(const BasinsReclusterRequest& req) {
+    BasinsReclusterResult res;
+    if (cancelled(req.cancel)) return cancelled_result<BasinsReclusterResult>();
+    const size_t n = (size_t)std::max(0, req.n_pts) * (size_t)std::max(0, req.n_pts);
+    if (req.avg_peaks.size() != n || req.avg_intervals.size() != n || req.helpful_array.size() != n) {
+        res.error = "avg_peaks, avg_intervals and helpful_array must hold n_pts * n_pts values";
+        return res;
+    }
+    if (!(req.eps_dbscan > 0.0)) { res.error = "eps_dbscan must be > 0"; return res; }
+    std::vector<int> ok(n);
+    for (size_t i = 0; i < n; ++i) ok[i] = req.helpful_array[i] == 1;
+    res.n_clusters = cluster_features(req.avg_peaks, req.avg_intervals, ok, req.eps_dbscan, res.basin_idx);
+    res.min_cluster_idx = 0;
+    res.n_pts = req.n_pts;
+    set_progress(req.progress, 1.0f);
     res.ok = true;
     return res;
 }

@@ -47,7 +47,8 @@ class Common(C.Structure):
                 ("n_axes", C.c_int), ("axis", Axis * 2),
                 ("h", C.c_double), ("transient_time", C.c_double), ("t_max", C.c_double),
                 ("max_value", C.c_double), ("precision", C.c_int), ("strict", C.c_int),
-                ("ic_seed", C.c_ulonglong), ("ic_lo", C.c_double), ("ic_hi", C.c_double)]
+                ("ic_seed", C.c_ulonglong), ("ic_lo", C.c_double), ("ic_hi", C.c_double),
+                ("cell_begin", C.c_longlong), ("cell_count", C.c_longlong)]
 
 
 class BifRequest(C.Structure):
@@ -426,13 +427,14 @@ def _stream(stream):
 def run_bif(sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pre_scaller=1,
             max_peaks=256, max_value=1e6, precision=PREC_F64, strict=False, traj_len=0,
             want_final_state=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0, out=None, stream=None, sync=True,
-            record_precision=PREC_F64):
+            record_precision=PREC_F64, cells=None):
     """Bifurcation sweep; returns dict(flags, peaks[max_peaks, cells], intervals, traj, final_state).
     record_precision (F32C sweeps): PREC_F64 records in double after the
-    F32C transient, PREC_F32C records in float stages too."""
+    F32C transient, PREC_F32C records in float stages too.
+    cells=(begin, count): run only that range of the grid's cells."""
     import torch
     eng = sysm.engine
-    n = n_cells(axes)
+    n = n_cells(axes) if cells is None else int(cells[1])
     dev = torch.device("cuda", eng.device)
     if out is None:
         out = dict(flags=torch.empty(n, dtype=torch.int32, device=dev),
@@ -446,6 +448,8 @@ def run_bif(sysm, axes, ic, values, h, transient_time, t_max, writable_var=0, pr
                         ic_seed, ic_lo, ic_hi)
     r.writable_var, r.pre_scaller, r.max_peaks = writable_var, pre_scaller, max_peaks
     r.record_precision = record_precision
+    if cells is not None:
+        r.c.cell_begin, r.c.cell_count = int(cells[0]), int(cells[1])
     o = BifOutput(_ptr(out["flags"]).value, _ptr(out["peaks"]).value, _ptr(out["intervals"]).value,
                   _ptr(out.get("traj")).value, traj_len if out.get("traj") is not None else 0,
                   _ptr(out.get("final_state")).value)
@@ -607,13 +611,14 @@ def run_fastsync(sysm, values, ic_master, ic_slave, k_forward, k_backward, with_
 def run_lyap(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=False, eps=1e-4,
              method=METHOD_VARIATIONAL, probe_seed=REFERENCE_PROBE_SEED, max_value=1e6,
              precision=PREC_F64, strict=False, want_final_state=False, ic_seed=0, ic_lo=-1.0, ic_hi=1.0,
-             out=None, stream=None, sync=True, noalloc=False):
+             out=None, stream=None, sync=True, noalloc=False, cells=None):
     """LLE map (spectrum=False, values[1, cells]) or full spectrum (values[N, cells]).
     noalloc=True: uc_run_lle_noalloc (tables in the static device scratch;
-    the system's module must be loaded: sysm.compile(strict) first)."""
+    the system's module must be loaded: sysm.compile(strict) first).
+    cells=(begin, count): run only that range of the grid's cells."""
     import torch
     eng = sysm.engine
-    n = n_cells(axes)
+    n = n_cells(axes) if cells is None else int(cells[1])
     k = sysm.n_vars if spectrum else 1
     dev = torch.device("cuda", eng.device)
     if out is None:
@@ -625,6 +630,8 @@ def run_lyap(sysm, axes, ic, values, h, transient_time, t_max, NT, spectrum=Fals
     r.c, keep = _common(sysm, axes, ic, values, h, transient_time, t_max, max_value, precision, strict,
                         ic_seed, ic_lo, ic_hi)
     r.NT, r.eps, r.method, r.probe_seed = float(NT), float(eps), int(method), int(probe_seed)
+    if cells is not None:
+        r.c.cell_begin, r.c.cell_count = int(cells[0]), int(cells[1])
     o = LyapOutput(_ptr(out["values"]).value, _ptr(out["flags"]).value, _ptr(out.get("final_state")).value)
     if noalloc:
         if spectrum:
