@@ -606,24 +606,38 @@ def extra_basins(eng, n=2048):
 
 
 def extra_batch(eng, n_jobs=64, seed=0):
-    """64 heterogeneous LLE-map jobs, systems uniform over Lorenz / Rossler /
-    Chen / Duffing, grid side round(64*16^U), transient round(10^(3+2U))
-    steps, 2e3 averaged steps, job seed 1000+j; one persistent hipGraph launch
-    after the launch-shape tuner."""
+    """64 heterogeneous jobs in one persistent hipGraph launch after the
+    launch-shape tuner, systems uniform over Lorenz / Rossler / Chen /
+    Duffing, grid side round(64*16^U), transient round(10^(3+2U)) steps, job
+    seed 1000+j. Lorenz / Rossler / Chen jobs are LLE maps (2e3 averaged
+    steps); Duffing jobs are forced-Duffing basin maps over [-2.5, 2.5]^2 at
+    rtol = atol = 1e-9 with max(16, steps / 500) forcing periods, their
+    attractor catalogues and labels included in the timed launch. All
+    FP32-compensated."""
+    import math
     import numpy as np
     import torch
     import ucuda
     rng = np.random.default_rng(seed)
     systems = {k: ucuda.preset(k, eng) for k in ("lorenz", "rossler", "chen", "duffing")}
     boxes = {"lorenz": (2, 20.0, 200.0, 3, 1.0, 4.0), "rossler": (2, 0.05, 2.0, 3, 1.0, 20.0),
-             "chen": (1, 30.0, 45.0, 3, 20.0, 30.0), "duffing": (1, 0.1, 0.4, 2, 0.2, 0.5)}
+             "chen": (1, 30.0, 45.0, 3, 20.0, 30.0)}
     b = ucuda.Batch(eng)
-    total = 0
+    lle_steps = 0
+    basins_out = []
+    kinds = {"lle": 0, "basins": 0}
     for j in range(n_jobs):
         name = list(systems)[int(rng.integers(0, 4))]
         sysm = systems[name]
         side = int(round(64 * 16 ** rng.uniform()))
         transient_steps = int(round(10 ** (3 + 2 * rng.uniform())))
+        if name == "duffing":
+            axes = [ucuda.axis(ucuda.AXIS_VAR, 0, side, -2.5, 2.5), ucuda.axis(ucuda.AXIS_VAR, 1, side, -2.5, 2.5)]
+            basins_out.append(b.add_basins(sysm, axes, [0.0, 0.0], sysm.values(),
+                                           n_sections=max(16, transient_steps // 500), section_period=2 * math.pi,
+                                           precision=ucuda.PREC_F32C, want_points=False))
+            kinds["basins"] += 1
+            continue
         h = 0.0025 if name == "chen" else 0.01
         p1, lo1, hi1, p2, lo2, hi2 = boxes[name]
         axes = [ucuda.axis(ucuda.AXIS_PARAM, p1, side, lo1, hi1), ucuda.axis(ucuda.AXIS_PARAM, p2, side, lo2, hi2)]
@@ -631,7 +645,8 @@ def extra_batch(eng, n_jobs=64, seed=0):
         t_max = 2000 * h
         b.add_lyap(sysm, axes, ic, sysm.values(), h, transient_steps * h, t_max, 10 * h,
                    precision=ucuda.PREC_F32C, probe_seed=1000 + j)
-        total += side * side * (transient_steps + 1 + int(t_max / (10 * h)) * 11)
+        lle_steps += side * side * (transient_steps + 1 + int(t_max / (10 * h)) * 11)
+        kinds["lle"] += 1
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         tuned = b.tune(stream)
@@ -641,7 +656,11 @@ def extra_batch(eng, n_jobs=64, seed=0):
         b.launch(stream)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-    return dict(jobs=n_jobs, traj_steps=total, s=dt, traj_steps_per_s=total / dt, tuned=tuned, **b.info())
+    bas_acc = sum(int(o["steps"][0].sum()) for o in basins_out)
+    bas_cells = sum(int(o["labels"].numel()) for o in basins_out)
+    return dict(jobs=n_jobs, jobs_by_kind=kinds, s=dt, lle_traj_steps=lle_steps, basins_cells=bas_cells,
+                basins_accepted_steps=bas_acc, traj_steps_per_s=(lle_steps + bas_acc) / dt,
+                basins_attractors=[len(o["attractors"]) for o in basins_out], tuned=tuned, **b.info())
 
 
 def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):

@@ -377,11 +377,19 @@ void uc_batch_destroy(uc_batch* b);
 int uc_batch_add_bif(uc_batch* b, uc_system* s, const uc_bif_request* r, const uc_bif_output* o, int* job_id);
 int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int spectrum, const uc_lyap_output* o,
                       int* job_id);
+/* Basin maps (as uc_run_basins): the adaptive integration runs in the
+ * batch's graph; the attractor catalogue and the labels are computed after
+ * each uc_batch_launch, which then synchronises its stream. The catalogue
+ * arrays of `o` are host memory that must stay valid while the batch
+ * lives; cycle_point may be NULL (the batch keeps a buffer). */
+int uc_batch_add_basins(uc_batch* b, uc_system* s, const uc_basins_request* r, const uc_basins_output* o,
+                        int* job_id);
 /* launch shape: block (64/128/256 lanes) and resident waves per SIMD (1..8) */
 int uc_batch_set_launch(uc_batch* b, int block, int waves_per_simd);
 /* autotuner: times the batch for each candidate shape, keeps the fastest */
 int uc_batch_tune(uc_batch* b, void* stream, int* best_block, int* best_waves_per_simd, double* best_ms);
-/* one graph launch of the whole batch on a non-default stream */
+/* one graph launch of the whole batch on a non-default stream (then the
+ * basins jobs' catalogues, synchronously, when the batch has any) */
 int uc_batch_launch(uc_batch* b, void* stream);
 int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int* waves_per_simd);
 

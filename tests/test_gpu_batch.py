@@ -1,7 +1,8 @@
 """Persistent heterogeneous batch: every job's result from the captured-graph
 batch launch is bit-identical to the same job run alone (same kernel code
-per cell, placement does not matter), across systems, analyses, precisions
-and grid sizes; graph replays reproduce the first launch."""
+per cell, placement does not matter), across systems, analyses (LLE, LS,
+bifurcation, basins with their attractor catalogue), precisions and grid
+sizes; graph replays reproduce the first launch."""
 import numpy as np
 import pytest
 import torch
@@ -28,11 +29,17 @@ def _jobs(eng):
                                 h=0.0025, transient_time=5.0, t_max=10.0, max_peaks=32)))
     J.append(("bif", lor, dict(axes=[ucuda.axis(1, 2, 7, 100.0, 200.0)], ic=[0.1, 0.1, 0.1], values=lor.values(),
                                h=0.01, transient_time=20.0, t_max=20.0, max_peaks=32, precision=ucuda.PREC_F32C)))
+    J.append(("basins", duf, dict(axes=[ucuda.axis(2, 0, 40, -2.5, 2.5), ucuda.axis(2, 1, 36, -2.5, 2.5)],
+                                  ic=[0.0, 0.0], values=duf.values(), n_sections=60)))
+    J.append(("basins", duf, dict(axes=[ucuda.axis(2, 0, 24, -2.0, 2.0), ucuda.axis(2, 1, 20, -2.0, 2.0)],
+                                  ic=[0.0, 0.0], values=duf.values(), n_sections=40, precision=ucuda.PREC_F32C)))
     return J
 
 
 def _alone(kind, sysm, kw):
     kw = dict(kw)
+    if kind == "basins":
+        return ucuda.run_basins(sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), **kw)
     if kind == "lyap":
         return ucuda.run_lyap(sysm, kw.pop("axes"), kw.pop("ic"), kw.pop("values"), kw.pop("h"),
                               kw.pop("transient_time"), kw.pop("t_max"), kw.pop("NT"), **kw)
@@ -53,45 +60,61 @@ def _written(o, key):
 
 
 def _same(a, b):
+    if isinstance(a, list):
+        return a == b
     return a.shape == b.shape and bool(torch.all((a == b) | (torch.isnan(a) & torch.isnan(b)))) \
         if a.is_floating_point() else torch.equal(a, b)
 
 
+def _add(b, kind, sysm, kw):
+    kw2 = dict(kw)
+    if kind == "basins":
+        return b.add_basins(sysm, kw2.pop("axes"), kw2.pop("ic"), kw2.pop("values"), **kw2)
+    args = (sysm, kw2.pop("axes"), kw2.pop("ic"), kw2.pop("values"), kw2.pop("h"), kw2.pop("transient_time"),
+            kw2.pop("t_max"))
+    if kind == "lyap":
+        return b.add_lyap(*args, kw2.pop("NT"), **kw2)
+    return b.add_bif(*args, **kw2)
+
+
+def _snapshot(o):
+    return {k: (list(v) if k == "attractors" else _written(o, k)) for k, v in o.items() if v is not None}
+
+
 def test_batch_equals_alone_and_replays(gpu_engine):
-    print("case: 6 heterogeneous jobs (5 systems, lle/ls/bif, f64/f32c) in one graph launch", flush=True)
+    print("case: 8 heterogeneous jobs (5 systems, lle/ls/bif/basins, f64/f32c) in one graph launch", flush=True)
     jobs = _jobs(gpu_engine)
     b = ucuda.Batch(gpu_engine)
-    outs = []
-    for kind, sysm, kw in jobs:
-        kw2 = dict(kw)
-        if kind == "lyap":
-            outs.append(b.add_lyap(sysm, kw2.pop("axes"), kw2.pop("ic"), kw2.pop("values"), kw2.pop("h"),
-                                   kw2.pop("transient_time"), kw2.pop("t_max"), kw2.pop("NT"), **kw2))
-        else:
-            outs.append(b.add_bif(sysm, kw2.pop("axes"), kw2.pop("ic"), kw2.pop("values"), kw2.pop("h"),
-                                  kw2.pop("transient_time"), kw2.pop("t_max"), **kw2))
+    outs = [_add(b, kind, sysm, kw) for kind, sysm, kw in jobs]
     info = b.info()
-    assert info["n_jobs"] == 6 and info["n_groups"] >= 5
+    assert info["n_jobs"] == 8 and info["n_groups"] >= 7
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         b.launch(stream)
     torch.cuda.synchronize()
-    first = [{k: _written(o, k) for k in o} for o in outs]
+    first = [_snapshot(o) for o in outs]
     for (kind, sysm, kw), f in zip(jobs, first):
-        ref = _alone(kind, sysm, kw)
+        ref = _snapshot(_alone(kind, sysm, kw))
+        assert set(f) == set(ref), (kind, sysm.name)
         for key in f:
-            assert _same(f[key], _written(ref, key)), (kind, sysm.name, key)
+            assert _same(f[key], ref[key]), (kind, sysm.name, key)
+        if kind == "basins":
+            print(sysm.name, "attractors:", f["attractors"])
+            assert len(f["attractors"]) >= 2
     # replays with eager work in between reproduce the first launch
     for _ in range(3):
         for o in outs:
+            o.pop("attractors", None)
             for v in o.values():
-                v.fill_(-7)
+                if v is not None:
+                    v.fill_(-7)
         with torch.cuda.stream(stream):
             b.launch(stream)
         torch.cuda.synchronize()
         for o, f in zip(outs, first):
-            for key in o:
-                assert _same(_written(o, key), f[key]), key
+            g = _snapshot(o)
+            for key in f:
+                assert _same(g[key], f[key]), key
 
 
 def test_batch_tuner_and_shapes(gpu_engine):

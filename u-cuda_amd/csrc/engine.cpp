@@ -1999,6 +1999,15 @@ struct uc_batch {
         double cost;         // steps per cell, for longest-first ordering
     };
     std::vector<Job> jobs;
+    // basins jobs: the whole-grid descriptor for the catalogue and labels
+    // built after each graph launch, and where the catalogue goes
+    struct Basins {
+        Module* m;
+        UcJob job;
+        double quantum;
+        uc_basins_output out;
+    };
+    std::vector<Basins> basins;
     std::vector<void*> owned;  // device allocations (tables, job arrays, counters)
     struct Group {
         Module* m;
@@ -2209,6 +2218,42 @@ int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int s
     return UC_OK;
 }
 
+int uc_batch_add_basins(uc_batch* b, uc_system* s, const uc_basins_request* r, const uc_basins_output* o,
+                        int* job_id) {
+    if (!b || !s || !r || !o) return UC_ERR_ARGUMENT;
+    uc_engine* e = b->e;
+    precompile(e, s, r->c.strict != 0);
+    std::lock_guard<std::recursive_mutex> lk(e->mu);
+    uc_common c;
+    int lbl = 0;
+    int rc = prep_basins(e, s, r, c, lbl);
+    if (rc) return rc;
+    if (!o->labels || !o->periods) return e->fail(UC_ERR_ARGUMENT, "labels / periods outputs missing");
+    if ((rc = enter(e))) return rc;
+    Module* m = nullptr;
+    if ((rc = get_module(e, s, c.strict != 0, true, &m))) return rc;
+    UcJob j;
+    if ((rc = batch_stage(b, s, c, j))) return rc;
+    fill_basins(r, lbl, j);
+    j.flag = o->periods;
+    j.counters = o->steps;
+    j.label = o->labels;
+    double* points = o->cycle_point;
+    if (!points) {
+        hipError_t he = hipMalloc((void**)&points, sizeof(double) * (size_t)lbl * (size_t)j.n_cells);
+        if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(batch basins points)"));
+        b->owned.push_back(points);
+    }
+    j.out0 = points;
+    // ordering weight only (the integrator is adaptive): ~64 steps per section
+    b->jobs.push_back({m, c.precision == UC_PREC_F64 ? "uc_bas_f64_q" : "uc_bas_f32c_q", j, 64.0 * r->n_sections});
+    b->basins.push_back({m, j, r->quantum, *o});
+    ++m->pins;
+    b->built_block = -1;
+    if (job_id) *job_id = (int)b->jobs.size() - 1;
+    return UC_OK;
+}
+
 int uc_batch_set_launch(uc_batch* b, int block, int waves_per_simd) {
     if (!b) return UC_ERR_ARGUMENT;
     if (block != 64 && block != 128 && block != 256) return b->fail(UC_ERR_ARGUMENT, "block must be 64, 128 or 256");
@@ -2279,6 +2324,16 @@ int uc_batch_launch(uc_batch* b, void* stream) {
     }
     hipError_t he = hipGraphLaunch(b->exec, st);
     if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "hipGraphLaunch"));
+    // basins jobs: attractor catalogue and labels once the integration is
+    // done (host clustering in between, so outside the graph; synchronises)
+    for (auto& bj : b->basins) {
+        BasinsCatalogue cat;
+        const std::string why = basins_catalogue(
+            bj.m, bj.job, bj.quantum, st, [](void** p, size_t n) { return hipMalloc(p, n); },
+            [](void* p) { (void)hipFree(p); }, cat);
+        if (!why.empty()) return b->fail(why.rfind("too many", 0) == 0 ? UC_ERR_ARGUMENT : UC_ERR_HIP, why);
+        copy_catalogue(cat, &bj.out);
+    }
     return UC_OK;
 }
 

@@ -170,6 +170,8 @@ _SIGNATURES = {
                                    C.POINTER(C.c_int)]),
     "uc_batch_add_lyap": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(LyapRequest), C.c_int, C.POINTER(LyapOutput),
                                     C.POINTER(C.c_int)]),
+    "uc_batch_add_basins": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BasinsRequest), C.POINTER(BasinsOutput),
+                                      C.POINTER(C.c_int)]),
     "uc_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "uc_batch_tune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_double)]),
@@ -733,6 +735,7 @@ class Batch:
         self.h = h
         self.jobs = []
         self._keep = []
+        self._basins = []  # (out, n_attractors, periods, points, max_attractors)
 
     def __del__(self):
         try:
@@ -783,6 +786,35 @@ class Batch:
         self.jobs.append(out)
         return out
 
+    def add_basins(self, sysm, axes, ic, values, h0=0.01, rtol=1e-9, atol=1e-9, section_period=2 * math.pi,
+                   n_sections=200, n_record=16, max_period=8, period_tol=1e-4, quantum=1e-3, h_min=1e-12,
+                   h_max=None, max_attempts=0, max_value=1e6, precision=PREC_F64, strict=False,
+                   want_points=True, want_steps=True, max_attractors=64):
+        """A basin map as run_basins; out["attractors"] is filled by launch()."""
+        import torch
+        n = n_cells(axes)
+        dev = torch.device("cuda", self.engine.device)
+        lbl = min(sysm.n_vars, 2)
+        out = dict(labels=torch.empty(n, dtype=torch.int32, device=dev),
+                   periods=torch.empty(n, dtype=torch.int32, device=dev),
+                   points=torch.empty((lbl, n), dtype=torch.float64, device=dev) if want_points else None,
+                   steps=torch.empty((2, n), dtype=torch.int64, device=dev) if want_steps else None)
+        r, keep = _basins_request(sysm, axes, ic, values, h0, rtol, atol, section_period, n_sections, n_record,
+                                  max_period, period_tol, quantum, h_min, h_max, max_attempts, max_value, precision,
+                                  strict)
+        na = C.c_int(0)
+        ap = (C.c_int * max_attractors)()
+        apt = (C.c_double * (2 * max_attractors))()
+        o = BasinsOutput(_ptr(out["labels"]).value, _ptr(out["periods"]).value, _ptr(out["points"]).value,
+                         _ptr(out["steps"]).value, max_attractors, C.pointer(na), ap, apt)
+        jid = C.c_int()
+        self.engine.check(lib().uc_batch_add_basins(self.h, sysm.h, C.byref(r), C.byref(o), C.byref(jid)),
+                          "uc_batch_add_basins")
+        self._keep.append((keep, sysm, o))
+        self._basins.append((out, na, ap, apt, max_attractors))
+        self.jobs.append(out)
+        return out
+
     def set_launch(self, block, waves_per_simd):
         self.engine.check(lib().uc_batch_set_launch(self.h, block, waves_per_simd), "uc_batch_set_launch")
 
@@ -797,6 +829,9 @@ class Batch:
         if stream is None:
             stream = torch.cuda.current_stream()
         self.engine.check(lib().uc_batch_launch(self.h, _stream(stream)), "uc_batch_launch")
+        for out, na, ap, apt, cap in self._basins:
+            k = min(na.value, cap)
+            out["attractors"] = [dict(label=i + 1, period=ap[i], point=(apt[2 * i], apt[2 * i + 1])) for i in range(k)]
         if sync:
             torch.cuda.synchronize()
 
