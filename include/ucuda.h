@@ -384,9 +384,27 @@ int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int s
  * lives; cycle_point may be NULL (the batch keeps a buffer). */
 int uc_batch_add_basins(uc_batch* b, uc_system* s, const uc_basins_request* r, const uc_basins_output* o,
                         int* job_id);
-/* launch shape: block (64/128/256 lanes) and resident waves per SIMD (1..8) */
+/* A batch plan: the launch shape, the trajectory-steps each cell runs per
+ * launch (0: whole runs; else jobs whose cells all step the same counts run
+ * in stages of about that many steps, their cells' state kept in HBM
+ * between stages) and the cells per lane (2: LLE maps that can run two
+ * cells per lane do). Results do not depend on the plan. */
+typedef struct uc_batch_plan {
+    int block;             /* lanes per workgroup: 64, 128 or 256 */
+    int waves_per_simd;    /* resident waves per SIMD, 1..8 */
+    double slice_steps;    /* trajectory-steps per cell per launch, 0 = whole runs */
+    int cells_per_lane;    /* 1 or 2 */
+    double ms;             /* set by the tuner: the plan's measured time */
+} uc_batch_plan;
+int uc_batch_set_plan(uc_batch* b, const uc_batch_plan* p);
+int uc_batch_get_plan(const uc_batch* b, uc_batch_plan* p);
+/* autotuner: times the batch for each candidate plan (launch shapes with
+ * whole runs, then steps per launch and cells per lane at the best shape)
+ * and keeps the fastest */
+int uc_batch_tune_plan(uc_batch* b, void* stream, uc_batch_plan* best);
+/* launch shape only: block (64/128/256 lanes) and resident waves per SIMD (1..8) */
 int uc_batch_set_launch(uc_batch* b, int block, int waves_per_simd);
-/* autotuner: times the batch for each candidate shape, keeps the fastest */
+/* uc_batch_tune_plan, returning the shape */
 int uc_batch_tune(uc_batch* b, void* stream, int* best_block, int* best_waves_per_simd, double* best_ms);
 /* one graph launch of the whole batch on a non-default stream (then the
  * basins jobs' catalogues, synchronously, when the batch has any) */
@@ -415,6 +433,10 @@ typedef struct uc_server_config {
     double round_traj_steps;   /* optional cap on a round's trajectory-steps (0 -> none) */
     int block;                 /* lanes per workgroup (0 -> 256)                          */
     int waves_per_simd;        /* resident waves per SIMD (0 -> 8)                        */
+    double slice_steps;        /* trajectory-steps per cell per round for jobs whose cells */
+                               /* all step the same counts: longer jobs resume over several */
+                               /* rounds, their cells' state kept in HBM (0 -> 4096,      */
+                               /* < 0 -> whole runs)                                      */
 } uc_server_config;
 typedef struct uc_server_stats_t {
     long long completed, rounds, queued, active;

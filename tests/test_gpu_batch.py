@@ -81,11 +81,28 @@ def _snapshot(o):
     return {k: (list(v) if k == "attractors" else _written(o, k)) for k, v in o.items() if v is not None}
 
 
-def test_batch_equals_alone_and_replays(gpu_engine):
-    print("case: 8 heterogeneous jobs (5 systems, lle/ls/bif/basins, f64/f32c) in one graph launch", flush=True)
+_ALONE = {}
+
+
+# plans: whole runs; stages of 512 and 2048 traj-steps per cell (every job
+# is sliced, the basins maps one or two sections per stage); two cells per
+# lane for the LLE maps that can
+PLANS = [None, dict(slice_steps=512.0), dict(block=64, waves_per_simd=2, slice_steps=2048.0),
+         dict(cells_per_lane=2)]
+
+
+@pytest.mark.parametrize("plan", PLANS, ids=["whole", "slice512", "slice2048_b64", "pair"])
+def test_batch_equals_alone_and_replays(gpu_engine, plan):
+    print("case: 8 heterogeneous jobs (5 systems, lle/ls/bif/basins, f64/f32c) in one graph launch, plan %s"
+          % plan, flush=True)
     jobs = _jobs(gpu_engine)
     b = ucuda.Batch(gpu_engine)
     outs = [_add(b, kind, sysm, kw) for kind, sysm, kw in jobs]
+    if plan:
+        b.set_plan(**plan)
+        got = b.plan()
+        for k, v in plan.items():
+            assert got[k] == v
     info = b.info()
     assert info["n_jobs"] == 8 and info["n_groups"] >= 7
     stream = torch.cuda.Stream()
@@ -93,8 +110,10 @@ def test_batch_equals_alone_and_replays(gpu_engine):
         b.launch(stream)
     torch.cuda.synchronize()
     first = [_snapshot(o) for o in outs]
-    for (kind, sysm, kw), f in zip(jobs, first):
-        ref = _snapshot(_alone(kind, sysm, kw))
+    for i, ((kind, sysm, kw), f) in enumerate(zip(jobs, first)):
+        if i not in _ALONE:
+            _ALONE[i] = _snapshot(_alone(kind, sysm, kw))
+        ref = _ALONE[i]
         assert set(f) == set(ref), (kind, sysm.name)
         for key in f:
             assert _same(f[key], ref[key]), (kind, sysm.name, key)
@@ -118,19 +137,23 @@ def test_batch_equals_alone_and_replays(gpu_engine):
 
 
 def test_batch_tuner_and_shapes(gpu_engine):
-    print("case: tuner over launch shapes; every shape gives identical results", flush=True)
+    print("case: tuner over launch shapes, steps per launch and cells per lane; every plan gives identical "
+          "results", flush=True)
     ros = ucuda.preset("rossler", gpu_engine)
     b = ucuda.Batch(gpu_engine)
     outs = [b.add_lyap(ros, [ucuda.axis(1, 2, 64, 0.05, 2.0), ucuda.axis(1, 3, n, 1.0, 20.0)], [1, 1, 1],
-                       ros.values(), 0.01, 5.0, 20.0, 0.1, precision=ucuda.PREC_F32C) for n in (3, 50, 200)]
+                       ros.values(), 0.01, 5.0, 200.0, 0.1, precision=ucuda.PREC_F32C) for n in (3, 50, 200)]
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         best = b.tune(stream)
         torch.cuda.synchronize()
+        print("tuned plan:", best)
         ref = [o["values"].clone() for o in outs]
         assert best["block"] in (64, 256) and 1 <= best["waves_per_simd"] <= 8 and best["ms"] > 0
-        for blk, w in ((64, 2), (256, 8), (128, 1)):
-            b.set_launch(blk, w)
+        assert best["slice_steps"] in (0.0, 16384.0, 4096.0) and best["cells_per_lane"] in (1, 2)
+        assert b.plan() == best
+        for blk, w, sl, cpl in ((64, 2, 0.0, 1), (256, 8, 1000.0, 1), (128, 1, 0.0, 2), (256, 4, 300.0, 1)):
+            b.set_plan(blk, w, sl, cpl)
             b.launch(stream)
             for o, r in zip(outs, ref):
-                assert torch.equal(o["values"], r)
+                assert torch.equal(o["values"], r), (blk, w, sl, cpl)

@@ -1,8 +1,8 @@
 """Continuous-batching server: jobs submitted while others run are admitted
 into later rounds, share rounds with them, and every result (bifurcation,
 Lyapunov, basins with their attractor catalogue) is bit-identical to the
-same request run alone; cancellation, admission by memory budget and
-latency statistics."""
+same request run alone, whole or resumed over several rounds (slices);
+cancellation, admission by memory budget and latency statistics."""
 import time
 
 import numpy as np
@@ -62,11 +62,18 @@ def _alone(kind, sysm, kw):
                 intervals=out["intervals"].cpu().numpy())
 
 
-def test_results_identical_alone_and_in_the_mix(gpu_engine):
-    print("case: 10 staggered submissions (lle f64/f32c, bif, ls, basins f64/f32c) with small rounds", flush=True)
+_ALONE = {}
+
+
+@pytest.mark.parametrize("slice_steps", [0.0, 300.0, -1.0], ids=["slice4096", "slice300", "whole"])
+def test_results_identical_alone_and_in_the_mix(gpu_engine, slice_steps):
+    print("case: 10 staggered submissions (lle f64/f32c, bif, ls, basins f64/f32c) with small rounds, "
+          "slice_steps %g" % slice_steps, flush=True)
     jobs = _mix(gpu_engine)
-    alone = [_alone(k, s, kw) for k, s, kw in jobs]
-    srv = ucuda.Server(gpu_engine, round_traj_steps=2e7)
+    if not _ALONE:
+        _ALONE.update({i: _alone(k, s, kw) for i, (k, s, kw) in enumerate(jobs)})
+    alone = [_ALONE[i] for i in range(len(jobs))]
+    srv = ucuda.Server(gpu_engine, round_traj_steps=2e7, slice_steps=slice_steps)
     ids = []
     for i, (k, s, kw) in enumerate(jobs):
         ids.append(_submit(srv, k, s, kw))
@@ -127,11 +134,8 @@ def test_admission_budget_and_cancel(gpu_engine):
 
 
 def test_short_jobs_do_not_wait_for_long_cells(gpu_engine):
-    """Rounds hold one duration class (decade of steps per cell), so short
-    jobs submitted right after a job of long cells never share its round:
-    with the GPU not full (the long job is 4,096 cells), they finish long
-    before it. (A long job that fills every wave slot still holds them until
-    its cells finish: rounds are not preemptible.)"""
+    """Short jobs submitted right after a job of long cells finish long
+    before it: the long job runs in slices of 4,096 steps per cell."""
     print("case: one 64x64 LLE job of 110,001 steps per cell, then five 16x16 jobs of 111", flush=True)
     ros = ucuda.preset("rossler", gpu_engine)
     vals = ros.values()
@@ -151,5 +155,32 @@ def test_short_jobs_do_not_wait_for_long_cells(gpu_engine):
         print("long job %.1f ms; short jobs %s ms" % (lat_long, " ".join("%.1f" % v for v in lat_short)))
         assert all(srv.poll(j)[0] == 2 for j in short_ids + [long_id])
         assert max(lat_short) < 0.5 * lat_long
+    finally:
+        srv.close()
+
+
+def test_short_jobs_pass_a_job_that_fills_the_gpu(gpu_engine):
+    """A long job with more cells than the GPU holds resident (1024^2) runs
+    in slices, so short jobs submitted after it get into the next rounds
+    instead of waiting for its cells to finish."""
+    print("case: one 1024x1024 LLE job of 55,001 steps per cell, then five 16x16 jobs of 111", flush=True)
+    ros = ucuda.preset("rossler", gpu_engine)
+    vals = ros.values()
+    big = [ucuda.axis(1, 2, 1024, 0.05, 2.0), ucuda.axis(1, 3, 1024, 1.0, 20.0)]
+    small = [ucuda.axis(1, 2, 16, 0.05, 2.0), ucuda.axis(1, 3, 16, 1.0, 20.0)]
+    ucuda.run_lyap(ros, small, [1, 1, 1], vals, 0.01, 0.0, 1.0, 0.1, precision=ucuda.PREC_F32C)
+    srv = ucuda.Server(gpu_engine)
+    try:
+        long_id = srv.submit_lyap(ros, big, [1, 1, 1], vals, 0.01, 0.0, 500.0, 0.1, precision=ucuda.PREC_F32C)
+        time.sleep(0.005)
+        short_ids = [srv.submit_lyap(ros, small, [1, 1, 1], vals, 0.01, 0.0, 1.0, 0.1, precision=ucuda.PREC_F32C)
+                     for _ in range(5)]
+        for jid in short_ids + [long_id]:
+            srv.wait(jid, timeout_s=120.0)
+        lat_long = srv.poll(long_id)[1]
+        lat_short = [srv.poll(j)[1] for j in short_ids]
+        print("long job %.1f ms; short jobs %s ms" % (lat_long, " ".join("%.1f" % v for v in lat_short)))
+        assert all(srv.poll(j)[0] == 2 for j in short_ids + [long_id])
+        assert max(lat_short) < 0.3 * lat_long
     finally:
         srv.close()

@@ -223,9 +223,14 @@ std::string module_key(const std::string& src, bool strict) {
 
 // The code object of a program: from the disk cache, else hipRTC for gfx950.
 // Touches no engine state, so it runs without the engine lock.
+// Fast modules fuse a * b + c inside one source expression only
+// (-ffp-contract=on): which operations fuse is then fixed by the source
+// text, not by how a kernel's code is laid out in basic blocks, so every
+// kernel variant running the same cell code (direct, persistent, sliced, two
+// cells per lane) gives the same bits. Strict modules fuse nothing.
 int compile_code(const std::string& src, bool strict, std::vector<char>& code, std::string& err) {
     std::vector<const char*> opts = {"--offload-arch=gfx950", "-O3", "-std=c++17",
-                                     strict ? "-ffp-contract=off" : "-ffp-contract=fast",
+                                     strict ? "-ffp-contract=off" : "-ffp-contract=on",
                                      "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
     int vmaj = 0, vmin = 0;
     (void)hiprtcVersion(&vmaj, &vmin);
@@ -1987,16 +1992,66 @@ int uc_bif_dbscan_periods(uc_engine* e, uc_system* s, const double* peaks, const
 }  // extern "C"
 
 // =====================================================================
+// Resumable slices. A job whose cells all step the same counts (no sweep
+// over h) can run in slices: each launch of a sliced persistent kernel
+// ("*_s") runs a range of every cell's work units and keeps unfinished
+// cells in a slab of `words` doubles per cell. Units: transient chunks of
+// UC_UNIT_STEPS steps, then one per renormalisation block (LLE / LS), per
+// UC_UNIT_STEPS recorded samples (bifurcation) or per section (basins).
+// Results equal a whole run's bit for bit.
+namespace {
+enum { kKindBif = 0, kKindLle = 1, kKindLs = 2, kKindBasins = 3 };
+struct SliceInfo {
+    long long units = 0;    // 0: cannot be sliced
+    double unit_cost = 0.0; // traj-steps per unit, on average
+    int words = 0;
+};
+long long unit_chunks(long long steps) { return steps <= 0 ? 1 : (steps + UC_UNIT_STEPS - 1) / UC_UNIT_STEPS; }
+SliceInfo slice_info(int kind, const UcJob& j, int n_vars, double step_cost) {
+    SliceInfo si;
+    if (j.axis_kind[0] == UC_AXIS_STEP || j.axis_kind[1] == UC_AXIS_STEP || j.raw_flags == 2) return si;
+    if (kind == kKindBif) {
+        si.units = unit_chunks(j.skip_steps) + unit_chunks(j.n_iters);
+        si.words = UC_SLAB_BIF_WORDS(n_vars);
+    } else if (kind == kKindBasins) {
+        si.units = j.n_sections;
+        si.words = UC_SLAB_BAS_WORDS(n_vars);
+    } else {
+        si.units = unit_chunks(j.skip_steps) + j.n_iters;
+        si.words = UC_SLAB_LYAP_WORDS(n_vars, kind == kKindLs ? n_vars : 1);
+    }
+    si.unit_cost = step_cost / (double)std::max(1LL, si.units);
+    return si;
+}
+// units per launch for a target of `slice_steps` traj-steps per cell
+long long slice_units(const SliceInfo& si, double slice_steps) {
+    return std::max(1LL, (long long)(slice_steps / std::max(1e-9, si.unit_cost)));
+}
+std::string sliced_kernel(const std::string& q) { return q.substr(0, q.size() - 2) + "_s"; }  // "*_q" -> "*_s"
+// adaptive basins: scheduling weight of one section in traj-steps (the
+// measured FP32C / FP64 basin rates against the LLE map's)
+double basins_section_cost(int precision) { return precision == UC_PREC_F64 ? 1024.0 : 256.0; }
+}  // namespace
+
+// =====================================================================
 // Persistent heterogeneous batch: every job of the same (module, kernel)
-// shares one persistent launch over a job table; all groups are captured
-// into one hipGraph and launched once.
+// shares one persistent launch over a job table. A plan fixes the launch
+// shape (block, resident waves per SIMD), the trajectory-steps per launch
+// (0: whole runs; else the sliceable jobs run in stages of that many steps
+// per cell, stage k + 1 after stage k) and the cells per lane (2: LLE maps
+// that can run two cells per lane do). The reset kernel and every stage's
+// launches are captured into one hipGraph and launched once.
 struct uc_batch {
     uc_engine* e = nullptr;
     struct Job {
         Module* m;
-        std::string kernel;  // persistent kernel name
+        std::string kernel;  // persistent kernel for a whole run ("*_q")
         UcJob job;
         double cost;         // steps per cell, for longest-first ordering
+        int kind;
+        SliceInfo si;
+        bool pair;           // can run two cells per lane ("*_x2_q")
+        double* slab;        // allocated by the first plan that slices the job
     };
     std::vector<Job> jobs;
     // basins jobs: the whole-grid descriptor for the catalogue and labels
@@ -2008,18 +2063,21 @@ struct uc_batch {
         uc_basins_output out;
     };
     std::vector<Basins> basins;
-    std::vector<void*> owned;  // device allocations (tables, job arrays, counters)
+    std::vector<void*> owned;  // device allocations (tables, job arrays, counters, slabs)
     struct Group {
         Module* m;
         std::string kernel;
+        int cells_per_chunk_lane;  // 1, or 2 for the pair kernels
         std::vector<UcJob> table;
         UcJob* d_jobs = nullptr;
         long long* d_start = nullptr;
         long long total_chunks = 0;
     };
-    std::vector<Group> groups;
+    std::vector<std::vector<Group>> stages;
     unsigned long long* d_counters = nullptr;
-    int block = 0, waves_per_simd = 0, built_block = -1;
+    int n_counters = 0;
+    uc_batch_plan plan{256, 4, 0.0, 1, 0.0};
+    uc_batch_plan built{};  // plan the tables were built for (block 0: none)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipStream_t> aux;
@@ -2038,6 +2096,7 @@ struct uc_batch {
         if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
         if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
     }
+    void invalidate() { built.block = 0; drop_graph(); }
 };
 
 namespace {
@@ -2062,86 +2121,142 @@ int cu_count(int device) {
     return n;
 }
 
-// Builds the per-group job tables for a chunk size of `block` cells.
-int batch_build(uc_batch* b, int block) {
-    if (b->built_block == block) return UC_OK;
-    b->drop_graph();
-    std::map<std::pair<Module*, std::string>, size_t> gi;
-    b->groups.clear();
-    std::vector<uc_batch::Job> order = b->jobs;
-    std::stable_sort(order.begin(), order.end(), [](const uc_batch::Job& x, const uc_batch::Job& y) { return x.cost > y.cost; });
-    for (auto& jb : order) {
-        auto key = std::make_pair(jb.m, jb.kernel);
-        auto it = gi.find(key);
-        if (it == gi.end()) {
-            gi[key] = b->groups.size();
-            b->groups.push_back(uc_batch::Group{jb.m, jb.kernel, {}, nullptr, nullptr, 0});
-            it = gi.find(key);
+void batch_add(uc_batch* b, Module* m, const std::string& kernel, const UcJob& j, double cost, int kind, int n_vars,
+               bool pair) {
+    uc_batch::Job jb{m, kernel, j, cost, kind, slice_info(kind, j, n_vars, cost), pair, nullptr};
+    b->jobs.push_back(jb);
+    ++m->pins;
+    b->invalidate();
+}
+
+bool same_plan(const uc_batch_plan& a, const uc_batch_plan& p) {
+    return a.block == p.block && a.slice_steps == p.slice_steps && a.cells_per_lane == p.cells_per_lane;
+}
+
+// Builds the per-stage, per-group job tables for plan `p`.
+int batch_build(uc_batch* b, const uc_batch_plan& p) {
+    if (b->built.block && same_plan(b->built, p)) return UC_OK;
+    b->invalidate();
+    std::vector<const uc_batch::Job*> order;
+    for (auto& jb : b->jobs) order.push_back(&jb);
+    std::stable_sort(order.begin(), order.end(),
+                     [](const uc_batch::Job* x, const uc_batch::Job* y) { return x->cost > y->cost; });
+    b->stages.clear();
+    for (const uc_batch::Job* jp : order) {
+        uc_batch::Job& jb = const_cast<uc_batch::Job&>(*jp);
+        std::string kernel = jb.kernel;
+        int cpl = 1;
+        long long n_stages = 1, ups = 0;
+        if (p.cells_per_lane == 2 && jb.pair) {
+            kernel = kernel.substr(0, kernel.size() - 2) + "_x2_q";
+            cpl = 2;
+        } else if (p.slice_steps > 0.0 && jb.si.units > 0 && (ups = slice_units(jb.si, p.slice_steps)) < jb.si.units) {
+            kernel = sliced_kernel(kernel);
+            n_stages = (jb.si.units + ups - 1) / ups;
+            if (!jb.slab) {
+                const size_t bytes = sizeof(double) * (size_t)jb.si.words * (size_t)jb.job.n_cells;
+                hipError_t he = hipMalloc((void**)&jb.slab, bytes);
+                if (he != hipSuccess) { jb.slab = nullptr; return b->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(batch slab)")); }
+                b->owned.push_back(jb.slab);
+            }
         }
-        b->groups[it->second].table.push_back(jb.job);
+        if ((long long)b->stages.size() < n_stages) b->stages.resize((size_t)n_stages);
+        for (long long k = 0; k < n_stages; ++k) {
+            UcJob j = jb.job;
+            if (n_stages > 1) {
+                j.slab = jb.slab;
+                j.unit_begin = k * ups;
+                j.unit_end = std::min(jb.si.units, (k + 1) * ups);
+            }
+            auto& st = b->stages[(size_t)k];
+            size_t g = 0;
+            while (g < st.size() && !(st[g].m == jb.m && st[g].kernel == kernel)) ++g;
+            if (g == st.size()) st.push_back(uc_batch::Group{jb.m, kernel, cpl, {}, nullptr, nullptr, 0});
+            st[g].table.push_back(j);
+        }
     }
-    for (auto& g : b->groups) {
-        std::vector<long long> start(g.table.size() + 1, 0);
-        for (size_t i = 0; i < g.table.size(); ++i)
-            start[i + 1] = start[i] + (g.table[i].n_cells + block - 1) / block;
-        g.total_chunks = start.back();
-        hipError_t he = hipMalloc((void**)&g.d_jobs, sizeof(UcJob) * g.table.size());
-        if (he == hipSuccess) { b->owned.push_back(g.d_jobs); he = hipMalloc((void**)&g.d_start, sizeof(long long) * start.size()); }
-        if (he == hipSuccess) { b->owned.push_back(g.d_start); he = hipMemcpy(g.d_jobs, g.table.data(), sizeof(UcJob) * g.table.size(), hipMemcpyHostToDevice); }
-        if (he == hipSuccess) he = hipMemcpy(g.d_start, start.data(), sizeof(long long) * start.size(), hipMemcpyHostToDevice);
-        if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "batch job tables"));
-    }
-    if (!b->d_counters) {
-        hipError_t he = hipMalloc((void**)&b->d_counters, sizeof(unsigned long long) * 64);
-        if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "batch counters"));
+    int n_launch = 0;
+    for (auto& st : b->stages)
+        for (auto& g : st) {
+            std::vector<long long> start(g.table.size() + 1, 0);
+            const long long chunk = (long long)p.block * g.cells_per_chunk_lane;
+            for (size_t i = 0; i < g.table.size(); ++i) start[i + 1] = start[i] + (g.table[i].n_cells + chunk - 1) / chunk;
+            g.total_chunks = start.back();
+            hipError_t he = hipMalloc((void**)&g.d_jobs, sizeof(UcJob) * g.table.size());
+            if (he == hipSuccess) { b->owned.push_back(g.d_jobs); he = hipMalloc((void**)&g.d_start, sizeof(long long) * start.size()); }
+            if (he == hipSuccess) { b->owned.push_back(g.d_start); he = hipMemcpy(g.d_jobs, g.table.data(), sizeof(UcJob) * g.table.size(), hipMemcpyHostToDevice); }
+            if (he == hipSuccess) he = hipMemcpy(g.d_start, start.data(), sizeof(long long) * start.size(), hipMemcpyHostToDevice);
+            if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "batch job tables"));
+            ++n_launch;
+        }
+    if (n_launch > b->n_counters) {
+        hipError_t he = hipMalloc((void**)&b->d_counters, sizeof(unsigned long long) * (size_t)n_launch);
+        if (he != hipSuccess) { b->d_counters = nullptr; b->n_counters = 0; return b->fail(UC_ERR_HIP, hip_msg(he, "batch counters")); }
         b->owned.push_back(b->d_counters);
+        b->n_counters = n_launch;
     }
-    if (b->groups.size() > 64) return b->fail(UC_ERR_ARGUMENT, "more than 64 kernel groups in one batch");
-    b->built_block = block;
+    b->built = p;
     return UC_OK;
 }
 
-// Enqueues the reset kernel and one persistent launch per group; groups
-// run concurrently on auxiliary streams joined back into `st`.
-int batch_enqueue(uc_batch* b, hipStream_t st, int block, int wps) {
+// Enqueues the reset kernel, then each stage: one persistent launch per
+// group, the groups concurrently on auxiliary streams joined back into `st`
+// before the next stage.
+int batch_enqueue(uc_batch* b, hipStream_t st, const uc_batch_plan& p) {
     const int cus = cu_count(b->e->device);
-    const int waves_per_wg = block / 64;
-    long long wgs = (long long)cus * 4 * wps / waves_per_wg;
+    const int waves_per_wg = p.block / 64;
+    long long wgs = (long long)cus * 4 * p.waves_per_simd / waves_per_wg;
     if (wgs < 1) wgs = 1;
+    size_t max_groups = 0, n_launch = 0;
+    for (auto& s : b->stages) { max_groups = std::max(max_groups, s.size()); n_launch += s.size(); }
     hipFunction_t freset;
-    hipError_t he = hipModuleGetFunction(&freset, b->groups[0].m->mod, "uc_batch_reset");
-    int ng = (int)b->groups.size();
-    void* rargs[] = {&b->d_counters, &ng};
-    if (he == hipSuccess) he = hipModuleLaunchKernel(freset, 1, 1, 1, 64, 1, 1, 0, st, rargs, nullptr);
+    hipError_t he = hipModuleGetFunction(&freset, b->jobs[0].m->mod, "uc_batch_reset");
+    int nc = (int)n_launch;
+    void* rargs[] = {&b->d_counters, &nc};
+    if (he == hipSuccess) he = hipModuleLaunchKernel(freset, (unsigned)((nc + 63) / 64), 1, 1, 64, 1, 1, 0, st, rargs, nullptr);
     if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "batch reset"));
-    while ((int)b->aux.size() < ng - 1) {
+    while (b->aux.size() + 1 < max_groups) {
         hipStream_t s2;
         if ((he = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "aux stream"));
         b->aux.push_back(s2);
     }
-    while ((int)b->events.size() < 2 * ng) {
+    while (b->events.size() < 2 * n_launch) {
         hipEvent_t ev;
         if ((he = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "event"));
         b->events.push_back(ev);
     }
-    if (ng > 1 && (he = hipEventRecord(b->events[0], st)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "fork"));
-    for (int g = 0; g < ng; ++g) {
-        auto& G = b->groups[(size_t)g];
-        hipStream_t gs = g == 0 ? st : b->aux[(size_t)g - 1];
-        if (g > 0 && (he = hipStreamWaitEvent(gs, b->events[0], 0)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "fork wait"));
-        hipFunction_t f;
-        if ((he = hipModuleGetFunction(&f, G.m->mod, G.kernel.c_str())) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, G.kernel.c_str()));
-        int nj = (int)G.table.size();
-        unsigned long long* ctr = b->d_counters + g;
-        void* args[] = {&G.d_jobs, &G.d_start, &nj, &ctr};
-        const long long grid = std::min<long long>(wgs, std::max<long long>(1, G.total_chunks));
-        if ((he = hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)block, 1, 1, 0, gs, args, nullptr)) != hipSuccess)
-            return b->fail(UC_ERR_HIP, hip_msg(he, "persistent launch"));
-        if (g > 0) {
-            if ((he = hipEventRecord(b->events[(size_t)(ng + g)], gs)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "join"));
-            if ((he = hipStreamWaitEvent(st, b->events[(size_t)(ng + g)], 0)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "join wait"));
+    size_t ctr = 0, evi = 0;
+    for (auto& stage : b->stages) {
+        const int ng = (int)stage.size();
+        hipEvent_t fork = b->events[evi++];
+        if (ng > 1 && (he = hipEventRecord(fork, st)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "fork"));
+        for (int g = 0; g < ng; ++g) {
+            auto& G = stage[(size_t)g];
+            hipStream_t gs = g == 0 ? st : b->aux[(size_t)g - 1];
+            if (g > 0 && (he = hipStreamWaitEvent(gs, fork, 0)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "fork wait"));
+            hipFunction_t f;
+            if ((he = hipModuleGetFunction(&f, G.m->mod, G.kernel.c_str())) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, G.kernel.c_str()));
+            int nj = (int)G.table.size();
+            unsigned long long* c = b->d_counters + ctr++;
+            void* args[] = {&G.d_jobs, &G.d_start, &nj, &c};
+            const long long grid = std::min<long long>(wgs, std::max<long long>(1, G.total_chunks));
+            if ((he = hipModuleLaunchKernel(f, (unsigned)grid, 1, 1, (unsigned)p.block, 1, 1, 0, gs, args, nullptr)) != hipSuccess)
+                return b->fail(UC_ERR_HIP, hip_msg(he, "persistent launch"));
+            if (g > 0) {
+                hipEvent_t join = b->events[evi++];
+                if ((he = hipEventRecord(join, gs)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "join"));
+                if ((he = hipStreamWaitEvent(st, join, 0)) != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "join wait"));
+            }
         }
     }
+    return UC_OK;
+}
+
+int check_plan(uc_batch* b, const uc_batch_plan& p) {
+    if (p.block != 64 && p.block != 128 && p.block != 256) return b->fail(UC_ERR_ARGUMENT, "block must be 64, 128 or 256");
+    if (p.waves_per_simd < 1 || p.waves_per_simd > 8) return b->fail(UC_ERR_ARGUMENT, "waves_per_simd must be in [1, 8]");
+    if (!(p.slice_steps >= 0.0)) return b->fail(UC_ERR_ARGUMENT, "slice_steps must be >= 0");
+    if (p.cells_per_lane != 1 && p.cells_per_lane != 2) return b->fail(UC_ERR_ARGUMENT, "cells_per_lane must be 1 or 2");
     return UC_OK;
 }
 
@@ -2188,9 +2303,8 @@ int uc_batch_add_bif(uc_batch* b, uc_system* s, const uc_bif_request* r, const u
     if ((rc = batch_stage(b, s, r->c, j))) return rc;
     fill_bif(r, o, iters, j);
     const double cost = (double)(j.skip_steps + iters * r->pre_scaller);
-    b->jobs.push_back({m, r->c.precision == UC_PREC_F64 ? "uc_bif_f64_q" : "uc_bif_f32c_q", j, cost});
-    ++m->pins;
-    b->built_block = -1;
+    batch_add(b, m, r->c.precision == UC_PREC_F64 ? "uc_bif_f64_q" : "uc_bif_f32c_q", j, cost, kKindBif, s->n_vars,
+              false);
     if (job_id) *job_id = (int)b->jobs.size() - 1;
     return UC_OK;
 }
@@ -2211,9 +2325,14 @@ int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int s
     if ((rc = batch_stage(b, s, r->c, j))) return rc;
     fill_lyap(r, o, j);
     const double cost = (double)(j.skip_steps + j.n_iters * (j.nt_steps + 1)) * (spectrum ? 1.0 + s->n_vars : 2.0);
-    b->jobs.push_back({m, std::string(lyap_kernel(spectrum != 0, r->c.precision)) + "_q", j, cost});
-    ++m->pins;
-    b->built_block = -1;
+    // two cells per lane: the fast single-vector variational map with
+    // wave-uniform counts and blocks of 1..100 steps (uc_lle_pair)
+    bool step_axis = false;
+    for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
+    const bool pair = !spectrum && !r->c.strict && r->method == UC_METHOD_VARIATIONAL && !step_axis &&
+                      j.nt_steps >= 1 && j.nt_steps <= 100;
+    batch_add(b, m, std::string(lyap_kernel(spectrum != 0, r->c.precision)) + "_q", j, cost,
+              spectrum ? kKindLs : kKindLle, s->n_vars, pair);
     if (job_id) *job_id = (int)b->jobs.size() - 1;
     return UC_OK;
 }
@@ -2245,26 +2364,45 @@ int uc_batch_add_basins(uc_batch* b, uc_system* s, const uc_basins_request* r, c
         b->owned.push_back(points);
     }
     j.out0 = points;
-    // ordering weight only (the integrator is adaptive): ~64 steps per section
-    b->jobs.push_back({m, c.precision == UC_PREC_F64 ? "uc_bas_f64_q" : "uc_bas_f32c_q", j, 64.0 * r->n_sections});
+    batch_add(b, m, c.precision == UC_PREC_F64 ? "uc_bas_f64_q" : "uc_bas_f32c_q", j,
+              basins_section_cost(c.precision) * r->n_sections, kKindBasins, s->n_vars, false);
     b->basins.push_back({m, j, r->quantum, *o});
-    ++m->pins;
-    b->built_block = -1;
     if (job_id) *job_id = (int)b->jobs.size() - 1;
     return UC_OK;
 }
 
 int uc_batch_set_launch(uc_batch* b, int block, int waves_per_simd) {
     if (!b) return UC_ERR_ARGUMENT;
-    if (block != 64 && block != 128 && block != 256) return b->fail(UC_ERR_ARGUMENT, "block must be 64, 128 or 256");
-    if (waves_per_simd < 1 || waves_per_simd > 8) return b->fail(UC_ERR_ARGUMENT, "waves_per_simd must be in [1, 8]");
-    if (b->block != block || b->waves_per_simd != waves_per_simd) b->drop_graph();
-    b->block = block;
-    b->waves_per_simd = waves_per_simd;
+    uc_batch_plan p = b->plan;
+    p.block = block;
+    p.waves_per_simd = waves_per_simd;
+    return uc_batch_set_plan(b, &p);
+}
+
+int uc_batch_set_plan(uc_batch* b, const uc_batch_plan* p) {
+    if (!b || !p) return UC_ERR_ARGUMENT;
+    std::lock_guard<std::recursive_mutex> lk(b->e->mu);
+    int rc = check_plan(b, *p);
+    if (rc) return rc;
+    if (p->block != b->plan.block || p->waves_per_simd != b->plan.waves_per_simd ||
+        p->slice_steps != b->plan.slice_steps || p->cells_per_lane != b->plan.cells_per_lane)
+        b->drop_graph();
+    b->plan = *p;
+    b->plan.ms = 0.0;
     return UC_OK;
 }
 
-int uc_batch_tune(uc_batch* b, void* stream, int* best_block, int* best_wps, double* best_ms) {
+int uc_batch_get_plan(const uc_batch* b, uc_batch_plan* p) {
+    if (!b || !p) return UC_ERR_ARGUMENT;
+    *p = b->plan;
+    return UC_OK;
+}
+
+// Times the batch (eagerly, without the graph) for each candidate plan and
+// keeps the fastest: block {64, 256} x waves per SIMD {2, 4, 8} with whole
+// runs, then at the best shape the steps per launch {0, 16384, 4096} and,
+// when a job can run two cells per lane, cells per lane {1, 2}.
+int uc_batch_tune_plan(uc_batch* b, void* stream, uc_batch_plan* best_out) {
     if (!b) return UC_ERR_ARGUMENT;
     std::lock_guard<std::recursive_mutex> lk(b->e->mu);
     if (b->jobs.empty()) return b->fail(UC_ERR_ARGUMENT, "empty batch");
@@ -2273,30 +2411,46 @@ int uc_batch_tune(uc_batch* b, void* stream, int* best_block, int* best_wps, dou
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t t0, t1;
     if (hipEventCreate(&t0) != hipSuccess || hipEventCreate(&t1) != hipSuccess) return b->fail(UC_ERR_HIP, "event create");
-    const int blocks[] = {64, 256};
-    const int wpss[] = {2, 4, 8};
-    double best = 1e30;
-    int bb = 256, bw = 4;
-    for (int blk : blocks)
-        for (int w : wpss) {
-            if ((rc = batch_build(b, blk))) break;
-            (void)hipEventRecord(t0, st);
-            if ((rc = batch_enqueue(b, st, blk, w))) break;
-            (void)hipEventRecord(t1, st);
-            if (hipEventSynchronize(t1) != hipSuccess) { rc = b->fail(UC_ERR_HIP, "tune sync"); break; }
-            float ms = 0.0f;
-            (void)hipEventElapsedTime(&ms, t0, t1);
-            if (ms < best) { best = ms; bb = blk; bw = w; }
-        }
+    uc_batch_plan best = b->plan;
+    best.ms = 1e30;
+    auto time_plan = [&](const uc_batch_plan& p) -> int {
+        int r2 = batch_build(b, p);
+        if (r2) return r2;
+        (void)hipEventRecord(t0, st);
+        if ((r2 = batch_enqueue(b, st, p))) return r2;
+        (void)hipEventRecord(t1, st);
+        if (hipEventSynchronize(t1) != hipSuccess) return b->fail(UC_ERR_HIP, "tune sync");
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        if (ms < best.ms) { best = p; best.ms = ms; }
+        return UC_OK;
+    };
+    for (int blk : {64, 256})
+        for (int w : {2, 4, 8})
+            if (!rc) rc = time_plan(uc_batch_plan{blk, w, 0.0, 1, 0.0});
+    bool any_pair = false, any_slice = false;
+    for (auto& jb : b->jobs) { any_pair |= jb.pair; any_slice |= jb.si.units > 0; }
+    const uc_batch_plan shape = best;
+    if (any_slice)
+        for (double sl : {16384.0, 4096.0})
+            if (!rc) rc = time_plan(uc_batch_plan{shape.block, shape.waves_per_simd, sl, 1, 0.0});
+    if (any_pair && !rc) rc = time_plan(uc_batch_plan{shape.block, shape.waves_per_simd, 0.0, 2, 0.0});
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (rc) return rc;
     b->drop_graph();
-    b->block = bb;
-    b->waves_per_simd = bw;
-    if (best_block) *best_block = bb;
-    if (best_wps) *best_wps = bw;
-    if (best_ms) *best_ms = best;
+    b->plan = best;
+    if (best_out) *best_out = best;
+    return UC_OK;
+}
+
+int uc_batch_tune(uc_batch* b, void* stream, int* best_block, int* best_wps, double* best_ms) {
+    uc_batch_plan p{};
+    const int rc = uc_batch_tune_plan(b, stream, &p);
+    if (rc) return rc;
+    if (best_block) *best_block = p.block;
+    if (best_wps) *best_wps = p.waves_per_simd;
+    if (best_ms) *best_ms = p.ms;
     return UC_OK;
 }
 
@@ -2306,14 +2460,14 @@ int uc_batch_launch(uc_batch* b, void* stream) {
     if (b->jobs.empty()) return b->fail(UC_ERR_ARGUMENT, "empty batch");
     int rc = enter(b->e);
     if (rc) return rc;
-    const int block = b->block ? b->block : 256, wps = b->waves_per_simd ? b->waves_per_simd : 4;
     hipStream_t st = (hipStream_t)stream;
     if (!st) return b->fail(UC_ERR_ARGUMENT, "batch launch needs a non-default stream (graph capture)");
-    if ((rc = batch_build(b, block))) return rc;
+    const uc_batch_plan p = b->plan;
+    if ((rc = batch_build(b, p))) return rc;
     if (!b->exec) {
         hipError_t he = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (he != hipSuccess) return b->fail(UC_ERR_HIP, hip_msg(he, "hipStreamBeginCapture"));
-        rc = batch_enqueue(b, st, block, wps);
+        rc = batch_enqueue(b, st, p);
         hipGraph_t g = nullptr;
         he = hipStreamEndCapture(st, &g);
         if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -2329,8 +2483,8 @@ int uc_batch_launch(uc_batch* b, void* stream) {
     for (auto& bj : b->basins) {
         BasinsCatalogue cat;
         const std::string why = basins_catalogue(
-            bj.m, bj.job, bj.quantum, st, [](void** p, size_t n) { return hipMalloc(p, n); },
-            [](void* p) { (void)hipFree(p); }, cat);
+            bj.m, bj.job, bj.quantum, st, [](void** p2, size_t n) { return hipMalloc(p2, n); },
+            [](void* p2) { (void)hipFree(p2); }, cat);
         if (!why.empty()) return b->fail(why.rfind("too many", 0) == 0 ? UC_ERR_ARGUMENT : UC_ERR_HIP, why);
         copy_catalogue(cat, &bj.out);
     }
@@ -2345,8 +2499,8 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
         for (auto& j : b->jobs) g[std::make_pair(j.m, j.kernel)] = 1;
         *n_groups = (int)g.size();
     }
-    if (block) *block = b->block ? b->block : 256;
-    if (waves_per_simd) *waves_per_simd = b->waves_per_simd ? b->waves_per_simd : 4;
+    if (block) *block = b->plan.block;
+    if (waves_per_simd) *waves_per_simd = b->plan.waves_per_simd;
     return UC_OK;
 }
 
@@ -2387,8 +2541,16 @@ struct uc_server {
         size_t out0_n = 0, out1_n = 0;
         size_t bytes = 0;
         long long cells = 0;
-        long long next = 0;        // cells handed to rounds
-        long long done_cells = 0;  // cells of completed rounds
+        long long next = 0;        // cells handed to rounds (first units)
+        long long done_cells = 0;  // cells whose last units are done
+        // resumable slices: the sliced kernel, its slab, the units per round
+        // and the cell ranges waiting for their next units
+        SliceInfo si;
+        long long ups = 0;         // units per round; 0: whole runs
+        double* slab = nullptr;
+        size_t slab_reserved = 0;  // slab bytes counted at submission
+        struct Range { long long first, cells, unit; };
+        std::deque<Range> resume;
         int inflight = 0;          // rounds in flight that hold cells of this job
         double step_cost = 1.0;    // traj-steps per cell
         int state = 0;             // 0 queued, 1 running, 2 done, 3 cancelled, 4 failed, 5 fetched
@@ -2413,6 +2575,7 @@ struct uc_server {
     struct Entry {
         std::shared_ptr<Job> job;
         long long first, cells;
+        long long u0 = 0, u1 = 0;  // units of this round (sliced jobs)
     };
     struct Slot {
         hipStream_t stream = nullptr;
@@ -2434,6 +2597,7 @@ struct uc_server {
     std::map<long long, std::shared_ptr<Job>> all;
     long long next_id = 1;
     size_t used_bytes = 0, budget = 0;
+    double slice_steps = 0.0;  // traj-steps per cell per round of a sliced job; 0: whole runs
     bool stop = false;
     bool async_alloc = false;
     hipMemPool_t pool = nullptr;   // the server's own pool; keeps up to the budget mapped
@@ -2462,6 +2626,8 @@ void server_release(uc_server* S, void* p, hipStream_t s) {
 
 // Frees a job's device buffers; no round in flight may hold its cells.
 void server_free(uc_server* S, uc_server::Job& j) {
+    server_release(S, j.slab, S->stream);
+    j.slab = nullptr;
     server_release(S, j.flags, S->stream);
     server_release(S, j.out0, S->stream);
     server_release(S, j.out1, S->stream);
@@ -2485,8 +2651,7 @@ bool server_admit(uc_server* S, uc_server::Job& j) {
     if (he == hipSuccess && j.kind == 3) he = server_malloc(S, (void**)&j.labels, sizeof(int) * (size_t)j.cells, S->stream);
     if (he == hipSuccess && j.kind == 3)
         he = server_malloc(S, (void**)&j.steps, 2 * sizeof(long long) * (size_t)j.cells, S->stream);
-    if (he == hipSuccess) he = hipStreamSynchronize(S->stream);  // tables uploaded, buffers usable on any stream
-    if (he != hipSuccess) { server_free(S, j); j.error = hip_msg(he, "admission"); return false; }
+    if (he != hipSuccess) { (void)hipStreamSynchronize(S->stream); server_free(S, j); j.error = hip_msg(he, "admission"); return false; }
     wire_job(j.c, j.tables, j.tmpl);
     if (j.kind == 3) {
         fill_basins(&j.bas, j.lbl, j.tmpl);
@@ -2501,6 +2666,20 @@ bool server_admit(uc_server* S, uc_server::Job& j) {
         uc_lyap_output o{j.out0, j.flags, nullptr};
         fill_lyap(&j.lyap, &o, j.tmpl);
     }
+    // resumable slices when the job is longer than one round's steps (its
+    // slab was counted in j.bytes at submission; given back otherwise)
+    j.si = slice_info(j.kind, j.tmpl, j.n_vars, j.step_cost);
+    const size_t slab_bytes = sizeof(double) * (size_t)j.si.words * (size_t)j.cells;
+    if (S->slice_steps > 0.0 && j.si.units > 0 && slice_units(j.si, S->slice_steps) < j.si.units) {
+        j.ups = slice_units(j.si, S->slice_steps);
+        j.kernel = sliced_kernel(j.kernel);
+        he = server_malloc(S, (void**)&j.slab, slab_bytes, S->stream);
+    } else if (j.slab_reserved) {
+        j.bytes -= j.slab_reserved;
+    }
+    j.slab_reserved = 0;
+    if (he == hipSuccess) he = hipStreamSynchronize(S->stream);  // tables uploaded, buffers usable on any stream
+    if (he != hipSuccess) { server_free(S, j); j.error = hip_msg(he, "admission"); return false; }
     return true;
 }
 
@@ -2523,8 +2702,13 @@ bool slot_reserve(uc_server* S, uc_server::Slot& sl, size_t n_entries) {
     return true;
 }
 
-// Duration class of a job's cells: the decade of its per-cell step count.
-int server_class(const uc_server::Job& J) { return (int)std::floor(std::log10(std::max(1.0, J.step_cost))); }
+// Traj-steps per cell of one round of the job: the whole run, or one slice.
+double round_cost(const uc_server::Job& J) {
+    return J.ups ? std::min(J.step_cost, (double)J.ups * J.si.unit_cost) : J.step_cost;
+}
+// Duration class of a job's rounds: the decade of their per-cell step count.
+int server_class(const uc_server::Job& J) { return (int)std::floor(std::log10(std::max(1.0, round_cost(J)))); }
+bool server_has_work(const uc_server::Job& J) { return J.state == 1 && (J.next < J.cells || !J.resume.empty()); }
 
 // Picks the cells of the next round (under S->mu) and returns its duration
 // class in *cls_out. A round holds cells of one duration class only, so it
@@ -2541,7 +2725,7 @@ std::vector<uc_server::Entry> server_pick(uc_server* S, long long lanes, int blo
     std::vector<uc_server::Job*> open;
     int cls = -1, lowest = -1;
     for (auto& jp : S->active) {
-        if (jp->state != 1 || jp->next >= jp->cells) continue;
+        if (!server_has_work(*jp)) continue;
         const int c = server_class(*jp);
         if (!allow_long && c >= uc_server::kLongClass) continue;
         if (lowest < 0 || c < lowest) lowest = c;
@@ -2555,34 +2739,55 @@ std::vector<uc_server::Entry> server_pick(uc_server* S, long long lanes, int blo
     S->last_class = cls;
     *cls_out = cls;
     for (auto& jp : S->active)
-        if (jp->state == 1 && jp->next < jp->cells && server_class(*jp) == cls) open.push_back(jp.get());
+        if (server_has_work(*jp) && server_class(*jp) == cls) open.push_back(jp.get());
     const double cap_steps = S->cfg.round_traj_steps > 0 ? S->cfg.round_traj_steps : 0.0;
     std::map<uc_server::Job*, long long> take;
     std::map<std::pair<Module*, std::string>, int> groups;
     long long cells = 0;
     double steps = 0.0;
     const long long share = std::max<long long>(block, (lanes / (long long)open.size()) / block * block);
+    auto available = [](const uc_server::Job* J) {
+        long long n = J->cells - J->next;
+        for (auto& r : J->resume) n += r.cells;
+        return n;
+    };
     for (int pass = 0; pass < 2 && cells < lanes; ++pass) {
         for (uc_server::Job* J : open) {
             if (cells >= lanes || (cap_steps > 0.0 && steps >= cap_steps)) break;
             const auto g = std::make_pair(J->m, J->kernel);
             if (!groups.count(g) && (int)groups.size() >= uc_server::kMaxGroups) continue;
-            const long long left = J->cells - J->next - take[J];
+            const long long left = available(J) - take[J];
+            const double cost = round_cost(*J);
             long long n = std::min(left, pass == 0 ? share : lanes - cells);
-            if (cap_steps > 0.0) n = std::min<long long>(n, std::max<long long>(1, (long long)((cap_steps - steps) / J->step_cost)));
+            if (cap_steps > 0.0) n = std::min<long long>(n, std::max<long long>(1, (long long)((cap_steps - steps) / cost)));
             if (n <= 0) continue;
             take[J] += n;
             cells += n;
-            steps += (double)n * J->step_cost;
+            steps += (double)n * cost;
             groups[g] = 1;
         }
     }
+    // cells waiting for their next units first, then new cells
     for (auto& jp : S->active) {
         auto it = take.find(jp.get());
         if (it == take.end() || it->second <= 0) continue;
-        out.push_back({jp, jp->next, it->second});
-        jp->next += it->second;
-        ++jp->inflight;
+        uc_server::Job& J = *jp;
+        long long n = it->second;
+        while (n > 0 && !J.resume.empty()) {
+            auto& r = J.resume.front();
+            const long long k = std::min(n, r.cells);
+            out.push_back({jp, r.first, k, r.unit, std::min(J.si.units, r.unit + J.ups)});
+            r.first += k;
+            r.cells -= k;
+            n -= k;
+            if (r.cells == 0) J.resume.pop_front();
+            ++J.inflight;
+        }
+        if (n > 0) {
+            out.push_back({jp, J.next, n, 0, J.ups ? std::min(J.si.units, J.ups) : 0});
+            J.next += n;
+            ++J.inflight;
+        }
     }
     return out;
 }
@@ -2613,6 +2818,11 @@ std::string server_launch(uc_server* S, uc_server::Slot& sl, int block, int wps,
             j.out0 = en->job->out0 + en->first;
             j.out1 = en->job->out1 ? en->job->out1 + en->first : nullptr;
             j.counters = en->job->steps ? en->job->steps + en->first : nullptr;
+            if (en->job->ups) {
+                j.slab = en->job->slab + en->first;
+                j.unit_begin = en->u0;
+                j.unit_end = en->u1;
+            }
             sl.h_jobs[jo++] = j;
             g.chunks += (en->cells + block - 1) / block;
             sl.h_start[so++] = g.chunks;
@@ -2653,8 +2863,15 @@ void server_retire(uc_server* S, uc_server::Slot& sl, const std::string& why) {
             if (J.state == 1) { J.state = 4; J.error = why; J.t_done = now; }
             continue;
         }
+        // fixed-step work only
+        if (J.kind != 3)
+            S->traj_steps_done += J.ups ? (double)en.cells * (double)(en.u1 - en.u0) * J.si.unit_cost
+                                        : (double)en.cells * J.step_cost;
+        if (J.ups && en.u1 < J.si.units) {  // more units to run: back in the queue
+            if (J.state == 1) J.resume.push_back({en.first, en.cells, en.u1});
+            continue;
+        }
         J.done_cells += en.cells;
-        if (J.kind != 3) S->traj_steps_done += (double)en.cells * J.step_cost;  // fixed-step work only
         if (J.state == 1 && J.done_cells >= J.cells) {
             if (J.kind == 3) {
                 UcJob g = J.tmpl;  // whole grid
@@ -2791,6 +3008,22 @@ int server_submit(uc_server* S, std::shared_ptr<uc_server::Job> j, long long* id
     return UC_OK;
 }
 
+// Counts the slab of a job that may run in slices (no sweep over h) in its
+// bytes; admission gives the bytes back when the job runs whole.
+void server_reserve_slab(uc_server* S, uc_server::Job& j) {
+    bool step_axis = false;
+    for (int i = 0; i < j.c.n_axes; ++i) step_axis |= j.c.axis[i].kind == UC_AXIS_STEP;
+    // sliced exactly when one round's units are fewer than the job's, i.e.
+    // when the job's steps per cell exceed the round's
+    if (S->slice_steps <= 0.0 || step_axis || j.step_cost <= S->slice_steps) return;
+    const int n = j.n_vars;
+    const int words = j.kind == kKindBif ? UC_SLAB_BIF_WORDS(n)
+                      : j.kind == kKindBasins ? UC_SLAB_BAS_WORDS(n)
+                                              : UC_SLAB_LYAP_WORDS(n, j.kind == kKindLs ? n : 1);
+    j.slab_reserved = sizeof(double) * (size_t)words * (size_t)j.cells;
+    j.bytes += j.slab_reserved;
+}
+
 // copy the request so the job does not depend on caller memory
 void server_copy_common(uc_server::Job& j, const uc_common& c, int n_vars) {
     j.c = c;
@@ -2819,6 +3052,7 @@ int uc_server_create(uc_engine* e, const uc_server_config* cfg, uc_server** out)
     (void)hipMemGetInfo(&free_b, &total_b);
     const double frac = S->cfg.hbm_fraction > 0.0 ? S->cfg.hbm_fraction : 0.5;
     S->budget = S->cfg.max_bytes > 0 ? (size_t)S->cfg.max_bytes : (size_t)(frac * (double)free_b);
+    S->slice_steps = S->cfg.slice_steps == 0.0 ? 4096.0 : (S->cfg.slice_steps > 0.0 ? S->cfg.slice_steps : 0.0);
     int pools = 0;
     S->async_alloc = hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, e->device) == hipSuccess &&
                      pools != 0;
@@ -2912,6 +3146,7 @@ int uc_server_submit_lyap(uc_server* S, uc_system* s, const uc_lyap_request* r, 
     const long long skip = r->c.h > 0 ? (long long)(r->c.transient_time / r->c.h) : 0;
     const long long blocks = (long long)(r->c.t_max / r->NT), nts = r->c.h > 0 ? (long long)(r->NT / r->c.h) : 1;
     j->step_cost = (double)(skip + 1 + blocks * (nts + 1));
+    server_reserve_slab(S, *j);
     return server_submit(S, j, id);
 }
 
@@ -2944,6 +3179,7 @@ int uc_server_submit_bif(uc_server* S, uc_system* s, const uc_bif_request* r, lo
     j->bytes = 2 * j->out0_n * sizeof(double) + (size_t)j->cells * sizeof(int);
     const long long skip = r->c.h > 0 ? (long long)(r->c.transient_time / r->c.h) : 0;
     j->step_cost = (double)(skip + 1 + iters * r->pre_scaller + r->pre_scaller);
+    server_reserve_slab(S, *j);
     return server_submit(S, j, id);
 }
 
@@ -2968,8 +3204,9 @@ int uc_server_submit_basins(uc_server* S, uc_system* s, const uc_basins_request*
     j->kernel = c.precision == UC_PREC_F64 ? "uc_bas_f64_q" : "uc_bas_f32c_q";
     j->out0_n = (size_t)j->lbl * (size_t)j->cells;
     j->bytes = (size_t)j->cells * (2 * sizeof(int) + 2 * sizeof(long long) + (size_t)j->lbl * sizeof(double));
-    // scheduling weight only (the integrator is adaptive): ~64 steps per section
-    j->step_cost = 64.0 * (double)r->n_sections;
+    // scheduling weight only (the integrator is adaptive)
+    j->step_cost = basins_section_cost(c.precision) * (double)r->n_sections;
+    server_reserve_slab(S, *j);
     return server_submit(S, j, id);
 }
 

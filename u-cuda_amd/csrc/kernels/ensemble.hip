@@ -129,6 +129,76 @@ struct StateF32C {
     }
 };
 
+// ------------------------------------------------------ resumable slices --
+// A sliced cell (UcJob.slab set) keeps its state in the slab between
+// launches: word w of the cell at slab[w * stride + lane]. Values are stored
+// as bit patterns, so every one round-trips exactly, and saved and loaded by
+// the same sequence of calls; words past the kind's count are dropped.
+// Word 0 is the done mark: a cell whose results are written is skipped by
+// later launches.
+static_assert(UC_UNIT_STEPS == UC_CHECK_INTERVAL, "work units are magnitude-check intervals");
+#define UC_UNITS_ALL (1LL << 50)  // unit_end of a whole run
+
+struct UcSlab {
+    double* p;
+    long long stride;
+    int w, cap;
+    bool store;
+    __device__ __forceinline__ UcSlab(const UcJob& j, long long lane, int cap_, bool store_)
+        : p(j.slab + lane), stride(j.stride), w(0), cap(cap_), store(store_) {}
+    __device__ __forceinline__ void seek(int w_) { w = w_; }
+    __device__ __forceinline__ void u64w(u64& v) {
+        if (w < cap) {
+            double* q = p + (long long)w * stride;
+            if (store) *q = __builtin_bit_cast(double, v);
+            else v = __builtin_bit_cast(u64, *q);
+        }
+        ++w;
+    }
+    __device__ __forceinline__ void f64(double& v) { u64 b = __builtin_bit_cast(u64, v); u64w(b); v = __builtin_bit_cast(double, b); }
+    __device__ __forceinline__ void f32(float& v) {
+        u64 b = __builtin_bit_cast(unsigned int, v);
+        u64w(b);
+        v = __builtin_bit_cast(float, (unsigned int)b);
+    }
+    __device__ __forceinline__ void val(double& v) { f64(v); }
+    __device__ __forceinline__ void val(float& v) { f32(v); }
+    __device__ __forceinline__ void i64(long long& v) { u64 b = (u64)v; u64w(b); v = (long long)b; }
+    __device__ __forceinline__ void i32(int& v) { long long b = v; i64(b); v = (int)b; }
+    __device__ __forceinline__ void b8(bool& v) { int b = v ? 1 : 0; i32(b); v = b != 0; }
+    __device__ __forceinline__ void st(StateF64& s) {
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) f64(s.x[i]);
+    }
+    __device__ __forceinline__ void st(StateF32C& s) {
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) f32(s.hi[i]);
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) f32(s.c[i]);
+    }
+};
+template <class S> struct UcWords { static constexpr int n = 2 * UC_N; };
+template <> struct UcWords<StateF64> { static constexpr int n = UC_N; };
+
+// Units of a chunked phase of `steps` steps (at least one: the trailing step).
+__device__ __forceinline__ long long uc_chunks(long long steps) {
+    return steps <= 0 ? 1 : (steps + UC_CHECK_INTERVAL - 1) / UC_CHECK_INTERVAL;
+}
+// Steps [i0, i1) of such a phase, starting at unit `base`, that launch units
+// [u0, u1) cover; `last`: the launch reaches the phase's end (trailing step).
+struct UcRange {
+    long long i0, i1;
+    bool last;
+};
+__device__ __forceinline__ UcRange uc_range(long long steps, long long base, long long u0, long long u1) {
+    UcRange r;
+    const long long a = u0 > base ? u0 - base : 0, b = u1 - base;
+    r.i0 = a * UC_CHECK_INTERVAL;
+    r.i1 = b * UC_CHECK_INTERVAL < steps ? b * UC_CHECK_INTERVAL : steps;
+    r.last = b >= uc_chunks(steps);
+    return r;
+}
+
 #define UC_LN2 0.6931471805599453
 __device__ __forceinline__ float uc_ldexp(float x, int e) { return __builtin_ldexpf(x, e); }
 __device__ __forceinline__ double uc_ldexp(double x, int e) { return __builtin_ldexp(x, e); }
@@ -500,15 +570,18 @@ __device__ __forceinline__ void uc_iter(S& s, const typename Prec<S>::T* a, type
 
 __device__ __forceinline__ bool uc_wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0ULL; }
 
+// Iterations [i0, iters) (i0 a multiple of the check interval, so the
+// checks fall where a run from 0 has them), then, when `trail`, the trailing
+// steps and the fixed-point test; without it 1 (alive) or 0.
 template <class S, class Obs>
-__device__ int uc_loop_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t, long long iters,
-                         int pre, int wvar, double max_value, Obs& obs) {
+__device__ int uc_loop_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t, long long i0,
+                         long long iters, bool trail, int pre, int wvar, double max_value, Obs& obs) {
     typedef typename Prec<S>::T T;
     const double hd = (double)h;
     bool dead = false;
     S snap = s;
     double tsnap = t;
-    for (long long i = 0; i < iters;) {
+    for (long long i = i0; i < iters;) {
         uc_iter(s, a, h, t, i, pre, wvar, dead, obs);
         if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
         if (uc_wave_all(dead)) break;
@@ -517,11 +590,18 @@ __device__ int uc_loop_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T
         for (++i; i < stop; ++i) uc_iter(s, a, h, t, i, pre, wvar, dead, obs);
     }
     if (dead) { s = snap; t = tsnap; return 0; }
+    if (!trail) return 1;
     double prev[UC_N];
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
     for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += hd; }
     return uc_fixed_point_flag(s, prev, hd, pre);
+}
+
+template <class S, class Obs>
+__device__ __forceinline__ int uc_loop_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t,
+                                         long long iters, int pre, int wvar, double max_value, Obs& obs) {
+    return uc_loop_u(s, a, h, t, 0, iters, true, pre, wvar, max_value, obs);
 }
 
 template <int K, class S>
@@ -657,36 +737,78 @@ struct PeakObserver {
 // ================================================================ kernels ==
 // The recorded run of a bifurcation cell (after a transient that ended with
 // `flag`): streaming peaks, the cell's flag, the optional final state.
-template <class S>
+// Sliced: record units start at unit TU; an unfinished record saves the
+// cell (done, flag, t, state) and the observer from word 3 + 2N.
+template <bool SL, class S>
 __device__ __forceinline__ void uc_bif_record(const UcJob& j, long long lane, S& s, const typename Prec<S>::T* a,
                                               typename Prec<S>::T h, double hd, double& t, bool uni, long long iters,
-                                              int flag) {
+                                              int flag, long long u0, long long u1, long long TU) {
+    constexpr int kCap = UC_SLAB_BIF_WORDS(UC_N);
+    constexpr bool sliced = SL;
     PeakObserver obs;
     obs.w0 = 0.0; obs.w1 = 0.0; obs.n = iters; obs.c_next = 2; obs.mode = 0; obs.raw = 0;
     obs.last_t = 0.0; obs.time_step = hd * (double)j.pre_scaller; obs.max_peaks = j.max_peaks;
     obs.stride = j.stride; obs.peaks = j.out0 + lane; obs.intervals = j.out1 ? j.out1 + lane : nullptr;
     obs.traj = j.traj ? j.traj + lane : nullptr; obs.traj_len = j.traj ? j.traj_len : 0;
-    if (flag == 1 || flag == -1)
-        flag = uni ? uc_loop_u(s, a, h, t, j.n_iters, j.pre_scaller, j.writable_var, j.max_value, obs)
-                   : uc_loop(s, a, h, t, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
+    auto cell_state = [&](bool store) {
+        UcSlab io(j, lane, kCap, store);
+        int done = 0;
+        io.i32(done);
+        io.i32(flag);
+        io.f64(t);
+        io.st(s);
+        io.seek(3 + 2 * UC_N);
+        io.f64(obs.w0);
+        io.f64(obs.w1);
+        io.i64(obs.c_next);
+        io.i32(obs.mode);
+        io.i32(obs.raw);
+        io.f64(obs.last_t);
+    };
+    if (sliced && u0 >= TU) cell_state(false);  // a record in progress
+    if (flag == 1 || flag == -1) {
+        if (!sliced) {
+            flag = uni ? uc_loop_u(s, a, h, t, j.n_iters, j.pre_scaller, j.writable_var, j.max_value, obs)
+                       : uc_loop(s, a, h, t, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
+        } else {
+            const UcRange r = uc_range(j.n_iters, TU, u0, u1);
+            const int f = uc_loop_u(s, a, h, t, r.i0, r.i1, r.last, j.pre_scaller, j.writable_var, j.max_value, obs);
+            if (f != 0 && !r.last) { cell_state(true); return; }
+            flag = f;
+        }
+    }
     j.flag[lane] = (flag == 1 && !j.raw_flags) ? obs.count() : flag;
     if (j.final_state) {
 #pragma unroll
         for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.stride + lane] = s.get(i);
+    }
+    if (sliced) {
+        UcSlab io(j, lane, kCap, true);
+        int done = 1;
+        io.i32(done);
     }
 }
 
 // Bifurcation sweep (calculateDiscreteModelCUDA + peakFinderCUDA fused):
 // transient, recorded run with streaming peaks, flag = peak count / -1 / 0.
 // F32C with record_f64: the transient in F32C, the recorded run in double.
-template <class S>
+// Sliced (SL, j.slab set): units are the transient's chunks, then the record's
+// chunks of UC_CHECK_INTERVAL samples; the launch that ends the transient
+// goes on into the record (and saves the record's state).
+template <class S, bool SL = false>
 __device__ void uc_bif_cell(const UcJob& j, long long lane) {
     typedef typename Prec<S>::T T;
     const long long cell = j.cell_offset + lane;
+    constexpr bool sliced = SL;
+    const long long u0 = sliced ? j.unit_begin : 0, u1 = sliced ? j.unit_end : UC_UNITS_ALL;
+    constexpr int kCap = UC_SLAB_BIF_WORDS(UC_N);
 
     double ad[UC_M], x0[UC_N], hd;
     const bool ok = uc_setup(j, cell, ad, x0, hd);
-    if (!ok) { j.flag[lane] = -1; return; }
+    if (!ok) {
+        if (u0 == 0) j.flag[lane] = -1;  // a sliced cell gets its flag from its first launch
+        return;
+    }
     long long skip = j.skip_steps, iters = j.n_iters;
     if (uc_sweeps_step(j)) {
         skip = (long long)ceil(j.transient_time / hd);
@@ -708,7 +830,7 @@ __device__ void uc_bif_cell(const UcJob& j, long long lane) {
         // (int)(t_max / h / pre) samples capped at n_iters, unusable (-1)
         // when it holds <= 2; transient of (int)(transient / h) single steps
         // plus one trailing step, an escape there (0) marks the cell -1; the
-        // block's own loop flag otherwise.
+        // block's own loop flag otherwise. Never sliced.
         long long blk = (long long)(j.t_max / hd / (double)j.pre_scaller);
         if (blk > j.n_iters) blk = j.n_iters;
         if (blk <= 2) { j.flag[lane] = -1; return; }
@@ -722,8 +844,36 @@ __device__ void uc_bif_cell(const UcJob& j, long long lane) {
         return;
     }
     const bool uni = !uc_sweeps_step(j);  // step counts are wave-uniform unless the sweep is over h
-    int flag = uni ? uc_loop_u(s, a, h, t, j.skip_steps, j.pre_scaller, 0, j.max_value, none)
+    int flag = 1;
+    const long long TU = sliced ? uc_chunks(j.skip_steps) : 0;
+    if (sliced && u0 > 0) {
+        UcSlab io(j, lane, kCap, false);
+        int done = 0;
+        io.i32(done);
+        if (done) return;
+        if (u0 < TU) {  // inside the transient; the record loads its own state
+            io.i32(flag);
+            io.f64(t);
+            io.st(s);
+        }
+    }
+    if (!sliced) {
+        flag = uni ? uc_loop_u(s, a, h, t, j.skip_steps, j.pre_scaller, 0, j.max_value, none)
                    : uc_loop(s, a, h, t, skip, j.pre_scaller, 0, j.max_value, none);
+    } else if (u0 < TU) {
+        const UcRange r = uc_range(j.skip_steps, 0, u0, u1);
+        const int f = uc_loop_u(s, a, h, t, r.i0, r.i1, r.last, j.pre_scaller, 0, j.max_value, none);
+        if (f != 0 && !r.last) {
+            UcSlab io(j, lane, kCap, true);
+            int done = 0;
+            io.i32(done);
+            io.i32(flag);
+            io.f64(t);
+            io.st(s);
+            return;
+        }
+        flag = f;
+    }
     if constexpr (sizeof(T) == 4) {
         if (j.record_f64) {
             // The recorded window in double from the compensated state, with
@@ -732,11 +882,11 @@ __device__ void uc_bif_cell(const UcJob& j, long long lane) {
             StateF64 s64;
 #pragma unroll
             for (int i = 0; i < UC_N; ++i) s64.x[i] = s.get(i);
-            uc_bif_record(j, lane, s64, ad, hd, hd, t, uni, iters, flag);
+            uc_bif_record<SL>(j, lane, s64, ad, hd, hd, t, uni, iters, flag, u0, u1, TU);
             return;
         }
     }
-    uc_bif_record(j, lane, s, a, h, hd, t, uni, iters, flag);
+    uc_bif_record<SL>(j, lane, s, a, h, hd, t, uni, iters, flag, u0, u1, TU);
 }
 
 // Random unit probe vectors (U-CUDA/cudaLibrary.cu:2603-2626 for one vector,
@@ -826,16 +976,27 @@ __device__ __forceinline__ void uc_cgs(const double* a, double* b, double* den) 
 // reference's Benettin scheme with probe copies at distance eps.
 // Results: out0[k * stride + cell] = sum_k / t_max; flag 1, or -1 with 999
 // (diverged) / NaN (non-positive step).
-template <int K, class S>
+// Sliced (SL, j.slab set): units are the transient's chunks, then one per
+// block.
+// Slab: done, state, t, sum[K], then the block state from word
+// 2 + words(S) + K (variational: V, Vs, start2, ksum, ref_mode, lsum,
+// prod, esum; finite differences: the K probe states).
+template <int K, class S, bool SL = false>
 __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
     typedef typename Prec<S>::T T;
     const long long cell = j.cell_offset + lane;
+    constexpr bool sliced = SL;
+    const long long u0 = sliced ? j.unit_begin : 0, u1 = sliced ? j.unit_end : UC_UNITS_ALL;
+    constexpr int kCap = UC_SLAB_LYAP_WORDS(UC_N, K);
+    constexpr int kBlockWord = 2 + UcWords<S>::n + K;
 
     double ad[UC_M], x0[UC_N], hd;
     if (!uc_setup(j, cell, ad, x0, hd)) {
+        if (u0 == 0) {  // a sliced cell gets its results from its first launch
 #pragma unroll
-        for (int k = 0; k < K; ++k) j.out0[(long long)k * j.stride + lane] = __builtin_nan("");
-        j.flag[lane] = -1;
+            for (int k = 0; k < K; ++k) j.out0[(long long)k * j.stride + lane] = __builtin_nan("");
+            j.flag[lane] = -1;
+        }
         return;
     }
     long long skip = j.skip_steps, nts = j.nt_steps;
@@ -848,24 +1009,48 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
     for (int i = 0; i < UC_M; ++i) a[i] = Prec<S>::to_t(ad[i]);
     const T h = Prec<S>::to_t(hd);
     S s;
-    s.load(x0);
-
     double sum[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) sum[k] = 0.0;
     bool alive = true;
+    double t = 0.0;
+    if (u0 == 0) {
+        s.load(x0);
+    } else {
+        UcSlab io(j, lane, kCap, false);
+        int done = 0;
+        io.i32(done);
+        if (done) return;
+        io.st(s);
+        io.f64(t);
+#pragma unroll
+        for (int k = 0; k < K; ++k) io.f64(sum[k]);
+    }
 
     NoObserver none;
-    double t = 0.0;
-    // step counts are wave-uniform unless the sweep is over h
+    // step counts are wave-uniform unless the sweep is over h (sliced cells
+    // always are)
     const bool uni = !uc_sweeps_step(j);
     auto run = [&](S& st, double& tt, long long n_job, long long n_cell) -> int {
         return uni ? uc_loop_u(st, a, h, tt, n_job, 1, 0, j.max_value, none)
                    : uc_loop(st, a, h, tt, n_cell, 1, 0, j.max_value, none);
     };
-    if (run(s, t, j.skip_steps, skip) == 0) alive = false;
+    const long long TU = sliced ? uc_chunks(j.skip_steps) : 0;
+    if (!sliced) {
+        if (run(s, t, j.skip_steps, skip) == 0) alive = false;
+    } else if (u0 < TU) {
+        const UcRange r = uc_range(j.skip_steps, 0, u0, u1);
+        if (uc_loop_u(s, a, h, t, r.i0, r.i1, r.last, 1, 0, j.max_value, none) == 0) alive = false;
+    }
+    // blocks [b0, b1) in this launch; the block state starts fresh in the
+    // launch that ends the transient and is loaded after that
+    const bool in_blocks = !sliced || u1 >= TU;
+    const bool fresh = !sliced || u0 < TU;
+    const long long b0 = (!sliced || u0 < TU) ? 0 : u0 - TU;
+    const long long b1 = (!sliced || u1 - TU >= j.n_iters) ? j.n_iters : u1 - TU;
+    const bool ends = b1 >= j.n_iters;
 
-    if (alive) {
+    if (alive && in_blocks) {
         double z[K * UC_N];
         uc_probe_basis<K>(j.probe_seed, cell, z);
         if (j.method == UC_METHOD_VARIATIONAL) {
@@ -908,7 +1093,24 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
 #else
             const bool one_check = false;
 #endif
-            for (long long b = 0; b < j.n_iters; ++b) {
+            auto block_state = [&](bool store) {
+                UcSlab io(j, lane, kCap, store);
+                io.seek(kBlockWord);
+#pragma unroll
+                for (int i = 0; i < K * UC_N; ++i) io.val(V[i]);
+#if !UC_STRICT
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) io.val(Vs[i]);
+                io.val(start2);
+                io.i32(ksum);
+                io.b8(ref_mode);
+                io.f64(lsum);
+#pragma unroll
+                for (int k = 0; k < K; ++k) { io.f64(prod[k]); io.i32(esum[k]); }
+#endif
+            };
+            if (!fresh) block_state(false);
+            for (long long b = b0; b < b1; ++b) {
                 const int r = one_check ? uc_block_tan_u<K>(s, V, a, h, t, (int)j.nt_steps, j.max_value)
                               : uni     ? uc_loop_tan_u<K>(s, V, a, h, t, j.nt_steps, j.max_value)
                                         : uc_loop_tan<K>(s, V, a, h, t, nts, j.max_value);
@@ -980,20 +1182,24 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
 #pragma unroll
                 for (int i = 0; i < K * UC_N; ++i) V[i] = Prec<S>::to_t(v[i]);
             }
-#if !UC_STRICT
-            if (K == 1) {
-                double s2 = 1.0;
-                if (j.n_iters > 0) {
-                    s2 = 0.0;
-#pragma unroll
-                    for (int i = 0; i < UC_N; ++i) s2 = s2 + (double)V[i] * (double)V[i];
-                }
-                sum[0] = ref_mode ? lsum : (double)ksum * UC_LN2 + 0.5 * log(s2);
+            if (alive && !ends) {
+                block_state(true);
             } else {
+#if !UC_STRICT
+                if (K == 1) {
+                    double s2 = 1.0;
+                    if (j.n_iters > 0) {
+                        s2 = 0.0;
 #pragma unroll
-                for (int k = 0; k < K; ++k) sum[k] = log(prod[k]) + (double)esum[k] * UC_LN2;
-            }
+                        for (int i = 0; i < UC_N; ++i) s2 = s2 + (double)V[i] * (double)V[i];
+                    }
+                    sum[0] = ref_mode ? lsum : (double)ksum * UC_LN2 + 0.5 * log(s2);
+                } else {
+#pragma unroll
+                    for (int k = 0; k < K; ++k) sum[k] = log(prod[k]) + (double)esum[k] * UC_LN2;
+                }
 #endif
+            }
         } else {
             // Reference finite-difference Benettin (LLEKernelCUDA / LSKernelCUDA).
             const double eps = j.eps;
@@ -1004,14 +1210,24 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
 #pragma unroll
                 for (int i = 0; i < UC_N; ++i) q[i] = z[i];
             }
+            auto probe_state = [&](bool store) {
+                UcSlab io(j, lane, kCap, store);
+                io.seek(kBlockWord);
 #pragma unroll
-            for (int k = 0; k < K; ++k) {
-                double yk[UC_N];
+                for (int k = 0; k < K; ++k) io.st(y[k]);
+            };
+            if (fresh) {
 #pragma unroll
-                for (int i = 0; i < UC_N; ++i) yk[i] = q[k * UC_N + i] * eps + s.get(i);
-                y[k].load(yk);
+                for (int k = 0; k < K; ++k) {
+                    double yk[UC_N];
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) yk[i] = q[k * UC_N + i] * eps + s.get(i);
+                    y[k].load(yk);
+                }
+            } else {
+                probe_state(false);
             }
-            for (long long b = 0; b < j.n_iters && alive; ++b) {
+            for (long long b = b0; b < b1 && alive; ++b) {
                 const double t0 = t;
                 if (run(s, t, j.nt_steps, nts) == 0) { alive = false; break; }
 #pragma unroll
@@ -1052,7 +1268,19 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
                     }
                 }
             }
+            if (alive && !ends) probe_state(true);
         }
+    }
+    if (alive && !(in_blocks && ends)) {
+        // sliced, unfinished: the launch ended inside the transient or the blocks
+        UcSlab io(j, lane, kCap, true);
+        int done = 0;
+        io.i32(done);
+        io.st(s);
+        io.f64(t);
+#pragma unroll
+        for (int k = 0; k < K; ++k) io.f64(sum[k]);
+        return;
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) j.out0[(long long)k * j.stride + lane] = alive ? sum[k] / j.t_max : 999.0;
@@ -1060,6 +1288,11 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
     if (j.final_state) {
 #pragma unroll
         for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.stride + lane] = s.get(i);
+    }
+    if (sliced) {
+        UcSlab io(j, lane, kCap, true);
+        int done = 1;
+        io.i32(done);
     }
 }
 
@@ -2088,6 +2321,14 @@ UC_PERSISTENT(uc_lle_f64_q, (uc_lyap_cell<1, StateF64>))
 UC_PERSISTENT(uc_lle_f32c_q, (uc_lyap_cell<1, StateF32C>))
 UC_PERSISTENT(uc_ls_f64_q, (uc_lyap_cell<UC_N, StateF64>))
 UC_PERSISTENT(uc_ls_f32c_q, (uc_lyap_cell<UC_N, StateF32C>))
+// Sliced persistent kernels: each launch runs its entries' units
+// [unit_begin, unit_end) and keeps unfinished cells in the slab.
+UC_PERSISTENT(uc_bif_f64_s, (uc_bif_cell<StateF64, true>))
+UC_PERSISTENT(uc_bif_f32c_s, (uc_bif_cell<StateF32C, true>))
+UC_PERSISTENT(uc_lle_f64_s, (uc_lyap_cell<1, StateF64, true>))
+UC_PERSISTENT(uc_lle_f32c_s, (uc_lyap_cell<1, StateF32C, true>))
+UC_PERSISTENT(uc_ls_f64_s, (uc_lyap_cell<UC_N, StateF64, true>))
+UC_PERSISTENT(uc_ls_f32c_s, (uc_lyap_cell<UC_N, StateF32C, true>))
 #if !UC_STRICT
 // Two cells per lane (uc_lle_pair): lane l runs cells l and l + ceil(n / 2).
 #define UC_DIRECT_PAIR(NAME, CELL)                                              \
@@ -2098,6 +2339,32 @@ UC_PERSISTENT(uc_ls_f32c_q, (uc_lyap_cell<UC_N, StateF32C>))
     }
 UC_DIRECT_PAIR(uc_lle_f64_x2, uc_lle_pair<StateF64>)
 UC_DIRECT_PAIR(uc_lle_f32c_x2, uc_lle_pair<StateF32C>)
+// Persistent form (whole runs only): a chunk is 2 * blockDim cells, lane l
+// of the chunk runs its cells l and l + blockDim.
+#define UC_PERSISTENT_PAIR(NAME, CELL)                                                              \
+    extern "C" __global__ void __launch_bounds__(UC_BLOCK) NAME(const UcJob* __restrict__ jobs,     \
+                                                                const long long* __restrict__ start, \
+                                                                int n_jobs, u64* counter) {          \
+        __shared__ long long s_chunk;                                                               \
+        const long long total = start[n_jobs];                                                      \
+        for (;;) {                                                                                  \
+            if (threadIdx.x == 0) s_chunk = (long long)atomicAdd((unsigned long long*)counter, 1ULL); \
+            __syncthreads();                                                                        \
+            const long long c = s_chunk;                                                            \
+            __syncthreads();                                                                        \
+            if (c >= total) break;                                                                  \
+            int lo = 0, hi = n_jobs - 1;                                                            \
+            while (lo < hi) {                                                                       \
+                const int mid = (lo + hi + 1) >> 1;                                                 \
+                if (start[mid] <= c) lo = mid; else hi = mid - 1;                                   \
+            }                                                                                       \
+            lo = __builtin_amdgcn_readfirstlane(lo);                                                \
+            const long long lane = (c - start[lo]) * 2LL * blockDim.x + threadIdx.x;               \
+            if (lane < jobs[lo].n_cells) CELL(jobs[lo], lane, (long long)blockDim.x);              \
+        }                                                                                           \
+    }
+UC_PERSISTENT_PAIR(uc_lle_f64_x2_q, uc_lle_pair<StateF64>)
+UC_PERSISTENT_PAIR(uc_lle_f32c_x2_q, uc_lle_pair<StateF32C>)
 #endif
 
 // ================================================================ basins ==
@@ -2255,6 +2522,16 @@ __device__ __forceinline__ double uc_bas_attempt(BasF32C& s, const float* a, dou
     }
     return err;
 }
+__device__ __forceinline__ void uc_bas_state(UcSlab& io, BasF64& s) {
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) io.f64(s.x[i]);
+}
+__device__ __forceinline__ void uc_bas_state(UcSlab& io, BasF32C& s) {
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) io.f32(s.hi[i]);
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) io.f32(s.c[i]);
+}
 __device__ __forceinline__ void uc_bas_k1(BasF64& s, const double* a, double* k1) { uc_rhs<double>(s.x, a, s.t, k1); }
 __device__ __forceinline__ void uc_bas_k1(BasF32C& s, const float* a, float* k1) { uc_rhs<float>(s.hi, a, s.thi, k1); }
 
@@ -2269,10 +2546,18 @@ template <> struct BasPrec<BasF32C> {
     static __device__ __forceinline__ double tol(double v) { return fmax(v, 9.5367431640625e-07); }
 };
 
-template <class S>
+// Sliced (SL, j.slab set): one unit per section; slab words done, h,
+// accepted, rejected, state, then the section shift register from word
+// 4 + 2N.
+static_assert(UC_REC * UC_LBL <= 32, "basins slab layout");
+template <class S, bool SL = false>
 __device__ void uc_basins_cell(const UcJob& j, long long lane) {
     typedef typename BasPrec<S>::T T;
     const long long cell = j.cell_offset + lane;
+    constexpr bool sliced = SL;
+    constexpr int kCap = UC_SLAB_BAS_WORDS(UC_N);
+    const long long sec0 = sliced ? j.unit_begin : 0;
+    const long long sec1 = (sliced && j.unit_end < j.n_sections) ? j.unit_end : j.n_sections;
     double ad[UC_M], x0[UC_N], hd;
     uc_setup(j, cell, ad, x0, hd);
     T a[UC_M];
@@ -2290,8 +2575,29 @@ __device__ void uc_basins_cell(const UcJob& j, long long lane) {
         for (int i = 0; i < UC_LBL; ++i) buf[r][i] = 0.0;
     long long acc = 0, rej = 0;
     int status = 0;  // 0 running, -1 unbound, -2 unresolved
+    auto cell_state = [&](bool store) {
+        UcSlab io(j, lane, kCap, store);
+        int done = 0;
+        io.i32(done);
+        io.f64(h);
+        io.i64(acc);
+        io.i64(rej);
+        uc_bas_state(io, s);
+        io.seek(4 + 2 * UC_N);
+#pragma unroll
+        for (int r = 0; r < UC_REC; ++r)
+#pragma unroll
+            for (int i = 0; i < UC_LBL; ++i) io.f64(buf[r][i]);
+    };
+    if (sliced && sec0 > 0) {
+        UcSlab io(j, lane, kCap, false);
+        int done = 0;
+        io.i32(done);
+        if (done) return;
+        cell_state(false);
+    }
     T k1[UC_N];
-    for (int sec = 0; sec < j.n_sections && status == 0; ++sec) {
+    for (long long sec = sec0; sec < sec1 && status == 0; ++sec) {
         s.reset_time();
         uc_bas_k1(s, a, k1);
         for (;;) {
@@ -2338,6 +2644,10 @@ __device__ void uc_basins_cell(const UcJob& j, long long lane) {
             for (int i = 0; i < UC_LBL; ++i) buf[r][i] = buf[r - 1][i];
 #pragma unroll
         for (int i = 0; i < UC_LBL; ++i) buf[0][i] = s.get(i);
+    }
+    if (sliced && status == 0 && sec1 < j.n_sections) {
+        cell_state(true);
+        return;
     }
     int p = status;
     double pt[UC_LBL];
@@ -2389,12 +2699,19 @@ __device__ void uc_basins_cell(const UcJob& j, long long lane) {
         j.counters[lane] = acc;
         j.counters[j.stride + lane] = rej;
     }
+    if (sliced) {
+        UcSlab io(j, lane, kCap, true);
+        int done = 1;
+        io.i32(done);
+    }
 }
 
 UC_DIRECT(uc_bas_f64, uc_basins_cell<BasF64>)
 UC_DIRECT(uc_bas_f32c, uc_basins_cell<BasF32C>)
 UC_PERSISTENT(uc_bas_f64_q, uc_basins_cell<BasF64>)
 UC_PERSISTENT(uc_bas_f32c_q, uc_basins_cell<BasF32C>)
+UC_PERSISTENT(uc_bas_f64_s, (uc_basins_cell<BasF64, true>))
+UC_PERSISTENT(uc_bas_f32c_s, (uc_basins_cell<BasF32C, true>))
 
 // ---- attractor catalogue: quantised signature keys in an open-addressing
 // hash set (linear probing, bounded by the table size), then a per-cell

@@ -83,7 +83,23 @@ typedef struct UcJob {
     double* final_state;          /* final_state[i * n_cells + cell]          */
     long long* counters;          /* per cell: accepted, rejected steps       */
     int* label;                   /* basins: int32 attractor signature hash   */
+
+    /* ---- resumable slices (persistent kernels, wave-uniform counts) ---- */
+    double* slab;                 /* cell state between launches,             */
+                                  /* slab[w * stride + cell]; NULL: whole run */
+    long long unit_begin;         /* work units run by this launch:           */
+    long long unit_end;           /* [unit_begin, unit_end)                   */
 } UcJob;
+
+/* Work units of a sliced cell: transient (and bifurcation record) steps in
+ * chunks of UC_UNIT_STEPS, the magnitude-check interval, so the checks fall
+ * on the same steps as in a whole run; one unit per renormalisation block
+ * (LLE / LS) or per section (basins). Slab words per cell for N state
+ * variables (and K tangent vectors): */
+#define UC_UNIT_STEPS 100
+#define UC_SLAB_LYAP_WORDS(N, K) (8 + 3 * (K) + 3 * (N) + 2 * (K) * (N))
+#define UC_SLAB_BIF_WORDS(N) (10 + 2 * (N))
+#define UC_SLAB_BAS_WORDS(N) (40 + 2 * (N))
 
 /* Fast synchronisation (the reference's FastSynchro): a slave copy of the
  * system is pulled towards a master by N = K (master - slave), integrated

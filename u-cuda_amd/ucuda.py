@@ -97,7 +97,12 @@ class FastSyncOutput(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_fraction", C.c_double), ("max_bytes", C.c_longlong), ("round_traj_steps", C.c_double),
-                ("block", C.c_int), ("waves_per_simd", C.c_int)]
+                ("block", C.c_int), ("waves_per_simd", C.c_int), ("slice_steps", C.c_double)]
+
+
+class BatchPlan(C.Structure):
+    _fields_ = [("block", C.c_int), ("waves_per_simd", C.c_int), ("slice_steps", C.c_double),
+                ("cells_per_lane", C.c_int), ("ms", C.c_double)]
 
 
 class ServerStats(C.Structure):
@@ -173,6 +178,9 @@ _SIGNATURES = {
     "uc_batch_add_basins": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BasinsRequest), C.POINTER(BasinsOutput),
                                       C.POINTER(C.c_int)]),
     "uc_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "uc_batch_set_plan": (C.c_int, [C.c_void_p, C.POINTER(BatchPlan)]),
+    "uc_batch_get_plan": (C.c_int, [C.c_void_p, C.POINTER(BatchPlan)]),
+    "uc_batch_tune_plan": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(BatchPlan)]),
     "uc_batch_tune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_double)]),
     "uc_batch_launch": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -818,11 +826,24 @@ class Batch:
     def set_launch(self, block, waves_per_simd):
         self.engine.check(lib().uc_batch_set_launch(self.h, block, waves_per_simd), "uc_batch_set_launch")
 
+    def set_plan(self, block=256, waves_per_simd=4, slice_steps=0.0, cells_per_lane=1):
+        """Launch shape, trajectory-steps per cell per launch (0: whole runs)
+        and cells per lane (1, or 2 for LLE maps that can)."""
+        p = BatchPlan(block, waves_per_simd, float(slice_steps), cells_per_lane, 0.0)
+        self.engine.check(lib().uc_batch_set_plan(self.h, C.byref(p)), "uc_batch_set_plan")
+
+    def plan(self):
+        p = BatchPlan()
+        self.engine.check(lib().uc_batch_get_plan(self.h, C.byref(p)), "uc_batch_get_plan")
+        return dict(block=p.block, waves_per_simd=p.waves_per_simd, slice_steps=p.slice_steps,
+                    cells_per_lane=p.cells_per_lane, ms=p.ms)
+
     def tune(self, stream=None):
-        b, w, ms = C.c_int(), C.c_int(), C.c_double()
-        self.engine.check(lib().uc_batch_tune(self.h, _stream(stream), C.byref(b), C.byref(w), C.byref(ms)),
-                          "uc_batch_tune")
-        return dict(block=b.value, waves_per_simd=w.value, ms=ms.value)
+        """Times the candidate plans and keeps the fastest (all four knobs)."""
+        p = BatchPlan()
+        self.engine.check(lib().uc_batch_tune_plan(self.h, _stream(stream), C.byref(p)), "uc_batch_tune_plan")
+        return dict(block=p.block, waves_per_simd=p.waves_per_simd, slice_steps=p.slice_steps,
+                    cells_per_lane=p.cells_per_lane, ms=p.ms)
 
     def launch(self, stream=None, sync=True):
         import torch
@@ -862,9 +883,12 @@ class Server:
     stream has a hardware queue of its own: with HIP's default of 4 the
     round streams share two queues and rounds on one queue run in turn."""
 
-    def __init__(self, engine, hbm_fraction=0.5, max_bytes=0, round_traj_steps=0.0, block=0, waves_per_simd=0):
+    def __init__(self, engine, hbm_fraction=0.5, max_bytes=0, round_traj_steps=0.0, block=0, waves_per_simd=0,
+                 slice_steps=0.0):
+        """slice_steps: trajectory-steps per cell per round for jobs whose
+        cells all step the same counts (0: 4096; < 0: whole runs)."""
         self.engine = engine
-        cfg = ServerConfig(hbm_fraction, max_bytes, round_traj_steps, block, waves_per_simd)
+        cfg = ServerConfig(hbm_fraction, max_bytes, round_traj_steps, block, waves_per_simd, float(slice_steps))
         h = C.c_void_p()
         engine.check(lib().uc_server_create(engine.h, C.byref(cfg), C.byref(h)), "uc_server_create")
         self.h = h
