@@ -663,7 +663,7 @@ def extra_batch(eng, n_jobs=64, seed=0):
                 basins_attractors=[len(o["attractors"]) for o in basins_out], tuned=tuned, **b.info())
 
 
-def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
+def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None, dump_jobs=None, server_kw=None):
     """Poisson arrivals of mixed requests, a third each: Lorenz bifurcation
     sweeps (side^2 rho values), Rossler LLE maps (side x side) and forced
     Duffing basin maps (side x side initial conditions, rtol = atol = 1e-9),
@@ -674,10 +674,16 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
     weighting a bifurcation step 0.5 and a basin section 410 LLE steps (the
     measured FP32C rates of the three kernels); if it would reach 0.8 the
     rate is lowered to 70% of capacity.
+    The server runs jobs of more than 65,536 cells and 1,024 steps per cell
+    in slices of about 1,024 steps (their cells resume over several rounds)
+    and gives the waves of small grids issue priority.
     Reports throughput and p50 / p99 job latency, the slowdown (latency over
-    the job's estimated time alone at the measured rates) and p99 latency of
-    the smaller and larger half of the jobs by size; every 100th job is run
-    again alone in FP64 and compared."""
+    the same request's latency alone on an idle server, measured after the
+    stream for every job) and p99 latency of the smaller and larger half of
+    the jobs by size (cells x steps); 1% of the jobs of each kind are run
+    again alone in FP64 and compared. `dump_jobs`: a .json path for the
+    per-job records (kind, side, steps, submit time, latency in the stream and
+    alone); `server_kw`: ucuda.Server options."""
     import numpy as np
     import ucuda
     rng = np.random.default_rng(seed)
@@ -699,8 +705,10 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
         rate = 50.0
         if capacity and rate * est / capacity >= 0.8:
             rate = 0.7 * capacity / est
-    srv = ucuda.Server(eng)
+    server_kw = dict(server_kw or {})
+    srv = ucuda.Server(eng, **server_kw)
     ids = []
+    t_sub = []
     t = 0.0
     t0 = time.perf_counter()
     i = 0
@@ -736,6 +744,7 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
         else:
             jid = srv.submit_basins(*rq["args"], **kw)
         ids.append((jid, kind, side, steps))
+        t_sub.append(time.perf_counter() - t0)
     progress("serving: %d jobs submitted, draining" % len(ids))
     pending, last = [jid for jid, *_ in ids], time.perf_counter()
     deadline = last + 300.0
@@ -751,9 +760,11 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
     wall = time.perf_counter() - t0
     st = srv.stats()
     failed = [jid for jid, *_ in ids if srv.poll(jid)[0] != 2]
-    # audit: every 100th job recomputed alone in FP64
+    # audit: 1% of the jobs of each kind (every 100th, from the first)
+    # recomputed alone in FP64
     audit = []
-    for jid, kind, side, steps in ids[::100]:
+    sample = [x for kind_ in ("lle", "bif", "basins") for x in [y for y in ids if y[1] == kind_][::100]]
+    for jid, kind, side, steps in sample:
         got = srv.fetch(jid)
         axes, rq = request(kind, side, steps)
         if kind == "lle":
@@ -778,23 +789,41 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None):
             mapped = np.array([m.get(int(v), 0 if v == 0 else -99) for v in got["labels"]])
             audit.append(dict(kind=kind, cells=side * side,
                               label_agreement=float((mapped == ref["labels"].cpu().numpy()).mean())))
-    # per-job latency against the job's own size: the estimated time it
-    # would take alone on the whole GPU at the measured rates
     lat = np.array([srv.poll(jid)[1] for jid, *_ in ids])
     srv.close()
+    # each job's latency alone: the same request on an idle server (same
+    # admission, rounds and kernels), one job at a time
+    progress("serving: %d jobs again, each alone" % len(ids))
+    solo = ucuda.Server(eng, **server_kw)
+    solo_ms = []
+    for jid, kind, side, steps in ids:
+        axes, rq = request(kind, side, steps)
+        kw = dict(rq.get("kw", {}), precision=ucuda.PREC_F32C)
+        sub = {"lyap": solo.submit_lyap, "bif": solo.submit_bif, "basins": solo.submit_basins}[rq["fn"]]
+        j2 = sub(*rq["args"], **kw)
+        solo.wait(j2, 120.0)
+        solo_ms.append(solo.poll(j2)[1])
+        solo.fetch(j2)
+    solo.close()
+    solo_ms = np.array(solo_ms)
     costs = np.array([cost(kind, side, steps) for _, kind, side, steps in ids])
+    if dump_jobs:
+        with open(dump_jobs, "w") as f:
+            json.dump([dict(kind=k, side=sd, steps=sp, t_submit=ts, latency_ms=float(l), solo_ms=float(so))
+                       for (_, k, sd, sp), ts, l, so in zip(ids, t_sub, lat, solo_ms)], f)
     sizes = {}
-    if capacity and len(ids):
-        solo_ms = costs / capacity * 1e3
+    if len(ids):
         slow = lat / solo_ms
         small = costs <= np.median(costs)
-        sizes = dict(solo_est_ms_p50=float(np.percentile(solo_ms, 50)),
-                     solo_est_ms_p99=float(np.percentile(solo_ms, 99)),
+        sizes = dict(solo_ms_p50=float(np.percentile(solo_ms, 50)), solo_ms_p99=float(np.percentile(solo_ms, 99)),
                      slowdown_p50=float(np.percentile(slow, 50)), slowdown_p99=float(np.percentile(slow, 99)),
+                     slowdown_max=float(slow.max()),
+                     p50_ms_smaller_half=float(np.percentile(lat[small], 50)),
                      p99_ms_smaller_half=float(np.percentile(lat[small], 99)),
                      p99_ms_larger_half=float(np.percentile(lat[~small], 99)))
     kinds = {k: sum(1 for _, kk, _, _ in ids if kk == k) for k in ("lle", "bif", "basins")}
     return dict(arrival_rate_per_s=rate, offered_utilisation_est=rate * est / capacity if capacity else None,
+                server=dict(server_kw, slice_steps=server_kw.get("slice_steps", 1024)),
                 jobs=len(ids), jobs_by_kind=kinds, completed=st["completed"], failed=len(failed),
                 p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], max_ms=st["max_ms"], **sizes,
                 fixed_step_traj_steps_per_s=st["traj_steps"] / wall, rounds=st["rounds"],

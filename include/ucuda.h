@@ -414,13 +414,17 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
 /* ---- serving: continuous batching ----
  * A scheduler thread admits jobs in arrival order while their outputs fit
  * the HBM budget and keeps up to six rounds in flight, one per stream:
- * each round takes cells of one duration class (the decade of the job's
- * per-cell step count) from the admitted jobs (a fair share per job, then
- * oldest first) up to the GPU's resident lanes and launches the persistent
- * kernels once per (system, analysis) group; a new round is queued whenever
- * a slot frees, and jobs submitted meanwhile join it. The shortest class
- * goes first when none of its rounds is in flight, the others take turns,
- * and classes of >= 1e4 steps per cell hold at most three of the six
+ * each round takes cells of one duration class (the decade of the steps a
+ * cell runs in the round) from the admitted jobs (a fair share per job,
+ * then oldest first) up to the GPU's resident lanes and launches the
+ * persistent kernels once per (system, analysis) group; a new round is
+ * queued whenever a slot frees, and jobs submitted meanwhile join it. A job
+ * of more than slice_min_cells cells and more than slice_steps steps per
+ * cell runs in slices: each round advances its cells by about slice_steps
+ * steps and keeps their state in HBM, so no round holds the GPU for long.
+ * The waves of small grids run at a higher issue priority. The shortest
+ * class goes first when none of its rounds is in flight, the others take
+ * turns, and classes of >= 1e4 steps per cell hold at most three of the six
  * slots. Results are bit-identical to a direct uc_run_* of the same
  * request. Outputs stay on the device until uc_server_fetch copies them
  * out. Run the process with GPU_MAX_HW_QUEUES=8 (before the HIP runtime
@@ -435,8 +439,10 @@ typedef struct uc_server_config {
     int waves_per_simd;        /* resident waves per SIMD (0 -> 8)                        */
     double slice_steps;        /* trajectory-steps per cell per round for jobs whose cells */
                                /* all step the same counts: longer jobs resume over several */
-                               /* rounds, their cells' state kept in HBM (0 -> 4096,      */
+                               /* rounds, their cells' state kept in HBM (0 -> 1024,      */
                                /* < 0 -> whole runs)                                      */
+    long long slice_min_cells; /* jobs of at most this many cells run whole (0 -> 65536,  */
+                               /* < 0 -> any)                                             */
 } uc_server_config;
 typedef struct uc_server_stats_t {
     long long completed, rounds, queued, active;

@@ -65,7 +65,7 @@ def _alone(kind, sysm, kw):
 _ALONE = {}
 
 
-@pytest.mark.parametrize("slice_steps", [0.0, 300.0, -1.0], ids=["slice4096", "slice300", "whole"])
+@pytest.mark.parametrize("slice_steps", [0.0, 300.0, -1.0], ids=["slice1024", "slice300", "whole"])
 def test_results_identical_alone_and_in_the_mix(gpu_engine, slice_steps):
     print("case: 10 staggered submissions (lle f64/f32c, bif, ls, basins f64/f32c) with small rounds, "
           "slice_steps %g" % slice_steps, flush=True)
@@ -73,7 +73,7 @@ def test_results_identical_alone_and_in_the_mix(gpu_engine, slice_steps):
     if not _ALONE:
         _ALONE.update({i: _alone(k, s, kw) for i, (k, s, kw) in enumerate(jobs)})
     alone = [_ALONE[i] for i in range(len(jobs))]
-    srv = ucuda.Server(gpu_engine, round_traj_steps=2e7, slice_steps=slice_steps)
+    srv = ucuda.Server(gpu_engine, round_traj_steps=2e7, slice_steps=slice_steps, slice_min_cells=-1)
     ids = []
     for i, (k, s, kw) in enumerate(jobs):
         ids.append(_submit(srv, k, s, kw))
@@ -135,7 +135,9 @@ def test_admission_budget_and_cancel(gpu_engine):
 
 def test_short_jobs_do_not_wait_for_long_cells(gpu_engine):
     """Short jobs submitted right after a job of long cells finish long
-    before it: the long job runs in slices of 4,096 steps per cell."""
+    before it: rounds hold one duration class (decade of steps per cell), so
+    they never share the long job's round, and with the GPU not full (the
+    long job is 4,096 cells, run whole) their rounds run beside it."""
     print("case: one 64x64 LLE job of 110,001 steps per cell, then five 16x16 jobs of 111", flush=True)
     ros = ucuda.preset("rossler", gpu_engine)
     vals = ros.values()

@@ -2514,8 +2514,12 @@ int uc_batch_info(const uc_batch* b, int* n_jobs, int* n_groups, int* block, int
 // oldest first) up to the GPU's resident lanes, and launches one persistent
 // kernel per (module, kernel) group over the round's job table; the next
 // round is built and launched as soon as a slot is free, so rounds queue
-// back to back on the device and the tail of one overlaps the next. Jobs that arrive meanwhile join the next
-// round. A job's cells compute exactly as in a direct launch (same kernel
+// back to back on the device and the tail of one overlaps the next. Jobs
+// that arrive meanwhile join the next round. Large jobs run in slices
+// (sliced kernels, state in a slab): a round advances their cells by about
+// slice_steps steps and hands them back, so a round lasts about as long as
+// one slice and short jobs never wait behind a long job's cells. Entries of
+// small grids run at a higher wave issue priority. A job's cells compute exactly as in a direct launch (same kernel
 // code per cell; the output plane stride stays the job's own), so results
 // do not depend on what else shares a round. Outputs stay in device memory
 // until fetched. Device buffers use the stream-ordered allocator where the
@@ -2598,6 +2602,10 @@ struct uc_server {
     long long next_id = 1;
     size_t used_bytes = 0, budget = 0;
     double slice_steps = 0.0;  // traj-steps per cell per round of a sliced job; 0: whole runs
+    // jobs of more cells than this run in slices; fewer cells hold few wave
+    // slots, so they run whole (a thin job sliced would wait for a round per
+    // slice)
+    long long slice_min_cells = 65536;
     bool stop = false;
     bool async_alloc = false;
     hipMemPool_t pool = nullptr;   // the server's own pool; keeps up to the budget mapped
@@ -2670,7 +2678,8 @@ bool server_admit(uc_server* S, uc_server::Job& j) {
     // slab was counted in j.bytes at submission; given back otherwise)
     j.si = slice_info(j.kind, j.tmpl, j.n_vars, j.step_cost);
     const size_t slab_bytes = sizeof(double) * (size_t)j.si.words * (size_t)j.cells;
-    if (S->slice_steps > 0.0 && j.si.units > 0 && slice_units(j.si, S->slice_steps) < j.si.units) {
+    if (S->slice_steps > 0.0 && j.cells > S->slice_min_cells && j.si.units > 0 &&
+        slice_units(j.si, S->slice_steps) < j.si.units) {
         j.ups = slice_units(j.si, S->slice_steps);
         j.kernel = sliced_kernel(j.kernel);
         he = server_malloc(S, (void**)&j.slab, slab_bytes, S->stream);
@@ -2823,6 +2832,10 @@ std::string server_launch(uc_server* S, uc_server::Slot& sl, int block, int wps,
                 j.unit_begin = en->u0;
                 j.unit_end = en->u1;
             }
+            // issue priority by job size: small grids cannot fill the GPU
+            // and their latency is the pace of their own few waves
+            const long long nc = en->job->cells;
+            j.priority = nc <= 16384 ? 3 : nc <= 65536 ? 2 : nc <= 262144 ? 1 : 0;
             sl.h_jobs[jo++] = j;
             g.chunks += (en->cells + block - 1) / block;
             sl.h_start[so++] = g.chunks;
@@ -3015,7 +3028,7 @@ void server_reserve_slab(uc_server* S, uc_server::Job& j) {
     for (int i = 0; i < j.c.n_axes; ++i) step_axis |= j.c.axis[i].kind == UC_AXIS_STEP;
     // sliced exactly when one round's units are fewer than the job's, i.e.
     // when the job's steps per cell exceed the round's
-    if (S->slice_steps <= 0.0 || step_axis || j.step_cost <= S->slice_steps) return;
+    if (S->slice_steps <= 0.0 || step_axis || j.step_cost <= S->slice_steps || j.cells <= S->slice_min_cells) return;
     const int n = j.n_vars;
     const int words = j.kind == kKindBif ? UC_SLAB_BIF_WORDS(n)
                       : j.kind == kKindBasins ? UC_SLAB_BAS_WORDS(n)
@@ -3052,7 +3065,8 @@ int uc_server_create(uc_engine* e, const uc_server_config* cfg, uc_server** out)
     (void)hipMemGetInfo(&free_b, &total_b);
     const double frac = S->cfg.hbm_fraction > 0.0 ? S->cfg.hbm_fraction : 0.5;
     S->budget = S->cfg.max_bytes > 0 ? (size_t)S->cfg.max_bytes : (size_t)(frac * (double)free_b);
-    S->slice_steps = S->cfg.slice_steps == 0.0 ? 4096.0 : (S->cfg.slice_steps > 0.0 ? S->cfg.slice_steps : 0.0);
+    S->slice_steps = S->cfg.slice_steps == 0.0 ? 1024.0 : (S->cfg.slice_steps > 0.0 ? S->cfg.slice_steps : 0.0);
+    S->slice_min_cells = S->cfg.slice_min_cells == 0 ? 65536 : std::max(0LL, S->cfg.slice_min_cells);
     int pools = 0;
     S->async_alloc = hipDeviceGetAttribute(&pools, hipDeviceAttributeMemoryPoolsSupported, e->device) == hipSuccess &&
                      pools != 0;

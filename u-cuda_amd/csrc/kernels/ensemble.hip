@@ -1496,7 +1496,15 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_peaks_series(const dou
 // one index space, chunk_start = prefix sums, jobs ordered by cost so the
 // longest work is taken first) until the counter passes the end; every
 // workgroup reaches that exit, so the grid drains. A workgroup's lanes
-// always belong to one job.
+// always belong to one job. The waves run at the entry's issue priority
+// (the SIMD arbiter prefers higher-priority waves when several are ready),
+// so a small job's few waves keep their pace next to a full round's.
+__device__ __forceinline__ void uc_set_priority(int p) {
+    if (p == 3) __builtin_amdgcn_s_setprio(3);
+    else if (p == 2) __builtin_amdgcn_s_setprio(2);
+    else if (p == 1) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
 #define UC_PERSISTENT(NAME, CELL)                                                                   \
     extern "C" __global__ void __launch_bounds__(UC_BLOCK) NAME(const UcJob* __restrict__ jobs,     \
                                                                 const long long* __restrict__ start, \
@@ -1515,6 +1523,7 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_peaks_series(const dou
                 if (start[mid] <= c) lo = mid; else hi = mid - 1;                                   \
             }                                                                                       \
             lo = __builtin_amdgcn_readfirstlane(lo); /* uniform: job fields become scalar loads */   \
+            uc_set_priority(jobs[lo].priority);                                                     \
             const long long lane = (c - start[lo]) * (long long)blockDim.x + threadIdx.x;           \
             if (lane < jobs[lo].n_cells) CELL(jobs[lo], lane);                                      \
         }                                                                                           \
@@ -2359,6 +2368,7 @@ UC_DIRECT_PAIR(uc_lle_f32c_x2, uc_lle_pair<StateF32C>)
                 if (start[mid] <= c) lo = mid; else hi = mid - 1;                                   \
             }                                                                                       \
             lo = __builtin_amdgcn_readfirstlane(lo);                                                \
+            uc_set_priority(jobs[lo].priority);                                                     \
             const long long lane = (c - start[lo]) * 2LL * blockDim.x + threadIdx.x;               \
             if (lane < jobs[lo].n_cells) CELL(jobs[lo], lane, (long long)blockDim.x);              \
         }                                                                                           \

@@ -89,6 +89,8 @@ typedef struct UcJob {
                                   /* slab[w * stride + cell]; NULL: whole run */
     long long unit_begin;         /* work units run by this launch:           */
     long long unit_end;           /* [unit_begin, unit_end)                   */
+    int priority;                 /* persistent kernels: issue priority of    */
+                                  /* the entry's waves, 0..3 (s_setprio)      */
 } UcJob;
 
 /* Work units of a sliced cell: transient (and bifurcation record) steps in

@@ -97,7 +97,8 @@ class FastSyncOutput(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_fraction", C.c_double), ("max_bytes", C.c_longlong), ("round_traj_steps", C.c_double),
-                ("block", C.c_int), ("waves_per_simd", C.c_int), ("slice_steps", C.c_double)]
+                ("block", C.c_int), ("waves_per_simd", C.c_int), ("slice_steps", C.c_double),
+                ("slice_min_cells", C.c_longlong)]
 
 
 class BatchPlan(C.Structure):
@@ -884,11 +885,14 @@ class Server:
     round streams share two queues and rounds on one queue run in turn."""
 
     def __init__(self, engine, hbm_fraction=0.5, max_bytes=0, round_traj_steps=0.0, block=0, waves_per_simd=0,
-                 slice_steps=0.0):
+                 slice_steps=0.0, slice_min_cells=0):
         """slice_steps: trajectory-steps per cell per round for jobs whose
-        cells all step the same counts (0: 4096; < 0: whole runs)."""
+        cells all step the same counts (0: 1024; < 0: whole runs);
+        slice_min_cells: jobs of at most this many cells run whole
+        (0: 65536; < 0: any job may be sliced)."""
         self.engine = engine
-        cfg = ServerConfig(hbm_fraction, max_bytes, round_traj_steps, block, waves_per_simd, float(slice_steps))
+        cfg = ServerConfig(hbm_fraction, max_bytes, round_traj_steps, block, waves_per_simd, float(slice_steps),
+                           int(slice_min_cells))
         h = C.c_void_p()
         engine.check(lib().uc_server_create(engine.h, C.byref(cfg), C.byref(h)), "uc_server_create")
         self.h = h
