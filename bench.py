@@ -41,6 +41,28 @@ PEAK_FP32_VALU = 157.3e12   # MI355X published FP32 vector peak
 PEAK_FP64_VALU = 78.6e12    # MI355X published FP64 vector peak
 
 
+def valu_roofline(kernel, flops, ms, fp64, bound="valu", note=None):
+    """Roofline entry of an analysis: algorithmic flops of the launch(es)
+    (+, -, *, / = 1, FMA = 2; transcendental calls, compares and integer
+    work not counted) over the measured time, against the MI355X vector
+    peak of the precision the stages compute in."""
+    peak = PEAK_FP64_VALU if fp64 else PEAK_FP32_VALU
+    achieved = flops / (ms * 1e-3)
+    d = dict(bound=bound, kernel=kernel, achieved=achieved / 1e12, peak=peak / 1e12, unit="TFLOP/s",
+             frac=achieved / peak, flops_per_launch=float(flops), ms=ms)
+    if note:
+        d["note"] = note
+    return d
+
+
+def dp5_flops(ops, n):
+    """One Dormand-Prince 5(4) attempt: 6 new right-hand sides (FSAL), the
+    stage sums (1 + 2 + 3 + 4 + 5 terms), the 5th-order solution (5 terms)
+    and the error estimate (6 terms) as FMAs per component, and the scaled
+    RMS norm (5 flop per component)."""
+    return 6 * ops["rhs_flops"] + (2 * (15 + 5 + 6) + 5) * n
+
+
 def workload(n):
     import ucuda
     return dict(axes=[ucuda.axis(ucuda.AXIS_PARAM, 2, n, 0.05, 2.0), ucuda.axis(ucuda.AXIS_PARAM, 3, n, 1.0, 20.0)],
@@ -210,6 +232,12 @@ def extra_bif_config1(eng, threads_all):
     ref = oracle.bif_cells(step, job, range(n), threads=1)
     cpu_s = time.perf_counter() - t0
     out["cpu_oracle"] = dict(s=cpu_s, traj_steps_per_s=traj_steps / cpu_s, cores=1)
+    sf = ucuda.rk4_flops(lor.op_counts(), 3)
+    out["roofline"] = {
+        name: valu_roofline("uc_bif_" + name, traj_steps * sf, out[name]["ms"], name == "f64", bound="latency",
+                            note="1,024 cells = 16 waves on 1,024 SIMDs: each lane's dependent step chain sets the "
+                                 "time; f32c: transient in FP32, recorded window in FP64; peak tests not counted")
+        for name in ("f64", "f32c")}
     g = gpu["f64_strict"]
     same = np.array_equal(g["flags"], ref["flags"])
     for c in range(n):
@@ -274,6 +302,12 @@ def extra_bif2d(eng, n=512, stride=8):
         maps[name] = r.cpu().numpy()
         out[name] = dict(ms=float(np.mean(ms)), traj_steps_per_s=traj_steps / (np.mean(ms) / 1e3))
         progress("bif2d %s: %.1f ms per map" % (name, np.mean(ms)))
+    sf = ucuda.rk4_flops(lor.op_counts(), 3)
+    out["roofline"] = {
+        name: valu_roofline("uc_bif_%s + uc_bif_dbscan_wave" % name, traj_steps * sf, out[name]["ms"], name == "f64",
+                            note="whole call: sweep with streaming peaks, then per-cell DBSCAN (its work not "
+                                 "counted); f32c: transient in FP32, recorded window in FP64")
+        for name in ("f64", "f32c")}
     idx = np.arange(0, n, stride)
     cells = (idx[:, None] * n + idx[None, :]).ravel()
     step = oracle.KrsStep(eng.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
@@ -322,6 +356,13 @@ def extra_dft(eng, n=16384, n_freq=200, stride=64):
         out[name] = dict(ms=float(np.mean(ms)), cells_per_s=n / (np.mean(ms) / 1e3),
                          dft_terms_per_s=n * n_freq * N / (np.mean(ms) / 1e3))
         progress("dft %s: %.1f ms per sweep" % (name, np.mean(ms)))
+    # recording: 1e4 + 1 transient and 1e4 + 1 block steps per cell; DFT:
+    # 11 flop per (cell, frequency, sample)
+    flops = n * (2 * (N + 1)) * ucuda.rk4_flops(lor.op_counts(), 3) + 11.0 * n * n_freq * N
+    out["roofline"] = {
+        name: valu_roofline("uc_bif_%s (raw record) + uc_dft" % name, flops, out[name]["ms"], True,
+                            note="whole call; the DFT runs in FP64 for both modes, so the FP64 peak")
+        for name in ("f64", "f32c")}
     cells = np.arange(0, n, stride)
     step = oracle.KrsStep(eng.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
     job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], lor.values(), 0.01, 100.0, 100.0)
@@ -382,6 +423,11 @@ def extra_fastsync(eng, n=1024, stride=64):
         res[name] = r["sync_error"].cpu().numpy()
         out["grid_" + name] = dict(ms=float(np.mean(ms)), coupled_steps_per_s=n * n * steps_cell / (np.mean(ms) / 1e3))
         progress("fastsync grid %s: %.1f ms" % (name, np.mean(ms)))
+    # a coupled step: one RK4 of the slave plus the coupling term (3 N)
+    out["roofline"] = dict(grid_f64=valu_roofline(
+        "uc_fs_grid", n * n * steps_cell * (ucuda.rk4_flops(lor.op_counts(), 3) + 9), out["grid_f64"]["ms"], True,
+        note="whole call; the backward passes read the stored master window (8 N B per step) and the fast "
+             "module recomputes the forward master (not counted)"))
     idx = np.arange(0, n, stride)
     cells = (idx[:, None] * n + idx[None, :]).ravel()
     t0 = time.perf_counter()
@@ -435,6 +481,13 @@ def extra_continuation(eng, n_pts=128, n_chains=1024):
     ms, rm = _timed_calls(lambda: ucuda.run_lyap_cont(lor, many, *args, strict=True), reps=1)
     out["%d_chains" % n_chains] = dict(ms=float(np.mean(ms)), steps_per_s=n_chains * steps / (np.mean(ms) / 1e3))
     progress("continuation: %d chains %.1f ms" % (n_chains, np.mean(ms)))
+    sf = ucuda.rk4_flops(lor.op_counts(), 3)
+    out["roofline"] = {
+        "one_chain": valu_roofline("uc_cont_lle", steps * sf, out["one_chain"]["ms"], True, bound="latency",
+                                   note="one lane: a lone wave issues one instruction per ~4-5 cycles"),
+        "%d_chains" % n_chains: valu_roofline("uc_cont_lle", n_chains * steps * sf,
+                                              out["%d_chains" % n_chains]["ms"], True, bound="latency",
+                                              note="1,024 chains = 16 waves on 1,024 SIMDs")}
     step = oracle.KrsStep(eng.codegen_scheme(lor.vars, lor.params, lor.rhs), 3, 4)
     job = oracle.make_job(3, one, [0.1, 0.1, 0.1], vals, 0.01, 5.0, 20.0, NT=0.1, eps=1e-4)
     t0 = time.perf_counter()
@@ -486,6 +539,14 @@ def extra_ls(eng, n=1024, steps_scale=1.0):
             progress("LS %s %s: %.2fs, %d bounded cells" % (case, name, dt, ok))
     a32, a64 = out["attractor_f32c"], out["attractor_f64"]
     out["attractor_f32c_over_f64"] = a64["s"] / a32["s"]
+    # bounded cells only: one state-only step, then every step advances the
+    # state and 4 tangent vectors (the Gram-Schmidt per block not counted)
+    sf4 = ucuda.rk4_flops(s.op_counts(), 4, tangents=4)
+    out["roofline"] = {
+        name: valu_roofline("uc_ls_" + name, out["attractor_" + name]["bounded_cells"] * spc * sf4,
+                            out["attractor_" + name]["s"] * 1e3, name == "f64",
+                            note="attractor map; bounded cells' work only (a lower bound); whole call")
+        for name in ("f32c", "f64")}
     # full-size parity and CPU baseline on a fixed subsample of the attractor
     # map: every 64th point per axis, the whole 1e5-step horizon
     import numpy as np
@@ -550,6 +611,18 @@ def extra_basins(eng, n=2048):
         labels[name] = (r["labels"].cpu().numpy(), r["attractors"])
         out[name] = dict(s=dt, cells=n * n, accepted_steps=int(st[0].sum()), rejected_steps=int(st[1].sum()),
                          traj_steps_per_s=float(st[0].sum()) / dt, attractors=r["attractors"])
+    # float stages cannot estimate errors below 2^-20 of the state: the
+    # FP32C integration runs at max(requested, 2^-20)
+    out["f32c_tolerance"] = dict(requested_rtol=1e-9, requested_atol=1e-9, rtol=max(1e-9, 2.0 ** -20),
+                                 atol=max(1e-9, 2.0 ** -20), rule="max(requested, 2^-20)")
+    df = dp5_flops(s.op_counts(), s.n_vars)
+    out["roofline"] = {
+        name: valu_roofline("uc_bas_" + name, (out[name]["accepted_steps"] + out[name]["rejected_steps"]) * df,
+                            out[name]["s"] * 1e3, name == "f64", bound="valu+transcendental",
+                            note="DP5(4) attempts x %d flop; the forcing cos in each right-hand side (6 per "
+                                 "attempt), the step-factor log2/exp2 and the catalogue not counted; whole call"
+                                 % df)
+        for name in ("f64", "f32c")}
     # label agreement FP32C vs FP64 (catalogues mapped by period + point)
     l64, a64 = labels["f64"]
     l32, a32 = labels["f32c"]
@@ -624,6 +697,7 @@ def extra_batch(eng, n_jobs=64, seed=0):
              "chen": (1, 30.0, 45.0, 3, 20.0, 30.0)}
     b = ucuda.Batch(eng)
     lle_steps = 0
+    lle_flops = 0.0
     basins_out = []
     kinds = {"lle": 0, "basins": 0}
     for j in range(n_jobs):
@@ -646,6 +720,9 @@ def extra_batch(eng, n_jobs=64, seed=0):
         b.add_lyap(sysm, axes, ic, sysm.values(), h, transient_steps * h, t_max, 10 * h,
                    precision=ucuda.PREC_F32C, probe_seed=1000 + j)
         lle_steps += side * side * (transient_steps + 1 + int(t_max / (10 * h)) * 11)
+        ops = sysm.op_counts()
+        lle_flops += side * side * ((transient_steps + 1) * ucuda.rk4_flops(ops, sysm.n_vars) +
+                                    int(t_max / (10 * h)) * 11 * ucuda.rk4_flops(ops, sysm.n_vars, tangents=1))
         kinds["lle"] += 1
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
@@ -657,8 +734,14 @@ def extra_batch(eng, n_jobs=64, seed=0):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
     bas_acc = sum(int(o["steps"][0].sum()) for o in basins_out)
+    bas_att = sum(int(o["steps"].sum()) for o in basins_out)
     bas_cells = sum(int(o["labels"].numel()) for o in basins_out)
-    return dict(jobs=n_jobs, jobs_by_kind=kinds, s=dt, lle_traj_steps=lle_steps, basins_cells=bas_cells,
+    flops = lle_flops + bas_att * dp5_flops(systems["duffing"].op_counts(), systems["duffing"].n_vars)
+    roofline = valu_roofline("uc_lle_f32c_*_q + uc_bas_f32c_q", flops, dt * 1e3, False,
+                             note="LLE maps (state-only transient, state + tangent blocks) and basin DP5 attempts; "
+                                  "one graph launch plus the basins catalogues")
+    return dict(jobs=n_jobs, jobs_by_kind=kinds, s=dt, roofline=roofline, lle_traj_steps=lle_steps,
+                basins_cells=bas_cells,
                 basins_accepted_steps=bas_acc, traj_steps_per_s=(lle_steps + bas_acc) / dt,
                 basins_attractors=[len(o["attractors"]) for o in basins_out], tuned=tuned, **b.info())
 
@@ -827,6 +910,9 @@ def extra_serving(eng, duration_s=8.0, rate=None, seed=1, capacity=None, dump_jo
                 jobs=len(ids), jobs_by_kind=kinds, completed=st["completed"], failed=len(failed),
                 p50_ms=st["p50_ms"], p99_ms=st["p99_ms"], max_ms=st["max_ms"], **sizes,
                 fixed_step_traj_steps_per_s=st["traj_steps"] / wall, rounds=st["rounds"],
+                roofline=dict(bound="arrivals", gpu_busy_frac=st["busy_s"] / wall,
+                              note="throughput is the offered load (50 jobs/s), not a kernel bound; the kernels' "
+                                   "own rooflines are the other extras'"),
                 gpu_busy_frac=st["busy_s"] / wall, wall_s=wall, audit_fp64=audit)
 
 
