@@ -83,11 +83,12 @@ def tangent_steps_per_cell(w):
 
 
 def headline_kernel(cells):
-    """Kernel the engine launches for the FP32C map: two cells per lane
-    (uc_lle_f32c_x2) when that gives 1.5..2 lane-waves per SIMD on the 256
-    CUs of an MI355X (the engine's measured rule), else one (uc_lle_f32c)."""
+    """Kernel the engine launches for the FP32C map: the lean loop with two
+    cells per lane (uc_lle_f32c_x2) when that gives 1.5..2 lane-waves per
+    SIMD on the 256 CUs of an MI355X (the engine's measured rule), else with
+    one (uc_lle_f32c_x1)."""
     pw = ((cells + 1) // 2) / (256 * 4 * 64)
-    return "uc_lle_f32c_x2" if 1.5 < pw <= 2.0 else "uc_lle_f32c"
+    return "uc_lle_f32c_x2" if 1.5 < pw <= 2.0 else "uc_lle_f32c_x1"
 
 
 def profiled_traffic():

@@ -800,19 +800,28 @@ static const char* lyap_kernel(bool spectrum, int precision) {
                     : (precision == UC_PREC_F64 ? "uc_lle_f64" : "uc_lle_f32c");
 }
 
-// Whether a LLE map runs two cells per lane (uc_lle_*_x2): fast mode, one
-// tangent vector by the variational method, wave-uniform counts (no sweep
-// over h), blocks of 1..100 steps, and a grid shape where it measured
-// faster. With pw = pair-lane waves per SIMD: FP32C only for 1.5 < pw <= 2
-// (-6% at 512^2; 4-9% slower at pw 2.75..4.9, even at 4), FP64 for
-// pw >= 0.95 (-3..-7% from 362^2 to 1024^2). Below one wave per SIMD a lone
-// wave cannot hide latency even with two cells (1.6x slower at 256^2).
-static bool pair_lle(uc_engine* e, bool spectrum, const uc_lyap_request& r, bool step_axis, long long cells) {
-    if (spectrum || r.c.strict || r.method != UC_METHOD_VARIATIONAL || step_axis) return false;
+// Cells per lane of a LLE map on the lean loop (uc_lle_*_x1 / uc_lle_*_x2),
+// or 0 for the general kernel. The lean loop needs fast mode, one tangent
+// vector by the variational method, wave-uniform counts (no sweep over h)
+// and blocks of 1..100 steps; it gives the general kernel's bits. Two cells
+// per lane where that measured faster, with pw = pair-lane waves per SIMD:
+// FP32C for 1.5 < pw <= 2 (-6% at 512^2; 4-9% slower at pw 2.75..4.9, even
+// at 4), FP64 for pw >= 0.95 (-3..-7% from 362^2 to 1024^2). Below one wave
+// per SIMD a lone wave cannot hide latency even with two cells (1.6x slower
+// at 256^2).
+static int lle_lanes(uc_engine* e, bool spectrum, const uc_lyap_request& r, bool step_axis, long long cells) {
+    if (spectrum || r.c.strict || r.method != UC_METHOD_VARIATIONAL || step_axis) return 0;
     const long long nts = r.c.h > 0.0 ? (long long)(r.NT / r.c.h) : 0;
-    if (nts < 1 || nts > 100) return false;
+    if (nts < 1 || nts > 100) return 0;
     const double pw = (double)((cells + 1) / 2) / ((double)cu_count(e->device) * 4.0 * 64.0);
-    return r.c.precision == UC_PREC_F64 ? pw >= 0.95 : (pw > 1.5 && pw <= 2.0);
+    const bool two = r.c.precision == UC_PREC_F64 ? pw >= 0.95 : (pw > 1.5 && pw <= 2.0);
+    return two ? 2 : 1;
+}
+
+// The LLE kernel a lane rule picks, and its lane count.
+static int launch_lle(uc_engine* e, Module* m, int lanes, int precision, UcJob& j, hipStream_t st, TableBuffer* tb) {
+    const std::string name = std::string(lyap_kernel(false, precision)) + (lanes == 2 ? "_x2" : "_x1");
+    return launch(e, m, name.c_str(), j, st, tb, lanes == 2 ? (j.n_cells + 1) / 2 : j.n_cells);
 }
 
 static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const uc_lyap_output* o, void* stream, bool spectrum) {
@@ -831,12 +840,10 @@ static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const 
     fill_lyap(r, o, j);
     bool step_axis = false;
     for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
-    // Two cells per lane where the single-cell grid leaves few waves per SIMD
-    // (fast single-vector variational maps with blocks of <= 100 steps).
-    if (pair_lle(e, spectrum, *r, step_axis, j.n_cells)) {
-        const std::string name = std::string(lyap_kernel(false, r->c.precision)) + "_x2";
-        return launch(e, m, name.c_str(), j, st, tb, (j.n_cells + 1) / 2);
-    }
+    // the lean loop (one or two cells per lane) for fast single-vector
+    // variational maps with blocks of <= 100 steps
+    if (const int lanes = lle_lanes(e, spectrum, *r, step_axis, j.n_cells))
+        return launch_lle(e, m, lanes, r->c.precision, j, st, tb);
     return launch(e, m, lyap_kernel(spectrum, r->c.precision), j, st, tb);
 }
 
@@ -886,10 +893,8 @@ int uc_run_lle_noalloc(uc_engine* e, uc_system* s, const uc_lyap_request* r, con
     fill_lyap(r, o, j);
     bool step_axis = false;
     for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
-    if (pair_lle(e, false, *r, step_axis, j.n_cells)) {
-        const std::string name = std::string(lyap_kernel(false, r->c.precision)) + "_x2";
-        return launch(e, m, name.c_str(), j, st, nullptr, (j.n_cells + 1) / 2);
-    }
+    if (const int lanes = lle_lanes(e, false, *r, step_axis, j.n_cells))
+        return launch_lle(e, m, lanes, r->c.precision, j, st, nullptr);
     return launch(e, m, lyap_kernel(false, r->c.precision), j, st, nullptr);
 }
 
@@ -2050,7 +2055,7 @@ struct uc_batch {
         double cost;         // steps per cell, for longest-first ordering
         int kind;
         SliceInfo si;
-        bool pair;           // can run two cells per lane ("*_x2_q")
+        bool pair;           // can run the lean loop ("*_x1_q", two cells per lane "*_x2_q")
         double* slab;        // allocated by the first plan that slices the job
     };
     std::vector<Job> jobs;
@@ -2159,6 +2164,8 @@ int batch_build(uc_batch* b, const uc_batch_plan& p) {
                 if (he != hipSuccess) { jb.slab = nullptr; return b->fail(UC_ERR_HIP, hip_msg(he, "hipMalloc(batch slab)")); }
                 b->owned.push_back(jb.slab);
             }
+        } else if (jb.pair) {
+            kernel = kernel.substr(0, kernel.size() - 2) + "_x1_q";  // the lean loop, one cell per lane
         }
         if ((long long)b->stages.size() < n_stages) b->stages.resize((size_t)n_stages);
         for (long long k = 0; k < n_stages; ++k) {
@@ -2325,8 +2332,9 @@ int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int s
     if ((rc = batch_stage(b, s, r->c, j))) return rc;
     fill_lyap(r, o, j);
     const double cost = (double)(j.skip_steps + j.n_iters * (j.nt_steps + 1)) * (spectrum ? 1.0 + s->n_vars : 2.0);
-    // two cells per lane: the fast single-vector variational map with
-    // wave-uniform counts and blocks of 1..100 steps (uc_lle_pair)
+    // the lean loop (uc_lle_lanes, one or two cells per lane): the fast
+    // single-vector variational map with wave-uniform counts and blocks of
+    // 1..100 steps
     bool step_axis = false;
     for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
     const bool pair = !spectrum && !r->c.strict && r->method == UC_METHOD_VARIATIONAL && !step_axis &&

@@ -24,6 +24,7 @@
 #define UC_PI 3.1415926535897932384626433832795
 #define UC_E 2.7182818284590452353602874713527
 #define UC_BLOCK 256
+#define UC_LLE_BLOCK 10                    // block length the LLE lane kernels unroll fully
 
 typedef unsigned long long u64;
 
@@ -1297,26 +1298,28 @@ __device__ void uc_lyap_cell(const UcJob& j, long long lane) {
 }
 
 #if !UC_STRICT
-// Two cells per lane, cells `lane` and `lane + half`, for the fast
-// single-vector variational LLE with wave-uniform counts and blocks of at
-// most the check interval (the host picks this kernel only then). Each cell
-// runs exactly the steps, magnitude checks, rescales and switch of
-// uc_lyap_cell's fast path; the two cells' instruction streams are
+// C cells per lane (C = 1 or 2; with two, cells `lane` and `lane + half`)
+// for the fast single-vector variational LLE with wave-uniform counts and
+// blocks of at most the check interval (the host picks this kernel only
+// then). Each cell runs exactly the steps, magnitude checks, rescales and
+// switch of uc_lyap_cell's fast path. Two cells' instruction streams are
 // independent, which doubles the work a wave has in flight where the grid
-// gives few waves per SIMD. A cell found diverged keeps stepping with the
+// gives few waves per SIMD; one cell per lane is the same lean loop for
+// grids with enough waves. A cell found diverged keeps stepping with the
 // other (its results are discarded) and gets its state at the check back.
-template <class S>
-__device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
+template <class S, int C>
+__device__ void uc_lle_lanes(const UcJob& j, long long lane, long long half) {
+    static_assert(C == 1 || C == 2, "one or two cells per lane");
     typedef typename Prec<S>::T T;
     long long cell[2] = {j.cell_offset + lane, j.cell_offset + lane + half};
-    const bool has1 = lane + half < j.n_cells;
+    const bool has1 = C == 2 && lane + half < j.n_cells;
     if (!has1) cell[1] = cell[0];  // a stand-in copy of cell 0, never written
     double ad[UC_M], x0[UC_N], hd = 0.0;
     T a[2][UC_M];
     S s[2], snap[2];
-    bool dead[2], ok[2];
+    bool dead[2] = {true, true}, ok[2] = {false, false};
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < C; ++c) {
         ok[c] = uc_setup(j, cell[c], ad, x0, hd);
 #pragma unroll
         for (int i = 0; i < UC_M; ++i) a[c][i] = Prec<S>::to_t(ad[i]);
@@ -1333,22 +1336,22 @@ __device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
     // transient: uc_loop_u with pre = 1 and no observer, per cell
     for (long long i = 0; i < j.skip_steps;) {
 #pragma unroll
-        for (int c = 0; c < 2; ++c) uc_rk4(s[c], a[c], h, (T)t);
+        for (int c = 0; c < C; ++c) uc_rk4(s[c], a[c], h, (T)t);
         if (UC_USES_TIME) t += hdd;
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < C; ++c)
             if (!dead[c] && uc_diverged(s[c], j.max_value)) { dead[c] = true; snap[c] = s[c]; }
-        if (uc_wave_all(dead[0] && dead[1])) break;
+        if (uc_wave_all(dead[0] && (C == 1 || dead[1]))) break;
         const long long stop = (j.skip_steps - i > UC_CHECK_INTERVAL) ? i + UC_CHECK_INTERVAL : j.skip_steps;
 #pragma unroll 2
         for (++i; i < stop; ++i) {
 #pragma unroll
-            for (int c = 0; c < 2; ++c) uc_rk4(s[c], a[c], h, (T)t);
+            for (int c = 0; c < C; ++c) uc_rk4(s[c], a[c], h, (T)t);
             if (UC_USES_TIME) t += hdd;
         }
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) uc_rk4(s[c], a[c], h, (T)t);  // the trailing step
+    for (int c = 0; c < C; ++c) uc_rk4(s[c], a[c], h, (T)t);  // the trailing step
     if (UC_USES_TIME) t += hdd;
 
     T V[2][UC_N], Vs[2][UC_N], start2[2];
@@ -1356,7 +1359,7 @@ __device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
     bool ref_mode[2];
     double lsum[2];
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < C; ++c) {
         double z[UC_N];
         uc_probe_basis<1>(j.probe_seed, cell[c], z);
 #pragma unroll
@@ -1369,30 +1372,43 @@ __device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
     const T big = sizeof(T) == 4 ? (T)__FLT_MAX__ : (T)1e300;
     const int nts = (int)j.nt_steps;
     for (long long b = 0; b < j.n_iters; ++b) {
-        // uc_block_tan_u for both cells, with the magnitude check and the
+        // uc_block_tan_u for each cell, with the magnitude check and the
         // power-of-two rescale as selects: the per-cell branches of the
         // check cost more than the check at blocks of ~10 steps
 #pragma unroll
-        for (int c = 0; c < 2; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
+        for (int c = 0; c < C; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
         if (UC_USES_TIME) t += hdd;
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < C; ++c) {
             const bool now = !dead[c] && uc_diverged_u(s[c], j);
             uc_select(snap[c], s[c], now);  // a lane found diverged keeps its state at the check
             dead[c] = dead[c] || now;
         }
-        if (uc_wave_all(dead[0] && dead[1])) break;
-#pragma unroll 4
-        for (int i = 0; i < nts; ++i) {
+        // the rest of the block; blocks of UC_LLE_BLOCK steps (h = NT / 10,
+        // the usual renormalisation interval) run as straight-line code
+        if (nts == UC_LLE_BLOCK) {
 #pragma unroll
-            for (int c = 0; c < 2; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
-            if (UC_USES_TIME) t += hdd;
+            for (int i = 0; i < UC_LLE_BLOCK; ++i) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
+                if (UC_USES_TIME) t += hdd;
+            }
+        } else {
+#pragma unroll 4
+            for (int i = 0; i < nts; ++i) {
+#pragma unroll
+                for (int c = 0; c < C; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
+                if (UC_USES_TIME) t += hdd;
+            }
         }
+        // a wave whose cells have all diverged stops here: its extra steps
+        // changed no result (each such cell's output comes from its check)
+        if (uc_wave_all(dead[0] && (C == 1 || dead[1]))) break;
         // the fast path's renormalisation: |V|^2 back into [0.5, 2) by a
         // power of two (k = 0 leaves V as it is on the other lanes)
-        bool need[2];
+        bool need[2] = {false, false};
 #pragma unroll
-        for (int c = 0; c < 2; ++c) {
+        for (int c = 0; c < C; ++c) {
             T n2 = (T)0.0;
 #pragma unroll
             for (int i = 0; i < UC_N; ++i) n2 = n2 + V[c][i] * V[c][i];
@@ -1406,9 +1422,9 @@ __device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
         }
         // the rest switch to (or stay in) the reference's unit
         // renormalisation in double; Vs still holds their block-start vector
-        if (!uc_wave_all(!need[0] && !need[1])) {
+        if (!uc_wave_all(!need[0] && (C == 1 || !need[1]))) {
 #pragma unroll
-            for (int c = 0; c < 2; ++c) {
+            for (int c = 0; c < C; ++c) {
                 if (!need[c]) continue;
                 double inv0 = 1.0;
                 if (!ref_mode[c]) {
@@ -1434,12 +1450,12 @@ __device__ void uc_lle_pair(const UcJob& j, long long lane, long long half) {
             }
         }
 #pragma unroll
-        for (int c = 0; c < 2; ++c)
+        for (int c = 0; c < C; ++c)
 #pragma unroll
             for (int i = 0; i < UC_N; ++i) Vs[c][i] = V[c][i];
     }
 #pragma unroll
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < C; ++c) {
         if (c == 1 && !has1) break;
         const long long out = lane + (c == 1 ? half : 0);
         if (!ok[c]) {
@@ -2339,15 +2355,22 @@ UC_PERSISTENT(uc_lle_f32c_s, (uc_lyap_cell<1, StateF32C, true>))
 UC_PERSISTENT(uc_ls_f64_s, (uc_lyap_cell<UC_N, StateF64, true>))
 UC_PERSISTENT(uc_ls_f32c_s, (uc_lyap_cell<UC_N, StateF32C, true>))
 #if !UC_STRICT
-// Two cells per lane (uc_lle_pair): lane l runs cells l and l + ceil(n / 2).
+// The lean LLE loop (uc_lle_lanes) with one cell per lane.
+template <class S>
+__device__ __forceinline__ void uc_lle_one(const UcJob& j, long long lane) { uc_lle_lanes<S, 1>(j, lane, 0); }
+UC_DIRECT(uc_lle_f64_x1, uc_lle_one<StateF64>)
+UC_DIRECT(uc_lle_f32c_x1, uc_lle_one<StateF32C>)
+UC_PERSISTENT(uc_lle_f64_x1_q, uc_lle_one<StateF64>)
+UC_PERSISTENT(uc_lle_f32c_x1_q, uc_lle_one<StateF32C>)
+// Two cells per lane (uc_lle_lanes): lane l runs cells l and l + ceil(n / 2).
 #define UC_DIRECT_PAIR(NAME, CELL)                                              \
     extern "C" __global__ void __launch_bounds__(UC_BLOCK) NAME(UcJob j) {      \
         const long long half = (j.n_cells + 1) / 2;                             \
         const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x; \
         if (lane < half) CELL(j, lane, half);                                   \
     }
-UC_DIRECT_PAIR(uc_lle_f64_x2, uc_lle_pair<StateF64>)
-UC_DIRECT_PAIR(uc_lle_f32c_x2, uc_lle_pair<StateF32C>)
+UC_DIRECT_PAIR(uc_lle_f64_x2, (uc_lle_lanes<StateF64, 2>))
+UC_DIRECT_PAIR(uc_lle_f32c_x2, (uc_lle_lanes<StateF32C, 2>))
 // Persistent form (whole runs only): a chunk is 2 * blockDim cells, lane l
 // of the chunk runs its cells l and l + blockDim.
 #define UC_PERSISTENT_PAIR(NAME, CELL)                                                              \
@@ -2373,8 +2396,8 @@ UC_DIRECT_PAIR(uc_lle_f32c_x2, uc_lle_pair<StateF32C>)
             if (lane < jobs[lo].n_cells) CELL(jobs[lo], lane, (long long)blockDim.x);              \
         }                                                                                           \
     }
-UC_PERSISTENT_PAIR(uc_lle_f64_x2_q, uc_lle_pair<StateF64>)
-UC_PERSISTENT_PAIR(uc_lle_f32c_x2_q, uc_lle_pair<StateF32C>)
+UC_PERSISTENT_PAIR(uc_lle_f64_x2_q, (uc_lle_lanes<StateF64, 2>))
+UC_PERSISTENT_PAIR(uc_lle_f32c_x2_q, (uc_lle_lanes<StateF32C, 2>))
 #endif
 
 // ================================================================ basins ==
