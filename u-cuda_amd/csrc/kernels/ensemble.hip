@@ -91,6 +91,16 @@ __device__ __forceinline__ bool uc_sweeps_step(const UcJob& j) {
 
 // --------------------------------------------------------- state policies --
 // F64 state: plain doubles.
+// A value the optimiser must treat as computed, not loaded. Without it a
+// chain of selects among state components becomes a load through a selected
+// address: the state array is then indexed dynamically, stays in memory, and
+// the backend moves it to LDS, so every step goes through LDS. A chain that
+// starts from such a value has no pair of loads to fold.
+template <typename T> __device__ __forceinline__ T uc_opaque(T v) {
+    asm("" : "+v"(v));
+    return v;
+}
+
 struct StateF64 {
     double x[UC_N];
     __device__ __forceinline__ void load(const double* x0) {
@@ -100,7 +110,7 @@ struct StateF64 {
     __device__ __forceinline__ double get(int i) const { return x[i]; }
     // component q (0 <= q < N) selected without indexing the array
     __device__ __forceinline__ double get_sel(int q) const {
-        double v = x[0];
+        double v = uc_opaque(x[0]);
 #pragma unroll
         for (int i = 1; i < UC_N; ++i) v = (i == q) ? x[i] : v;
         return v;
@@ -120,7 +130,7 @@ struct StateF32C {
     __device__ __forceinline__ double get(int i) const { return (double)hi[i] - (double)c[i]; }
     // the pair is selected in float and converted once (same value as get(q))
     __device__ __forceinline__ double get_sel(int q) const {
-        float h = hi[0], cc = c[0];
+        float h = uc_opaque(hi[0]), cc = uc_opaque(c[0]);
 #pragma unroll
         for (int i = 1; i < UC_N; ++i) {
             h = (i == q) ? hi[i] : h;
