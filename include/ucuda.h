@@ -394,13 +394,15 @@ typedef struct uc_batch_plan {
     int waves_per_simd;    /* resident waves per SIMD, 1..8 */
     double slice_steps;    /* trajectory-steps per cell per launch, 0 = whole runs */
     int cells_per_lane;    /* 1 or 2 */
+    int vector_lanes;      /* 1: spectrum maps run one lane per tangent vector
+                              (N lanes per cell); 0: one lane per cell */
     double ms;             /* set by the tuner: the plan's measured time */
 } uc_batch_plan;
 int uc_batch_set_plan(uc_batch* b, const uc_batch_plan* p);
 int uc_batch_get_plan(const uc_batch* b, uc_batch_plan* p);
 /* autotuner: times the batch for each candidate plan (launch shapes with
- * whole runs, then steps per launch and cells per lane at the best shape)
- * and keeps the fastest */
+ * whole runs, then steps per launch, cells per lane and lanes per tangent
+ * vector at the best shape) and keeps the fastest */
 int uc_batch_tune_plan(uc_batch* b, void* stream, uc_batch_plan* best);
 /* launch shape only: block (64/128/256 lanes) and resident waves per SIMD (1..8) */
 int uc_batch_set_launch(uc_batch* b, int block, int waves_per_simd);

@@ -86,12 +86,13 @@ _ALONE = {}
 
 # plans: whole runs; stages of 512 and 2048 traj-steps per cell (every job
 # is sliced, the basins maps one or two sections per stage); two cells per
-# lane for the LLE maps that can
+# lane for the LLE maps that can; one lane per tangent vector for the
+# spectrum map (alone, its 132 cells take the same N-lanes kernel)
 PLANS = [None, dict(slice_steps=512.0), dict(block=64, waves_per_simd=2, slice_steps=2048.0),
-         dict(cells_per_lane=2)]
+         dict(cells_per_lane=2), dict(block=128, vector_lanes=1)]
 
 
-@pytest.mark.parametrize("plan", PLANS, ids=["whole", "slice512", "slice2048_b64", "pair"])
+@pytest.mark.parametrize("plan", PLANS, ids=["whole", "slice512", "slice2048_b64", "pair", "vector_lanes"])
 def test_batch_equals_alone_and_replays(gpu_engine, plan):
     print("case: 8 heterogeneous jobs (5 systems, lle/ls/bif/basins, f64/f32c) in one graph launch, plan %s"
           % plan, flush=True)
@@ -137,12 +138,17 @@ def test_batch_equals_alone_and_replays(gpu_engine, plan):
 
 
 def test_batch_tuner_and_shapes(gpu_engine):
-    print("case: tuner over launch shapes, steps per launch and cells per lane; every plan gives identical "
+    print("case: tuner over launch shapes, steps per launch, cells per lane and lanes per tangent vector; "
+          "every plan gives identical "
           "results", flush=True)
     ros = ucuda.preset("rossler", gpu_engine)
     b = ucuda.Batch(gpu_engine)
     outs = [b.add_lyap(ros, [ucuda.axis(1, 2, 64, 0.05, 2.0), ucuda.axis(1, 3, n, 1.0, 20.0)], [1, 1, 1],
                        ros.values(), 0.01, 5.0, 200.0, 0.1, precision=ucuda.PREC_F32C) for n in (3, 50, 200)]
+    hr = ucuda.preset("hyper_rossler", gpu_engine)
+    outs.append(b.add_lyap(hr, [ucuda.axis(2, 0, 40, -12.0, -8.0), ucuda.axis(2, 3, 40, 8.1, 12.1)],
+                           [-10.0, -6.0, 0.0, 10.1], hr.values(), 0.005, 2.0, 20.0, 0.1, spectrum=True,
+                           precision=ucuda.PREC_F32C))
     stream = torch.cuda.Stream()
     with torch.cuda.stream(stream):
         best = b.tune(stream)
@@ -151,9 +157,11 @@ def test_batch_tuner_and_shapes(gpu_engine):
         ref = [o["values"].clone() for o in outs]
         assert best["block"] in (64, 256) and 1 <= best["waves_per_simd"] <= 8 and best["ms"] > 0
         assert best["slice_steps"] in (0.0, 16384.0, 4096.0) and best["cells_per_lane"] in (1, 2)
+        assert best["vector_lanes"] in (0, 1)
         assert b.plan() == best
-        for blk, w, sl, cpl in ((64, 2, 0.0, 1), (256, 8, 1000.0, 1), (128, 1, 0.0, 2), (256, 4, 300.0, 1)):
-            b.set_plan(blk, w, sl, cpl)
+        for blk, w, sl, cpl, vl in ((64, 2, 0.0, 1, 0), (256, 8, 1000.0, 1, 0), (128, 1, 0.0, 2, 0),
+                                    (256, 4, 300.0, 1, 0), (64, 4, 0.0, 2, 1), (256, 2, 0.0, 1, 1)):
+            b.set_plan(blk, w, sl, cpl, vl)
             b.launch(stream)
-            for o, r in zip(outs, ref):
-                assert torch.equal(o["values"], r), (blk, w, sl, cpl)
+            for o, r in zip(outs, ref):  # NaN exponents (an escape inside the last block) compare equal
+                assert torch.equal(o["values"].nan_to_num(nan=-777.0), r.nan_to_num(nan=-777.0)), (blk, w, sl, cpl, vl)

@@ -438,6 +438,44 @@ def test_ls_hyper_rossler_random_ics(gpu_engine):
                                    np.median(err[:, ok & fin], axis=1)))
 
 
+def test_ls_spread_lanes_equal_one_lane(gpu_engine):
+    """Spectrum maps on grids of few waves run each cell's N tangent vectors
+    on N lanes (uc_ls_*_xn, Gram-Schmidt across the lanes). A 256 x 256 map
+    (one wave per SIMD: one lane per cell) and small ranges of it (the N-lane
+    kernel) give the same bits: exponents, flags and final states, with
+    cells that fail the magnitude check, in strict FP64, fast FP64 and
+    FP32C; hyper-Rossler (N = 4, 16 cells per wave) and Lorenz (N = 3, 21
+    cells per wave and one idle lane)."""
+    cases = [("hyper_rossler", [ucuda.axis(ucuda.AXIS_VAR, 0, 256, -12.0, -8.0),
+                                ucuda.axis(ucuda.AXIS_VAR, 3, 256, 8.1, 12.1)], [-10.0, -6.0, 0.0, 10.1],
+              0.005, 1.0, 4.0, 40.0),
+             ("lorenz", [ucuda.axis(ucuda.AXIS_PARAM, 2, 256, 20.0, 200.0),
+                         ucuda.axis(ucuda.AXIS_PARAM, 1, 256, 5.0, 16.0)], [0.1, 0.1, 0.1], 0.01, 2.0, 6.0, 200.0)]
+    for name, axes, ic, h, tt, tmax, maxv in cases:
+        s = ucuda.preset(name, gpu_engine)
+        for kw in (dict(strict=True), dict(precision=ucuda.PREC_F64), dict(precision=ucuda.PREC_F32C)):
+            print("case: %s spectrum 256x256 whole vs ranges, %s" % (name, kw), flush=True)
+            args = (s, axes, ic, s.values(), h, tt, tmax, 0.1)
+            opt = dict(spectrum=True, max_value=maxv, want_final_state=True, **kw)
+            whole = ucuda.run_lyap(*args, **opt)
+            wv = whole["values"].cpu().numpy()
+            wf = whole["flags"].cpu().numpy()
+            ws = whole["final_state"].cpu().numpy()
+            dead = 0
+            for b, c in ((0, 1000), (1000, 37), (40000, 4099), (65536 - 21, 21)):
+                part = ucuda.run_lyap(*args, cells=(b, c), **opt)
+                pv = part["values"].cpu().numpy()
+                pf = part["flags"].cpu().numpy()
+                np.testing.assert_array_equal(pf, wf[b:b + c])
+                np.testing.assert_array_equal(pv, wv[:, b:b + c])
+                np.testing.assert_array_equal(part["final_state"].cpu().numpy(), ws[:, b:b + c])
+                dead += int((pf == -1).sum())
+            # the oracle's flags on every 7th compared cell: ~0.4% (hyper-Rossler, max
+            # value 40) and ~56% (Lorenz, 200) of the cells diverge
+            print("  %d cells compared, %d of them diverged" % (1000 + 37 + 4099 + 21, dead))
+            assert 0 < dead < 5000
+
+
 def test_custom_krs_bodies(gpu_engine):
     print("case: custom KRS step bodies (Euler, midpoint, RK4, DOPRI78, CD text) -> bif + FD LLE vs oracle",
           flush=True)

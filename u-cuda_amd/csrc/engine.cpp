@@ -818,6 +818,31 @@ static int lle_lanes(uc_engine* e, bool spectrum, const uc_lyap_request& r, bool
     return two ? 2 : 1;
 }
 
+// Whether a spectrum map runs its N tangent vectors on N lanes per cell
+// (uc_ls_*_xn): variational method, counts the same for every cell (no
+// sweep over h), N >= 2, and a grid with fewer one-lane waves per SIMD than
+// UC_SPREAD_BELOW, where one lane per cell leaves the SIMDs latency-bound.
+// Measured (hyper-Rossler, 1.1e4 steps): up to 128^2 cells (0.25 waves) N
+// lanes take 3.1 ms against 6.1 ms; at 181^2 (0.5) 3.9 / 5.6 ms (FP32C /
+// FP64) against 6.2 / 6.0 ms; at 256^2 (1.0) 7.6 / 10.6 ms against 6.2 /
+// 6.5 ms; Lorenz alike (3.1 against 4.9 ms on small grids).
+#ifndef UC_SPREAD_BELOW
+#define UC_SPREAD_BELOW 0.6
+#endif
+static bool ls_spread(uc_engine* e, const uc_lyap_request& r, int n_vars, bool step_axis, long long cells) {
+    if (r.method != UC_METHOD_VARIATIONAL || step_axis || n_vars < 2) return false;
+    const double w1 = (double)cells / ((double)cu_count(e->device) * 4.0 * 64.0);
+    return w1 < UC_SPREAD_BELOW;
+}
+
+// Launch of the spread spectrum kernel: floor(64 / N) cells per wave.
+static int launch_spread(uc_engine* e, Module* m, int precision, int n_vars, UcJob& j, hipStream_t st,
+                         TableBuffer* tb) {
+    const long long per = 64 / n_vars;
+    const long long waves = (j.n_cells + per - 1) / per;
+    return launch(e, m, precision == UC_PREC_F64 ? "uc_ls_f64_xn" : "uc_ls_f32c_xn", j, st, tb, waves * 64);
+}
+
 // The LLE kernel a lane rule picks, and its lane count.
 static int launch_lle(uc_engine* e, Module* m, int lanes, int precision, UcJob& j, hipStream_t st, TableBuffer* tb) {
     const std::string name = std::string(lyap_kernel(false, precision)) + (lanes == 2 ? "_x2" : "_x1");
@@ -844,6 +869,9 @@ static int run_lyap(uc_engine* e, uc_system* s, const uc_lyap_request* r, const 
     // variational maps with blocks of <= 100 steps
     if (const int lanes = lle_lanes(e, spectrum, *r, step_axis, j.n_cells))
         return launch_lle(e, m, lanes, r->c.precision, j, st, tb);
+    // spectrum maps on small grids: one lane per tangent vector
+    if (spectrum && ls_spread(e, *r, s->n_vars, step_axis, j.n_cells))
+        return launch_spread(e, m, r->c.precision, s->n_vars, j, st, tb);
     return launch(e, m, lyap_kernel(spectrum, r->c.precision), j, st, tb);
 }
 
@@ -2057,6 +2085,7 @@ struct uc_batch {
         SliceInfo si;
         bool pair;           // can run the lean loop ("*_x1_q", two cells per lane "*_x2_q")
         double* slab;        // allocated by the first plan that slices the job
+        int spread_n;        // N when a spectrum map can run one lane per tangent vector ("*_xn_q"), else 0
     };
     std::vector<Job> jobs;
     // basins jobs: the whole-grid descriptor for the catalogue and labels
@@ -2073,6 +2102,7 @@ struct uc_batch {
         Module* m;
         std::string kernel;
         int cells_per_chunk_lane;  // 1, or 2 for the pair kernels
+        int spread_n;              // N for the N-lanes-per-cell kernels (floor(64 / N) cells per wave), else 0
         std::vector<UcJob> table;
         UcJob* d_jobs = nullptr;
         long long* d_start = nullptr;
@@ -2081,7 +2111,7 @@ struct uc_batch {
     std::vector<std::vector<Group>> stages;
     unsigned long long* d_counters = nullptr;
     int n_counters = 0;
-    uc_batch_plan plan{256, 4, 0.0, 1, 0.0};
+    uc_batch_plan plan{256, 4, 0.0, 1, 0, 0.0};
     uc_batch_plan built{};  // plan the tables were built for (block 0: none)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -2127,15 +2157,16 @@ int cu_count(int device) {
 }
 
 void batch_add(uc_batch* b, Module* m, const std::string& kernel, const UcJob& j, double cost, int kind, int n_vars,
-               bool pair) {
-    uc_batch::Job jb{m, kernel, j, cost, kind, slice_info(kind, j, n_vars, cost), pair, nullptr};
+               bool pair, int spread_n = 0) {
+    uc_batch::Job jb{m, kernel, j, cost, kind, slice_info(kind, j, n_vars, cost), pair, nullptr, spread_n};
     b->jobs.push_back(jb);
     ++m->pins;
     b->invalidate();
 }
 
 bool same_plan(const uc_batch_plan& a, const uc_batch_plan& p) {
-    return a.block == p.block && a.slice_steps == p.slice_steps && a.cells_per_lane == p.cells_per_lane;
+    return a.block == p.block && a.slice_steps == p.slice_steps && a.cells_per_lane == p.cells_per_lane &&
+           a.vector_lanes == p.vector_lanes;
 }
 
 // Builds the per-stage, per-group job tables for plan `p`.
@@ -2152,7 +2183,11 @@ int batch_build(uc_batch* b, const uc_batch_plan& p) {
         std::string kernel = jb.kernel;
         int cpl = 1;
         long long n_stages = 1, ups = 0;
-        if (p.cells_per_lane == 2 && jb.pair) {
+        int spread = 0;
+        if (p.vector_lanes && jb.spread_n) {
+            kernel = kernel.substr(0, kernel.size() - 2) + "_xn_q";  // one lane per tangent vector
+            spread = jb.spread_n;
+        } else if (p.cells_per_lane == 2 && jb.pair) {
             kernel = kernel.substr(0, kernel.size() - 2) + "_x2_q";
             cpl = 2;
         } else if (p.slice_steps > 0.0 && jb.si.units > 0 && (ups = slice_units(jb.si, p.slice_steps)) < jb.si.units) {
@@ -2178,7 +2213,7 @@ int batch_build(uc_batch* b, const uc_batch_plan& p) {
             auto& st = b->stages[(size_t)k];
             size_t g = 0;
             while (g < st.size() && !(st[g].m == jb.m && st[g].kernel == kernel)) ++g;
-            if (g == st.size()) st.push_back(uc_batch::Group{jb.m, kernel, cpl, {}, nullptr, nullptr, 0});
+            if (g == st.size()) st.push_back(uc_batch::Group{jb.m, kernel, cpl, spread, {}, nullptr, nullptr, 0});
             st[g].table.push_back(j);
         }
     }
@@ -2186,7 +2221,8 @@ int batch_build(uc_batch* b, const uc_batch_plan& p) {
     for (auto& st : b->stages)
         for (auto& g : st) {
             std::vector<long long> start(g.table.size() + 1, 0);
-            const long long chunk = (long long)p.block * g.cells_per_chunk_lane;
+            const long long chunk = g.spread_n ? (long long)(p.block / 64) * (64 / g.spread_n)
+                                               : (long long)p.block * g.cells_per_chunk_lane;
             for (size_t i = 0; i < g.table.size(); ++i) start[i + 1] = start[i] + (g.table[i].n_cells + chunk - 1) / chunk;
             g.total_chunks = start.back();
             hipError_t he = hipMalloc((void**)&g.d_jobs, sizeof(UcJob) * g.table.size());
@@ -2264,6 +2300,7 @@ int check_plan(uc_batch* b, const uc_batch_plan& p) {
     if (p.waves_per_simd < 1 || p.waves_per_simd > 8) return b->fail(UC_ERR_ARGUMENT, "waves_per_simd must be in [1, 8]");
     if (!(p.slice_steps >= 0.0)) return b->fail(UC_ERR_ARGUMENT, "slice_steps must be >= 0");
     if (p.cells_per_lane != 1 && p.cells_per_lane != 2) return b->fail(UC_ERR_ARGUMENT, "cells_per_lane must be 1 or 2");
+    if (p.vector_lanes != 0 && p.vector_lanes != 1) return b->fail(UC_ERR_ARGUMENT, "vector_lanes must be 0 or 1");
     return UC_OK;
 }
 
@@ -2339,8 +2376,11 @@ int uc_batch_add_lyap(uc_batch* b, uc_system* s, const uc_lyap_request* r, int s
     for (int i = 0; i < r->c.n_axes; ++i) step_axis |= r->c.axis[i].kind == UC_AXIS_STEP;
     const bool pair = !spectrum && !r->c.strict && r->method == UC_METHOD_VARIATIONAL && !step_axis &&
                       j.nt_steps >= 1 && j.nt_steps <= 100;
+    // one lane per tangent vector: spectrum maps by the variational method
+    // with wave-uniform counts (uc_ls_spread)
+    const int spread_n = spectrum && r->method == UC_METHOD_VARIATIONAL && !step_axis && s->n_vars >= 2 ? s->n_vars : 0;
     batch_add(b, m, std::string(lyap_kernel(spectrum != 0, r->c.precision)) + "_q", j, cost,
-              spectrum ? kKindLs : kKindLle, s->n_vars, pair);
+              spectrum ? kKindLs : kKindLle, s->n_vars, pair, spread_n);
     if (job_id) *job_id = (int)b->jobs.size() - 1;
     return UC_OK;
 }
@@ -2393,7 +2433,8 @@ int uc_batch_set_plan(uc_batch* b, const uc_batch_plan* p) {
     int rc = check_plan(b, *p);
     if (rc) return rc;
     if (p->block != b->plan.block || p->waves_per_simd != b->plan.waves_per_simd ||
-        p->slice_steps != b->plan.slice_steps || p->cells_per_lane != b->plan.cells_per_lane)
+        p->slice_steps != b->plan.slice_steps || p->cells_per_lane != b->plan.cells_per_lane ||
+        p->vector_lanes != b->plan.vector_lanes)
         b->drop_graph();
     b->plan = *p;
     b->plan.ms = 0.0;
@@ -2435,14 +2476,18 @@ int uc_batch_tune_plan(uc_batch* b, void* stream, uc_batch_plan* best_out) {
     };
     for (int blk : {64, 256})
         for (int w : {2, 4, 8})
-            if (!rc) rc = time_plan(uc_batch_plan{blk, w, 0.0, 1, 0.0});
-    bool any_pair = false, any_slice = false;
-    for (auto& jb : b->jobs) { any_pair |= jb.pair; any_slice |= jb.si.units > 0; }
+            if (!rc) rc = time_plan(uc_batch_plan{blk, w, 0.0, 1, 0, 0.0});
+    bool any_pair = false, any_slice = false, any_spread = false;
+    for (auto& jb : b->jobs) { any_pair |= jb.pair; any_slice |= jb.si.units > 0; any_spread |= jb.spread_n > 0; }
     const uc_batch_plan shape = best;
     if (any_slice)
         for (double sl : {16384.0, 4096.0})
-            if (!rc) rc = time_plan(uc_batch_plan{shape.block, shape.waves_per_simd, sl, 1, 0.0});
-    if (any_pair && !rc) rc = time_plan(uc_batch_plan{shape.block, shape.waves_per_simd, 0.0, 2, 0.0});
+            if (!rc) rc = time_plan(uc_batch_plan{shape.block, shape.waves_per_simd, sl, 1, 0, 0.0});
+    if (any_pair && !rc) rc = time_plan(uc_batch_plan{shape.block, shape.waves_per_simd, 0.0, 2, 0, 0.0});
+    if (any_spread && !rc) {
+        rc = time_plan(uc_batch_plan{shape.block, shape.waves_per_simd, 0.0, 1, 1, 0.0});
+        if (any_pair && !rc) rc = time_plan(uc_batch_plan{shape.block, shape.waves_per_simd, 0.0, 2, 1, 0.0});
+    }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
     if (rc) return rc;

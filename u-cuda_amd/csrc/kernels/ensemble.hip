@@ -1494,6 +1494,129 @@ __device__ void uc_lle_lanes(const UcJob& j, long long lane, long long half) {
 }
 #endif
 
+// ------------------------------------------- spectrum, one lane per vector --
+// A value of lane `src` of the wave (every lane of the wave must be active).
+__device__ __forceinline__ double uc_lane_f64(double v, int src) { return __shfl(v, src, 64); }
+
+// The Lyapunov spectrum (K = N tangent vectors, variational method, counts
+// the same for every cell) with a cell's vectors on N neighbouring lanes of
+// one wave: lane k of the group steps the state (the same arithmetic, so the
+// same values, on every lane of the group) and tangent vector k, and the
+// modified Gram-Schmidt of each block runs across the group: for q = 0..K-1
+// lane q normalises its vector and the group reads it, lanes k > q subtract
+// their projection on it. Every value is the one uc_lyap_cell<N> computes
+// (same operations in the same order), so the results are bit-identical to
+// it, with N times the lanes and 1/N of the tangent work per lane: for grids
+// with few waves per SIMD, where one lane per cell is latency-bound.
+// `has`: the group holds a cell of the job (else it runs a stand-in copy of
+// the cell and writes nothing); every lane of the wave stays active to the
+// end, as the group reads need. A lane whose cell diverged keeps stepping
+// (its group with it: the state, hence the verdict, is the group's) and its
+// results come from its check.
+template <class S>
+__device__ void uc_ls_spread(const UcJob& j, long long lane, bool has, int k, int base) {
+    constexpr int K = UC_N;
+    static_assert(UC_N <= 64, "a cell's vectors on the lanes of one wave");
+    typedef typename Prec<S>::T T;
+    const long long cell = j.cell_offset + lane;
+    double ad[UC_M], x0[UC_N], hd;
+    const bool ok = uc_setup(j, cell, ad, x0, hd);
+    T a[UC_M];
+#pragma unroll
+    for (int i = 0; i < UC_M; ++i) a[i] = Prec<S>::to_t(ad[i]);
+    const T h = Prec<S>::to_t(hd);
+    S s;
+    s.load(x0);
+    double t = 0.0;
+    NoObserver none;
+    bool alive = ok && uc_loop_u(s, a, h, t, j.skip_steps, 1, 0, j.max_value, none) != 0;
+    S fin = s;  // the state written out: at the check that found the cell diverged, else at the end
+
+    // the cell's starting basis (the same on every lane of the group), row k
+    double z[K * UC_N], d[K];
+    uc_probe_basis<K>(j.probe_seed, cell, z);
+    if (K > 1) uc_mgs<K>(z, d);
+    T V[UC_N];
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) {
+        double zi = uc_opaque(z[i]);
+#pragma unroll
+        for (int q = 1; q < K; ++q) zi = (k == q) ? z[q * UC_N + i] : zi;
+        V[i] = Prec<S>::to_t(zi);
+    }
+    double sum = 0.0, prod = 1.0;
+    int esum = 0;
+    for (long long b = 0; b < j.n_iters; ++b) {
+        const int r = uc_loop_tan_u<1>(s, V, a, h, t, j.nt_steps, j.max_value);
+        uc_select(fin, s, alive);  // a live cell's state; a cell that died here, at its check
+        alive = alive && r != 0;
+        if (uc_wave_all(!alive)) break;
+        // uc_mgs<K> across the group
+        double v[UC_N];
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) v[i] = (double)V[i];
+        double dk = 0.0;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            if (k == q) {
+                double nrm = 0.0;
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) nrm = nrm + v[i] * v[i];
+                nrm = sqrt(nrm);
+                dk = nrm;
+#if UC_STRICT
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) v[i] = (nrm == 0.0) ? 0.0 : v[i] / nrm;
+#else
+                const double inv = (nrm == 0.0) ? 0.0 : 1.0 / nrm;
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) v[i] = v[i] * inv;
+#endif
+            }
+            if (q + 1 < K) {
+                double vq[UC_N];
+#pragma unroll
+                for (int i = 0; i < UC_N; ++i) vq[i] = uc_lane_f64(v[i], base + q);
+                if (k > q) {
+                    double r2 = 0.0;
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) r2 = r2 + v[i] * vq[i];
+#pragma unroll
+                    for (int i = 0; i < UC_N; ++i) v[i] = v[i] - r2 * vq[i];
+                }
+            }
+        }
+        if (alive) {
+#if UC_STRICT
+            sum = sum + log(dk);
+#else
+            int e;
+            prod = __builtin_frexp(prod * dk, &e);
+            esum += e;
+#endif
+        }
+#pragma unroll
+        for (int i = 0; i < UC_N; ++i) V[i] = Prec<S>::to_t(v[i]);
+    }
+#if !UC_STRICT
+    sum = log(prod) + (double)esum * UC_LN2;
+#endif
+    if (!has) return;
+    if (!ok) {
+        j.out0[(long long)k * j.stride + lane] = __builtin_nan("");
+        if (k == 0) j.flag[lane] = -1;
+        return;
+    }
+    j.out0[(long long)k * j.stride + lane] = alive ? sum / j.t_max : 999.0;
+    if (k == 0) {
+        j.flag[lane] = alive ? 1 : -1;
+        if (j.final_state) {
+#pragma unroll
+            for (int i = 0; i < UC_N; ++i) j.final_state[(long long)i * j.stride + lane] = fin.get(i);
+        }
+    }
+}
+
 // The same streaming finder over stored series: data[i * n_series + s].
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_peaks_series(const double* data, long long n_samples,
                                                                       long long n_series, double time_step,
@@ -2409,6 +2532,67 @@ UC_DIRECT_PAIR(uc_lle_f32c_x2, (uc_lle_lanes<StateF32C, 2>))
 UC_PERSISTENT_PAIR(uc_lle_f64_x2_q, (uc_lle_lanes<StateF64, 2>))
 UC_PERSISTENT_PAIR(uc_lle_f32c_x2_q, (uc_lle_lanes<StateF32C, 2>))
 #endif
+
+// The spectrum with N lanes per cell (uc_ls_spread): a wave holds
+// floor(64 / N) cells, cell c of the wave on lanes c * N .. c * N + N - 1;
+// the lanes past the last group, and the groups past the grid, run stand-in
+// copies of the grid's last cell.
+__device__ __forceinline__ void uc_spread_lane(long long first, long long n_cells, long long& cell, bool& has,
+                                               int& k, int& base) {
+    constexpr int K = UC_N, PER = 64 / UC_N;
+    const int w = (int)(threadIdx.x & 63);
+    const int g = w / K < PER ? w / K : PER - 1;
+    k = w / K < PER ? w % K : 0;
+    base = g * K;
+    cell = first + g;
+    has = w / K < PER && cell < n_cells;
+    if (cell >= n_cells) cell = n_cells - 1;
+}
+#define UC_DIRECT_SPREAD(NAME, CELL)                                                             \
+    extern "C" __global__ void __launch_bounds__(UC_BLOCK) NAME(UcJob j) {                       \
+        const long long wave = ((long long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;          \
+        long long cell;                                                                          \
+        bool has;                                                                                \
+        int k, base;                                                                             \
+        if (wave * (64 / UC_N) >= j.n_cells) return; /* a whole wave past the grid */            \
+        uc_spread_lane(wave * (64 / UC_N), j.n_cells, cell, has, k, base);                       \
+        CELL(j, cell, has, k, base);                                                             \
+    }
+UC_DIRECT_SPREAD(uc_ls_f64_xn, uc_ls_spread<StateF64>)
+UC_DIRECT_SPREAD(uc_ls_f32c_xn, uc_ls_spread<StateF32C>)
+// Persistent form: a chunk is the workgroup's waves x floor(64 / N) cells.
+#define UC_PERSISTENT_SPREAD(NAME, CELL)                                                            \
+    extern "C" __global__ void __launch_bounds__(UC_BLOCK) NAME(const UcJob* __restrict__ jobs,     \
+                                                                const long long* __restrict__ start, \
+                                                                int n_jobs, u64* counter) {          \
+        __shared__ long long s_chunk;                                                               \
+        const long long total = start[n_jobs];                                                      \
+        for (;;) {                                                                                  \
+            if (threadIdx.x == 0) s_chunk = (long long)atomicAdd((unsigned long long*)counter, 1ULL); \
+            __syncthreads();                                                                        \
+            const long long c = s_chunk;                                                            \
+            __syncthreads();                                                                        \
+            if (c >= total) break;                                                                  \
+            int lo = 0, hi = n_jobs - 1;                                                            \
+            while (lo < hi) {                                                                       \
+                const int mid = (lo + hi + 1) >> 1;                                                 \
+                if (start[mid] <= c) lo = mid; else hi = mid - 1;                                   \
+            }                                                                                       \
+            lo = __builtin_amdgcn_readfirstlane(lo);                                                \
+            uc_set_priority(jobs[lo].priority);                                                     \
+            const long long per_chunk = (long long)(blockDim.x >> 6) * (64 / UC_N);                \
+            long long cell;                                                                         \
+            bool has;                                                                               \
+            int k, base;                                                                            \
+            const long long first = (c - start[lo]) * per_chunk + (long long)(threadIdx.x >> 6) * (64 / UC_N); \
+            if (first < jobs[lo].n_cells) {                                                         \
+                uc_spread_lane(first, jobs[lo].n_cells, cell, has, k, base);                        \
+                CELL(jobs[lo], cell, has, k, base);                                                 \
+            }                                                                                       \
+        }                                                                                           \
+    }
+UC_PERSISTENT_SPREAD(uc_ls_f64_xn_q, uc_ls_spread<StateF64>)
+UC_PERSISTENT_SPREAD(uc_ls_f32c_xn_q, uc_ls_spread<StateF32C>)
 
 // ================================================================ basins ==
 // Adaptive Dormand-Prince 5(4) (Dormand & Prince 1980 coefficients; Hairer's

@@ -103,7 +103,7 @@ class ServerConfig(C.Structure):
 
 class BatchPlan(C.Structure):
     _fields_ = [("block", C.c_int), ("waves_per_simd", C.c_int), ("slice_steps", C.c_double),
-                ("cells_per_lane", C.c_int), ("ms", C.c_double)]
+                ("cells_per_lane", C.c_int), ("vector_lanes", C.c_int), ("ms", C.c_double)]
 
 
 class ServerStats(C.Structure):
@@ -827,24 +827,25 @@ class Batch:
     def set_launch(self, block, waves_per_simd):
         self.engine.check(lib().uc_batch_set_launch(self.h, block, waves_per_simd), "uc_batch_set_launch")
 
-    def set_plan(self, block=256, waves_per_simd=4, slice_steps=0.0, cells_per_lane=1):
-        """Launch shape, trajectory-steps per cell per launch (0: whole runs)
-        and cells per lane (1, or 2 for LLE maps that can)."""
-        p = BatchPlan(block, waves_per_simd, float(slice_steps), cells_per_lane, 0.0)
+    def set_plan(self, block=256, waves_per_simd=4, slice_steps=0.0, cells_per_lane=1, vector_lanes=0):
+        """Launch shape, trajectory-steps per cell per launch (0: whole runs),
+        cells per lane (1, or 2 for LLE maps that can) and vector_lanes (1:
+        spectrum maps run one lane per tangent vector)."""
+        p = BatchPlan(block, waves_per_simd, float(slice_steps), cells_per_lane, vector_lanes, 0.0)
         self.engine.check(lib().uc_batch_set_plan(self.h, C.byref(p)), "uc_batch_set_plan")
 
     def plan(self):
         p = BatchPlan()
         self.engine.check(lib().uc_batch_get_plan(self.h, C.byref(p)), "uc_batch_get_plan")
         return dict(block=p.block, waves_per_simd=p.waves_per_simd, slice_steps=p.slice_steps,
-                    cells_per_lane=p.cells_per_lane, ms=p.ms)
+                    cells_per_lane=p.cells_per_lane, vector_lanes=p.vector_lanes, ms=p.ms)
 
     def tune(self, stream=None):
-        """Times the candidate plans and keeps the fastest (all four knobs)."""
+        """Times the candidate plans and keeps the fastest (all five knobs)."""
         p = BatchPlan()
         self.engine.check(lib().uc_batch_tune_plan(self.h, _stream(stream), C.byref(p)), "uc_batch_tune_plan")
         return dict(block=p.block, waves_per_simd=p.waves_per_simd, slice_steps=p.slice_steps,
-                    cells_per_lane=p.cells_per_lane, ms=p.ms)
+                    cells_per_lane=p.cells_per_lane, vector_lanes=p.vector_lanes, ms=p.ms)
 
     def launch(self, stream=None, sync=True):
         import torch
