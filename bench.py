@@ -616,6 +616,21 @@ def extra_basins(eng, n=2048):
     # FP32C integration runs at max(requested, 2^-20)
     out["f32c_tolerance"] = dict(requested_rtol=1e-9, requested_atol=1e-9, rtol=max(1e-9, 2.0 ** -20),
                                  atol=max(1e-9, 2.0 ** -20), rule="max(requested, 2^-20)")
+    # FP64 at the tolerance FP32C actually runs, so that the two precisions
+    # are also compared at equal accuracy (the 1e-9 FP64 map takes ~4x the
+    # steps of either)
+    tol = max(1e-9, 2.0 ** -20)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = ucuda.run_basins(s, axes, [0.0, 0.0], s.values(), rtol=tol, atol=tol, n_sections=200,
+                         precision=ucuda.PREC_F64, want_points=False)
+    dt = time.perf_counter() - t0
+    st = r["steps"].cpu().numpy()
+    lab_eq = r["labels"].cpu().numpy()
+    out["f64_at_f32c_tolerance"] = dict(s=dt, rtol=tol, atol=tol, accepted_steps=int(st[0].sum()),
+                                        rejected_steps=int(st[1].sum()), traj_steps_per_s=float(st[0].sum()) / dt,
+                                        attractors=r["attractors"])
+    out["f32c_over_f64_at_equal_tolerance"] = dt / out["f32c"]["s"]
     df = dp5_flops(s.op_counts(), s.n_vars)
     out["roofline"] = {
         name: valu_roofline("uc_bas_" + name, (out[name]["accepted_steps"] + out[name]["rejected_steps"]) * df,
@@ -635,6 +650,20 @@ def extra_basins(eng, n=2048):
                 m[a["label"]] = b["label"]
     l32m = np.array([m.get(int(v), 0 if v == 0 else -99) for v in l32])
     out["f32c_label_agreement"] = float((l32m == l64).mean())
+    # the FP64 map at 2^-20 against the 1e-9 one (same catalogue numbering
+    # when the attractors match by period and point)
+    meq = {}
+    for a in out["f64_at_f32c_tolerance"]["attractors"]:
+        for b in a64:
+            if a["period"] == b["period"] and (a["period"] == 0 or (abs(a["point"][0] - b["point"][0]) <= 3e-3 and
+                                                                    abs(a["point"][1] - b["point"][1]) <= 3e-3)):
+                meq[a["label"]] = b["label"]
+    lut = np.full(int(lab_eq.max()) + 1, -99, dtype=np.int64)
+    lut[0] = 0
+    for k, v in meq.items():
+        lut[k] = v
+    leqm = lut[lab_eq]
+    out["f64_at_f32c_tolerance"]["label_agreement_with_f64_1e9"] = float((leqm == l64).mean())
     # full-size parity and CPU baseline on every 32nd point per axis
     # (64 x 64 = 4096 cells) of the same 2048^2 grid; boundary cells are
     # those whose FP64 label map changes within 2 grid spacings
