@@ -677,6 +677,30 @@ __device__ __forceinline__ int uc_block_tan_u(S& s, typename Prec<S>::T* V, cons
 // peak) or d[j] - d[j+1] > 1e-14 (peak at j, parabolic refinement); either
 // way the next candidate is j + 2; a walk reaching n - 2 ends the search.
 // The first peak only anchors the first interval (the reference's shift-left).
+//
+// Parabolic refinement of a peak at d0 between dm and dp: the offset of the
+// vertex in samples (0 when the parabola is flat).
+__device__ __forceinline__ double uc_peak_delta(double dm, double d0, double dp) {
+    const double denom = dm - 2.0 * d0 + dp;
+    double delta = 0.0;
+#if UC_STRICT
+    if (fabs(denom) > 1e-12) delta = 0.5 * (dm - dp) / denom;
+#else
+    // A wave runs this whenever any of its lanes confirms a peak (most
+    // steps): the reciprocal and one Newton step replace the division
+    // (the quotient moves by an ulp or two).
+    if (fabs(denom) > 1e-12) {
+        double r = __builtin_amdgcn_rcp(denom);
+        r = fma(r, fma(-denom, r, 1.0), r);
+        delta = 0.5 * (dm - dp) * r;
+    }
+#endif
+    return delta;
+}
+__device__ __forceinline__ double uc_peak_value(double dm, double d0, double dp, double delta) {
+    return d0 - 0.25 * (dm - dp) * delta;
+}
+
 struct PeakObserver {
     static constexpr bool records = true;
     double w0, w1;          // d[k-2], d[k-1]
@@ -694,21 +718,8 @@ struct PeakObserver {
     int traj_len;
 
     __device__ __forceinline__ void emit(double dm, double d0, double dp, long long j) {
-        const double denom = dm - 2.0 * d0 + dp;
-        double delta = 0.0;
-#if UC_STRICT
-        if (fabs(denom) > 1e-12) delta = 0.5 * (dm - dp) / denom;
-#else
-        // A wave runs this whenever any of its lanes confirms a peak (most
-        // steps): the reciprocal and one Newton step replace the division
-        // (the quotient moves by an ulp or two).
-        if (fabs(denom) > 1e-12) {
-            double r = __builtin_amdgcn_rcp(denom);
-            r = fma(r, fma(-denom, r, 1.0), r);
-            delta = 0.5 * (dm - dp) * r;
-        }
-#endif
-        const double value = d0 - 0.25 * (dm - dp) * delta;
+        const double delta = uc_peak_delta(dm, d0, dp);
+        const double value = uc_peak_value(dm, d0, dp, delta);
         const double tpk = (double)(j - 1) + delta;
         if (raw >= 1 && raw - 1 < max_peaks) {
             peaks[(long long)(raw - 1) * stride] = value;
@@ -745,6 +756,102 @@ struct PeakObserver {
     }
 };
 
+// uc_loop_u with a PeakObserver for the common record: one step per sample,
+// a state component as the sample, no stored trajectory, fewer than 2^30
+// samples, and an observer that is either fresh (search mode, c_next = 2)
+// or resumed at i0 >= 2. Same iterations, magnitude checks, dead-lane rule,
+// peaks, intervals and observer state as that loop; leaner per sample:
+// 32-bit sample counters (scalar compares), the search as one bitwise
+// expression (the branch around a confirmed peak is the only one), and the
+// peak slots addressed by a running offset. uc_loop_u's `k >= 2` guard is
+// implied: before sample 2 a fresh observer has c = k - 1 < c_next and is
+// not in a plateau. A dead lane goes on updating the search but stores
+// nothing; its observer is never read (its flag is 0 and it is not saved).
+template <class S>
+__device__ int uc_record_u1(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t, int i0,
+                            int iters, bool trail, int wvar, double max_value, PeakObserver& o) {
+    typedef typename Prec<S>::T T;
+    const double hd = (double)h;
+    double w0 = o.w0, w1 = o.w1, last_t = o.last_t;
+    int cn = (int)o.c_next, mode = o.mode, raw = o.raw;
+    const int n2 = (int)o.n - 2;
+    const long long stride = o.stride;
+    long long off = (long long)(raw - 1) * stride;  // slot of peak number raw - 1 (raw 0: the anchor)
+    const unsigned maxp = (unsigned)o.max_peaks;
+    double* const pk = o.peaks;
+    double* const iv = o.intervals;
+    const double ts = o.time_step;
+    bool dead = false;
+    S snap = s;
+    double tsnap = t;
+    // PeakObserver::sample's transitions, restated: `fall` (w1 - v > eps)
+    // implies !(w1 < v), so a plateau in the window ends on `lt` (drop) or
+    // `fall` (confirm), and otherwise goes on (mode 1); a plateau reaching
+    // the record's end finishes the search (mode 2).
+    auto iter = [&](int k) {
+        const double v = s.get_sel(wvar);
+        const int c = k - 1;
+        const bool inwin = c < n2;
+        const bool lt = w1 < v;
+        const bool fall = w1 - v > UC_EPS_PEAK_DELTA;
+        const bool start = (mode == 0) & (c >= cn) & inwin & (w1 - w0 > UC_EPS_PEAK_DELTA) &
+                           (w1 > UC_PEAK_THRESHOLD) & (w1 >= v);
+        const bool plateau = (mode == 1) | start;
+        const bool ends = plateau & inwin & (lt | fall);
+        if (plateau & inwin & fall & !dead) {
+            const double delta = uc_peak_delta(w0, w1, v);
+            const double value = uc_peak_value(w0, w1, v, delta);
+            const double tpk = (double)(c - 1) + delta;
+            if ((unsigned)(raw - 1) < maxp) {
+                pk[off] = value;
+                if (iv) iv[off] = (tpk - last_t) * ts;
+            }
+            last_t = tpk;
+            ++raw;
+            off += stride;
+        }
+        const int in_window = (lt | fall) ? 0 : 1;
+        const int next = inwin ? in_window : 2;
+        mode = plateau ? next : mode;
+        cn = ends ? c + 2 : cn;
+        w0 = w1;
+        w1 = v;
+        uc_rk4(s, a, h, (T)t);
+        if (UC_USES_TIME) t += hd;
+    };
+    for (int i = i0; i < iters;) {
+        iter(i);
+        if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
+        if (uc_wave_all(dead)) break;
+        const int stop = (iters - i > UC_CHECK_INTERVAL) ? i + UC_CHECK_INTERVAL : iters;
+        for (++i; i + 1 < stop; i += 2) {  // two samples per trip: the window rotates without copies
+            iter(i);
+            iter(i + 1);
+        }
+        if (i < stop) iter(i++);
+    }
+    o.w0 = w0; o.w1 = w1; o.last_t = last_t; o.c_next = cn; o.mode = mode; o.raw = raw;
+    if (dead) { s = snap; t = tsnap; return 0; }
+    if (!trail) return 1;
+    double prev[UC_N];
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
+    uc_rk4(s, a, h, (T)t);
+    if (UC_USES_TIME) t += hd;
+    return uc_fixed_point_flag(s, prev, hd, 1);
+}
+
+// The recorded iterations [i0, i1) of a wave-uniform record: uc_record_u1
+// where it applies, uc_loop_u otherwise.
+template <class S>
+__device__ __forceinline__ int uc_record_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t,
+                                           long long i0, long long i1, bool trail, const UcJob& j, PeakObserver& o) {
+    if (j.pre_scaller == 1 && j.writable_var >= 0 && o.traj_len == 0 && o.n < (1LL << 30) &&
+        (i0 >= 2 || (o.mode == 0 && o.c_next >= 2)))
+        return uc_record_u1(s, a, h, t, (int)i0, (int)i1, trail, j.writable_var, j.max_value, o);
+    return uc_loop_u(s, a, h, t, i0, i1, trail, j.pre_scaller, j.writable_var, j.max_value, o);
+}
+
 // ================================================================ kernels ==
 // The recorded run of a bifurcation cell (after a transient that ended with
 // `flag`): streaming peaks, the cell's flag, the optional final state.
@@ -779,11 +886,11 @@ __device__ __forceinline__ void uc_bif_record(const UcJob& j, long long lane, S&
     if (sliced && u0 >= TU) cell_state(false);  // a record in progress
     if (flag == 1 || flag == -1) {
         if (!sliced) {
-            flag = uni ? uc_loop_u(s, a, h, t, j.n_iters, j.pre_scaller, j.writable_var, j.max_value, obs)
+            flag = uni ? uc_record_u(s, a, h, t, 0, j.n_iters, true, j, obs)
                        : uc_loop(s, a, h, t, iters, j.pre_scaller, j.writable_var, j.max_value, obs);
         } else {
             const UcRange r = uc_range(j.n_iters, TU, u0, u1);
-            const int f = uc_loop_u(s, a, h, t, r.i0, r.i1, r.last, j.pre_scaller, j.writable_var, j.max_value, obs);
+            const int f = uc_record_u(s, a, h, t, r.i0, r.i1, r.last, j, obs);
             if (f != 0 && !r.last) { cell_state(true); return; }
             flag = f;
         }
