@@ -570,7 +570,7 @@ def _dbscan_cases(rng):
     and the 1024-peak wave / lane boundary, uniform values, levels on a grid
     of eps (distances at the eps boundary in floating point), tight chains,
     repeated values, period tiles, and the -1 / 0 / 1 counts."""
-    sizes = [2, 3, 5, 63, 64, 65, 66, 127, 128, 129, 500, 1023, 1024, 1025, 1026, 1300]
+    sizes = [2, 3, 5, 63, 64, 65, 66, 127, 128, 129, 200, 256, 300, 500, 700, 960, 1023, 1024, 1025, 1026, 1300]
     cases = []
     for n in sizes:
         cases.append(rng.uniform(0.0, 3.0, n))
@@ -581,12 +581,20 @@ def _dbscan_cases(rng):
     return cases
 
 
-def test_dbscan_periods_stored_peaks(gpu_engine):
-    print("case: DBSCAN periods of stored peak sets (wave kernel n <= 1024, lane kernel above) vs oracle", flush=True)
+@pytest.mark.parametrize("eps", [0.1, 1e-160, 1e160])
+def test_dbscan_periods_stored_peaks(gpu_engine, eps):
+    """eps 0.1: the wave kernel's |dp| < eps test; 1e-160 and 1e160 lie
+    outside [2^-500, 2^500], where it keeps sqrt(dp^2) < eps (the peak sets
+    then also hold values near 1e200, whose squares overflow)."""
+    print("case: DBSCAN periods of stored peak sets (wave kernel n <= 1024, lane kernel above) vs oracle, eps %g"
+          % eps, flush=True)
     import torch
     s = ucuda.preset("lorenz", gpu_engine)
     rng = np.random.default_rng(20261203)
     cases = _dbscan_cases(rng)
+    if eps != 0.1:
+        cases += [np.concatenate([c[:len(c) // 2], 1e200 * c[len(c) // 2:]]) for c in cases[::5]]
+        cases += [c * 1e-162 for c in cases[1::5]]
     max_peaks = max(len(c) for c in cases)
     n_cells = len(cases) + 3
     peaks = np.full((max_peaks, n_cells), np.nan)
@@ -596,14 +604,14 @@ def test_dbscan_periods_stored_peaks(gpu_engine):
         counts[c] = len(v)
     counts[-3:] = [-1, 0, 1]
     peaks[0, -1] = 4.2
-    expect = np.array([oracle.dbscan_count(peaks[:counts[c], c], 0.1) if counts[c] > 1 else counts[c]
+    expect = np.array([oracle.dbscan_count(peaks[:counts[c], c], eps) if counts[c] > 1 else counts[c]
                        for c in range(n_cells)], dtype=np.int32)
-    assert len(set(expect.tolist())) > 20
+    assert len(set(expect.tolist())) > (20 if eps == 0.1 else 3)
     for mp in (max_peaks, 1024):  # all cells; then only the wave kernel (longer counts read as 1024)
         g = ucuda.dbscan_periods(s, torch.from_numpy(np.ascontiguousarray(peaks[:mp])).cuda(),
-                                 torch.from_numpy(counts).cuda(), 0.1).cpu().numpy()
+                                 torch.from_numpy(counts).cuda(), eps).cpu().numpy()
         exp = expect if mp == max_peaks else np.array(
-            [oracle.dbscan_count(peaks[:min(counts[c], mp), c], 0.1) if counts[c] > 1 else counts[c]
+            [oracle.dbscan_count(peaks[:min(counts[c], mp), c], eps) if counts[c] > 1 else counts[c]
              for c in range(n_cells)], dtype=np.int32)
         np.testing.assert_array_equal(g, exp)
 

@@ -1815,12 +1815,56 @@ __device__ __forceinline__ double uc_readlane_f64(double v, int l) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+//
+// The neighbour test is the reference's sqrt(dp^2 + di^2) < eps with di = 0.
+// In binary floating point with rounding to nearest, sqrt(fl(x * x)) = |x|
+// whenever x * x neither underflows nor overflows (S. Boldo, "Stupid is as
+// stupid does: taking the square root of the square of a floating-point
+// number", 2015); for 2^-500 <= eps <= 2^500 the cases where it does
+// (|dp| < 2^-511: both sides below eps; |dp| >= 2^511.99...: both sides not)
+// give the same verdict too, so there the test is |dp| < eps, bit for bit.
+//
+// Workgroups are dealt round-robin over the chip's XCDs, each with its own
+// L2; neighbouring cells share cache lines of the peak-major array, so the
+// blocks are renumbered to give each XCD one contiguous range of cells.
+#define UC_XCDS 8
+__device__ __forceinline__ unsigned uc_xcd_block() {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned per = nb / UC_XCDS, rem = nb % UC_XCDS, x = b % UC_XCDS;
+    return x * per + min(x, rem) + b / UC_XCDS;
+}
+
+template <bool ABS>
+__device__ __forceinline__ bool uc_db_near(double dp, double eps) {
+    if (ABS) return fabs(dp) < eps;
+    const double di = 0.0;
+    return sqrt(dp * dp + di * di) < eps;
+}
+
+// JM: the number of 64-peak chunks the cell's loops are unrolled over (a
+// power of two >= the cell's chunk count); the chunks past the cell's
+// peaks hold no candidates, so they never join.
+template <bool ABS, int JM>
+__device__ void uc_dbscan_wave_cell(const double* __restrict__ peaks, long long stride, long long c, int n, double eps,
+                                    int* out);
+
+template <bool ABS>
+__device__ __forceinline__ void uc_dbscan_wave_jm(const double* __restrict__ peaks, long long stride, long long c,
+                                                  int n, double eps, int* out) {
+    const int J = (n + 63) >> 6;
+    if (J <= 1) uc_dbscan_wave_cell<ABS, 1>(peaks, stride, c, n, eps, out);
+    else if (J <= 2) uc_dbscan_wave_cell<ABS, 2>(peaks, stride, c, n, eps, out);
+    else if (J <= 4) uc_dbscan_wave_cell<ABS, 4>(peaks, stride, c, n, eps, out);
+    else if (J <= 8) uc_dbscan_wave_cell<ABS, 8>(peaks, stride, c, n, eps, out);
+    else uc_dbscan_wave_cell<ABS, 16>(peaks, stride, c, n, eps, out);
+}
+
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan_wave(const double* __restrict__ peaks,
                                                                             const int* __restrict__ counts,
                                                                             long long stride, long long n_cells,
                                                                             int max_n, double eps, int* out) {
     const int lane = threadIdx.x & 63;
-    const long long c = (long long)blockIdx.x * (UC_BLOCK / 64) + (threadIdx.x >> 6);
+    const long long c = (long long)uc_xcd_block() * (UC_BLOCK / 64) + (threadIdx.x >> 6);
     if (c >= n_cells) return;  // wave-uniform
     const int n = __builtin_amdgcn_readfirstlane(min(counts[c], max_n));
     if (n <= 1) {
@@ -1828,13 +1872,21 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan_wave(const 
         return;
     }
     if (n > UC_DB_WAVE_MAX) return;
+    if (eps >= 0x1p-500 && eps <= 0x1p500) uc_dbscan_wave_jm<true>(peaks, stride, c, n, eps, out);
+    else uc_dbscan_wave_jm<false>(peaks, stride, c, n, eps, out);
+}
+
+template <bool ABS, int JM>
+__device__ void uc_dbscan_wave_cell(const double* __restrict__ peaks, long long stride, long long c, int n, double eps,
+                                    int* out) {
+    const int lane = threadIdx.x & 63;
     const int J = (n + 63) >> 6;
-    double p[16];
+    double p[JM];
     unsigned cand = 0;  // bit j: peak 64 j + lane exists and is < n - 1 (scanned as a neighbour)
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int j = 0; j < JM; ++j) {
         const int k = 64 * j + lane;
-        p[j] = (j < J && k < n) ? peaks[(long long)k * stride + c] : 0.0;
+        p[j] = k < n ? peaks[(long long)k * stride + c] : 0.0;
         if (k < n - 1) cand |= 1u << j;
     }
     unsigned lab = 0, slot = 0;
@@ -1863,18 +1915,15 @@ extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan_wave(const 
         }
         double pi = 0.0;
 #pragma unroll
-        for (int j = 0; j < 16; ++j)
+        for (int j = 0; j < JM; ++j)
             if (j == (i >> 6)) pi = uc_readlane_f64(p[j], i & 63);
         const unsigned open = cand & ~lab;
         unsigned joined = 0;
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            if (j < J) {
-                const double dp = pi - p[j], di = 0.0;
-                const bool hit = ((open >> j) & 1u) && (64 * j + lane) != i && sqrt(dp * dp + di * di) < eps;
-                nb += __builtin_popcountll(__builtin_amdgcn_ballot_w64(hit));
-                if (hit) joined |= 1u << j;
-            }
+        for (int j = 0; j < JM; ++j) {
+            const bool hit = ((open >> j) & 1u) && (64 * j + lane) != i && uc_db_near<ABS>(pi - p[j], eps);
+            nb += __builtin_popcountll(__builtin_amdgcn_ballot_w64(hit));
+            if (hit) joined |= 1u << j;
         }
         lab |= joined;
         slot |= joined;
