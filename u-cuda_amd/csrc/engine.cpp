@@ -1132,23 +1132,29 @@ int uc_run_basins(uc_engine* e, uc_system* s, const uc_basins_request* r, const 
 }
 
 // Peak clustering launches shared by uc_run_bif_periods and
-// uc_bif_dbscan_periods: the wave-per-cell kernel for cells with at most
-// kDbWaveMax peaks, then the lane-per-cell kernel (HBM scratch labels /
-// slots of max_n rows) for the others; the latter is skipped when no cell
-// can have more.
-static const int kDbWaveMax = 1024;  // UC_DB_WAVE_MAX in the kernel source
+// uc_bif_dbscan_periods: the wave-per-cell kernels for cells with at most
+// kDbNarrowMax peaks (and the 0 / 1 / -1 counts) and with at most
+// kDbWaveMax, then the lane-per-cell kernel (HBM scratch labels / slots of
+// max_n rows) for the others; a kernel is skipped when no cell can reach
+// its range.
+static const int kDbNarrowMax = 512;  // UC_DB_NARROW_MAX in the kernel source
+static const int kDbWaveMax = 1024;   // UC_DB_WAVE_MAX in the kernel source
 
 static int launch_dbscan(uc_engine* e, Module* m, const double* peaks, const int* counts, long long stride,
                          long long n_cells, int max_n, double eps, int* labels, int* slots, int* out, hipStream_t st) {
-    hipFunction_t fw, fl;
+    hipFunction_t fw, fww, fl;
     hipError_t he = hipModuleGetFunction(&fw, m->mod, "uc_bif_dbscan_wave");
+    if (he == hipSuccess) he = hipModuleGetFunction(&fww, m->mod, "uc_bif_dbscan_wave_wide");
     if (he == hipSuccess) he = hipModuleGetFunction(&fl, m->mod, "uc_bif_dbscan");
     if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan"));
     void* wargs[] = {(void*)&peaks, (void*)&counts, &stride, &n_cells, &max_n, &eps, &out};
     const long long waves_per_block = kBlock / 64;
-    he = hipModuleLaunchKernel(fw, (unsigned)((n_cells + waves_per_block - 1) / waves_per_block), 1, 1, kBlock, 1, 1,
-                               0, st, wargs, nullptr);
+    const unsigned wave_blocks = (unsigned)((n_cells + waves_per_block - 1) / waves_per_block);
+    he = hipModuleLaunchKernel(fw, wave_blocks, 1, 1, kBlock, 1, 1, 0, st, wargs, nullptr);
     if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan_wave launch"));
+    if (max_n <= kDbNarrowMax) return UC_OK;
+    he = hipModuleLaunchKernel(fww, wave_blocks, 1, 1, kBlock, 1, 1, 0, st, wargs, nullptr);
+    if (he != hipSuccess) return e->fail(UC_ERR_HIP, hip_msg(he, "uc_bif_dbscan_wave_wide launch"));
     if (max_n <= kDbWaveMax) return UC_OK;
     int min_n = kDbWaveMax;
     void* largs[] = {(void*)&peaks, (void*)&counts, &stride, &n_cells, &max_n, &eps, &min_n, &labels, &slots, &out};

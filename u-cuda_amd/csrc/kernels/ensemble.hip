@@ -1848,32 +1848,50 @@ template <bool ABS, int JM>
 __device__ void uc_dbscan_wave_cell(const double* __restrict__ peaks, long long stride, long long c, int n, double eps,
                                     int* out);
 
-template <bool ABS>
+template <bool ABS, bool WIDE>
 __device__ __forceinline__ void uc_dbscan_wave_jm(const double* __restrict__ peaks, long long stride, long long c,
                                                   int n, double eps, int* out) {
     const int J = (n + 63) >> 6;
-    if (J <= 1) uc_dbscan_wave_cell<ABS, 1>(peaks, stride, c, n, eps, out);
+    if (WIDE) uc_dbscan_wave_cell<ABS, 16>(peaks, stride, c, n, eps, out);
+    else if (J <= 1) uc_dbscan_wave_cell<ABS, 1>(peaks, stride, c, n, eps, out);
     else if (J <= 2) uc_dbscan_wave_cell<ABS, 2>(peaks, stride, c, n, eps, out);
     else if (J <= 4) uc_dbscan_wave_cell<ABS, 4>(peaks, stride, c, n, eps, out);
-    else if (J <= 8) uc_dbscan_wave_cell<ABS, 8>(peaks, stride, c, n, eps, out);
-    else uc_dbscan_wave_cell<ABS, 16>(peaks, stride, c, n, eps, out);
+    else uc_dbscan_wave_cell<ABS, 8>(peaks, stride, c, n, eps, out);
+}
+
+// Cells of up to UC_DB_NARROW_MAX peaks (and the 0 / 1 / -1 counts) in
+// uc_bif_dbscan_wave, the rest up to UC_DB_WAVE_MAX in uc_bif_dbscan_wave_wide:
+// the 16-chunk form needs twice the registers, which would halve the
+// waves per SIMD of the common one.
+#define UC_DB_NARROW_MAX 512
+template <bool WIDE>
+__device__ __forceinline__ void uc_dbscan_wave_run(const double* __restrict__ peaks, const int* __restrict__ counts,
+                                                   long long stride, long long n_cells, int max_n, double eps,
+                                                   int* out) {
+    const int lane = threadIdx.x & 63;
+    const long long c = (long long)uc_xcd_block() * (UC_BLOCK / 64) + (threadIdx.x >> 6);
+    if (c >= n_cells) return;  // wave-uniform
+    const int n = __builtin_amdgcn_readfirstlane(min(counts[c], max_n));
+    if (n <= 1) {
+        if (!WIDE && lane == 0) out[c] = n;
+        return;
+    }
+    if (WIDE ? (n <= UC_DB_NARROW_MAX || n > UC_DB_WAVE_MAX) : n > UC_DB_NARROW_MAX) return;
+    if (eps >= 0x1p-500 && eps <= 0x1p500) uc_dbscan_wave_jm<true, WIDE>(peaks, stride, c, n, eps, out);
+    else uc_dbscan_wave_jm<false, WIDE>(peaks, stride, c, n, eps, out);
 }
 
 extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan_wave(const double* __restrict__ peaks,
                                                                             const int* __restrict__ counts,
                                                                             long long stride, long long n_cells,
                                                                             int max_n, double eps, int* out) {
-    const int lane = threadIdx.x & 63;
-    const long long c = (long long)uc_xcd_block() * (UC_BLOCK / 64) + (threadIdx.x >> 6);
-    if (c >= n_cells) return;  // wave-uniform
-    const int n = __builtin_amdgcn_readfirstlane(min(counts[c], max_n));
-    if (n <= 1) {
-        if (lane == 0) out[c] = n;
-        return;
-    }
-    if (n > UC_DB_WAVE_MAX) return;
-    if (eps >= 0x1p-500 && eps <= 0x1p500) uc_dbscan_wave_jm<true>(peaks, stride, c, n, eps, out);
-    else uc_dbscan_wave_jm<false>(peaks, stride, c, n, eps, out);
+    uc_dbscan_wave_run<false>(peaks, counts, stride, n_cells, max_n, eps, out);
+}
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) uc_bif_dbscan_wave_wide(const double* __restrict__ peaks,
+                                                                                 const int* __restrict__ counts,
+                                                                                 long long stride, long long n_cells,
+                                                                                 int max_n, double eps, int* out) {
+    uc_dbscan_wave_run<true>(peaks, counts, stride, n_cells, max_n, eps, out);
 }
 
 template <bool ABS, int JM>
