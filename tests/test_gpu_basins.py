@@ -118,3 +118,47 @@ def test_basins_unforced_two_wells(gpu_engine):
     inner = _interior(lab).ravel() & _interior(flipped).ravel()
     l1, l2 = lab.ravel()[inner], flipped.ravel()[inner]
     assert ((l1 != l2) | (l1 == 0)).all()
+
+
+def _map_labels(g_from, g_to, tol=3e-3):
+    m = {}
+    for a in g_from["attractors"]:
+        for b in g_to["attractors"]:
+            if a["period"] == b["period"] and (a["period"] == 0 or (abs(a["point"][0] - b["point"][0]) <= tol and
+                                                                   abs(a["point"][1] - b["point"][1]) <= tol)):
+                m[a["label"]] = b["label"]
+    return np.array([m.get(v, 0 if v == 0 else -99) for v in g_from["labels"].cpu().numpy()], dtype=np.int32)
+
+
+@pytest.mark.parametrize("rtol,h_max", [(1e-9, None), (1e-4, 2 * math.pi)])
+def test_basins_harmonic_forcing_fast_vs_strict(gpu_engine, rtol, h_max):
+    """Two harmonics with a phase, g cos(w t) + 0.05 sin(2 w t + f) (d = 0.3,
+    g = 0.2, w = 1, f = 0.7; ICs in [-1.5, 1.5]^2): the fast
+    modules rotate their values to the stage times, strict evaluates cos /
+    sin like the oracle. At 1e-4 with steps up to a whole period the
+    rotation angles exceed 0.8 and take the reduction branch."""
+    print("case: two-harmonic forced Duffing 48x48, fast rotation vs strict calls, rtol %g" % rtol, flush=True)
+    n = 48
+    s = ucuda.System(gpu_engine, ["x", "v"], ["d", "g", "w", "f"],
+                     ["v", "-d*v + x - x^3 + g*cos(w*t) + 0.05*sin(2*w*t + f)"], time_symbol="t")
+    assert "#define UC_HARMONICS 2" in s.source(0)
+    axes = [ucuda.axis(ucuda.AXIS_VAR, 0, n, -1.5, 1.5), ucuda.axis(ucuda.AXIS_VAR, 1, n, -1.5, 1.5)]
+    vals = [0.5, 0.3, 0.2, 1.0, 0.7]  # two period-1 attractors, one per well
+    kw = dict(rtol=rtol, atol=rtol, n_sections=120, h_max=h_max)
+    gs = ucuda.run_basins(s, axes, [0.0, 0.0], vals, strict=True, **kw)
+    ls = gs["labels"].cpu().numpy()
+    inner = _interior(ls.reshape(n, n)).ravel()
+    assert inner.sum() > n * n // 4 and len(gs["attractors"]) >= 2
+    for prec in (ucuda.PREC_F64, ucuda.PREC_F32C):
+        g = ucuda.run_basins(s, axes, [0.0, 0.0], vals, precision=prec, **kw)
+        lf = _map_labels(g, gs)
+        agree = (lf[inner] == ls[inner]).mean()
+        print("precision %d: interior agreement %.4f, attractors %d / %d" % (prec, agree, len(g["attractors"]),
+                                                                           len(gs["attractors"])))
+        if prec == ucuda.PREC_F64 and rtol == 1e-9:
+            np.testing.assert_array_equal(lf[inner], ls[inner])
+        else:
+            assert agree >= 0.99
+        if prec == ucuda.PREC_F64:  # same tolerance, same controller: the same accepted-step counts
+            st, ss = g["steps"].cpu().numpy(), gs["steps"].cpu().numpy()
+            assert abs(st[0].mean() / ss[0].mean() - 1.0) < 0.05

@@ -29,7 +29,9 @@ namespace ucuda {
 namespace {
 
 // ------------------------------------------------------------------ trees --
-enum class K { Num, Var, Par, Time, Const, Call, Add, Sub, Mul, Div, Pow, Neg };
+// Raw: text printed as is (the harmonic inputs hc[k] / hs[k] of the
+// device source), never parsed, differentiated or evaluated.
+enum class K { Num, Var, Par, Time, Const, Call, Add, Sub, Mul, Div, Pow, Neg, Raw };
 
 struct Node;
 using P = std::shared_ptr<const Node>;
@@ -295,7 +297,7 @@ P resolve(const P& n, const System& s) {
         if (n->name == "pi") return n;
         throw std::runtime_error("unknown symbol: " + n->name);
     }
-    case K::Num: case K::Var: case K::Par: case K::Time: return n;
+    case K::Num: case K::Var: case K::Par: case K::Time: case K::Raw: return n;
     case K::Neg: return neg(resolve(n->a, s));
     case K::Call: {
         std::vector<P> a;
@@ -366,6 +368,7 @@ void print(const P& n, const Style& st, std::ostream& o) {
     case K::Par: o << st.params << "[" << n->idx << "]"; break;
     case K::Time: o << st.time; break;
     case K::Const: o << (st.typed_literals ? "((T)3.14159265358979323846)" : n->name); break;
+    case K::Raw: o << n->name; break;
     case K::Neg: o << "-"; print_child(n->a, st, o, precedence(K::Neg)); break;
     case K::Add: print_child(n->a, st, o, 1); o << " + "; print_child(n->b, st, o, 1); break;
     case K::Sub: print_child(n->a, st, o, 1); o << " - "; print_child(n->b, st, o, 2); break;
@@ -443,7 +446,7 @@ P f1(const char* f, P x) { return call(f, {std::move(x)}); }
 // d n / d X[j]
 P diff(const P& n, int j) {
     switch (n->k) {
-    case K::Num: case K::Par: case K::Time: case K::Const: return num(0.0);
+    case K::Num: case K::Par: case K::Time: case K::Const: case K::Raw: return num(0.0);
     case K::Var: return num(n->idx == j ? 1.0 : 0.0);
     case K::Neg: return s_neg(diff(n->a, j));
     case K::Add: return s_add(diff(n->a, j), diff(n->b, j));
@@ -517,7 +520,7 @@ P diff(const P& n, int j) {
 // ------------------------------------------------------------ op counting --
 void count(const P& n, int& flops, int& calls) {
     switch (n->k) {
-    case K::Num: case K::Var: case K::Par: case K::Time: case K::Const: return;
+    case K::Num: case K::Var: case K::Par: case K::Time: case K::Const: case K::Raw: return;
     case K::Neg: count(n->a, flops, calls); return;  // sign flips are free modifiers
     case K::Call: ++calls; for (auto& a : n->args) count(a, flops, calls); return;
     case K::Pow:
@@ -539,6 +542,7 @@ double eval(const P& n, const double* X, const double* a, double t) {
     case K::Par: return a[n->idx];
     case K::Time: return t;
     case K::Const: return 3.14159265358979323846;
+    case K::Raw: throw std::runtime_error("raw text has no value");
     case K::Neg: return -eval(n->a, X, a, t);
     case K::Add: return eval(n->a, X, a, t) + eval(n->b, X, a, t);
     case K::Sub: return eval(n->a, X, a, t) - eval(n->b, X, a, t);
@@ -897,6 +901,104 @@ std::string krs_cd(const System& s) {
     return o.str();
 }
 
+// ------------------------------------------------------ harmonic forcing --
+// A cos / sin whose argument is linear in time, w * t + phi with w and phi
+// free of the state and of t (periodic forcing), is an input of the fast
+// modules' right-hand side: the kernels rotate its value from one stage
+// time to the next instead of evaluating the function (the device source's
+// uc_rhs_h, hc[k] / hs[k] = cos / sin(w[k] * t + phi[k])).
+bool has_state_or_time(const P& n) {
+    if (n->k == K::Var || n->k == K::Time) return true;
+    if (n->a && has_state_or_time(n->a)) return true;
+    if (n->b && has_state_or_time(n->b)) return true;
+    for (auto& x : n->args)
+        if (has_state_or_time(x)) return true;
+    return false;
+}
+bool has_state(const P& n) {
+    if (n->k == K::Var) return true;
+    if (n->a && has_state(n->a)) return true;
+    if (n->b && has_state(n->b)) return true;
+    for (auto& x : n->args)
+        if (has_state(x)) return true;
+    return false;
+}
+
+// n = w * t + phi (phi null: 0) with w, phi free of the state and of t
+bool linear_in_time(const P& n, P& w, P& phi) {
+    P w0, p0;
+    switch (n->k) {
+    case K::Time: w = num(1.0); phi = nullptr; return true;
+    case K::Neg:
+        if (!linear_in_time(n->a, w0, p0)) return false;
+        w = s_neg(w0);
+        phi = p0 ? s_neg(p0) : nullptr;
+        return true;
+    case K::Mul:
+    case K::Div: {
+        const bool left = has_state_or_time(n->a), right = has_state_or_time(n->b);
+        if (left == right) return false;
+        if (n->k == K::Div && right) return false;  // t in a denominator
+        const P& lin = left ? n->a : n->b;
+        const P& c = left ? n->b : n->a;
+        if (!linear_in_time(lin, w0, p0) || p0) return false;
+        w = n->k == K::Div ? s_div(w0, c) : s_mul(w0, c);
+        phi = nullptr;
+        return true;
+    }
+    case K::Add:
+    case K::Sub: {
+        const bool left = has_state_or_time(n->a), right = has_state_or_time(n->b);
+        if (left == right) return false;
+        if (!linear_in_time(left ? n->a : n->b, w0, p0) || p0) return false;
+        const P& c = left ? n->b : n->a;
+        if (left) { w = w0; phi = n->k == K::Sub ? s_neg(c) : c; }
+        else { w = n->k == K::Sub ? s_neg(w0) : w0; phi = c; }
+        return true;
+    }
+    default: return false;
+    }
+}
+
+struct Harmonics {
+    std::vector<P> w, phi;
+    std::map<std::string, int> index;
+};
+
+// n with every harmonic cos / sin replaced by hc[k] / hs[k]
+P extract_harmonics(const P& n, Harmonics& h, const Style& st) {
+    if (n->k == K::Call && (n->name == "cos" || n->name == "sin") && n->args.size() == 1 && !has_state(n->args[0])) {
+        P w, phi;
+        if (linear_in_time(n->args[0], w, phi)) {
+            const std::string key = to_text(w, st) + "|" + (phi ? to_text(phi, st) : std::string());
+            auto it = h.index.find(key);
+            int k = (int)h.w.size();
+            if (it == h.index.end()) {
+                h.index[key] = k;
+                h.w.push_back(w);
+                h.phi.push_back(phi);
+            } else {
+                k = it->second;
+            }
+            auto r = std::make_shared<Node>();
+            r->k = K::Raw;
+            r->name = std::string(n->name == "cos" ? "hc[" : "hs[") + std::to_string(k) + "]";
+            return r;
+        }
+    }
+    switch (n->k) {
+    case K::Neg: return neg(extract_harmonics(n->a, h, st));
+    case K::Call: {
+        std::vector<P> a;
+        for (auto& x : n->args) a.push_back(extract_harmonics(x, h, st));
+        return call(n->name, std::move(a));
+    }
+    case K::Add: case K::Sub: case K::Mul: case K::Div: case K::Pow:
+        return bin(n->k, extract_harmonics(n->a, h, st), extract_harmonics(n->b, h, st));
+    default: return n;
+    }
+}
+
 }  // namespace
 
 // ================================================================= public ==
@@ -976,6 +1078,28 @@ std::string emit_system_source(const System& s, Dialect d) {
     for (int i = 0; i < N; ++i) o << "    dX[" << i << "] = " << to_text(trees[i], st) << ";\n";
     o << "}\n";
 
+    if (hip) {
+        // the right-hand side again with its harmonic forcing terms as inputs
+        Harmonics hm;
+        std::vector<P> forced;
+        for (auto& t : trees) forced.push_back(extract_harmonics(t, hm, st));
+        const int nh = (int)hm.w.size();
+        o << "#define UC_HARMONICS " << nh << "\n";
+        if (nh > 0) {
+            o << "template <typename T>\n__device__ __forceinline__ void uc_harm_coef(const T* __restrict__ a, T* w, T* phi) "
+                 "{\n    (void)a;\n";
+            for (int k = 0; k < nh; ++k)
+                o << "    w[" << k << "] = " << to_text(hm.w[k], st) << ";\n    phi[" << k
+                  << "] = " << (hm.phi[k] ? to_text(hm.phi[k], st) : std::string("(T)0")) << ";\n";
+            o << "}\n";
+            o << "template <typename T>\n__device__ __forceinline__ void uc_rhs_h(const T* __restrict__ X, "
+                 "const T* __restrict__ a, const T t, const T* hc, const T* hs, T* __restrict__ dX) {\n";
+            o << "    (void)X; (void)a; (void)t; (void)hc; (void)hs;\n";
+            for (int i = 0; i < N; ++i) o << "    dX[" << i << "] = " << to_text(forced[i], st) << ";\n";
+            o << "}\n";
+        }
+    }
+
     // tangent: Jacobian entries once, then JV for each of the K vectors
     if (hip) o << "template <typename T, int K>\n__device__ __forceinline__ void";
     else o << "static inline void";
@@ -1016,7 +1140,8 @@ std::string emit_system_source(const System& s, Dialect d) {
 
 std::string emit_custom_krs_source(const std::string& body, int n_vars, int n_values) {
     std::ostringstream o;
-    o << "#define UC_N " << n_vars << "\n#define UC_M " << n_values << "\n#define UC_USES_TIME 0\n#define UC_CUSTOM_KRS 1\n";
+    o << "#define UC_N " << n_vars << "\n#define UC_M " << n_values
+      << "\n#define UC_USES_TIME 0\n#define UC_CUSTOM_KRS 1\n#define UC_HARMONICS 0\n";
     // the step body in the reference prelude's environment (U-CUDA/krs_cpu.cpp:293-316):
     // numb, pi and euler of that type (a body may shadow them), min / max
     o << "template <typename numb>\n__device__ __forceinline__ void uc_krs_step(numb* X, const numb* a, const numb h) {\n"

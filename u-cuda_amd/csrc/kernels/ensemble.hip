@@ -2822,35 +2822,39 @@ __device__ __forceinline__ double uc_rcp(double v) { return __builtin_amdgcn_rcp
 // a hardware reciprocal: the norm only decides acceptance (err^2 <= 1 exactly
 // when err <= 1) and sizes the next step, so it needs no correctly rounded
 // division or square root.
-template <typename T>
-__device__ __forceinline__ double uc_dp5(const T* x, const T* a, T t, T h, const T* k1, T* dx, T* k7, double rtol,
-                                         double atol) {
+//
+// `rhs(xs, stage, time, out)` evaluates the right-hand side for stages 2..7
+// (uc_dp5: the generated uc_rhs; the forced variant below supplies the
+// harmonic forcing terms).
+template <typename T, class R>
+__device__ __forceinline__ double uc_dp5_core(const T* x, T t, T h, const T* k1, T* dx, T* k7, double rtol,
+                                              double atol, R&& rhs) {
     T k2[UC_N], k3[UC_N], k4[UC_N], k5[UC_N], k6[UC_N], xs[UC_N];
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) xs[i] = x[i] + h * ((T)DPA21 * k1[i]);
-    uc_rhs<T>(xs, a, t + (T)(1.0 / 5.0) * h, k2);
+    rhs(xs, 2, t + (T)(1.0 / 5.0) * h, k2);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) xs[i] = x[i] + h * ((T)DPA31 * k1[i] + (T)DPA32 * k2[i]);
-    uc_rhs<T>(xs, a, t + (T)(3.0 / 10.0) * h, k3);
+    rhs(xs, 3, t + (T)(3.0 / 10.0) * h, k3);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) xs[i] = x[i] + h * ((T)DPA41 * k1[i] + (T)DPA42 * k2[i] + (T)DPA43 * k3[i]);
-    uc_rhs<T>(xs, a, t + (T)(4.0 / 5.0) * h, k4);
+    rhs(xs, 4, t + (T)(4.0 / 5.0) * h, k4);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i)
         xs[i] = x[i] + h * ((T)DPA51 * k1[i] + (T)DPA52 * k2[i] + (T)DPA53 * k3[i] + (T)DPA54 * k4[i]);
-    uc_rhs<T>(xs, a, t + (T)(8.0 / 9.0) * h, k5);
+    rhs(xs, 5, t + (T)(8.0 / 9.0) * h, k5);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i)
         xs[i] = x[i] + h * ((T)DPA61 * k1[i] + (T)DPA62 * k2[i] + (T)DPA63 * k3[i] + (T)DPA64 * k4[i] +
                             (T)DPA65 * k5[i]);
-    uc_rhs<T>(xs, a, t + h, k6);
+    rhs(xs, 6, t + h, k6);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) {
         dx[i] = h * ((T)DPB1 * k1[i] + (T)0.0 * k2[i] + (T)DPB3 * k3[i] + (T)DPB4 * k4[i] + (T)DPB5 * k5[i] +
                      (T)DPB6 * k6[i]);
         xs[i] = x[i] + dx[i];
     }
-    uc_rhs<T>(xs, a, t + h, k7);
+    rhs(xs, 7, t + h, k7);
     double err = 0.0;
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) {
@@ -2871,6 +2875,157 @@ __device__ __forceinline__ double uc_dp5(const T* x, const T* a, T t, T h, const
     return err * (1.0 / (double)UC_N);
 #endif
 }
+
+template <typename T>
+__device__ __forceinline__ double uc_dp5(const T* x, const T* a, T t, T h, const T* k1, T* dx, T* k7, double rtol,
+                                         double atol) {
+    return uc_dp5_core<T>(x, t, h, k1, dx, k7, rtol, atol,
+                          [&](const T* xs, int, T ts, T* out) { uc_rhs<T>(xs, a, ts, out); });
+}
+
+// Harmonic forcing in the fast modules. The generated source lists the
+// right-hand side's cos / sin terms whose argument is w * t + phi (w, phi
+// from the parameters; uc_harm_coef) and gives uc_rhs_h, the right-hand
+// side with those terms as inputs hc[k] / hs[k]. The basins integration
+// keeps cos / sin of every phase at the current attempt's start time
+// (exact at each section start, local time 0: cos / sin(phi)) and moves
+// them to the stage times t + c h by the rotation (C, S) -> (C cd - S sd,
+// S cd + C sd), cd / sd = cos / sin(w c h): a short Taylor series instead
+// of the library's range reduction and polynomial, and one rotation for
+// the two stages at t + h. An accepted step takes the t + h values on, so
+// within a section the values carry the rounding of up to a few hundred
+// rotations (~1e-14 relative in double), below the rounding the library
+// call sees in its argument w * t itself. Strict modules keep the
+// reference's calls.
+#if !UC_STRICT && defined(UC_HARMONICS) && UC_HARMONICS > 0 && UC_HARMONICS <= 8
+#define UC_HARM UC_HARMONICS
+#else
+#define UC_HARM 0
+#endif
+
+// cos / sin of d: the Taylor series for |d| <= 0.8 (truncation below 1e-18
+// relative in double, 2e-10 in float). Larger d (rare: a step longer than
+// 0.8 / w) is reduced by multiples of 2 pi (two-part constant, exact
+// enough while d's own ulp is far below 2 pi: |d| < 1e15 in double, 1e7 in
+// float; beyond that, and for inf / NaN, the result is NaN), halved until
+// |d| <= 0.8 (at most twice after the reduction) and doubled back, a few
+// ulps and no library range reduction.
+// [plain C++ from here to the matching end mark, compiled on the host too]
+__device__ __forceinline__ void uc_sincos_poly(double d, double& c, double& s) {
+    const double z = d * d;
+    double p = 1.0 / 6402373705728000.0;  // 1/18!
+    p = fma(p, z, -1.0 / 20922789888000.0);
+    p = fma(p, z, 1.0 / 87178291200.0);
+    p = fma(p, z, -1.0 / 479001600.0);
+    p = fma(p, z, 1.0 / 3628800.0);
+    p = fma(p, z, -1.0 / 40320.0);
+    p = fma(p, z, 1.0 / 720.0);
+    p = fma(p, z, -1.0 / 24.0);
+    p = fma(p, z, 0.5);
+    c = fma(-p, z, 1.0);
+    double q = 1.0 / 355687428096000.0;  // 1/17!
+    q = fma(q, z, -1.0 / 1307674368000.0);
+    q = fma(q, z, 1.0 / 6227020800.0);
+    q = fma(q, z, -1.0 / 39916800.0);
+    q = fma(q, z, 1.0 / 362880.0);
+    q = fma(q, z, -1.0 / 5040.0);
+    q = fma(q, z, 1.0 / 120.0);
+    q = fma(q, z, -1.0 / 6.0);
+    s = fma(q * z, d, d);
+}
+__device__ __forceinline__ void uc_sincos_poly(float d, float& c, float& s) {
+    const float z = d * d;
+    float p = -1.0f / 3628800.0f;  // -1/10!
+    p = fmaf(p, z, 1.0f / 40320.0f);
+    p = fmaf(p, z, -1.0f / 720.0f);
+    p = fmaf(p, z, 1.0f / 24.0f);
+    p = fmaf(p, z, -0.5f);
+    c = fmaf(p, z, 1.0f);
+    float q = -1.0f / 39916800.0f;  // -1/11!
+    q = fmaf(q, z, 1.0f / 362880.0f);
+    q = fmaf(q, z, -1.0f / 5040.0f);
+    q = fmaf(q, z, 1.0f / 120.0f);
+    q = fmaf(q, z, -1.0f / 6.0f);
+    s = fmaf(q * z, d, d);
+}
+template <typename T> struct UcTwoPi;
+template <> struct UcTwoPi<double> {
+    static constexpr double hi = 6.283185307179586, lo = 2.4492935982947064e-16, inv = 0.15915494309189535,
+                            cap = 1e15;
+};
+template <> struct UcTwoPi<float> {
+    static constexpr float hi = 6.2831854820251465f, lo = -1.7484555314695172e-07f, inv = 0.15915494f, cap = 1e7f;
+};
+template <typename T> __device__ __forceinline__ void uc_sincos_small(T d, T& c, T& s) {
+    if (fabs(d) <= (T)0.8) {
+        uc_sincos_poly(d, c, s);
+        return;
+    }
+    if (!(fabs(d) < UcTwoPi<T>::cap)) {
+        c = s = (T)__builtin_nan("");
+        return;
+    }
+    const T n = rint(d * UcTwoPi<T>::inv);
+    d = fma(-n, UcTwoPi<T>::hi, d);
+    d = fma(-n, UcTwoPi<T>::lo, d);
+    int halvings = 0;
+    while (fabs(d) > (T)0.8) {
+        d *= (T)0.5;
+        ++halvings;
+    }
+    uc_sincos_poly(d, c, s);
+    for (int i = 0; i < halvings; ++i) {
+        const T s2 = (T)2 * s * c;
+        c = (c - s) * (c + s);
+        s = s2;
+    }
+}
+
+// [end of plain C++]
+
+// cos / sin of each harmonic phase at the current attempt's start time; the
+// frequencies and phases are taken from the parameters where they are
+// needed (no registers held for them).
+template <typename T> struct UcForcing {
+#if UC_HARM
+    T c[UC_HARM], s[UC_HARM];
+    __device__ __forceinline__ void reset(const T* a) {  // local time 0
+        T w[UC_HARM], phi[UC_HARM];
+        uc_harm_coef<T>(a, w, phi);
+#pragma unroll
+        for (int k = 0; k < UC_HARM; ++k) uc_sincos_small(phi[k], c[k], s[k]);
+    }
+#else
+    __device__ __forceinline__ void reset(const T*) {}
+#endif
+};
+
+#if UC_HARM
+// uc_dp5 with the forcing rotated to the stage times; c1 / s1 receive the
+// values at t + h (taken on when the attempt is accepted).
+template <typename T>
+__device__ __forceinline__ double uc_dp5_forced(const T* x, const T* a, T t, T h, const T* k1, T* dx, T* k7,
+                                                double rtol, double atol, const UcForcing<T>& f, T* c1, T* s1) {
+    T hc[UC_HARM], hs[UC_HARM];
+    return uc_dp5_core<T>(x, t, h, k1, dx, k7, rtol, atol, [&](const T* xs, const int st, T ts, T* out) {
+        if (st != 7) {  // stage 7 shares stage 6's time
+            const T cst = st == 2 ? (T)(1.0 / 5.0) : st == 3 ? (T)(3.0 / 10.0) : st == 4 ? (T)(4.0 / 5.0)
+                        : st == 5 ? (T)(8.0 / 9.0) : (T)1.0;
+            T w[UC_HARM], phi[UC_HARM];
+            uc_harm_coef<T>(a, w, phi);
+#pragma unroll
+            for (int k = 0; k < UC_HARM; ++k) {
+                T cd, sd;
+                uc_sincos_small(w[k] * (cst * h), cd, sd);
+                hc[k] = f.c[k] * cd - f.s[k] * sd;
+                hs[k] = f.s[k] * cd + f.c[k] * sd;
+                if (st == 6) { c1[k] = hc[k]; s1[k] = hs[k]; }
+            }
+        }
+        uc_rhs_h<T>(xs, a, ts, hc, hs, out);
+    });
+}
+#endif
 
 struct BasF64 {
     double x[UC_N];
@@ -2903,23 +3058,45 @@ struct BasF32C {
 // One accepted-or-rejected attempt for each state policy. Returns uc_dp5's
 // norm (its square in fast modules); on acceptance (<= 1) the state moves
 // and k1 <- k7 (FSAL).
+// With harmonic forcing (fast modules) the forcing values move to t + h on
+// acceptance as well.
+template <typename T>
+__device__ __forceinline__ double uc_bas_dp5(const T* x, const T* a, T t, T h, const T* k1, T* dx, T* k7,
+                                             double rtol, double atol, UcForcing<T>& f, T* c1, T* s1) {
+#if UC_HARM
+    return uc_dp5_forced<T>(x, a, t, h, k1, dx, k7, rtol, atol, f, c1, s1);
+#else
+    (void)f; (void)c1; (void)s1;
+    return uc_dp5<T>(x, a, t, h, k1, dx, k7, rtol, atol);
+#endif
+}
+template <typename T> __device__ __forceinline__ void uc_bas_forcing_on(UcForcing<T>& f, const T* c1, const T* s1) {
+#if UC_HARM
+#pragma unroll
+    for (int k = 0; k < UC_HARM; ++k) { f.c[k] = c1[k]; f.s[k] = s1[k]; }
+#else
+    (void)f; (void)c1; (void)s1;
+#endif
+}
 __device__ __forceinline__ double uc_bas_attempt(BasF64& s, const double* a, double hs, double* k1, double rtol,
-                                                 double atol) {
-    double dx[UC_N], k7[UC_N];
-    const double err = uc_dp5<double>(s.x, a, s.t, hs, k1, dx, k7, rtol, atol);
+                                                 double atol, UcForcing<double>& f) {
+    double dx[UC_N], k7[UC_N], c1[UC_HARM > 0 ? UC_HARM : 1], s1[UC_HARM > 0 ? UC_HARM : 1];
+    const double err = uc_bas_dp5<double>(s.x, a, s.t, hs, k1, dx, k7, rtol, atol, f, c1, s1);
     if (err <= 1.0) {
 #pragma unroll
         for (int i = 0; i < UC_N; ++i) { s.x[i] = s.x[i] + dx[i]; k1[i] = k7[i]; }
+        uc_bas_forcing_on(f, c1, s1);
     }
     return err;
 }
 __device__ __forceinline__ double uc_bas_attempt(BasF32C& s, const float* a, double hs, float* k1, double rtol,
-                                                 double atol) {
-    float dx[UC_N], k7[UC_N];
-    const double err = uc_dp5<float>(s.hi, a, s.thi, (float)hs, k1, dx, k7, rtol, atol);
+                                                 double atol, UcForcing<float>& f) {
+    float dx[UC_N], k7[UC_N], c1[UC_HARM > 0 ? UC_HARM : 1], s1[UC_HARM > 0 ? UC_HARM : 1];
+    const double err = uc_bas_dp5<float>(s.hi, a, s.thi, (float)hs, k1, dx, k7, rtol, atol, f, c1, s1);
     if (err <= 1.0) {
 #pragma unroll
         for (int i = 0; i < UC_N; ++i) { uc_kahan_add(s.hi[i], s.c[i], dx[i]); k1[i] = k7[i]; }
+        uc_bas_forcing_on(f, c1, s1);
     }
     return err;
 }
@@ -2933,8 +3110,22 @@ __device__ __forceinline__ void uc_bas_state(UcSlab& io, BasF32C& s) {
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) io.f32(s.c[i]);
 }
-__device__ __forceinline__ void uc_bas_k1(BasF64& s, const double* a, double* k1) { uc_rhs<double>(s.x, a, s.t, k1); }
-__device__ __forceinline__ void uc_bas_k1(BasF32C& s, const float* a, float* k1) { uc_rhs<float>(s.hi, a, s.thi, k1); }
+// k1 at a section start (local time 0, forcing just reset)
+template <typename T> __device__ __forceinline__ void uc_bas_rhs(const T* x, const T* a, T t, const UcForcing<T>& f,
+                                                                T* k1) {
+#if UC_HARM
+    uc_rhs_h<T>(x, a, t, f.c, f.s, k1);
+#else
+    (void)f;
+    uc_rhs<T>(x, a, t, k1);
+#endif
+}
+__device__ __forceinline__ void uc_bas_k1(BasF64& s, const double* a, double* k1, const UcForcing<double>& f) {
+    uc_bas_rhs<double>(s.x, a, s.t, f, k1);
+}
+__device__ __forceinline__ void uc_bas_k1(BasF32C& s, const float* a, float* k1, const UcForcing<float>& f) {
+    uc_bas_rhs<float>(s.hi, a, s.thi, f, k1);
+}
 
 template <class S> struct BasPrec;
 template <> struct BasPrec<BasF64> {
@@ -2998,15 +3189,17 @@ __device__ void uc_basins_cell(const UcJob& j, long long lane) {
         cell_state(false);
     }
     T k1[UC_N];
+    UcForcing<T> frc;
     for (long long sec = sec0; sec < sec1 && status == 0; ++sec) {
         s.reset_time();
-        uc_bas_k1(s, a, k1);
+        frc.reset(a);
+        uc_bas_k1(s, a, k1, frc);
         for (;;) {
             if (acc + rej >= j.max_attempts) { status = -2; break; }
             const double rem = s.remaining(T_sec);
             const bool last = h >= rem;
             const double hs = last ? rem : h;
-            const double err = uc_bas_attempt(s, a, hs, k1, rtol, atol);
+            const double err = uc_bas_attempt(s, a, hs, k1, rtol, atol, frc);
 #if UC_STRICT
             double fac = err == 0.0 ? 5.0 : 0.9 * pow(err, -0.2);
 #else
@@ -3107,7 +3300,18 @@ __device__ void uc_basins_cell(const UcJob& j, long long lane) {
     }
 }
 
-UC_DIRECT(uc_bas_f64, uc_basins_cell<BasF64>)
+#if UC_HARM
+// The forced integration needs a few more registers than the library calls
+// it replaces, just past the three waves per SIMD the FP64 map runs at;
+// the register budget is held there.
+#define UC_BAS_WAVES __attribute__((amdgpu_waves_per_eu(3)))
+#else
+#define UC_BAS_WAVES
+#endif
+extern "C" __global__ void __launch_bounds__(UC_BLOCK) UC_BAS_WAVES uc_bas_f64(UcJob j) {
+    const long long lane = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (lane < j.n_cells) uc_basins_cell<BasF64>(j, lane);
+}
 UC_DIRECT(uc_bas_f32c, uc_basins_cell<BasF32C>)
 UC_PERSISTENT(uc_bas_f64_q, uc_basins_cell<BasF64>)
 UC_PERSISTENT(uc_bas_f32c_q, uc_basins_cell<BasF32C>)
