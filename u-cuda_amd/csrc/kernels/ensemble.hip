@@ -101,11 +101,125 @@ template <typename T> __device__ __forceinline__ T uc_opaque(T v) {
     return v;
 }
 
+// Harmonic forcing in the fast modules: the generated source lists the
+// right-hand side's cos / sin terms whose argument is w * t + phi (w, phi
+// from the parameters; uc_harm_coef) and gives uc_rhs_h, the right-hand
+// side with those terms as inputs hc[k] / hs[k]. Strict modules keep the
+// reference's calls.
+#if !UC_STRICT && defined(UC_HARMONICS) && UC_HARMONICS > 0 && UC_HARMONICS <= 8
+#define UC_HARM UC_HARMONICS
+#else
+#define UC_HARM 0
+#endif
+
+// cos / sin of d: the Taylor series for |d| <= 0.8 (truncation below 1e-18
+// relative in double, 2e-10 in float). Larger d (rare: a step longer than
+// 0.8 / w) is reduced by multiples of 2 pi (two-part constant, exact
+// enough while d's own ulp is far below 2 pi: |d| < 1e15 in double, 1e7 in
+// float; beyond that, and for inf / NaN, the result is NaN), halved until
+// |d| <= 0.8 (at most twice after the reduction) and doubled back, a few
+// ulps and no library range reduction.
+// [plain C++ from here to the matching end mark, compiled on the host too]
+__device__ __forceinline__ void uc_sincos_poly(double d, double& c, double& s) {
+    const double z = d * d;
+    double p = 1.0 / 6402373705728000.0;  // 1/18!
+    p = fma(p, z, -1.0 / 20922789888000.0);
+    p = fma(p, z, 1.0 / 87178291200.0);
+    p = fma(p, z, -1.0 / 479001600.0);
+    p = fma(p, z, 1.0 / 3628800.0);
+    p = fma(p, z, -1.0 / 40320.0);
+    p = fma(p, z, 1.0 / 720.0);
+    p = fma(p, z, -1.0 / 24.0);
+    p = fma(p, z, 0.5);
+    c = fma(-p, z, 1.0);
+    double q = 1.0 / 355687428096000.0;  // 1/17!
+    q = fma(q, z, -1.0 / 1307674368000.0);
+    q = fma(q, z, 1.0 / 6227020800.0);
+    q = fma(q, z, -1.0 / 39916800.0);
+    q = fma(q, z, 1.0 / 362880.0);
+    q = fma(q, z, -1.0 / 5040.0);
+    q = fma(q, z, 1.0 / 120.0);
+    q = fma(q, z, -1.0 / 6.0);
+    s = fma(q * z, d, d);
+}
+__device__ __forceinline__ void uc_sincos_poly(float d, float& c, float& s) {
+    const float z = d * d;
+    float p = -1.0f / 3628800.0f;  // -1/10!
+    p = fmaf(p, z, 1.0f / 40320.0f);
+    p = fmaf(p, z, -1.0f / 720.0f);
+    p = fmaf(p, z, 1.0f / 24.0f);
+    p = fmaf(p, z, -0.5f);
+    c = fmaf(p, z, 1.0f);
+    float q = -1.0f / 39916800.0f;  // -1/11!
+    q = fmaf(q, z, 1.0f / 362880.0f);
+    q = fmaf(q, z, -1.0f / 5040.0f);
+    q = fmaf(q, z, 1.0f / 120.0f);
+    q = fmaf(q, z, -1.0f / 6.0f);
+    s = fmaf(q * z, d, d);
+}
+template <typename T> struct UcTwoPi;
+template <> struct UcTwoPi<double> {
+    static constexpr double hi = 6.283185307179586, lo = 2.4492935982947064e-16, inv = 0.15915494309189535,
+                            cap = 1e15;
+};
+template <> struct UcTwoPi<float> {
+    static constexpr float hi = 6.2831854820251465f, lo = -1.7484555314695172e-07f, inv = 0.15915494f, cap = 1e7f;
+};
+template <typename T> __device__ __forceinline__ void uc_sincos_small(T d, T& c, T& s) {
+    if (fabs(d) <= (T)0.8) {
+        uc_sincos_poly(d, c, s);
+        return;
+    }
+    if (!(fabs(d) < UcTwoPi<T>::cap)) {
+        c = s = (T)__builtin_nan("");
+        return;
+    }
+    const T n = rint(d * UcTwoPi<T>::inv);
+    d = fma(-n, UcTwoPi<T>::hi, d);
+    d = fma(-n, UcTwoPi<T>::lo, d);
+    int halvings = 0;
+    while (fabs(d) > (T)0.8) {
+        d *= (T)0.5;
+        ++halvings;
+    }
+    uc_sincos_poly(d, c, s);
+    for (int i = 0; i < halvings; ++i) {
+        const T s2 = (T)2 * s * c;
+        c = (c - s) * (c + s);
+        s = s2;
+    }
+}
+
+// [end of plain C++]
+
+#if UC_HARM
+// cos / sin of every forcing phase at the state's own time, for the
+// fixed-step RK4 loops: two rotations per step (to t + h/2, then t + h) by
+// the cached half-step angle, in double. They are evaluated afresh from
+// w t + phi (uc_sincos_small: reduction by 2 pi, Taylor series, a few ulps;
+// the library's range reduction would be inlined into every unrolled step)
+// when the caller's time or step is not the one tracked (a fresh state, one
+// restored from the slab, a reset clock) and every 128 steps, so rounding
+// never builds up over more than 128 steps.
+struct UcPhase {
+    double c[UC_HARM], s[UC_HARM];    // at time t
+    double ch[UC_HARM], sh[UC_HARM];  // cos / sin(w hh)
+    double t, hh;
+    int n = -1;                       // steps since the exact evaluation; -1: none
+};
+#endif
+
 struct StateF64 {
     double x[UC_N];
+#if UC_HARM
+    UcPhase ph;
+#endif
     __device__ __forceinline__ void load(const double* x0) {
 #pragma unroll
         for (int i = 0; i < UC_N; ++i) x[i] = x0[i];
+#if UC_HARM
+        ph.n = -1;
+#endif
     }
     __device__ __forceinline__ double get(int i) const { return x[i]; }
     // component q (0 <= q < N) selected without indexing the array
@@ -120,12 +234,18 @@ struct StateF64 {
 // F32C state: hi + Kahan compensation c (true value ~ hi - c).
 struct StateF32C {
     float hi[UC_N], c[UC_N];
+#if UC_HARM
+    UcPhase ph;
+#endif
     __device__ __forceinline__ void load(const double* x0) {
 #pragma unroll
         for (int i = 0; i < UC_N; ++i) {
             hi[i] = (float)x0[i];
             c[i] = (float)((double)hi[i] - x0[i]);
         }
+#if UC_HARM
+        ph.n = -1;
+#endif
     }
     __device__ __forceinline__ double get(int i) const { return (double)hi[i] - (double)c[i]; }
     // the pair is selected in float and converted once (same value as get(q))
@@ -139,6 +259,18 @@ struct StateF32C {
         return (double)h - (double)cc;
     }
 };
+
+// Forcing phases are evaluated afresh at the start of every unit a sliced
+// launch can begin with (a 100-step chunk, a renormalisation block, 100
+// recorded samples), in every kernel, so a cell resumed from the slab
+// carries the same values as one run whole.
+template <class S> __device__ __forceinline__ void uc_phase_mark(S& s) {
+#if UC_HARM
+    s.ph.n = -1;
+#else
+    (void)s;
+#endif
+}
 
 // ------------------------------------------------------ resumable slices --
 // A sliced cell (UcJob.slab set) keeps its state in the slab between
@@ -249,31 +381,84 @@ template <typename T> __device__ __forceinline__ T uc_h6(const StepU<T>& s) { re
 __device__ __forceinline__ void uc_job_step(const UcJob& j, StepU<double>& s) { s.h = j.h; s.hh = j.hh; s.h6 = j.h6; }
 __device__ __forceinline__ void uc_job_step(const UcJob& j, StepU<float>& s) { s.h = j.hf; s.hh = j.hhf; s.h6 = j.h6f; }
 
+// UcRhs: the right-hand side of the four stages. With harmonic forcing the
+// stage values come from the state's tracked phase (start):
+// stage 1 at t, stages 2 and 3 at t + h/2, stage 4 at t + h.
+template <typename T, class S> struct UcRhs {
+#if UC_HARM
+    T c0[UC_HARM], s0[UC_HARM], c1[UC_HARM], s1[UC_HARM], c2[UC_HARM], s2[UC_HARM];
+#endif
+    __device__ __forceinline__ void start(S& st, const T* a, double t, double h, double hh) {
+#if UC_HARM
+        UcPhase& ph = st.ph;
+        if (ph.n < 0 || ph.n >= 128 || ph.t != t || ph.hh != hh) {
+            T w[UC_HARM], phi[UC_HARM];
+            uc_harm_coef<T>(a, w, phi);
+#pragma unroll
+            for (int k = 0; k < UC_HARM; ++k) {
+                uc_sincos_small((double)w[k] * t + (double)phi[k], ph.c[k], ph.s[k]);
+                uc_sincos_small((double)w[k] * hh, ph.ch[k], ph.sh[k]);
+            }
+            ph.hh = hh;
+            ph.n = 0;
+        }
+#pragma unroll
+        for (int k = 0; k < UC_HARM; ++k) {
+            const double c = ph.c[k], s = ph.s[k], ch = ph.ch[k], sh = ph.sh[k];
+            const double cm = c * ch - s * sh, sm = s * ch + c * sh;
+            const double ce = cm * ch - sm * sh, se = sm * ch + cm * sh;
+            c0[k] = (T)c; s0[k] = (T)s;
+            c1[k] = (T)cm; s1[k] = (T)sm;
+            c2[k] = (T)ce; s2[k] = (T)se;
+            ph.c[k] = ce;
+            ph.s[k] = se;
+        }
+        ph.t = t + h;
+        ++ph.n;
+#else
+        (void)st; (void)a; (void)t; (void)h; (void)hh;
+#endif
+    }
+    // stage q = 0..3 at time ts
+    __device__ __forceinline__ void eval(int q, const T* x, const T* a, T ts, T* k) const {
+#if UC_HARM
+        uc_rhs_h<T>(x, a, ts, q == 0 ? c0 : q == 3 ? c2 : c1, q == 0 ? s0 : q == 3 ? s2 : s1, k);
+#else
+        (void)q;
+        uc_rhs<T>(x, a, ts, k);
+#endif
+    }
+};
+
 #if UC_CUSTOM_KRS
 // A caller-supplied step body (the reference's custom KRS schemes,
 // U-CUDA/system_record.h:10-13) is the step map itself: numb = double, or
 // float without compensation (the body forms X + increment internally).
 template <class H> __device__ __forceinline__ void uc_rk4(StateF64& s, const double* a, H h, double) { uc_krs_step<double>(s.x, a, uc_h(h)); }
-template <class H> __device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, H h, float) { uc_krs_step<float>(s.hi, a, uc_h(h)); }
+template <class H> __device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, H h, double) { uc_krs_step<float>(s.hi, a, uc_h(h)); }
 #else
 // Classic RK4 with the reference's stage structure and update order
 // (U-CUDA/codegen.cpp:597-606): X1 = X + 0.5*h*k for stages 0,1, X1 = X + h*k
 // for stage 2, X += h*(k0 + 2*k1 + 2*k2 + k3)/6. Accumulating the bracket
-// left to right keeps the reference's summation order.
+// left to right keeps the reference's summation order. t is the step's
+// start time (double; the stages see it in their precision).
+//
 template <class H>
 __device__ __forceinline__ void uc_rk4(StateF64& s, const double* a, H hs, double t) {
     double x1[UC_N], k[UC_N], acc[UC_N];
     const double h = uc_h(hs), hh = uc_hh(hs);
-    uc_rhs<double>(s.x, a, t, k);
+    UcRhs<double, StateF64> f;
+    f.start(s, a, t, h, hh);
+    f.eval(0, s.x, a, t, k);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = k[i]; x1[i] = s.x[i] + hh * k[i]; }
-    uc_rhs<double>(x1, a, t + hh, k);
+    f.eval(1, x1, a, t + hh, k);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = s.x[i] + hh * k[i]; }
-    uc_rhs<double>(x1, a, t + hh, k);
+    f.eval(2, x1, a, t + hh, k);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = s.x[i] + h * k[i]; }
-    uc_rhs<double>(x1, a, t + h, k);
+    f.eval(3, x1, a, t + h, k);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) {
         acc[i] = acc[i] + k[i];
@@ -286,19 +471,21 @@ __device__ __forceinline__ void uc_rk4(StateF64& s, const double* a, H hs, doubl
 }
 
 template <class H>
-__device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, H hs, float t) {
+__device__ __forceinline__ void uc_rk4(StateF32C& s, const float* a, H hs, double td) {
     float x1[UC_N], k[UC_N], acc[UC_N];
-    const float h = uc_h(hs), hh = uc_hh(hs);
-    uc_rhs<float>(s.hi, a, t, k);
+    const float h = uc_h(hs), hh = uc_hh(hs), t = (float)td;
+    UcRhs<float, StateF32C> f;
+    f.start(s, a, td, (double)h, (double)hh);
+    f.eval(0, s.hi, a, t, k);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = k[i]; x1[i] = s.hi[i] + hh * k[i]; }
-    uc_rhs<float>(x1, a, t + hh, k);
+    f.eval(1, x1, a, t + hh, k);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = acc[i] + 2.0f * k[i]; x1[i] = s.hi[i] + hh * k[i]; }
-    uc_rhs<float>(x1, a, t + hh, k);
+    f.eval(2, x1, a, t + hh, k);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = acc[i] + 2.0f * k[i]; x1[i] = s.hi[i] + h * k[i]; }
-    uc_rhs<float>(x1, a, t + h, k);
+    f.eval(3, x1, a, t + h, k);
     const float h6 = uc_h6(hs);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) {
@@ -317,25 +504,27 @@ __device__ __forceinline__ void uc_rk4_tan(StateF64& s, double* V, const double*
     double x1[UC_N], k[UC_N], acc[UC_N];
     double v1[K * UC_N], l[K * UC_N], lacc[K * UC_N];
     const double h = uc_h(hs), hh = uc_hh(hs);
-    uc_rhs<double>(s.x, a, t, k);
+    UcRhs<double, StateF64> f;
+    f.start(s, a, t, h, hh);
+    f.eval(0, s.x, a, t, k);
     uc_tangent<double, K>(s.x, a, t, V, l);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = k[i]; x1[i] = s.x[i] + hh * k[i]; }
 #pragma unroll
     for (int i = 0; i < K * UC_N; ++i) { lacc[i] = l[i]; v1[i] = V[i] + hh * l[i]; }
-    uc_rhs<double>(x1, a, t + hh, k);
+    f.eval(1, x1, a, t + hh, k);
     uc_tangent<double, K>(x1, a, t + hh, v1, l);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = s.x[i] + hh * k[i]; }
 #pragma unroll
     for (int i = 0; i < K * UC_N; ++i) { lacc[i] = lacc[i] + 2.0 * l[i]; v1[i] = V[i] + hh * l[i]; }
-    uc_rhs<double>(x1, a, t + hh, k);
+    f.eval(2, x1, a, t + hh, k);
     uc_tangent<double, K>(x1, a, t + hh, v1, l);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = acc[i] + 2.0 * k[i]; x1[i] = s.x[i] + h * k[i]; }
 #pragma unroll
     for (int i = 0; i < K * UC_N; ++i) { lacc[i] = lacc[i] + 2.0 * l[i]; v1[i] = V[i] + h * l[i]; }
-    uc_rhs<double>(x1, a, t + h, k);
+    f.eval(3, x1, a, t + h, k);
     uc_tangent<double, K>(x1, a, t + h, v1, l);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) {
@@ -358,29 +547,31 @@ __device__ __forceinline__ void uc_rk4_tan(StateF64& s, double* V, const double*
 }
 
 template <int K, class H>
-__device__ __forceinline__ void uc_rk4_tan(StateF32C& s, float* V, const float* a, H hs, float t) {
+__device__ __forceinline__ void uc_rk4_tan(StateF32C& s, float* V, const float* a, H hs, double td) {
     float x1[UC_N], k[UC_N], acc[UC_N];
     float v1[K * UC_N], l[K * UC_N], lacc[K * UC_N];
-    const float h = uc_h(hs), hh = uc_hh(hs);
-    uc_rhs<float>(s.hi, a, t, k);
+    const float h = uc_h(hs), hh = uc_hh(hs), t = (float)td;
+    UcRhs<float, StateF32C> f;
+    f.start(s, a, td, (double)h, (double)hh);
+    f.eval(0, s.hi, a, t, k);
     uc_tangent<float, K>(s.hi, a, t, V, l);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = k[i]; x1[i] = s.hi[i] + hh * k[i]; }
 #pragma unroll
     for (int i = 0; i < K * UC_N; ++i) { lacc[i] = l[i]; v1[i] = V[i] + hh * l[i]; }
-    uc_rhs<float>(x1, a, t + hh, k);
+    f.eval(1, x1, a, t + hh, k);
     uc_tangent<float, K>(x1, a, t + hh, v1, l);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = acc[i] + 2.0f * k[i]; x1[i] = s.hi[i] + hh * k[i]; }
 #pragma unroll
     for (int i = 0; i < K * UC_N; ++i) { lacc[i] = lacc[i] + 2.0f * l[i]; v1[i] = V[i] + hh * l[i]; }
-    uc_rhs<float>(x1, a, t + hh, k);
+    f.eval(2, x1, a, t + hh, k);
     uc_tangent<float, K>(x1, a, t + hh, v1, l);
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) { acc[i] = acc[i] + 2.0f * k[i]; x1[i] = s.hi[i] + h * k[i]; }
 #pragma unroll
     for (int i = 0; i < K * UC_N; ++i) { lacc[i] = lacc[i] + 2.0f * l[i]; v1[i] = V[i] + h * l[i]; }
-    uc_rhs<float>(x1, a, t + h, k);
+    f.eval(3, x1, a, t + h, k);
     uc_tangent<float, K>(x1, a, t + h, v1, l);
     const float h6 = uc_h6(hs);
 #pragma unroll
@@ -500,9 +691,9 @@ __device__ int uc_loop(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h
                 v = s.get_sel(wvar);
             }
             obs.sample(i, v);
-            for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += hd; }
+            for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, t); if (UC_USES_TIME) t += hd; }
         } else {
-            uc_rk4(s, a, h, (T)t);
+            uc_rk4(s, a, h, t);
             if (UC_USES_TIME) t += hd;
         }
         if (until_check == 0) {
@@ -514,7 +705,7 @@ __device__ int uc_loop(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h
     double prev[UC_N];
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
-    for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += hd; }
+    for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, t); if (UC_USES_TIME) t += hd; }
     return uc_fixed_point_flag(s, prev, hd, pre);
 }
 
@@ -528,7 +719,7 @@ __device__ int uc_loop_tan(S& s, typename Prec<S>::T* V, const typename Prec<S>:
     const double hd = (double)h;
     int until_check = 0;
     for (long long i = 0; i < iters; ++i) {
-        uc_rk4_tan<K>(s, V, a, h, (T)t);
+        uc_rk4_tan<K>(s, V, a, h, t);
         if (UC_USES_TIME) t += hd;
         if (until_check == 0) {
             if (uc_diverged(s, max_value)) return 0;
@@ -539,7 +730,7 @@ __device__ int uc_loop_tan(S& s, typename Prec<S>::T* V, const typename Prec<S>:
     double prev[UC_N];
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
-    uc_rk4_tan<K>(s, V, a, h, (T)t);
+    uc_rk4_tan<K>(s, V, a, h, t);
     if (UC_USES_TIME) t += hd;
     return uc_fixed_point_flag(s, prev, hd, 1);
 }
@@ -572,9 +763,9 @@ __device__ __forceinline__ void uc_iter(S& s, const typename Prec<S>::T* a, type
     typedef typename Prec<S>::T T;
     if (Obs::records) {
         if (!dead) obs.sample(i, uc_sample_value(s, wvar));
-        for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += (double)h; }
+        for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, t); if (UC_USES_TIME) t += (double)h; }
     } else {
-        uc_rk4(s, a, h, (T)t);
+        uc_rk4(s, a, h, t);
         if (UC_USES_TIME) t += (double)h;
     }
 }
@@ -593,6 +784,7 @@ __device__ int uc_loop_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T
     S snap = s;
     double tsnap = t;
     for (long long i = i0; i < iters;) {
+        uc_phase_mark(s);
         uc_iter(s, a, h, t, i, pre, wvar, dead, obs);
         if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
         if (uc_wave_all(dead)) break;
@@ -605,7 +797,7 @@ __device__ int uc_loop_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T
     double prev[UC_N];
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
-    for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, (T)t); if (UC_USES_TIME) t += hd; }
+    for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, t); if (UC_USES_TIME) t += hd; }
     return uc_fixed_point_flag(s, prev, hd, pre);
 }
 
@@ -624,14 +816,15 @@ __device__ int uc_loop_tan_u(S& s, typename Prec<S>::T* V, const typename Prec<S
     S snap = s;
     double tsnap = t;
     for (long long i = 0; i < iters;) {
-        uc_rk4_tan<K>(s, V, a, h, (T)t);
+        uc_phase_mark(s);
+        uc_rk4_tan<K>(s, V, a, h, t);
         if (UC_USES_TIME) t += hd;
         if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
         if (uc_wave_all(dead)) break;
         const long long stop = (iters - i > UC_CHECK_INTERVAL) ? i + UC_CHECK_INTERVAL : iters;
 #pragma unroll 2
         for (++i; i < stop; ++i) {
-            uc_rk4_tan<K>(s, V, a, h, (T)t);
+            uc_rk4_tan<K>(s, V, a, h, t);
             if (UC_USES_TIME) t += hd;
         }
     }
@@ -639,7 +832,7 @@ __device__ int uc_loop_tan_u(S& s, typename Prec<S>::T* V, const typename Prec<S
     double prev[UC_N];
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
-    uc_rk4_tan<K>(s, V, a, h, (T)t);
+    uc_rk4_tan<K>(s, V, a, h, t);
     if (UC_USES_TIME) t += hd;
     return uc_fixed_point_flag(s, prev, hd, 1);
 }
@@ -656,14 +849,15 @@ template <int K, class S>
 __device__ __forceinline__ int uc_block_tan_u(S& s, typename Prec<S>::T* V, const typename Prec<S>::T* a,
                                               typename Prec<S>::T h, double& t, int iters, double max_value) {
     typedef typename Prec<S>::T T;
-    uc_rk4_tan<K>(s, V, a, h, (T)t);
+    uc_phase_mark(s);
+    uc_rk4_tan<K>(s, V, a, h, t);
     if (UC_USES_TIME) t += (double)h;
     const bool dead = uc_diverged(s, max_value);
     if (uc_wave_all(dead)) return 0;
     if (!dead) {
 #pragma unroll 2
         for (int i = 0; i < iters; ++i) {
-            uc_rk4_tan<K>(s, V, a, h, (T)t);
+            uc_rk4_tan<K>(s, V, a, h, t);
             if (UC_USES_TIME) t += (double)h;
         }
     }
@@ -816,10 +1010,11 @@ __device__ int uc_record_u1(S& s, const typename Prec<S>::T* a, typename Prec<S>
         cn = ends ? c + 2 : cn;
         w0 = w1;
         w1 = v;
-        uc_rk4(s, a, h, (T)t);
+        uc_rk4(s, a, h, t);
         if (UC_USES_TIME) t += hd;
     };
     for (int i = i0; i < iters;) {
+        uc_phase_mark(s);
         iter(i);
         if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
         if (uc_wave_all(dead)) break;
@@ -836,7 +1031,7 @@ __device__ int uc_record_u1(S& s, const typename Prec<S>::T* a, typename Prec<S>
     double prev[UC_N];
 #pragma unroll
     for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
-    uc_rk4(s, a, h, (T)t);
+    uc_rk4(s, a, h, t);
     if (UC_USES_TIME) t += hd;
     return uc_fixed_point_flag(s, prev, hd, 1);
 }
@@ -1453,7 +1648,10 @@ __device__ void uc_lle_lanes(const UcJob& j, long long lane, long long half) {
     // transient: uc_loop_u with pre = 1 and no observer, per cell
     for (long long i = 0; i < j.skip_steps;) {
 #pragma unroll
-        for (int c = 0; c < C; ++c) uc_rk4(s[c], a[c], h, (T)t);
+        for (int c = 0; c < C; ++c) {
+            uc_phase_mark(s[c]);
+            uc_rk4(s[c], a[c], h, t);
+        }
         if (UC_USES_TIME) t += hdd;
 #pragma unroll
         for (int c = 0; c < C; ++c)
@@ -1463,12 +1661,12 @@ __device__ void uc_lle_lanes(const UcJob& j, long long lane, long long half) {
 #pragma unroll 2
         for (++i; i < stop; ++i) {
 #pragma unroll
-            for (int c = 0; c < C; ++c) uc_rk4(s[c], a[c], h, (T)t);
+            for (int c = 0; c < C; ++c) uc_rk4(s[c], a[c], h, t);
             if (UC_USES_TIME) t += hdd;
         }
     }
 #pragma unroll
-    for (int c = 0; c < C; ++c) uc_rk4(s[c], a[c], h, (T)t);  // the trailing step
+    for (int c = 0; c < C; ++c) uc_rk4(s[c], a[c], h, t);  // the trailing step
     if (UC_USES_TIME) t += hdd;
 
     T V[2][UC_N], Vs[2][UC_N], start2[2];
@@ -1493,7 +1691,10 @@ __device__ void uc_lle_lanes(const UcJob& j, long long lane, long long half) {
         // power-of-two rescale as selects: the per-cell branches of the
         // check cost more than the check at blocks of ~10 steps
 #pragma unroll
-        for (int c = 0; c < C; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
+        for (int c = 0; c < C; ++c) {
+            uc_phase_mark(s[c]);
+            uc_rk4_tan<1>(s[c], V[c], a[c], h, t);
+        }
         if (UC_USES_TIME) t += hdd;
 #pragma unroll
         for (int c = 0; c < C; ++c) {
@@ -1507,14 +1708,14 @@ __device__ void uc_lle_lanes(const UcJob& j, long long lane, long long half) {
 #pragma unroll
             for (int i = 0; i < UC_LLE_BLOCK; ++i) {
 #pragma unroll
-                for (int c = 0; c < C; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
+                for (int c = 0; c < C; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, t);
                 if (UC_USES_TIME) t += hdd;
             }
         } else {
 #pragma unroll 4
             for (int i = 0; i < nts; ++i) {
 #pragma unroll
-                for (int c = 0; c < C; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, (T)t);
+                for (int c = 0; c < C; ++c) uc_rk4_tan<1>(s[c], V[c], a[c], h, t);
                 if (UC_USES_TIME) t += hdd;
             }
         }
@@ -2883,10 +3084,7 @@ __device__ __forceinline__ double uc_dp5(const T* x, const T* a, T t, T h, const
                           [&](const T* xs, int, T ts, T* out) { uc_rhs<T>(xs, a, ts, out); });
 }
 
-// Harmonic forcing in the fast modules. The generated source lists the
-// right-hand side's cos / sin terms whose argument is w * t + phi (w, phi
-// from the parameters; uc_harm_coef) and gives uc_rhs_h, the right-hand
-// side with those terms as inputs hc[k] / hs[k]. The basins integration
+// Harmonic forcing (UC_HARM) in the adaptive integration: the basins loop
 // keeps cos / sin of every phase at the current attempt's start time
 // (exact at each section start, local time 0: cos / sin(phi)) and moves
 // them to the stage times t + c h by the rotation (C, S) -> (C cd - S sd,
@@ -2895,93 +3093,7 @@ __device__ __forceinline__ double uc_dp5(const T* x, const T* a, T t, T h, const
 // the two stages at t + h. An accepted step takes the t + h values on, so
 // within a section the values carry the rounding of up to a few hundred
 // rotations (~1e-14 relative in double), below the rounding the library
-// call sees in its argument w * t itself. Strict modules keep the
-// reference's calls.
-#if !UC_STRICT && defined(UC_HARMONICS) && UC_HARMONICS > 0 && UC_HARMONICS <= 8
-#define UC_HARM UC_HARMONICS
-#else
-#define UC_HARM 0
-#endif
-
-// cos / sin of d: the Taylor series for |d| <= 0.8 (truncation below 1e-18
-// relative in double, 2e-10 in float). Larger d (rare: a step longer than
-// 0.8 / w) is reduced by multiples of 2 pi (two-part constant, exact
-// enough while d's own ulp is far below 2 pi: |d| < 1e15 in double, 1e7 in
-// float; beyond that, and for inf / NaN, the result is NaN), halved until
-// |d| <= 0.8 (at most twice after the reduction) and doubled back, a few
-// ulps and no library range reduction.
-// [plain C++ from here to the matching end mark, compiled on the host too]
-__device__ __forceinline__ void uc_sincos_poly(double d, double& c, double& s) {
-    const double z = d * d;
-    double p = 1.0 / 6402373705728000.0;  // 1/18!
-    p = fma(p, z, -1.0 / 20922789888000.0);
-    p = fma(p, z, 1.0 / 87178291200.0);
-    p = fma(p, z, -1.0 / 479001600.0);
-    p = fma(p, z, 1.0 / 3628800.0);
-    p = fma(p, z, -1.0 / 40320.0);
-    p = fma(p, z, 1.0 / 720.0);
-    p = fma(p, z, -1.0 / 24.0);
-    p = fma(p, z, 0.5);
-    c = fma(-p, z, 1.0);
-    double q = 1.0 / 355687428096000.0;  // 1/17!
-    q = fma(q, z, -1.0 / 1307674368000.0);
-    q = fma(q, z, 1.0 / 6227020800.0);
-    q = fma(q, z, -1.0 / 39916800.0);
-    q = fma(q, z, 1.0 / 362880.0);
-    q = fma(q, z, -1.0 / 5040.0);
-    q = fma(q, z, 1.0 / 120.0);
-    q = fma(q, z, -1.0 / 6.0);
-    s = fma(q * z, d, d);
-}
-__device__ __forceinline__ void uc_sincos_poly(float d, float& c, float& s) {
-    const float z = d * d;
-    float p = -1.0f / 3628800.0f;  // -1/10!
-    p = fmaf(p, z, 1.0f / 40320.0f);
-    p = fmaf(p, z, -1.0f / 720.0f);
-    p = fmaf(p, z, 1.0f / 24.0f);
-    p = fmaf(p, z, -0.5f);
-    c = fmaf(p, z, 1.0f);
-    float q = -1.0f / 39916800.0f;  // -1/11!
-    q = fmaf(q, z, 1.0f / 362880.0f);
-    q = fmaf(q, z, -1.0f / 5040.0f);
-    q = fmaf(q, z, 1.0f / 120.0f);
-    q = fmaf(q, z, -1.0f / 6.0f);
-    s = fmaf(q * z, d, d);
-}
-template <typename T> struct UcTwoPi;
-template <> struct UcTwoPi<double> {
-    static constexpr double hi = 6.283185307179586, lo = 2.4492935982947064e-16, inv = 0.15915494309189535,
-                            cap = 1e15;
-};
-template <> struct UcTwoPi<float> {
-    static constexpr float hi = 6.2831854820251465f, lo = -1.7484555314695172e-07f, inv = 0.15915494f, cap = 1e7f;
-};
-template <typename T> __device__ __forceinline__ void uc_sincos_small(T d, T& c, T& s) {
-    if (fabs(d) <= (T)0.8) {
-        uc_sincos_poly(d, c, s);
-        return;
-    }
-    if (!(fabs(d) < UcTwoPi<T>::cap)) {
-        c = s = (T)__builtin_nan("");
-        return;
-    }
-    const T n = rint(d * UcTwoPi<T>::inv);
-    d = fma(-n, UcTwoPi<T>::hi, d);
-    d = fma(-n, UcTwoPi<T>::lo, d);
-    int halvings = 0;
-    while (fabs(d) > (T)0.8) {
-        d *= (T)0.5;
-        ++halvings;
-    }
-    uc_sincos_poly(d, c, s);
-    for (int i = 0; i < halvings; ++i) {
-        const T s2 = (T)2 * s * c;
-        c = (c - s) * (c + s);
-        s = s2;
-    }
-}
-
-// [end of plain C++]
+// call sees in its argument w * t itself.
 
 // cos / sin of each harmonic phase at the current attempt's start time; the
 // frequencies and phases are taken from the parameters where they are
