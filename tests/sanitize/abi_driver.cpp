@@ -22,6 +22,10 @@ int main() {
         {{"x", "v"}, {"delta", "gamma", "omega"}, {"v", "-delta*v+x-x^3+gamma*cos(omega*t)"}, "t", 0},
         {{"x", "y", "z", "w"}, {"a", "d"}, {"-y-z", "x+a*y+w", "3+x*z", "-0.5*z+d*w"}, nullptr, 0},
         {{"x", "y"}, {"a"}, {"\\frac{y}{2}", "-a \\cdot \\sin{x}"}, nullptr, 1},
+        // harmonic forcing terms of every linear form, and terms that stay calls
+        {{"x", "v"}, {"d", "g", "w", "f"},
+         {"v", "-d*v - x^3 + g*cos(w*t) + g*sin(2*w*t + f) - cos(t/w) + sin(x*t) + cos(w)*t + sin(-(w*t) - f)"
+               " + cos(f - w*t)"}, "t", 0},
     };
     std::vector<uc_system*> made;
     for (auto& S : systems) {
