@@ -677,3 +677,29 @@ def test_cell_ranges_equal_the_whole_grid(gpu_engine):
     # beyond the grid, and on an entry point that runs whole grids only
     with pytest.raises(ucuda.UcError, match="beyond the grid"):
         ucuda.run_lyap(ros, axes, [1.0, 1.0, 1.0], ros.values(), 0.01, 2.0, 20.0, 0.1, cells=(1000, 201))
+
+
+@pytest.mark.parametrize("t_max,wvar", [(10.01, 0), (10.02, 2)])
+def test_bif_record_odd_lengths_strict(gpu_engine, t_max, wvar):
+    """The recorded loop takes two samples per trip and ends a chunk on a
+    single one when its length is odd: records of 1,001 / 1,002 samples
+    after a 57-step transient, the x or z component, a cap of 8 peaks;
+    strict against the oracle bit for bit (flags, peaks, intervals)."""
+    print("case: Lorenz bif 64 rho, record t_max %g (var %d), transient 0.57, strict vs oracle" % (t_max, wvar),
+          flush=True)
+    s = ucuda.preset("lorenz", gpu_engine)
+    axes = [ucuda.axis(ucuda.AXIS_PARAM, 2, 64, 0.0, 250.0)]
+    vals = s.values()
+    step = oracle.KrsStep(gpu_engine.codegen_scheme(s.vars, s.params, s.rhs), 3, 4)
+    g = ucuda.run_bif(s, axes, [0.1, 0.1, 0.1], vals, 0.01, 0.57, t_max, writable_var=wvar, max_peaks=8,
+                      strict=True)
+    job = oracle.make_job(3, axes, [0.1, 0.1, 0.1], vals, 0.01, 0.57, t_max, writable_var=wvar, max_peaks=8)
+    r = oracle.bif_cells(step, job, range(64), threads=8)
+    gf, gp, gi = g["flags"].cpu().numpy(), g["peaks"].cpu().numpy(), g["intervals"].cpu().numpy()
+    np.testing.assert_array_equal(gf, r["flags"])
+    assert (gf == 8).any() and (gf >= 1).sum() > 32  # cells at the cap and many with peaks
+    for c in range(64):
+        k = min(int(gf[c]), 8)
+        if k > 0:
+            np.testing.assert_array_equal(gp[:k, c], r["peaks"][:k, c])
+            np.testing.assert_array_equal(gi[:k, c], r["intervals"][:k, c])
