@@ -1036,11 +1036,53 @@ __device__ int uc_record_u1(S& s, const typename Prec<S>::T* a, typename Prec<S>
     return uc_fixed_point_flag(s, prev, hd, 1);
 }
 
-// The recorded iterations [i0, i1) of a wave-uniform record: uc_record_u1
-// where it applies, uc_loop_u otherwise.
+// uc_loop_u with a PeakObserver whose search nobody reads (no peak slots,
+// raw loop flags: the recording of the DFT sweep), without the search: the
+// same samples stored (while the lane is live), steps, magnitude checks,
+// dead-lane rule and flag.
+template <class S>
+__device__ int uc_trace_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t, long long i0,
+                          long long iters, bool trail, int pre, int wvar, double max_value, double* traj,
+                          long long stride, long long traj_len) {
+    const double hd = (double)h;
+    bool dead = false;
+    S snap = s;
+    double tsnap = t;
+    auto iter = [&](long long k) {
+        const double v = uc_sample_value(s, wvar);
+        if (!dead && k < traj_len) traj[k * stride] = v;
+        for (int p = 0; p < pre; ++p) {
+            uc_rk4(s, a, h, t);
+            if (UC_USES_TIME) t += hd;
+        }
+    };
+    for (long long i = i0; i < iters;) {
+        uc_phase_mark(s);
+        iter(i);
+        if (!dead && uc_diverged(s, max_value)) { dead = true; snap = s; tsnap = t; }
+        if (uc_wave_all(dead)) break;
+        const long long stop = (iters - i > UC_CHECK_INTERVAL) ? i + UC_CHECK_INTERVAL : iters;
+#pragma unroll 2
+        for (++i; i < stop; ++i) iter(i);
+    }
+    if (dead) { s = snap; t = tsnap; return 0; }
+    if (!trail) return 1;
+    double prev[UC_N];
+#pragma unroll
+    for (int i = 0; i < UC_N; ++i) prev[i] = s.get(i);
+    for (int p = 0; p < pre; ++p) { uc_rk4(s, a, h, t); if (UC_USES_TIME) t += hd; }
+    return uc_fixed_point_flag(s, prev, hd, pre);
+}
+
+// The recorded iterations [i0, i1) of a wave-uniform record: uc_trace_u when
+// only the samples are wanted, uc_record_u1 where it applies, uc_loop_u
+// otherwise.
 template <class S>
 __device__ __forceinline__ int uc_record_u(S& s, const typename Prec<S>::T* a, typename Prec<S>::T h, double& t,
                                            long long i0, long long i1, bool trail, const UcJob& j, PeakObserver& o) {
+    if (o.max_peaks == 0 && j.raw_flags != 0 && o.traj_len > 0)
+        return uc_trace_u(s, a, h, t, i0, i1, trail, j.pre_scaller, j.writable_var, j.max_value, o.traj, o.stride,
+                          o.traj_len);
     if (j.pre_scaller == 1 && j.writable_var >= 0 && o.traj_len == 0 && o.n < (1LL << 30) &&
         (i0 >= 2 || (o.mode == 0 && o.c_next >= 2)))
         return uc_record_u1(s, a, h, t, (int)i0, (int)i1, trail, j.writable_var, j.max_value, o);
